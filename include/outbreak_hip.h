@@ -1,0 +1,155 @@
+/*
+ * liboutbreak_hip -- C ABI of the MI355X (gfx950) model time step.
+ *
+ * The reference has no FFI: its step API is the JavaScript class pair
+ * RegionalModel (model.js:1156-1496) and PoissonDirect (solver.js:105-165)
+ * over WebGPU objects. Each entry point below replaces one of their methods
+ * (cited per function); a Node N-API addon and a Python ctypes layer rebuild
+ * the two classes on top of it with the reference's names and semantics.
+ *
+ * Conventions
+ *   - Every call returns an int status: 0 = OK, < 0 = error. Nothing throws
+ *     across the ABI. The last error text of a handle is kept until the next
+ *     failing call on it (obl_last_error); failures without a handle set the
+ *     process-wide text (obl_global_error).
+ *   - Host arrays are copied during the call (the reference's GPU.write /
+ *     queue.writeBuffer semantics, util.js:190).
+ *   - GPU work is enqueued on the handle's own HIP stream and runs in order,
+ *     like the reference's single queue; obl_sync waits for it
+ *     (onSubmittedWorkDone, app.js:1704).
+ *   - Layouts are the reference's texel layouts (x fastest):
+ *       V   float4 (nx+1)(ny+1)(nz+1)    u, v, w, theta     (two copies, parity)
+ *       Q1  float4 nx ny nz              qv, qc, qi, qr     (two copies)
+ *       Q2  float4 nx ny nz              qs, qg, Nr, spare  (two copies)
+ *       aux float2 nx ny nz              nu_h, nu_v
+ *       phi, rhs float (nx+2p)(ny+2p)nz  the padded pressure grid
+ *     The uniform block is the reference's 76-float MODEL_FIELDS block
+ *     (model.js:23-37), packed by the host exactly like writeUniforms
+ *     (model.js:1391-1425).
+ */
+#ifndef OUTBREAK_HIP_H
+#define OUTBREAK_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OBL_ABI_VERSION 1
+#define OBL_UNIFORM_FLOATS 76
+#define OBL_BOUND_FLOATS 148 /* one boundary record, model.js:1371-1381 */
+#define OBL_MAX_BOUNDS 8     /* MAX_BOUNDS, model.js:38 */
+#define OBL_CI_FLOATS 12     /* one CI tower, model.js:1383-1389 */
+#define OBL_MAX_CI 256       /* MAX_CI, model.js:38 */
+
+typedef struct obl_model obl_model;
+typedef struct obl_poisson obl_poisson;
+
+/* Grid of RegionalModel.allocTextures (model.js:1202-1238): dz = 20000 / nz,
+ * pad = gridPad(dx) (model.js:1122) cells of solve padding per side. */
+typedef struct {
+  int nx, ny, nz;
+  double dx, dz;
+  int pad;
+} obl_grid;
+
+/* Field ids for obl_upload / obl_download / obl_field_ptr */
+enum {
+  OBL_FIELD_V = 0,    /* copy = parity 0/1 */
+  OBL_FIELD_Q1 = 1,   /* copy 0/1 */
+  OBL_FIELD_Q2 = 2,   /* copy 0/1 */
+  OBL_FIELD_VT = 3,
+  OBL_FIELD_Q1T = 4,
+  OBL_FIELD_Q2T = 5,
+  OBL_FIELD_AUX = 6,
+  OBL_FIELD_PHT = 7,
+  OBL_FIELD_LSFA = 8,
+  OBL_FIELD_LSFB = 9,
+  OBL_FIELD_CIMAP = 10,
+  OBL_FIELD_PHI = 11,
+  OBL_FIELD_RHS = 12,
+  OBL_FIELD_BS = 13
+};
+
+/* obl_step flags */
+#define OBL_STEP_LSF 1u /* run the large-scale forcing this step (model.js:1461) */
+
+/* Solver precision (obl_set_solver_mode / obl_poisson_solve) */
+enum {
+  OBL_GEMM_FP32 = 0,   /* FP32 MFMA sine transforms */
+  OBL_GEMM_BF16X3 = 1  /* split hi/lo BF16 products, FP32 accumulate */
+};
+enum {
+  OBL_THOMAS_FP32 = 0,  /* the reference's FP32 Thomas (solver.js:86-101) */
+  OBL_THOMAS_FP64 = 1,  /* FP64 arithmetic for every (l, m) */
+  OBL_THOMAS_MIXED = 2  /* FP64 only where the vertical system is near-singular */
+};
+
+/* ---------------- errors and version ---------------- */
+int obl_abi_version(void);
+const char* obl_last_error(const obl_model* h);
+const char* obl_global_error(void);
+
+/* ---------------- RegionalModel ---------------- */
+/* allocate (model.js:1186-1238): all fields, the padded solve and its matrices.
+ * Fails with the reference's size message when a field would not fit. */
+int obl_create(const obl_grid* g, obl_model** out);
+/* destroy (model.js:1239-1247) */
+void obl_destroy(obl_model* h);
+/* setSounding -> bs + solver.setDensity (model.js:1276-1288): B holds (nz+1)*8 floats */
+int obl_set_base_state(obl_model* h, const float* B, int nz);
+/* setBoundaries (model.js:1371-1381): count records of 148 floats */
+int obl_set_boundaries(obl_model* h, const float* f, int count);
+/* setCI (model.js:1383-1389): count towers of 12 floats */
+int obl_set_ci(obl_model* h, const float* f, int count);
+/* setSurface (model.js:1298-1303): NULL = outbreak mode (no sea). Hurricane
+ * surfaces are outside this library's scope and are rejected. */
+int obl_set_surface(obl_model* h, const float* surf);
+/* reset (model.js:1427-1440): lsf + init with the given uniform block (written
+ * for dt = 1), clears phi/rhs, copies V0 -> V1. seed 0 = the reference's noise. */
+int obl_reset(obl_model* h, const float* uniforms, int seed);
+/* encStep (model.js:1442-1476): enqueue one step with this uniform block; the
+ * state moves from V[parity] to V[1-parity]. The caller keeps time/steps/parity
+ * as the reference does and passes parity explicitly. */
+int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags);
+/* Warm bubble, theta += dth cos^2(pi r/2) (no reference counterpart: the configs
+ * start storms with bubbles instead of CI towers), on both V copies. */
+int obl_add_bubble(obl_model* h, double x, double y, double z, double rh, double rz, double dth);
+/* GPU.write / writeTexture and readback copies (util.js:190; model.js:1478-1494) */
+int obl_upload(obl_model* h, int field, int copy, const void* src, size_t bytes, size_t offset);
+int obl_download(obl_model* h, int field, int copy, void* dst, size_t bytes, size_t offset);
+/* Device pointer of a field, for in-process consumers (radar.js binds m.V[par], m.Q1[0], ...) */
+int obl_field_ptr(obl_model* h, int field, int copy, void** ptr, size_t* bytes);
+/* onSubmittedWorkDone (app.js:1704) */
+int obl_sync(obl_model* h);
+/* Precision of the model's pressure solve (defaults: FP32 GEMM, mixed Thomas) */
+int obl_set_solver_mode(obl_model* h, int gemm_mode, int thomas_mode, double fp64_cond);
+/* Per-kernel HIP-event timing of subsequent obl_step calls (0 = off). */
+int obl_profile(obl_model* h, int on);
+/* Average device time per launch (ms) of each timed kernel since obl_profile(h, 1);
+ * names are written as a '\n'-separated list. Returns the number of kernels. */
+int obl_profile_read(obl_model* h, char* names, size_t names_len, double* ms, int* launches, int max);
+
+/* ---------------- PoissonDirect ---------------- */
+/* constructor (solver.js:106-137): nx, ny even; dz, dx in metres */
+int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, obl_poisson** out);
+void obl_poisson_destroy(obl_poisson* p);
+/* setDensity (solver.js:139-146) */
+int obl_poisson_set_density(obl_poisson* p, const double* rhoC, const double* rhoW);
+/* enc (solver.js:155-163): device pointers rhs -> phi, enqueued on stream (0 = the
+ * solver's own stream) */
+int obl_poisson_solve(obl_poisson* p, const float* rhs_dev, float* phi_dev, int gemm_mode, int thomas_mode,
+                      double fp64_cond, void* stream);
+/* host-memory convenience: copies in, solves, copies out, synchronises */
+int obl_poisson_solve_host(obl_poisson* p, const float* rhs, float* phi, int gemm_mode, int thomas_mode,
+                           double fp64_cond);
+const char* obl_poisson_last_error(const obl_poisson* p);
+/* Per-pass device time (ms) of the last solve when timing is on */
+int obl_poisson_profile(obl_poisson* p, int on);
+int obl_poisson_profile_read(obl_poisson* p, double* ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+"""ctypes binding of liboutbreak_hip.so (the C ABI in include/outbreak_hip.h).
+
+The product path has no fallback: if the gfx950 library is missing or fails
+to load, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("OBL_LIB", os.path.join(_PKG, "liboutbreak_hip.so"))
+
+_lib = None
+
+
+class OblError(RuntimeError):
+    pass
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("pad", C.c_int)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "obl_abi_version": (C.c_int, []),
+    "obl_last_error": (C.c_char_p, [C.c_void_p]),
+    "obl_global_error": (C.c_char_p, []),
+    "obl_create": (C.c_int, [C.POINTER(Grid), C.POINTER(C.c_void_p)]),
+    "obl_destroy": (None, [C.c_void_p]),
+    "obl_set_base_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_set_ci": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
+    "obl_add_bubble": (C.c_int, [C.c_void_p] + [C.c_double] * 6),
+    "obl_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),
+    "obl_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),
+    "obl_field_ptr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "obl_sync": (C.c_int, [C.c_void_p]),
+    "obl_set_solver_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+    "obl_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "obl_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    "obl_poisson_destroy": (None, [C.c_void_p]),
+    "obl_poisson_set_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "obl_poisson_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]),
+    "obl_poisson_solve_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]),
+    "obl_poisson_last_error": (C.c_char_p, [C.c_void_p]),
+    "obl_poisson_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "obl_poisson_profile_read": (C.c_int, [C.c_void_p, C.c_void_p]),
+}
+
+FIELD = {"V": 0, "Q1": 1, "Q2": 2, "Vt": 3, "Q1t": 4, "Q2t": 5, "aux": 6, "phT": 7, "lsfA": 8, "lsfB": 9,
+         "ciMap": 10, "phi": 11, "rhs": 12, "bs": 13}
+STEP_LSF = 1
+GEMM = {"fp32": 0, "bf16x3": 1}
+THOMAS = {"fp32": 0, "fp64": 1, "mixed": 2}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OblError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc for gfx950)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, handle=None, poisson=False):
+    if rc == 0:
+        return
+    L = lib()
+    if handle is None:
+        msg = L.obl_global_error()
+    elif poisson:
+        msg = L.obl_poisson_last_error(handle)
+    else:
+        msg = L.obl_last_error(handle)
+    raise OblError((msg or b"unknown error").decode(errors="replace"))

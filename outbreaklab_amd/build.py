@@ -1,0 +1,100 @@
+"""Build the gfx950 shared library (and, for tests, the CPU oracle).
+
+``build_hip()`` compiles every HIP source under ``csrc/`` with hipcc for
+gfx950 into ``outbreaklab_amd/liboutbreak_hip.so``. ``build_napi()`` builds
+the Node addon that wraps the same C ABI. ``build_oracle()`` runs the
+oracle's Makefile (test infrastructure; the product never loads it).
+"""
+from __future__ import annotations
+
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "liboutbreak_hip.so")
+NAPI_SRC = os.path.join(PKG, "js", "obl_napi.c")
+NAPI_OUT = os.path.join(PKG, "js", "obl_napi.node")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # no packed FP32 (v_pk_*_f32) anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+    return r.stdout
+
+
+def build_hip(jobs: int = 4) -> str:
+    """Compile csrc/*.hip separately (in parallel) and link liboutbreak_hip.so."""
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objdir = os.path.join(CSRC, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    procs = []
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s) + ".o")
+        objs.append(o)
+        cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + ["-c", s, "-o", o]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        if len(procs) >= jobs:
+            cmd0, p0 = procs.pop(0)
+            out = p0.communicate()[0]
+            if p0.returncode:
+                sys.stderr.write(out)
+                raise RuntimeError("hipcc failed: " + " ".join(cmd0))
+    for cmd0, p0 in procs:
+        out = p0.communicate()[0]
+        if p0.returncode:
+            sys.stderr.write(out)
+            raise RuntimeError("hipcc failed: " + " ".join(cmd0))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_napi() -> str | None:
+    """Build the Node N-API addon (plain C against node_api.h); None if Node headers are absent."""
+    inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(inc, "node_api.h")) or not os.path.exists(NAPI_SRC):
+        return None
+    cc = shutil.which("gcc") or "cc"
+    tmp = NAPI_OUT + ".tmp"
+    _run([cc, "-O2", "-fPIC", "-shared", "-std=gnu11", "-Wall", "-I", inc, "-I", os.path.join(ROOT, "include"),
+          NAPI_SRC, "-o", tmp, "-ldl"])
+    os.replace(tmp, NAPI_OUT)
+    return NAPI_OUT
+
+
+def build_oracle() -> str:
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-j4"], cwd=odir)
+    return os.path.join(odir, "liboracle.so")
+
+
+def build_all() -> None:
+    build_hip()
+    build_napi()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

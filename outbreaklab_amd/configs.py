@@ -1,0 +1,104 @@
+"""The workloads the step is measured and checked on, as concrete synthetic inputs.
+
+The reference ships no such cases (it is a browser app driven by user
+settings); these builders define them with the reference's own API:
+``allocate('custom', spec)``, ``setSounding(text)``, params, ``reset()``, then
+warm bubbles (a build-defined write, see ``RegionalModel.add_bubble``).
+
+  plain_128      128 x 128 x 40 at 2 km (dz 500 m, pad 8), Lin, one 2 K bubble, dt 6 s
+  solve_1024     the pressure solve alone, 1024 x 1024 x 80 (dx 500 m, dz 250 m)
+  plain_1024     1024 x 1024 x 80 at 500 m (pad 24), Thompson, 8 bubbles, dt 2 s
+  combined_1024  1024 x 1024 x 100 at 250 m (pad 40), one cold-pool boundary, dt 1 s
+"""
+from __future__ import annotations
+
+import functools
+import math
+
+import numpy as np
+
+from .sounding import analyze, parse_sharppy, wk_sounding_text
+
+
+@functools.lru_cache(maxsize=16)
+def wk_text_for_cape(target_cape: float, shear6: float, th0: float = 300.0, thtr: float = 343.0) -> str:
+    """WK sounding whose surface-based CAPE (the reference's liftParcel) is ``target_cape``,
+    found by bisection on the boundary-layer mixing ratio qv0."""
+    lo, hi = 0.008, 0.024
+    for _ in range(22):
+        mid = 0.5 * (lo + hi)
+        cape = analyze(parse_sharppy(wk_sounding_text(th0=th0, thtr=thtr, qv0=mid, shear6=shear6)))["P"]["sbcape"]
+        if cape < target_cape:
+            lo = mid
+        else:
+            hi = mid
+    return wk_sounding_text(th0=th0, thtr=thtr, qv0=0.5 * (lo + hi), shear6=shear6)
+
+
+def bubble_positions(n: int, Lx: float, Ly: float, margin: float, min_sep: float, seed: int):
+    rng = np.random.default_rng(seed)
+    pts = []
+    tries = 0
+    while len(pts) < n:
+        tries += 1
+        if tries > 100000:
+            raise RuntimeError("could not place the bubbles")
+        x, y = rng.uniform(margin, Lx - margin), rng.uniform(margin, Ly - margin)
+        if all(math.hypot(x - a, y - b) >= min_sep for a, b in pts):
+            pts.append((x, y))
+    return pts
+
+
+def setup_plain_128(model, seed: int = 0, scheme: str = "lin"):
+    """128 x 128 x 40, 2 km: WK (300 K, 343 K at 12 km, 14 g/kg), quarter-circle hodograph to 30 m/s,
+    one 2 K bubble (10 km x 1.5 km radius, centre 1.5 km AGL), +-0.1 K noise, dt 6 s."""
+    model.allocate("custom", {"key": "custom:2000:128x128x40", "dx": 2000, "n": [128, 128, 40]})
+    model.params["scheme"] = scheme
+    model.params["noise"] = 0.1
+    model.params["lat"] = 35.9
+    model.noise_seed = seed
+    model.setSounding(wk_sounding_text(th0=300.0, thtr=343.0, qv0=0.014, shear6=30.0, hodo_r=5.0))
+    model.setBoundaries([])
+    model.setCI([])
+    model.setSurface(None, None)
+    model.reset()
+    model.add_bubble(0.5 * model.Lx, 0.5 * model.Ly, 1500.0, 10000.0, 1500.0, 2.0)
+    return 6.0
+
+
+def setup_plain_1024(model, seed: int = 1, scheme: str = "thompson", n=(1024, 1024, 80), cape: float = 3000.0):
+    """1024 x 1024 x 80 at 500 m: WK at 3000 J/kg SBCAPE with 25 m/s 0-6 km shear, Thompson,
+    8 seeded 2 K bubbles >= 35 km apart and >= spongeW + 10 km from the edges, dt 2 s."""
+    nx, ny, nz = n
+    model.allocate("custom", {"key": f"custom:500:{nx}x{ny}x{nz}", "dx": 500, "n": [nx, ny, nz]})
+    model.params["scheme"] = scheme
+    model.params["noise"] = 0.1
+    model.params["lat"] = 35.9
+    model.noise_seed = seed
+    model.setSounding(wk_text_for_cape(cape, 25.0))
+    model.setBoundaries([])
+    model.setCI([])
+    model.setSurface(None, None)
+    model.reset()
+    margin = model.params["spongeW"] + 10000.0
+    nb = 8 if min(model.Lx, model.Ly) > 2 * margin + 70000 else 1
+    for x, y in bubble_positions(nb, model.Lx, model.Ly, margin, 35000.0, seed):
+        model.add_bubble(x, y, 1500.0, 10000.0, 1500.0, 2.0)
+    return 2.0
+
+
+def solve_rhs_1024(nx=1024, ny=1024, nz=80, seed=1, rms=1e-5):
+    """Seeded zero-mean Gaussian divergence RHS with a k^-5/3 horizontal spectrum:
+    in DST space g(l, m, k) kappa^(-4/3), kappa = sqrt((l+1)^2 + (m+1)^2), g ~ N(0, 1);
+    inverse DST in FP64, per-level mean removed, scaled to the given RMS (s^-2)."""
+    from scipy.fft import idstn
+
+    rng = np.random.default_rng(seed)
+    l = np.arange(nx) + 1.0
+    m = np.arange(ny) + 1.0
+    kap = np.sqrt(l[None, :] ** 2 + m[:, None] ** 2)
+    spec = rng.standard_normal((nz, ny, nx)) * kap[None] ** (-4.0 / 3.0)
+    f = idstn(spec, type=2, norm="ortho", axes=(1, 2))
+    f -= f.mean(axis=(1, 2), keepdims=True)
+    f *= rms / np.sqrt(np.mean(f * f))
+    return f

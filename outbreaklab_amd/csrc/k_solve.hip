@@ -1,0 +1,239 @@
+// Exact anelastic pressure solve for gfx950 (replaces solver.js:12-103).
+//
+// The reference applies dense DST-II matrices (one GEMM per transform pass).
+// Here every pass uses the even/odd symmetry of the DST-II basis,
+// v_l(N-1-i) = (-1)^l v_l(i): the even modes see only S(i) = F(i) + F(N-1-i),
+// the odd modes only D(i) = F(i) - F(N-1-i), i < N/2. Each pass is therefore
+// two half-size GEMMs, computed together by one kernel on the MFMA units
+// (v_mfma_f32_16x16x4f32), and the flops halve. Spectral arrays keep the even
+// modes in the first half of each axis and the odd modes in the second half;
+// the Thomas kernel uses eigenvalues permuted the same way.
+//
+// Passes (rows of the padded grid F[z][y][x], x fastest):
+//   XF  G[r][s]      = sum_i S/D[r][i] W[s][i]            r = y + nyp z
+//   YF  G[z][s][x]   = sum_j W[s][j] S/D[z][j][x]
+//   XI  F[r][i], F[r][N-1-i] = E +- O, E = sum_s G[r][s] WT[i][s]
+//   YI  F[z][j][x], F[z][N-1-j][x] = E +- O
+// Thomas: one (sx, sy) mode per work-item, z swept in registers; FP32 with the
+// reference's operation order, FP64, or FP64 only for the modes whose vertical
+// system is near-singular.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "obl_kernels.h"
+
+namespace obl {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 64, BN = 64, BK = 16, LDK = BK + 4;
+
+// bf16 split of an fp32 value: hi = RNE(x), lo = RNE(x - hi)
+__device__ __forceinline__ short bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (short)(r >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+
+template <int MODE, int PREC>
+__global__ void __launch_bounds__(256) k_dst(DstArgs a) {
+  __shared__ float As[2][BM][LDK];
+  __shared__ float Bs[2][BN][LDK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, kq = lane >> 4;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int plane = blockIdx.z;
+  const int N = a.N, Nh = a.Nh;
+  const size_t planeSz = (size_t)N * (size_t)a.inner;
+
+  v4f acc[2][2][2];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[p][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < Nh; k0 += BK) {
+    // ---------------- global -> LDS ----------------
+    if (MODE == DST_XF || MODE == DST_XI) {
+      // A: data rows (K contiguous); B: matrix rows
+      int row = tid >> 2, q = tid & 3;
+      int r = m0 + row, k = k0 + 4 * q;
+      float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+      if (r < a.rows && k < Nh) {
+        const float* src = a.in + (size_t)r * N;
+        if (MODE == DST_XF) {
+          float4 f = *reinterpret_cast<const float4*>(src + k);
+          float4 g = *reinterpret_cast<const float4*>(src + (N - 4 - k));
+          s0 = make_float4(f.x + g.w, f.y + g.z, f.z + g.y, f.w + g.x);
+          s1 = make_float4(f.x - g.w, f.y - g.z, f.z - g.y, f.w - g.x);
+        } else {
+          s0 = *reinterpret_cast<const float4*>(src + k);
+          s1 = *reinterpret_cast<const float4*>(src + Nh + k);
+        }
+      }
+      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = s0;
+      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = s1;
+      const size_t wo = (size_t)(n0 + row) * a.Kp + k0 + 4 * q;
+      *reinterpret_cast<float4*>(&Bs[0][row][4 * q]) = *reinterpret_cast<const float4*>(a.W0 + wo);
+      *reinterpret_cast<float4*>(&Bs[1][row][4 * q]) = *reinterpret_cast<const float4*>(a.W1 + wo);
+    } else {
+      // A: matrix rows; B: data [k][n] (n contiguous), stored transposed
+      int row = tid >> 2, q = tid & 3;
+      const size_t wo = (size_t)(m0 + row) * a.Kp + k0 + 4 * q;
+      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = *reinterpret_cast<const float4*>(a.W0 + wo);
+      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = *reinterpret_cast<const float4*>(a.W1 + wo);
+      int kk = tid >> 4, qq = tid & 15;
+      int k = k0 + kk, n = n0 + 4 * qq;
+      float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+      if (k < Nh && n < a.inner) {
+        const float* pl = a.in + (size_t)plane * planeSz;
+        if (MODE == DST_YF) {
+          float4 f = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + n);
+          float4 g = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + n);
+          s0 = make_float4(f.x + g.x, f.y + g.y, f.z + g.z, f.w + g.w);
+          s1 = make_float4(f.x - g.x, f.y - g.y, f.z - g.z, f.w - g.w);
+        } else {
+          s0 = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + n);
+          s1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + n);
+        }
+      }
+      Bs[0][4 * qq + 0][kk] = s0.x; Bs[0][4 * qq + 1][kk] = s0.y; Bs[0][4 * qq + 2][kk] = s0.z; Bs[0][4 * qq + 3][kk] = s0.w;
+      Bs[1][4 * qq + 0][kk] = s1.x; Bs[1][4 * qq + 1][kk] = s1.y; Bs[1][4 * qq + 2][kk] = s1.z; Bs[1][4 * qq + 3][kk] = s1.w;
+    }
+    __syncthreads();
+    // ---------------- MFMA ----------------
+    // Lane (lr, kq) feeds A[m = lr][k] and B[k][n = lr] for k = 4 kq + t at step t:
+    // the same k permutation on both operands leaves the sum unchanged.
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      float4 af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) af[i] = *reinterpret_cast<const float4*>(&As[p][wm * 32 + i * 16 + lr][4 * kq]);
+#pragma unroll
+      for (int j = 0; j < 2; j++) bf[j] = *reinterpret_cast<const float4*>(&Bs[p][wn * 32 + j * 16 + lr][4 * kq]);
+      if (PREC == 0) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].x, bf[j].x, acc[p][i][j], 0, 0, 0);
+            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].y, bf[j].y, acc[p][i][j], 0, 0, 0);
+            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].z, bf[j].z, acc[p][i][j], 0, 0, 0);
+            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].w, bf[j].w, acc[p][i][j], 0, 0, 0);
+          }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------- epilogue ----------------
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        int m = m0 + wm * 32 + i * 16 + 4 * kq + e;
+        int n = n0 + wn * 32 + j * 16 + lr;
+        float v0 = acc[0][i][j][e], v1 = acc[1][i][j][e];
+        if (MODE == DST_XF) {
+          if (m < a.rows && n < Nh) {
+            float* dst = a.out + (size_t)m * N;
+            dst[n] = v0;
+            dst[Nh + n] = v1;
+          }
+        } else if (MODE == DST_XI) {
+          if (m < a.rows && n < Nh) {
+            float* dst = a.out + (size_t)m * N;
+            dst[n] = v0 + v1;
+            dst[N - 1 - n] = v0 - v1;
+          }
+        } else if (MODE == DST_YF) {
+          if (m < Nh && n < a.inner) {
+            float* pl = a.out + (size_t)plane * planeSz;
+            pl[(size_t)m * a.inner + n] = v0;
+            pl[(size_t)(Nh + m) * a.inner + n] = v1;
+          }
+        } else {
+          if (m < Nh && n < a.inner) {
+            float* pl = a.out + (size_t)plane * planeSz;
+            pl[(size_t)m * a.inner + n] = v0 + v1;
+            pl[(size_t)(N - 1 - m) * a.inner + n] = v0 - v1;
+          }
+        }
+      }
+}
+
+void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
+  dim3 blk(256);
+  if (mode == DST_XF || mode == DST_XI) {
+    dim3 grid((a.Nh + BN - 1) / BN, (a.rows + BM - 1) / BM, 1);
+    if (mode == DST_XF) hipLaunchKernelGGL((k_dst<DST_XF, 0>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_dst<DST_XI, 0>), grid, blk, 0, s, a);
+  } else {
+    dim3 grid((a.inner + BN - 1) / BN, (a.Nh + BM - 1) / BM, a.rows);
+    if (mode == DST_YF) hipLaunchKernelGGL((k_dst<DST_YF, 0>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_dst<DST_YI, 0>), grid, blk, 0, s, a);
+  }
+  (void)prec;
+}
+
+// ---------------- Thomas ----------------
+__global__ void __launch_bounds__(256) k_thomas(ThomasArgs a) {
+  int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (sx >= a.nx || sy >= a.ny) return;
+  const size_t sxy = (size_t)a.nx * a.ny, base = (size_t)sx + (size_t)a.nx * sy;
+  bool use64 = a.mode == 1;
+  if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
+  if (!use64) {
+    float shift = a.lam32[sx] + a.lam32[a.nx + sy];
+    float cprev = 0.0f, dprev = 0.0f;
+    for (int k = 0; k < a.nz; k++) {
+      float4 o = a.zop32[k];
+      size_t idx = base + sxy * k;
+      float den = o.z + shift * o.w - o.x * cprev;
+      cprev = o.y / den;
+      dprev = (a.hat[idx] - o.x * dprev) / den;
+      a.cp32[idx] = cprev;
+      a.hat[idx] = dprev;
+    }
+    float above = a.hat[base + sxy * (a.nz - 1)];
+    for (int k = a.nz - 2; k >= 0; k--) {
+      size_t idx = base + sxy * k;
+      above = a.hat[idx] - a.cp32[idx] * above;
+      a.hat[idx] = above;
+    }
+  } else {
+    double shift = a.lam64[sx] + a.lam64[a.nx + sy];
+    double cprev = 0.0, dprev = 0.0;
+    for (int k = 0; k < a.nz; k++) {
+      const double* o = a.zop64 + 4 * k;
+      size_t idx = base + sxy * k;
+      double den = o[2] + shift * o[3] - o[0] * cprev;
+      cprev = o[1] / den;
+      dprev = ((double)a.hat[idx] - o[0] * dprev) / den;
+      a.cp64[idx] = cprev;
+      a.dp64[idx] = dprev;
+    }
+    double above = a.dp64[base + sxy * (a.nz - 1)];
+    a.hat[base + sxy * (a.nz - 1)] = (float)above;
+    for (int k = a.nz - 2; k >= 0; k--) {
+      size_t idx = base + sxy * k;
+      above = a.dp64[idx] - a.cp64[idx] * above;
+      a.hat[idx] = (float)above;
+    }
+  }
+}
+
+void launch_thomas(hipStream_t s, const ThomasArgs& a) {
+  dim3 blk(64, 4, 1), grid((a.nx + 63) / 64, (a.ny + 3) / 4, 1);
+  hipLaunchKernelGGL(k_thomas, grid, blk, 0, s, a);
+}
+
+}  // namespace obl

@@ -1,0 +1,875 @@
+// Model-step kernels for gfx950: large-scale forcing, CI map, initial state,
+// eddy viscosity, semi-Lagrangian + MacCormack advection, fused forces and
+// microphysics, divergence and projection.
+//
+// Each kernel keeps the arithmetic of the reference kernel it replaces
+// (model.js, cited per kernel) and the reference's texel conventions; the
+// data movement is this build's own: one work-item per point, x fastest so
+// that every wave reads contiguous float4 texels, and per-channel gathers for
+// the trilinear samples.
+#include "obl_device.h"
+#include "obl_kernels.h"
+
+namespace obl {
+
+// ====================== LSF (model.js:142-244) ======================
+struct BndView {
+  const float* b;  // 148 floats: p0..p4 then 32 points
+};
+__device__ static void bpt(const float* b, int i, float t, float& x, float& y) {
+  const float* q = b + 20 + 4 * i;
+  x = q[0] + q[2] * t;
+  y = q[1] + q[3] * t;
+}
+__device__ static void sdist(const float* b, float t, float px, float py, float& sd, float& over_, float& along_) {
+  int np = (int)b[0];
+  float best = 1e30f, sg = 1.0f, over = 0.0f, along = 0.0f, s0 = 0.0f;
+  for (int i = 0; i < np - 1; i++) {
+    float ax, ay, ex, ey;
+    bpt(b, i, t, ax, ay);
+    bpt(b, i + 1, t, ex, ey);
+    float abx = ex - ax, aby = ey - ay;
+    float la = fmaxf(sqrtf(abx * abx + aby * aby), 1.0f);
+    float t0 = ((px - ax) * abx + (py - ay) * aby) / (la * la);
+    float tt = t0;
+    if (i > 0) tt = fmaxf(tt, 0.0f);
+    if (i < np - 2) tt = fminf(tt, 1.0f);
+    float qx = px - (ax + tt * abx), qy = py - (ay + tt * aby);
+    float d = sqrtf(qx * qx + qy * qy);
+    if (d < best) {
+      best = d;
+      sg = (abx * (py - ay) - aby * (px - ax) > 0.0f) ? 1.0f : -1.0f;
+      over = (i == 0 ? fmaxf(-t0, 0.0f) * la : 0.0f) + (i == np - 2 ? fmaxf(t0 - 1.0f, 0.0f) * la : 0.0f);
+      along = s0 + tt * la;
+    }
+    s0 += la;
+  }
+  sd = best * sg * b[15];
+  over_ = over;
+  along_ = along;
+}
+__device__ static float alongVar(const float* b, float time, float s) {
+  if (b[16] <= 0.0f) return 0.0f;
+  return clampf_(fbm2(s / b[18], b[17] + time / 5400.0f, 2) * 1.6f, -1.0f, 1.0f);
+}
+__device__ static float hotSpots(const float* b, float time, float s) {
+  float L = b[18];
+  int sd = (int)(b[17] * 1000.0f);
+  float ph = time / 4800.0f;
+  int e = (int)floorf(ph);
+  float fe = sstep(0.0f, 1.0f, ph - floorf(ph));
+  int k0 = (int)floorf(s / L);
+  float acc = 0.0f;
+  for (int k = k0 - 1; k <= k0 + 1; k++) {
+    for (int q = 0; q < 2; q++) {
+      uint32_t h = hash3i(k, e + q, sd);
+      float r1 = (float)(h & 0xffffu) / 65535.0f;
+      float r2 = (float)((h >> 16u) & 0xffffu) / 65535.0f;
+      float z = (s - ((float)k + 0.5f + 0.7f * (r1 - 0.5f)) * L) / (0.17f * L);
+      acc += (q == 1 ? fe : 1.0f - fe) * (0.35f + 0.65f * r2) * expf(-z * z);
+    }
+  }
+  return clampf_(acc, 0.0f, 1.0f);
+}
+
+__global__ void __launch_bounds__(256) k_lsf(MU u, const float4* __restrict__ BS, const float* __restrict__ BD,
+                                             float4* __restrict__ lsfA, float4* __restrict__ lsfB,
+                                             float* __restrict__ phT) {
+  Dims d = dims(u);
+  int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  float time = u.v[U_STEP].y;
+  float px = ((float)i + 0.5f) * d.dx, py = ((float)j + 0.5f) * d.dx;
+  float sT = 0, sH = 0, sQ = 0, sHq = 0, sU = 0, sV = 0, sHw = 0, sw = 0, nud = 0;
+  int nb = (int)u.v[U_COR].z;
+  for (int bi = 0; bi < nb; bi++) {
+    const float* b = BD + 148 * bi;
+    float dd, over, along;
+    sdist(b, time, px, py, dd, over, along);
+    float w = sstep(-b[3], b[3], dd) * (1.0f - sstep(0.0f, 60000.0f, over));
+    if (w <= 0.0f) continue;
+    float vr = b[16];
+    float fd = 1.0f - sstep(30000.0f, 90000.0f, dd);
+    float av = vr * alongVar(b, time, along) * fd;
+    float lift = vr > 0.0f ? mixf(1.0f, mixf(1.0f - 0.85f * vr, 1.0f + 1.3f * vr, hotSpots(b, time, along)), fd) : 1.0f;
+    float Hmin = b[7] * clampf_(lift, 0.15f, 3.0f);
+    float H = clampf_(b[6] * fmaxf(dd + b[3], 0.0f) * (1.0f + 0.4f * av), Hmin, b[5]);
+    sT += w * b[4] * expf(0.5f * av);
+    sQ += w * b[8];
+    sU += w * b[10] * lift;
+    sV += w * b[11] * lift;
+    sH += w * H;
+    sHq += w * H * b[9];
+    sHw += w * H * b[12];
+    sw += w;
+    nud = fmaxf(nud, b[13] * sstep(b[14], b[14] + 40000.0f, dd) * (1.0f - sstep(0.0f, 60000.0f, over)));
+  }
+  float iw = 1.0f / fmaxf(sw, 1e-6f);
+  float4 A = make_float4(clampf_(sT, -20.0f, 10.0f), fmaxf(sH * iw, 100.0f), clampf_(sQ, 0.0f, 0.92f), fmaxf(sHq * iw, 100.0f));
+  float4 B = make_float4(sU, sV, fmaxf(sHw * iw, 100.0f), nud);
+  lsfA[(size_t)i + (size_t)d.nx * j] = A;
+  lsfB[(size_t)i + (size_t)d.nx * j] = B;
+  float ph = 0.0f, bprev = 0.0f;
+  for (int k = d.nz - 1; k >= 0; k--) {
+    float z = ((float)k + 0.5f) * d.dz;
+    float4 a = bsA(BS, k, d.nz), bb = bsB(BS, k, d.nz);
+    float dth = A.x / bb.x * tprof(z, A.y);
+    float dqv = -a.y * A.z * tprof(z, A.w);
+    float bt = 9.81f * (dth / a.x + 0.608f * dqv);
+    if (k == d.nz - 1) ph = -bt * 0.5f * d.dz;
+    else ph -= 0.5f * (bt + bprev) * d.dz;
+    bprev = bt;
+    phT[cidx(d, i, j, k)] = ph;
+  }
+}
+
+// ====================== CI map (model.js:249-281) ======================
+__global__ void __launch_bounds__(256) k_ci(MU u, const float* __restrict__ CI, float4* __restrict__ ciMap) {
+  Dims d = dims(u);
+  int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= d.nx || j >= d.ny) return;
+  float t = u.v[U_STEP].y;
+  float px = ((float)i + 0.5f) * d.dx, py = ((float)j + 0.5f) * d.dx;
+  float4 best = make_float4(0.0f, 0.0f, 1500.0f, 1500.0f);
+  int ne = (int)u.v[U_CI].x;
+  for (int e = 0; e < ne; e++) {
+    const float* E = CI + 12 * e;
+    if (t < E[6] || t > E[7]) continue;
+    float dxv = px - (E[0] + E[8] * (t - E[6]));
+    float dyv = py - (E[1] + E[9] * (t - E[6]));
+    float ang = atan2f(dyv, dxv);
+    float wob = 1.0f + 0.25f * gnoise2(cosf(ang) * 1.6f + E[10], sinf(ang) * 1.6f + 3.7f * E[10]);
+    float g = sstep(E[6], E[6] + fmaxf(E[11], 60.0f), t);
+    float r = sqrtf(dxv * dxv + dyv * dyv) / (E[3] * wob * (0.35f + 0.65f * g));
+    if (r >= 1.0f) continue;
+    float Lx = (float)d.nx * d.dx, Ly = (float)d.ny * d.dx;
+    float edge = fminf(fminf(px, Lx - px), fminf(py, Ly - py));
+    float h = sstep(E[6], E[6] + 60.0f, t) * (1.0f - sstep(E[7] - 180.0f, E[7], t)) *
+              sstep(u.v[U_SPONGE].x, u.v[U_SPONGE].x + 10000.0f, edge);
+    float wt = E[5] * (0.4f + 0.6f * g) * cosf(1.5707963f * r);
+    if (h * wt > best.x * best.y) best = make_float4(h, wt, E[2], E[4]);
+  }
+  ciMap[(size_t)i + (size_t)d.nx * j] = best;
+}
+
+// ---------------- target state helpers (model.js:125-134) ----------------
+__device__ __forceinline__ float tgtTh(const float4* BS, const Dims& d, float4 A, int k) {
+  float z = ((float)k + 0.5f) * d.dz;
+  return bsA(BS, k, d.nz).x + A.x / bsB(BS, k, d.nz).x * tprof(z, A.y);
+}
+__device__ __forceinline__ float tgtQv(const float4* BS, const Dims& d, float4 A, int k) {
+  float z = ((float)k + 0.5f) * d.dz;
+  return bsA(BS, k, d.nz).y * (1.0f - A.z * tprof(z, A.w));
+}
+__device__ __forceinline__ float tgtU(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+  float zz = ((float)z + 0.5f) * d.dz;
+  float4 a = ld2d(lsfB, d, x - 1, y), b = ld2d(lsfB, d, x, y);
+  return bsB(BS, z, d.nz).z + 0.5f * (a.x * tprof(zz, a.z) + b.x * tprof(zz, b.z));
+}
+__device__ __forceinline__ float tgtV(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+  float zz = ((float)z + 0.5f) * d.dz;
+  float4 a = ld2d(lsfB, d, x, y - 1), b = ld2d(lsfB, d, x, y);
+  return bsB(BS, z, d.nz).w + 0.5f * (a.y * tprof(zz, a.z) + b.y * tprof(zz, b.z));
+}
+
+// ====================== INIT (model.js:284-339), outbreak mode ======================
+__global__ void __launch_bounds__(256) k_init(MU u, const float4* __restrict__ BS, const float4* __restrict__ lsfA,
+                                              const float4* __restrict__ lsfB, float4* __restrict__ V,
+                                              float4* __restrict__ Q1, float4* __restrict__ Q2, int seed) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  int k = min(z, d.nz - 1);
+  float4 A = ld2d(lsfA, d, x, y);
+  float zz = ((float)k + 0.5f) * d.dz;
+  float nz0 = sstep(800.0f, 1500.0f, zz);
+  float noise = u.v[U_MISC].z * (rnd3(x + 17, y + 3 + 65536 * seed, z + 5) - 0.5f) * 2.0f * (1.0f - nz0);
+  float th = tgtTh(BS, d, A, k) + noise;
+  float uu = (y < d.ny && z < d.nz) ? tgtU(BS, lsfB, d, x, y, k) : 0.0f;
+  float vv = (x < d.nx && z < d.nz) ? tgtV(BS, lsfB, d, x, y, k) : 0.0f;
+  float4 B = bsB(BS, k, d.nz);
+  float T = th * B.x;
+  float qv = fminf(tgtQv(BS, d, A, k), 0.95f * qvsat(T, B.y, fIce(T)));
+  V[vidx(d, x, y, z)] = make_float4(uu, vv, 0.0f, th);
+  if (x < d.nx && y < d.ny && z < d.nz) {
+    Q1[cidx(d, x, y, z)] = make_float4(qv, 0.0f, 0.0f, 0.0f);
+    Q2[cidx(d, x, y, z)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+}
+
+// Build-defined warm bubble (no reference counterpart): theta += dth cos^2(pi r / 2)
+// at each texel's cell centre, applied to both state copies.
+__global__ void __launch_bounds__(256) k_bubble(MU u, float4* __restrict__ V0, float4* __restrict__ V1, double x0,
+                                                double y0, double z0, double rh, double rz, double dth) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  double px = (x + 0.5) * (double)d.dx, py = (y + 0.5) * (double)d.dx, pz = (z + 0.5) * (double)d.dz;
+  double a = (px - x0) / rh, b = (py - y0) / rh, c = (pz - z0) / rz;
+  double r = sqrt(a * a + b * b + c * c);
+  if (r < 1.0) {
+    double cc = cos(1.5707963267948966 * r);
+    float add = (float)(dth * cc * cc);
+    size_t i = vidx(d, x, y, z);
+    V0[i].w += add;
+    V1[i].w += add;
+  }
+}
+
+// ====================== NUT (model.js:342-389), outbreak mode ======================
+__device__ __forceinline__ float nut_uc(const float4* V, const float4* BS, const Dims& d, int x, int y, int z) {
+  return 0.5f * (ldVc(V, d, x, y, z, 0) + ldVc(V, d, x + 1, y, z, 0)) - bsB(BS, z, d.nz).z;
+}
+__device__ __forceinline__ float nut_vc(const float4* V, const float4* BS, const Dims& d, int x, int y, int z) {
+  return 0.5f * (ldVc(V, d, x, y, z, 1) + ldVc(V, d, x, y + 1, z, 1)) - bsB(BS, z, d.nz).w;
+}
+__device__ __forceinline__ float nut_wc(const float4* V, const Dims& d, int x, int y, int z) {
+  return 0.5f * (ldVc(V, d, x, y, z, 2) + ldVc(V, d, x, y, z + 1, 2));
+}
+__global__ void __launch_bounds__(256) k_nut(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                             const float4* __restrict__ Q1, float2* __restrict__ aux) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  float dx = d.dx, dz = d.dz;
+  float4 v0 = ldV(V, d, x, y, z);
+  float dudx = (ldVc(V, d, x + 1, y, z, 0) - v0.x) / dx;
+  float dvdy = (ldVc(V, d, x, y + 1, z, 1) - v0.y) / dx;
+  float dwdz = (ldVc(V, d, x, y, z + 1, 2) - v0.z) / dz;
+  float dudy = (nut_uc(V, BS, d, x, y + 1, z) - nut_uc(V, BS, d, x, y - 1, z)) / (2.0f * dx);
+  float dudz = (nut_uc(V, BS, d, x, y, z + 1) - nut_uc(V, BS, d, x, y, z - 1)) / (2.0f * dz);
+  float dvdx = (nut_vc(V, BS, d, x + 1, y, z) - nut_vc(V, BS, d, x - 1, y, z)) / (2.0f * dx);
+  float dvdz = (nut_vc(V, BS, d, x, y, z + 1) - nut_vc(V, BS, d, x, y, z - 1)) / (2.0f * dz);
+  float dwdx = (nut_wc(V, d, x + 1, y, z) - nut_wc(V, d, x - 1, y, z)) / (2.0f * dx);
+  float dwdy = (nut_wc(V, d, x, y + 1, z) - nut_wc(V, d, x, y - 1, z)) / (2.0f * dx);
+  float s12 = 0.5f * (dudy + dvdx), s13 = 0.5f * (dudz + dwdx), s23 = 0.5f * (dvdz + dwdy);
+  float S2 = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + 4.0f * (s12 * s12 + s13 * s13 + s23 * s23);
+  int kz0 = max(z - 1, 0), kz1 = min(z + 1, d.nz - 1);
+  float N2 = 9.81f / bsA(BS, z, d.nz).x * (ldVc(V, d, x, y, kz1, 3) - ldVc(V, d, x, y, kz0, 3)) /
+             ((float)(kz1 - kz0) * dz);
+  float4 q1 = Q1[cidx(d, x, y, z)];
+  if (u.v[U_COR].w > 0.5f && q1.y + q1.z > 1e-5f) N2 = fminf(N2, 0.0f);
+  float Ri = N2 / fmaxf(S2, 1e-10f);
+  float fst = sqrtf(fmaxf(0.0f, 1.0f - Ri / 0.33f));
+  float sS = sqrtf(fmaxf(S2, 0.0f));
+  float lh = u.v[U_TURB].x * dx, lv = 0.18f * dz;
+  float nuh = fminf(lh * lh * sS * fst, 0.08f * dx * dx / fmaxf(d.dt, 0.1f));
+  float nuv = fminf(lv * lv * sS * fst, 0.15f * dz * dz / fmaxf(d.dt, 0.1f));
+  aux[cidx(d, x, y, z)] = make_float2(nuh, nuv);
+}
+
+// ====================== advection of V (model.js:392-481) ======================
+struct Loc {
+  float3 vu, vv, vw, vc;
+};
+__device__ __forceinline__ Loc localVel(const float4* V, const Dims& d, int x, int y, int z) {
+  float4 p = ldV(V, d, x, y, z);
+  float4 xm = ldV(V, d, x - 1, y, z), xp = ldV(V, d, x + 1, y, z);
+  float4 ym = ldV(V, d, x, y - 1, z), yp = ldV(V, d, x, y + 1, z);
+  float4 zm = ldV(V, d, x, y, z - 1), zp = ldV(V, d, x, y, z + 1);
+  float xmyp = ldVc(V, d, x - 1, y + 1, z, 1), xmzp = ldVc(V, d, x - 1, y, z + 1, 2);
+  float xpym = ldVc(V, d, x + 1, y - 1, z, 0), ymzp = ldVc(V, d, x, y - 1, z + 1, 2);
+  float xpzm = ldVc(V, d, x + 1, y, z - 1, 0), ypzm = ldVc(V, d, x, y + 1, z - 1, 1);
+  Loc L;
+  L.vu = make_float3(p.x, 0.25f * (xm.y + p.y + xmyp + yp.y), 0.25f * (xm.z + p.z + xmzp + zp.z));
+  L.vv = make_float3(0.25f * (ym.x + p.x + xpym + xp.x), p.y, 0.25f * (ym.z + p.z + ymzp + zp.z));
+  L.vw = make_float3(0.25f * (zm.x + p.x + xpzm + xp.x), 0.25f * (zm.y + p.y + ypzm + yp.y), p.z);
+  L.vc = make_float3(0.5f * (p.x + xp.x), 0.5f * (p.y + yp.y), 0.5f * (p.z + zp.z));
+  return L;
+}
+// first-order departure (s = 1) / forward (s = -1) point, clamped to [0, N]
+__device__ __forceinline__ float3 traceV(const Dims& d, float3 k, float3 p, float3 v0, float s) {
+  return make_float3(clampf_(p.x - s * k.x * v0.x, 0.0f, (float)d.nx), clampf_(p.y - s * k.y * v0.y, 0.0f, (float)d.ny),
+                     clampf_(p.z - s * k.z * v0.z, 0.0f, (float)d.nz));
+}
+#define VS(T, tx, ty, tz, ch) sample1(T, d.nx + 1, d.ny + 1, d.nz + 1, tx, ty, tz, ch)
+__device__ __forceinline__ float sampU(const float4* T, const Dims& d, float3 p) { return VS(T, p.x, p.y - 0.5f, p.z - 0.5f, 0); }
+__device__ __forceinline__ float sampV(const float4* T, const Dims& d, float3 p) { return VS(T, p.x - 0.5f, p.y, p.z - 0.5f, 1); }
+__device__ __forceinline__ float sampW(const float4* T, const Dims& d, float3 p) { return VS(T, p.x - 0.5f, p.y - 0.5f, p.z, 2); }
+__device__ __forceinline__ float sampC(const float4* T, const Dims& d, float3 p) {
+  return VS(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, 3);
+}
+
+__global__ void __launch_bounds__(256) k_advv1(MU u, const float4* __restrict__ V, float4* __restrict__ Vt) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  Loc L = localVel(V, d, x, y, z);
+  float fx = (float)x, fy = (float)y, fz = (float)z;
+  float4 o;
+  o.x = sampU(V, d, traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
+  o.y = sampV(V, d, traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
+  o.z = sampW(V, d, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f));
+  o.w = sampC(V, d, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
+  Vt[vidx(d, x, y, z)] = o;
+}
+
+__device__ __forceinline__ float2 mmV(const float4* T, const Dims& d, float3 p, float ox, float oy, float oz, int ch) {
+  int i0 = (int)floorf(p.x - ox), j0 = (int)floorf(p.y - oy), k0 = (int)floorf(p.z - oz);
+  float lo = 1e30f, hi = -1e30f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float s = ldVc(T, d, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
+    lo = fminf(lo, s);
+    hi = fmaxf(hi, s);
+  }
+  return make_float2(lo, hi);
+}
+__global__ void __launch_bounds__(256) k_advv2(MU u, const float4* __restrict__ V, const float4* __restrict__ Vt,
+                                               float4* __restrict__ Vout) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  Loc L = localVel(V, d, x, y, z);
+  float fx = (float)x, fy = (float)y, fz = (float)z;
+  float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
+  float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
+  size_t i = vidx(d, x, y, z);
+  float4 hat = Vt[i], old = V[i];
+  float4 b;
+  b.x = sampU(Vt, d, traceV(d, k, pu, L.vu, -1.0f));
+  b.y = sampV(Vt, d, traceV(d, k, pv, L.vv, -1.0f));
+  b.z = sampW(Vt, d, traceV(d, k, pw, L.vw, -1.0f));
+  b.w = sampC(Vt, d, traceV(d, k, pc, L.vc, -1.0f));
+  float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
+  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
+                         hat.w + sm * (old.w - b.w));
+  float2 m;
+  m = mmV(V, d, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = clampf_(r.x, m.x, m.y);
+  m = mmV(V, d, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = clampf_(r.y, m.x, m.y);
+  m = mmV(V, d, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = clampf_(r.z, m.x, m.y);
+  m = mmV(V, d, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = clampf_(r.w, m.x, m.y);
+  Vout[i] = r;
+}
+
+// ====================== advection of moisture (model.js:482-576) ======================
+__device__ __forceinline__ float3 cellVel(const float4* V, const Dims& d, int x, int y, int z) {
+  float4 p = ldV(V, d, x, y, z);
+  return make_float3(0.5f * (p.x + ldVc(V, d, x + 1, y, z, 0)), 0.5f * (p.y + ldVc(V, d, x, y + 1, z, 1)),
+                     0.5f * (p.z + ldVc(V, d, x, y, z + 1, 2)));
+}
+__device__ __forceinline__ float3 traceQ(const Dims& d, float3 k, float3 p, float3 v0, float fall, float s) {
+  return make_float3(clampf_(p.x - s * k.x * v0.x, 0.5f, (float)d.nx - 0.5f),
+                     clampf_(p.y - s * k.y * v0.y, 0.5f, (float)d.ny - 0.5f),
+                     clampf_(p.z - s * k.z * (v0.z - fall), 0.5f, (float)d.nz - 0.5f));
+}
+__device__ __forceinline__ float4 sampQ(const float4* T, const Dims& d, float3 p) {
+  return sample4(T, d.nx, d.ny, d.nz, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f);
+}
+__device__ __forceinline__ float sampQ1(const float4* T, const Dims& d, float3 p, int ch) {
+  return sample1(T, d.nx, d.ny, d.nz, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch);
+}
+__device__ __forceinline__ void mmQ(const float4* T, const Dims& d, float3 p, float4& lo, float4& hi) {
+  int i0 = (int)floorf(p.x - 0.5f), j0 = (int)floorf(p.y - 0.5f), k0 = (int)floorf(p.z - 0.5f);
+  lo = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
+  hi = make_float4(-1e30f, -1e30f, -1e30f, -1e30f);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float4 s = ldQ(T, d, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2));
+    lo.x = fminf(lo.x, s.x); lo.y = fminf(lo.y, s.y); lo.z = fminf(lo.z, s.z); lo.w = fminf(lo.w, s.w);
+    hi.x = fmaxf(hi.x, s.x); hi.y = fmaxf(hi.y, s.y); hi.z = fmaxf(hi.z, s.z); hi.w = fmaxf(hi.w, s.w);
+  }
+}
+__device__ __forceinline__ float2 mmQ1(const float4* T, const Dims& d, float3 p, int ch) {
+  int i0 = (int)floorf(p.x - 0.5f), j0 = (int)floorf(p.y - 0.5f), k0 = (int)floorf(p.z - 0.5f);
+  const float* Tf = reinterpret_cast<const float*>(T);
+  float lo = 1e30f, hi = -1e30f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float s = Tf[4 * cidx(d, clampi(i0 + (q & 1), 0, d.nx - 1), clampi(j0 + ((q >> 1) & 1), 0, d.ny - 1),
+                         clampi(k0 + (q >> 2), 0, d.nz - 1)) + ch];
+    lo = fminf(lo, s);
+    hi = fmaxf(hi, s);
+  }
+  return make_float2(lo, hi);
+}
+
+__global__ void __launch_bounds__(256) k_advq1(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                               const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                               float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  bool thm = u.v[U_MICRO2].z > 0.5f;
+  float rsfc = u.v[U_STEP].z;
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 v = cellVel(V, d, x, y, z);
+  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  size_t i = cidx(d, x, y, z);
+  float4 q1 = Q1[i], q2 = Q2[i];
+  float rho = bsA(BS, z, d.nz).z;
+  float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
+  float fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
+  float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
+  float4 a = sampQ(Q1, d, p0);
+  float r = sampQ1(Q1, d, traceQ(d, k, p, v, fv.x, 1.0f), 3);
+  float4 b = sampQ(Q2, d, p0);
+  float s = sampQ1(Q2, d, traceQ(d, k, p, v, fs, 1.0f), 0);
+  float g = sampQ1(Q2, d, traceQ(d, k, p, v, fg, 1.0f), 1);
+  float nr = b.z;
+  if (thm) nr = sampQ1(Q2, d, traceQ(d, k, p, v, fv.y, 1.0f), 2);
+  Q1t[i] = make_float4(a.x, a.y, a.z, r);
+  Q2t[i] = make_float4(s, g, nr, b.w);
+}
+
+__global__ void __launch_bounds__(256) k_advq2(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                               const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                               const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
+                                               float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  bool thm = u.v[U_MICRO2].z > 0.5f;
+  float rsfc = u.v[U_STEP].z;
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 v = cellVel(V, d, x, y, z);
+  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  size_t i = cidx(d, x, y, z);
+  float4 q1 = Q1[i], q2 = Q2[i];
+  float rho = bsA(BS, z, d.nz).z;
+  float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
+  float fr = fv.x, fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
+  float4 h1 = Q1t[i], h2 = Q2t[i];
+  float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
+  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  float fn = thm ? fv.y : 0.0f;
+  float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f);
+  float4 b1 = sampQ(Q1t, d, f0);
+  float br = sampQ1(Q1t, d, traceQ(d, k, p, v, fr, -1.0f), 3);
+  float4 b2 = sampQ(Q2t, d, f0);
+  float bs = sampQ1(Q2t, d, traceQ(d, k, p, v, fs, -1.0f), 0);
+  float bg = sampQ1(Q2t, d, traceQ(d, k, p, v, fg, -1.0f), 1);
+  float bn = sampQ1(Q2t, d, traceQ(d, k, p, v, fn, -1.0f), 2);
+  float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - b1.z),
+                          h1.w + sm * (q1.w - br));
+  float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
+                          h2.w + sm * (q2.w - b2.w));
+  float3 d0 = traceQ(d, k, p, v, 0.0f, 1.0f);
+  float4 lo, hi;
+  mmQ(Q1, d, d0, lo, hi);
+  r1.x = clampf_(r1.x, lo.x, hi.x);
+  r1.y = clampf_(r1.y, lo.y, hi.y);
+  r1.z = clampf_(r1.z, lo.z, hi.z);
+  float2 m = mmQ1(Q1, d, traceQ(d, k, p, v, fr, 1.0f), 3);
+  r1.w = clampf_(r1.w, m.x, m.y);
+  m = mmQ1(Q2, d, traceQ(d, k, p, v, fs, 1.0f), 0);
+  r2.x = clampf_(r2.x, m.x, m.y);
+  m = mmQ1(Q2, d, traceQ(d, k, p, v, fg, 1.0f), 1);
+  r2.y = clampf_(r2.y, m.x, m.y);
+  m = mmQ1(Q2, d, traceQ(d, k, p, v, fn, 1.0f), 2);
+  r2.z = clampf_(r2.z, m.x, m.y);
+  m = mmQ1(Q2, d, d0, 3);
+  r2.w = clampf_(r2.w, m.x, m.y);
+  Q1o[i] = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
+  Q2o[i] = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
+}
+
+// ====================== forces + microphysics (model.js:579-907) ======================
+__device__ __forceinline__ float lapH(const float4* V, const Dims& d, int x, int y, int z, int ch, float me) {
+  return ldVc(V, d, x + 1, y, z, ch) + ldVc(V, d, x - 1, y, z, ch) + ldVc(V, d, x, y + 1, z, ch) +
+         ldVc(V, d, x, y - 1, z, ch) - 4.0f * me;
+}
+__device__ __forceinline__ float bsV(const float4* BS, const Dims& d, int k, int ch) {
+  int kk = clampi(k, 0, d.nz - 1);
+  if (ch == 0) return bsB(BS, kk, d.nz).z;
+  if (ch == 1) return bsB(BS, kk, d.nz).w;
+  if (ch == 3) return bsA(BS, kk, d.nz).x;
+  return 0.0f;
+}
+__device__ __forceinline__ float lapZp(const float4* V, const float4* BS, const Dims& d, int x, int y, int z, int ch,
+                                       float me, bool lo, bool hi) {
+  float m0 = me - bsV(BS, d, z, ch);
+  float a = lo ? m0 : ldVc(V, d, x, y, z - 1, ch) - bsV(BS, d, z - 1, ch);
+  float b = hi ? m0 : ldVc(V, d, x, y, z + 1, ch) - bsV(BS, d, z + 1, ch);
+  return a + b - 2.0f * m0;
+}
+__device__ __forceinline__ float spongeLat(const MU& u, const Dims& d, float xm, float ym) {
+  float Lx = (float)d.nx * d.dx, Ly = (float)d.ny * d.dx;
+  float e = fminf(fminf(xm, Lx - xm), fminf(ym, Ly - ym));
+  float dd = clampf_(1.0f - e / u.v[U_SPONGE].x, 0.0f, 1.0f);
+  return dd * (0.3f + 0.7f * dd) / u.v[U_SPONGE].y;
+}
+__device__ __forceinline__ float spongeTop(const MU& u, const Dims& d, float z) {
+  float ztop = (float)d.nz * d.dz;
+  if (z < u.v[U_SPONGE].z) return 0.0f;
+  float s = sinf(1.5707963f * (z - u.v[U_SPONGE].z) / (ztop - u.v[U_SPONGE].z));
+  return s * s / u.v[U_SPONGE].w;
+}
+__device__ __forceinline__ float buoy(const float4* V, const float4* Q1, const float4* Q2, const float4* BS,
+                                      const Dims& d, int x, int y, int z) {
+  float4 A = bsA(BS, z, d.nz);
+  float th = ldVc(V, d, x, y, z, 3);
+  float4 q1 = ldQ(Q1, d, x, y, z), q2 = ldQ(Q2, d, x, y, z);
+  return 9.81f * ((th - A.x) / A.x + 0.608f * (q1.x - A.y) - (q1.y + q1.z + q1.w + q2.x + q2.y));
+}
+__device__ __forceinline__ float2 ldA(const float2* aux, const Dims& d, int x, int y, int z) {
+  return aux[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
+}
+__device__ __forceinline__ float ldP(const float* phT, const Dims& d, int x, int y, int z) {
+  return phT[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
+}
+
+// The microphysics block (model.js:775-899): T, qv..qg, nr in and out.
+__device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, float rho, bool thm, float& T, float& qv,
+                                            float& qc, float& qi, float& qr, float& qs, float& qg, float& nr) {
+  float rsfc = u.v[U_STEP].z;
+  float rq = 1000.0f * rho;
+  float rs = sqrtf(rsfc / rho);
+  float pauto = u.v[U_MICRO].x * fmaxf(qc - u.v[U_MICRO].y, 0.0f);
+  float accr = 2.2f * qc * powf(qr, 0.875f);
+  if (thm && qr > 1e-9f) {
+    float l = lamR(qr, nr);
+    accr = 3100.0f * rho * nr * powf(l, -2.8f) * qc * rs;
+  }
+  float da = fminf(pauto * dt, qc);
+  float dc = fminf(accr * dt, qc - da);
+  qc -= da + dc;
+  qr += da + dc;
+  nr += da / 2.68e-10f;
+  float dTc = T - OBL_T0K;
+  if (dTc < 0.0f) {
+    float e25 = expf(0.025f * dTc);
+    float pqs = powf(fmaxf(rq * qs, 0.0f), 0.81f);
+    float pqg = powf(fmaxf(rq * qg, 0.0f), 0.875f);
+    float psaut = 1e-3f * e25 * fmaxf(qi - 6e-4f, 0.0f);
+    float psacw = 5.4e-3f * qc * pqs * rs;
+    float psaci = 5.4e-3f * qi * pqs * e25 * rs;
+    float ewet = 1.0f / (1.0f + qg / 0.008f);
+    float pgacw = 3.1e-3f * qc * pqg * rs * ewet;
+    float pgaci = 3.0e-4f * qi * pqg * rs;
+    float pgacr = 3.1e-3f * qr * pqg * rs * ewet;
+    float pgfr = fminf(2e-4f * u.v[U_MICRO].z * (expf(0.66f * fminf(-dTc, 20.0f)) - 1.0f), 0.08f) * qr;
+    float pgaut = 1e-3f * expf(0.09f * dTc) * fmaxf(qs - 6e-4f, 0.0f) * (thm ? 0.25f : 1.0f);
+    float a1 = fminf((psaut + psaci + pgaci) * dt, qi);
+    float fs1 = (psaut + psaci) / fmaxf(psaut + psaci + pgaci, 1e-20f);
+    qi -= a1;
+    qs += a1 * fs1;
+    qg += a1 * (1.0f - fs1);
+    float a2 = fminf((psacw + pgacw) * dt, qc);
+    float fs2 = psacw / fmaxf(psacw + pgacw, 1e-20f);
+    float rimeS = thm ? 0.95f : 0.7f;
+    qc -= a2;
+    qs += a2 * fs2 * rimeS;
+    qg += a2 * (1.0f - fs2 * rimeS);
+    float a3 = fminf((pgacr + pgfr) * dt, qr);
+    if (T < 235.0f) a3 = qr;
+    if (qr > 1e-12f) nr *= (qr - a3) / qr;
+    qr -= a3;
+    qg += a3;
+    float a4 = fminf(pgaut * dt, qs);
+    qs -= a4;
+    qg += a4;
+    T += OBL_LFC / OBL_CPD * (a2 + a3);
+  } else {
+    float m1 = fminf(0.012f * dTc * qs * dt, qs);
+    float m2 = fminf(0.0022f * dTc * qg * dt, qg);
+    if (thm) {
+      if (qg > 1e-9f) {
+        float l = lamG(qg, rho), n0 = n0G(qg, rho);
+        float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.89f * 5430.0f * powf(rsfc / rho, 0.25f) * powf(l, -2.945f);
+        float heat = 2.4e-2f * dTc - OBL_LVC * 2.26e-5f * rho * (qvsat(OBL_T0K, p0, 0.0f) - qv);
+        m2 = clampf_(6.2832f / (rho * OBL_LFC) * heat * n0 * vent * dt, 0.0f, qg);
+        nr += m2 / qg * n0 / (rho * l);
+      }
+      if (qs > 1e-9f) {
+        float ls = powf(3.14159f * 100.0f * 2e6f / (rho * qs), 0.25f);
+        nr += m1 / qs * 2e6f / (rho * ls);
+      }
+    }
+    qs -= m1;
+    qg -= m2;
+    qr += m1 + m2;
+    T -= OBL_LFC / OBL_CPD * (m1 + m2);
+  }
+  float qvsl = qvsat(T, p0, 0.0f);
+  if (qr > 0.0f && qv < qvsl) {
+    float rc = 0.001f * rho * qr;
+    float ern = ((1.6f + 124.9f * powf(rc, 0.2046f)) * powf(rc, 0.525f) / (2.55e8f / (p0 * qvsl) + 5.4e5f)) *
+                ((qvsl - qv) / (0.001f * rho * qvsl));
+    float srt = u.v[U_MICRO2].y > 0.0f ? sqrtf(rc / (rc + 0.001f * rho * u.v[U_MICRO2].y)) : 1.0f;
+    float e = ern * srt * dt * u.v[U_TURB].z;
+    if (thm) {
+      float l = lamR(qr, nr), n0 = rho * nr * l;
+      float Ap = OBL_LVC * OBL_LVC / (2.4e-2f * OBL_RVC * T * T), Bp = 1.0f / (rho * qvsl * 2.26e-5f);
+      float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.827f * 7492.0f * powf(rsfc / rho, 0.25f) * powf(l, -2.9f);
+      e = 6.2832f * (1.0f - qv / qvsl) * n0 * vent / (rho * (Ap + Bp)) * dt * u.v[U_MICRO2].w;
+    }
+    e = clampf_(e, 0.0f, fminf(qr, qvsl - qv));
+    if (thm && qr > 1e-12f) nr *= (qr - e) / qr;
+    qr -= e;
+    qv += e;
+    T -= OBL_LVC / OBL_CPD * e;
+  }
+  if (thm) {
+    if (qr > 1e-9f) {
+      float dm = 1.817f / lamR(qr, nr);
+      float fb = clampf_(1.0f - expf(2300.0f * (dm - 1.1e-3f)), -2.0f, 1.0f);
+      nr *= expf(-7.12f * rho * qr * fb * dt);
+      nr = clampf_(nr, qr * 3.183e5f, qr * 3.183e11f);
+    } else {
+      nr = 0.0f;
+    }
+  }
+  float qvsi = qvsat(T, p0, 1.0f);
+  if (dTc < 0.0f && qv < qvsi) {
+    float sub = fmaxf(1.0f - qv / qvsi, 0.0f);
+    float e = fminf((2e-3f * qs + 5e-4f * qg) * sub * dt, (qvsi - qv) * 0.5f);
+    float fsn = qs / fmaxf(qs + qg, 1e-20f);
+    e = fminf(e, qs + qg);
+    qs -= e * fsn;
+    qg -= e * (1.0f - fsn);
+    qv += e;
+    T -= OBL_LSC / OBL_CPD * e;
+  }
+  float qi0 = qi, qcl = qc + qi, dsum = 0.0f;
+#pragma unroll
+  for (int it = 0; it < 3; it++) {
+    float f = fIce(T);
+    float qs_ = qvsat(T, p0, f);
+    float L = mixf(OBL_LVC, OBL_LSC, f);
+    float dd = (qv - qs_) / (1.0f + L * L * qs_ / (OBL_CPD * OBL_RVC * T * T));
+    dd = fmaxf(dd, -qcl);
+    qv -= dd;
+    qcl += dd;
+    T += L / OBL_CPD * dd;
+    dsum += dd;
+  }
+  float f = fIce(T);
+  qi = f * qcl;
+  qc = qcl - qi;
+  T += OBL_LFC / OBL_CPD * (qi - qi0 - f * dsum);
+}
+
+__global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
+                                              const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
+                                              const float2* __restrict__ aux, const float4* __restrict__ lsfA,
+                                              const float4* __restrict__ lsfB, const float* __restrict__ phT,
+                                              const float4* __restrict__ ciMap, float4* __restrict__ Vout,
+                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  int nx = d.nx, ny = d.ny, nz = d.nz;
+  float dx = d.dx, dz = d.dz, dt = d.dt;
+  float ix2 = 1.0f / (dx * dx), iz2 = 1.0f / (dz * dz);
+  float fc = u.v[U_COR].x, nudR = 1.0f / u.v[U_COR].y;
+  float nu0 = u.v[U_TURB].y;
+  bool thm = u.v[U_MICRO2].z > 0.5f;
+  bool drag = u.v[U_TURB].w > 0.5f;
+  size_t vi = vidx(d, x, y, z);
+  float4 V = Vin[vi], o = V;
+  // ---- u ----
+  if (y < ny && z < nz) {
+    float xmx = (float)x * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
+    float2 a0 = ldA(aux, d, x - 1, y, z), a1 = ldA(aux, d, x, y, z);
+    float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
+    float du = nux * lapH(Vin, d, x, y, z, 0, V.x) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 0, V.x, z == 0, z == nz - 1) * iz2;
+    float ut = tgtU(BS, lsfB, d, x, y, z);
+    float vbar = 0.25f * (ldVc(Vin, d, x - 1, y, z, 1) + V.y + ldVc(Vin, d, x - 1, y + 1, z, 1) + ldVc(Vin, d, x, y + 1, z, 1));
+    float vtb = 0.5f * (tgtV(BS, lsfB, d, x - 1, y, z) + tgtV(BS, lsfB, d, x, y + 1, z));
+    float nw = 0.5f * (ld2d(lsfB, d, x - 1, y).w + ld2d(lsfB, d, x, y).w);
+    du += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (ut - V.x);
+    du += fc * (vbar - vtb);
+    if (x > 0 && x < nx) du -= (ldP(phT, d, x, y, z) - ldP(phT, d, x - 1, y, z)) / dx;
+    if (z == 0 && drag) {
+      float lg = sqrtf(V.x * V.x + vbar * vbar), lg0 = sqrtf(ut * ut + vtb * vtb);
+      du += u.v[U_MICRO].w * (-lg * V.x + lg0 * ut) / dz;
+    }
+    o.x = V.x + dt * du;
+  } else {
+    o.x = 0.0f;
+  }
+  // ---- v ----
+  if (x < nx && z < nz) {
+    float xmx = ((float)x + 0.5f) * dx, xmy = (float)y * dx, zz = ((float)z + 0.5f) * dz;
+    float2 a0 = ldA(aux, d, x, y - 1, z), a1 = ldA(aux, d, x, y, z);
+    float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
+    float dv = nux * lapH(Vin, d, x, y, z, 1, V.y) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 1, V.y, z == 0, z == nz - 1) * iz2;
+    float vt = tgtV(BS, lsfB, d, x, y, z);
+    float ubar = 0.25f * (ldVc(Vin, d, x, y - 1, z, 0) + V.x + ldVc(Vin, d, x + 1, y - 1, z, 0) + ldVc(Vin, d, x + 1, y, z, 0));
+    float utb = 0.5f * (tgtU(BS, lsfB, d, x, y - 1, z) + tgtU(BS, lsfB, d, x + 1, y, z));
+    float nw = 0.5f * (ld2d(lsfB, d, x, y - 1).w + ld2d(lsfB, d, x, y).w);
+    dv += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (vt - V.y);
+    dv -= fc * (ubar - utb);
+    if (y > 0 && y < ny) dv -= (ldP(phT, d, x, y, z) - ldP(phT, d, x, y - 1, z)) / dx;
+    if (z == 0 && drag) {
+      float lg = sqrtf(ubar * ubar + V.y * V.y), lg0 = sqrtf(utb * utb + vt * vt);
+      dv += u.v[U_MICRO].w * (-lg * V.y + lg0 * vt) / dz;
+    }
+    o.y = V.y + dt * dv;
+  } else {
+    o.y = 0.0f;
+  }
+  // ---- w ----
+  if (x < nx && y < ny && z > 0 && z < nz) {
+    float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = (float)z * dz;
+    float2 a0 = ldA(aux, d, x, y, z - 1), a1 = ldA(aux, d, x, y, z);
+    float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
+    float lz = ldVc(Vin, d, x, y, z - 1, 2) + ldVc(Vin, d, x, y, z + 1, 2) - 2.0f * V.z;
+    float dw = nux * lapH(Vin, d, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
+    dw += 0.5f * (buoy(Vin, Q1in, Q2in, BS, d, x, y, z - 1) + buoy(Vin, Q1in, Q2in, BS, d, x, y, z)) -
+          (ldP(phT, d, x, y, z) - ldP(phT, d, x, y, z - 1)) / dz;
+    dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
+    float4 ci = ciMap[(size_t)x + (size_t)nx * y];
+    if (ci.x > 0.0f) {
+      float zr = fabsf(zz - ci.z) / ci.w;
+      if (zr < 1.0f) {
+        float cv = cosf(1.5707963f * zr);
+        dw += u.v[U_MISC].w * ci.x * cv * fmaxf(ci.y * cv - V.z, 0.0f);
+      }
+    }
+    o.z = V.z + dt * dw;
+  } else {
+    o.z = 0.0f;
+  }
+  // ---- theta + microphysics ----
+  if (x < nx && y < ny && z < nz) {
+    float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
+    float4 A = bsA(BS, z, nz), B = bsB(BS, z, nz);
+    float4 LA = ld2d(lsfA, d, x, y);
+    float nw = ld2d(lsfB, d, x, y).w;
+    float tth = tgtTh(BS, d, LA, z), tqv = tgtQv(BS, d, LA, z);
+    float sl = spongeLat(u, d, xmx, xmy);
+    float relax = sl + nw * nudR;
+    float2 a1 = ldA(aux, d, x, y, z);
+    float nux = a1.x * 3.0f + nu0, nuy = a1.y * 3.0f + nu0;
+    float th = V.w + dt * (nux * lapH(Vin, d, x, y, z, 3, V.w) * ix2 +
+                           nuy * lapZp(Vin, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
+                           (relax + spongeTop(u, d, zz)) * (tth - V.w));
+    size_t ci_ = cidx(d, x, y, z);
+    float4 q1 = Q1in[ci_], q2 = Q2in[ci_];
+    float4 e1 = ldQ(Q1in, d, x + 1, y, z), w1 = ldQ(Q1in, d, x - 1, y, z);
+    float4 n1 = ldQ(Q1in, d, x, y + 1, z), s1 = ldQ(Q1in, d, x, y - 1, z);
+    float4 dn = z == 0 ? q1 : ldQ(Q1in, d, x, y, z - 1);
+    float4 up = z == nz - 1 ? q1 : ldQ(Q1in, d, x, y, z + 1);
+    float l1x = e1.x + w1.x + n1.x + s1.x - 4.0f * q1.x, l1y = e1.y + w1.y + n1.y + s1.y - 4.0f * q1.y;
+    float l1zz = e1.z + w1.z + n1.z + s1.z - 4.0f * q1.z;
+    float lzx = dn.x + up.x - 2.0f * q1.x, lzy = dn.y + up.y - 2.0f * q1.y, lzz = dn.z + up.z - 2.0f * q1.z;
+    float qvb = (z == 0 ? A.y : bsA(BS, z - 1, nz).y) + (z == nz - 1 ? A.y : bsA(BS, z + 1, nz).y) - 2.0f * A.y;
+    lzx -= qvb;
+    q1.x = q1.x + dt * (nux * l1x * ix2 + nuy * lzx * iz2);
+    q1.y = q1.y + dt * (nux * l1y * ix2 + nuy * lzy * iz2);
+    q1.z = q1.z + dt * (nux * l1zz * ix2 + nuy * lzz * iz2);
+    q1.x += dt * relax * (tqv - q1.x);
+    q1.x = fmaxf(q1.x, 0.0f); q1.y = fmaxf(q1.y, 0.0f); q1.z = fmaxf(q1.z, 0.0f); q1.w = fmaxf(q1.w, 0.0f);
+    q2.x = fmaxf(q2.x, 0.0f); q2.y = fmaxf(q2.y, 0.0f); q2.z = fmaxf(q2.z, 0.0f); q2.w = fmaxf(q2.w, 0.0f);
+    float T = th * B.x;
+    float qv = q1.x, qc = q1.y, qi = q1.z, qr = q1.w, qs = q2.x, qg = q2.y, nr = fmaxf(q2.z, 0.0f);
+    micro_block(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
+    th = T / B.x;
+    float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
+    qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
+    Q1o[ci_] = make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f));
+    Q2o[ci_] = make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w);
+    o.w = th;
+  }
+  float sp = sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
+  float vcap = u.v[U_MISC].x;
+  if (sp > vcap) {
+    float f = vcap / sp;
+    o.x *= f; o.y *= f; o.z *= f;
+  }
+  if (o.x != o.x || o.y != o.y || o.z != o.z || o.w != o.w) {
+    float4 Bz = bsB(BS, z, nz);
+    o = make_float4(Bz.z, Bz.w, 0.0f, bsA(BS, z, nz).x);
+  }
+  Vout[vi] = o;
+}
+
+// ====================== DIV (model.js:910-925) ======================
+__global__ void __launch_bounds__(256) k_div(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                             float* __restrict__ rhs) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  float4 p = V[vidx(d, x, y, z)];
+  float rc = bsA(BS, z, d.nz).z;
+  float wt = z == d.nz - 1 ? 0.0f : ldVc(V, d, x, y, z + 1, 2);
+  float wb = z == 0 ? 0.0f : p.z;
+  float dd = rc * (ldVc(V, d, x + 1, y, z, 0) - p.x + ldVc(V, d, x, y + 1, z, 1) - p.y) / d.dx +
+             (rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb) / d.dz;
+  int pd = (int)u.v[U_PAD].x, nxp = (int)u.v[U_PAD].y, nyp = (int)u.v[U_PAD].z;
+  rhs[(size_t)(x + pd) + (size_t)nxp * ((size_t)(y + pd) + (size_t)nyp * z)] = dd / d.dt;
+}
+
+// ====================== PROJ (model.js:928-946) ======================
+__device__ __forceinline__ float Pphi(const float* phi, const MU& u, const Dims& d, int x, int y, int z) {
+  int p = (int)u.v[U_PAD].x, nxp = (int)u.v[U_PAD].y, nyp = (int)u.v[U_PAD].z;
+  x = clampi(x, -p, d.nx + p - 1);
+  y = clampi(y, -p, d.ny + p - 1);
+  z = clampi(z, 0, d.nz - 1);
+  return phi[(size_t)(x + p) + (size_t)nxp * ((size_t)(y + p) + (size_t)nyp * z)];
+}
+__global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V, const float* __restrict__ phi,
+                                              float4* __restrict__ Vout) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  size_t i = vidx(d, x, y, z);
+  float4 Vv = V[i], o = Vv;
+  float pc = Pphi(phi, u, d, x, y, z);
+  if (y < d.ny && z < d.nz) o.x = Vv.x - kx * (pc - Pphi(phi, u, d, x - 1, y, z));
+  if (x < d.nx && z < d.nz) o.y = Vv.y - kx * (pc - Pphi(phi, u, d, x, y - 1, z));
+  if (x < d.nx && y < d.ny && z > 0 && z < d.nz) o.z = Vv.z - kz * (pc - Pphi(phi, u, d, x, y, z - 1));
+  else o.z = 0.0f;
+  Vout[i] = o;
+}
+
+// ====================== launchers ======================
+static dim3 faces_grid(const Dims& d) { return dim3((d.nx + 1 + 63) / 64, (d.ny + 1 + 3) / 4, d.nz + 1); }
+static dim3 cells_grid(const Dims& d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
+static dim3 cols_grid(const Dims& d) { return dim3((d.nx + 15) / 16, (d.ny + 15) / 16, 1); }
+
+void launch_lsf(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float* BD, float4* lsfA,
+                float4* lsfB, float* phT) {
+  hipLaunchKernelGGL(k_lsf, cols_grid(d), dim3(16, 16, 1), 0, s, u, BS, BD, lsfA, lsfB, phT);
+}
+void launch_ci(hipStream_t s, const MU& u, const Dims& d, const float* CI, float4* ciMap) {
+  hipLaunchKernelGGL(k_ci, cols_grid(d), dim3(16, 16, 1), 0, s, u, CI, ciMap);
+}
+void launch_init(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* lsfA, const float4* lsfB,
+                 float4* V, float4* Q1, float4* Q2, int seed) {
+  hipLaunchKernelGGL(k_init, faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, lsfA, lsfB, V, Q1, Q2, seed);
+}
+void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4* V1, double x, double y, double z,
+                   double rh, double rz, double dth) {
+  hipLaunchKernelGGL(k_bubble, faces_grid(d), dim3(64, 4, 1), 0, s, u, V0, V1, x, y, z, rh, rz, dth);
+}
+void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                float2* aux) {
+  hipLaunchKernelGGL(k_nut, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, aux);
+}
+void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt) {
+  hipLaunchKernelGGL(k_advv1, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vt);
+}
+void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout) {
+  hipLaunchKernelGGL(k_advv2, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vt, Vout);
+}
+void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                  const float4* Q2, float4* Q1t, float4* Q2t) {
+  hipLaunchKernelGGL(k_advq1, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
+}
+void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                  const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o) {
+  hipLaunchKernelGGL(k_advq2, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
+}
+void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                 const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                 const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
+  hipLaunchKernelGGL(k_phys, faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT, ciMap,
+                     Vout, Q1o, Q2o);
+}
+void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
+  hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);
+}
+void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout) {
+  hipLaunchKernelGGL(k_proj, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, phi, Vout);
+}
+
+}  // namespace obl

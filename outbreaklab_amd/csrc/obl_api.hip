@@ -1,0 +1,659 @@
+// C ABI of the gfx950 model step (declared in include/outbreak_hip.h).
+//
+// Owns the device fields of one RegionalModel, its HIP stream and its
+// PoissonDirect, and records the step's launches in the reference's order
+// (RegionalModel.encStep, model.js:1442-1476).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/outbreak_hip.h"
+#include "obl_kernels.h"
+
+using namespace obl;
+
+static thread_local std::string g_err;
+
+static int set_global(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+// ---------------------------------------------------------------- timing
+struct Timer {
+  bool on = false;
+  std::vector<std::string> names;
+  std::vector<double> total;
+  std::vector<int> count;
+  struct Pend {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Pend> pend;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+  }
+  int id_of(const char* n) {
+    for (size_t i = 0; i < names.size(); i++)
+      if (names[i] == n) return (int)i;
+    names.push_back(n);
+    total.push_back(0.0);
+    count.push_back(0);
+    return (int)names.size() - 1;
+  }
+  void begin(hipStream_t s, const char* n, hipEvent_t* a) {
+    if (!on) return;
+    (void)n;
+    *a = ev();
+    hipEventRecord(*a, s);
+  }
+  void end(hipStream_t s, const char* n, hipEvent_t a) {
+    if (!on) return;
+    hipEvent_t b = ev();
+    hipEventRecord(b, s);
+    pend.push_back({id_of(n), a, b});
+  }
+  void collect() {
+    for (auto& p : pend) {
+      hipEventSynchronize(p.b);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, p.a, p.b);
+      total[p.id] += ms;
+      count[p.id] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pend.clear();
+  }
+  void reset() {
+    collect();
+    for (auto& t : total) t = 0.0;
+    for (auto& c : count) c = 0;
+  }
+  ~Timer() {
+    for (auto& p : pend) {
+      hipEventDestroy(p.a);
+      hipEventDestroy(p.b);
+    }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+#define TIMED(T, S, NAME, CALL)   \
+  do {                            \
+    hipEvent_t _a = nullptr;      \
+    (T).begin((S), (NAME), &_a);  \
+    CALL;                         \
+    (T).end((S), (NAME), _a);     \
+  } while (0)
+
+// ---------------------------------------------------------------- PoissonDirect
+struct AxisMats {
+  int N = 0, Nh = 0, Npad = 0, Kp = 0;
+  float* W[4] = {nullptr, nullptr, nullptr, nullptr};  // We, Wo, WeT, WoT
+};
+
+struct obl_poisson {
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0, dz = 0;
+  hipStream_t stream = nullptr;
+  AxisMats ax, ay;
+  float *tmp = nullptr, *hat = nullptr, *cp32 = nullptr, *lam32 = nullptr;
+  double *cp64 = nullptr, *dp64 = nullptr, *lam64 = nullptr, *zop64 = nullptr;
+  float4* zop32 = nullptr;
+  float *hrhs = nullptr, *hphi = nullptr;  // device staging for solve_host
+  std::string err;
+  Timer timer;
+  int fail(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return -1;
+  }
+};
+
+static int hip_ok(hipError_t e, std::string* err, const char* what) {
+  if (e == hipSuccess) return 0;
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  if (err) *err = buf;
+  else g_err = buf;
+  return -1;
+}
+
+// orthonormal DST-II v_l(i) = c_l sin(pi (l+1)(i+1/2)/N), c_l = sqrt(2/N) (sqrt(1/N) for
+// l = N-1), computed in FP64 and rounded to FP32 (solver.js:115-123), split into the
+// folded even/odd half matrices.
+static int build_axis(AxisMats& A, int N, std::string* err) {
+  A.N = N;
+  A.Nh = N / 2;
+  A.Npad = (A.Nh + 63) / 64 * 64;
+  A.Kp = (A.Nh + 15) / 16 * 16;
+  size_t n = (size_t)A.Npad * A.Kp;
+  std::vector<float> h[4];
+  for (int t = 0; t < 4; t++) h[t].assign(n, 0.0f);
+  auto v = [&](int l, int i) {
+    double c = sqrt((l == N - 1 ? 1.0 : 2.0) / N);
+    return c * sin(M_PI * (l + 1) * (i + 0.5) / N);
+  };
+  for (int s = 0; s < A.Nh; s++)
+    for (int i = 0; i < A.Nh; i++) {
+      float ve = (float)v(2 * s, i), vo = (float)v(2 * s + 1, i);
+      h[0][(size_t)s * A.Kp + i] = ve;  // We[s][i]
+      h[1][(size_t)s * A.Kp + i] = vo;  // Wo[s][i]
+      h[2][(size_t)i * A.Kp + s] = ve;  // WeT[i][s]
+      h[3][(size_t)i * A.Kp + s] = vo;  // WoT[i][s]
+    }
+  for (int t = 0; t < 4; t++) {
+    if (hip_ok(hipMalloc(&A.W[t], n * sizeof(float)), err, "hipMalloc(sine matrix)")) return -1;
+    if (hip_ok(hipMemcpy(A.W[t], h[t].data(), n * sizeof(float), hipMemcpyHostToDevice), err, "hipMemcpy(sine matrix)"))
+      return -1;
+  }
+  return 0;
+}
+
+static void free_axis(AxisMats& A) {
+  for (int t = 0; t < 4; t++)
+    if (A.W[t]) hipFree(A.W[t]);
+}
+
+// permuted eigenvalues: spectral slot s < N/2 holds mode 2s, slot N/2 + s holds mode 2s+1
+static void perm_eig(int N, double dx, double* out) {
+  int Nh = N / 2;
+  for (int s = 0; s < N; s++) {
+    int l = s < Nh ? 2 * s : 2 * (s - Nh) + 1;
+    double sn = sin(M_PI * (l + 1) / (2.0 * N));
+    out[s] = -4.0 / (dx * dx) * sn * sn;
+  }
+}
+
+extern "C" int obl_poisson_set_density(obl_poisson* p, const double* rhoC, const double* rhoW) {
+  if (!p || !rhoC || !rhoW) return set_global("obl_poisson_set_density: null argument");
+  int nz = p->nz;
+  double a = 1.0 / (p->dz * p->dz);
+  std::vector<double> z64(4 * (size_t)nz);
+  std::vector<float4> z32(nz);
+  for (int k = 0; k < nz; k++) {
+    double lo = k > 0 ? rhoW[k] * a : 0.0, up = k < nz - 1 ? rhoW[k + 1] * a : 0.0;
+    z64[4 * k] = lo; z64[4 * k + 1] = up; z64[4 * k + 2] = -(lo + up); z64[4 * k + 3] = rhoC[k];
+    z32[k] = make_float4((float)lo, (float)up, (float)(-(lo + up)), (float)rhoC[k]);
+  }
+  if (hip_ok(hipMemcpyAsync(p->zop64, z64.data(), z64.size() * sizeof(double), hipMemcpyHostToDevice, p->stream), &p->err,
+             "setDensity"))
+    return -1;
+  if (hip_ok(hipMemcpyAsync(p->zop32, z32.data(), z32.size() * sizeof(float4), hipMemcpyHostToDevice, p->stream), &p->err,
+             "setDensity"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(p->stream), &p->err, "setDensity");
+}
+
+extern "C" void obl_poisson_destroy(obl_poisson* p) {
+  if (!p) return;
+  if (p->stream) hipStreamSynchronize(p->stream);
+  free_axis(p->ax);
+  free_axis(p->ay);
+  void* bufs[] = {p->tmp, p->hat, p->cp32, p->lam32, p->cp64, p->dp64, p->lam64, p->zop64, p->zop32, p->hrhs, p->hphi};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+}
+
+extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, obl_poisson** out) {
+  if (!out) return set_global("obl_poisson_create: null output");
+  *out = nullptr;
+  if (nx < 8 || ny < 8 || nz < 2 || nx % 8 || ny % 8)
+    return set_global("PoissonDirect: nx and ny must be multiples of 8 (>= 8) and nz >= 2, got %d x %d x %d", nx, ny, nz);
+  if (!(dx > 0) || !(dz > 0)) return set_global("PoissonDirect: dx and dz must be positive");
+  obl_poisson* p = new obl_poisson();
+  p->nx = nx; p->ny = ny; p->nz = nz; p->dx = dx; p->dz = dz;
+  std::string e;
+  size_t cells = (size_t)nx * ny * nz;
+  if (hip_ok(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), &e, "hipStreamCreate") ||
+      build_axis(p->ax, nx, &e) || build_axis(p->ay, ny, &e) ||
+      hip_ok(hipMalloc(&p->tmp, cells * 4), &e, "hipMalloc(tmp)") ||
+      hip_ok(hipMalloc(&p->hat, cells * 4), &e, "hipMalloc(hat)") ||
+      hip_ok(hipMalloc(&p->cp32, cells * 4), &e, "hipMalloc(cp)") ||
+      hip_ok(hipMalloc(&p->cp64, cells * 8), &e, "hipMalloc(cp64)") ||
+      hip_ok(hipMalloc(&p->dp64, cells * 8), &e, "hipMalloc(dp64)") ||
+      hip_ok(hipMalloc(&p->lam32, (nx + ny) * 4), &e, "hipMalloc(lam)") ||
+      hip_ok(hipMalloc(&p->lam64, (nx + ny) * 8), &e, "hipMalloc(lam64)") ||
+      hip_ok(hipMalloc(&p->zop32, nz * 16), &e, "hipMalloc(zop)") ||
+      hip_ok(hipMalloc(&p->zop64, nz * 32), &e, "hipMalloc(zop64)")) {
+    obl_poisson_destroy(p);
+    return set_global("PoissonDirect(%d, %d, %d): %s", nx, ny, nz, e.c_str());
+  }
+  std::vector<double> l64(nx + ny);
+  perm_eig(nx, dx, l64.data());
+  perm_eig(ny, dx, l64.data() + nx);
+  std::vector<float> l32(nx + ny);
+  for (int i = 0; i < nx + ny; i++) l32[i] = (float)l64[i];
+  hipMemcpy(p->lam64, l64.data(), l64.size() * 8, hipMemcpyHostToDevice);
+  hipMemcpy(p->lam32, l32.data(), l32.size() * 4, hipMemcpyHostToDevice);
+  std::vector<double> ones(nz + 1, 1.0);
+  if (obl_poisson_set_density(p, ones.data(), ones.data())) {
+    std::string m = p->err;
+    obl_poisson_destroy(p);
+    return set_global("%s", m.c_str());
+  }
+  *out = p;
+  return 0;
+}
+
+static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, int thomas, double cond,
+                       hipStream_t s) {
+  if (gemm != OBL_GEMM_FP32 && gemm != OBL_GEMM_BF16X3) return p->fail("unknown gemm mode %d", gemm);
+  if (thomas < 0 || thomas > 2) return p->fail("unknown Thomas mode %d", thomas);
+  int nx = p->nx, ny = p->ny, nz = p->nz;
+  DstArgs a;
+  // forward x: rhs -> tmp
+  a = {rhs, p->tmp, p->ax.W[0], p->ax.W[1], nx, nx / 2, ny * nz, nx, p->ax.Kp};
+  TIMED(p->timer, s, "ps.xf", launch_dst(s, DST_XF, gemm, a));
+  // forward y: tmp -> hat
+  a = {p->tmp, p->hat, p->ay.W[0], p->ay.W[1], ny, ny / 2, nz, nx, p->ay.Kp};
+  TIMED(p->timer, s, "ps.yf", launch_dst(s, DST_YF, gemm, a));
+  ThomasArgs t;
+  t.hat = p->hat; t.cp32 = p->cp32; t.cp64 = p->cp64; t.dp64 = p->dp64;
+  t.lam32 = p->lam32; t.lam64 = p->lam64; t.zop32 = p->zop32; t.zop64 = p->zop64;
+  t.nx = nx; t.ny = ny; t.nz = nz; t.mode = thomas;
+  // FP64 where the estimated condition 4 / (|shift| dz^2) exceeds `cond`
+  t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
+  TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
+  // inverse y: hat -> tmp
+  a = {p->hat, p->tmp, p->ay.W[2], p->ay.W[3], ny, ny / 2, nz, nx, p->ay.Kp};
+  TIMED(p->timer, s, "ps.yi", launch_dst(s, DST_YI, gemm, a));
+  // inverse x: tmp -> phi
+  a = {p->tmp, phi, p->ax.W[2], p->ax.W[3], nx, nx / 2, ny * nz, nx, p->ax.Kp};
+  TIMED(p->timer, s, "ps.xi", launch_dst(s, DST_XI, gemm, a));
+  return hip_ok(hipGetLastError(), &p->err, "PoissonDirect.enc launch");
+}
+
+extern "C" int obl_poisson_solve(obl_poisson* p, const float* rhs_dev, float* phi_dev, int gemm_mode, int thomas_mode,
+                                 double fp64_cond, void* stream) {
+  if (!p) return set_global("obl_poisson_solve: null handle");
+  if (!rhs_dev || !phi_dev) return p->fail("obl_poisson_solve: null buffer");
+  hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+  return poisson_enc(p, rhs_dev, phi_dev, gemm_mode, thomas_mode, fp64_cond, s);
+}
+
+extern "C" int obl_poisson_solve_host(obl_poisson* p, const float* rhs, float* phi, int gemm_mode, int thomas_mode,
+                                      double fp64_cond) {
+  if (!p) return set_global("obl_poisson_solve_host: null handle");
+  size_t bytes = (size_t)p->nx * p->ny * p->nz * 4;
+  if (!p->hrhs && (hip_ok(hipMalloc(&p->hrhs, bytes), &p->err, "hipMalloc") || hip_ok(hipMalloc(&p->hphi, bytes), &p->err, "hipMalloc")))
+    return -1;
+  if (hip_ok(hipMemcpyAsync(p->hrhs, rhs, bytes, hipMemcpyHostToDevice, p->stream), &p->err, "copy rhs")) return -1;
+  if (poisson_enc(p, p->hrhs, p->hphi, gemm_mode, thomas_mode, fp64_cond, p->stream)) return -1;
+  if (hip_ok(hipMemcpyAsync(phi, p->hphi, bytes, hipMemcpyDeviceToHost, p->stream), &p->err, "copy phi")) return -1;
+  return hip_ok(hipStreamSynchronize(p->stream), &p->err, "solve");
+}
+
+extern "C" const char* obl_poisson_last_error(const obl_poisson* p) { return p ? p->err.c_str() : g_err.c_str(); }
+
+extern "C" int obl_poisson_profile(obl_poisson* p, int on) {
+  if (!p) return set_global("null handle");
+  hipStreamSynchronize(p->stream);
+  p->timer.reset();
+  p->timer.on = on != 0;
+  return 0;
+}
+
+extern "C" int obl_poisson_profile_read(obl_poisson* p, double* ms5) {
+  if (!p) return set_global("null handle");
+  p->timer.collect();
+  const char* nm[5] = {"ps.xf", "ps.yf", "ps.tri", "ps.yi", "ps.xi"};
+  for (int i = 0; i < 5; i++) {
+    int id = p->timer.id_of(nm[i]);
+    ms5[i] = p->timer.count[id] ? p->timer.total[id] / p->timer.count[id] : 0.0;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- RegionalModel
+struct obl_model {
+  int nx = 0, ny = 0, nz = 0, pad = 0, nxp = 0, nyp = 0;
+  double dx = 0, dz = 0;
+  hipStream_t stream = nullptr;
+  float4 *V[2] = {nullptr, nullptr}, *Vt = nullptr;
+  float4 *Q1[2] = {nullptr, nullptr}, *Q2[2] = {nullptr, nullptr}, *Q1t = nullptr, *Q2t = nullptr;
+  float4 *lsfA = nullptr, *lsfB = nullptr, *ciMap = nullptr, *BS = nullptr;
+  float2* aux = nullptr;
+  float *phT = nullptr, *phi = nullptr, *rhs = nullptr, *BD = nullptr, *CI = nullptr;
+  obl_poisson* solver = nullptr;
+  int gemm_mode = OBL_GEMM_FP32, thomas_mode = OBL_THOMAS_MIXED;
+  double fp64_cond = 16.0;
+  std::string err;
+  Timer timer;
+  int fail(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return -1;
+  }
+  size_t vcount() const { return (size_t)(nx + 1) * (ny + 1) * (nz + 1); }
+  size_t ccount() const { return (size_t)nx * ny * nz; }
+  size_t pcount() const { return (size_t)nxp * nyp * nz; }
+};
+
+extern "C" int obl_abi_version(void) { return OBL_ABI_VERSION; }
+extern "C" const char* obl_last_error(const obl_model* h) { return h ? h->err.c_str() : g_err.c_str(); }
+extern "C" const char* obl_global_error(void) { return g_err.c_str(); }
+
+extern "C" void obl_destroy(obl_model* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  void* bufs[] = {h->V[0], h->V[1], h->Vt, h->Q1[0], h->Q1[1], h->Q2[0], h->Q2[1], h->Q1t, h->Q2t, h->lsfA,
+                  h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (h->solver) obl_poisson_destroy(h->solver);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+// bytes of all fields (for the out-of-memory message, like gridBytes, model.js:1125-1128)
+static double grid_bytes(const obl_model* h) {
+  double v = (double)h->vcount() * 16 * 3, c = (double)h->ccount();
+  double pc = (double)h->pcount();
+  return v + c * (16 * 6 + 8 + 4) + pc * (4 * 5 + 16) + (double)h->nx * h->ny * 48;
+}
+
+extern "C" int obl_create(const obl_grid* g, obl_model** out) {
+  if (!g || !out) return set_global("obl_create: null argument");
+  *out = nullptr;
+  if (g->nx < 8 || g->ny < 8 || g->nz < 2 || g->nx % 8 || g->ny % 8)
+    return set_global("allocate: nx and ny must be multiples of 8 and nz >= 2 (got %d x %d x %d)", g->nx, g->ny, g->nz);
+  if (g->pad < 0 || g->pad % 4) return set_global("allocate: the solve padding must be a non-negative multiple of 4");
+  obl_model* h = new obl_model();
+  h->nx = g->nx; h->ny = g->ny; h->nz = g->nz; h->dx = g->dx; h->dz = g->dz; h->pad = g->pad;
+  h->nxp = h->nx + 2 * h->pad; h->nyp = h->ny + 2 * h->pad;
+  size_t freeb = 0, totalb = 0;
+  hipMemGetInfo(&freeb, &totalb);
+  double need = grid_bytes(h);
+  if (freeb && need > (double)freeb) {
+    delete h;
+    return set_global("not enough GPU memory for this grid (it needs about %.1f GB, %.1f GB free)", need / 1e9,
+                      freeb / 1e9);
+  }
+  std::string e;
+  size_t vb = h->vcount() * 16, cb = h->ccount() * 16, pb = h->pcount() * 4, mb = (size_t)h->nx * h->ny * 16;
+  int bad = hip_ok(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), &e, "hipStreamCreate") ||
+            hip_ok(hipMalloc(&h->V[0], vb), &e, "hipMalloc(V0)") || hip_ok(hipMalloc(&h->V[1], vb), &e, "hipMalloc(V1)") ||
+            hip_ok(hipMalloc(&h->Vt, vb), &e, "hipMalloc(Vt)") ||
+            hip_ok(hipMalloc(&h->Q1[0], cb), &e, "hipMalloc(Q1a)") || hip_ok(hipMalloc(&h->Q1[1], cb), &e, "hipMalloc(Q1b)") ||
+            hip_ok(hipMalloc(&h->Q2[0], cb), &e, "hipMalloc(Q2a)") || hip_ok(hipMalloc(&h->Q2[1], cb), &e, "hipMalloc(Q2b)") ||
+            hip_ok(hipMalloc(&h->Q1t, cb), &e, "hipMalloc(Q1t)") || hip_ok(hipMalloc(&h->Q2t, cb), &e, "hipMalloc(Q2t)") ||
+            hip_ok(hipMalloc(&h->aux, h->ccount() * 8), &e, "hipMalloc(aux)") ||
+            hip_ok(hipMalloc(&h->phT, h->ccount() * 4), &e, "hipMalloc(phT)") ||
+            hip_ok(hipMalloc(&h->lsfA, mb), &e, "hipMalloc(lsfA)") || hip_ok(hipMalloc(&h->lsfB, mb), &e, "hipMalloc(lsfB)") ||
+            hip_ok(hipMalloc(&h->ciMap, mb), &e, "hipMalloc(ciMap)") ||
+            hip_ok(hipMalloc(&h->BS, (size_t)(h->nz + 1) * 32), &e, "hipMalloc(bs)") ||
+            hip_ok(hipMalloc(&h->BD, OBL_MAX_BOUNDS * OBL_BOUND_FLOATS * 4), &e, "hipMalloc(bounds)") ||
+            hip_ok(hipMalloc(&h->CI, OBL_MAX_CI * OBL_CI_FLOATS * 4), &e, "hipMalloc(ci)") ||
+            hip_ok(hipMalloc(&h->phi, pb), &e, "hipMalloc(phi)") || hip_ok(hipMalloc(&h->rhs, pb), &e, "hipMalloc(rhs)");
+  if (bad) {
+    obl_destroy(h);
+    return set_global("not enough GPU memory for this grid (it needs about %.1f GB): %s", need / 1e9, e.c_str());
+  }
+  if (obl_poisson_create(h->nxp, h->nyp, h->nz, h->dx, h->dz, &h->solver)) {
+    std::string m = g_err;
+    obl_destroy(h);
+    return set_global("%s", m.c_str());
+  }
+  // fields start cleared (WebGPU zero-fills new textures and buffers)
+  void* bufs[] = {h->V[0], h->V[1], h->Vt, h->Q1[0], h->Q1[1], h->Q2[0], h->Q2[1], h->Q1t, h->Q2t,
+                  h->lsfA, h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI};
+  size_t szs[] = {vb, vb, vb, cb, cb, cb, cb, cb, cb, mb, mb, mb, (size_t)(h->nz + 1) * 32, h->ccount() * 8,
+                  h->ccount() * 4, pb, pb, OBL_MAX_BOUNDS * OBL_BOUND_FLOATS * 4, OBL_MAX_CI * OBL_CI_FLOATS * 4};
+  for (size_t i = 0; i < sizeof(szs) / sizeof(szs[0]); i++) hipMemsetAsync(bufs[i], 0, szs[i], h->stream);
+  if (hip_ok(hipStreamSynchronize(h->stream), &e, "clear fields")) {
+    obl_destroy(h);
+    return set_global("%s", e.c_str());
+  }
+  *out = h;
+  return 0;
+}
+
+static int check_u(obl_model* h, const float* u) {
+  if (!u) return h->fail("uniform block is null");
+  if ((int)u[0] != h->nx || (int)u[1] != h->ny || (int)u[2] != h->nz)
+    return h->fail("uniform dims (%g, %g, %g) do not match the grid %d x %d x %d", u[0], u[1], u[2], h->nx, h->ny, h->nz);
+  if ((int)u[36] != h->pad || (int)u[37] != h->nxp || (int)u[38] != h->nyp)
+    return h->fail("uniform pad block (%g, %g, %g) does not match pad %d (%d x %d)", u[36], u[37], u[38], h->pad, h->nxp,
+                   h->nyp);
+  if (u[44] > 0.5f) return h->fail("hurricane mode (hur.x = 1) is not supported by this library");
+  if (u[70] > 0.5f) return h->fail("initial vortex (vtx2.z = 1) is not supported by this library");
+  if (u[75] > 0.5f) return h->fail("midpoint trajectories (cn.w = 1) are not supported by this library");
+  if (!(u[8] > 0.0f)) return h->fail("dt must be positive");
+  return 0;
+}
+
+static MU to_mu(const float* u) {
+  MU m;
+  memcpy(&m, u, sizeof(MU));
+  return m;
+}
+
+extern "C" int obl_set_base_state(obl_model* h, const float* B, int nz) {
+  if (!h) return set_global("null handle");
+  if (!B || nz != h->nz) return h->fail("base state must hold (nz + 1) * 8 floats for nz = %d", h->nz);
+  if (hip_ok(hipMemcpyAsync(h->BS, B, (size_t)(nz + 1) * 32, hipMemcpyHostToDevice, h->stream), &h->err, "setSounding"))
+    return -1;
+  std::vector<double> rc(nz), rw(nz + 1);
+  for (int k = 0; k < nz; k++) rc[k] = B[k * 8 + 2];
+  for (int k = 0; k <= nz; k++) rw[k] = B[k * 8 + 3];
+  if (hip_ok(hipStreamSynchronize(h->stream), &h->err, "setSounding")) return -1;
+  if (obl_poisson_set_density(h->solver, rc.data(), rw.data())) return h->fail("%s", h->solver->err.c_str());
+  return 0;
+}
+
+extern "C" int obl_set_boundaries(obl_model* h, const float* f, int count) {
+  if (!h) return set_global("null handle");
+  if (count < 0 || count > OBL_MAX_BOUNDS || (count && !f)) return h->fail("setBoundaries: 0..%d records", OBL_MAX_BOUNDS);
+  std::vector<float> buf((size_t)OBL_MAX_BOUNDS * OBL_BOUND_FLOATS, 0.0f);
+  if (count) memcpy(buf.data(), f, (size_t)count * OBL_BOUND_FLOATS * 4);
+  if (hip_ok(hipMemcpyAsync(h->BD, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, h->stream), &h->err, "setBoundaries"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setBoundaries");
+}
+
+extern "C" int obl_set_ci(obl_model* h, const float* f, int count) {
+  if (!h) return set_global("null handle");
+  if (count < 0 || count > OBL_MAX_CI || (count && !f)) return h->fail("setCI: 0..%d towers", OBL_MAX_CI);
+  if (count == 0) return 0;
+  if (hip_ok(hipMemcpyAsync(h->CI, f, (size_t)count * OBL_CI_FLOATS * 4, hipMemcpyHostToDevice, h->stream), &h->err, "setCI"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setCI");
+}
+
+extern "C" int obl_set_surface(obl_model* h, const float* surf) {
+  if (!h) return set_global("null handle");
+  if (surf) return h->fail("setSurface: hurricane-mode surfaces are not supported by this library");
+  return 0;
+}
+
+static Dims dims_of(const obl_model* h, const MU& u) {
+  Dims d;
+  d.nx = h->nx; d.ny = h->ny; d.nz = h->nz;
+  d.dx = u.v[U_DIMS].w; d.dz = u.v[U_DZV].x; d.dt = u.v[U_STEP].x;
+  return d;
+}
+
+extern "C" int obl_reset(obl_model* h, const float* uniforms, int seed) {
+  if (!h) return set_global("null handle");
+  if (check_u(h, uniforms)) return -1;
+  MU u = to_mu(uniforms);
+  Dims d = dims_of(h, u);
+  hipStream_t s = h->stream;
+  launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT);
+  launch_init(s, u, d, h->BS, h->lsfA, h->lsfB, h->V[0], h->Q1[0], h->Q2[0], seed);
+  hipMemsetAsync(h->phi, 0, h->pcount() * 4, s);
+  hipMemsetAsync(h->rhs, 0, h->pcount() * 4, s);
+  hipMemcpyAsync(h->V[1], h->V[0], h->vcount() * 16, hipMemcpyDeviceToDevice, s);
+  if (hip_ok(hipGetLastError(), &h->err, "reset")) return -1;
+  return hip_ok(hipStreamSynchronize(s), &h->err, "reset");
+}
+
+extern "C" int obl_add_bubble(obl_model* h, double x, double y, double z, double rh, double rz, double dth) {
+  if (!h) return set_global("null handle");
+  if (!(rh > 0) || !(rz > 0)) return h->fail("bubble radii must be positive");
+  MU u;
+  memset(&u, 0, sizeof u);
+  u.v[U_DIMS] = make_float4((float)h->nx, (float)h->ny, (float)h->nz, (float)h->dx);
+  u.v[U_DZV].x = (float)h->dz;
+  Dims d = dims_of(h, u);
+  launch_bubble(h->stream, u, d, h->V[0], h->V[1], x, y, z, rh, rz, dth);
+  return hip_ok(hipGetLastError(), &h->err, "bubble");
+}
+
+extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags) {
+  if (!h) return set_global("null handle");
+  if (check_u(h, uniforms)) return -1;
+  if (parity != 0 && parity != 1) return h->fail("parity must be 0 or 1");
+  MU u = to_mu(uniforms);
+  Dims d = dims_of(h, u);
+  hipStream_t s = h->stream;
+  float4 *Vc = h->V[parity], *Vn = h->V[1 - parity];
+  Timer& T = h->timer;
+  if (flags & OBL_STEP_LSF) TIMED(T, s, "m.lsf", launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT));
+  TIMED(T, s, "m.ci", launch_ci(s, u, d, h->CI, h->ciMap));
+  TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], h->aux));
+  TIMED(T, s, "m.advV1", launch_advv1(s, u, d, Vc, h->Vt));
+  TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
+  TIMED(T, s, "m.advQ1", launch_advq1(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
+  TIMED(T, s, "m.advQ2", launch_advq2(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t, h->Q1[1], h->Q2[1]));
+  TIMED(T, s, "m.phys", launch_phys(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT, h->ciMap, Vc,
+                                    h->Q1[0], h->Q2[0]));
+  TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
+  h->solver->timer.on = T.on;
+  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s))
+    return h->fail("%s", h->solver->err.c_str());
+  TIMED(T, s, "m.proj", launch_proj(s, u, d, Vc, h->phi, Vn));
+  return hip_ok(hipGetLastError(), &h->err, "encStep");
+}
+
+static int field_info(obl_model* h, int field, int copy, void** ptr, size_t* bytes) {
+  size_t vb = h->vcount() * 16, cb = h->ccount() * 16, pb = h->pcount() * 4, mb = (size_t)h->nx * h->ny * 16;
+  int c2 = copy == 1 ? 1 : 0;
+  if ((field == OBL_FIELD_V || field == OBL_FIELD_Q1 || field == OBL_FIELD_Q2) && copy != 0 && copy != 1)
+    return h->fail("copy must be 0 or 1 for field %d", field);
+  switch (field) {
+    case OBL_FIELD_V: *ptr = h->V[c2]; *bytes = vb; return 0;
+    case OBL_FIELD_Q1: *ptr = h->Q1[c2]; *bytes = cb; return 0;
+    case OBL_FIELD_Q2: *ptr = h->Q2[c2]; *bytes = cb; return 0;
+    case OBL_FIELD_VT: *ptr = h->Vt; *bytes = vb; return 0;
+    case OBL_FIELD_Q1T: *ptr = h->Q1t; *bytes = cb; return 0;
+    case OBL_FIELD_Q2T: *ptr = h->Q2t; *bytes = cb; return 0;
+    case OBL_FIELD_AUX: *ptr = h->aux; *bytes = h->ccount() * 8; return 0;
+    case OBL_FIELD_PHT: *ptr = h->phT; *bytes = h->ccount() * 4; return 0;
+    case OBL_FIELD_LSFA: *ptr = h->lsfA; *bytes = mb; return 0;
+    case OBL_FIELD_LSFB: *ptr = h->lsfB; *bytes = mb; return 0;
+    case OBL_FIELD_CIMAP: *ptr = h->ciMap; *bytes = mb; return 0;
+    case OBL_FIELD_PHI: *ptr = h->phi; *bytes = pb; return 0;
+    case OBL_FIELD_RHS: *ptr = h->rhs; *bytes = pb; return 0;
+    case OBL_FIELD_BS: *ptr = h->BS; *bytes = (size_t)(h->nz + 1) * 32; return 0;
+  }
+  return h->fail("unknown field id %d", field);
+}
+
+extern "C" int obl_field_ptr(obl_model* h, int field, int copy, void** ptr, size_t* bytes) {
+  if (!h) return set_global("null handle");
+  if (!ptr || !bytes) return h->fail("null output");
+  return field_info(h, field, copy, ptr, bytes);
+}
+
+extern "C" int obl_upload(obl_model* h, int field, int copy, const void* src, size_t bytes, size_t offset) {
+  if (!h) return set_global("null handle");
+  void* p;
+  size_t n;
+  if (field_info(h, field, copy, &p, &n)) return -1;
+  if (!src || offset > n || bytes > n - offset) return h->fail("upload of %zu bytes at %zu exceeds the field (%zu)", bytes, offset, n);
+  if (hip_ok(hipMemcpyAsync((char*)p + offset, src, bytes, hipMemcpyHostToDevice, h->stream), &h->err, "upload")) return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "upload");
+}
+
+extern "C" int obl_download(obl_model* h, int field, int copy, void* dst, size_t bytes, size_t offset) {
+  if (!h) return set_global("null handle");
+  void* p;
+  size_t n;
+  if (field_info(h, field, copy, &p, &n)) return -1;
+  if (!dst || offset > n || bytes > n - offset) return h->fail("download of %zu bytes at %zu exceeds the field (%zu)", bytes, offset, n);
+  if (hip_ok(hipMemcpyAsync(dst, (const char*)p + offset, bytes, hipMemcpyDeviceToHost, h->stream), &h->err, "download"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "download");
+}
+
+extern "C" int obl_sync(obl_model* h) {
+  if (!h) return set_global("null handle");
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "sync");
+}
+
+extern "C" int obl_set_solver_mode(obl_model* h, int gemm_mode, int thomas_mode, double fp64_cond) {
+  if (!h) return set_global("null handle");
+  if (gemm_mode != OBL_GEMM_FP32 && gemm_mode != OBL_GEMM_BF16X3) return h->fail("unknown gemm mode %d", gemm_mode);
+  if (thomas_mode < 0 || thomas_mode > 2) return h->fail("unknown Thomas mode %d", thomas_mode);
+  h->gemm_mode = gemm_mode;
+  h->thomas_mode = thomas_mode;
+  h->fp64_cond = fp64_cond;
+  return 0;
+}
+
+extern "C" int obl_profile(obl_model* h, int on) {
+  if (!h) return set_global("null handle");
+  hipStreamSynchronize(h->stream);
+  h->timer.reset();
+  h->solver->timer.reset();
+  h->timer.on = on != 0;
+  h->solver->timer.on = on != 0;
+  return 0;
+}
+
+extern "C" int obl_profile_read(obl_model* h, char* names, size_t names_len, double* ms, int* launches, int max) {
+  if (!h) return set_global("null handle");
+  hipStreamSynchronize(h->stream);
+  h->timer.collect();
+  h->solver->timer.collect();
+  std::string nm;
+  int n = 0;
+  auto add = [&](Timer& T) {
+    for (size_t i = 0; i < T.names.size() && n < max; i++) {
+      if (!T.count[i]) continue;
+      nm += T.names[i];
+      nm += '\n';
+      if (ms) ms[n] = T.total[i] / T.count[i];
+      if (launches) launches[n] = T.count[i];
+      n++;
+    }
+  };
+  add(h->timer);
+  add(h->solver->timer);
+  if (names && names_len) {
+    size_t c = nm.size() < names_len - 1 ? nm.size() : names_len - 1;
+    memcpy(names, nm.data(), c);
+    names[c] = 0;
+  }
+  return n;
+}

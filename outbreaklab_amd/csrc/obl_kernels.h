@@ -1,0 +1,63 @@
+// Kernel launchers shared between the kernel files and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "obl_device.h"
+
+namespace obl {
+
+// ---- model step (k_step.hip) ----
+void launch_lsf(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float* BD, float4* lsfA,
+                float4* lsfB, float* phT);
+void launch_ci(hipStream_t s, const MU& u, const Dims& d, const float* CI, float4* ciMap);
+void launch_init(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* lsfA, const float4* lsfB,
+                 float4* V, float4* Q1, float4* Q2, int seed);
+void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4* V1, double x, double y, double z,
+                   double rh, double rz, double dth);
+void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                float2* aux);
+void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt);
+void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout);
+void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                  const float4* Q2, float4* Q1t, float4* Q2t);
+void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                  const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o);
+void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                 const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                 const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o);
+void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
+void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
+
+// ---- pressure solve (k_solve.hip) ----
+struct DstArgs {
+  const float* in;
+  float* out;
+  const float* W0;  // even-mode matrix for this pass, [Npad][Kp] (zero-padded)
+  const float* W1;  // odd-mode matrix
+  int N;            // transform length (even)
+  int Nh;           // N / 2
+  int rows;         // X passes: rows (nyp * nz); Y passes: planes (nz)
+  int inner;        // Y passes: row length (nxp); X passes: N
+  int Kp;           // row stride of W0/W1
+};
+enum { DST_XF = 0, DST_XI = 1, DST_YF = 2, DST_YI = 3 };
+enum { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };
+void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a);
+
+struct ThomasArgs {
+  float* hat;
+  float* cp32;
+  double* cp64;
+  double* dp64;
+  const float* lam32;   // permuted: [nx] x-modes then [ny] y-modes
+  const double* lam64;
+  const float4* zop32;  // (lo, up, diag, rho_c) per level
+  const double* zop64;  // the same in FP64, 4 per level
+  int nx, ny, nz;
+  int mode;             // 0 FP32 (reference order), 1 FP64 all, 2 FP64 where near-singular
+  double shift_lo;      // mode 2: FP64 when |shift| < shift_lo
+};
+enum { THOMAS_FP32 = 0, THOMAS_FP64 = 1, THOMAS_MIXED = 2 };
+void launch_thomas(hipStream_t s, const ThomasArgs& a);
+
+}  // namespace obl

@@ -1,0 +1,304 @@
+"""Host mirror of the reference's step API: RegionalModel (model.js:1156-1496) and
+PoissonDirect (solver.js:105-165), driving the gfx950 library through its C ABI.
+
+Method names, argument shapes, parity convention and error texts follow the
+reference. ``enc`` arguments are accepted for signature compatibility and
+ignored: work is enqueued on the model's own HIP stream in call order, which
+is what the reference's single command queue guarantees.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .grid import GRID_PRESETS, grid_bytes, grid_pad
+from .sounding import analyze, base_state, parse_sharppy
+from .uniforms import OUTBREAK_OVERRIDES, UBlock, default_params, write_uniforms
+
+MAX_BOUNDS, MAX_BPTS, MAX_CI = 8, 32, 256
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Field:
+    """A device field of the model: download()/upload() copy whole arrays (numpy) in the reference layout."""
+
+    def __init__(self, model: "RegionalModel", name: str, copy: int, shape, dtype=np.float32):
+        self.model, self.name, self.copy, self.shape, self.dtype = model, name, copy, tuple(shape), dtype
+
+    @property
+    def ptr(self) -> int:
+        p, n = C.c_void_p(), C.c_size_t()
+        _lib.check(_lib.lib().obl_field_ptr(self.model.h, _lib.FIELD[self.name], self.copy, C.byref(p), C.byref(n)),
+                   self.model.h)
+        return p.value
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _lib.check(_lib.lib().obl_download(self.model.h, _lib.FIELD[self.name], self.copy, _ptr(out), out.nbytes, 0),
+                   self.model.h)
+        return out
+
+    def upload(self, a) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size * a.itemsize != int(np.prod(self.shape)) * np.dtype(self.dtype).itemsize:
+            raise ValueError(f"{self.name}: expected {self.shape}, got {a.shape}")
+        _lib.check(_lib.lib().obl_upload(self.model.h, _lib.FIELD[self.name], self.copy, _ptr(a), a.nbytes, 0),
+                   self.model.h)
+
+
+class RegionalModel:
+    """RegionalModel (model.js:1156-1496) on MI355X."""
+
+    def __init__(self):
+        self.U = UBlock()
+        self.params = default_params()
+        self.params.update(OUTBREAK_OVERRIDES)
+        self.parity = 0
+        self.time = 0.0
+        self.steps = 0
+        self.bounds = []
+        self.ciEvents = []
+        self.ciActive = 0
+        self.rhoSfc = None
+        self.phiScale = 1
+        self.h = None
+        self.key = None
+        self.base = None
+        self.noise_seed = 0
+        self.lsfT = 0.0
+
+    # ---------------- allocate / destroy (model.js:1186-1247) ----------------
+    def allocate(self, key, spec=None):
+        self.destroy()
+        pr = spec or GRID_PRESETS[key]
+        nx, ny, nz = pr["n"]
+        self.key, self.nx, self.ny, self.nz, self.dx = key, nx, ny, nz, float(pr["dx"])
+        self.dz = 20000.0 / nz
+        self.Lx, self.Ly, self.Lz = nx * self.dx, ny * self.dx, nz * self.dz
+        self.params["spongeW"] = min(max(0.08 * min(self.Lx, self.Ly), 5000), 40000)
+        self.pad = grid_pad(self.dx)
+        self.nxp, self.nyp = nx + 2 * self.pad, ny + 2 * self.pad
+        g = _lib.Grid(nx, ny, nz, self.dx, self.dz, self.pad)
+        h = C.c_void_p()
+        rc = _lib.lib().obl_create(C.byref(g), C.byref(h))
+        if rc != 0:
+            self.key = None
+            msg = _lib.lib().obl_global_error().decode()
+            if "memory" in msg:
+                raise MemoryError(f"not enough GPU memory for this grid (it needs about {grid_bytes(pr) / 1e9:.1f} GB): "
+                                  + msg)
+            raise _lib.OblError(msg)
+        self.h = h
+        self.U.set("pad", self.pad, self.nxp, self.nyp, 0)
+        self.U.set("dims", nx, ny, nz, self.dx)
+        self.U.set("dzv", self.dz, 1 / self.dx, 1 / self.dz, 0)
+        vshape, cshape = (nz + 1, ny + 1, nx + 1, 4), (nz, ny, nx, 4)
+        self.V = [Field(self, "V", 0, vshape), Field(self, "V", 1, vshape)]
+        self.Q1 = [Field(self, "Q1", 0, cshape), Field(self, "Q1", 1, cshape)]
+        self.Q2 = [Field(self, "Q2", 0, cshape), Field(self, "Q2", 1, cshape)]
+        self.Vt = Field(self, "Vt", 0, vshape)
+        self.Q1t = Field(self, "Q1t", 0, cshape)
+        self.Q2t = Field(self, "Q2t", 0, cshape)
+        self.aux = Field(self, "aux", 0, (nz, ny, nx, 2))
+        self.phT = Field(self, "phT", 0, (nz, ny, nx))
+        self.lsfA = Field(self, "lsfA", 0, (ny, nx, 4))
+        self.lsfB = Field(self, "lsfB", 0, (ny, nx, 4))
+        self.ciMap = Field(self, "ciMap", 0, (ny, nx, 4))
+        self.phi = Field(self, "phi", 0, (nz, self.nyp, self.nxp))
+        self.rhs = Field(self, "rhs", 0, (nz, self.nyp, self.nxp))
+        self.bs = Field(self, "bs", 0, ((nz + 1) * 8,))
+        return self
+
+    def destroy(self):
+        if self.h is not None:
+            _lib.lib().obl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # ---------------- inputs ----------------
+    def setSounding(self, text: str):
+        """setSounding (model.js:1276-1288): parse, analyse, base state -> bs + solver density."""
+        snd = parse_sharppy(text)
+        A = analyze(snd)
+        self.analysis, self.snd = A, snd
+        bsx = base_state(snd, self.nz, self.dz, (0.0, 0.0), analysis=A)
+        self.base = bsx
+        self.rhoSfc = float(bsx["B"][2])
+        self.set_base_table(bsx["B"])
+        return A
+
+    def set_base_table(self, B: np.ndarray):
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        _lib.check(_lib.lib().obl_set_base_state(self.h, _ptr(B), self.nz), self.h)
+        self.rhoSfc = float(B[2])
+        self._B = B
+
+    def setBoundaries(self, bnds):
+        """setBoundaries (model.js:1371-1381): 148 floats per boundary."""
+        self.bounds = list(bnds)[:MAX_BOUNDS]
+        f = np.zeros(MAX_BOUNDS * 148, dtype=np.float32)
+        for i, b in enumerate(self.bounds):
+            o = i * 148
+            pts = b["pts"][:MAX_BPTS]
+            f[o:o + 20] = [len(pts), b["motion"][0], b["motion"][1], b["halfW"], b["dT"], b["Hmax"], b["slope"],
+                           b["Hmin"], b["dq"], b["qRatio"], b["du"], b["dv"], b["wRatio"], 1 if b.get("nudge") else 0,
+                           b["dFree"], b["side"], b.get("vary") or 0, b.get("seed") or 0, b.get("varyL") or 60000, 0]
+            for k, p in enumerate(pts):
+                v = (b.get("vel") and b["vel"][k]) or b["motion"]
+                f[o + 20 + 4 * k:o + 24 + 4 * k] = [p[0], p[1], v[0], v[1]]
+        self._bd = f
+        _lib.check(_lib.lib().obl_set_boundaries(self.h, _ptr(f), len(self.bounds)), self.h)
+
+    def setCI(self, events):
+        """setCI (model.js:1383-1389): active towers, 12 floats each, at most 256."""
+        act = [e for e in events if e["t1"] > self.time][-MAX_CI:]
+        self.ciActive = len(act)
+        f = np.zeros(max(1, len(act)) * 12, dtype=np.float32)
+        for i, e in enumerate(act):
+            f[i * 12:(i + 1) * 12] = [e["x"], e["y"], e["zc"], e["rh"], e["rz"], e["wt"], e["t0"], e["t1"],
+                                      e.get("vx") or 0, e.get("vy") or 0, e.get("seed") or i * 1.37,
+                                      e.get("ramp") or 150]
+        self._ci = f
+        _lib.check(_lib.lib().obl_set_ci(self.h, _ptr(f), len(act)), self.h)
+
+    def setSurface(self, surf=None, hur=None):
+        """setSurface (model.js:1298-1303): outbreak mode only (surf None)."""
+        if surf is not None or hur is not None:
+            raise NotImplementedError("hurricane mode is outside this library's scope")
+        _lib.check(_lib.lib().obl_set_surface(self.h, None), self.h)
+        self.hur = None
+
+    def setVortex(self, v=None):
+        if v is not None:
+            raise NotImplementedError("the initial vortex (hurricane mode) is outside this library's scope")
+        self.vortex = None
+        return None
+
+    def baseWind(self, z):
+        k = min(max(int(math.floor(z / self.dz)), 0), self.nz - 1)
+        B = self.base["B"]
+        return [float(B[k * 8 + 6]), float(B[k * 8 + 7])]
+
+    # ---------------- uniforms, reset, step ----------------
+    def writeUniforms(self, dt):
+        return write_uniforms(self.U, self, dt)
+
+    def reset(self):
+        """reset (model.js:1427-1440)."""
+        self.time, self.steps, self.parity, self.phiScale = 0.0, 0, 0, 1
+        u = self.writeUniforms(1)
+        self._u_reset = u
+        _lib.check(_lib.lib().obl_reset(self.h, _ptr(u), int(self.noise_seed)), self.h)
+        self.lsfT = 0.0
+
+    def add_bubble(self, x, y, z, rh, rz, dth):
+        _lib.check(_lib.lib().obl_add_bubble(self.h, x, y, z, rh, rz, dth), self.h)
+
+    def lsf_due(self) -> bool:
+        """The reference's LSF schedule: at t = 0 and every >= 60 s of sim time (model.js:1461)."""
+        return self.time - self.lsfT >= 60 or self.time == 0
+
+    def encStep(self, enc, dt):
+        """encStep (model.js:1442-1476): the state ends in V[parity], Q1[0], Q2[0]."""
+        u = self.writeUniforms(dt)
+        flags = 0
+        if self.lsf_due():
+            flags |= _lib.STEP_LSF
+            self.lsfT = self.time
+        self._u_last = u
+        _lib.check(_lib.lib().obl_step(self.h, _ptr(u), self.parity, flags), self.h)
+        self.parity = 1 - self.parity
+        self.time += dt
+        self.steps += 1
+        self.phiScale = 1
+
+    def step(self, dt):
+        self.encStep(None, dt)
+
+    def sync(self):
+        _lib.check(_lib.lib().obl_sync(self.h), self.h)
+
+    def set_solver_mode(self, gemm="fp32", thomas="mixed", fp64_cond=16.0):
+        _lib.check(_lib.lib().obl_set_solver_mode(self.h, _lib.GEMM[gemm], _lib.THOMAS[thomas], float(fp64_cond)),
+                   self.h)
+
+    def profile(self, on=True):
+        _lib.check(_lib.lib().obl_profile(self.h, 1 if on else 0), self.h)
+
+    def profile_read(self) -> dict:
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(64)
+        cnt = np.zeros(64, dtype=np.int32)
+        n = _lib.lib().obl_profile_read(self.h, names, 4096, _ptr(ms), _ptr(cnt), 64)
+        if n < 0:
+            _lib.check(n, self.h)
+        nm = names.value.decode().split("\n")[:n]
+        return {nm[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    @property
+    def Vcur(self):
+        return self.V[self.parity]
+
+    def state(self):
+        """Current prognostic state as numpy arrays: V[parity], Q1[0], Q2[0]."""
+        return self.V[self.parity].download(), self.Q1[0].download(), self.Q2[0].download()
+
+
+class PoissonDirect:
+    """PoissonDirect (solver.js:105-165): exact separable solve on MI355X."""
+
+    def __init__(self, nx, ny, nz, dx, dz):
+        self.nx, self.ny, self.nz, self.dx, self.dz = nx, ny, nz, dx, dz
+        h = C.c_void_p()
+        _lib.check(_lib.lib().obl_poisson_create(nx, ny, nz, float(dx), float(dz), C.byref(h)))
+        self.h = h
+        self.setDensity(np.ones(nz), np.ones(nz + 1))
+
+    def setDensity(self, rhoC, rhoW):
+        rc = np.ascontiguousarray(rhoC, dtype=np.float64)
+        rw = np.ascontiguousarray(rhoW, dtype=np.float64)
+        if rc.size != self.nz or rw.size != self.nz + 1:
+            raise ValueError("setDensity: rhoC needs nz values and rhoW nz + 1")
+        _lib.check(_lib.lib().obl_poisson_set_density(self.h, _ptr(rc), _ptr(rw)), self.h, poisson=True)
+
+    def solve(self, rhs, gemm="fp32", thomas="mixed", fp64_cond=16.0) -> np.ndarray:
+        """Host arrays in/out (x fastest, shape (nz, ny, nx))."""
+        r = np.ascontiguousarray(rhs, dtype=np.float32).reshape(self.nz, self.ny, self.nx)
+        phi = np.empty_like(r)
+        _lib.check(_lib.lib().obl_poisson_solve_host(self.h, _ptr(r), _ptr(phi), _lib.GEMM[gemm], _lib.THOMAS[thomas],
+                                                     float(fp64_cond)), self.h, poisson=True)
+        return phi
+
+    def enc_device(self, rhs_ptr: int, phi_ptr: int, gemm="fp32", thomas="mixed", fp64_cond=16.0, stream=None):
+        _lib.check(_lib.lib().obl_poisson_solve(self.h, C.c_void_p(rhs_ptr), C.c_void_p(phi_ptr), _lib.GEMM[gemm],
+                                                _lib.THOMAS[thomas], float(fp64_cond), stream), self.h, poisson=True)
+
+    def profile(self, on=True):
+        _lib.check(_lib.lib().obl_poisson_profile(self.h, 1 if on else 0), self.h, poisson=True)
+
+    def profile_read(self):
+        ms = np.zeros(5)
+        _lib.check(_lib.lib().obl_poisson_profile_read(self.h, _ptr(ms)), self.h, poisson=True)
+        return dict(zip(["ps.xf", "ps.yf", "ps.tri", "ps.yi", "ps.xi"], ms.tolist()))
+
+    def destroy(self):
+        if getattr(self, "h", None) is not None:
+            _lib.lib().obl_poisson_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
