@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X model step on the 1024 x 1024 x 80, 500 m workload.
+
+One "step" is RegionalModel.encStep (model.js:1442-1476): CI map, eddy viscosity,
+velocity and moisture advection, forces + Thompson microphysics, divergence, the padded
+1072 x 1072 x 80 pressure solve and the projection. Inputs are resident in HBM.
+
+Prints one JSON line: sim-s per wall-s (``value``), cell-updates/s, the per-kernel HIP-event
+times, the roofline of the dominant kernel and a CPU baseline (the oracle's FP32 step on a
+bounded sample, rank 0 only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFS = 157.3
+BF16_MFMA_PEAK_TFS = 2500.0
+
+# compulsory bytes per model cell of each stencil kernel: each field read once + written once
+# (float4 texel 16 B, aux 8 B, rhs 4 B); halo re-reads excluded
+CELL_BYTES = {"m.nut": 40, "m.advV1": 32, "m.advV2": 48, "m.advQ1": 80, "m.advQ2": 112, "m.phys": 112,
+              "m.div": 20, "m.proj": 36}
+
+
+def kernel_roofline(name, ms, nx, ny, nz, nxp, nyp, gemm):
+    cells = nx * ny * nz
+    cells_p = nxp * nyp * nz
+    if name in CELL_BYTES:
+        b = CELL_BYTES[name] * cells
+        return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "algorithmic_bytes": b}
+    if name in ("ps.xf", "ps.xi", "ps.yf", "ps.yi"):
+        n = nxp if name in ("ps.xf", "ps.xi") else nyp
+        fl = cells_p * n  # folded DST: two (M x n/2 x n/2) GEMMs per pass = cells_p * n flops
+        peak = FP32_MFMA_PEAK_TFS if gemm == "fp32" else BF16_MFMA_PEAK_TFS / 3.0
+        return {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
+                "algorithmic_flops": fl}
+    if name == "ps.tri":
+        b = 8 * cells_p
+        return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "algorithmic_bytes": b}
+    return None
+
+
+def cpu_baseline(threads: int, dt: float, full_cells: int):
+    """The oracle's FP32 step (C, OpenMP) on a 256 x 256 x 80 slice of the workload at 500 m."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _cases import host_model, uniforms  # noqa: E402
+    from _oracle import Oracle, step_f32  # noqa: E402
+    from outbreaklab_amd.configs import wk_text_for_cape  # noqa: E402
+
+    nx = ny = 256
+    nz = 80
+    m = host_model(nx, ny, nz, 500.0, scheme="thompson", text=wk_text_for_cape(3000.0, 25.0))
+    o = Oracle(uniforms(m, 1.0), m.B, threads=threads)
+    A, B, ph = o.lsf(np.zeros(8 * 148, np.float32))
+    V, Q1, Q2 = o.init(A, B, 1)
+    V = o.bubble(V, 0.5 * m.Lx, 0.5 * m.Ly, 1500.0, 10000.0, 1500.0, 2.0)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    V2, Q12, Q22 = f32(np.stack([V, V])), f32(np.stack([Q1, Q1])), f32(np.stack([Q2, Q2]))
+    Vt, Q1t, Q2t = np.zeros_like(V2[0]), np.zeros_like(Q12[0]), np.zeros_like(Q12[0])
+    aux = np.zeros((nz, ny, nx, 2), np.float32)
+    ciMap = np.zeros((ny, nx, 4), np.float32)
+    rhs = np.zeros((nz, m.nyp, m.nxp), np.float32)
+    phi = np.zeros_like(rhs)
+    A, B, ph = f32(A), f32(B), f32(ph)
+    bd = np.zeros(8 * 148, np.float32)
+    par = 0
+    times = []
+    for s in range(3):
+        m.time = s * dt
+        U = uniforms(m, dt)
+        t0 = time.perf_counter()
+        step_f32(U, m.B, bd, None, s == 0, par, V2, Q12, Q22, Vt, Q1t, Q2t, aux, ph, A, B, ciMap, rhs, phi,
+                 threads=threads)
+        times.append(time.perf_counter() - t0)
+        par = 1 - par
+    t = sorted(times[1:])[len(times[1:]) // 2]
+    rate = nx * ny * nz / t
+    return {"value": rate * dt / full_cells, "unit": "sim-s/wall-s", "cores": threads, "kind": "port",
+            "cell_updates_per_s": rate,
+            "sample": f"oracle FP32 step (C/OpenMP, dense-GEMM solve as the reference) on 256x256x80 at 500 m "
+                      f"(pad 24), median of 2 steps after 1 warm-up ({t:.2f} s/step), scaled by cells to "
+                      f"1024x1024x80; the reference's own WGSL on a CPU WebGPU adapter is unavailable here"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--gemm", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--thomas", default="mixed", choices=["fp32", "fp64", "mixed"])
+    ap.add_argument("--size", default="1024,1024,80")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
+              "decomposition is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_plain_1024
+
+    nx, ny, nz = (int(v) for v in args.size.split(","))
+    m = RegionalModel()
+    dt = setup_plain_1024(m, seed=1, scheme="thompson", n=(nx, ny, nz))
+    m.set_solver_mode(args.gemm, args.thomas, 16.0)
+    print(f"# workload {nx}x{ny}x{nz} dx 500 m, pad {m.pad}, Thompson, dt {dt} s; warmup {args.warmup}", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        m.step(dt)
+    m.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.step(dt)
+    m.sync()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    cells = nx * ny * nz
+    print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event times on the model's stream (separate, shorter run)
+    m.profile(True)
+    for _ in range(5):
+        m.step(dt)
+    prof = m.profile_read()
+    m.profile(False)
+    kern = {}
+    for name, (ms, cnt) in prof.items():
+        r = kernel_roofline(name, ms, nx, ny, nz, m.nxp, m.nyp, args.gemm)
+        kern[name] = {"ms": round(ms, 4), "launches": cnt}
+        if r:
+            r["frac"] = r["achieved"] / r["peak"]
+            kern[name].update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()})
+    dom = max((k for k in kern if "bound" in kern[k]), key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    roof = {"kernel": dom, "bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+            "frac": d["frac"], "traffic": None}
+    V, Q1, _ = m.state()
+    import numpy as np
+
+    finite = bool(np.isfinite(V).all() and np.isfinite(Q1).all())
+    cpu = None
+    if not args.no_cpu:
+        print("# CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_threads, dt, cells)
+    out = {
+        "metric": "sim-s per wall-s at 1024x1024x80 (plain FP32 step)",
+        "value": dt / (ms_step * 1e-3),
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "cell_updates_per_s": cells / (ms_step * 1e-3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (WK sounding tuned to 3000 J/kg SBCAPE, 25 m/s shear, 8 seeded 2 K bubbles)",
+        "config": {"workload": f"plain step {nx}x{ny}x{nz} at 500 m, Thompson, dt {dt} s, solve "
+                               f"{m.nxp}x{m.nyp}x{nz}", "gemm": args.gemm, "thomas": args.thomas},
+        "state_finite": finite,
+        "roofline": roof,
+        "kernels": kern,
+        "cpu_baseline": cpu,
+        "reference_rtx5090_sim_s_per_wall_s_at_dt2": 21.3,
+    }
+    print(json.dumps(out))
+    m.destroy()
+
+
+if __name__ == "__main__":
+    main()

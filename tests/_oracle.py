@@ -1,0 +1,207 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) -- test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-j4"], cwd=ORACLE_DIR, check=True)
+        _L = C.CDLL(ORACLE_SO)
+        _L.orc_pcg.restype = C.c_uint32
+        _L.orc_pcg.argtypes = [C.c_uint32]
+        _L.orc_hash3i.restype = C.c_uint32
+        _L.orc_hash3i.argtypes = [C.c_int32] * 3
+        _L.orc_set_threads.argtypes = [C.c_int]
+    return _L
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _dt(sfx):
+    return np.float32 if sfx == "f32" else np.float64
+
+
+class Oracle:
+    """One precision (``f32`` or ``f64``) of the kernel restatement, on numpy arrays."""
+
+    def __init__(self, U, BS, sfx="f32", threads=None):
+        self.U = np.ascontiguousarray(U, dtype=np.float32)
+        self.BS = np.ascontiguousarray(BS, dtype=np.float32)
+        self.sfx = sfx
+        self.dt = _dt(sfx)
+        self.nx, self.ny, self.nz = int(self.U[0]), int(self.U[1]), int(self.U[2])
+        self.pad, self.nxp, self.nyp = int(self.U[36]), int(self.U[37]), int(self.U[38])
+        lib().orc_set_threads(threads or min(8, os.cpu_count() or 1))
+
+    def _f(self, name):
+        f = getattr(lib(), f"orc_{name}_{self.sfx}")
+        f.restype = None
+        return f
+
+    def a(self, x):
+        return np.ascontiguousarray(x, dtype=self.dt)
+
+    def vshape(self):
+        return (self.nz + 1, self.ny + 1, self.nx + 1, 4)
+
+    def cshape(self, c=4):
+        return (self.nz, self.ny, self.nx, c) if c > 1 else (self.nz, self.ny, self.nx)
+
+    def lsf(self, BD):
+        A = np.zeros((self.ny, self.nx, 4), self.dt)
+        B = np.zeros_like(A)
+        ph = np.zeros(self.cshape(1), self.dt)
+        self._f("lsf")(P(self.U), P(self.BS), P(np.ascontiguousarray(BD, np.float32)), P(A), P(B), P(ph))
+        return A, B, ph
+
+    def ci(self, CI):
+        m = np.zeros((self.ny, self.nx, 4), self.dt)
+        ci = np.ascontiguousarray(CI if CI is not None and len(CI) else np.zeros(12), np.float32)
+        self._f("ci")(P(self.U), P(self.BS), P(ci), P(m))
+        return m
+
+    def init(self, lsfA, lsfB, seed=0):
+        V = np.zeros(self.vshape(), self.dt)
+        Q1 = np.zeros(self.cshape(), self.dt)
+        Q2 = np.zeros(self.cshape(), self.dt)
+        f = self._f("init")
+        f.argtypes = [C.c_void_p] * 7 + [C.c_int]
+        f(P(self.U), P(self.BS), P(self.a(lsfA)), P(self.a(lsfB)), P(V), P(Q1), P(Q2), seed)
+        return V, Q1, Q2
+
+    def bubble(self, V, x, y, z, rh, rz, dth):
+        V = self.a(V).copy()
+        f = self._f("bubble")
+        f.argtypes = [C.c_void_p, C.c_void_p] + [C.c_double] * 6
+        f(P(self.U), P(V), x, y, z, rh, rz, dth)
+        return V
+
+    def nut(self, V, Q1):
+        aux = np.zeros(self.cshape(2), self.dt)
+        self._f("nut")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(aux))
+        return aux
+
+    def advv1(self, V):
+        Vt = np.zeros(self.vshape(), self.dt)
+        self._f("advv1")(P(self.U), P(self.BS), P(self.a(V)), P(Vt))
+        return Vt
+
+    def advv2(self, V, Vt):
+        Vo = np.zeros(self.vshape(), self.dt)
+        self._f("advv2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Vt)), P(Vo))
+        return Vo
+
+    def advq1(self, V, Q1, Q2):
+        a = np.zeros(self.cshape(), self.dt)
+        b = np.zeros(self.cshape(), self.dt)
+        self._f("advq1")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.a(Q2)), P(a), P(b))
+        return a, b
+
+    def advq2(self, V, Q1, Q2, Q1t, Q2t):
+        a = np.zeros(self.cshape(), self.dt)
+        b = np.zeros(self.cshape(), self.dt)
+        self._f("advq2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.a(Q2)), P(self.a(Q1t)),
+                         P(self.a(Q2t)), P(a), P(b))
+        return a, b
+
+    def phys(self, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT, ciMap):
+        Vo = np.zeros(self.vshape(), self.dt)
+        Q1 = np.zeros(self.cshape(), self.dt)
+        Q2 = np.zeros(self.cshape(), self.dt)
+        self._f("phys")(P(self.U), P(self.BS), P(self.a(Vin)), P(self.a(Q1in)), P(self.a(Q2in)), P(self.a(aux)),
+                        P(self.a(lsfA)), P(self.a(lsfB)), P(self.a(phT)), P(self.a(ciMap)), P(Vo), P(Q1), P(Q2))
+        return Vo, Q1, Q2
+
+    def micro_point(self, k, st):
+        st = self.a(st).copy()
+        f = self._f("micro_point")
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        f(P(self.U), P(self.BS), k, P(st))
+        return st
+
+    def div(self, V):
+        rhs = np.zeros((self.nz, self.nyp, self.nxp), self.dt)
+        self._f("div")(P(self.U), P(self.BS), P(self.a(V)), P(rhs))
+        return rhs
+
+    def proj(self, V, phi):
+        Vo = np.zeros(self.vshape(), self.dt)
+        self._f("proj")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(phi)), P(Vo))
+        return Vo
+
+    def densities(self):
+        rc = self.BS[2:8 * self.nz:8].astype(np.float64)[: self.nz]
+        rw = self.BS[3::8].astype(np.float64)[: self.nz + 1]
+        return rc, rw
+
+    def solve(self, rhs, mode="f64"):
+        return solve(self.nxp, self.nyp, self.nz, float(self.U[3]), float(self.U[4]), *self.densities(), rhs, mode)
+
+
+def solve(nx, ny, nz, dx, dz, rhoC, rhoW, rhs, mode="f64", threads=None):
+    """FP64 oracle solve (mode f64) or the reference's FP32 arithmetic (mode f32ref)."""
+    rc = np.ascontiguousarray(rhoC, np.float64)
+    rw = np.ascontiguousarray(rhoW, np.float64)
+    nth = threads or min(8, os.cpu_count() or 1)
+    if mode == "f64":
+        r = np.ascontiguousarray(rhs, np.float64)
+        out = np.zeros_like(r)
+        f = lib().orc_solve_f64
+    else:
+        r = np.ascontiguousarray(rhs, np.float32)
+        out = np.zeros_like(r)
+        f = lib().orc_solve_f32ref
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_int]
+    f(nx, ny, nz, dx, dz, P(rc), P(rw), P(r), P(out), nth)
+    return out
+
+
+def dst_matrix(N):
+    q = np.zeros((N, N))
+    qi = np.zeros((N, N))
+    f = lib().orc_dst_matrix_f64
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+    f(N, P(q), P(qi))
+    return q, qi
+
+
+def dst_eig(N, dx):
+    lam = np.zeros(N)
+    f = lib().orc_dst_eig_f64
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_double, C.c_void_p]
+    f(N, dx, P(lam))
+    return lam
+
+
+def step_f32(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi,
+             threads=None):
+    """The whole FP32 step; every array is modified in place (float32, contiguous)."""
+    f = lib().orc_step_f32
+    f.restype = None
+    f.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int] + [C.c_void_p] * 13 + [C.c_int]
+    arrs = [V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi]
+    for a in arrs:
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    ci = np.ascontiguousarray(CI if CI is not None and len(CI) else np.zeros(12), np.float32)
+    f(P(np.ascontiguousarray(U, np.float32)), P(np.ascontiguousarray(BS, np.float32)),
+      P(np.ascontiguousarray(BD, np.float32)), P(ci), int(run_lsf), int(parity), *[P(a) for a in arrs],
+      threads or min(8, os.cpu_count() or 1))

@@ -1,0 +1,104 @@
+"""GPU parity of the pressure solve (PoissonDirect, solver.js:105-165) against the FP64 oracle.
+
+Tolerances:
+  FP32-MFMA transforms + FP64 Thomas vs FP64: relative L2 <= 1e-5 (the north-star bound on
+  pressure), operator residual <= 1e-5 relative to the RHS.
+  FP32 Thomas (the reference's arithmetic) vs FP64: bounded by cond * eps32 of the
+  near-singular low modes, documented below; vs the FP32 restatement it is checked with
+  the same bound because both carry that error independently.
+"""
+import numpy as np
+import pytest
+
+from _oracle import solve as oracle_solve
+from _ref_solve import apply_operator, solve_scipy
+from _util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def wk_density(nz, dz):
+    from outbreaklab_amd.sounding import base_state, parse_sharppy, wk_sounding_text
+
+    B = base_state(parse_sharppy(wk_sounding_text()), nz, dz)["B"]
+    return B[2:8 * nz:8].astype(np.float64)[:nz], B[3::8].astype(np.float64)[: nz + 1]
+
+
+def rough_rhs(nz, ny, nx, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((nz, ny, nx)) * 1e-5).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", [(40, 144, 144), (16, 64, 96), (24, 40, 56)])
+def test_solve_fp64_thomas_matches_fp64_oracle(shape):
+    from outbreaklab_amd import PoissonDirect
+
+    nz, ny, nx = shape
+    dx, dz = 2000.0, 20000.0 / nz
+    rc, rw = wk_density(nz, dz)
+    rhs = rough_rhs(nz, ny, nx, 7)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    phi = ps.solve(rhs, gemm="fp32", thomas="fp64")
+    ref = oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs.astype(np.float64), "f64")
+    e = rel_l2(phi, ref)
+    res = rel_l2(apply_operator(phi, dx, dz, rc, rw), rhs)
+    print(f"shape {shape}: rel err {e:.3e}  residual {res:.3e}")
+    assert e <= 1e-5
+    assert res <= 1e-5
+    ps.destroy()
+
+
+def test_solve_mixed_thomas_matches_fp64():
+    from outbreaklab_amd import PoissonDirect
+
+    nz, ny, nx = 40, 144, 144
+    dx, dz = 2000.0, 500.0
+    rc, rw = wk_density(nz, dz)
+    rhs = rough_rhs(nz, ny, nx, 3)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    ref = solve_scipy(rhs, dx, dz, rc, rw)
+    e_mix = rel_l2(ps.solve(rhs, gemm="fp32", thomas="mixed", fp64_cond=16.0), ref)
+    e_32 = rel_l2(ps.solve(rhs, gemm="fp32", thomas="fp32"), ref)
+    print(f"mixed {e_mix:.3e}  fp32 Thomas {e_32:.3e}")
+    assert e_mix <= 1e-5
+    # FP32 Thomas: the (0,0) vertical system has cond ~ 4/(|lambda| dz^2) = 6.7e4 at this grid,
+    # so its error is ~cond * 6e-8 on the low modes
+    assert e_32 <= 5e-3
+
+
+def test_solve_fp32_thomas_vs_reference_arithmetic():
+    from outbreaklab_amd import PoissonDirect
+
+    nz, ny, nx = 40, 144, 144
+    dx, dz = 2000.0, 500.0
+    rc, rw = wk_density(nz, dz)
+    rhs = rough_rhs(nz, ny, nx, 5)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    g = ps.solve(rhs, gemm="fp32", thomas="fp32")
+    r32 = oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs, "f32ref")
+    r64 = oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs.astype(np.float64), "f64")
+    print(f"gpu-fp32 vs f32ref {rel_l2(g, r32):.3e}; f32ref vs f64 {rel_l2(r32, r64):.3e}; gpu vs f64 {rel_l2(g, r64):.3e}")
+    assert rel_l2(g, r32) <= 2 * rel_l2(r32, r64) + 1e-5
+
+
+def test_solve_1024_config():
+    """The 1024 x 1024 x 80 solve (dx 500 m, dz 250 m), k^-5/3 RHS, against scipy FP64."""
+    from outbreaklab_amd import PoissonDirect
+    from outbreaklab_amd.configs import solve_rhs_1024
+
+    nz, ny, nx = 80, 1024, 1024
+    dx, dz = 500.0, 250.0
+    rc, rw = wk_density(nz, dz)
+    rhs = solve_rhs_1024(nx, ny, nz, seed=1)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    ref = solve_scipy(rhs, dx, dz, rc, rw)
+    for thomas in ("fp64", "mixed"):
+        phi = ps.solve(rhs.astype(np.float32), gemm="fp32", thomas=thomas)
+        e = rel_l2(phi, ref)
+        print(f"1024 {thomas}: rel err {e:.3e}")
+        assert e <= 1e-5
+    ps.destroy()

@@ -1,0 +1,130 @@
+"""GPU parity of every kernel of the model step against the FP32 oracle, on the 128 x 128 x 40
+workload (2 km, WK sounding, one 2 K bubble) after it has developed cloud and rain.
+
+Each kernel is checked in isolation: the oracle kernel is fed the GPU's own inputs, and its
+output is compared with the GPU's (relative L2 per channel; theta as the perturbation from
+the base state). The bound is the north-star FP32 tolerance, 1e-5 per field after one step.
+"""
+import numpy as np
+import pytest
+
+from _oracle import Oracle, solve as oracle_solve, step_f32
+from _util import rel_l2, theta_pert
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+V_CH = ["u", "v", "w", "theta"]
+Q1_CH = ["qv", "qc", "qi", "qr"]
+Q2_CH = ["qs", "qg", "Nr", "spare"]
+
+
+def _cmp(name, gpu, ref, chans, errs, BS=None, nz=None):
+    for i, c in enumerate(chans):
+        if c == "theta":
+            e = rel_l2(theta_pert(gpu, BS, nz), theta_pert(ref, BS, nz))
+        else:
+            e = rel_l2(gpu[..., i], ref[..., i])
+        errs[f"{name}.{c}"] = e
+
+
+@pytest.fixture(scope="module", params=["lin", "thompson"])
+def developed(request):
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_plain_128
+
+    m = RegionalModel()
+    dt = setup_plain_128(m, scheme=request.param)
+    for _ in range(150):
+        m.step(dt)
+    m.sync()
+    yield m, dt, request.param
+    m.destroy()
+
+
+def test_init_matches_oracle():
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_plain_128
+
+    m = RegionalModel()
+    setup_plain_128(m)
+    o = Oracle(m._u_reset, m._B)
+    A, B, ph = o.lsf(np.zeros(8 * 148, np.float32))
+    V, Q1, Q2 = o.init(A, B, 0)
+    V = o.bubble(V, 0.5 * m.Lx, 0.5 * m.Ly, 1500.0, 10000.0, 1500.0, 2.0)
+    assert np.array_equal(m.lsfA.download(), A) and np.array_equal(m.lsfB.download(), B)
+    assert np.array_equal(m.phT.download(), ph)
+    Vg = m.V[0].download()
+    assert np.array_equal(Vg, m.V[1].download())
+    assert np.max(np.abs(Vg - V)) <= 1e-4  # theta ~300 K: a few ULP
+    q = m.Q1[0].download()
+    assert rel_l2(q, Q1) <= 1e-6
+    assert np.array_equal(m.Q2[0].download(), Q2)
+    m.destroy()
+
+
+def test_step_kernels_match_oracle(developed):
+    m, dt, scheme = developed
+    p = m.parity
+    V0, Q10, Q20 = m.V[p].download(), m.Q1[0].download(), m.Q2[0].download()
+    lsfA, lsfB, phT = m.lsfA.download(), m.lsfB.download(), m.phT.download()
+    assert Q10[..., 1].max() > 1e-4, "the workload should have cloud water by now"
+    m.step(dt)
+    m.sync()
+    U = m._u_last
+    o = Oracle(U, m._B)
+    ciMap = o.ci(None)
+    errs = {}
+    aux = m.aux.download()
+    _cmp("nut", aux, o.nut(V0, Q10), ["nuh", "nuv"], errs)
+    Vt = m.Vt.download()
+    _cmp("advV1", Vt, o.advv1(V0), V_CH, errs, m._B, m.nz)
+    Q1t, Q2t = m.Q1t.download(), m.Q2t.download()
+    r1, r2 = o.advq1(V0, Q10, Q20)
+    _cmp("advQ1.q1", Q1t, r1, Q1_CH, errs)
+    _cmp("advQ1.q2", Q2t, r2, Q2_CH[:3], errs)
+    Q1a, Q2a = m.Q1[1].download(), m.Q2[1].download()
+    r1, r2 = o.advq2(V0, Q10, Q20, Q1t, Q2t)
+    _cmp("advQ2.q1", Q1a, r1, Q1_CH, errs)
+    _cmp("advQ2.q2", Q2a, r2, Q2_CH[:3], errs)
+    Vadv = o.advv2(V0, Vt)
+    Vphys_g = m.V[p].download()
+    Vp, Q1p, Q2p = o.phys(Vadv, Q1a, Q2a, aux, lsfA, lsfB, phT, ciMap)
+    _cmp("advV2+phys", Vphys_g, Vp, V_CH, errs, m._B, m.nz)
+    _cmp("phys.q1", m.Q1[0].download(), Q1p, Q1_CH, errs)
+    _cmp("phys.q2", m.Q2[0].download(), Q2p, Q2_CH[:3], errs)
+    rhs_g = m.rhs.download()
+    errs["div"] = rel_l2(rhs_g, o.div(Vphys_g))
+    phi_g = m.phi.download()
+    errs["solve"] = rel_l2(phi_g, oracle_solve(m.nxp, m.nyp, m.nz, m.dx, m.dz, *o.densities(),
+                                                rhs_g.astype(np.float64), "f64"))
+    _cmp("proj", m.V[1 - p].download(), o.proj(Vphys_g, phi_g), V_CH, errs, m._B, m.nz)
+    for k, v in errs.items():
+        print(f"{scheme:9s} {k:14s} {v:.3e}")
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, bad
+
+
+def test_whole_step_matches_oracle(developed):
+    """The whole step chained on the oracle (reference FP32 solve) against the GPU (mixed Thomas)."""
+    m, dt, scheme = developed
+    p = m.parity
+    V2 = np.stack([m.V[0].download(), m.V[1].download()])
+    Q12 = np.stack([m.Q1[0].download(), m.Q1[1].download()])
+    Q22 = np.stack([m.Q2[0].download(), m.Q2[1].download()])
+    lsfA, lsfB, phT, ciMap = m.lsfA.download(), m.lsfB.download(), m.phT.download(), m.ciMap.download()
+    aux, Vt, Q1t, Q2t = m.aux.download(), m.Vt.download(), m.Q1t.download(), m.Q2t.download()
+    rhs, phi = m.rhs.download(), m.phi.download()
+    run_lsf = m.lsf_due()
+    m.step(dt)
+    m.sync()
+    step_f32(m._u_last, m._B, m._bd, m._ci if m.ciActive else None, run_lsf, p, V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT,
+             lsfA, lsfB, ciMap, rhs, phi)
+    errs = {}
+    _cmp("V", m.V[1 - p].download(), V2[1 - p], V_CH, errs, m._B, m.nz)
+    _cmp("Q1", m.Q1[0].download(), Q12[0], Q1_CH, errs)
+    _cmp("Q2", m.Q2[0].download(), Q22[0], Q2_CH[:3], errs)
+    for k, v in errs.items():
+        print(f"{scheme:9s} whole {k:10s} {v:.3e}")
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, bad
