@@ -225,6 +225,8 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
   if (nx < 8 || ny < 8 || nz < 2 || nx % 8 || ny % 8)
     return set_global("PoissonDirect: nx and ny must be multiples of 8 (>= 8) and nz >= 2, got %d x %d x %d", nx, ny, nz);
   if (!(dx > 0) || !(dz > 0)) return set_global("PoissonDirect: dx and dz must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_global("PoissonDirect: no ROCm GPU is available");
   obl_poisson* p = new obl_poisson();
   p->nx = nx; p->ny = ny; p->nz = nz; p->dx = dx; p->dz = dz;
   std::string e;
@@ -387,6 +389,8 @@ extern "C" int obl_create(const obl_grid* g, obl_model** out) {
   if (g->nx < 8 || g->ny < 8 || g->nz < 2 || g->nx % 8 || g->ny % 8)
     return set_global("allocate: nx and ny must be multiples of 8 and nz >= 2 (got %d x %d x %d)", g->nx, g->ny, g->nz);
   if (g->pad < 0 || g->pad % 4) return set_global("allocate: the solve padding must be a non-negative multiple of 4");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_global("allocate: no ROCm GPU is available");
   obl_model* h = new obl_model();
   h->nx = g->nx; h->ny = g->ny; h->nz = g->nz; h->dx = g->dx; h->dz = g->dz; h->pad = g->pad;
   h->nxp = h->nx + 2 * h->pad; h->nyp = h->ny + 2 * h->pad;

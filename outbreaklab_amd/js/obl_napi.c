@@ -1,0 +1,443 @@
+/*
+ * Node N-API addon over liboutbreak_hip.so (include/outbreak_hip.h).
+ *
+ * A thin, 1:1 layer: each export calls one C-ABI entry point with the data of a
+ * TypedArray; errors become JS exceptions carrying the library's message.
+ * obl_sync runs as napi_async_work and resolves a Promise, the counterpart of
+ * the reference's queue.onSubmittedWorkDone() (app.js:1704). The JavaScript
+ * classes RegionalModel / PoissonDirect (outbreak.js) are built on these.
+ *
+ * Plain C against node_api.h (NAPI_VERSION 8); the library is loaded with
+ * dlopen from the package directory, so the addon itself links no HIP code.
+ */
+#define NAPI_VERSION 8
+#include <dlfcn.h>
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "outbreak_hip.h"
+
+typedef struct {
+  void* so;
+  int (*create)(const obl_grid*, obl_model**);
+  void (*destroy)(obl_model*);
+  const char* (*last_error)(const obl_model*);
+  const char* (*global_error)(void);
+  int (*set_base_state)(obl_model*, const float*, int);
+  int (*set_boundaries)(obl_model*, const float*, int);
+  int (*set_ci)(obl_model*, const float*, int);
+  int (*set_surface)(obl_model*, const float*);
+  int (*reset)(obl_model*, const float*, int);
+  int (*step)(obl_model*, const float*, int, unsigned);
+  int (*add_bubble)(obl_model*, double, double, double, double, double, double);
+  int (*upload)(obl_model*, int, int, const void*, size_t, size_t);
+  int (*download)(obl_model*, int, int, void*, size_t, size_t);
+  int (*field_ptr)(obl_model*, int, int, void**, size_t*);
+  int (*sync)(obl_model*);
+  int (*set_solver_mode)(obl_model*, int, int, double);
+  int (*profile)(obl_model*, int);
+  int (*profile_read)(obl_model*, char*, size_t, double*, int*, int);
+  int (*p_create)(int, int, int, double, double, obl_poisson**);
+  void (*p_destroy)(obl_poisson*);
+  int (*p_set_density)(obl_poisson*, const double*, const double*);
+  int (*p_solve_host)(obl_poisson*, const float*, float*, int, int, double);
+  const char* (*p_last_error)(const obl_poisson*);
+} Api;
+
+static Api A;
+
+#define CHECK_NAPI(call)                                              \
+  do {                                                                \
+    if ((call) != napi_ok) {                                          \
+      napi_throw_error(env, NULL, "N-API call failed: " #call);       \
+      return NULL;                                                    \
+    }                                                                 \
+  } while (0)
+
+static napi_value throw_msg(napi_env env, const char* msg) {
+  napi_throw_error(env, NULL, msg && *msg ? msg : "outbreak_hip: unknown error");
+  return NULL;
+}
+
+static int load_api(const char* path, char* err, size_t n) {
+  if (A.so) return 0;
+  void* so = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!so) {
+    snprintf(err, n, "cannot load %s: %s", path, dlerror());
+    return -1;
+  }
+#define SYM(field, name)                                         \
+  *(void**)(&A.field) = dlsym(so, name);                         \
+  if (!A.field) {                                                \
+    snprintf(err, n, "%s does not export %s", path, name);       \
+    return -1;                                                   \
+  }
+  SYM(create, "obl_create");
+  SYM(destroy, "obl_destroy");
+  SYM(last_error, "obl_last_error");
+  SYM(global_error, "obl_global_error");
+  SYM(set_base_state, "obl_set_base_state");
+  SYM(set_boundaries, "obl_set_boundaries");
+  SYM(set_ci, "obl_set_ci");
+  SYM(set_surface, "obl_set_surface");
+  SYM(reset, "obl_reset");
+  SYM(step, "obl_step");
+  SYM(add_bubble, "obl_add_bubble");
+  SYM(upload, "obl_upload");
+  SYM(download, "obl_download");
+  SYM(field_ptr, "obl_field_ptr");
+  SYM(sync, "obl_sync");
+  SYM(set_solver_mode, "obl_set_solver_mode");
+  SYM(profile, "obl_profile");
+  SYM(profile_read, "obl_profile_read");
+  SYM(p_create, "obl_poisson_create");
+  SYM(p_destroy, "obl_poisson_destroy");
+  SYM(p_set_density, "obl_poisson_set_density");
+  SYM(p_solve_host, "obl_poisson_solve_host");
+  SYM(p_last_error, "obl_poisson_last_error");
+#undef SYM
+  A.so = so;
+  return 0;
+}
+
+/* ---------------- argument helpers ---------------- */
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return -1;
+  for (size_t i = argc; i < want; i++) napi_get_undefined(env, &argv[i]);
+  return 0;
+}
+static double num(napi_env env, napi_value v) {
+  double d = 0;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+static void* ext(napi_env env, napi_value v) {
+  void* p = NULL;
+  napi_get_value_external(env, v, &p);
+  return p;
+}
+/* data pointer and byte length of a TypedArray (or NULL) */
+static void* tarr(napi_env env, napi_value v, size_t* bytes) {
+  bool is = false;
+  *bytes = 0;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return NULL;
+  napi_typedarray_type t;
+  size_t len = 0, off = 0;
+  void* data = NULL;
+  napi_value ab;
+  napi_get_typedarray_info(env, v, &t, &len, &data, &ab, &off);
+  size_t el = 1;
+  switch (t) {
+    case napi_int16_array: case napi_uint16_array: el = 2; break;
+    case napi_int32_array: case napi_uint32_array: case napi_float32_array: el = 4; break;
+    case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: el = 8; break;
+    default: el = 1;
+  }
+  *bytes = len * el;
+  return data;
+}
+static napi_value ok_or_throw(napi_env env, int rc, obl_model* h) {
+  if (rc) return throw_msg(env, h ? A.last_error(h) : A.global_error());
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+
+/* ---------------- exports ---------------- */
+static napi_value js_load(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  char path[4096], err[4200];
+  size_t n = 0;
+  napi_get_value_string_utf8(env, argv[0], path, sizeof path, &n);
+  if (load_api(path, err, sizeof err)) return throw_msg(env, err);
+  napi_value t;
+  napi_get_boolean(env, true, &t);
+  return t;
+}
+
+static void model_finalize(napi_env env, void* data, void* hint) {
+  (void)env; (void)hint;
+  if (data && A.destroy) A.destroy((obl_model*)data);
+}
+
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  get_args(env, info, 6, argv);
+  if (!A.so) return throw_msg(env, "outbreak_hip library not loaded");
+  obl_grid g = {(int)num(env, argv[0]), (int)num(env, argv[1]), (int)num(env, argv[2]), num(env, argv[3]),
+                num(env, argv[4]), (int)num(env, argv[5])};
+  obl_model* h = NULL;
+  if (A.create(&g, &h)) return throw_msg(env, A.global_error());
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, h, model_finalize, NULL, &out));
+  return out;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  /* freed by the finalizer; an explicit destroy only synchronises */
+  obl_model* h = (obl_model*)ext(env, argv[0]);
+  if (h) A.sync(h);
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+
+static napi_value js_set_base_state(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* B = tarr(env, argv[1], &b);
+  int nz = (int)num(env, argv[2]);
+  if (!B || b < (size_t)(nz + 1) * 32) return throw_msg(env, "setBaseState: Float32Array of (nz+1)*8 expected");
+  return ok_or_throw(env, A.set_base_state(h, B, nz), h);
+}
+
+static napi_value js_set_boundaries(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* f = tarr(env, argv[1], &b);
+  int count = (int)num(env, argv[2]);
+  if (count && (!f || b < (size_t)count * OBL_BOUND_FLOATS * 4)) return throw_msg(env, "setBoundaries: short array");
+  return ok_or_throw(env, A.set_boundaries(h, f, count), h);
+}
+
+static napi_value js_set_ci(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* f = tarr(env, argv[1], &b);
+  int count = (int)num(env, argv[2]);
+  if (count && (!f || b < (size_t)count * OBL_CI_FLOATS * 4)) return throw_msg(env, "setCI: short array");
+  return ok_or_throw(env, A.set_ci(h, f, count), h);
+}
+
+static napi_value js_reset(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* u = tarr(env, argv[1], &b);
+  if (!u || b < OBL_UNIFORM_FLOATS * 4) return throw_msg(env, "reset: Float32Array(76) uniform block expected");
+  return ok_or_throw(env, A.reset(h, u, (int)num(env, argv[2])), h);
+}
+
+static napi_value js_step(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* u = tarr(env, argv[1], &b);
+  if (!u || b < OBL_UNIFORM_FLOATS * 4) return throw_msg(env, "step: Float32Array(76) uniform block expected");
+  return ok_or_throw(env, A.step(h, u, (int)num(env, argv[2]), (unsigned)num(env, argv[3])), h);
+}
+
+static napi_value js_add_bubble(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  get_args(env, info, 7, argv);
+  obl_model* h = ext(env, argv[0]);
+  return ok_or_throw(env, A.add_bubble(h, num(env, argv[1]), num(env, argv[2]), num(env, argv[3]), num(env, argv[4]),
+                                       num(env, argv[5]), num(env, argv[6])), h);
+}
+
+static napi_value js_download(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  void* dst = tarr(env, argv[3], &b);
+  if (!dst) return throw_msg(env, "download: TypedArray expected");
+  return ok_or_throw(env, A.download(h, (int)num(env, argv[1]), (int)num(env, argv[2]), dst, b, 0), h);
+}
+
+static napi_value js_upload(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  void* src = tarr(env, argv[3], &b);
+  if (!src) return throw_msg(env, "upload: TypedArray expected");
+  return ok_or_throw(env, A.upload(h, (int)num(env, argv[1]), (int)num(env, argv[2]), src, b, 0), h);
+}
+
+static napi_value js_field_ptr(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  void* p = NULL;
+  size_t n = 0;
+  if (A.field_ptr(h, (int)num(env, argv[1]), (int)num(env, argv[2]), &p, &n)) return throw_msg(env, A.last_error(h));
+  napi_value o, vp, vn;
+  CHECK_NAPI(napi_create_object(env, &o));
+  CHECK_NAPI(napi_create_bigint_uint64(env, (uint64_t)(uintptr_t)p, &vp));
+  CHECK_NAPI(napi_create_double(env, (double)n, &vn));
+  napi_set_named_property(env, o, "ptr", vp);
+  napi_set_named_property(env, o, "bytes", vn);
+  return o;
+}
+
+static napi_value js_set_solver_mode(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  return ok_or_throw(env, A.set_solver_mode(h, (int)num(env, argv[1]), (int)num(env, argv[2]), num(env, argv[3])), h);
+}
+
+static napi_value js_profile(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_model* h = ext(env, argv[0]);
+  return ok_or_throw(env, A.profile(h, (int)num(env, argv[1])), h);
+}
+
+static napi_value js_profile_read(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  obl_model* h = ext(env, argv[0]);
+  char names[4096];
+  double ms[64];
+  int cnt[64];
+  int n = A.profile_read(h, names, sizeof names, ms, cnt, 64);
+  if (n < 0) return throw_msg(env, A.last_error(h));
+  napi_value o;
+  CHECK_NAPI(napi_create_object(env, &o));
+  char* s = names;
+  for (int i = 0; i < n; i++) {
+    char* e = strchr(s, '\n');
+    if (e) *e = 0;
+    napi_value v;
+    napi_create_double(env, ms[i], &v);
+    napi_set_named_property(env, o, s, v);
+    if (!e) break;
+    s = e + 1;
+  }
+  return o;
+}
+
+/* sync as async work -> Promise (onSubmittedWorkDone) */
+typedef struct {
+  obl_model* h;
+  int rc;
+  napi_deferred d;
+  napi_async_work w;
+} SyncJob;
+static void sync_exec(napi_env env, void* data) {
+  (void)env;
+  SyncJob* j = data;
+  j->rc = A.sync(j->h);
+}
+static void sync_done(napi_env env, napi_status st, void* data) {
+  SyncJob* j = data;
+  napi_value v;
+  if (st == napi_ok && j->rc == 0) {
+    napi_get_undefined(env, &v);
+    napi_resolve_deferred(env, j->d, v);
+  } else {
+    napi_value msg;
+    const char* m = A.last_error(j->h);
+    napi_create_string_utf8(env, m && *m ? m : "sync failed", NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &v);
+    napi_reject_deferred(env, j->d, v);
+  }
+  napi_delete_async_work(env, j->w);
+  free(j);
+}
+static napi_value js_sync(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  SyncJob* j = calloc(1, sizeof *j);
+  j->h = ext(env, argv[0]);
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &j->d, &promise));
+  napi_create_string_utf8(env, "obl_sync", NAPI_AUTO_LENGTH, &name);
+  CHECK_NAPI(napi_create_async_work(env, NULL, name, sync_exec, sync_done, j, &j->w));
+  CHECK_NAPI(napi_queue_async_work(env, j->w));
+  return promise;
+}
+
+/* ---------------- PoissonDirect ---------------- */
+static void poisson_finalize(napi_env env, void* data, void* hint) {
+  (void)env; (void)hint;
+  if (data && A.p_destroy) A.p_destroy((obl_poisson*)data);
+}
+static napi_value js_p_create(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  get_args(env, info, 5, argv);
+  if (!A.so) return throw_msg(env, "outbreak_hip library not loaded");
+  obl_poisson* p = NULL;
+  if (A.p_create((int)num(env, argv[0]), (int)num(env, argv[1]), (int)num(env, argv[2]), num(env, argv[3]),
+                 num(env, argv[4]), &p))
+    return throw_msg(env, A.global_error());
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, p, poisson_finalize, NULL, &out));
+  return out;
+}
+static napi_value js_p_set_density(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_poisson* p = ext(env, argv[0]);
+  size_t b1, b2;
+  double* rc = tarr(env, argv[1], &b1);
+  double* rw = tarr(env, argv[2], &b2);
+  if (!rc || !rw) return throw_msg(env, "setDensity: Float64Arrays expected");
+  if (A.p_set_density(p, rc, rw)) return throw_msg(env, A.p_last_error(p));
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+static napi_value js_p_solve(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  get_args(env, info, 6, argv);
+  obl_poisson* p = ext(env, argv[0]);
+  size_t b1, b2;
+  float* rhs = tarr(env, argv[1], &b1);
+  float* phi = tarr(env, argv[2], &b2);
+  if (!rhs || !phi || b1 != b2) return throw_msg(env, "solve: two Float32Arrays of the grid size expected");
+  if (A.p_solve_host(p, rhs, phi, (int)num(env, argv[3]), (int)num(env, argv[4]), num(env, argv[5])))
+    return throw_msg(env, A.p_last_error(p));
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+
+#define EXPORT(name, fn)                                                      \
+  do {                                                                        \
+    napi_value f;                                                             \
+    napi_create_function(env, name, NAPI_AUTO_LENGTH, fn, NULL, &f);          \
+    napi_set_named_property(env, exports, name, f);                          \
+  } while (0)
+
+static napi_value init(napi_env env, napi_value exports) {
+  EXPORT("load", js_load);
+  EXPORT("create", js_create);
+  EXPORT("destroy", js_destroy);
+  EXPORT("setBaseState", js_set_base_state);
+  EXPORT("setBoundaries", js_set_boundaries);
+  EXPORT("setCI", js_set_ci);
+  EXPORT("reset", js_reset);
+  EXPORT("step", js_step);
+  EXPORT("addBubble", js_add_bubble);
+  EXPORT("download", js_download);
+  EXPORT("upload", js_upload);
+  EXPORT("fieldPtr", js_field_ptr);
+  EXPORT("setSolverMode", js_set_solver_mode);
+  EXPORT("profile", js_profile);
+  EXPORT("profileRead", js_profile_read);
+  EXPORT("sync", js_sync);
+  EXPORT("poissonCreate", js_p_create);
+  EXPORT("poissonSetDensity", js_p_set_density);
+  EXPORT("poissonSolve", js_p_solve);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -1,0 +1,341 @@
+'use strict';
+// ─────────────────────────────────────────────────────────────────────────────
+//  RegionalModel / PoissonDirect for Node on MI355X.
+//
+//  The same class and method names, argument shapes, parity convention and
+//  error texts as the reference's model.js:1156-1496 and solver.js:105-165; the
+//  GPU work goes through the N-API addon (obl_napi.node) to liboutbreak_hip.so
+//  (gfx950 HIP kernels). No WGSL is dispatched and there is no WebGPU fallback.
+//
+//  `enc` / `pass` arguments are accepted and ignored: work is enqueued on the
+//  model's HIP stream in call order, as on the reference's single queue.
+//  A minimal `GPU` object gives the driver loop its queue calls
+//  (createCommandEncoder, queue.submit, queue.onSubmittedWorkDone).
+//
+//  Written in ES2019 (no ?? or ?.), so Node 12 runs it.
+// ─────────────────────────────────────────────────────────────────────────────
+const path = require('path');
+const addon = require('./obl_napi.node');
+addon.load(process.env.OBL_LIB || path.join(__dirname, '..', 'liboutbreak_hip.so'));
+
+const DEG = Math.PI / 180;
+const clamp = (v, a, b) => Math.min(Math.max(v, a), b);
+const lerp = (a, b, t) => a + (b - a) * t;
+const dflt = (v, d) => (v === undefined || v === null ? d : v);
+const FIELD = { V: 0, Q1: 1, Q2: 2, Vt: 3, Q1t: 4, Q2t: 5, aux: 6, phT: 7, lsfA: 8, lsfB: 9, ciMap: 10, phi: 11, rhs: 12, bs: 13 };
+const GEMM = { fp32: 0, bf16x3: 1 };
+const THOMAS = { fp32: 0, fp64: 1, mixed: 2 };
+const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256;
+
+// ── the uniform block: MODEL_FIELDS (model.js:23-37), UBlock (util.js:83-98) ──
+const MODEL_FIELDS = ['dims', 'dzv', 'step', 'sponge', 'turb', 'micro', 'misc', 'cor', 'ci', 'pad', 'micro2',
+  'hur', 'hur2', 'hur3', 'hur4', 'hur5', 'vtx', 'vtx2', 'cn'];
+class UBlock {
+  constructor(fields) {
+    this.off = {};
+    fields.forEach((n, i) => { this.off[n] = 4 * i; });
+    this.f = new Float32Array(4 * fields.length);
+    this.size = this.f.byteLength;
+  }
+  set(name, a, b, c, d) {
+    const o = this.off[name];
+    if (o === undefined) throw new Error('UBlock: no field ' + name);
+    this.f[o] = dflt(a, 0); this.f[o + 1] = dflt(b, 0); this.f[o + 2] = dflt(c, 0); this.f[o + 3] = dflt(d, 0);
+    return this;
+  }
+}
+
+// ── grid helpers (model.js:1104-1154) ──
+const LEVELS_BY_DX = [[100, 160], [250, 100], [500, 80], [1000, 64], [1500, 60], [2000, 56], [3000, 52], [4000, 48]];
+function levelsFor(dx) {
+  const T = LEVELS_BY_DX, l = Math.log(clamp(dx, T[0][0], T[T.length - 1][0]));
+  for (let i = 0; i < T.length - 1; i++) {
+    const a = Math.log(T[i][0]), b = Math.log(T[i + 1][0]);
+    if (l <= b) return Math.round(lerp(T[i][1], T[i + 1][1], (l - a) / (b - a)) / 4) * 4;
+  }
+  return T[T.length - 1][1];
+}
+function customGrid(c) {
+  const dx = clamp(+c.dx || 500, 100, 4000);
+  const side = km => clamp(Math.round((+km || 256) * 1000 / dx / 8) * 8, 64, 2040);
+  const nx = side(c.w), ny = side(c.h), nz = levelsFor(dx);
+  return { key: `custom:${dx}:${nx}x${ny}x${nz}`, dx, n: [nx, ny, nz] };
+}
+function gridPad(dx) { return clamp(Math.round(10000 / dx / 8) * 8, 8, 64); }
+function gridBytes(g) {
+  const [nx, ny, nz] = g.n, p = gridPad(g.dx), nxp = nx + 2 * p, nyp = ny + 2 * p;
+  return 1.05 * ((nx + 1) * (ny + 1) * (nz + 1) * 48 + nx * ny * nz * 108 + nxp * nyp * nz * 20 + nx * ny * 192 + 8 * (nxp * nxp + nyp * nyp));
+}
+const GRID_PRESETS = {
+  quick: { dx: 2000, n: [320, 320, 56] }, standard: { dx: 1500, n: [448, 448, 60] }, large: { dx: 1500, n: [640, 640, 60] },
+  fine: { dx: 1000, n: [640, 640, 64] }, m500: { dx: 500, n: [512, 512, 80] }, m250: { dx: 250, n: [512, 512, 100] },
+  m100: { dx: 100, n: [512, 512, 160] }, h2: { dx: 2000, n: [640, 640, 56] }, h3: { dx: 3000, n: [640, 640, 52] },
+  h15: { dx: 1500, n: [960, 960, 60] },
+};
+
+// ── soundings: SHARPpy parser, winds and the hydrostatic base state (sounding.js:10-197) ──
+const Sounding = (function () {
+  const KT = 0.514444, RD = 287.04, CP = 1005.7, EPS = 0.622, G = 9.80665;
+  function parseSharppy(text) {
+    const out = { title: '', station: '', time: null, levels: [] };
+    let mode = '';
+    for (const raw of text.replace(/\r/g, '').split('\n')) {
+      const L = raw.trim();
+      if (!L) continue;
+      if (/^%TITLE%/i.test(L)) { mode = 'title'; continue; }
+      if (/^%RAW%/i.test(L)) { mode = 'raw'; continue; }
+      if (/^%END%/i.test(L)) { mode = ''; continue; }
+      if (mode === 'title') { if (!out.title) { out.title = L; out.station = L.split(/\s+/)[0]; } continue; }
+      if (mode === 'raw' || (!mode && /^-?\d/.test(L))) {
+        const f = L.split(/[,\s]+/).filter(Boolean).map(Number);
+        if (f.length < 6 || f.some(v => !isFinite(v))) continue;
+        const [p, h, t, td, wd, ws] = f;
+        if (p <= 0 || h < -1000 || h > 40000 || p === -9999) continue;
+        out.levels.push({ p, h, t: t > -9990 ? t : NaN, td: td > -9990 ? td : NaN, wd: wd > -9990 ? wd : NaN, ws: ws > -9990 ? ws : NaN });
+      }
+    }
+    out.levels.sort((a, b) => b.p - a.p);
+    if (out.levels.length < 4) throw new Error('Could not read at least 4 levels. Expected SHARPpy format: %RAW% rows of  PRES, HGHT, TEMP, DWPT, WDIR, WSPD(kt)  ... %END%');
+    return out;
+  }
+  function es(t) { return 6.112 * Math.exp(17.67 * t / (t + 243.5)); }
+  function mixr(p, t) { const e = es(t); return EPS * e / Math.max(p - e, 1e-3); }
+  function winds(snd) {
+    const lv = snd.levels.filter(l => isFinite(l.h));
+    const sfcH = lv[0].h;
+    const wl = lv.filter(l => isFinite(l.wd) && isFinite(l.ws)).map(l => {
+      const d = l.wd * Math.PI / 180, s = l.ws * KT;
+      return { z: (l.h - sfcH) / 1000, u: -s * Math.sin(d), v: -s * Math.cos(d) };
+    });
+    return function windAt(z) {
+      if (z <= wl[0].z) return [wl[0].u, wl[0].v];
+      for (let i = 0; i < wl.length - 1; i++) {
+        if (z >= wl[i].z && z <= wl[i + 1].z) {
+          const f = (z - wl[i].z) / Math.max(1e-6, wl[i + 1].z - wl[i].z);
+          return [wl[i].u + (wl[i + 1].u - wl[i].u) * f, wl[i].v + (wl[i + 1].v - wl[i].v) * f];
+        }
+      }
+      const l = wl[wl.length - 1];
+      return [l.u, l.v];
+    };
+  }
+  function baseState(snd, nz, dz, motion) {
+    const lv = snd.levels.filter(l => isFinite(l.h) && isFinite(l.t)).slice().sort((a, b) => a.h - b.h);
+    const sfcH = lv[0].h, sfcP = lv[0].p, top = lv[lv.length - 1];
+    const byH = (key, zAgl) => {
+      const h = zAgl + sfcH;
+      if (h <= lv[0].h) return lv[0][key];
+      for (let i = 0; i < lv.length - 1; i++) {
+        if (h <= lv[i + 1].h) {
+          const a = lv[i], b = lv[i + 1], f = (h - a.h) / Math.max(b.h - a.h, 1e-3), va = a[key], vb = b[key];
+          if (!isFinite(va) || !isFinite(vb)) return isFinite(va) ? va : vb;
+          if (key === 'p') return Math.exp(Math.log(va) + (Math.log(vb) - Math.log(va)) * f);
+          return va + (vb - va) * f;
+        }
+      }
+      return NaN;
+    };
+    const windAt = winds(snd);
+    const B = new Float32Array((nz + 1) * 8), th = [], qv = [];
+    for (let k = 0; k < nz; k++) {
+      const z = (k + 0.5) * dz;
+      let T = byH('t', z), Td = byH('td', z), p = byH('p', z);
+      if (!isFinite(T) || !isFinite(p)) {
+        const dzt = z + sfcH - top.h;
+        T = top.t; p = top.p * Math.exp(-G * dzt / (RD * (top.t + 273.15))); Td = Math.min(top.td, -80);
+      }
+      if (!isFinite(Td)) Td = T - 30;
+      Td = Math.min(Td, T - 0.3);
+      th.push((T + 273.15) * Math.pow(1000 / p, RD / CP));
+      qv.push(Math.max(1e-7, mixr(p, Td)));
+    }
+    const piF = new Float64Array(nz + 1);
+    piF[0] = Math.pow(sfcP / 1000, RD / CP);
+    for (let k = 0; k < nz; k++) piF[k + 1] = piF[k] - G * dz / (CP * th[k] * (1 + 0.61 * qv[k]));
+    for (let k = 0; k < nz; k++) {
+      const pi = 0.5 * (piF[k] + piF[k + 1]), p = 100000 * Math.pow(pi, CP / RD), T = th[k] * pi;
+      const esx = 611.2 * Math.exp(17.67 * (T - 273.15) / (T - 29.65)), qvs = EPS * esx / (p - esx);
+      qv[k] = Math.min(qv[k], 0.97 * qvs);
+      const rho = p / (RD * T * (1 + 0.61 * qv[k])), w = windAt((k + 0.5) * dz / 1000);
+      B.set([th[k], qv[k], rho, 0, pi, p, w[0] - motion[0], w[1] - motion[1]], k * 8);
+    }
+    for (let k = 0; k <= nz; k++) {
+      B[k * 8 + 3] = k === 0 ? B[2] + (B[2] - B[10]) * 0.5 : k === nz ? B[(nz - 1) * 8 + 2] * 0.94 : 0.5 * (B[(k - 1) * 8 + 2] + B[k * 8 + 2]);
+    }
+    return { B, sfcP, sfcH };
+  }
+  return { parseSharppy, baseState, winds, mixr, KT };
+})();
+
+// ── a minimal stand-in for the driver's GPU object (util.js:101-191): queue calls only ──
+function makeGPU() {
+  const models = new Set();
+  return {
+    errors: [],
+    device: {
+      createCommandEncoder() { return { beginComputePass() { return { end() {} }; }, finish() { return {}; } }; },
+      queue: {
+        submit() {},
+        onSubmittedWorkDone() { return Promise.all(Array.from(models).map(m => m.h ? addon.sync(m.h) : null)); },
+      },
+    },
+    track(m) { models.add(m); },
+    untrack(m) { models.delete(m); },
+  };
+}
+const GPU = makeGPU();
+
+class PoissonDirect {
+  // solver.js:106-137; nx, ny must be multiples of 8 on this GPU path
+  constructor(nx, ny, nz, dx, dz) {
+    Object.assign(this, { nx, ny, nz, dx, dz });
+    this.h = addon.poissonCreate(nx, ny, nz, dx, dz);
+    this.setDensity(new Array(nz).fill(1), new Array(nz + 1).fill(1));
+  }
+  setDensity(rhoC, rhoW) { addon.poissonSetDensity(this.h, Float64Array.from(rhoC), Float64Array.from(rhoW)); }
+  // host arrays (x fastest); precision: {gemm: 'fp32'|'bf16x3', thomas: 'fp32'|'fp64'|'mixed', cond}
+  solve(rhs, opt) {
+    const o = opt || {};
+    const phi = new Float32Array(rhs.length);
+    addon.poissonSolve(this.h, rhs instanceof Float32Array ? rhs : Float32Array.from(rhs), phi,
+      GEMM[o.gemm || 'fp32'], THOMAS[o.thomas || 'mixed'], dflt(o.cond, 16));
+    return phi;
+  }
+  destroy() { this.h = null; }
+}
+
+// a device field: download()/upload() copy it as a typed array in the reference's texel layout
+class Field {
+  constructor(m, name, copy, count, Ctor) { Object.assign(this, { m, name, copy, count, Ctor: Ctor || Float32Array }); }
+  download() { const a = new this.Ctor(this.count); addon.download(this.m.h, FIELD[this.name], this.copy, a); return a; }
+  upload(a) { addon.upload(this.m.h, FIELD[this.name], this.copy, a); }
+  get ptr() { return addon.fieldPtr(this.m.h, FIELD[this.name], this.copy); }
+}
+
+class RegionalModel {
+  constructor() {
+    this.U = new UBlock(MODEL_FIELDS);
+    // model.js:1176-1180, with the outbreak overrides of app.js:1613
+    this.params = {
+      cs: 0.18, nu0: 1.0, drag: 1, z0: 0.1, spongeW: 36000, spongeTau: 100, dampZ: 15000, dampTau: 150,
+      auto: 1e-3, qc0: 2e-3, freeze: 1.0, noise: 0.25, mc: 1.0, vcap: 110, evap: 0.22, sortQ: 5e-4, ciRate: 0.5, nudgeTau: 5400, lat: 35,
+      moistMix: 0, mcFloor: 0.25,
+    };
+    this.parity = 0; this.time = 0; this.steps = 0;
+    this.bounds = []; this.ciEvents = []; this.ciActive = 0;
+    this.mapBusy = [false, false];
+    this.noiseSeed = 0;
+    this.h = null;
+  }
+  // model.js:1186-1201: throws with a user-facing message when the grid does not fit
+  async allocate(key, spec) {
+    this.destroy();
+    const pr = spec || GRID_PRESETS[key], [nx, ny, nz] = pr.n;
+    Object.assign(this, { key, nx, ny, nz, dx: pr.dx, dz: 20000 / nz });
+    this.Lx = nx * this.dx; this.Ly = ny * this.dx; this.Lz = nz * this.dz;
+    this.params.spongeW = clamp(0.08 * Math.min(this.Lx, this.Ly), 5000, 40000);
+    this.pad = gridPad(this.dx);
+    const nxp = nx + 2 * this.pad, nyp = ny + 2 * this.pad;
+    try { this.h = addon.create(nx, ny, nz, this.dx, this.dz, this.pad); } catch (e) {
+      this.key = null;
+      if (/memory/.test(e.message) && !/^not enough GPU memory/.test(e.message)) throw new Error(`not enough GPU memory for this grid (it needs about ${(gridBytes(pr) / 1e9).toFixed(1)} GB): ${e.message}`);
+      throw e;
+    }
+    GPU.track(this);
+    this.U.set('pad', this.pad, nxp, nyp, 0);
+    this.U.set('dims', nx, ny, nz, this.dx);
+    this.U.set('dzv', this.dz, 1 / this.dx, 1 / this.dz, 0);
+    const nv = (nx + 1) * (ny + 1) * (nz + 1) * 4, nc = nx * ny * nz * 4;
+    this.V = [new Field(this, 'V', 0, nv), new Field(this, 'V', 1, nv)];
+    this.Q1 = [new Field(this, 'Q1', 0, nc), new Field(this, 'Q1', 1, nc)];
+    this.Q2 = [new Field(this, 'Q2', 0, nc), new Field(this, 'Q2', 1, nc)];
+    this.Vt = new Field(this, 'Vt', 0, nv);
+    this.aux = new Field(this, 'aux', 0, nx * ny * nz * 2);
+    this.phi = new Field(this, 'phi', 0, nxp * nyp * nz);
+    this.rhs = new Field(this, 'rhs', 0, nxp * nyp * nz);
+    this.bs = new Field(this, 'bs', 0, (nz + 1) * 8);
+    this.solver = { nx: nxp, ny: nyp, nz, dx: this.dx, dz: this.dz };
+  }
+  destroy() {
+    if (!this.h) return;
+    addon.destroy(this.h);
+    GPU.untrack(this);
+    this.h = null; this.V = null; this.solver = null;
+  }
+  setSounding(text) {
+    const snd = Sounding.parseSharppy(text);
+    const bsx = Sounding.baseState(snd, this.nz, this.dz, [0, 0]);
+    this.snd = snd; this.base = bsx; this.rhoSfc = bsx.B[2];
+    addon.setBaseState(this.h, bsx.B, this.nz);
+    return { snd, windAt: Sounding.winds(snd) };
+  }
+  setPrepField(on) { if (on) throw new Error('the trajectory-averaged pressure force (hurricane mode) is not supported on this GPU path'); }
+  setSurface(surf, hur) {
+    if (surf || hur) throw new Error('hurricane mode is not supported on this GPU path');
+    this.hur = null;
+  }
+  setVortex(v) {
+    if (v) throw new Error('the initial vortex (hurricane mode) is not supported on this GPU path');
+    this.vortex = null;
+    return null;
+  }
+  baseWind(z) { const k = clamp(Math.floor(z / this.dz), 0, this.nz - 1), B = this.base.B; return [B[k * 8 + 6], B[k * 8 + 7]]; }
+  setBoundaries(bnds) {
+    this.bounds = bnds.slice(0, MAX_BOUNDS);
+    const f = new Float32Array(MAX_BOUNDS * 148);
+    this.bounds.forEach((b, i) => {
+      const o = i * 148, pts = b.pts.slice(0, MAX_BPTS);
+      f.set([pts.length, b.motion[0], b.motion[1], b.halfW, b.dT, b.Hmax, b.slope, b.Hmin, b.dq, b.qRatio, b.du, b.dv, b.wRatio, b.nudge ? 1 : 0, b.dFree, b.side,
+        b.vary || 0, b.seed || 0, b.varyL || 60000, 0], o);
+      pts.forEach((p, k) => { const v = (b.vel && b.vel[k]) || b.motion; f.set([p[0], p[1], v[0], v[1]], o + 20 + 4 * k); });
+    });
+    addon.setBoundaries(this.h, f, this.bounds.length);
+  }
+  setCI(events) {
+    const act = events.filter(e => e.t1 > this.time).slice(-MAX_CI);
+    this.ciActive = act.length;
+    const f = new Float32Array(Math.max(1, act.length) * 12);
+    act.forEach((e, i) => f.set([e.x, e.y, e.zc, e.rh, e.rz, e.wt, e.t0, e.t1, e.vx || 0, e.vy || 0, e.seed || i * 1.37, e.ramp || 150], i * 12));
+    addon.setCI(this.h, f, act.length);
+  }
+  writeUniforms(dt) {
+    const U = this.U, P = this.params;
+    U.set('step', dt, this.time, this.rhoSfc || 1.15, 9.81);
+    U.set('sponge', P.spongeW, P.spongeTau, P.dampZ, P.dampTau);
+    U.set('turb', P.cs, P.nu0, P.evap, P.drag ? 1 : 0);
+    const cd = Math.pow(0.41 / Math.log(0.5 * this.dz / P.z0 + 1), 2);
+    U.set('micro', P.auto, P.qc0, P.freeze, cd);
+    U.set('misc', P.vcap, P.mc, P.noise, P.ciRate);
+    U.set('cor', 2 * 7.292e-5 * Math.sin(P.lat * DEG), P.nudgeTau, this.bounds.length, dflt(P.moistMix, 0));
+    U.set('ci', this.ciActive || 0, 0, 0, 0);
+    U.set('micro2', 0, dflt(P.sortQ, 5e-4), P.scheme === 'thompson' ? 1 : 0, dflt(P.evapT, 0.3));
+    U.set('hur', 0, 0, 0, 0); U.set('hur2', 1, 0, 1, 0.8); U.set('hur3', 0, 1, 0, 0); U.set('hur4', 0, 0, 1, 1e9); U.set('hur5', 2.4e-3, 0, 0, 0);
+    U.set('cn', 0, this.phiScale || 1, 1, 0);
+    U.set('vtx', 0, 0, 1, 0); U.set('vtx2', 1, 0, 0, 0);
+    U.set('dzv', this.dz, 1 / this.dx, 1 / this.dz, dflt(P.mcFloor, 0.25));
+    return U.f;
+  }
+  reset() {
+    this.time = 0; this.steps = 0; this.parity = 0; this.phiScale = 1;
+    addon.reset(this.h, this.writeUniforms(1), this.noiseSeed || 0);
+    this.lsfT = 0;
+  }
+  // build-defined warm bubble (no reference counterpart), on both state copies
+  addBubble(x, y, z, rh, rz, dth) { addon.addBubble(this.h, x, y, z, rh, rz, dth); }
+  encStep(enc, dt) {
+    void enc;
+    const u = this.writeUniforms(dt);
+    let flags = 0;
+    if (this.time - this.lsfT >= 60 || this.time === 0) { flags |= 1; this.lsfT = this.time; }
+    addon.step(this.h, u, this.parity, flags);
+    this.parity = 1 - this.parity;
+    this.time += dt; this.steps++;
+    this.phiScale = 1;
+  }
+  setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'fp32'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
+  sync() { return addon.sync(this.h); }
+  requestMaps() { return null; }  // the 2-D map kernel (model.js:1033-1100) is not part of this library yet
+  get Vcur() { return this.V[this.parity]; }
+}
+
+module.exports = { RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

@@ -90,7 +90,7 @@ class RegionalModel:
         if rc != 0:
             self.key = None
             msg = _lib.lib().obl_global_error().decode()
-            if "memory" in msg:
+            if "memory" in msg and not msg.startswith("not enough GPU memory"):
                 raise MemoryError(f"not enough GPU memory for this grid (it needs about {grid_bytes(pr) / 1e9:.1f} GB): "
                                   + msg)
             raise _lib.OblError(msg)

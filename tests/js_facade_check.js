@@ -1,0 +1,28 @@
+// CPU checks of the Node facade (run by tests/test_js.py): the addon loads and exports its
+// functions, the JS base state equals the reference sounding.js golden tables, and the
+// uniform block for a given model state is printed for comparison with the Python host.
+'use strict';
+const fs = require('fs');
+const path = require('path');
+const O = require(path.join(__dirname, '..', 'outbreaklab_amd', 'js', 'outbreak.js'));
+const G = JSON.parse(fs.readFileSync(path.join(__dirname, 'golden', 'sounding_golden.json'), 'utf8'));
+let mismatches = 0;
+for (const c of G) {
+  for (const k in c.base) {
+    const [nz, dz] = k.split('x').map(Number);
+    const B = O.Sounding.baseState(O.Sounding.parseSharppy(c.text), nz, dz, [0, 0]).B;
+    for (let i = 0; i < B.length; i++) if (B[i] !== Math.fround(c.base[k][i])) mismatches++;
+  }
+}
+// uniform block of a 128 x 128 x 40 (2 km) Thompson model at t = 42 s, dt 6 s
+const m = new O.RegionalModel();
+Object.assign(m, { nx: 128, ny: 128, nz: 40, dx: 2000, dz: 500, Lx: 256000, Ly: 256000, pad: 8 });
+m.params.spongeW = 20480; m.params.scheme = 'thompson'; m.params.noise = 0.1; m.params.lat = 35.9;
+m.U.set('pad', 8, 144, 144, 0); m.U.set('dims', 128, 128, 40, 2000);
+const wk = O.Sounding.baseState(O.Sounding.parseSharppy(fs.readFileSync(path.join(__dirname, 'golden', 'wk_sounding.txt'), 'utf8')), 40, 500, [0, 0]);
+m.rhoSfc = wk.B[2]; m.time = 42;
+const u = Array.from(m.writeUniforms(6));
+let perr = '';
+try { new O.PoissonDirect(30, 30, 10, 1000, 500); } catch (e) { perr = e.message; }
+console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr,
+  levels: [O.levelsFor(2000), O.levelsFor(500), O.levelsFor(250)], pads: [O.gridPad(2000), O.gridPad(500), O.gridPad(250)] }));
