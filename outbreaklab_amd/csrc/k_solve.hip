@@ -170,8 +170,211 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
       }
 }
 
+// ---------------- split hi/lo BF16 form ----------------
+// Each FP32 operand x is split into hi = RNE_bf16(x) and lo = RNE_bf16(x - hi); the
+// product is hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo term, ~2^-18 relative,
+// is dropped). Sine matrices are split once on the host; data operands are split while
+// they are staged into LDS. v_mfma_f32_16x16x32_bf16, 128 x 128 block tiles, 64 x 64 per wave.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int BBM = 128, BBN = 128, BBK = 32, BLD = BBK + 8;  // LDS row: 40 bf16 = 80 B
+
+__device__ __forceinline__ void split8(const float* v, uint4& hi, uint4& lo) {
+  uint32_t h[4], l[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    short h0 = bf16_rne(v[2 * i]), h1 = bf16_rne(v[2 * i + 1]);
+    short l0 = bf16_rne(v[2 * i] - bf16_to_f(h0)), l1 = bf16_rne(v[2 * i + 1] - bf16_to_f(h1));
+    h[i] = (uint32_t)(uint16_t)h0 | ((uint32_t)(uint16_t)h1 << 16);
+    l[i] = (uint32_t)(uint16_t)l0 | ((uint32_t)(uint16_t)l1 << 16);
+  }
+  hi = make_uint4(h[0], h[1], h[2], h[3]);
+  lo = make_uint4(l[0], l[1], l[2], l[3]);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
+  // [parity][hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][2][BBM][BLD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][BBN][BLD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, g = lane >> 4;
+  const int m0 = blockIdx.y * BBM, n0 = blockIdx.x * BBN;
+  const int plane = blockIdx.z;
+  const int N = a.N, Nh = a.Nh;
+  const size_t planeSz = (size_t)N * (size_t)a.inner;
+  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
+
+  v4f acc[2][4][4];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[p][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < Nh; k0 += BBK) {
+    // ---- matrix operand (pre-split bf16): 128 rows x 32 k per array ----
+    {
+      int row = tid >> 1, half = tid & 1;
+      int mrow = (XMODE ? n0 : m0) + row;
+      const size_t wo = (size_t)mrow * a.KpB + k0 + 16 * half;
+      uint16_t(*M)[2][BBN][BLD] = XMODE ? Bs : As;
+      const uint16_t* src[4] = {a.W0h, a.W0l, a.W1h, a.W1l};
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const uint4* q = reinterpret_cast<const uint4*>(src[t] + wo);
+        uint4 v0 = q[0], v1 = q[1];
+        *reinterpret_cast<uint4*>(&M[t >> 1][t & 1][row][16 * half]) = v0;
+        *reinterpret_cast<uint4*>(&M[t >> 1][t & 1][row][16 * half + 8]) = v1;
+      }
+    }
+    // ---- data operand, split while staging ----
+    if (XMODE) {
+      int row = tid >> 1, half = tid & 1;
+      int r = m0 + row, k = k0 + 16 * half;
+      float s0[16], s1[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) { s0[i] = 0.f; s1[i] = 0.f; }
+      if (r < a.rows) {
+        const float* src = a.in + (size_t)r * N;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          int kk = k + 4 * c;
+          if (kk < Nh) {
+            float4 f = *reinterpret_cast<const float4*>(src + kk);
+            if (MODE == DST_XF) {
+              float4 gg = *reinterpret_cast<const float4*>(src + (N - 4 - kk));
+              s0[4 * c + 0] = f.x + gg.w; s0[4 * c + 1] = f.y + gg.z; s0[4 * c + 2] = f.z + gg.y; s0[4 * c + 3] = f.w + gg.x;
+              s1[4 * c + 0] = f.x - gg.w; s1[4 * c + 1] = f.y - gg.z; s1[4 * c + 2] = f.z - gg.y; s1[4 * c + 3] = f.w - gg.x;
+            } else {
+              float4 o = *reinterpret_cast<const float4*>(src + Nh + kk);
+              s0[4 * c + 0] = f.x; s0[4 * c + 1] = f.y; s0[4 * c + 2] = f.z; s0[4 * c + 3] = f.w;
+              s1[4 * c + 0] = o.x; s1[4 * c + 1] = o.y; s1[4 * c + 2] = o.z; s1[4 * c + 3] = o.w;
+            }
+          }
+        }
+      }
+      uint4 h, l;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        split8(s0 + 8 * c, h, l);
+        *reinterpret_cast<uint4*>(&As[0][0][row][16 * half + 8 * c]) = h;
+        *reinterpret_cast<uint4*>(&As[0][1][row][16 * half + 8 * c]) = l;
+        split8(s1 + 8 * c, h, l);
+        *reinterpret_cast<uint4*>(&As[1][0][row][16 * half + 8 * c]) = h;
+        *reinterpret_cast<uint4*>(&As[1][1][row][16 * half + 8 * c]) = l;
+      }
+    } else {
+      int n = tid & 127, kg = tid >> 7;
+      int gn = n0 + n;
+      float s0[16], s1[16];
+      const float* pl = a.in + (size_t)plane * planeSz;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        int k = k0 + 16 * kg + i;
+        float v0 = 0.f, v1 = 0.f;
+        if (k < Nh && gn < a.inner) {
+          float f = pl[(size_t)k * a.inner + gn];
+          if (MODE == DST_YF) {
+            float m = pl[(size_t)(N - 1 - k) * a.inner + gn];
+            v0 = f + m;
+            v1 = f - m;
+          } else {
+            v0 = f;
+            v1 = pl[(size_t)(Nh + k) * a.inner + gn];
+          }
+        }
+        s0[i] = v0;
+        s1[i] = v1;
+      }
+      uint4 h, l;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        split8(s0 + 8 * c, h, l);
+        *reinterpret_cast<uint4*>(&Bs[0][0][n][16 * kg + 8 * c]) = h;
+        *reinterpret_cast<uint4*>(&Bs[0][1][n][16 * kg + 8 * c]) = l;
+        split8(s1 + 8 * c, h, l);
+        *reinterpret_cast<uint4*>(&Bs[1][0][n][16 * kg + 8 * c]) = h;
+        *reinterpret_cast<uint4*>(&Bs[1][1][n][16 * kg + 8 * c]) = l;
+      }
+    }
+    __syncthreads();
+    // ---- MFMA: lane group g feeds k = 8 g .. 8 g + 7 of both operands ----
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        ah[i] = *reinterpret_cast<const bf16x8*>(&As[p][0][wm * 64 + i * 16 + lr][8 * g]);
+        al[i] = *reinterpret_cast<const bf16x8*>(&As[p][1][wm * 64 + i * 16 + lr][8 * g]);
+        bh[i] = *reinterpret_cast<const bf16x8*>(&Bs[p][0][wn * 64 + i * 16 + lr][8 * g]);
+        bl[i] = *reinterpret_cast<const bf16x8*>(&Bs[p][1][wn * 64 + i * 16 + lr][8 * g]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[p][i][j], 0, 0, 0);
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[p][i][j], 0, 0, 0);
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[p][i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // ---- epilogue (as the FP32 form) ----
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        int m = m0 + wm * 64 + i * 16 + 4 * g + e;
+        int n = n0 + wn * 64 + j * 16 + lr;
+        float v0 = acc[0][i][j][e], v1 = acc[1][i][j][e];
+        if (MODE == DST_XF) {
+          if (m < a.rows && n < Nh) {
+            float* dst = a.out + (size_t)m * N;
+            dst[n] = v0;
+            dst[Nh + n] = v1;
+          }
+        } else if (MODE == DST_XI) {
+          if (m < a.rows && n < Nh) {
+            float* dst = a.out + (size_t)m * N;
+            dst[n] = v0 + v1;
+            dst[N - 1 - n] = v0 - v1;
+          }
+        } else if (MODE == DST_YF) {
+          if (m < Nh && n < a.inner) {
+            float* pl = a.out + (size_t)plane * planeSz;
+            pl[(size_t)m * a.inner + n] = v0;
+            pl[(size_t)(Nh + m) * a.inner + n] = v1;
+          }
+        } else {
+          if (m < Nh && n < a.inner) {
+            float* pl = a.out + (size_t)plane * planeSz;
+            pl[(size_t)m * a.inner + n] = v0 + v1;
+            pl[(size_t)(N - 1 - m) * a.inner + n] = v0 - v1;
+          }
+        }
+      }
+}
+
 void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
   dim3 blk(256);
+  if (prec == GEMM_BF16X3) {
+    if (mode == DST_XF || mode == DST_XI) {
+      dim3 grid((a.Nh + BBN - 1) / BBN, (a.rows + BBM - 1) / BBM, 1);
+      if (mode == DST_XF) hipLaunchKernelGGL((k_dst_bf16<DST_XF>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k_dst_bf16<DST_XI>), grid, blk, 0, s, a);
+    } else {
+      dim3 grid((a.inner + BBN - 1) / BBN, (a.Nh + BBM - 1) / BBM, a.rows);
+      if (mode == DST_YF) hipLaunchKernelGGL((k_dst_bf16<DST_YF>), grid, blk, 0, s, a);
+      else hipLaunchKernelGGL((k_dst_bf16<DST_YI>), grid, blk, 0, s, a);
+    }
+    return;
+  }
   if (mode == DST_XF || mode == DST_XI) {
     dim3 grid((a.Nh + BN - 1) / BN, (a.rows + BM - 1) / BM, 1);
     if (mode == DST_XF) hipLaunchKernelGGL((k_dst<DST_XF, 0>), grid, blk, 0, s, a);
@@ -181,7 +384,6 @@ void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
     if (mode == DST_YF) hipLaunchKernelGGL((k_dst<DST_YF, 0>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k_dst<DST_YI, 0>), grid, blk, 0, s, a);
   }
-  (void)prec;
 }
 
 // ---------------- Thomas ----------------

@@ -106,9 +106,24 @@ struct Timer {
 
 // ---------------------------------------------------------------- PoissonDirect
 struct AxisMats {
-  int N = 0, Nh = 0, Npad = 0, Kp = 0;
+  int N = 0, Nh = 0, Npad = 0, Kp = 0, NpadB = 0, KpB = 0;
   float* W[4] = {nullptr, nullptr, nullptr, nullptr};  // We, Wo, WeT, WoT
+  uint16_t* Wb[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // hi, lo
 };
+
+// round-to-nearest-even FP32 -> BF16 bits
+static uint16_t bf16_bits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float bf16_value(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
 
 struct obl_poisson {
   int nx = 0, ny = 0, nz = 0;
@@ -169,12 +184,62 @@ static int build_axis(AxisMats& A, int N, std::string* err) {
     if (hip_ok(hipMemcpy(A.W[t], h[t].data(), n * sizeof(float), hipMemcpyHostToDevice), err, "hipMemcpy(sine matrix)"))
       return -1;
   }
+  // split-BF16 copies: hi = bf16(v), lo = bf16(v - hi) from the FP64 values, padded to the
+  // 128 x 32 tiles of that kernel
+  A.NpadB = (A.Nh + 127) / 128 * 128;
+  A.KpB = (A.Nh + 31) / 32 * 32;
+  size_t nb = (size_t)A.NpadB * A.KpB;
+  std::vector<uint16_t> hb[4][2];
+  for (int t = 0; t < 4; t++)
+    for (int q = 0; q < 2; q++) hb[t][q].assign(nb, 0);
+  auto put = [&](int t, size_t idx, double val) {
+    uint16_t hi = bf16_bits((float)val);
+    uint16_t lo = bf16_bits((float)(val - (double)bf16_value(hi)));
+    hb[t][0][idx] = hi;
+    hb[t][1][idx] = lo;
+  };
+  for (int s = 0; s < A.Nh; s++)
+    for (int i = 0; i < A.Nh; i++) {
+      double ve = v(2 * s, i), vo = v(2 * s + 1, i);
+      put(0, (size_t)s * A.KpB + i, ve);
+      put(1, (size_t)s * A.KpB + i, vo);
+      put(2, (size_t)i * A.KpB + s, ve);
+      put(3, (size_t)i * A.KpB + s, vo);
+    }
+  for (int t = 0; t < 4; t++)
+    for (int q = 0; q < 2; q++) {
+      if (hip_ok(hipMalloc(&A.Wb[t][q], nb * 2), err, "hipMalloc(bf16 sine matrix)")) return -1;
+      if (hip_ok(hipMemcpy(A.Wb[t][q], hb[t][q].data(), nb * 2, hipMemcpyHostToDevice), err, "hipMemcpy(bf16 matrix)"))
+        return -1;
+    }
   return 0;
 }
 
 static void free_axis(AxisMats& A) {
-  for (int t = 0; t < 4; t++)
+  for (int t = 0; t < 4; t++) {
     if (A.W[t]) hipFree(A.W[t]);
+    for (int q = 0; q < 2; q++)
+      if (A.Wb[t][q]) hipFree(A.Wb[t][q]);
+  }
+}
+
+static DstArgs dst_args(const float* in, float* out, const AxisMats& M, int inv, int rows, int inner) {
+  DstArgs a;
+  a.in = in;
+  a.out = out;
+  a.W0 = M.W[inv ? 2 : 0];
+  a.W1 = M.W[inv ? 3 : 1];
+  a.N = M.N;
+  a.Nh = M.Nh;
+  a.rows = rows;
+  a.inner = inner;
+  a.Kp = M.Kp;
+  a.W0h = M.Wb[inv ? 2 : 0][0];
+  a.W0l = M.Wb[inv ? 2 : 0][1];
+  a.W1h = M.Wb[inv ? 3 : 1][0];
+  a.W1l = M.Wb[inv ? 3 : 1][1];
+  a.KpB = M.KpB;
+  return a;
 }
 
 // permuted eigenvalues: spectral slot s < N/2 holds mode 2s, slot N/2 + s holds mode 2s+1
@@ -269,10 +334,10 @@ static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, i
   int nx = p->nx, ny = p->ny, nz = p->nz;
   DstArgs a;
   // forward x: rhs -> tmp
-  a = {rhs, p->tmp, p->ax.W[0], p->ax.W[1], nx, nx / 2, ny * nz, nx, p->ax.Kp};
+  a = dst_args(rhs, p->tmp, p->ax, 0, ny * nz, nx);
   TIMED(p->timer, s, "ps.xf", launch_dst(s, DST_XF, gemm, a));
   // forward y: tmp -> hat
-  a = {p->tmp, p->hat, p->ay.W[0], p->ay.W[1], ny, ny / 2, nz, nx, p->ay.Kp};
+  a = dst_args(p->tmp, p->hat, p->ay, 0, nz, nx);
   TIMED(p->timer, s, "ps.yf", launch_dst(s, DST_YF, gemm, a));
   ThomasArgs t;
   t.hat = p->hat; t.cp32 = p->cp32; t.cp64 = p->cp64; t.dp64 = p->dp64;
@@ -282,10 +347,10 @@ static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, i
   t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
   TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
   // inverse y: hat -> tmp
-  a = {p->hat, p->tmp, p->ay.W[2], p->ay.W[3], ny, ny / 2, nz, nx, p->ay.Kp};
+  a = dst_args(p->hat, p->tmp, p->ay, 1, nz, nx);
   TIMED(p->timer, s, "ps.yi", launch_dst(s, DST_YI, gemm, a));
   // inverse x: tmp -> phi
-  a = {p->tmp, phi, p->ax.W[2], p->ax.W[3], nx, nx / 2, ny * nz, nx, p->ax.Kp};
+  a = dst_args(p->tmp, phi, p->ax, 1, ny * nz, nx);
   TIMED(p->timer, s, "ps.xi", launch_dst(s, DST_XI, gemm, a));
   return hip_ok(hipGetLastError(), &p->err, "PoissonDirect.enc launch");
 }

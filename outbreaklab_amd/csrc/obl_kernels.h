@@ -39,6 +39,12 @@ struct DstArgs {
   int rows;         // X passes: rows (nyp * nz); Y passes: planes (nz)
   int inner;        // Y passes: row length (nxp); X passes: N
   int Kp;           // row stride of W0/W1
+  // split-BF16 form: hi/lo halves of the two matrices, [NpadB][KpB] bf16 (zero-padded)
+  const uint16_t* W0h;
+  const uint16_t* W0l;
+  const uint16_t* W1h;
+  const uint16_t* W1l;
+  int KpB;
 };
 enum { DST_XF = 0, DST_XI = 1, DST_YF = 2, DST_YI = 3 };
 enum { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };

@@ -102,3 +102,42 @@ def test_solve_1024_config():
         print(f"1024 {thomas}: rel err {e:.3e}")
         assert e <= 1e-5
     ps.destroy()
+
+
+@pytest.mark.parametrize("shape", [(40, 144, 144), (24, 40, 56)])
+def test_solve_split_bf16_matches_fp64(shape):
+    """Split hi/lo BF16 transforms (three MFMA products, FP32 accumulate) + FP64 Thomas vs the
+    FP64 oracle: the north-star pressure bound, relative L2 <= 1e-5."""
+    from outbreaklab_amd import PoissonDirect
+
+    nz, ny, nx = shape
+    dx, dz = 2000.0, 20000.0 / nz
+    rc, rw = wk_density(nz, dz)
+    rhs = rough_rhs(nz, ny, nx, 11)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    ref = oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs.astype(np.float64), "f64")
+    e32 = rel_l2(ps.solve(rhs, gemm="fp32", thomas="fp64"), ref)
+    e16 = rel_l2(ps.solve(rhs, gemm="bf16x3", thomas="fp64"), ref)
+    print(f"shape {shape}: fp32 {e32:.3e}  bf16x3 {e16:.3e}")
+    assert e16 <= 1e-5
+    ps.destroy()
+
+
+def test_solve_1024_split_bf16():
+    from outbreaklab_amd import PoissonDirect
+    from outbreaklab_amd.configs import solve_rhs_1024
+
+    nz, ny, nx = 80, 1024, 1024
+    dx, dz = 500.0, 250.0
+    rc, rw = wk_density(nz, dz)
+    rhs = solve_rhs_1024(nx, ny, nz, seed=1)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    ref = solve_scipy(rhs, dx, dz, rc, rw)
+    phi = ps.solve(rhs.astype(np.float32), gemm="bf16x3", thomas="mixed")
+    e = rel_l2(phi, ref)
+    res = rel_l2(apply_operator(phi, dx, dz, rc, rw), rhs)
+    print(f"1024 bf16x3 mixed: rel err {e:.3e}, residual {res:.3e}")
+    assert e <= 1e-5
+    ps.destroy()
