@@ -131,6 +131,18 @@ int obl_profile(obl_model* h, int on);
  * names are written as a '\n'-separated list. Returns the number of kernels. */
 int obl_profile_read(obl_model* h, char* names, size_t names_len, double* ms, int* launches, int max);
 
+/* ---------------- microphysics surrogate: teacher sampling ----------------
+ * No reference counterpart (the reference has no surrogate). While enabled, each obl_step
+ * appends sampled (input, output) pairs of the microphysics block (model.js:775-899) to a
+ * device buffer of `capacity` rows of 24 floats: T, p0, rho0, qv, qc, qr, qi, qs, qg, Nr, Ni,
+ * w, dz, dt, then dT, dqv, dqc, dqi, dqr, dqs, dqg, dNr, dNi, level. Points with condensate or
+ * a phase change are kept with probability p_active, the rest with p_clear.
+ * capacity 0 disables sampling; enabling again clears the buffer. */
+int obl_dump_enable(obl_model* h, unsigned capacity, float p_active, float p_clear);
+/* rows written so far (<= capacity); copies up to max_rows rows to out (may be NULL) and
+ * returns the device buffer in *dev_buf (may be NULL) */
+int obl_dump_read(obl_model* h, float* out, size_t max_rows, size_t* rows, void** dev_buf);
+
 /* ---------------- PoissonDirect ---------------- */
 /* constructor (solver.js:106-137): nx, ny even; dz, dx in metres */
 int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, obl_poisson** out);

@@ -44,6 +44,8 @@ _SIGS = {
     "obl_set_solver_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "obl_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "obl_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_dump_enable": (C.c_int, [C.c_void_p, C.c_uint, C.c_float, C.c_float]),
+    "obl_dump_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]),
     "obl_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     "obl_poisson_destroy": (None, [C.c_void_p]),
     "obl_poisson_set_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

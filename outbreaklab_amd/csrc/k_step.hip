@@ -645,12 +645,13 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   T += OBL_LFC / OBL_CPD * (qi - qi0 - f * dsum);
 }
 
+template <bool DUMP>
 __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
                                               const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
                                               const float2* __restrict__ aux, const float4* __restrict__ lsfA,
                                               const float4* __restrict__ lsfB, const float* __restrict__ phT,
                                               const float4* __restrict__ ciMap, float4* __restrict__ Vout,
-                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
   Dims d = dims(u);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > d.nx || y > d.ny || z > d.nz) return;
@@ -760,7 +761,27 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
     q2.x = fmaxf(q2.x, 0.0f); q2.y = fmaxf(q2.y, 0.0f); q2.z = fmaxf(q2.z, 0.0f); q2.w = fmaxf(q2.w, 0.0f);
     float T = th * B.x;
     float qv = q1.x, qc = q1.y, qi = q1.z, qr = q1.w, qs = q2.x, qg = q2.y, nr = fmaxf(q2.z, 0.0f);
+    const float T0 = T, qv0 = qv, qc0 = qc, qi0 = qi, qr0 = qr, qs0 = qs, qg0 = qg, nr0 = nr;
     micro_block(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
+    if (DUMP) {
+      // teacher pairs of the microphysics block: stratified sample, active points (condensate present or
+      // T changed) with probability p_active, the rest with p_clear
+      bool active = (qc0 + qi0 + qr0 + qs0 + qg0 > 0.0f) || (T != T0);
+      float r = (float)(hash3i(x, y, z ^ (int)dump.salt) >> 8) * (1.0f / 16777216.0f);
+      if (r < (active ? dump.p_active : dump.p_clear)) {
+        unsigned slot = atomicAdd(dump.count, 1u);
+        if (slot < dump.cap) {
+          float wc = 0.5f * (V.z + ldVc(Vin, d, x, y, z + 1, 2));
+          float* row = dump.buf + (size_t)slot * OBL_DUMP_STRIDE;
+          float in[14] = {T0, B.y, A.z, qv0, qc0, qr0, qi0, qs0, qg0, nr0, 0.0f, wc, dz, dt};
+          float out[10] = {T - T0, qv - qv0, qc - qc0, qi - qi0, qr - qr0, qs - qs0, qg - qg0, nr - nr0, 0.0f, (float)z};
+#pragma unroll
+          for (int t = 0; t < 14; t++) row[t] = in[t];
+#pragma unroll
+          for (int t = 0; t < 10; t++) row[14 + t] = out[t];
+        }
+      }
+    }
     th = T / B.x;
     float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
     qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
@@ -862,8 +883,15 @@ void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, c
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
-  hipLaunchKernelGGL(k_phys, faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT, ciMap,
-                     Vout, Q1o, Q2o);
+  MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
+  hipLaunchKernelGGL((k_phys<false>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+                     ciMap, Vout, Q1o, Q2o, none);
+}
+void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                      const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                      const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
+  hipLaunchKernelGGL((k_phys<true>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+                     ciMap, Vout, Q1o, Q2o, dump);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);

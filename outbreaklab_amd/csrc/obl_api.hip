@@ -409,6 +409,12 @@ struct obl_model {
   obl_poisson* solver = nullptr;
   int gemm_mode = OBL_GEMM_FP32, thomas_mode = OBL_THOMAS_MIXED;
   double fp64_cond = 16.0;
+  // microphysics teacher sampling (surrogate training data)
+  float* dump_buf = nullptr;
+  unsigned* dump_count = nullptr;
+  unsigned dump_cap = 0, dump_salt = 0;
+  float dump_pa = 0.f, dump_pc = 0.f;
+  bool dump_on = false;
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -433,7 +439,7 @@ extern "C" void obl_destroy(obl_model* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   void* bufs[] = {h->V[0], h->V[1], h->Vt, h->Q1[0], h->Q1[1], h->Q2[0], h->Q2[1], h->Q1t, h->Q2t, h->lsfA,
-                  h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI};
+                  h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI, h->dump_buf, h->dump_count};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (h->solver) obl_poisson_destroy(h->solver);
@@ -614,8 +620,14 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
   TIMED(T, s, "m.advQ1", launch_advq1(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
   TIMED(T, s, "m.advQ2", launch_advq2(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t, h->Q1[1], h->Q2[1]));
-  TIMED(T, s, "m.phys", launch_phys(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT, h->ciMap, Vc,
-                                    h->Q1[0], h->Q2[0]));
+  if (h->dump_on) {
+    MicroDump md = {h->dump_buf, h->dump_count, h->dump_cap, 0x9e3779b9u * (++h->dump_salt), h->dump_pa, h->dump_pc};
+    TIMED(T, s, "m.phys", launch_phys_dump(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT,
+                                           h->ciMap, Vc, h->Q1[0], h->Q2[0], md));
+  } else {
+    TIMED(T, s, "m.phys", launch_phys(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT, h->ciMap,
+                                      Vc, h->Q1[0], h->Q2[0]));
+  }
   TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
   h->solver->timer.on = T.on;
   if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s))
@@ -725,4 +737,42 @@ extern "C" int obl_profile_read(obl_model* h, char* names, size_t names_len, dou
     names[c] = 0;
   }
   return n;
+}
+
+// ---------------------------------------------------------------- surrogate teacher sampling
+extern "C" int obl_dump_enable(obl_model* h, unsigned capacity, float p_active, float p_clear) {
+  if (!h) return set_global("null handle");
+  if (capacity == 0) {
+    h->dump_on = false;
+    return 0;
+  }
+  if (capacity != h->dump_cap) {
+    if (h->dump_buf) hipFree(h->dump_buf);
+    h->dump_buf = nullptr;
+    if (hip_ok(hipMalloc(&h->dump_buf, (size_t)capacity * OBL_DUMP_STRIDE * 4), &h->err, "hipMalloc(dump)")) return -1;
+    h->dump_cap = capacity;
+  }
+  if (!h->dump_count && hip_ok(hipMalloc(&h->dump_count, sizeof(unsigned)), &h->err, "hipMalloc(dump count)")) return -1;
+  if (hip_ok(hipMemsetAsync(h->dump_count, 0, sizeof(unsigned), h->stream), &h->err, "dump reset")) return -1;
+  h->dump_pa = p_active;
+  h->dump_pc = p_clear;
+  h->dump_on = true;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "dump enable");
+}
+
+extern "C" int obl_dump_read(obl_model* h, float* out, size_t max_rows, size_t* rows, void** dev_buf) {
+  if (!h) return set_global("null handle");
+  if (!h->dump_count) return h->fail("sampling is not enabled");
+  unsigned n = 0;
+  if (hip_ok(hipMemcpyAsync(&n, h->dump_count, sizeof n, hipMemcpyDeviceToHost, h->stream), &h->err, "dump count") ||
+      hip_ok(hipStreamSynchronize(h->stream), &h->err, "dump count"))
+    return -1;
+  size_t m = n < h->dump_cap ? n : h->dump_cap;
+  if (rows) *rows = m;
+  if (dev_buf) *dev_buf = h->dump_buf;
+  if (out && max_rows) {
+    size_t c = m < max_rows ? m : max_rows;
+    if (hip_ok(hipMemcpy(out, h->dump_buf, c * OBL_DUMP_STRIDE * 4, hipMemcpyDeviceToHost), &h->err, "dump read")) return -1;
+  }
+  return 0;
 }

@@ -25,6 +25,21 @@ void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, c
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o);
+// teacher pairs of the microphysics block: rows of OBL_DUMP_STRIDE floats =
+// 14 inputs (T, p0, rho0, qv, qc, qr, qi, qs, qg, Nr, Ni, w, dz, dt),
+// 9 outputs (dT, dqv, dqc, dqi, dqr, dqs, dqg, dNr, dNi), then the level index
+#define OBL_DUMP_STRIDE 24
+struct MicroDump {
+  float* buf;
+  unsigned* count;
+  unsigned cap;
+  unsigned salt;
+  float p_active;
+  float p_clear;
+};
+void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                      const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                      const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump);
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
 

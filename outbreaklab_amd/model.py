@@ -246,6 +246,22 @@ class RegionalModel:
         nm = names.value.decode().split("\n")[:n]
         return {nm[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}
 
+    # ---------------- surrogate teacher sampling (no reference counterpart) ----------------
+    def sample_micro(self, capacity, p_active=1.0, p_clear=0.0):
+        _lib.check(_lib.lib().obl_dump_enable(self.h, int(capacity), float(p_active), float(p_clear)), self.h)
+
+    def micro_samples(self, copy=True):
+        rows = C.c_size_t()
+        dev = C.c_void_p()
+        _lib.check(_lib.lib().obl_dump_read(self.h, None, 0, C.byref(rows), C.byref(dev)), self.h)
+        n = rows.value
+        if not copy:
+            return n, dev.value
+        out = np.empty((n, 24), np.float32)
+        if n:
+            _lib.check(_lib.lib().obl_dump_read(self.h, _ptr(out), n, C.byref(rows), None), self.h)
+        return out
+
     @property
     def Vcur(self):
         return self.V[self.parity]

@@ -43,9 +43,12 @@ def test_solve_fp64_thomas_matches_fp64_oracle(shape):
     ref = oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs.astype(np.float64), "f64")
     e = rel_l2(phi, ref)
     res = rel_l2(apply_operator(phi, dx, dz, rc, rw), rhs)
-    print(f"shape {shape}: rel err {e:.3e}  residual {res:.3e}")
+    # the operator amplifies the rounding of any FP32 phi: bound the residual by that of the FP64
+    # solution rounded to FP32
+    res_floor = rel_l2(apply_operator(ref.astype(np.float32), dx, dz, rc, rw), rhs)
+    print(f"shape {shape}: rel err {e:.3e}  residual {res:.3e} (FP32-rounded exact: {res_floor:.3e})")
     assert e <= 1e-5
-    assert res <= 1e-5
+    assert res <= 3 * res_floor
     ps.destroy()
 
 
@@ -138,6 +141,7 @@ def test_solve_1024_split_bf16():
     phi = ps.solve(rhs.astype(np.float32), gemm="bf16x3", thomas="mixed")
     e = rel_l2(phi, ref)
     res = rel_l2(apply_operator(phi, dx, dz, rc, rw), rhs)
-    print(f"1024 bf16x3 mixed: rel err {e:.3e}, residual {res:.3e}")
+    res_floor = rel_l2(apply_operator(ref.astype(np.float32), dx, dz, rc, rw), rhs)
+    print(f"1024 bf16x3 mixed: rel err {e:.3e}, residual {res:.3e} (FP32-rounded exact: {res_floor:.3e})")
     assert e <= 1e-5
     ps.destroy()
