@@ -9,6 +9,7 @@
 // the trilinear samples.
 #include "obl_device.h"
 #include "obl_kernels.h"
+#include "obl_tile.h"
 
 namespace obl {
 
@@ -217,36 +218,37 @@ __global__ void __launch_bounds__(256) k_bubble(MU u, float4* __restrict__ V0, f
 }
 
 // ====================== NUT (model.js:342-389), outbreak mode ======================
-__device__ __forceinline__ float nut_uc(const float4* V, const float4* BS, const Dims& d, int x, int y, int z) {
-  return 0.5f * (ldVc(V, d, x, y, z, 0) + ldVc(V, d, x + 1, y, z, 0)) - bsB(BS, z, d.nz).z;
+template <class A>
+__device__ __forceinline__ float nut_uc(const A& V, const float4* BS, int nz, int x, int y, int z) {
+  return 0.5f * (V.ldc(x, y, z, 0) + V.ldc(x + 1, y, z, 0)) - bsB(BS, z, nz).z;
 }
-__device__ __forceinline__ float nut_vc(const float4* V, const float4* BS, const Dims& d, int x, int y, int z) {
-  return 0.5f * (ldVc(V, d, x, y, z, 1) + ldVc(V, d, x, y + 1, z, 1)) - bsB(BS, z, d.nz).w;
+template <class A>
+__device__ __forceinline__ float nut_vc(const A& V, const float4* BS, int nz, int x, int y, int z) {
+  return 0.5f * (V.ldc(x, y, z, 1) + V.ldc(x, y + 1, z, 1)) - bsB(BS, z, nz).w;
 }
-__device__ __forceinline__ float nut_wc(const float4* V, const Dims& d, int x, int y, int z) {
-  return 0.5f * (ldVc(V, d, x, y, z, 2) + ldVc(V, d, x, y, z + 1, 2));
+template <class A>
+__device__ __forceinline__ float nut_wc(const A& V, int x, int y, int z) {
+  return 0.5f * (V.ldc(x, y, z, 2) + V.ldc(x, y, z + 1, 2));
 }
-__global__ void __launch_bounds__(256) k_nut(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                             const float4* __restrict__ Q1, float2* __restrict__ aux) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+template <class A>
+__device__ __forceinline__ float2 nut_point(const MU& u, const Dims& d, const float4* BS, const A& V,
+                                            const float4* Q1, int x, int y, int z) {
   float dx = d.dx, dz = d.dz;
-  float4 v0 = ldV(V, d, x, y, z);
-  float dudx = (ldVc(V, d, x + 1, y, z, 0) - v0.x) / dx;
-  float dvdy = (ldVc(V, d, x, y + 1, z, 1) - v0.y) / dx;
-  float dwdz = (ldVc(V, d, x, y, z + 1, 2) - v0.z) / dz;
-  float dudy = (nut_uc(V, BS, d, x, y + 1, z) - nut_uc(V, BS, d, x, y - 1, z)) / (2.0f * dx);
-  float dudz = (nut_uc(V, BS, d, x, y, z + 1) - nut_uc(V, BS, d, x, y, z - 1)) / (2.0f * dz);
-  float dvdx = (nut_vc(V, BS, d, x + 1, y, z) - nut_vc(V, BS, d, x - 1, y, z)) / (2.0f * dx);
-  float dvdz = (nut_vc(V, BS, d, x, y, z + 1) - nut_vc(V, BS, d, x, y, z - 1)) / (2.0f * dz);
-  float dwdx = (nut_wc(V, d, x + 1, y, z) - nut_wc(V, d, x - 1, y, z)) / (2.0f * dx);
-  float dwdy = (nut_wc(V, d, x, y + 1, z) - nut_wc(V, d, x, y - 1, z)) / (2.0f * dx);
+  int nz = d.nz;
+  float4 v0 = V.ld4(x, y, z);
+  float dudx = (V.ldc(x + 1, y, z, 0) - v0.x) / dx;
+  float dvdy = (V.ldc(x, y + 1, z, 1) - v0.y) / dx;
+  float dwdz = (V.ldc(x, y, z + 1, 2) - v0.z) / dz;
+  float dudy = (nut_uc(V, BS, nz, x, y + 1, z) - nut_uc(V, BS, nz, x, y - 1, z)) / (2.0f * dx);
+  float dudz = (nut_uc(V, BS, nz, x, y, z + 1) - nut_uc(V, BS, nz, x, y, z - 1)) / (2.0f * dz);
+  float dvdx = (nut_vc(V, BS, nz, x + 1, y, z) - nut_vc(V, BS, nz, x - 1, y, z)) / (2.0f * dx);
+  float dvdz = (nut_vc(V, BS, nz, x, y, z + 1) - nut_vc(V, BS, nz, x, y, z - 1)) / (2.0f * dz);
+  float dwdx = (nut_wc(V, x + 1, y, z) - nut_wc(V, x - 1, y, z)) / (2.0f * dx);
+  float dwdy = (nut_wc(V, x, y + 1, z) - nut_wc(V, x, y - 1, z)) / (2.0f * dx);
   float s12 = 0.5f * (dudy + dvdx), s13 = 0.5f * (dudz + dwdx), s23 = 0.5f * (dvdz + dwdy);
   float S2 = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + 4.0f * (s12 * s12 + s13 * s13 + s23 * s23);
-  int kz0 = max(z - 1, 0), kz1 = min(z + 1, d.nz - 1);
-  float N2 = 9.81f / bsA(BS, z, d.nz).x * (ldVc(V, d, x, y, kz1, 3) - ldVc(V, d, x, y, kz0, 3)) /
-             ((float)(kz1 - kz0) * dz);
+  int kz0 = max(z - 1, 0), kz1 = min(z + 1, nz - 1);
+  float N2 = 9.81f / bsA(BS, z, nz).x * (V.ldc(x, y, kz1, 3) - V.ldc(x, y, kz0, 3)) / ((float)(kz1 - kz0) * dz);
   float4 q1 = Q1[cidx(d, x, y, z)];
   if (u.v[U_COR].w > 0.5f && q1.y + q1.z > 1e-5f) N2 = fminf(N2, 0.0f);
   float Ri = N2 / fmaxf(S2, 1e-10f);
@@ -255,21 +257,22 @@ __global__ void __launch_bounds__(256) k_nut(MU u, const float4* __restrict__ BS
   float lh = u.v[U_TURB].x * dx, lv = 0.18f * dz;
   float nuh = fminf(lh * lh * sS * fst, 0.08f * dx * dx / fmaxf(d.dt, 0.1f));
   float nuv = fminf(lv * lv * sS * fst, 0.15f * dz * dz / fmaxf(d.dt, 0.1f));
-  aux[cidx(d, x, y, z)] = make_float2(nuh, nuv);
+  return make_float2(nuh, nuv);
 }
 
 // ====================== advection of V (model.js:392-481) ======================
 struct Loc {
   float3 vu, vv, vw, vc;
 };
-__device__ __forceinline__ Loc localVel(const float4* V, const Dims& d, int x, int y, int z) {
-  float4 p = ldV(V, d, x, y, z);
-  float4 xm = ldV(V, d, x - 1, y, z), xp = ldV(V, d, x + 1, y, z);
-  float4 ym = ldV(V, d, x, y - 1, z), yp = ldV(V, d, x, y + 1, z);
-  float4 zm = ldV(V, d, x, y, z - 1), zp = ldV(V, d, x, y, z + 1);
-  float xmyp = ldVc(V, d, x - 1, y + 1, z, 1), xmzp = ldVc(V, d, x - 1, y, z + 1, 2);
-  float xpym = ldVc(V, d, x + 1, y - 1, z, 0), ymzp = ldVc(V, d, x, y - 1, z + 1, 2);
-  float xpzm = ldVc(V, d, x + 1, y, z - 1, 0), ypzm = ldVc(V, d, x, y + 1, z - 1, 1);
+template <class A>
+__device__ __forceinline__ Loc localVel(const A& V, int x, int y, int z) {
+  float4 p = V.ld4(x, y, z);
+  float4 xm = V.ld4(x - 1, y, z), xp = V.ld4(x + 1, y, z);
+  float4 ym = V.ld4(x, y - 1, z), yp = V.ld4(x, y + 1, z);
+  float4 zm = V.ld4(x, y, z - 1), zp = V.ld4(x, y, z + 1);
+  float xmyp = V.ldc(x - 1, y + 1, z, 1), xmzp = V.ldc(x - 1, y, z + 1, 2);
+  float xpym = V.ldc(x + 1, y - 1, z, 0), ymzp = V.ldc(x, y - 1, z + 1, 2);
+  float xpzm = V.ldc(x + 1, y, z - 1, 0), ypzm = V.ldc(x, y + 1, z - 1, 1);
   Loc L;
   L.vu = make_float3(p.x, 0.25f * (xm.y + p.y + xmyp + yp.y), 0.25f * (xm.z + p.z + xmzp + zp.z));
   L.vv = make_float3(0.25f * (ym.x + p.x + xpym + xp.x), p.y, 0.25f * (ym.z + p.z + ymzp + zp.z));
@@ -282,197 +285,153 @@ __device__ __forceinline__ float3 traceV(const Dims& d, float3 k, float3 p, floa
   return make_float3(clampf_(p.x - s * k.x * v0.x, 0.0f, (float)d.nx), clampf_(p.y - s * k.y * v0.y, 0.0f, (float)d.ny),
                      clampf_(p.z - s * k.z * v0.z, 0.0f, (float)d.nz));
 }
-#define VS(T, tx, ty, tz, ch) sample1(T, d.nx + 1, d.ny + 1, d.nz + 1, tx, ty, tz, ch)
-__device__ __forceinline__ float sampU(const float4* T, const Dims& d, float3 p) { return VS(T, p.x, p.y - 0.5f, p.z - 0.5f, 0); }
-__device__ __forceinline__ float sampV(const float4* T, const Dims& d, float3 p) { return VS(T, p.x - 0.5f, p.y, p.z - 0.5f, 1); }
-__device__ __forceinline__ float sampW(const float4* T, const Dims& d, float3 p) { return VS(T, p.x - 0.5f, p.y - 0.5f, p.z, 2); }
-__device__ __forceinline__ float sampC(const float4* T, const Dims& d, float3 p) {
-  return VS(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, 3);
-}
+// texU / texVv / texW / texC (model.js:411-414) in texel coordinates
+template <class A>
+__device__ __forceinline__ float sampU(const A& T, float3 p) { return samp1(T, p.x, p.y - 0.5f, p.z - 0.5f, 0); }
+template <class A>
+__device__ __forceinline__ float sampV(const A& T, float3 p) { return samp1(T, p.x - 0.5f, p.y, p.z - 0.5f, 1); }
+template <class A>
+__device__ __forceinline__ float sampW(const A& T, float3 p) { return samp1(T, p.x - 0.5f, p.y - 0.5f, p.z, 2); }
+template <class A>
+__device__ __forceinline__ float sampC(const A& T, float3 p) { return samp1(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, 3); }
 
-__global__ void __launch_bounds__(256) k_advv1(MU u, const float4* __restrict__ V, float4* __restrict__ Vt) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x > d.nx || y > d.ny || z > d.nz) return;
+template <class A>
+__device__ __forceinline__ float4 advv1_point(const Dims& d, const A& V, int x, int y, int z) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  Loc L = localVel(V, d, x, y, z);
+  Loc L = localVel(V, x, y, z);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   float4 o;
-  o.x = sampU(V, d, traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
-  o.y = sampV(V, d, traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
-  o.z = sampW(V, d, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f));
-  o.w = sampC(V, d, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
-  Vt[vidx(d, x, y, z)] = o;
+  o.x = sampU(V, traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
+  o.y = sampV(V, traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
+  o.z = sampW(V, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f));
+  o.w = sampC(V, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
+  return o;
 }
 
-__device__ __forceinline__ float2 mmV(const float4* T, const Dims& d, float3 p, float ox, float oy, float oz, int ch) {
-  int i0 = (int)floorf(p.x - ox), j0 = (int)floorf(p.y - oy), k0 = (int)floorf(p.z - oz);
-  float lo = 1e30f, hi = -1e30f;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    float s = ldVc(T, d, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
-    lo = fminf(lo, s);
-    hi = fmaxf(hi, s);
-  }
-  return make_float2(lo, hi);
+template <class A>
+__device__ __forceinline__ float2 mmV(const A& T, float3 p, float ox, float oy, float oz, int ch) {
+  return minmax1(T, (int)floorf(p.x - ox), (int)floorf(p.y - oy), (int)floorf(p.z - oz), ch);
 }
-__global__ void __launch_bounds__(256) k_advv2(MU u, const float4* __restrict__ V, const float4* __restrict__ Vt,
-                                               float4* __restrict__ Vout) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x > d.nx || y > d.ny || z > d.nz) return;
+template <class A, class B>
+__device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const A& V, const B& Vt, int x, int y, int z) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  Loc L = localVel(V, d, x, y, z);
+  Loc L = localVel(V, x, y, z);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
   float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
-  size_t i = vidx(d, x, y, z);
-  float4 hat = Vt[i], old = V[i];
+  float4 hat = Vt.ld4(x, y, z), old = V.ld4(x, y, z);
   float4 b;
-  b.x = sampU(Vt, d, traceV(d, k, pu, L.vu, -1.0f));
-  b.y = sampV(Vt, d, traceV(d, k, pv, L.vv, -1.0f));
-  b.z = sampW(Vt, d, traceV(d, k, pw, L.vw, -1.0f));
-  b.w = sampC(Vt, d, traceV(d, k, pc, L.vc, -1.0f));
+  b.x = sampU(Vt, traceV(d, k, pu, L.vu, -1.0f));
+  b.y = sampV(Vt, traceV(d, k, pv, L.vv, -1.0f));
+  b.z = sampW(Vt, traceV(d, k, pw, L.vw, -1.0f));
+  b.w = sampC(Vt, traceV(d, k, pc, L.vc, -1.0f));
   float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
   float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
   float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
                          hat.w + sm * (old.w - b.w));
   float2 m;
-  m = mmV(V, d, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = clampf_(r.x, m.x, m.y);
-  m = mmV(V, d, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = clampf_(r.y, m.x, m.y);
-  m = mmV(V, d, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = clampf_(r.z, m.x, m.y);
-  m = mmV(V, d, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = clampf_(r.w, m.x, m.y);
-  Vout[i] = r;
+  m = mmV(V, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = clampf_(r.x, m.x, m.y);
+  m = mmV(V, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = clampf_(r.y, m.x, m.y);
+  m = mmV(V, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = clampf_(r.z, m.x, m.y);
+  m = mmV(V, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = clampf_(r.w, m.x, m.y);
+  return r;
 }
 
 // ====================== advection of moisture (model.js:482-576) ======================
-__device__ __forceinline__ float3 cellVel(const float4* V, const Dims& d, int x, int y, int z) {
-  float4 p = ldV(V, d, x, y, z);
-  return make_float3(0.5f * (p.x + ldVc(V, d, x + 1, y, z, 0)), 0.5f * (p.y + ldVc(V, d, x, y + 1, z, 1)),
-                     0.5f * (p.z + ldVc(V, d, x, y, z + 1, 2)));
+template <class A>
+__device__ __forceinline__ float3 cellVel(const A& V, int x, int y, int z) {
+  float4 p = V.ld4(x, y, z);
+  return make_float3(0.5f * (p.x + V.ldc(x + 1, y, z, 0)), 0.5f * (p.y + V.ldc(x, y + 1, z, 1)),
+                     0.5f * (p.z + V.ldc(x, y, z + 1, 2)));
 }
 __device__ __forceinline__ float3 traceQ(const Dims& d, float3 k, float3 p, float3 v0, float fall, float s) {
   return make_float3(clampf_(p.x - s * k.x * v0.x, 0.5f, (float)d.nx - 0.5f),
                      clampf_(p.y - s * k.y * v0.y, 0.5f, (float)d.ny - 0.5f),
                      clampf_(p.z - s * k.z * (v0.z - fall), 0.5f, (float)d.nz - 0.5f));
 }
-__device__ __forceinline__ float4 sampQ(const float4* T, const Dims& d, float3 p) {
-  return sample4(T, d.nx, d.ny, d.nz, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f);
-}
-__device__ __forceinline__ float sampQ1(const float4* T, const Dims& d, float3 p, int ch) {
-  return sample1(T, d.nx, d.ny, d.nz, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch);
-}
-__device__ __forceinline__ void mmQ(const float4* T, const Dims& d, float3 p, float4& lo, float4& hi) {
-  int i0 = (int)floorf(p.x - 0.5f), j0 = (int)floorf(p.y - 0.5f), k0 = (int)floorf(p.z - 0.5f);
-  lo = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
-  hi = make_float4(-1e30f, -1e30f, -1e30f, -1e30f);
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    float4 s = ldQ(T, d, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2));
-    lo.x = fminf(lo.x, s.x); lo.y = fminf(lo.y, s.y); lo.z = fminf(lo.z, s.z); lo.w = fminf(lo.w, s.w);
-    hi.x = fmaxf(hi.x, s.x); hi.y = fmaxf(hi.y, s.y); hi.z = fmaxf(hi.z, s.z); hi.w = fmaxf(hi.w, s.w);
-  }
-}
-__device__ __forceinline__ float2 mmQ1(const float4* T, const Dims& d, float3 p, int ch) {
-  int i0 = (int)floorf(p.x - 0.5f), j0 = (int)floorf(p.y - 0.5f), k0 = (int)floorf(p.z - 0.5f);
-  const float* Tf = reinterpret_cast<const float*>(T);
-  float lo = 1e30f, hi = -1e30f;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    float s = Tf[4 * cidx(d, clampi(i0 + (q & 1), 0, d.nx - 1), clampi(j0 + ((q >> 1) & 1), 0, d.ny - 1),
-                         clampi(k0 + (q >> 2), 0, d.nz - 1)) + ch];
-    lo = fminf(lo, s);
-    hi = fmaxf(hi, s);
-  }
-  return make_float2(lo, hi);
+// texS(x) = x / N: texel coordinate x - 1/2
+template <class A>
+__device__ __forceinline__ float4 sampQ(const A& T, float3 p) { return samp4(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f); }
+template <class A>
+__device__ __forceinline__ float sampQ1(const A& T, float3 p, int ch) { return samp1(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch); }
+template <class A>
+__device__ __forceinline__ float2 mmQ1(const A& T, float3 p, int ch) {
+  return minmax1(T, (int)floorf(p.x - 0.5f), (int)floorf(p.y - 0.5f), (int)floorf(p.z - 0.5f), ch);
 }
 
-__global__ void __launch_bounds__(256) k_advq1(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                               const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                               float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+template <class AV, class AQ>
+__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, const AV& V, const AQ& Q1,
+                                            const AQ& Q2, int x, int y, int z, float4& o1, float4& o2) {
   bool thm = u.v[U_MICRO2].z > 0.5f;
   float rsfc = u.v[U_STEP].z;
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float3 v = cellVel(V, d, x, y, z);
+  float3 v = cellVel(V, x, y, z);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  size_t i = cidx(d, x, y, z);
-  float4 q1 = Q1[i], q2 = Q2[i];
+  float4 q1 = Q1.ld4(x, y, z), q2 = Q2.ld4(x, y, z);
   float rho = bsA(BS, z, d.nz).z;
   float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
   float fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
   float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
-  float4 a = sampQ(Q1, d, p0);
-  float r = sampQ1(Q1, d, traceQ(d, k, p, v, fv.x, 1.0f), 3);
-  float4 b = sampQ(Q2, d, p0);
-  float s = sampQ1(Q2, d, traceQ(d, k, p, v, fs, 1.0f), 0);
-  float g = sampQ1(Q2, d, traceQ(d, k, p, v, fg, 1.0f), 1);
-  float nr = b.z;
-  if (thm) nr = sampQ1(Q2, d, traceQ(d, k, p, v, fv.y, 1.0f), 2);
-  Q1t[i] = make_float4(a.x, a.y, a.z, r);
-  Q2t[i] = make_float4(s, g, nr, b.w);
+  float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
+  float a0 = samp1(Q1, tx, ty, tz, 0), a1 = samp1(Q1, tx, ty, tz, 1), a2 = samp1(Q1, tx, ty, tz, 2);
+  float r = sampQ1(Q1, traceQ(d, k, p, v, fv.x, 1.0f), 3);
+  float bw = samp1(Q2, tx, ty, tz, 3);
+  float s = sampQ1(Q2, traceQ(d, k, p, v, fs, 1.0f), 0);
+  float g = sampQ1(Q2, traceQ(d, k, p, v, fg, 1.0f), 1);
+  float nr = thm ? sampQ1(Q2, traceQ(d, k, p, v, fv.y, 1.0f), 2) : samp1(Q2, tx, ty, tz, 2);
+  o1 = make_float4(a0, a1, a2, r);
+  o2 = make_float4(s, g, nr, bw);
 }
 
-__global__ void __launch_bounds__(256) k_advq2(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                               const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                               const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
-                                               float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+template <class AV, class AQ>
+__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, const AV& V, const AQ& Q1,
+                                            const AQ& Q2, const AQ& Q1t, const AQ& Q2t, int x, int y, int z, float4& o1,
+                                            float4& o2) {
   bool thm = u.v[U_MICRO2].z > 0.5f;
   float rsfc = u.v[U_STEP].z;
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float3 v = cellVel(V, d, x, y, z);
+  float3 v = cellVel(V, x, y, z);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  size_t i = cidx(d, x, y, z);
-  float4 q1 = Q1[i], q2 = Q2[i];
+  float4 q1 = Q1.ld4(x, y, z), q2 = Q2.ld4(x, y, z);
   float rho = bsA(BS, z, d.nz).z;
   float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
   float fr = fv.x, fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
-  float4 h1 = Q1t[i], h2 = Q2t[i];
+  float4 h1 = Q1t.ld4(x, y, z), h2 = Q2t.ld4(x, y, z);
   float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
   float fn = thm ? fv.y : 0.0f;
   float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f);
-  float4 b1 = sampQ(Q1t, d, f0);
-  float br = sampQ1(Q1t, d, traceQ(d, k, p, v, fr, -1.0f), 3);
-  float4 b2 = sampQ(Q2t, d, f0);
-  float bs = sampQ1(Q2t, d, traceQ(d, k, p, v, fs, -1.0f), 0);
-  float bg = sampQ1(Q2t, d, traceQ(d, k, p, v, fg, -1.0f), 1);
-  float bn = sampQ1(Q2t, d, traceQ(d, k, p, v, fn, -1.0f), 2);
-  float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - b1.z),
+  float tx = f0.x - 0.5f, ty = f0.y - 0.5f, tz = f0.z - 0.5f;
+  float b1x = samp1(Q1t, tx, ty, tz, 0), b1y = samp1(Q1t, tx, ty, tz, 1), b1z = samp1(Q1t, tx, ty, tz, 2);
+  float br = sampQ1(Q1t, traceQ(d, k, p, v, fr, -1.0f), 3);
+  float b2w = samp1(Q2t, tx, ty, tz, 3);
+  float bs = sampQ1(Q2t, traceQ(d, k, p, v, fs, -1.0f), 0);
+  float bg = sampQ1(Q2t, traceQ(d, k, p, v, fg, -1.0f), 1);
+  float bn = sampQ1(Q2t, traceQ(d, k, p, v, fn, -1.0f), 2);
+  float4 r1 = make_float4(h1.x + sm * (q1.x - b1x), h1.y + sm * (q1.y - b1y), h1.z + sm * (q1.z - b1z),
                           h1.w + sm * (q1.w - br));
   float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
-                          h2.w + sm * (q2.w - b2.w));
+                          h2.w + sm * (q2.w - b2w));
   float3 d0 = traceQ(d, k, p, v, 0.0f, 1.0f);
-  float4 lo, hi;
-  mmQ(Q1, d, d0, lo, hi);
-  r1.x = clampf_(r1.x, lo.x, hi.x);
-  r1.y = clampf_(r1.y, lo.y, hi.y);
-  r1.z = clampf_(r1.z, lo.z, hi.z);
-  float2 m = mmQ1(Q1, d, traceQ(d, k, p, v, fr, 1.0f), 3);
-  r1.w = clampf_(r1.w, m.x, m.y);
-  m = mmQ1(Q2, d, traceQ(d, k, p, v, fs, 1.0f), 0);
-  r2.x = clampf_(r2.x, m.x, m.y);
-  m = mmQ1(Q2, d, traceQ(d, k, p, v, fg, 1.0f), 1);
-  r2.y = clampf_(r2.y, m.x, m.y);
-  m = mmQ1(Q2, d, traceQ(d, k, p, v, fn, 1.0f), 2);
-  r2.z = clampf_(r2.z, m.x, m.y);
-  m = mmQ1(Q2, d, d0, 3);
-  r2.w = clampf_(r2.w, m.x, m.y);
-  Q1o[i] = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
-  Q2o[i] = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
+  float2 m;
+  m = mmQ1(Q1, d0, 0); r1.x = clampf_(r1.x, m.x, m.y);
+  m = mmQ1(Q1, d0, 1); r1.y = clampf_(r1.y, m.x, m.y);
+  m = mmQ1(Q1, d0, 2); r1.z = clampf_(r1.z, m.x, m.y);
+  m = mmQ1(Q1, traceQ(d, k, p, v, fr, 1.0f), 3); r1.w = clampf_(r1.w, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, fs, 1.0f), 0); r2.x = clampf_(r2.x, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, fg, 1.0f), 1); r2.y = clampf_(r2.y, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, fn, 1.0f), 2); r2.z = clampf_(r2.z, m.x, m.y);
+  m = mmQ1(Q2, d0, 3); r2.w = clampf_(r2.w, m.x, m.y);
+  o1 = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
+  o2 = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
 }
 
 // ====================== forces + microphysics (model.js:579-907) ======================
-__device__ __forceinline__ float lapH(const float4* V, const Dims& d, int x, int y, int z, int ch, float me) {
-  return ldVc(V, d, x + 1, y, z, ch) + ldVc(V, d, x - 1, y, z, ch) + ldVc(V, d, x, y + 1, z, ch) +
-         ldVc(V, d, x, y - 1, z, ch) - 4.0f * me;
+template <class A>
+__device__ __forceinline__ float lapH(const A& V, int x, int y, int z, int ch, float me) {
+  return V.ldc(x + 1, y, z, ch) + V.ldc(x - 1, y, z, ch) + V.ldc(x, y + 1, z, ch) + V.ldc(x, y - 1, z, ch) - 4.0f * me;
 }
 __device__ __forceinline__ float bsV(const float4* BS, const Dims& d, int k, int ch) {
   int kk = clampi(k, 0, d.nz - 1);
@@ -481,11 +440,12 @@ __device__ __forceinline__ float bsV(const float4* BS, const Dims& d, int k, int
   if (ch == 3) return bsA(BS, kk, d.nz).x;
   return 0.0f;
 }
-__device__ __forceinline__ float lapZp(const float4* V, const float4* BS, const Dims& d, int x, int y, int z, int ch,
-                                       float me, bool lo, bool hi) {
+template <class A>
+__device__ __forceinline__ float lapZp(const A& V, const float4* BS, const Dims& d, int x, int y, int z, int ch, float me,
+                                       bool lo, bool hi) {
   float m0 = me - bsV(BS, d, z, ch);
-  float a = lo ? m0 : ldVc(V, d, x, y, z - 1, ch) - bsV(BS, d, z - 1, ch);
-  float b = hi ? m0 : ldVc(V, d, x, y, z + 1, ch) - bsV(BS, d, z + 1, ch);
+  float a = lo ? m0 : V.ldc(x, y, z - 1, ch) - bsV(BS, d, z - 1, ch);
+  float b = hi ? m0 : V.ldc(x, y, z + 1, ch) - bsV(BS, d, z + 1, ch);
   return a + b - 2.0f * m0;
 }
 __device__ __forceinline__ float spongeLat(const MU& u, const Dims& d, float xm, float ym) {
@@ -500,11 +460,12 @@ __device__ __forceinline__ float spongeTop(const MU& u, const Dims& d, float z) 
   float s = sinf(1.5707963f * (z - u.v[U_SPONGE].z) / (ztop - u.v[U_SPONGE].z));
   return s * s / u.v[U_SPONGE].w;
 }
-__device__ __forceinline__ float buoy(const float4* V, const float4* Q1, const float4* Q2, const float4* BS,
-                                      const Dims& d, int x, int y, int z) {
+template <class AV, class AQ>
+__device__ __forceinline__ float buoy(const AV& V, const AQ& Q1, const float4* Q2, const float4* BS, const Dims& d, int x,
+                                      int y, int z) {
   float4 A = bsA(BS, z, d.nz);
-  float th = ldVc(V, d, x, y, z, 3);
-  float4 q1 = ldQ(Q1, d, x, y, z), q2 = ldQ(Q2, d, x, y, z);
+  float th = V.ldc(x, y, z, 3);
+  float4 q1 = Q1.ld4(x, y, z), q2 = ldQ(Q2, d, x, y, z);
   return 9.81f * ((th - A.x) / A.x + 0.608f * (q1.x - A.y) - (q1.y + q1.z + q1.w + q2.x + q2.y));
 }
 __device__ __forceinline__ float2 ldA(const float2* aux, const Dims& d, int x, int y, int z) {
@@ -645,16 +606,13 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   T += OBL_LFC / OBL_CPD * (qi - qi0 - f * dsum);
 }
 
-template <bool DUMP>
-__global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
-                                              const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
-                                              const float2* __restrict__ aux, const float4* __restrict__ lsfA,
-                                              const float4* __restrict__ lsfB, const float* __restrict__ phT,
-                                              const float4* __restrict__ ciMap, float4* __restrict__ Vout,
-                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
-  Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-  if (x > d.nx || y > d.ny || z > d.nz) return;
+template <bool DUMP, class AV, class AQ>
+__device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const float4* __restrict__ BS, const AV& Vin,
+                                           const AQ& Q1in, const float4* __restrict__ Q2in, const float2* __restrict__ aux,
+                                           const float4* __restrict__ lsfA, const float4* __restrict__ lsfB,
+                                           const float* __restrict__ phT, const float4* __restrict__ ciMap,
+                                           float4* __restrict__ Vout, float4* __restrict__ Q1o, float4* __restrict__ Q2o,
+                                           const MicroDump& dump, int x, int y, int z) {
   int nx = d.nx, ny = d.ny, nz = d.nz;
   float dx = d.dx, dz = d.dz, dt = d.dt;
   float ix2 = 1.0f / (dx * dx), iz2 = 1.0f / (dz * dz);
@@ -663,15 +621,15 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
   bool thm = u.v[U_MICRO2].z > 0.5f;
   bool drag = u.v[U_TURB].w > 0.5f;
   size_t vi = vidx(d, x, y, z);
-  float4 V = Vin[vi], o = V;
+  float4 V = Vin.ld4(x, y, z), o = V;
   // ---- u ----
   if (y < ny && z < nz) {
     float xmx = (float)x * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
     float2 a0 = ldA(aux, d, x - 1, y, z), a1 = ldA(aux, d, x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float du = nux * lapH(Vin, d, x, y, z, 0, V.x) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 0, V.x, z == 0, z == nz - 1) * iz2;
+    float du = nux * lapH(Vin, x, y, z, 0, V.x) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 0, V.x, z == 0, z == nz - 1) * iz2;
     float ut = tgtU(BS, lsfB, d, x, y, z);
-    float vbar = 0.25f * (ldVc(Vin, d, x - 1, y, z, 1) + V.y + ldVc(Vin, d, x - 1, y + 1, z, 1) + ldVc(Vin, d, x, y + 1, z, 1));
+    float vbar = 0.25f * (Vin.ldc(x - 1, y, z, 1) + V.y + Vin.ldc(x - 1, y + 1, z, 1) + Vin.ldc(x, y + 1, z, 1));
     float vtb = 0.5f * (tgtV(BS, lsfB, d, x - 1, y, z) + tgtV(BS, lsfB, d, x, y + 1, z));
     float nw = 0.5f * (ld2d(lsfB, d, x - 1, y).w + ld2d(lsfB, d, x, y).w);
     du += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (ut - V.x);
@@ -690,9 +648,9 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
     float xmx = ((float)x + 0.5f) * dx, xmy = (float)y * dx, zz = ((float)z + 0.5f) * dz;
     float2 a0 = ldA(aux, d, x, y - 1, z), a1 = ldA(aux, d, x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float dv = nux * lapH(Vin, d, x, y, z, 1, V.y) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 1, V.y, z == 0, z == nz - 1) * iz2;
+    float dv = nux * lapH(Vin, x, y, z, 1, V.y) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 1, V.y, z == 0, z == nz - 1) * iz2;
     float vt = tgtV(BS, lsfB, d, x, y, z);
-    float ubar = 0.25f * (ldVc(Vin, d, x, y - 1, z, 0) + V.x + ldVc(Vin, d, x + 1, y - 1, z, 0) + ldVc(Vin, d, x + 1, y, z, 0));
+    float ubar = 0.25f * (Vin.ldc(x, y - 1, z, 0) + V.x + Vin.ldc(x + 1, y - 1, z, 0) + Vin.ldc(x + 1, y, z, 0));
     float utb = 0.5f * (tgtU(BS, lsfB, d, x, y - 1, z) + tgtU(BS, lsfB, d, x + 1, y, z));
     float nw = 0.5f * (ld2d(lsfB, d, x, y - 1).w + ld2d(lsfB, d, x, y).w);
     dv += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (vt - V.y);
@@ -711,8 +669,8 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
     float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = (float)z * dz;
     float2 a0 = ldA(aux, d, x, y, z - 1), a1 = ldA(aux, d, x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float lz = ldVc(Vin, d, x, y, z - 1, 2) + ldVc(Vin, d, x, y, z + 1, 2) - 2.0f * V.z;
-    float dw = nux * lapH(Vin, d, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
+    float lz = Vin.ldc(x, y, z - 1, 2) + Vin.ldc(x, y, z + 1, 2) - 2.0f * V.z;
+    float dw = nux * lapH(Vin, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
     dw += 0.5f * (buoy(Vin, Q1in, Q2in, BS, d, x, y, z - 1) + buoy(Vin, Q1in, Q2in, BS, d, x, y, z)) -
           (ldP(phT, d, x, y, z) - ldP(phT, d, x, y, z - 1)) / dz;
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
@@ -739,15 +697,15 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
     float relax = sl + nw * nudR;
     float2 a1 = ldA(aux, d, x, y, z);
     float nux = a1.x * 3.0f + nu0, nuy = a1.y * 3.0f + nu0;
-    float th = V.w + dt * (nux * lapH(Vin, d, x, y, z, 3, V.w) * ix2 +
+    float th = V.w + dt * (nux * lapH(Vin, x, y, z, 3, V.w) * ix2 +
                            nuy * lapZp(Vin, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
                            (relax + spongeTop(u, d, zz)) * (tth - V.w));
     size_t ci_ = cidx(d, x, y, z);
-    float4 q1 = Q1in[ci_], q2 = Q2in[ci_];
-    float4 e1 = ldQ(Q1in, d, x + 1, y, z), w1 = ldQ(Q1in, d, x - 1, y, z);
-    float4 n1 = ldQ(Q1in, d, x, y + 1, z), s1 = ldQ(Q1in, d, x, y - 1, z);
-    float4 dn = z == 0 ? q1 : ldQ(Q1in, d, x, y, z - 1);
-    float4 up = z == nz - 1 ? q1 : ldQ(Q1in, d, x, y, z + 1);
+    float4 q1 = Q1in.ld4(x, y, z), q2 = Q2in[ci_];
+    float4 e1 = Q1in.ld4(x + 1, y, z), w1 = Q1in.ld4(x - 1, y, z);
+    float4 n1 = Q1in.ld4(x, y + 1, z), s1 = Q1in.ld4(x, y - 1, z);
+    float4 dn = z == 0 ? q1 : Q1in.ld4(x, y, z - 1);
+    float4 up = z == nz - 1 ? q1 : Q1in.ld4(x, y, z + 1);
     float l1x = e1.x + w1.x + n1.x + s1.x - 4.0f * q1.x, l1y = e1.y + w1.y + n1.y + s1.y - 4.0f * q1.y;
     float l1zz = e1.z + w1.z + n1.z + s1.z - 4.0f * q1.z;
     float lzx = dn.x + up.x - 2.0f * q1.x, lzy = dn.y + up.y - 2.0f * q1.y, lzz = dn.z + up.z - 2.0f * q1.z;
@@ -771,7 +729,7 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
       if (r < (active ? dump.p_active : dump.p_clear)) {
         unsigned slot = atomicAdd(dump.count, 1u);
         if (slot < dump.cap) {
-          float wc = 0.5f * (V.z + ldVc(Vin, d, x, y, z + 1, 2));
+          float wc = 0.5f * (V.z + Vin.ldc(x, y, z + 1, 2));
           float* row = dump.buf + (size_t)slot * OBL_DUMP_STRIDE;
           float in[14] = {T0, B.y, A.z, qv0, qc0, qr0, qi0, qs0, qg0, nr0, 0.0f, wc, dz, dt};
           float out[10] = {T - T0, qv - qv0, qc - qc0, qi - qi0, qr - qr0, qs - qs0, qg - qg0, nr - nr0, 0.0f, (float)z};
@@ -800,6 +758,165 @@ __global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ B
     o = make_float4(Bz.z, Bz.w, 0.0f, bsA(BS, z, nz).x);
   }
   Vout[vi] = o;
+}
+
+
+// ====================== z-marching drivers ======================
+// One block per TX x TY column of points, walking z upward; every field a kernel gathers
+// from is held as a three-plane LDS ring (obl_tile.h), the next plane prefetched into
+// registers while the current one is computed.
+#define TILE_SETUP                                          \
+  Dims d = dims(u);                                         \
+  const int tid = threadIdx.x;                              \
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;     \
+  const int x = x0 + tid % TX, y = y0 + tid / TX;
+
+__global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                        const float4* __restrict__ Q1, float2* __restrict__ aux) {
+  __shared__ float lds[RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  rv.load(-1, tid); rv.load(0, tid); rv.load(1, tid);
+  __syncthreads();
+  const bool act = x < d.nx && y < d.ny;
+  for (int z = 0; z < d.nz; z++) {
+    Ring::Pre pv;
+    const bool more = z + 1 < d.nz;
+    if (more) rv.fetch(z + 2, tid, pv);
+    rv.zc = z;
+    if (act) aux[cidx(d, x, y, z)] = nut_point(u, d, BS, rv, Q1, x, y, z);
+    __syncthreads();
+    if (more) rv.store(z + 2, tid, pv);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_advv1_t(MU u, const float4* __restrict__ V, float4* __restrict__ Vt) {
+  __shared__ float lds[RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  rv.load(-1, tid); rv.load(0, tid); rv.load(1, tid);
+  __syncthreads();
+  const bool act = x <= d.nx && y <= d.ny;
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv;
+    const bool more = z < d.nz;
+    if (more) rv.fetch(z + 2, tid, pv);
+    rv.zc = z;
+    if (act) Vt[vidx(d, x, y, z)] = advv1_point(d, rv, x, y, z);
+    __syncthreads();
+    if (more) rv.store(z + 2, tid, pv);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_advv2_t(MU u, const float4* __restrict__ V,
+                                                          const float4* __restrict__ Vt, float4* __restrict__ Vout) {
+  __shared__ float lds[2 * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rt{lds + RING_FLOATS, Vt, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) { rv.load(k, tid); rt.load(k, tid); }
+  __syncthreads();
+  const bool act = x <= d.nx && y <= d.ny;
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv, pt;
+    const bool more = z < d.nz;
+    if (more) { rv.fetch(z + 2, tid, pv); rt.fetch(z + 2, tid, pt); }
+    rv.zc = z; rt.zc = z;
+    if (act) Vout[vidx(d, x, y, z)] = advv2_point(u, d, rv, rt, x, y, z);
+    __syncthreads();
+    if (more) { rv.store(z + 2, tid, pv); rt.store(z + 2, tid, pt); }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_advq1_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                          float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
+  __shared__ float lds[2 * RING_FLOATS];
+  TILE_SETUP
+  Ring r1{lds, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r2{lds + RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
+  GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); }
+  __syncthreads();
+  const bool act = x < d.nx && y < d.ny;
+  for (int z = 0; z < d.nz; z++) {
+    Ring::Pre p1, p2;
+    const bool more = z + 1 < d.nz;
+    if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); }
+    r1.zc = z; r2.zc = z;
+    if (act) {
+      float4 o1, o2;
+      advq1_point(u, d, BS, gv, r1, r2, x, y, z, o1, o2);
+      size_t i = cidx(d, x, y, z);
+      Q1t[i] = o1;
+      Q2t[i] = o2;
+    }
+    __syncthreads();
+    if (more) { r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                          const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
+                                                          float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  __shared__ float lds[4 * RING_FLOATS];
+  TILE_SETUP
+  Ring r1{lds, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r2{lds + RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r3{lds + 2 * RING_FLOATS, Q1t, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r4{lds + 3 * RING_FLOATS, Q2t, d.nx, d.ny, d.nz, x0, y0, 0};
+  GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); r3.load(k, tid); r4.load(k, tid); }
+  __syncthreads();
+  const bool act = x < d.nx && y < d.ny;
+  for (int z = 0; z < d.nz; z++) {
+    Ring::Pre p1, p2, p3, p4;
+    const bool more = z + 1 < d.nz;
+    if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4); }
+    r1.zc = z; r2.zc = z; r3.zc = z; r4.zc = z;
+    if (act) {
+      float4 o1, o2;
+      advq2_point(u, d, BS, gv, r1, r2, r3, r4, x, y, z, o1, o2);
+      size_t i = cidx(d, x, y, z);
+      Q1o[i] = o1;
+      Q2o[i] = o2;
+    }
+    __syncthreads();
+    if (more) { r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4); }
+    __syncthreads();
+  }
+}
+
+template <bool DUMP>
+__global__ void __launch_bounds__(TILE_THREADS) k_phys_t(MU u, const float4* __restrict__ BS,
+                                                         const float4* __restrict__ Vin, const float4* __restrict__ Q1in,
+                                                         const float4* __restrict__ Q2in, const float2* __restrict__ aux,
+                                                         const float4* __restrict__ lsfA, const float4* __restrict__ lsfB,
+                                                         const float* __restrict__ phT, const float4* __restrict__ ciMap,
+                                                         float4* __restrict__ Vout, float4* __restrict__ Q1o,
+                                                         float4* __restrict__ Q2o, MicroDump dump) {
+  __shared__ float lds[2 * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, Vin, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rq{lds + RING_FLOATS, Q1in, d.nx, d.ny, d.nz, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) { rv.load(k, tid); rq.load(k, tid); }
+  __syncthreads();
+  const bool act = x <= d.nx && y <= d.ny;
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv, pq;
+    const bool more = z < d.nz;
+    if (more) { rv.fetch(z + 2, tid, pv); rq.fetch(z + 2, tid, pq); }
+    rv.zc = z; rq.zc = z;
+    if (act) phys_point<DUMP>(u, d, BS, rv, rq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
+    __syncthreads();
+    if (more) { rv.store(z + 2, tid, pv); rq.store(z + 2, tid, pq); }
+    __syncthreads();
+  }
 }
 
 // ====================== DIV (model.js:910-925) ======================
@@ -862,36 +979,38 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
                    double rh, double rz, double dth) {
   hipLaunchKernelGGL(k_bubble, faces_grid(d), dim3(64, 4, 1), 0, s, u, V0, V1, x, y, z, rh, rz, dth);
 }
+static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
+static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 float2* aux) {
-  hipLaunchKernelGGL(k_nut, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, aux);
+  hipLaunchKernelGGL(k_nut_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, aux);
 }
 void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt) {
-  hipLaunchKernelGGL(k_advv1, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vt);
+  hipLaunchKernelGGL(k_advv1_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt);
 }
 void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout) {
-  hipLaunchKernelGGL(k_advv2, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vt, Vout);
+  hipLaunchKernelGGL(k_advv2_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt, Vout);
 }
 void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                   const float4* Q2, float4* Q1t, float4* Q2t) {
-  hipLaunchKernelGGL(k_advq1, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
+  hipLaunchKernelGGL(k_advq1_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
 }
 void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                   const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o) {
-  hipLaunchKernelGGL(k_advq2, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
+  hipLaunchKernelGGL(k_advq2_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
 }
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys<false>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
-                     ciMap, Vout, Q1o, Q2o, none);
+  hipLaunchKernelGGL((k_phys_t<false>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB,
+                     phT, ciMap, Vout, Q1o, Q2o, none);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
-  hipLaunchKernelGGL((k_phys<true>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
-                     ciMap, Vout, Q1o, Q2o, dump);
+  hipLaunchKernelGGL((k_phys_t<true>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB,
+                     phT, ciMap, Vout, Q1o, Q2o, dump);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);

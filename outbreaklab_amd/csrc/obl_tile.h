@@ -1,0 +1,173 @@
+// LDS plane rings for the z-marching stencil kernels (gfx950).
+//
+// A block owns a TX x TY column of points and walks it from the ground to the
+// lid. For every field it reads, the block keeps three z-planes (k-1, k, k+1)
+// of a (TX+2) x (TY+2) tile -- its points plus a one-point halo -- in LDS,
+// stored channel by channel (SoA) so that a wave's per-channel gathers hit
+// distinct banks. Each plane is fetched from HBM once per block as whole
+// float4 texels, with the reference's clamp-to-edge addressing applied at load
+// time: a tile slot outside the field holds the clamped texel, which is what
+// textureLoad(clamp(c)) and the clamp-to-edge sampler return.
+//
+// Reads inside the window come from LDS; anything outside it (a departure
+// point more than one cell away, or a far neighbour) falls back to the same
+// clamped read from global memory, so results never depend on the window.
+#pragma once
+#include "obl_device.h"
+
+namespace obl {
+
+constexpr int TX = 32, TY = 8, TPW = TX + 2, TPH = TY + 2, TPS = TPW * TPH;
+constexpr int TILE_THREADS = TX * TY;
+constexpr int RING_FLOATS = 3 * 4 * TPS;  // 3 planes x 4 channels
+
+struct Ring {
+  float* s;          // LDS, [3 planes][4 channels][TPH][TPW]
+  const float4* g;   // the field in HBM
+  int W, H, D;       // texel extents: W x H x D
+  int x0, y0, zc;    // tile origin (first interior point) and the centre plane
+
+  __device__ __forceinline__ static int slot(int k) { return (k + 3 * 4096) % 3; }
+  __device__ __forceinline__ bool in(int i, int j, int k) const {
+    return (unsigned)(i - x0 + 1) < (unsigned)TPW && (unsigned)(j - y0 + 1) < (unsigned)TPH &&
+           (unsigned)(k - zc + 1) < 3u;
+  }
+  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
+    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  }
+  __device__ __forceinline__ int sidx(int i, int j, int k, int ch) const {
+    return (slot(k) * 4 + ch) * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
+    if (in(i, j, k)) return s[sidx(i, j, k, ch)];
+    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
+  }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
+    if (in(i, j, k)) {
+      int b = sidx(i, j, k, 0);
+      return make_float4(s[b], s[b + TPS], s[b + 2 * TPS], s[b + 3 * TPS]);
+    }
+    return g[gidx(i, j, k)];
+  }
+  // software-pipelined plane load: fetch plane k (clamped) into registers early, store it to
+  // its slot after the compute that still reads the slot's old plane
+  struct Pre {
+    float4 v[(TPS + TILE_THREADS - 1) / TILE_THREADS];
+  };
+  __device__ __forceinline__ void fetch(int k, int tid, Pre& r) const {
+    int kk = clampi(k, 0, D - 1);
+#pragma unroll
+    for (int q = 0; q < (TPS + TILE_THREADS - 1) / TILE_THREADS; q++) {
+      int t = tid + q * TILE_THREADS;
+      if (t < TPS) {
+        int ti = t % TPW, tj = t / TPW;
+        r.v[q] = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
+                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
+      }
+    }
+  }
+  __device__ __forceinline__ void store(int k, int tid, const Pre& r) {
+    int sl = slot(k);
+#pragma unroll
+    for (int q = 0; q < (TPS + TILE_THREADS - 1) / TILE_THREADS; q++) {
+      int t = tid + q * TILE_THREADS;
+      if (t < TPS) {
+        float* p = s + sl * 4 * TPS + t;
+        p[0] = r.v[q].x;
+        p[TPS] = r.v[q].y;
+        p[2 * TPS] = r.v[q].z;
+        p[3 * TPS] = r.v[q].w;
+      }
+    }
+  }
+  // cooperative load of plane k (clamped) into its slot
+  __device__ __forceinline__ void load(int k, int tid) {
+    int kk = clampi(k, 0, D - 1);
+    int sl = slot(k);
+    for (int t = tid; t < TPS; t += TILE_THREADS) {
+      int ti = t % TPW, tj = t / TPW;
+      float4 v = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
+                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
+      float* p = s + sl * 4 * TPS + t;
+      p[0] = v.x;
+      p[TPS] = v.y;
+      p[2 * TPS] = v.z;
+      p[3 * TPS] = v.w;
+    }
+  }
+};
+
+// The same interface over HBM only (clamped reads, no window): for fields a kernel reads at a
+// few points, and for the fallback path.
+struct GAcc {
+  const float4* g;
+  int W, H, D;
+  float* s;
+  __device__ __forceinline__ bool in(int, int, int) const { return false; }
+  __device__ __forceinline__ int sidx(int, int, int, int) const { return 0; }
+  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
+    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
+    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
+  }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return g[gidx(i, j, k)]; }
+};
+
+// Trilinear, clamp-to-edge sample of one channel at texel coordinates (t - 1/2 convention
+// applied by the caller); x, then y, then z lerps as a + t (b - a).
+template <class A>
+__device__ __forceinline__ float samp1(const A& a, float tx, float ty, float tz, int ch) {
+  float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+  float ax = tx - fx, ay = ty - fy, az = tz - fz;
+  int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+  if (a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1)) {
+    int b = a.sidx(i0, j0, k0, ch);
+    int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
+    c000 = a.s[b]; c100 = a.s[b + 1]; c010 = a.s[b + TPW]; c110 = a.s[b + TPW + 1];
+    c001 = a.s[b + dz]; c101 = a.s[b + dz + 1]; c011 = a.s[b + dz + TPW]; c111 = a.s[b + dz + TPW + 1];
+  } else {
+    const float* g = reinterpret_cast<const float*>(a.g);
+    c000 = g[4 * a.gidx(i0, j0, k0) + ch]; c100 = g[4 * a.gidx(i0 + 1, j0, k0) + ch];
+    c010 = g[4 * a.gidx(i0, j0 + 1, k0) + ch]; c110 = g[4 * a.gidx(i0 + 1, j0 + 1, k0) + ch];
+    c001 = g[4 * a.gidx(i0, j0, k0 + 1) + ch]; c101 = g[4 * a.gidx(i0 + 1, j0, k0 + 1) + ch];
+    c011 = g[4 * a.gidx(i0, j0 + 1, k0 + 1) + ch]; c111 = g[4 * a.gidx(i0 + 1, j0 + 1, k0 + 1) + ch];
+  }
+  float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
+  float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+
+template <class A>
+__device__ __forceinline__ float4 samp4(const A& a, float tx, float ty, float tz) {
+  return make_float4(samp1(a, tx, ty, tz, 0), samp1(a, tx, ty, tz, 1), samp1(a, tx, ty, tz, 2),
+                     samp1(a, tx, ty, tz, 3));
+}
+
+// min / max of one channel over the 8 texels around a point (the MacCormack limiter)
+template <class A>
+__device__ __forceinline__ float2 minmax1(const A& a, int i0, int j0, int k0, int ch) {
+  float lo = 1e30f, hi = -1e30f;
+  if (a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1)) {
+    int b = a.sidx(i0, j0, k0, ch);
+    int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
+    const int off[8] = {0, 1, TPW, TPW + 1, dz, dz + 1, dz + TPW, dz + TPW + 1};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float s = a.s[b + off[q]];
+      lo = fminf(lo, s);
+      hi = fmaxf(hi, s);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float s = reinterpret_cast<const float*>(a.g)[4 * a.gidx(i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2)) + ch];
+      lo = fminf(lo, s);
+      hi = fmaxf(hi, s);
+    }
+  }
+  return make_float2(lo, hi);
+}
+
+}  // namespace obl
