@@ -143,6 +143,28 @@ int obl_dump_enable(obl_model* h, unsigned capacity, float p_active, float p_cle
  * returns the device buffer in *dev_buf (may be NULL) */
 int obl_dump_read(obl_model* h, float* out, size_t max_rows, size_t* rows, void** dev_buf);
 
+/* ---------------- microphysics surrogate network ----------------
+ * No reference counterpart. A 16 -> 128 -> 128 -> 128 -> 16 MLP (GELU) whose 9 used outputs are
+ * the tendencies of the microphysics block, scaled by out_scale; inputs are the 14 values of a
+ * sampled row (obl_dump_*), mapped to features on the device. FP32 master parameters, Adam. */
+typedef struct obl_mlp obl_mlp;
+int obl_mlp_create(obl_mlp** out);
+void obl_mlp_destroy(obl_mlp* m);
+const char* obl_mlp_last_error(const obl_mlp* m);
+int obl_mlp_nparams(void);
+int obl_mlp_set_params(obl_mlp* m, const float* p, int n);
+int obl_mlp_get_params(obl_mlp* m, float* p, int n);
+/* out_scale[9]: tendency scales (targets are tendency / scale); in_lo/in_hi[14]: the training
+ * envelope (NULL = unbounded); points outside it use the plain scheme at inference */
+int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const float* in_lo, const float* in_hi);
+/* `steps` Adam steps on batches of `batch` rows drawn (hashed with `seed`) from nrows device rows;
+ * quant != 0 rounds weights, features and activations to BF16 in the forward pass as inference does */
+int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, int stride, int steps, int batch, float lr,
+                  unsigned seed, int quant, float* mean_loss);
+/* exact evaluation of n host rows: mean loss, and optionally the gradient of the mean loss
+ * (obl_mlp_nparams floats) and the raw outputs (n x 16) */
+int obl_mlp_eval(obl_mlp* m, const float* rows_host, int n, int stride, int quant, float* loss, float* grads, float* y);
+
 /* ---------------- PoissonDirect ---------------- */
 /* constructor (solver.js:106-137): nx, ny even; dz, dx in metres */
 int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, obl_poisson** out);

@@ -46,6 +46,17 @@ _SIGS = {
     "obl_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int]),
     "obl_dump_enable": (C.c_int, [C.c_void_p, C.c_uint, C.c_float, C.c_float]),
     "obl_dump_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]),
+    "obl_mlp_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "obl_mlp_destroy": (None, [C.c_void_p]),
+    "obl_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "obl_mlp_nparams": (C.c_int, []),
+    "obl_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_mlp_set_scales": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "obl_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint,
+                                C.c_int, C.POINTER(C.c_float)]),
+    "obl_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_void_p,
+                               C.c_void_p]),
     "obl_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     "obl_poisson_destroy": (None, [C.c_void_p]),
     "obl_poisson_set_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -81,11 +92,13 @@ def symbols():
     return list(_SIGS)
 
 
-def check(rc: int, handle=None, poisson=False):
+def check(rc: int, handle=None, poisson=False, mlp=False):
     if rc == 0:
         return
     L = lib()
-    if handle is None:
+    if mlp:
+        msg = L.obl_mlp_last_error(handle)
+    elif handle is None:
         msg = L.obl_global_error()
     elif poisson:
         msg = L.obl_poisson_last_error(handle)

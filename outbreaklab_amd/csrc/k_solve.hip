@@ -191,6 +191,19 @@ __device__ __forceinline__ void split8(const float* v, uint4& hi, uint4& lo) {
   lo = make_uint4(l[0], l[1], l[2], l[3]);
 }
 
+__device__ __forceinline__ void split4(const float* v, uint2& hi, uint2& lo) {
+  uint32_t h[2], l[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    short h0 = bf16_rne(v[2 * i]), h1 = bf16_rne(v[2 * i + 1]);
+    short l0 = bf16_rne(v[2 * i] - bf16_to_f(h0)), l1 = bf16_rne(v[2 * i + 1] - bf16_to_f(h1));
+    h[i] = (uint32_t)(uint16_t)h0 | ((uint32_t)(uint16_t)h1 << 16);
+    l[i] = (uint32_t)(uint16_t)l0 | ((uint32_t)(uint16_t)l1 << 16);
+  }
+  hi = make_uint2(h[0], h[1]);
+  lo = make_uint2(l[0], l[1]);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
   // [parity][hi/lo][row][k]
@@ -267,37 +280,40 @@ __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
         *reinterpret_cast<uint4*>(&As[1][1][row][16 * half + 8 * c]) = l;
       }
     } else {
-      int n = tid & 127, kg = tid >> 7;
-      int gn = n0 + n;
-      float s0[16], s1[16];
+      // 4 n x 4 k per thread: float4 rows in, packed 4 x bf16 (8 B) stores along k out
+      int ng = tid & 31, kq = tid >> 5;
+      int gn = n0 + 4 * ng;
       const float* pl = a.in + (size_t)plane * planeSz;
+      float s0[4][4], s1[4][4];  // [k][n]
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        int k = k0 + 16 * kg + i;
-        float v0 = 0.f, v1 = 0.f;
+      for (int kk = 0; kk < 4; kk++) {
+        int k = k0 + 4 * kq + kk;
+        float4 f0 = make_float4(0.f, 0.f, 0.f, 0.f), f1 = f0;
         if (k < Nh && gn < a.inner) {
-          float f = pl[(size_t)k * a.inner + gn];
+          float4 f = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + gn);
           if (MODE == DST_YF) {
-            float m = pl[(size_t)(N - 1 - k) * a.inner + gn];
-            v0 = f + m;
-            v1 = f - m;
+            float4 m = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + gn);
+            f0 = make_float4(f.x + m.x, f.y + m.y, f.z + m.z, f.w + m.w);
+            f1 = make_float4(f.x - m.x, f.y - m.y, f.z - m.z, f.w - m.w);
           } else {
-            v0 = f;
-            v1 = pl[(size_t)(Nh + k) * a.inner + gn];
+            f0 = f;
+            f1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + gn);
           }
         }
-        s0[i] = v0;
-        s1[i] = v1;
+        s0[kk][0] = f0.x; s0[kk][1] = f0.y; s0[kk][2] = f0.z; s0[kk][3] = f0.w;
+        s1[kk][0] = f1.x; s1[kk][1] = f1.y; s1[kk][2] = f1.z; s1[kk][3] = f1.w;
       }
-      uint4 h, l;
 #pragma unroll
-      for (int c = 0; c < 2; c++) {
-        split8(s0 + 8 * c, h, l);
-        *reinterpret_cast<uint4*>(&Bs[0][0][n][16 * kg + 8 * c]) = h;
-        *reinterpret_cast<uint4*>(&Bs[0][1][n][16 * kg + 8 * c]) = l;
-        split8(s1 + 8 * c, h, l);
-        *reinterpret_cast<uint4*>(&Bs[1][0][n][16 * kg + 8 * c]) = h;
-        *reinterpret_cast<uint4*>(&Bs[1][1][n][16 * kg + 8 * c]) = l;
+      for (int nn = 0; nn < 4; nn++) {
+        float v0[4] = {s0[0][nn], s0[1][nn], s0[2][nn], s0[3][nn]};
+        float v1[4] = {s1[0][nn], s1[1][nn], s1[2][nn], s1[3][nn]};
+        uint2 h, l;
+        split4(v0, h, l);
+        *reinterpret_cast<uint2*>(&Bs[0][0][4 * ng + nn][4 * kq]) = h;
+        *reinterpret_cast<uint2*>(&Bs[0][1][4 * ng + nn][4 * kq]) = l;
+        split4(v1, h, l);
+        *reinterpret_cast<uint2*>(&Bs[1][0][4 * ng + nn][4 * kq]) = h;
+        *reinterpret_cast<uint2*>(&Bs[1][1][4 * ng + nn][4 * kq]) = l;
       }
     }
     __syncthreads();

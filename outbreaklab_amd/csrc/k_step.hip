@@ -67,7 +67,7 @@ __device__ static float hotSpots(const float* b, float time, float s) {
       float r1 = (float)(h & 0xffffu) / 65535.0f;
       float r2 = (float)((h >> 16u) & 0xffffu) / 65535.0f;
       float z = (s - ((float)k + 0.5f + 0.7f * (r1 - 0.5f)) * L) / (0.17f * L);
-      acc += (q == 1 ? fe : 1.0f - fe) * (0.35f + 0.65f * r2) * expf(-z * z);
+      acc += (q == 1 ? fe : 1.0f - fe) * (0.35f + 0.65f * r2) * ex(-z * z);
     }
   }
   return clampf_(acc, 0.0f, 1.0f);
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_lsf(MU u, const float4* __restrict__ BS
     float lift = vr > 0.0f ? mixf(1.0f, mixf(1.0f - 0.85f * vr, 1.0f + 1.3f * vr, hotSpots(b, time, along)), fd) : 1.0f;
     float Hmin = b[7] * clampf_(lift, 0.15f, 3.0f);
     float H = clampf_(b[6] * fmaxf(dd + b[3], 0.0f) * (1.0f + 0.4f * av), Hmin, b[5]);
-    sT += w * b[4] * expf(0.5f * av);
+    sT += w * b[4] * ex(0.5f * av);
     sQ += w * b[8];
     sU += w * b[10] * lift;
     sV += w * b[11] * lift;
@@ -482,10 +482,10 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   float rq = 1000.0f * rho;
   float rs = sqrtf(rsfc / rho);
   float pauto = u.v[U_MICRO].x * fmaxf(qc - u.v[U_MICRO].y, 0.0f);
-  float accr = 2.2f * qc * powf(qr, 0.875f);
+  float accr = 2.2f * qc * pw(qr, 0.875f);
   if (thm && qr > 1e-9f) {
     float l = lamR(qr, nr);
-    accr = 3100.0f * rho * nr * powf(l, -2.8f) * qc * rs;
+    accr = 3100.0f * rho * nr * pw(l, -2.8f) * qc * rs;
   }
   float da = fminf(pauto * dt, qc);
   float dc = fminf(accr * dt, qc - da);
@@ -494,9 +494,9 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   nr += da / 2.68e-10f;
   float dTc = T - OBL_T0K;
   if (dTc < 0.0f) {
-    float e25 = expf(0.025f * dTc);
-    float pqs = powf(fmaxf(rq * qs, 0.0f), 0.81f);
-    float pqg = powf(fmaxf(rq * qg, 0.0f), 0.875f);
+    float e25 = ex(0.025f * dTc);
+    float pqs = pw(fmaxf(rq * qs, 0.0f), 0.81f);
+    float pqg = pw(fmaxf(rq * qg, 0.0f), 0.875f);
     float psaut = 1e-3f * e25 * fmaxf(qi - 6e-4f, 0.0f);
     float psacw = 5.4e-3f * qc * pqs * rs;
     float psaci = 5.4e-3f * qi * pqs * e25 * rs;
@@ -504,8 +504,8 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
     float pgacw = 3.1e-3f * qc * pqg * rs * ewet;
     float pgaci = 3.0e-4f * qi * pqg * rs;
     float pgacr = 3.1e-3f * qr * pqg * rs * ewet;
-    float pgfr = fminf(2e-4f * u.v[U_MICRO].z * (expf(0.66f * fminf(-dTc, 20.0f)) - 1.0f), 0.08f) * qr;
-    float pgaut = 1e-3f * expf(0.09f * dTc) * fmaxf(qs - 6e-4f, 0.0f) * (thm ? 0.25f : 1.0f);
+    float pgfr = fminf(2e-4f * u.v[U_MICRO].z * (ex(0.66f * fminf(-dTc, 20.0f)) - 1.0f), 0.08f) * qr;
+    float pgaut = 1e-3f * ex(0.09f * dTc) * fmaxf(qs - 6e-4f, 0.0f) * (thm ? 0.25f : 1.0f);
     float a1 = fminf((psaut + psaci + pgaci) * dt, qi);
     float fs1 = (psaut + psaci) / fmaxf(psaut + psaci + pgaci, 1e-20f);
     qi -= a1;
@@ -532,13 +532,13 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
     if (thm) {
       if (qg > 1e-9f) {
         float l = lamG(qg, rho), n0 = n0G(qg, rho);
-        float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.89f * 5430.0f * powf(rsfc / rho, 0.25f) * powf(l, -2.945f);
+        float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.89f * 5430.0f * pw(rsfc / rho, 0.25f) * pw(l, -2.945f);
         float heat = 2.4e-2f * dTc - OBL_LVC * 2.26e-5f * rho * (qvsat(OBL_T0K, p0, 0.0f) - qv);
         m2 = clampf_(6.2832f / (rho * OBL_LFC) * heat * n0 * vent * dt, 0.0f, qg);
         nr += m2 / qg * n0 / (rho * l);
       }
       if (qs > 1e-9f) {
-        float ls = powf(3.14159f * 100.0f * 2e6f / (rho * qs), 0.25f);
+        float ls = pw(3.14159f * 100.0f * 2e6f / (rho * qs), 0.25f);
         nr += m1 / qs * 2e6f / (rho * ls);
       }
     }
@@ -550,14 +550,14 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   float qvsl = qvsat(T, p0, 0.0f);
   if (qr > 0.0f && qv < qvsl) {
     float rc = 0.001f * rho * qr;
-    float ern = ((1.6f + 124.9f * powf(rc, 0.2046f)) * powf(rc, 0.525f) / (2.55e8f / (p0 * qvsl) + 5.4e5f)) *
+    float ern = ((1.6f + 124.9f * pw(rc, 0.2046f)) * pw(rc, 0.525f) / (2.55e8f / (p0 * qvsl) + 5.4e5f)) *
                 ((qvsl - qv) / (0.001f * rho * qvsl));
     float srt = u.v[U_MICRO2].y > 0.0f ? sqrtf(rc / (rc + 0.001f * rho * u.v[U_MICRO2].y)) : 1.0f;
     float e = ern * srt * dt * u.v[U_TURB].z;
     if (thm) {
       float l = lamR(qr, nr), n0 = rho * nr * l;
       float Ap = OBL_LVC * OBL_LVC / (2.4e-2f * OBL_RVC * T * T), Bp = 1.0f / (rho * qvsl * 2.26e-5f);
-      float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.827f * 7492.0f * powf(rsfc / rho, 0.25f) * powf(l, -2.9f);
+      float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.827f * 7492.0f * pw(rsfc / rho, 0.25f) * pw(l, -2.9f);
       e = 6.2832f * (1.0f - qv / qvsl) * n0 * vent / (rho * (Ap + Bp)) * dt * u.v[U_MICRO2].w;
     }
     e = clampf_(e, 0.0f, fminf(qr, qvsl - qv));
@@ -569,8 +569,8 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   if (thm) {
     if (qr > 1e-9f) {
       float dm = 1.817f / lamR(qr, nr);
-      float fb = clampf_(1.0f - expf(2300.0f * (dm - 1.1e-3f)), -2.0f, 1.0f);
-      nr *= expf(-7.12f * rho * qr * fb * dt);
+      float fb = clampf_(1.0f - ex(2300.0f * (dm - 1.1e-3f)), -2.0f, 1.0f);
+      nr *= ex(-7.12f * rho * qr * fb * dt);
       nr = clampf_(nr, qr * 3.183e5f, qr * 3.183e11f);
     } else {
       nr = 0.0f;
@@ -892,31 +892,21 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
   }
 }
 
+// phys is compute-heavy (forces and microphysics) with few gathers: a flat 3-D grid keeps more
+// waves in flight than a z-marching block would, and its neighbour reads hit L1/L2
 template <bool DUMP>
-__global__ void __launch_bounds__(TILE_THREADS) k_phys_t(MU u, const float4* __restrict__ BS,
-                                                         const float4* __restrict__ Vin, const float4* __restrict__ Q1in,
-                                                         const float4* __restrict__ Q2in, const float2* __restrict__ aux,
-                                                         const float4* __restrict__ lsfA, const float4* __restrict__ lsfB,
-                                                         const float* __restrict__ phT, const float4* __restrict__ ciMap,
-                                                         float4* __restrict__ Vout, float4* __restrict__ Q1o,
-                                                         float4* __restrict__ Q2o, MicroDump dump) {
-  __shared__ float lds[2 * RING_FLOATS];
-  TILE_SETUP
-  Ring rv{lds, Vin, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
-  Ring rq{lds + RING_FLOATS, Q1in, d.nx, d.ny, d.nz, x0, y0, 0};
-  for (int k = -1; k <= 1; k++) { rv.load(k, tid); rq.load(k, tid); }
-  __syncthreads();
-  const bool act = x <= d.nx && y <= d.ny;
-  for (int z = 0; z <= d.nz; z++) {
-    Ring::Pre pv, pq;
-    const bool more = z < d.nz;
-    if (more) { rv.fetch(z + 2, tid, pv); rq.fetch(z + 2, tid, pq); }
-    rv.zc = z; rq.zc = z;
-    if (act) phys_point<DUMP>(u, d, BS, rv, rq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
-    __syncthreads();
-    if (more) { rv.store(z + 2, tid, pv); rq.store(z + 2, tid, pq); }
-    __syncthreads();
-  }
+__global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
+                                              const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
+                                              const float2* __restrict__ aux, const float4* __restrict__ lsfA,
+                                              const float4* __restrict__ lsfB, const float* __restrict__ phT,
+                                              const float4* __restrict__ ciMap, float4* __restrict__ Vout,
+                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
+  Dims d = dims(u);
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  GAcc gv{Vin, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  GAcc gq{Q1in, d.nx, d.ny, d.nz, nullptr};
+  phys_point<DUMP>(u, d, BS, gv, gq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
 }
 
 // ====================== DIV (model.js:910-925) ======================
@@ -1003,14 +993,14 @@ void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, co
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys_t<false>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB,
-                     phT, ciMap, Vout, Q1o, Q2o, none);
+  hipLaunchKernelGGL((k_phys<false>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+                     ciMap, Vout, Q1o, Q2o, none);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
-  hipLaunchKernelGGL((k_phys_t<true>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB,
-                     phT, ciMap, Vout, Q1o, Q2o, dump);
+  hipLaunchKernelGGL((k_phys<true>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+                     ciMap, Vout, Q1o, Q2o, dump);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);

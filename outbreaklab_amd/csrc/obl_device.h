@@ -151,6 +151,13 @@ __device__ __forceinline__ float fbm2(float px, float py, int oct) {
   return s;
 }
 
+// ---------------- transcendentals ----------------
+// pow and exp on the hardware v_log_f32 / v_exp_f32 (log2, 2^x): pw(x, y) = 2^(y log2 x), which is
+// how WGSL specifies pow's accuracy (inherited from exp2(y * log2(x))); ex(x) = 2^(x log2 e).
+// Arguments here are >= 0 (pw) and moderate (ex); pw(0, y > 0) = 0.
+__device__ __forceinline__ float pw(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
 // ---------------- thermodynamics and fall speeds ----------------
 #define OBL_T0K 273.15f
 #define OBL_CPD 1005.7f
@@ -158,8 +165,8 @@ __device__ __forceinline__ float fbm2(float px, float py, int oct) {
 #define OBL_LVC 2.501e6f
 #define OBL_LFC 3.337e5f
 #define OBL_LSC 2.834e6f
-__device__ __forceinline__ float esl(float T) { return 611.2f * expf(17.67f * (T - OBL_T0K) / (T - 29.65f)); }
-__device__ __forceinline__ float esi(float T) { return 611.2f * expf(21.8745584f * (T - 273.16f) / (T - 7.66f)); }
+__device__ __forceinline__ float esl(float T) { return 611.2f * ex(17.67f * (T - OBL_T0K) / (T - 29.65f)); }
+__device__ __forceinline__ float esi(float T) { return 611.2f * ex(21.8745584f * (T - 273.16f) / (T - 7.66f)); }
 __device__ __forceinline__ float fIce(float T) { return clampf_((OBL_T0K - T) / 40.0f, 0.0f, 1.0f); }
 __device__ __forceinline__ float qvsat(float T, float p, float f) {
   float e = mixf(esl(T), esi(T), f);
@@ -167,30 +174,30 @@ __device__ __forceinline__ float qvsat(float T, float p, float f) {
 }
 __device__ __forceinline__ float vtR(float q, float rho, float rsfc) {
   float f = sqrtf(rsfc / rho);
-  return fminf(36.34f * powf(fmaxf(0.001f * rho * q, 0.0f), 0.1364f) * f, 10.0f * f);
+  return fminf(36.34f * pw(fmaxf(0.001f * rho * q, 0.0f), 0.1364f) * f, 10.0f * f);
 }
 __device__ __forceinline__ float vtS(float q, float rho, float rsfc) {
-  return clampf_(1.1f * powf(fmaxf(1000.0f * rho * q, 1e-6f), 0.11f), 0.3f, 2.2f) * sqrtf(rsfc / rho);
+  return clampf_(1.1f * pw(fmaxf(1000.0f * rho * q, 1e-6f), 0.11f), 0.3f, 2.2f) * sqrtf(rsfc / rho);
 }
 __device__ __forceinline__ float vtG(float q, float rho, float rsfc) {
-  float v = clampf_(9.0f * powf(fmaxf(1000.0f * rho * q, 1e-6f), 0.42f), 1.0f, 55.0f) * sqrtf(rsfc / rho);
+  float v = clampf_(9.0f * pw(fmaxf(1000.0f * rho * q, 1e-6f), 0.42f), 1.0f, 55.0f) * sqrtf(rsfc / rho);
   float r = q / 0.009f;
   return fminf(v * (1.0f + r * r), 65.0f);
 }
 __device__ __forceinline__ float lamR(float q, float n) {
-  return clampf_(powf(3.14159f * 1000.0f * fmaxf(n, 1e-3f) / fmaxf(q, 1e-12f), 0.33333f), 1000.0f, 1.0e5f);
+  return clampf_(pw(3.14159f * 1000.0f * fmaxf(n, 1e-3f) / fmaxf(q, 1e-12f), 0.33333f), 1000.0f, 1.0e5f);
 }
 __device__ __forceinline__ float2 vtRT(float q, float n, float rho, float rsfc) {
-  float l = powf(lamR(q, n), -0.8f);
+  float l = pw(lamR(q, n), -0.8f);
   float s = sqrtf(rsfc / rho);
   return make_float2(fminf(2503.0f * l, 10.0f) * s, fminf(784.0f * l, 10.0f) * s);
 }
 __device__ __forceinline__ float n0G(float q, float rho) { return clampf_(200.0f / fmaxf(rho * q, 1e-9f), 1e4f, 3e6f); }
 __device__ __forceinline__ float lamG(float q, float rho) {
-  return powf(3.14159f * 500.0f * n0G(q, rho) / fmaxf(rho * q, 1e-12f), 0.25f);
+  return pw(3.14159f * 500.0f * n0G(q, rho) / fmaxf(rho * q, 1e-12f), 0.25f);
 }
 __device__ __forceinline__ float vtGT(float q, float rho, float rsfc) {
-  return fminf(1503.0f * powf(lamG(q, rho), -0.89f), 25.0f) * sqrtf(rsfc / rho);
+  return fminf(1503.0f * pw(lamG(q, rho), -0.89f), 25.0f) * sqrtf(rsfc / rho);
 }
 __device__ __forceinline__ float2 vtRain(bool thm, float qr, float nr, float rho, float rsfc) {
   if (thm) return vtRT(qr, nr, rho, rsfc);

@@ -1,0 +1,183 @@
+// C ABI of the microphysics surrogate network (include/outbreak_hip.h, obl_mlp_*):
+// parameters on the device, GPU training with Adam, and exact evaluation for checks.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/outbreak_hip.h"
+#include "obl_mlp.h"
+
+using namespace obl;
+
+struct obl_mlp {
+  hipStream_t stream = nullptr;
+  float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr, *loss = nullptr;
+  float out_scale[MLP_OUT];
+  float in_lo[MLP_NIN], in_hi[MLP_NIN];
+  int adam_t = 0;
+  std::string err;
+  int fail(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return -1;
+  }
+  int ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    return fail("%s: %s", what, hipGetErrorString(e));
+  }
+};
+
+static std::string g_mlp_err;
+
+extern "C" const char* obl_mlp_last_error(const obl_mlp* m) { return m ? m->err.c_str() : g_mlp_err.c_str(); }
+
+extern "C" void obl_mlp_destroy(obl_mlp* m) {
+  if (!m) return;
+  if (m->stream) hipStreamSynchronize(m->stream);
+  void* b[] = {m->P, m->G, m->M, m->V, m->loss};
+  for (void* p : b)
+    if (p) hipFree(p);
+  if (m->stream) hipStreamDestroy(m->stream);
+  delete m;
+}
+
+extern "C" int obl_mlp_create(obl_mlp** out) {
+  if (!out) {
+    g_mlp_err = "null output";
+    return -1;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    g_mlp_err = "surrogate: no ROCm GPU is available";
+    return -1;
+  }
+  obl_mlp* m = new obl_mlp();
+  for (int j = 0; j < MLP_OUT; j++) m->out_scale[j] = 1.0f;
+  for (int i = 0; i < MLP_NIN; i++) { m->in_lo[i] = -3.0e38f; m->in_hi[i] = 3.0e38f; }
+  size_t n = (size_t)MLP_NPARAM * 4;
+  if (m->ok(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream") || m->ok(hipMalloc(&m->P, n), "P") ||
+      m->ok(hipMalloc(&m->G, n), "G") || m->ok(hipMalloc(&m->M, n), "M") || m->ok(hipMalloc(&m->V, n), "V") ||
+      m->ok(hipMalloc(&m->loss, 4), "loss") || m->ok(hipMemsetAsync(m->P, 0, n, m->stream), "clear") ||
+      m->ok(hipMemsetAsync(m->G, 0, n, m->stream), "clear") || m->ok(hipMemsetAsync(m->M, 0, n, m->stream), "clear") ||
+      m->ok(hipMemsetAsync(m->V, 0, n, m->stream), "clear") || m->ok(hipStreamSynchronize(m->stream), "sync")) {
+    g_mlp_err = m->err;
+    obl_mlp_destroy(m);
+    return -1;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int obl_mlp_nparams(void) { return MLP_NPARAM; }
+
+extern "C" int obl_mlp_set_params(obl_mlp* m, const float* p, int n) {
+  if (!m || !p || n != MLP_NPARAM) return m ? m->fail("expected %d parameters", MLP_NPARAM) : -1;
+  m->adam_t = 0;
+  size_t b = (size_t)MLP_NPARAM * 4;
+  if (m->ok(hipMemcpyAsync(m->P, p, b, hipMemcpyHostToDevice, m->stream), "set params") ||
+      m->ok(hipMemsetAsync(m->M, 0, b, m->stream), "clear") || m->ok(hipMemsetAsync(m->V, 0, b, m->stream), "clear") ||
+      m->ok(hipMemsetAsync(m->G, 0, b, m->stream), "clear"))
+    return -1;
+  return m->ok(hipStreamSynchronize(m->stream), "set params");
+}
+
+extern "C" int obl_mlp_get_params(obl_mlp* m, float* p, int n) {
+  if (!m || !p || n != MLP_NPARAM) return m ? m->fail("expected %d parameters", MLP_NPARAM) : -1;
+  if (m->ok(hipMemcpyAsync(p, m->P, (size_t)MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "get params")) return -1;
+  return m->ok(hipStreamSynchronize(m->stream), "get params");
+}
+
+extern "C" int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const float* in_lo, const float* in_hi) {
+  if (!m || !out_scale) return m ? m->fail("null scales") : -1;
+  for (int j = 0; j < MLP_OUT; j++) m->out_scale[j] = j < MLP_NOUT ? out_scale[j] : 1.0f;
+  for (int i = 0; i < MLP_NIN; i++) {
+    m->in_lo[i] = in_lo ? in_lo[i] : -3.0e38f;
+    m->in_hi[i] = in_hi ? in_hi[i] : 3.0e38f;
+  }
+  return 0;
+}
+
+static void fill_args(obl_mlp* m, MlpTrainArgs& a, const float* rows, size_t nrows, int stride, int batch, int quant) {
+  memset(&a, 0, sizeof a);
+  a.P = m->P;
+  a.rows = rows;
+  a.stride = stride;
+  a.nrows = nrows;
+  a.batch = batch;
+  for (int j = 0; j < MLP_OUT; j++) a.out_inv_scale[j] = 1.0f / m->out_scale[j];
+  a.loss_scale = 1.0f / ((float)batch * (float)MLP_NOUT);
+  a.loss = m->loss;
+  a.quant = quant;
+}
+
+extern "C" int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, int stride, int steps, int batch, float lr,
+                             unsigned seed, int quant, float* mean_loss) {
+  if (!m) return -1;
+  if (!rows_dev || nrows == 0 || stride < MLP_NIN + MLP_NOUT || batch < 1 || steps < 1)
+    return m->fail("train: need rows (stride >= %d), batch >= 1, steps >= 1", MLP_NIN + MLP_NOUT);
+  MlpTrainArgs a;
+  fill_args(m, a, rows_dev, nrows, stride, batch, quant);
+  a.G = m->G;
+  if (m->ok(hipMemsetAsync(m->loss, 0, 4, m->stream), "loss")) return -1;
+  for (int s = 0; s < steps; s++) {
+    a.seed = seed * 2654435761u + (unsigned)s;
+    launch_mlp_grad(m->stream, a);
+    launch_adam(m->stream, m->P, m->G, m->M, m->V, MLP_NPARAM, lr, 0.9f, 0.999f, 1e-8f, ++m->adam_t);
+  }
+  if (m->ok(hipGetLastError(), "train launch")) return -1;
+  float l = 0.f;
+  if (m->ok(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream), "loss") ||
+      m->ok(hipStreamSynchronize(m->stream), "train"))
+    return -1;
+  if (mean_loss) *mean_loss = l / ((float)steps * (float)batch * (float)MLP_NOUT);
+  return 0;
+}
+
+extern "C" int obl_mlp_eval(obl_mlp* m, const float* rows_host, int n, int stride, int quant, float* loss, float* grads,
+                            float* y) {
+  if (!m) return -1;
+  if (!rows_host || n < 1 || stride < MLP_NIN + MLP_NOUT) return m->fail("eval: need n >= 1 rows");
+  float *drows = nullptr, *dy = nullptr;
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  int* didx = nullptr;
+  size_t rb = (size_t)n * stride * 4;
+  if (m->ok(hipMalloc(&drows, rb), "rows") || m->ok(hipMalloc(&didx, (size_t)n * 4), "idx") ||
+      m->ok(hipMalloc(&dy, (size_t)n * MLP_OUT * 4), "y")) {
+    hipFree(drows); hipFree(didx); hipFree(dy);
+    return -1;
+  }
+  hipMemcpyAsync(drows, rows_host, rb, hipMemcpyHostToDevice, m->stream);
+  hipMemcpyAsync(didx, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->stream);
+  hipMemsetAsync(m->loss, 0, 4, m->stream);
+  if (grads) hipMemsetAsync(m->G, 0, (size_t)MLP_NPARAM * 4, m->stream);
+  MlpTrainArgs a;
+  fill_args(m, a, drows, (size_t)n, stride, n, quant);
+  a.idx = didx;
+  a.G = grads ? m->G : nullptr;
+  a.y_out = dy;
+  launch_mlp_grad(m->stream, a);
+  float l = 0.f;
+  int rc = m->ok(hipGetLastError(), "eval launch");
+  if (!rc) rc = m->ok(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream), "loss");
+  if (!rc && grads) rc = m->ok(hipMemcpyAsync(grads, m->G, (size_t)MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "grads");
+  if (!rc && y) rc = m->ok(hipMemcpyAsync(y, dy, (size_t)n * MLP_OUT * 4, hipMemcpyDeviceToHost, m->stream), "y");
+  if (!rc) rc = m->ok(hipStreamSynchronize(m->stream), "eval");
+  if (!rc && grads) hipMemsetAsync(m->G, 0, (size_t)MLP_NPARAM * 4, m->stream);
+  hipStreamSynchronize(m->stream);
+  hipFree(drows);
+  hipFree(didx);
+  hipFree(dy);
+  if (loss) *loss = l / ((float)n * (float)MLP_NOUT);
+  return rc;
+}
+

@@ -114,6 +114,35 @@ struct GAcc {
   __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return g[gidx(i, j, k)]; }
 };
 
+// Out-of-window paths, kept out of line: they are rare, and inlined they would hold registers
+// across every sample of the fast path.
+__device__ __noinline__ static float samp1_global(const float4* g4, int W, int H, int D, int i0, int j0, int k0, float ax,
+                                                  float ay, float az, int ch) {
+  const float* g = reinterpret_cast<const float*>(g4);
+  int x0 = clampi(i0, 0, W - 1), x1 = clampi(i0 + 1, 0, W - 1);
+  int y0 = clampi(j0, 0, H - 1), y1 = clampi(j0 + 1, 0, H - 1);
+  int z0 = clampi(k0, 0, D - 1), z1 = clampi(k0 + 1, 0, D - 1);
+  size_t r00 = (size_t)W * ((size_t)y0 + (size_t)H * z0), r10 = (size_t)W * ((size_t)y1 + (size_t)H * z0);
+  size_t r01 = (size_t)W * ((size_t)y0 + (size_t)H * z1), r11 = (size_t)W * ((size_t)y1 + (size_t)H * z1);
+  float c00 = lerpf(g[4 * (r00 + x0) + ch], g[4 * (r00 + x1) + ch], ax);
+  float c10 = lerpf(g[4 * (r10 + x0) + ch], g[4 * (r10 + x1) + ch], ax);
+  float c01 = lerpf(g[4 * (r01 + x0) + ch], g[4 * (r01 + x1) + ch], ax);
+  float c11 = lerpf(g[4 * (r11 + x0) + ch], g[4 * (r11 + x1) + ch], ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+__device__ __noinline__ static float2 minmax1_global(const float4* g4, int W, int H, int D, int i0, int j0, int k0,
+                                                     int ch) {
+  const float* g = reinterpret_cast<const float*>(g4);
+  float lo = 1e30f, hi = -1e30f;
+  for (int q = 0; q < 8; q++) {
+    int i = clampi(i0 + (q & 1), 0, W - 1), j = clampi(j0 + ((q >> 1) & 1), 0, H - 1), k = clampi(k0 + (q >> 2), 0, D - 1);
+    float s = g[4 * ((size_t)i + (size_t)W * ((size_t)j + (size_t)H * (size_t)k)) + ch];
+    lo = fminf(lo, s);
+    hi = fmaxf(hi, s);
+  }
+  return make_float2(lo, hi);
+}
+
 // Trilinear, clamp-to-edge sample of one channel at texel coordinates (t - 1/2 convention
 // applied by the caller); x, then y, then z lerps as a + t (b - a).
 template <class A>
@@ -121,19 +150,11 @@ __device__ __forceinline__ float samp1(const A& a, float tx, float ty, float tz,
   float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
   float ax = tx - fx, ay = ty - fy, az = tz - fz;
   int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-  float c000, c100, c010, c110, c001, c101, c011, c111;
-  if (a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1)) {
-    int b = a.sidx(i0, j0, k0, ch);
-    int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
-    c000 = a.s[b]; c100 = a.s[b + 1]; c010 = a.s[b + TPW]; c110 = a.s[b + TPW + 1];
-    c001 = a.s[b + dz]; c101 = a.s[b + dz + 1]; c011 = a.s[b + dz + TPW]; c111 = a.s[b + dz + TPW + 1];
-  } else {
-    const float* g = reinterpret_cast<const float*>(a.g);
-    c000 = g[4 * a.gidx(i0, j0, k0) + ch]; c100 = g[4 * a.gidx(i0 + 1, j0, k0) + ch];
-    c010 = g[4 * a.gidx(i0, j0 + 1, k0) + ch]; c110 = g[4 * a.gidx(i0 + 1, j0 + 1, k0) + ch];
-    c001 = g[4 * a.gidx(i0, j0, k0 + 1) + ch]; c101 = g[4 * a.gidx(i0 + 1, j0, k0 + 1) + ch];
-    c011 = g[4 * a.gidx(i0, j0 + 1, k0 + 1) + ch]; c111 = g[4 * a.gidx(i0 + 1, j0 + 1, k0 + 1) + ch];
-  }
+  if (!(a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1))) return samp1_global(a.g, a.W, a.H, a.D, i0, j0, k0, ax, ay, az, ch);
+  int b = a.sidx(i0, j0, k0, ch);
+  int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
+  float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + TPW], c110 = a.s[b + TPW + 1];
+  float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + TPW], c111 = a.s[b + dz + TPW + 1];
   float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
   float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
   return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
@@ -148,24 +169,16 @@ __device__ __forceinline__ float4 samp4(const A& a, float tx, float ty, float tz
 // min / max of one channel over the 8 texels around a point (the MacCormack limiter)
 template <class A>
 __device__ __forceinline__ float2 minmax1(const A& a, int i0, int j0, int k0, int ch) {
+  if (!(a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1))) return minmax1_global(a.g, a.W, a.H, a.D, i0, j0, k0, ch);
   float lo = 1e30f, hi = -1e30f;
-  if (a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1)) {
-    int b = a.sidx(i0, j0, k0, ch);
-    int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
-    const int off[8] = {0, 1, TPW, TPW + 1, dz, dz + 1, dz + TPW, dz + TPW + 1};
+  int b = a.sidx(i0, j0, k0, ch);
+  int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
+  const int off[8] = {0, 1, TPW, TPW + 1, dz, dz + 1, dz + TPW, dz + TPW + 1};
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      float s = a.s[b + off[q]];
-      lo = fminf(lo, s);
-      hi = fmaxf(hi, s);
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      float s = reinterpret_cast<const float*>(a.g)[4 * a.gidx(i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2)) + ch];
-      lo = fminf(lo, s);
-      hi = fmaxf(hi, s);
-    }
+  for (int q = 0; q < 8; q++) {
+    float s = a.s[b + off[q]];
+    lo = fminf(lo, s);
+    hi = fmaxf(hi, s);
   }
   return make_float2(lo, hi);
 }

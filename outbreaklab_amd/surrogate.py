@@ -1,0 +1,101 @@
+"""Host side of the microphysics surrogate (no reference counterpart): the network's parameters,
+scales and envelope, GPU training from the model's sampled (state, tendency) pairs, and
+evaluation. The kernels are in csrc/k_mlp.hip."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+NIN, NOUT, IN, H, OUT = 14, 9, 16, 128, 16
+ROW = 24  # floats per sampled row: 14 inputs, 9 tendencies, level
+OUT_NAMES = ["dT", "dqv", "dqc", "dqi", "dqr", "dqs", "dqg", "dNr", "dNi"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def init_params(seed=0):
+    """He-normal weights, zero biases, in the device layout W1 b1 W2 b2 W3 b3 W4 b4 (W[k][c])."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for a, b in ((IN, H), (H, H), (H, H), (H, OUT)):
+        parts += [(rng.standard_normal((a, b)) * np.sqrt(2.0 / a)).ravel(), np.zeros(b)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def unpack(p):
+    p = np.asarray(p, np.float64)
+    out, o = {}, 0
+    for name, (a, b) in (("1", (IN, H)), ("2", (H, H)), ("3", (H, H)), ("4", (H, OUT))):
+        out["W" + name] = p[o:o + a * b].reshape(a, b)
+        o += a * b
+        out["b" + name] = p[o:o + b]
+        o += b
+    return out
+
+
+class Surrogate:
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        _lib.check(_lib.lib().obl_mlp_create(C.byref(h)))
+        self.h = h
+        self.n = _lib.lib().obl_mlp_nparams()
+        self.set_params(init_params(seed))
+        self.out_scale = np.ones(NOUT, np.float32)
+        self.in_lo = None
+        self.in_hi = None
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _lib.check(_lib.lib().obl_mlp_set_params(self.h, _p(p), p.size), self.h, mlp=True)
+
+    def params(self):
+        p = np.empty(self.n, np.float32)
+        _lib.check(_lib.lib().obl_mlp_get_params(self.h, _p(p), self.n), self.h, mlp=True)
+        return p
+
+    def set_scales(self, out_scale, in_lo=None, in_hi=None):
+        self.out_scale = np.ascontiguousarray(out_scale, np.float32)
+        lo = None if in_lo is None else np.ascontiguousarray(in_lo, np.float32)
+        hi = None if in_hi is None else np.ascontiguousarray(in_hi, np.float32)
+        self.in_lo, self.in_hi = lo, hi
+        _lib.check(_lib.lib().obl_mlp_set_scales(self.h, _p(self.out_scale), None if lo is None else _p(lo),
+                                                 None if hi is None else _p(hi)), self.h, mlp=True)
+
+    def fit_scales(self, rows):
+        """Tendency scales = RMS of each output (floored), envelope = min/max of each input."""
+        rows = np.asarray(rows, np.float64)
+        s = np.sqrt(np.mean(rows[:, NIN:NIN + NOUT] ** 2, axis=0))
+        s = np.where(s > 0, s, 1.0)
+        self.set_scales(s, rows[:, :NIN].min(0), rows[:, :NIN].max(0))
+
+    def train(self, rows_dev_ptr, nrows, steps, batch=4096, lr=1e-3, seed=1, quant=True, stride=ROW):
+        loss = C.c_float()
+        _lib.check(_lib.lib().obl_mlp_train(self.h, C.c_void_p(rows_dev_ptr), nrows, stride, steps, batch, lr, seed,
+                                            1 if quant else 0, C.byref(loss)), self.h, mlp=True)
+        return loss.value
+
+    def eval(self, rows, quant=True, grads=False):
+        rows = np.ascontiguousarray(rows, np.float32)
+        n, stride = rows.shape
+        loss = C.c_float()
+        g = np.zeros(self.n, np.float32) if grads else None
+        y = np.zeros((n, OUT), np.float32)
+        _lib.check(_lib.lib().obl_mlp_eval(self.h, _p(rows), n, stride, 1 if quant else 0, C.byref(loss),
+                                           None if g is None else _p(g), _p(y)), self.h, mlp=True)
+        return loss.value, g, y
+
+    def destroy(self):
+        if getattr(self, "h", None) is not None:
+            _lib.lib().obl_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

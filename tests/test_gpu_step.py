@@ -2,7 +2,7 @@
 workload (2 km, WK sounding, one 2 K bubble) after it has developed cloud, rain and ice.
 
 Each kernel is checked in isolation: the oracle kernel is fed the GPU's own inputs, and its
-output is compared with the GPU's, per channel, by relative L2. The bound is the north-star
+output is compared with the GPU's, per channel, by relative L2. The bound is the stated
 FP32 tolerance, 1e-5 per field after one step. theta is also checked as the perturbation from
 the base state in kelvin (RMS <= 1e-4 K: a few ULP of 300-470 K values, which is what FMA
 contraction and exp/pow ULPs leave).
