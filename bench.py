@@ -23,6 +23,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3
 BF16_MFMA_PEAK_TFS = 2500.0
+# README.md:47 of the reference: 32x real time at 1024x1024x80, dt 3 s on an RTX 5090 (93.8 ms/step);
+# at the same cell-update rate and dt 2 s that is 21.3 sim-s per wall-s
+RTX5090_SIM_S_PER_WALL_S_DT2 = 2.0 / 0.09375
 
 # compulsory bytes per model cell of each stencil kernel: each field read once + written once
 # (float4 texel 16 B, aux 8 B, rhs 4 B); halo re-reads excluded
@@ -94,6 +97,21 @@ def cpu_baseline(threads: int, dt: float, full_cells: int):
                       f"1024x1024x80; the reference's own WGSL on a CPU WebGPU adapter is unavailable here"}
 
 
+def calibrate_copy(mib: int):
+    """Plain device-to-device copies of a known size: FETCH_SIZE / WRITE_SIZE of their dispatches
+    give the scale from the HBM counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +122,18 @@ def main():
     ap.add_argument("--size", default="1024,1024,80")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
               "decomposition is a design only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     from outbreaklab_amd import RegionalModel
     from outbreaklab_amd.configs import setup_plain_1024
@@ -133,7 +158,7 @@ def main():
 
     # per-kernel HIP-event times on the model's stream (separate, shorter run)
     m.profile(True)
-    for _ in range(5):
+    for _ in range(args.profile_steps):
         m.step(dt)
     prof = m.profile_read()
     m.profile(False)
@@ -167,7 +192,7 @@ def main():
         "cell_updates_per_s": cells / (ms_step * 1e-3),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": (dt / (ms_step * 1e-3)) / RTX5090_SIM_S_PER_WALL_S_DT2,
         "dtype": "f32",
         "data": "synthetic (WK sounding tuned to 3000 J/kg SBCAPE, 25 m/s shear, 8 seeded 2 K bubbles)",
         "config": {"workload": f"plain step {nx}x{ny}x{nz} at 500 m, Thompson, dt {dt} s, solve "
@@ -176,7 +201,8 @@ def main():
         "roofline": roof,
         "kernels": kern,
         "cpu_baseline": cpu,
-        "reference_rtx5090_sim_s_per_wall_s_at_dt2": 21.3,
+        "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
+                             "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
     }
     print(json.dumps(out))
     m.destroy()

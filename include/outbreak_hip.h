@@ -157,7 +157,11 @@ int obl_mlp_get_params(obl_mlp* m, float* p, int n);
 /* out_scale[9]: tendency scales (targets are tendency / scale); in_lo/in_hi[14]: the training
  * envelope (NULL = unbounded); points outside it use the plain scheme at inference */
 int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const float* in_lo, const float* in_hi);
-/* `steps` Adam steps on batches of `batch` rows drawn (hashed with `seed`) from nrows device rows;
+/* copies nrows host rows (stride floats each) into device memory the network owns (replacing any
+ * earlier copy); obl_mlp_train with rows_dev == NULL trains on them */
+int obl_mlp_load_rows(obl_mlp* m, const float* rows_host, size_t nrows, int stride);
+/* `steps` Adam steps on batches of `batch` rows drawn (hashed with `seed`) from nrows device rows
+ * (rows_dev == NULL: the rows of obl_mlp_load_rows, nrows and stride ignored);
  * quant != 0 rounds weights, features and activations to BF16 in the forward pass as inference does */
 int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, int stride, int steps, int batch, float lr,
                   unsigned seed, int quant, float* mean_loss);

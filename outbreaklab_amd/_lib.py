@@ -53,6 +53,7 @@ _SIGS = {
     "obl_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_mlp_set_scales": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "obl_mlp_load_rows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "obl_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint,
                                 C.c_int, C.POINTER(C.c_float)]),
     "obl_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_void_p,

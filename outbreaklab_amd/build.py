@@ -32,6 +32,14 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
+# Per-file additions. The stencil and physics kernels follow WGSL's FP32 accuracy contract, which
+# allows 2.5 ULP for x / y and an inverseSqrt-derived sqrt, and allows denormals to flush: division
+# becomes a range-scaled hardware reciprocal, sqrt a single v_sqrt_f32, division by a constant a
+# multiply. The solver (k_solve.hip) keeps correctly rounded FP32 division for the Thomas sweeps.
+FILE_FLAGS = {
+    "k_step.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
+}
+
 
 def _run(cmd, cwd=None):
     r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -51,7 +59,8 @@ def build_hip(jobs: int = 4) -> str:
     for s in srcs:
         o = os.path.join(objdir, os.path.basename(s) + ".o")
         objs.append(o)
-        cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + ["-c", s, "-o", o]
+        cmd = [HIPCC] + [f for f in HIP_FLAGS if f != "-shared"] + FILE_FLAGS.get(os.path.basename(s), []) + \
+            ["-c", s, "-o", o]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         if len(procs) >= jobs:
             cmd0, p0 = procs.pop(0)

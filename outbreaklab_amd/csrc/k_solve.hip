@@ -37,6 +37,35 @@ __device__ __forceinline__ short bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
+// Workgroup b of a launch runs on XCD b % 8. Renumber the tiles so that each XCD receives a
+// contiguous range of them: the tiles that share a data block (consecutive in the tile order)
+// then meet in one L2 instead of fetching the block into all eight.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  int per = G >> 3, rem = G & 7, x = b & 7, j = b >> 3;
+  return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+}
+
+// Tile coordinates of this workgroup. X passes: output-column tiles fastest (they share the data
+// rows); Y passes: output-row tiles fastest (they share the data columns of one plane).
+template <bool XMODE>
+__device__ __forceinline__ void dst_tile(int& mt, int& nt, int& plane) {
+  int gx = gridDim.x, gy = gridDim.y;
+  int G = gx * gy * gridDim.z;
+  int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int t = xcd_remap(b, G);
+  if (XMODE) {
+    nt = t % gx;
+    t /= gx;
+    mt = t % gy;
+    plane = t / gy;
+  } else {
+    mt = t % gy;
+    t /= gy;
+    nt = t % gx;
+    plane = t / gx;
+  }
+}
+
 template <int MODE, int PREC>
 __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
   __shared__ float As[2][BM][LDK];
@@ -45,8 +74,9 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, kq = lane >> 4;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int plane = blockIdx.z;
+  int mt, nt, plane;
+  dst_tile<MODE == DST_XF || MODE == DST_XI>(mt, nt, plane);
+  const int m0 = mt * BM, n0 = nt * BN;
   const int N = a.N, Nh = a.Nh;
   const size_t planeSz = (size_t)N * (size_t)a.inner;
 
@@ -213,8 +243,9 @@ __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, g = lane >> 4;
-  const int m0 = blockIdx.y * BBM, n0 = blockIdx.x * BBN;
-  const int plane = blockIdx.z;
+  int mt, nt, plane;
+  dst_tile<MODE == DST_XF || MODE == DST_XI>(mt, nt, plane);
+  const int m0 = mt * BBM, n0 = nt * BBN;
   const int N = a.N, Nh = a.Nh;
   const size_t planeSz = (size_t)N * (size_t)a.inner;
   constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);

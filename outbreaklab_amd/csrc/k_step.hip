@@ -894,19 +894,28 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
 
 // phys is compute-heavy (forces and microphysics) with few gathers: a flat 3-D grid keeps more
 // waves in flight than a z-marching block would, and its neighbour reads hit L1/L2
+// One block per 32 x 8 tile of one z-plane of the faces grid; the plane of V and of Q1 (with a
+// one-point halo) is staged in LDS, so the many x / y neighbour reads of the tendencies and
+// diffusion come from LDS; the z neighbours are read from global memory.
 template <bool DUMP>
-__global__ void __launch_bounds__(256) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
-                                              const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
-                                              const float2* __restrict__ aux, const float4* __restrict__ lsfA,
-                                              const float4* __restrict__ lsfB, const float* __restrict__ phT,
-                                              const float4* __restrict__ ciMap, float4* __restrict__ Vout,
-                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
+__global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
+                                                       const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
+                                                       const float2* __restrict__ aux, const float4* __restrict__ lsfA,
+                                                       const float4* __restrict__ lsfB, const float* __restrict__ phT,
+                                                       const float4* __restrict__ ciMap, float4* __restrict__ Vout,
+                                                       float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
+  __shared__ float sV[4 * TPS], sQ[4 * TPS];
   Dims d = dims(u);
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z = blockIdx.z;
+  Plane pv{(lds_float*)sV, Vin, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, z};
+  Plane pq{(lds_float*)sQ, Q1in, d.nx, d.ny, d.nz, x0, y0, z};
+  pv.load(tid);
+  pq.load(tid);
+  __syncthreads();
+  const int x = x0 + (tid % TX), y = y0 + (tid / TX);
   if (x > d.nx || y > d.ny || z > d.nz) return;
-  GAcc gv{Vin, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
-  GAcc gq{Q1in, d.nx, d.ny, d.nz, nullptr};
-  phys_point<DUMP>(u, d, BS, gv, gq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
+  phys_point<DUMP>(u, d, BS, pv, pq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
 }
 
 // ====================== DIV (model.js:910-925) ======================
@@ -970,6 +979,7 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
   hipLaunchKernelGGL(k_bubble, faces_grid(d), dim3(64, 4, 1), 0, s, u, V0, V1, x, y, z, rh, rz, dth);
 }
 static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
+static dim3 tface_grid3(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, d.nz + 1); }
 static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 float2* aux) {
@@ -993,13 +1003,13 @@ void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, co
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys<false>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+  hipLaunchKernelGGL((k_phys<false>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
                      ciMap, Vout, Q1o, Q2o, none);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
-  hipLaunchKernelGGL((k_phys<true>), faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
+  hipLaunchKernelGGL((k_phys<true>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
                      ciMap, Vout, Q1o, Q2o, dump);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {

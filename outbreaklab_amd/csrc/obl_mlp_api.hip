@@ -16,6 +16,9 @@ using namespace obl;
 struct obl_mlp {
   hipStream_t stream = nullptr;
   float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr, *loss = nullptr;
+  float* rows = nullptr;  // rows of obl_mlp_load_rows
+  size_t nrows = 0;
+  int stride = 0;
   float out_scale[MLP_OUT];
   float in_lo[MLP_NIN], in_hi[MLP_NIN];
   int adam_t = 0;
@@ -42,7 +45,7 @@ extern "C" const char* obl_mlp_last_error(const obl_mlp* m) { return m ? m->err.
 extern "C" void obl_mlp_destroy(obl_mlp* m) {
   if (!m) return;
   if (m->stream) hipStreamSynchronize(m->stream);
-  void* b[] = {m->P, m->G, m->M, m->V, m->loss};
+  void* b[] = {m->P, m->G, m->M, m->V, m->loss, m->rows};
   for (void* p : b)
     if (p) hipFree(p);
   if (m->stream) hipStreamDestroy(m->stream);
@@ -119,9 +122,35 @@ static void fill_args(obl_mlp* m, MlpTrainArgs& a, const float* rows, size_t nro
   a.quant = quant;
 }
 
+extern "C" int obl_mlp_load_rows(obl_mlp* m, const float* rows_host, size_t nrows, int stride) {
+  if (!m) return -1;
+  if (!rows_host || nrows == 0 || stride < MLP_NIN + MLP_NOUT)
+    return m->fail("load_rows: need rows (stride >= %d)", MLP_NIN + MLP_NOUT);
+  if (m->rows) {
+    hipStreamSynchronize(m->stream);
+    hipFree(m->rows);
+    m->rows = nullptr;
+    m->nrows = 0;
+  }
+  size_t b = nrows * (size_t)stride * 4;
+  if (m->ok(hipMalloc(&m->rows, b), "rows")) return -1;
+  if (m->ok(hipMemcpyAsync(m->rows, rows_host, b, hipMemcpyHostToDevice, m->stream), "rows") ||
+      m->ok(hipStreamSynchronize(m->stream), "rows"))
+    return -1;
+  m->nrows = nrows;
+  m->stride = stride;
+  return 0;
+}
+
 extern "C" int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, int stride, int steps, int batch, float lr,
                              unsigned seed, int quant, float* mean_loss) {
   if (!m) return -1;
+  if (!rows_dev) {
+    if (!m->rows) return m->fail("train: no device rows given and none loaded");
+    rows_dev = m->rows;
+    nrows = m->nrows;
+    stride = m->stride;
+  }
   if (!rows_dev || nrows == 0 || stride < MLP_NIN + MLP_NOUT || batch < 1 || steps < 1)
     return m->fail("train: need rows (stride >= %d), batch >= 1, steps >= 1", MLP_NIN + MLP_NOUT);
   MlpTrainArgs a;

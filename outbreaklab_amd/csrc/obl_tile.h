@@ -97,6 +97,51 @@ struct Ring {
   }
 };
 
+typedef __attribute__((address_space(3))) float lds_float;
+
+// One z-plane of a (TX+2) x (TY+2) tile in LDS, for kernels that run one block per plane tile
+// (a 3-D grid) and read x / y neighbours many times but z neighbours rarely. Contract: in-plane
+// reads stay within one point of the block's tile (the halo), so whether a read hits LDS depends
+// only on its z, which the compiler resolves statically at every call site: in-plane reads are
+// LDS reads, the others clamped global reads.
+struct Plane {
+  lds_float* s;      // LDS, [4 channels][TPH][TPW]
+  const float4* g;
+  int W, H, D;
+  int x0, y0, zc;
+  __device__ __forceinline__ bool in(int, int, int k) const { return k == zc; }
+  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
+    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  }
+  __device__ __forceinline__ int sidx(int i, int j, int, int ch) const {
+    return ch * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
+    if (in(i, j, k)) return s[sidx(i, j, k, ch)];
+    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
+  }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
+    if (in(i, j, k)) {
+      int b = sidx(i, j, k, 0);
+      return make_float4(s[b], s[b + TPS], s[b + 2 * TPS], s[b + 3 * TPS]);
+    }
+    return g[gidx(i, j, k)];
+  }
+  // cooperative load of the plane (clamped texels in the halo); the caller synchronises
+  __device__ __forceinline__ void load(int tid) {
+    int kk = clampi(zc, 0, D - 1);
+    for (int t = tid; t < TPS; t += TILE_THREADS) {
+      int ti = t % TPW, tj = t / TPW;
+      float4 v = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
+                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
+      s[t] = v.x;
+      s[TPS + t] = v.y;
+      s[2 * TPS + t] = v.z;
+      s[3 * TPS + t] = v.w;
+    }
+  }
+};
+
 // The same interface over HBM only (clamped reads, no window): for fields a kernel reads at a
 // few points, and for the fallback path.
 struct GAcc {

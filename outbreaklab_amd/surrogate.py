@@ -73,9 +73,17 @@ class Surrogate:
         s = np.where(s > 0, s, 1.0)
         self.set_scales(s, rows[:, :NIN].min(0), rows[:, :NIN].max(0))
 
-    def train(self, rows_dev_ptr, nrows, steps, batch=4096, lr=1e-3, seed=1, quant=True, stride=ROW):
+    def load_rows(self, rows):
+        """Copy host rows (n x stride float32) to device memory owned by the network."""
+        rows = np.ascontiguousarray(rows, np.float32)
+        _lib.check(_lib.lib().obl_mlp_load_rows(self.h, _p(rows), rows.shape[0], rows.shape[1]), self.h, mlp=True)
+
+    def train(self, rows_dev_ptr=None, nrows=0, steps=100, batch=4096, lr=1e-3, seed=1, quant=True, stride=ROW):
+        """Adam steps on device rows (a device pointer, e.g. the model's sample buffer), or on the
+        rows of load_rows when rows_dev_ptr is None. Returns the mean training loss."""
         loss = C.c_float()
-        _lib.check(_lib.lib().obl_mlp_train(self.h, C.c_void_p(rows_dev_ptr), nrows, stride, steps, batch, lr, seed,
+        ptr = None if rows_dev_ptr is None else C.c_void_p(rows_dev_ptr)
+        _lib.check(_lib.lib().obl_mlp_train(self.h, ptr, nrows, stride, steps, batch, lr, seed,
                                             1 if quant else 0, C.byref(loss)), self.h, mlp=True)
         return loss.value
 

@@ -1,0 +1,106 @@
+"""GPU surrogate network (csrc/k_mlp.hip) against the float64 oracle (oracle/mlp_oracle.py).
+
+The oracle's gradients are pinned by finite differences in test_mlp_oracle.py; here the GPU
+forward pass and the GPU gradients (FP32 arithmetic, atomic accumulation in any order) must
+match it to 1e-5 / 1e-4 relative L2 per parameter tensor, and Adam on the GPU must fit a
+smooth synthetic map.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from _util import rel_l2
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def synth_rows(n, seed):
+    """Rows of 24 floats: 14 physical inputs in plausible ranges, 9 target tendencies, level."""
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 24), np.float64)
+    x[:, 0] = rng.uniform(210, 305, n)
+    x[:, 1] = rng.uniform(2e4, 1e5, n)
+    x[:, 2] = rng.uniform(0.3, 1.2, n)
+    for j in range(3, 9):
+        x[:, j] = np.where(rng.random(n) < 0.3, 0.0, 10 ** rng.uniform(-7, -2, n))
+    x[:, 9] = np.where(rng.random(n) < 0.5, 0.0, 10 ** rng.uniform(1, 5, n))
+    x[:, 10] = np.where(rng.random(n) < 0.5, 0.0, 10 ** rng.uniform(1, 5, n))
+    x[:, 11] = rng.normal(0, 5, n)
+    x[:, 12] = 250.0
+    x[:, 13] = 2.0
+    f = M.features(x[:, :14])
+    A = np.random.default_rng(99).standard_normal((M.IN, 9)) / 3
+    x[:, 14:23] = np.tanh(f @ A)
+    x[:, 23] = rng.integers(0, 80, n)
+    return x.astype(np.float32)
+
+
+def oracle_params(p):
+    from outbreaklab_amd.surrogate import unpack
+
+    return unpack(p)
+
+
+def test_forward_and_gradients_match_oracle():
+    from outbreaklab_amd.surrogate import Surrogate, init_params
+
+    rows = synth_rows(300, 3)
+    p = init_params(5)
+    p = p + np.random.default_rng(6).standard_normal(p.size).astype(np.float32) * 0.02
+    s = Surrogate()
+    s.set_params(p)
+    scale = np.linspace(0.5, 2.0, 9).astype(np.float32)
+    s.set_scales(scale)
+    loss, g, y = s.eval(rows, quant=False, grads=True)
+    P = oracle_params(p)
+    f = M.features(rows[:, :14].astype(np.float64))
+    t = np.zeros((rows.shape[0], M.OUT))
+    t[:, :9] = rows[:, 14:23] / scale.astype(np.float32)
+    yr, _ = M.forward(P, f)
+    lr, gr = M.loss_and_grads(P, f, t)
+    print(f"forward rel {rel_l2(y, yr):.3e}  loss {loss:.6e} vs {lr:.6e}")
+    assert rel_l2(y, yr) <= 1e-5
+    assert abs(loss - lr) <= 1e-5 * lr
+    G = oracle_params(g)
+    for k in ["W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4"]:
+        e = rel_l2(G[k], gr[k]) if np.any(gr[k]) else np.abs(G[k]).max()
+        print(f"grad {k}: rel {e:.3e}")
+        assert e <= 1e-4, k
+    s.destroy()
+
+
+def test_quantised_forward_is_close_to_fp32():
+    """BF16 rounding of weights, features and activations (as inference does) moves the
+    outputs by about the BF16 ulp times the depth, not more."""
+    from outbreaklab_amd.surrogate import Surrogate, init_params
+
+    rows = synth_rows(256, 4)
+    s = Surrogate()
+    s.set_params(init_params(7))
+    _, _, y0 = s.eval(rows, quant=False)
+    _, _, y1 = s.eval(rows, quant=True)
+    e = rel_l2(y1[:, :9], y0[:, :9])
+    print(f"bf16 forward vs fp32: rel {e:.3e}")
+    assert 1e-5 < e <= 2e-2
+    s.destroy()
+
+
+def test_adam_fits_smooth_map():
+    from outbreaklab_amd.surrogate import Surrogate
+
+    rows = synth_rows(20000, 8)
+    s = Surrogate(seed=2)
+    s.set_scales(np.ones(9, np.float32))
+    s.load_rows(rows)
+    l0, _, _ = s.eval(rows[:2000], quant=True)
+    for i in range(4):
+        s.train(steps=250, batch=1024, lr=2e-3, seed=i + 1, quant=True)
+    l1, _, _ = s.eval(rows[:2000], quant=True)
+    print(f"loss {l0:.4e} -> {l1:.4e}")
+    assert l1 < 0.05 * l0
+    s.destroy()
