@@ -295,10 +295,20 @@ __device__ __forceinline__ float sampW(const A& T, float3 p) { return samp1(T, p
 template <class A>
 __device__ __forceinline__ float sampC(const A& T, float3 p) { return samp1(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, 3); }
 
+// The 2 x 2 x 2 texel box of every departure / forward point of a face stays within one point of
+// it while each Courant number of the local velocities is below one (traceV moves a point by
+// k * v; clamping moves it inward). Then every sample falls in an LDS ring's window.
+__device__ __forceinline__ bool near_traces(const Dims& d, const Loc& L) {
+  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  float mx = fmaxf(fmaxf(fabsf(L.vu.x), fabsf(L.vv.x)), fmaxf(fabsf(L.vw.x), fabsf(L.vc.x)));
+  float my = fmaxf(fmaxf(fabsf(L.vu.y), fabsf(L.vv.y)), fmaxf(fabsf(L.vw.y), fabsf(L.vc.y)));
+  float mz = fmaxf(fmaxf(fabsf(L.vu.z), fabsf(L.vv.z)), fmaxf(fabsf(L.vw.z), fabsf(L.vc.z)));
+  return mx * kx < 0.999f && my * kx < 0.999f && mz * kz < 0.999f;
+}
+
 template <class A>
-__device__ __forceinline__ float4 advv1_point(const Dims& d, const A& V, int x, int y, int z) {
+__device__ __forceinline__ float4 advv1_samples(const Dims& d, const A& V, const Loc& L, int x, int y, int z) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  Loc L = localVel(V, x, y, z);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   float4 o;
   o.x = sampU(V, traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
@@ -307,19 +317,27 @@ __device__ __forceinline__ float4 advv1_point(const Dims& d, const A& V, int x, 
   o.w = sampC(V, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
   return o;
 }
+// Neighbour reads (one point away) always hit the ring's window; the samples take the
+// window-guaranteed path when the whole wave's traces are near (wave-uniform branch), the
+// checked path otherwise. Both read the same texels with the same arithmetic.
+__device__ __forceinline__ float4 advv1_point(const Dims& d, const Ring& rv, int x, int y, int z) {
+  RingW w = rv.w();
+  Loc L = localVel(w, x, y, z);
+  if (wave_all(near_traces(d, L))) return advv1_samples(d, w, L, x, y, z);
+  return advv1_samples(d, rv, L, x, y, z);
+}
 
 template <class A>
 __device__ __forceinline__ float2 mmV(const A& T, float3 p, float ox, float oy, float oz, int ch) {
   return minmax1(T, (int)floorf(p.x - ox), (int)floorf(p.y - oy), (int)floorf(p.z - oz), ch);
 }
 template <class A, class B>
-__device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const A& V, const B& Vt, int x, int y, int z) {
+__device__ __forceinline__ float4 advv2_rest(const MU& u, const Dims& d, const A& V, const B& Vt, const Loc& L, float4 hat,
+                                             float4 old, int x, int y, int z) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  Loc L = localVel(V, x, y, z);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
   float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
-  float4 hat = Vt.ld4(x, y, z), old = V.ld4(x, y, z);
   float4 b;
   b.x = sampU(Vt, traceV(d, k, pu, L.vu, -1.0f));
   b.y = sampV(Vt, traceV(d, k, pv, L.vv, -1.0f));
@@ -336,6 +354,14 @@ __device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const 
   m = mmV(V, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = clampf_(r.z, m.x, m.y);
   m = mmV(V, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = clampf_(r.w, m.x, m.y);
   return r;
+}
+__device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const Ring& rv, const Ring& rt, int x, int y,
+                                              int z) {
+  RingW wv = rv.w(), wt = rt.w();
+  Loc L = localVel(wv, x, y, z);
+  float4 hat = wt.ld4(x, y, z), old = wv.ld4(x, y, z);
+  if (wave_all(near_traces(d, L))) return advv2_rest(u, d, wv, wt, L, hat, old, x, y, z);
+  return advv2_rest(u, d, rv, rt, L, hat, old, x, y, z);
 }
 
 // ====================== advection of moisture (model.js:482-576) ======================
@@ -360,56 +386,73 @@ __device__ __forceinline__ float2 mmQ1(const A& T, float3 p, int ch) {
   return minmax1(T, (int)floorf(p.x - 0.5f), (int)floorf(p.y - 0.5f), (int)floorf(p.z - 0.5f), ch);
 }
 
-template <class AV, class AQ>
-__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, const AV& V, const AQ& Q1,
-                                            const AQ& Q2, int x, int y, int z, float4& o1, float4& o2) {
-  bool thm = u.v[U_MICRO2].z > 0.5f;
-  float rsfc = u.v[U_STEP].z;
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float3 v = cellVel(V, x, y, z);
-  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  float4 q1 = Q1.ld4(x, y, z), q2 = Q2.ld4(x, y, z);
-  float rho = bsA(BS, z, d.nz).z;
+// Fall speeds of one cell (model.js:495-500): rain mass- and number-weighted, snow, graupel.
+struct Fall {
+  float r, n, s, g;  // n: number-weighted rain (Thompson), else 0
+};
+__device__ __forceinline__ Fall fall_speeds(bool thm, float4 q1, float4 q2, float rho, float rsfc) {
   float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
-  float fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
+  return Fall{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rsfc), vtGrau(thm, q2.y, rho, rsfc)};
+}
+// as near_traces, for cell-centred traces with per-species fall speeds: |k_z (w - f)| is largest
+// at the extreme f, so the bound over f in [min(0, f...), max(0, f...)] covers every species
+__device__ __forceinline__ bool near_traces_q(const Dims& d, float3 v, const Fall& f) {
+  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  float fmin = fminf(fminf(0.0f, fminf(f.r, f.n)), fminf(f.s, f.g));
+  float fmax = fmaxf(fmaxf(0.0f, fmaxf(f.r, f.n)), fmaxf(f.s, f.g));
+  return fabsf(v.x) * kx < 0.999f && fabsf(v.y) * kx < 0.999f && fabsf(v.z - fmin) * kz < 0.999f &&
+         fabsf(v.z - fmax) * kz < 0.999f;
+}
+
+template <class AQ>
+__device__ __forceinline__ void advq1_samples(const Dims& d, bool thm, const AQ& Q1, const AQ& Q2, float3 v, const Fall& f,
+                                              int x, int y, int z, float4& o1, float4& o2) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
   float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
   float a0 = samp1(Q1, tx, ty, tz, 0), a1 = samp1(Q1, tx, ty, tz, 1), a2 = samp1(Q1, tx, ty, tz, 2);
-  float r = sampQ1(Q1, traceQ(d, k, p, v, fv.x, 1.0f), 3);
+  float r = sampQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3);
   float bw = samp1(Q2, tx, ty, tz, 3);
-  float s = sampQ1(Q2, traceQ(d, k, p, v, fs, 1.0f), 0);
-  float g = sampQ1(Q2, traceQ(d, k, p, v, fg, 1.0f), 1);
-  float nr = thm ? sampQ1(Q2, traceQ(d, k, p, v, fv.y, 1.0f), 2) : samp1(Q2, tx, ty, tz, 2);
+  float s = sampQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0);
+  float g = sampQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1);
+  float nr = thm ? sampQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2) : samp1(Q2, tx, ty, tz, 2);
   o1 = make_float4(a0, a1, a2, r);
   o2 = make_float4(s, g, nr, bw);
 }
-
-template <class AV, class AQ>
-__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, const AV& V, const AQ& Q1,
-                                            const AQ& Q2, const AQ& Q1t, const AQ& Q2t, int x, int y, int z, float4& o1,
-                                            float4& o2) {
+__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, const GAcc& V, const Ring& Q1,
+                                            const Ring& Q2, int x, int y, int z, float4& o1, float4& o2) {
   bool thm = u.v[U_MICRO2].z > 0.5f;
-  float rsfc = u.v[U_STEP].z;
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  RingW w1 = Q1.w(), w2 = Q2.w();
   float3 v = cellVel(V, x, y, z);
+  float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
+  Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
+  if (wave_all(near_traces_q(d, v, f))) {
+    advq1_samples(d, thm, w1, w2, v, f, x, y, z, o1, o2);
+  } else {
+    // a far trace somewhere in the wave: every sample from global memory (same texels)
+    GAcc g1{Q1.g, Q1.W, Q1.H, Q1.D, nullptr}, g2{Q2.g, Q2.W, Q2.H, Q2.D, nullptr};
+    advq1_samples(d, thm, g1, g2, v, f, x, y, z, o1, o2);
+  }
+}
+
+template <class AQ>
+__device__ __forceinline__ void advq2_rest(const MU& u, const Dims& d, const AQ& Q1, const AQ& Q2, const AQ& Q1t,
+                                           const AQ& Q2t, float3 v, const Fall& f, float4 q1, float4 q2, float4 h1,
+                                           float4 h2, int x, int y, int z, float4& o1, float4& o2) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  float4 q1 = Q1.ld4(x, y, z), q2 = Q2.ld4(x, y, z);
-  float rho = bsA(BS, z, d.nz).z;
-  float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
-  float fr = fv.x, fs = vtS(q2.x, rho, rsfc), fg = vtGrau(thm, q2.y, rho, rsfc);
-  float4 h1 = Q1t.ld4(x, y, z), h2 = Q2t.ld4(x, y, z);
   float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  float fn = thm ? fv.y : 0.0f;
   float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f);
   float tx = f0.x - 0.5f, ty = f0.y - 0.5f, tz = f0.z - 0.5f;
   float b1x = samp1(Q1t, tx, ty, tz, 0), b1y = samp1(Q1t, tx, ty, tz, 1), b1z = samp1(Q1t, tx, ty, tz, 2);
-  float br = sampQ1(Q1t, traceQ(d, k, p, v, fr, -1.0f), 3);
+  float br = sampQ1(Q1t, traceQ(d, k, p, v, f.r, -1.0f), 3);
   float b2w = samp1(Q2t, tx, ty, tz, 3);
-  float bs = sampQ1(Q2t, traceQ(d, k, p, v, fs, -1.0f), 0);
-  float bg = sampQ1(Q2t, traceQ(d, k, p, v, fg, -1.0f), 1);
-  float bn = sampQ1(Q2t, traceQ(d, k, p, v, fn, -1.0f), 2);
+  float bs = sampQ1(Q2t, traceQ(d, k, p, v, f.s, -1.0f), 0);
+  float bg = sampQ1(Q2t, traceQ(d, k, p, v, f.g, -1.0f), 1);
+  float bn = sampQ1(Q2t, traceQ(d, k, p, v, f.n, -1.0f), 2);
   float4 r1 = make_float4(h1.x + sm * (q1.x - b1x), h1.y + sm * (q1.y - b1y), h1.z + sm * (q1.z - b1z),
                           h1.w + sm * (q1.w - br));
   float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
@@ -419,13 +462,31 @@ __device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const fl
   m = mmQ1(Q1, d0, 0); r1.x = clampf_(r1.x, m.x, m.y);
   m = mmQ1(Q1, d0, 1); r1.y = clampf_(r1.y, m.x, m.y);
   m = mmQ1(Q1, d0, 2); r1.z = clampf_(r1.z, m.x, m.y);
-  m = mmQ1(Q1, traceQ(d, k, p, v, fr, 1.0f), 3); r1.w = clampf_(r1.w, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, fs, 1.0f), 0); r2.x = clampf_(r2.x, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, fg, 1.0f), 1); r2.y = clampf_(r2.y, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, fn, 1.0f), 2); r2.z = clampf_(r2.z, m.x, m.y);
+  m = mmQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3); r1.w = clampf_(r1.w, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0); r2.x = clampf_(r2.x, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1); r2.y = clampf_(r2.y, m.x, m.y);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2); r2.z = clampf_(r2.z, m.x, m.y);
   m = mmQ1(Q2, d0, 3); r2.w = clampf_(r2.w, m.x, m.y);
   o1 = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
   o2 = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
+}
+__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, const GAcc& V, const Ring& Q1,
+                                            const Ring& Q2, const Ring& Q1t, const Ring& Q2t, int x, int y, int z,
+                                            float4& o1, float4& o2) {
+  bool thm = u.v[U_MICRO2].z > 0.5f;
+  RingW w1 = Q1.w(), w2 = Q2.w(), w1t = Q1t.w(), w2t = Q2t.w();
+  float3 v = cellVel(V, x, y, z);
+  float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
+  float4 h1 = w1t.ld4(x, y, z), h2 = w2t.ld4(x, y, z);
+  Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
+  if (wave_all(near_traces_q(d, v, f))) {
+    advq2_rest(u, d, w1, w2, w1t, w2t, v, f, q1, q2, h1, h2, x, y, z, o1, o2);
+  } else {
+    // a far trace somewhere in the wave: every sample from global memory (same texels)
+    GAcc g1{Q1.g, Q1.W, Q1.H, Q1.D, nullptr}, g2{Q2.g, Q2.W, Q2.H, Q2.D, nullptr};
+    GAcc g1t{Q1t.g, Q1t.W, Q1t.H, Q1t.D, nullptr}, g2t{Q2t.g, Q2t.W, Q2t.H, Q2t.D, nullptr};
+    advq2_rest(u, d, g1, g2, g1t, g2t, v, f, q1, q2, h1, h2, x, y, z, o1, o2);
+  }
 }
 
 // ====================== forces + microphysics (model.js:579-907) ======================
@@ -783,8 +844,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __re
     Ring::Pre pv;
     const bool more = z + 1 < d.nz;
     if (more) rv.fetch(z + 2, tid, pv);
-    rv.zc = z;
-    if (act) aux[cidx(d, x, y, z)] = nut_point(u, d, BS, rv, Q1, x, y, z);
+    rv.center(z);
+    if (act) aux[cidx(d, x, y, z)] = nut_point(u, d, BS, rv.w(), Q1, x, y, z);
     __syncthreads();
     if (more) rv.store(z + 2, tid, pv);
     __syncthreads();
@@ -802,7 +863,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advv1_t(MU u, const float4* __
     Ring::Pre pv;
     const bool more = z < d.nz;
     if (more) rv.fetch(z + 2, tid, pv);
-    rv.zc = z;
+    rv.center(z);
     if (act) Vt[vidx(d, x, y, z)] = advv1_point(d, rv, x, y, z);
     __syncthreads();
     if (more) rv.store(z + 2, tid, pv);
@@ -823,7 +884,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advv2_t(MU u, const float4* __
     Ring::Pre pv, pt;
     const bool more = z < d.nz;
     if (more) { rv.fetch(z + 2, tid, pv); rt.fetch(z + 2, tid, pt); }
-    rv.zc = z; rt.zc = z;
+    rv.center(z); rt.center(z);
     if (act) Vout[vidx(d, x, y, z)] = advv2_point(u, d, rv, rt, x, y, z);
     __syncthreads();
     if (more) { rv.store(z + 2, tid, pv); rt.store(z + 2, tid, pt); }
@@ -846,7 +907,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq1_t(MU u, const float4* __
     Ring::Pre p1, p2;
     const bool more = z + 1 < d.nz;
     if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); }
-    r1.zc = z; r2.zc = z;
+    r1.center(z); r2.center(z);
     if (act) {
       float4 o1, o2;
       advq1_point(u, d, BS, gv, r1, r2, x, y, z, o1, o2);
@@ -878,7 +939,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
     Ring::Pre p1, p2, p3, p4;
     const bool more = z + 1 < d.nz;
     if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4); }
-    r1.zc = z; r2.zc = z; r3.zc = z; r4.zc = z;
+    r1.center(z); r2.center(z); r3.center(z); r4.center(z);
     if (act) {
       float4 o1, o2;
       advq2_point(u, d, BS, gv, r1, r2, r3, r4, x, y, z, o1, o2);

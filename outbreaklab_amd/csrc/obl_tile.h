@@ -21,13 +21,45 @@ constexpr int TX = 32, TY = 8, TPW = TX + 2, TPH = TY + 2, TPS = TPW * TPH;
 constexpr int TILE_THREADS = TX * TY;
 constexpr int RING_FLOATS = 3 * 4 * TPS;  // 3 planes x 4 channels
 
+typedef __attribute__((address_space(3))) float lds_float;
+
+// Reads of a RingW view are known to fall in the ring's window: no window tests, no fallback,
+// and the plane offsets of k-1, k, k+1 are selects rather than a modulo.
+struct RingW {
+  const lds_float* s;
+  int x0, y0, zc;
+  int om, oc, op;  // float offsets of planes zc-1, zc, zc+1
+  __device__ __forceinline__ int plane(int k) const { return k < zc ? om : (k == zc ? oc : op); }
+  __device__ __forceinline__ int sidx(int i, int j, int k, int ch) const {
+    return plane(k) + ch * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return s[sidx(i, j, k, ch)]; }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
+    int b = sidx(i, j, k, 0);
+    return make_float4(s[b], s[b + TPS], s[b + 2 * TPS], s[b + 3 * TPS]);
+  }
+};
+
+// all lanes of the wave agree (wave-uniform branch condition)
+__device__ __forceinline__ bool wave_all(bool p) { return __all((int)p) != 0; }
+
 struct Ring {
   float* s;          // LDS, [3 planes][4 channels][TPH][TPW]
   const float4* g;   // the field in HBM
   int W, H, D;       // texel extents: W x H x D
   int x0, y0, zc;    // tile origin (first interior point) and the centre plane
 
+  int om = 0, oc = 0, op = 0;  // float offsets of planes zc-1, zc, zc+1 (set by center)
+
   __device__ __forceinline__ static int slot(int k) { return (k + 3 * 4096) % 3; }
+  __device__ __forceinline__ void center(int k) {
+    zc = k;
+    om = slot(k - 1) * 4 * TPS;
+    oc = slot(k) * 4 * TPS;
+    op = slot(k + 1) * 4 * TPS;
+  }
+  // view for reads known to lie within one point of the tile and of plane zc
+  __device__ __forceinline__ RingW w() const { return RingW{(const lds_float*)s, x0, y0, zc, om, oc, op}; }
   __device__ __forceinline__ bool in(int i, int j, int k) const {
     return (unsigned)(i - x0 + 1) < (unsigned)TPW && (unsigned)(j - y0 + 1) < (unsigned)TPH &&
            (unsigned)(k - zc + 1) < 3u;
@@ -96,8 +128,6 @@ struct Ring {
     }
   }
 };
-
-typedef __attribute__((address_space(3))) float lds_float;
 
 // One z-plane of a (TX+2) x (TY+2) tile in LDS, for kernels that run one block per plane tile
 // (a 3-D grid) and read x / y neighbours many times but z neighbours rarely. Contract: in-plane
@@ -186,6 +216,32 @@ __device__ __noinline__ static float2 minmax1_global(const float4* g4, int W, in
     hi = fmaxf(hi, s);
   }
   return make_float2(lo, hi);
+}
+
+// Window-guaranteed forms: the 2 x 2 x 2 texel box of the sample lies in the window, so
+// k0 is zc - 1 or zc.
+__device__ __forceinline__ float samp1(const RingW& a, float tx, float ty, float tz, int ch) {
+  float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+  float ax = tx - fx, ay = ty - fy, az = tz - fz;
+  int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+  bool lo = k0 < a.zc;
+  int b = (lo ? a.om : a.oc) + ch * TPS + (j0 - a.y0 + 1) * TPW + (i0 - a.x0 + 1);
+  int dz = lo ? a.oc - a.om : a.op - a.oc;
+  float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + TPW], c110 = a.s[b + TPW + 1];
+  float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + TPW], c111 = a.s[b + dz + TPW + 1];
+  float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
+  float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+__device__ __forceinline__ float2 minmax1(const RingW& a, int i0, int j0, int k0, int ch) {
+  bool lo = k0 < a.zc;
+  int b = (lo ? a.om : a.oc) + ch * TPS + (j0 - a.y0 + 1) * TPW + (i0 - a.x0 + 1);
+  int dz = lo ? a.oc - a.om : a.op - a.oc;
+  float v0 = a.s[b], v1 = a.s[b + 1], v2 = a.s[b + TPW], v3 = a.s[b + TPW + 1];
+  float v4 = a.s[b + dz], v5 = a.s[b + dz + 1], v6 = a.s[b + dz + TPW], v7 = a.s[b + dz + TPW + 1];
+  float lo_ = fminf(fminf(fminf(v0, v1), fminf(v2, v3)), fminf(fminf(v4, v5), fminf(v6, v7)));
+  float hi_ = fmaxf(fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)), fmaxf(fmaxf(v4, v5), fmaxf(v6, v7)));
+  return make_float2(lo_, hi_);
 }
 
 // Trilinear, clamp-to-edge sample of one channel at texel coordinates (t - 1/2 convention
