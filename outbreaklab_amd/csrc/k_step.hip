@@ -162,15 +162,29 @@ __device__ __forceinline__ float tgtQv(const float4* BS, const Dims& d, float4 A
   float z = ((float)k + 0.5f) * d.dz;
   return bsA(BS, k, d.nz).y * (1.0f - A.z * tprof(z, A.w));
 }
-__device__ __forceinline__ float tgtU(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+// 2-D map reads for the target wind: a functor (i, j) -> clamped texel
+struct G2d {
+  const float4* m;
+  Dims d;
+  __device__ __forceinline__ float4 operator()(int i, int j) const { return ld2d(m, d, i, j); }
+};
+template <class B2>
+__device__ __forceinline__ float tgtU(const float4* BS, const B2& lb, const Dims& d, int x, int y, int z) {
   float zz = ((float)z + 0.5f) * d.dz;
-  float4 a = ld2d(lsfB, d, x - 1, y), b = ld2d(lsfB, d, x, y);
+  float4 a = lb(x - 1, y), b = lb(x, y);
   return bsB(BS, z, d.nz).z + 0.5f * (a.x * tprof(zz, a.z) + b.x * tprof(zz, b.z));
 }
-__device__ __forceinline__ float tgtV(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+template <class B2>
+__device__ __forceinline__ float tgtV(const float4* BS, const B2& lb, const Dims& d, int x, int y, int z) {
   float zz = ((float)z + 0.5f) * d.dz;
-  float4 a = ld2d(lsfB, d, x, y - 1), b = ld2d(lsfB, d, x, y);
+  float4 a = lb(x, y - 1), b = lb(x, y);
   return bsB(BS, z, d.nz).w + 0.5f * (a.y * tprof(zz, a.z) + b.y * tprof(zz, b.z));
+}
+__device__ __forceinline__ float tgtU(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+  return tgtU(BS, G2d{lsfB, d}, d, x, y, z);
+}
+__device__ __forceinline__ float tgtV(const float4* BS, const float4* lsfB, const Dims& d, int x, int y, int z) {
+  return tgtV(BS, G2d{lsfB, d}, d, x, y, z);
 }
 
 // ====================== INIT (model.js:284-339), outbreak mode ======================
@@ -667,11 +681,63 @@ __device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, flo
   T += OBL_LFC / OBL_CPD * (qi - qi0 - f * dsum);
 }
 
-template <bool DUMP, class AV, class AQ>
+// Accessors of phys_point's cell fields and 2-D map: global (clamped) reads, or reads of a block's
+// LDS plane tile for the points of its own plane (the rest from global memory). The tile
+// loader applies the same clamping, so both give the same values.
+struct GAux {
+  const float2* p;
+  Dims d;
+  __device__ __forceinline__ float2 operator()(int x, int y, int z) const { return ldA(p, d, x, y, z); }
+};
+struct GPh {
+  const float* p;
+  Dims d;
+  __device__ __forceinline__ float operator()(int x, int y, int z) const { return ldP(p, d, x, y, z); }
+};
+struct TAux {
+  const lds_float* s;  // [2][TPS]
+  const float2* g;
+  Dims d;
+  int x0, y0, zc;
+  __device__ __forceinline__ float2 operator()(int x, int y, int z) const {
+    if (z != zc) return ldA(g, d, x, y, z);
+    int i = (y - y0 + 1) * TPW + (x - x0 + 1);
+    return make_float2(s[i], s[TPS + i]);
+  }
+};
+struct TPh {
+  const lds_float* s;  // [TPS]
+  const float* g;
+  Dims d;
+  int x0, y0, zc;
+  __device__ __forceinline__ float operator()(int x, int y, int z) const {
+    if (z != zc) return ldP(g, d, x, y, z);
+    return s[(y - y0 + 1) * TPW + (x - x0 + 1)];
+  }
+};
+struct T2d {
+  const lds_float* s;  // [4][TPS]
+  int x0, y0;
+  __device__ __forceinline__ float4 operator()(int i, int j) const {
+    int b = (j - y0 + 1) * TPW + (i - x0 + 1);
+    return make_float4(s[b], s[TPS + b], s[2 * TPS + b], s[3 * TPS + b]);
+  }
+};
+// the three texels of one column at z - 1, z, z + 1 (for the vertical stencils)
+struct ZCol {
+  float4 m, c, p;
+  int z;
+  __device__ __forceinline__ float ldc(int, int, int k, int ch) const {
+    float4 v = k < z ? m : (k > z ? p : c);
+    return ch == 0 ? v.x : (ch == 1 ? v.y : (ch == 2 ? v.z : v.w));
+  }
+};
+
+template <bool DUMP, class AV, class AQ, class AX, class AP, class AB>
 __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const float4* __restrict__ BS, const AV& Vin,
-                                           const AQ& Q1in, const float4* __restrict__ Q2in, const float2* __restrict__ aux,
-                                           const float4* __restrict__ lsfA, const float4* __restrict__ lsfB,
-                                           const float* __restrict__ phT, const float4* __restrict__ ciMap,
+                                           const AQ& Q1in, const float4* __restrict__ Q2in, const AX& aux,
+                                           const float4* __restrict__ lsfA, const AB& lsfB,
+                                           const AP& phT, const float4* __restrict__ ciMap,
                                            float4* __restrict__ Vout, float4* __restrict__ Q1o, float4* __restrict__ Q2o,
                                            const MicroDump& dump, int x, int y, int z) {
   int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -683,19 +749,21 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   bool drag = u.v[U_TURB].w > 0.5f;
   size_t vi = vidx(d, x, y, z);
   float4 V = Vin.ld4(x, y, z), o = V;
+  // the column's z neighbours, read once as whole texels (clamped)
+  const ZCol zcol{Vin.ld4(x, y, z - 1), V, Vin.ld4(x, y, z + 1), z};
   // ---- u ----
   if (y < ny && z < nz) {
     float xmx = (float)x * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
-    float2 a0 = ldA(aux, d, x - 1, y, z), a1 = ldA(aux, d, x, y, z);
+    float2 a0 = aux(x - 1, y, z), a1 = aux(x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float du = nux * lapH(Vin, x, y, z, 0, V.x) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 0, V.x, z == 0, z == nz - 1) * iz2;
+    float du = nux * lapH(Vin, x, y, z, 0, V.x) * ix2 + nuy * lapZp(zcol, BS, d, x, y, z, 0, V.x, z == 0, z == nz - 1) * iz2;
     float ut = tgtU(BS, lsfB, d, x, y, z);
     float vbar = 0.25f * (Vin.ldc(x - 1, y, z, 1) + V.y + Vin.ldc(x - 1, y + 1, z, 1) + Vin.ldc(x, y + 1, z, 1));
     float vtb = 0.5f * (tgtV(BS, lsfB, d, x - 1, y, z) + tgtV(BS, lsfB, d, x, y + 1, z));
-    float nw = 0.5f * (ld2d(lsfB, d, x - 1, y).w + ld2d(lsfB, d, x, y).w);
+    float nw = 0.5f * (lsfB(x - 1, y).w + lsfB(x, y).w);
     du += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (ut - V.x);
     du += fc * (vbar - vtb);
-    if (x > 0 && x < nx) du -= (ldP(phT, d, x, y, z) - ldP(phT, d, x - 1, y, z)) / dx;
+    if (x > 0 && x < nx) du -= (phT(x, y, z) - phT(x - 1, y, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(V.x * V.x + vbar * vbar), lg0 = sqrtf(ut * ut + vtb * vtb);
       du += u.v[U_MICRO].w * (-lg * V.x + lg0 * ut) / dz;
@@ -707,16 +775,16 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   // ---- v ----
   if (x < nx && z < nz) {
     float xmx = ((float)x + 0.5f) * dx, xmy = (float)y * dx, zz = ((float)z + 0.5f) * dz;
-    float2 a0 = ldA(aux, d, x, y - 1, z), a1 = ldA(aux, d, x, y, z);
+    float2 a0 = aux(x, y - 1, z), a1 = aux(x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float dv = nux * lapH(Vin, x, y, z, 1, V.y) * ix2 + nuy * lapZp(Vin, BS, d, x, y, z, 1, V.y, z == 0, z == nz - 1) * iz2;
+    float dv = nux * lapH(Vin, x, y, z, 1, V.y) * ix2 + nuy * lapZp(zcol, BS, d, x, y, z, 1, V.y, z == 0, z == nz - 1) * iz2;
     float vt = tgtV(BS, lsfB, d, x, y, z);
     float ubar = 0.25f * (Vin.ldc(x, y - 1, z, 0) + V.x + Vin.ldc(x + 1, y - 1, z, 0) + Vin.ldc(x + 1, y, z, 0));
     float utb = 0.5f * (tgtU(BS, lsfB, d, x, y - 1, z) + tgtU(BS, lsfB, d, x + 1, y, z));
-    float nw = 0.5f * (ld2d(lsfB, d, x, y - 1).w + ld2d(lsfB, d, x, y).w);
+    float nw = 0.5f * (lsfB(x, y - 1).w + lsfB(x, y).w);
     dv += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (vt - V.y);
     dv -= fc * (ubar - utb);
-    if (y > 0 && y < ny) dv -= (ldP(phT, d, x, y, z) - ldP(phT, d, x, y - 1, z)) / dx;
+    if (y > 0 && y < ny) dv -= (phT(x, y, z) - phT(x, y - 1, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(ubar * ubar + V.y * V.y), lg0 = sqrtf(utb * utb + vt * vt);
       dv += u.v[U_MICRO].w * (-lg * V.y + lg0 * vt) / dz;
@@ -728,12 +796,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   // ---- w ----
   if (x < nx && y < ny && z > 0 && z < nz) {
     float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = (float)z * dz;
-    float2 a0 = ldA(aux, d, x, y, z - 1), a1 = ldA(aux, d, x, y, z);
+    float2 a0 = aux(x, y, z - 1), a1 = aux(x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
-    float lz = Vin.ldc(x, y, z - 1, 2) + Vin.ldc(x, y, z + 1, 2) - 2.0f * V.z;
+    float lz = zcol.m.z + zcol.p.z - 2.0f * V.z;
     float dw = nux * lapH(Vin, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
-    dw += 0.5f * (buoy(Vin, Q1in, Q2in, BS, d, x, y, z - 1) + buoy(Vin, Q1in, Q2in, BS, d, x, y, z)) -
-          (ldP(phT, d, x, y, z) - ldP(phT, d, x, y, z - 1)) / dz;
+    dw += 0.5f * (buoy(zcol, Q1in, Q2in, BS, d, x, y, z - 1) + buoy(zcol, Q1in, Q2in, BS, d, x, y, z)) -
+          (phT(x, y, z) - phT(x, y, z - 1)) / dz;
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
     float4 ci = ciMap[(size_t)x + (size_t)nx * y];
     if (ci.x > 0.0f) {
@@ -752,14 +820,14 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
     float4 A = bsA(BS, z, nz), B = bsB(BS, z, nz);
     float4 LA = ld2d(lsfA, d, x, y);
-    float nw = ld2d(lsfB, d, x, y).w;
+    float nw = lsfB(x, y).w;
     float tth = tgtTh(BS, d, LA, z), tqv = tgtQv(BS, d, LA, z);
     float sl = spongeLat(u, d, xmx, xmy);
     float relax = sl + nw * nudR;
-    float2 a1 = ldA(aux, d, x, y, z);
+    float2 a1 = aux(x, y, z);
     float nux = a1.x * 3.0f + nu0, nuy = a1.y * 3.0f + nu0;
     float th = V.w + dt * (nux * lapH(Vin, x, y, z, 3, V.w) * ix2 +
-                           nuy * lapZp(Vin, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
+                           nuy * lapZp(zcol, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
                            (relax + spongeTop(u, d, zz)) * (tth - V.w));
     size_t ci_ = cidx(d, x, y, z);
     float4 q1 = Q1in.ld4(x, y, z), q2 = Q2in[ci_];
@@ -790,7 +858,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
       if (r < (active ? dump.p_active : dump.p_clear)) {
         unsigned slot = atomicAdd(dump.count, 1u);
         if (slot < dump.cap) {
-          float wc = 0.5f * (V.z + Vin.ldc(x, y, z + 1, 2));
+          float wc = 0.5f * (V.z + zcol.p.z);
           float* row = dump.buf + (size_t)slot * OBL_DUMP_STRIDE;
           float in[14] = {T0, B.y, A.z, qv0, qc0, qr0, qi0, qs0, qg0, nr0, 0.0f, wc, dz, dt};
           float out[10] = {T - T0, qv - qv0, qc - qc0, qi - qi0, qr - qr0, qs - qs0, qg - qg0, nr - nr0, 0.0f, (float)z};
@@ -965,7 +1033,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
                                                        const float4* __restrict__ lsfB, const float* __restrict__ phT,
                                                        const float4* __restrict__ ciMap, float4* __restrict__ Vout,
                                                        float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
-  __shared__ float sV[4 * TPS], sQ[4 * TPS];
+  __shared__ float sV[4 * TPS], sQ[4 * TPS], sA[2 * TPS], sP[TPS], sB[4 * TPS];
   Dims d = dims(u);
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z = blockIdx.z;
@@ -973,10 +1041,30 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
   Plane pq{(lds_float*)sQ, Q1in, d.nx, d.ny, d.nz, x0, y0, z};
   pv.load(tid);
   pq.load(tid);
+  {
+    // aux, phT (cells, this plane) and lsfB (2-D), clamped like ldA / ldP / ld2d
+    const int zz = clampi(z, 0, d.nz - 1);
+    for (int t = tid; t < TPS; t += TILE_THREADS) {
+      int i = clampi(x0 - 1 + t % TPW, 0, d.nx - 1), j = clampi(y0 - 1 + t / TPW, 0, d.ny - 1);
+      size_t c = (size_t)i + (size_t)d.nx * ((size_t)j + (size_t)d.ny * (size_t)zz);
+      float2 a = aux[c];
+      sA[t] = a.x;
+      sA[TPS + t] = a.y;
+      sP[t] = phT[c];
+      float4 b = lsfB[(size_t)i + (size_t)d.nx * (size_t)j];
+      sB[t] = b.x;
+      sB[TPS + t] = b.y;
+      sB[2 * TPS + t] = b.z;
+      sB[3 * TPS + t] = b.w;
+    }
+  }
   __syncthreads();
   const int x = x0 + (tid % TX), y = y0 + (tid / TX);
   if (x > d.nx || y > d.ny || z > d.nz) return;
-  phys_point<DUMP>(u, d, BS, pv, pq, Q2in, aux, lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
+  TAux ta{(const lds_float*)sA, aux, d, x0, y0, z};
+  TPh tp{(const lds_float*)sP, phT, d, x0, y0, z};
+  T2d tb{(const lds_float*)sB, x0, y0};
+  phys_point<DUMP>(u, d, BS, pv, pq, Q2in, ta, lsfA, tb, tp, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
 }
 
 // ====================== DIV (model.js:910-925) ======================
