@@ -68,6 +68,8 @@ __device__ __forceinline__ void dst_tile(int& mt, int& nt, int& plane) {
 
 template <int MODE, int PREC>
 __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
+  // one LDS stage; the global loads of stage s + 1 go to registers before the MFMAs of
+  // stage s, so their latency overlaps this wave's own MFMAs
   __shared__ float As[2][BM][LDK];
   __shared__ float Bs[2][BN][LDK];
   const int tid = threadIdx.x;
@@ -79,6 +81,7 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
   const int m0 = mt * BM, n0 = nt * BN;
   const int N = a.N, Nh = a.Nh;
   const size_t planeSz = (size_t)N * (size_t)a.inner;
+  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
 
   v4f acc[2][2][2];
 #pragma unroll
@@ -88,55 +91,64 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
 #pragma unroll
       for (int j = 0; j < 2; j++) acc[p][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < Nh; k0 += BK) {
-    // ---------------- global -> LDS ----------------
-    if (MODE == DST_XF || MODE == DST_XI) {
-      // A: data rows (K contiguous); B: matrix rows
-      int row = tid >> 2, q = tid & 3;
+  float4 d0, d1, w0, w1;
+  const int row = tid >> 2, q = tid & 3;   // matrix rows (and X-mode data rows)
+  const int kk = tid >> 4, qq = tid & 15;  // Y-mode data: k row, 4 columns
+  auto fetch = [&](int k0) {
+    const size_t wo = (size_t)((XMODE ? n0 : m0) + row) * a.Kp + k0 + 4 * q;
+    w0 = *reinterpret_cast<const float4*>(a.W0 + wo);
+    w1 = *reinterpret_cast<const float4*>(a.W1 + wo);
+    d0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    d1 = d0;
+    if (XMODE) {
       int r = m0 + row, k = k0 + 4 * q;
-      float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
       if (r < a.rows && k < Nh) {
         const float* src = a.in + (size_t)r * N;
         if (MODE == DST_XF) {
           float4 f = *reinterpret_cast<const float4*>(src + k);
           float4 g = *reinterpret_cast<const float4*>(src + (N - 4 - k));
-          s0 = make_float4(f.x + g.w, f.y + g.z, f.z + g.y, f.w + g.x);
-          s1 = make_float4(f.x - g.w, f.y - g.z, f.z - g.y, f.w - g.x);
+          d0 = make_float4(f.x + g.w, f.y + g.z, f.z + g.y, f.w + g.x);
+          d1 = make_float4(f.x - g.w, f.y - g.z, f.z - g.y, f.w - g.x);
         } else {
-          s0 = *reinterpret_cast<const float4*>(src + k);
-          s1 = *reinterpret_cast<const float4*>(src + Nh + k);
+          d0 = *reinterpret_cast<const float4*>(src + k);
+          d1 = *reinterpret_cast<const float4*>(src + Nh + k);
         }
       }
-      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = s0;
-      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = s1;
-      const size_t wo = (size_t)(n0 + row) * a.Kp + k0 + 4 * q;
-      *reinterpret_cast<float4*>(&Bs[0][row][4 * q]) = *reinterpret_cast<const float4*>(a.W0 + wo);
-      *reinterpret_cast<float4*>(&Bs[1][row][4 * q]) = *reinterpret_cast<const float4*>(a.W1 + wo);
     } else {
-      // A: matrix rows; B: data [k][n] (n contiguous), stored transposed
-      int row = tid >> 2, q = tid & 3;
-      const size_t wo = (size_t)(m0 + row) * a.Kp + k0 + 4 * q;
-      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = *reinterpret_cast<const float4*>(a.W0 + wo);
-      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = *reinterpret_cast<const float4*>(a.W1 + wo);
-      int kk = tid >> 4, qq = tid & 15;
       int k = k0 + kk, n = n0 + 4 * qq;
-      float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
       if (k < Nh && n < a.inner) {
         const float* pl = a.in + (size_t)plane * planeSz;
         if (MODE == DST_YF) {
           float4 f = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + n);
           float4 g = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + n);
-          s0 = make_float4(f.x + g.x, f.y + g.y, f.z + g.z, f.w + g.w);
-          s1 = make_float4(f.x - g.x, f.y - g.y, f.z - g.z, f.w - g.w);
+          d0 = make_float4(f.x + g.x, f.y + g.y, f.z + g.z, f.w + g.w);
+          d1 = make_float4(f.x - g.x, f.y - g.y, f.z - g.z, f.w - g.w);
         } else {
-          s0 = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + n);
-          s1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + n);
+          d0 = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + n);
+          d1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + n);
         }
       }
-      Bs[0][4 * qq + 0][kk] = s0.x; Bs[0][4 * qq + 1][kk] = s0.y; Bs[0][4 * qq + 2][kk] = s0.z; Bs[0][4 * qq + 3][kk] = s0.w;
-      Bs[1][4 * qq + 0][kk] = s1.x; Bs[1][4 * qq + 1][kk] = s1.y; Bs[1][4 * qq + 2][kk] = s1.z; Bs[1][4 * qq + 3][kk] = s1.w;
     }
+  };
+  auto stash = [&]() {
+    if (XMODE) {
+      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = d0;
+      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = d1;
+      *reinterpret_cast<float4*>(&Bs[0][row][4 * q]) = w0;
+      *reinterpret_cast<float4*>(&Bs[1][row][4 * q]) = w1;
+    } else {
+      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = w0;
+      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = w1;
+      Bs[0][4 * qq + 0][kk] = d0.x; Bs[0][4 * qq + 1][kk] = d0.y; Bs[0][4 * qq + 2][kk] = d0.z; Bs[0][4 * qq + 3][kk] = d0.w;
+      Bs[1][4 * qq + 0][kk] = d1.x; Bs[1][4 * qq + 1][kk] = d1.y; Bs[1][4 * qq + 2][kk] = d1.z; Bs[1][4 * qq + 3][kk] = d1.w;
+    }
+  };
+
+  fetch(0);
+  for (int k0 = 0; k0 < Nh; k0 += BK) {
+    stash();
     __syncthreads();
+    if (k0 + BK < Nh) fetch(k0 + BK);
     // ---------------- MFMA ----------------
     // Lane (lr, kq) feeds A[m = lr][k] and B[k][n = lr] for k = 4 kq + t at step t:
     // the same k permutation on both operands leaves the sum unchanged.
@@ -148,15 +160,23 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
 #pragma unroll
       for (int j = 0; j < 2; j++) bf[j] = *reinterpret_cast<const float4*>(&Bs[p][wn * 32 + j * 16 + lr][4 * kq]);
       if (PREC == 0) {
+        // k step outermost: four independent accumulators between two updates of one
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-          for (int j = 0; j < 2; j++) {
-            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].x, bf[j].x, acc[p][i][j], 0, 0, 0);
-            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].y, bf[j].y, acc[p][i][j], 0, 0, 0);
-            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].z, bf[j].z, acc[p][i][j], 0, 0, 0);
-            acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].w, bf[j].w, acc[p][i][j], 0, 0, 0);
-          }
+          for (int j = 0; j < 2; j++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].x, bf[j].x, acc[p][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].y, bf[j].y, acc[p][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].z, bf[j].z, acc[p][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i].w, bf[j].w, acc[p][i][j], 0, 0, 0);
       }
     }
     __syncthreads();
