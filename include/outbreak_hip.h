@@ -73,7 +73,10 @@ enum {
 };
 
 /* obl_step flags */
-#define OBL_STEP_LSF 1u /* run the large-scale forcing this step (model.js:1461) */
+#define OBL_STEP_LSF 1u   /* run the large-scale forcing this step (model.js:1461) */
+#define OBL_STEP_JUDGE 2u /* with a surrogate attached: also run the plain microphysics from the same
+                             inputs and add the difference to the judge statistics (not part of the
+                             step's result; no reference counterpart) */
 
 /* Solver precision (obl_set_solver_mode / obl_poisson_solve) */
 enum {
@@ -135,8 +138,10 @@ int obl_profile_read(obl_model* h, char* names, size_t names_len, double* ms, in
  * No reference counterpart (the reference has no surrogate). While enabled, each obl_step
  * appends sampled (input, output) pairs of the microphysics block (model.js:775-899) to a
  * device buffer of `capacity` rows of 24 floats: T, p0, rho0, qv, qc, qr, qi, qs, qg, Nr, Ni,
- * w, dz, dt, then dT, dqv, dqc, dqi, dqr, dqs, dqg, dNr, dNi, level. Points with condensate or
- * a phase change are kept with probability p_active, the rest with p_clear.
+ * w, dz, dt, then the tendencies of the block's process part (everything before its closing
+ * saturation adjustment, model.js:775-874) dT, dqv, dqc, dqi, dqr, dqs, dqg, dNr, dNi, then the
+ * level. Points holding more than 1e-9 kg/kg of condensate are kept with probability p_active,
+ * the rest with p_clear.
  * capacity 0 disables sampling; enabling again clears the buffer. */
 int obl_dump_enable(obl_model* h, unsigned capacity, float p_active, float p_clear);
 /* rows written so far (<= capacity); copies up to max_rows rows to out (may be NULL) and
@@ -145,8 +150,9 @@ int obl_dump_read(obl_model* h, float* out, size_t max_rows, size_t* rows, void*
 
 /* ---------------- microphysics surrogate network ----------------
  * No reference counterpart. A 16 -> 128 -> 128 -> 128 -> 16 MLP (GELU) whose 9 used outputs are
- * the tendencies of the microphysics block, scaled by out_scale; inputs are the 14 values of a
- * sampled row (obl_dump_*), mapped to features on the device. FP32 master parameters, Adam. */
+ * the tendencies of the microphysics block's process part, scaled by out_scale, except output 7
+ * (Nr), which is log((Nr + dNr + 1) / (Nr + 1)); inputs are the 14 values of a sampled row
+ * (obl_dump_*), mapped to features on the device. FP32 master parameters, Adam. */
 typedef struct obl_mlp obl_mlp;
 int obl_mlp_create(obl_mlp** out);
 void obl_mlp_destroy(obl_mlp* m);
@@ -157,6 +163,9 @@ int obl_mlp_get_params(obl_mlp* m, float* p, int n);
 /* out_scale[9]: tendency scales (targets are tendency / scale); in_lo/in_hi[14]: the training
  * envelope (NULL = unbounded); points outside it use the plain scheme at inference */
 int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const float* in_lo, const float* in_hi);
+/* a species absent before the block (below frac * its output scale) is changed at inference only
+ * when its predicted tendency exceeds frac * its output scale (default 0.02) */
+int obl_mlp_set_dead_zone(obl_mlp* m, float frac);
 /* copies nrows host rows (stride floats each) into device memory the network owns (replacing any
  * earlier copy); obl_mlp_train with rows_dev == NULL trains on them */
 int obl_mlp_load_rows(obl_mlp* m, const float* rows_host, size_t nrows, int stride);
@@ -168,6 +177,36 @@ int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, int stride, i
 /* exact evaluation of n host rows: mean loss, and optionally the gradient of the mean loss
  * (obl_mlp_nparams floats) and the raw outputs (n x 16) */
 int obl_mlp_eval(obl_mlp* m, const float* rows_host, int n, int stride, int quant, float* loss, float* grads, float* y);
+/* the inference form the model step uses (BF16 MFMA, FP32 accumulation): raw outputs (n x 16,
+ * before output scaling) for n host rows of >= 14 inputs */
+int obl_mlp_infer(obl_mlp* m, const float* rows_host, int n, int stride, float* y);
+
+/* ---------------- microphysics surrogate in the step ----------------
+ * No reference counterpart. With a network attached, each obl_step hands the points that hold more
+ * than 1e-9 kg/kg of condensate and whose inputs lie in the network's envelope to the network; the rest run the plain
+ * block. After inference: a dead zone for absent species, positivity, total-water conservation,
+ * latent-heat-consistent temperature, Thompson Nr bounds; a non-finite prediction falls back to
+ * the plain block. `capacity` bounds the points per step (0: half the
+ * cells); points beyond it run the plain block. net = NULL detaches. The network must stay alive
+ * while attached, and retraining it races with steps still in flight (obl_sync first). */
+int obl_set_surrogate(obl_model* h, obl_mlp* net, unsigned capacity);
+typedef struct {
+  double steps;      /* steps run with the surrogate */
+  double points;     /* points handed to the network, summed over steps */
+  double fallbacks;  /* of which fell back to the plain block */
+  double overflow;   /* candidates beyond the capacity (ran the plain block) */
+  double judged;     /* steps run with OBL_STEP_JUDGE */
+  /* over judged steps, surrogate (s) against plain (p) from the same inputs, all cells:
+   * sum (s - p)^2 and sum p^2 for qv qc qi qr qs qg Nr theta */
+  double d2[8], r2[8];
+  /* column condensate sum_k rho0 (qc + qi + qr + qs + qg) dz (kg m^-2), per column */
+  double cc_d2, cc_r2, cc_maxdiff, cc_max;
+  /* composite reflectivity (dBZ, model.js:1042-1050) over columns above 0 dBZ in either */
+  double dbz_d2, dbz_cols, dbz_maxdiff;
+  /* surface precipitation accumulated over judged steps (kg m^-2), per column */
+  double pr_d2, pr_r2, pr_sum_s, pr_sum_p;
+} obl_surr_stats;
+int obl_surrogate_stats(obl_model* h, obl_surr_stats* out, int reset);
 
 /* ---------------- PoissonDirect ---------------- */
 /* constructor (solver.js:106-137): nx, ny even; dz, dx in metres */

@@ -77,6 +77,22 @@ def loss_and_grads(p, f, t):
     return loss, g
 
 
+def bf16(a):
+    """Round to the nearest BF16 value, ties to even (finite inputs), returned as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, f):
+    """The quantised forward pass (training with quant on, and the BF16 inference kernels):
+    features, weights and hidden activations rounded to BF16, biases and sums in high precision."""
+    h = bf16(f)
+    for k in ("1", "2", "3"):
+        h = bf16(gelu(h @ bf16(p["W" + k]) + p["b" + k]))
+    return h @ bf16(p["W4"]) + p["b4"]
+
+
 def init_params(seed=0):
     rng = np.random.default_rng(seed)
     p = {}

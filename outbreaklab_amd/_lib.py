@@ -18,6 +18,13 @@ class OblError(RuntimeError):
     pass
 
 
+class SurrStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("steps", "points", "fallbacks", "overflow", "judged")] + [
+        ("d2", C.c_double * 8), ("r2", C.c_double * 8)] + [
+        (n, C.c_double) for n in ("cc_d2", "cc_r2", "cc_maxdiff", "cc_max", "dbz_d2", "dbz_cols", "dbz_maxdiff",
+                                  "pr_d2", "pr_r2", "pr_sum_s", "pr_sum_p")]
+
+
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
                 ("pad", C.c_int)]
@@ -58,6 +65,10 @@ _SIGS = {
                                 C.c_int, C.POINTER(C.c_float)]),
     "obl_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_void_p,
                                C.c_void_p]),
+    "obl_mlp_set_dead_zone": (C.c_int, [C.c_void_p, C.c_float]),
+    "obl_mlp_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "obl_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint]),
+    "obl_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(SurrStats), C.c_int]),
     "obl_poisson_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     "obl_poisson_destroy": (None, [C.c_void_p]),
     "obl_poisson_set_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -71,6 +82,7 @@ _SIGS = {
 FIELD = {"V": 0, "Q1": 1, "Q2": 2, "Vt": 3, "Q1t": 4, "Q2t": 5, "aux": 6, "phT": 7, "lsfA": 8, "lsfB": 9,
          "ciMap": 10, "phi": 11, "rhs": 12, "bs": 13}
 STEP_LSF = 1
+STEP_JUDGE = 2
 GEMM = {"fp32": 0, "bf16x3": 1}
 THOMAS = {"fp32": 0, "fp64": 1, "mixed": 2}
 

@@ -38,6 +38,7 @@ HIP_FLAGS = [
 # multiply. The solver (k_solve.hip) keeps correctly rounded FP32 division for the Thomas sweeps.
 FILE_FLAGS = {
     "k_step.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
+    "k_surr.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
 }
 
 

@@ -14,6 +14,7 @@
 #include "obl_device.h"
 #include "obl_kernels.h"
 #include "obl_mlp.h"
+#include "obl_mlp_infer.h"
 
 namespace obl {
 
@@ -209,6 +210,65 @@ __global__ void k_adam(float* P, float* G, float* m, float* v, int n, float lr, 
   v[i] = vi;
   P[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
   G[i] = 0.0f;
+}
+
+// ---------------- inference-form packing and evaluation ----------------
+// BF16 weight fragments in the order of obl_mlp_infer.h, and the FP32 biases, from the master
+// parameters (one thread per fragment element).
+__global__ void k_mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ F, float* __restrict__ bias) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < MLP_BIAS_FLOATS) {
+    int L = t / MLP_H, o = t % MLP_H;
+    const int boff[4] = {MLP_B1, MLP_B2, MLP_B3, MLP_B4};
+    bias[t] = P[boff[L] + o];
+  }
+  if (t >= MLP_PACK_HALVES) return;
+  int fi = t / MLP_FRAG, w = t % MLP_FRAG, lane = w / 8, s = w % 8;
+  int layer, mt, kb;
+  if (fi < MLP_FRAG_L2) { layer = 0; mt = fi; kb = 0; }
+  else if (fi < MLP_FRAG_L3) { layer = 1; mt = (fi - MLP_FRAG_L2) / 4; kb = (fi - MLP_FRAG_L2) % 4; }
+  else if (fi < MLP_FRAG_L4) { layer = 2; mt = (fi - MLP_FRAG_L3) / 4; kb = (fi - MLP_FRAG_L3) % 4; }
+  else { layer = 3; mt = 0; kb = fi - MLP_FRAG_L4; }
+  int lr = lane & 15, g = lane >> 4;
+  int k = 32 * kb + mlp_slot_feature(layer, g, s), o = 16 * mt + lr;
+  const int kin = layer == 0 ? MLP_IN : MLP_H, nout = layer == 3 ? MLP_OUT : MLP_H;
+  const int woff[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4};
+  float v = k < kin ? P[woff[layer] + k * nout + o] : 0.0f;
+  F[t] = (uint16_t)mlp_bf16_bits(v);
+}
+
+// Raw outputs of the inference form for sample rows (14 inputs each, `stride` floats apart):
+// y[row][MLP_OUT]. 256 threads, 64 rows per wave.
+__global__ void __launch_bounds__(256) k_mlp_eval(const uint16_t* __restrict__ F, const float* __restrict__ bias,
+                                                  const float* __restrict__ rows, int stride, int n,
+                                                  float* __restrict__ y) {
+  __shared__ float ys[4][64 * (MLP_OUT + 1)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15;
+  const int row0 = (blockIdx.x * 4 + wave) * 64;
+  auto feat = [&](int nt, float* f) {
+    int r = row0 + 16 * nt + lr;
+    float x[MLP_NIN];
+#pragma unroll
+    for (int i = 0; i < MLP_NIN; i++) x[i] = r < n ? rows[(size_t)r * stride + i] : 0.0f;
+    mlp_features(x, f);
+    if (r >= n)
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) f[i] = 0.0f;
+  };
+  mlp_wave64(F, bias, feat, ys[wave]);
+  __syncthreads();
+  int r = row0 + lane;
+  if (r < n)
+#pragma unroll
+    for (int j = 0; j < MLP_OUT; j++) y[(size_t)r * MLP_OUT + j] = ys[wave][lane * (MLP_OUT + 1) + j];
+}
+
+void launch_mlp_pack(hipStream_t s, const float* P, uint16_t* frag, float* bias) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_PACK_HALVES + 255) / 256), dim3(256), 0, s, P, frag, bias);
+}
+void launch_mlp_eval(hipStream_t s, const uint16_t* frag, const float* bias, const float* rows, int stride, int n,
+                     float* y) {
+  hipLaunchKernelGGL(k_mlp_eval, dim3((n + 255) / 256), dim3(256), 0, s, frag, bias, rows, stride, n, y);
 }
 
 void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a) {

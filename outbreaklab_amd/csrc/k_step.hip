@@ -9,6 +9,7 @@
 // the trilinear samples.
 #include "obl_device.h"
 #include "obl_kernels.h"
+#include "obl_micro.h"
 #include "obl_tile.h"
 
 namespace obl {
@@ -550,137 +551,6 @@ __device__ __forceinline__ float ldP(const float* phT, const Dims& d, int x, int
   return phT[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
 }
 
-// The microphysics block (model.js:775-899): T, qv..qg, nr in and out.
-__device__ __forceinline__ void micro_block(const MU& u, float dt, float p0, float rho, bool thm, float& T, float& qv,
-                                            float& qc, float& qi, float& qr, float& qs, float& qg, float& nr) {
-  float rsfc = u.v[U_STEP].z;
-  float rq = 1000.0f * rho;
-  float rs = sqrtf(rsfc / rho);
-  float pauto = u.v[U_MICRO].x * fmaxf(qc - u.v[U_MICRO].y, 0.0f);
-  float accr = 2.2f * qc * pw(qr, 0.875f);
-  if (thm && qr > 1e-9f) {
-    float l = lamR(qr, nr);
-    accr = 3100.0f * rho * nr * pw(l, -2.8f) * qc * rs;
-  }
-  float da = fminf(pauto * dt, qc);
-  float dc = fminf(accr * dt, qc - da);
-  qc -= da + dc;
-  qr += da + dc;
-  nr += da / 2.68e-10f;
-  float dTc = T - OBL_T0K;
-  if (dTc < 0.0f) {
-    float e25 = ex(0.025f * dTc);
-    float pqs = pw(fmaxf(rq * qs, 0.0f), 0.81f);
-    float pqg = pw(fmaxf(rq * qg, 0.0f), 0.875f);
-    float psaut = 1e-3f * e25 * fmaxf(qi - 6e-4f, 0.0f);
-    float psacw = 5.4e-3f * qc * pqs * rs;
-    float psaci = 5.4e-3f * qi * pqs * e25 * rs;
-    float ewet = 1.0f / (1.0f + qg / 0.008f);
-    float pgacw = 3.1e-3f * qc * pqg * rs * ewet;
-    float pgaci = 3.0e-4f * qi * pqg * rs;
-    float pgacr = 3.1e-3f * qr * pqg * rs * ewet;
-    float pgfr = fminf(2e-4f * u.v[U_MICRO].z * (ex(0.66f * fminf(-dTc, 20.0f)) - 1.0f), 0.08f) * qr;
-    float pgaut = 1e-3f * ex(0.09f * dTc) * fmaxf(qs - 6e-4f, 0.0f) * (thm ? 0.25f : 1.0f);
-    float a1 = fminf((psaut + psaci + pgaci) * dt, qi);
-    float fs1 = (psaut + psaci) / fmaxf(psaut + psaci + pgaci, 1e-20f);
-    qi -= a1;
-    qs += a1 * fs1;
-    qg += a1 * (1.0f - fs1);
-    float a2 = fminf((psacw + pgacw) * dt, qc);
-    float fs2 = psacw / fmaxf(psacw + pgacw, 1e-20f);
-    float rimeS = thm ? 0.95f : 0.7f;
-    qc -= a2;
-    qs += a2 * fs2 * rimeS;
-    qg += a2 * (1.0f - fs2 * rimeS);
-    float a3 = fminf((pgacr + pgfr) * dt, qr);
-    if (T < 235.0f) a3 = qr;
-    if (qr > 1e-12f) nr *= (qr - a3) / qr;
-    qr -= a3;
-    qg += a3;
-    float a4 = fminf(pgaut * dt, qs);
-    qs -= a4;
-    qg += a4;
-    T += OBL_LFC / OBL_CPD * (a2 + a3);
-  } else {
-    float m1 = fminf(0.012f * dTc * qs * dt, qs);
-    float m2 = fminf(0.0022f * dTc * qg * dt, qg);
-    if (thm) {
-      if (qg > 1e-9f) {
-        float l = lamG(qg, rho), n0 = n0G(qg, rho);
-        float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.89f * 5430.0f * pw(rsfc / rho, 0.25f) * pw(l, -2.945f);
-        float heat = 2.4e-2f * dTc - OBL_LVC * 2.26e-5f * rho * (qvsat(OBL_T0K, p0, 0.0f) - qv);
-        m2 = clampf_(6.2832f / (rho * OBL_LFC) * heat * n0 * vent * dt, 0.0f, qg);
-        nr += m2 / qg * n0 / (rho * l);
-      }
-      if (qs > 1e-9f) {
-        float ls = pw(3.14159f * 100.0f * 2e6f / (rho * qs), 0.25f);
-        nr += m1 / qs * 2e6f / (rho * ls);
-      }
-    }
-    qs -= m1;
-    qg -= m2;
-    qr += m1 + m2;
-    T -= OBL_LFC / OBL_CPD * (m1 + m2);
-  }
-  float qvsl = qvsat(T, p0, 0.0f);
-  if (qr > 0.0f && qv < qvsl) {
-    float rc = 0.001f * rho * qr;
-    float ern = ((1.6f + 124.9f * pw(rc, 0.2046f)) * pw(rc, 0.525f) / (2.55e8f / (p0 * qvsl) + 5.4e5f)) *
-                ((qvsl - qv) / (0.001f * rho * qvsl));
-    float srt = u.v[U_MICRO2].y > 0.0f ? sqrtf(rc / (rc + 0.001f * rho * u.v[U_MICRO2].y)) : 1.0f;
-    float e = ern * srt * dt * u.v[U_TURB].z;
-    if (thm) {
-      float l = lamR(qr, nr), n0 = rho * nr * l;
-      float Ap = OBL_LVC * OBL_LVC / (2.4e-2f * OBL_RVC * T * T), Bp = 1.0f / (rho * qvsl * 2.26e-5f);
-      float vent = 0.78f / (l * l) + 0.31f * 0.843f * 1.827f * 7492.0f * pw(rsfc / rho, 0.25f) * pw(l, -2.9f);
-      e = 6.2832f * (1.0f - qv / qvsl) * n0 * vent / (rho * (Ap + Bp)) * dt * u.v[U_MICRO2].w;
-    }
-    e = clampf_(e, 0.0f, fminf(qr, qvsl - qv));
-    if (thm && qr > 1e-12f) nr *= (qr - e) / qr;
-    qr -= e;
-    qv += e;
-    T -= OBL_LVC / OBL_CPD * e;
-  }
-  if (thm) {
-    if (qr > 1e-9f) {
-      float dm = 1.817f / lamR(qr, nr);
-      float fb = clampf_(1.0f - ex(2300.0f * (dm - 1.1e-3f)), -2.0f, 1.0f);
-      nr *= ex(-7.12f * rho * qr * fb * dt);
-      nr = clampf_(nr, qr * 3.183e5f, qr * 3.183e11f);
-    } else {
-      nr = 0.0f;
-    }
-  }
-  float qvsi = qvsat(T, p0, 1.0f);
-  if (dTc < 0.0f && qv < qvsi) {
-    float sub = fmaxf(1.0f - qv / qvsi, 0.0f);
-    float e = fminf((2e-3f * qs + 5e-4f * qg) * sub * dt, (qvsi - qv) * 0.5f);
-    float fsn = qs / fmaxf(qs + qg, 1e-20f);
-    e = fminf(e, qs + qg);
-    qs -= e * fsn;
-    qg -= e * (1.0f - fsn);
-    qv += e;
-    T -= OBL_LSC / OBL_CPD * e;
-  }
-  float qi0 = qi, qcl = qc + qi, dsum = 0.0f;
-#pragma unroll
-  for (int it = 0; it < 3; it++) {
-    float f = fIce(T);
-    float qs_ = qvsat(T, p0, f);
-    float L = mixf(OBL_LVC, OBL_LSC, f);
-    float dd = (qv - qs_) / (1.0f + L * L * qs_ / (OBL_CPD * OBL_RVC * T * T));
-    dd = fmaxf(dd, -qcl);
-    qv -= dd;
-    qcl += dd;
-    T += L / OBL_CPD * dd;
-    dsum += dd;
-  }
-  float f = fIce(T);
-  qi = f * qcl;
-  qc = qcl - qi;
-  T += OBL_LFC / OBL_CPD * (qi - qi0 - f * dsum);
-}
-
 // Accessors of phys_point's cell fields and 2-D map: global (clamped) reads, or reads of a block's
 // LDS plane tile for the points of its own plane (the rest from global memory). The tile
 // loader applies the same clamping, so both give the same values.
@@ -733,13 +603,29 @@ struct ZCol {
   }
 };
 
-template <bool DUMP, class AV, class AQ, class AX, class AP, class AB>
+enum { PHYS_PLAIN = 0, PHYS_DUMP = 1, PHYS_SURR = 2 };
+
+// Hand a point to the surrogate: one atomic per wave reserves consecutive record slots for the
+// wave's candidates (lane order). False when the list is full: the point then runs the plain block.
+__device__ __forceinline__ bool surr_slot(bool cand, unsigned* count, unsigned cap, unsigned& slot) {
+  unsigned long long m = __ballot(cand);
+  if (!cand) return false;
+  int leader = __ffsll((long long)m) - 1;
+  unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+  unsigned base = 0u;
+  if (below == 0u) base = atomicAdd(count, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  slot = base + below;
+  return slot < cap;
+}
+
+template <int MODE, class AV, class AQ, class AX, class AP, class AB>
 __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const float4* __restrict__ BS, const AV& Vin,
                                            const AQ& Q1in, const float4* __restrict__ Q2in, const AX& aux,
                                            const float4* __restrict__ lsfA, const AB& lsfB,
                                            const AP& phT, const float4* __restrict__ ciMap,
                                            float4* __restrict__ Vout, float4* __restrict__ Q1o, float4* __restrict__ Q2o,
-                                           const MicroDump& dump, int x, int y, int z) {
+                                           const MicroDump& dump, const SurrPhys& surr, int x, int y, int z) {
   int nx = d.nx, ny = d.ny, nz = d.nz;
   float dx = d.dx, dz = d.dz, dt = d.dt;
   float ix2 = 1.0f / (dx * dx), iz2 = 1.0f / (dz * dz);
@@ -849,11 +735,35 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float T = th * B.x;
     float qv = q1.x, qc = q1.y, qi = q1.z, qr = q1.w, qs = q2.x, qg = q2.y, nr = fmaxf(q2.z, 0.0f);
     const float T0 = T, qv0 = qv, qc0 = qc, qi0 = qi, qr0 = qr, qs0 = qs, qg0 = qg, nr0 = nr;
-    micro_block(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
-    if (DUMP) {
-      // teacher pairs of the microphysics block: stratified sample, active points (condensate present or
-      // T changed) with probability p_active, the rest with p_clear
-      bool active = (qc0 + qi0 + qr0 + qs0 + qg0 > 0.0f) || (T != T0);
+    bool handed = false;
+    if (MODE == PHYS_SURR) {
+      // points with condensate inside the training envelope go to the surrogate (k_surr.hip)
+      float wc = 0.5f * (V.z + zcol.p.z);
+      float in[14] = {T, B.y, A.z, qv, qc, qr, qi, qs, qg, nr, 0.0f, wc, dz, dt};
+      bool cand = micro_active(qc, qi, qr, qs, qg);
+#pragma unroll
+      for (int t = 0; t < 14; t++) cand = cand && in[t] >= surr.lo[t] && in[t] <= surr.hi[t];
+      unsigned slot;
+      if (surr_slot(cand, surr.count, surr.cap, slot)) {
+        float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
+        float4* R = reinterpret_cast<float4*>(surr.rec + (size_t)slot * SURR_REC);
+        R[0] = make_float4(__uint_as_float((unsigned)cidx(d, x, y, z)), T, B.y, A.z);
+        R[1] = make_float4(qv, qc, qi, qr);
+        R[2] = make_float4(qs, qg, nr, wc);
+        R[3] = make_float4(keep, B.x, q2.w, 0.0f);
+        handed = true;
+      }
+    }
+    float Tp = 0.0f, qvp = 0.0f, qcp = 0.0f, qip = 0.0f;  // after the process part (teacher targets)
+    if (!handed) {
+      micro_processes(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
+      Tp = T; qvp = qv; qcp = qc; qip = qi;
+      micro_satadj(B.y, T, qv, qc, qi);
+    }
+    if (MODE == PHYS_DUMP) {
+      // teacher pairs of the microphysics block: stratified sample, points where the block is
+      // active (the surrogate's domain) with probability p_active, the rest with p_clear
+      bool active = micro_active(qc0, qi0, qr0, qs0, qg0);
       float r = (float)(hash3i(x, y, z ^ (int)dump.salt) >> 8) * (1.0f / 16777216.0f);
       if (r < (active ? dump.p_active : dump.p_clear)) {
         unsigned slot = atomicAdd(dump.count, 1u);
@@ -861,7 +771,9 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
           float wc = 0.5f * (V.z + zcol.p.z);
           float* row = dump.buf + (size_t)slot * OBL_DUMP_STRIDE;
           float in[14] = {T0, B.y, A.z, qv0, qc0, qr0, qi0, qs0, qg0, nr0, 0.0f, wc, dz, dt};
-          float out[10] = {T - T0, qv - qv0, qc - qc0, qi - qi0, qr - qr0, qs - qs0, qg - qg0, nr - nr0, 0.0f, (float)z};
+          // targets: the process part's tendencies (the saturation adjustment is re-applied exactly)
+          float out[10] = {Tp - T0, qvp - qv0, qcp - qc0, qip - qi0, qr - qr0, qs - qs0, qg - qg0, nr - nr0, 0.0f,
+                           (float)z};
 #pragma unroll
           for (int t = 0; t < 14; t++) row[t] = in[t];
 #pragma unroll
@@ -869,12 +781,14 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
         }
       }
     }
-    th = T / B.x;
-    float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
-    qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
-    Q1o[ci_] = make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f));
-    Q2o[ci_] = make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w);
-    o.w = th;
+    if (!handed) {
+      th = T / B.x;
+      float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
+      qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
+      Q1o[ci_] = make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f));
+      Q2o[ci_] = make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w);
+    }
+    o.w = th;  // a handed point's theta is written by the surrogate
   }
   float sp = sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
   float vcap = u.v[U_MISC].x;
@@ -1026,13 +940,14 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
 // One block per 32 x 8 tile of one z-plane of the faces grid; the plane of V and of Q1 (with a
 // one-point halo) is staged in LDS, so the many x / y neighbour reads of the tendencies and
 // diffusion come from LDS; the z neighbours are read from global memory.
-template <bool DUMP>
+template <int MODE>
 __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
                                                        const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
                                                        const float2* __restrict__ aux, const float4* __restrict__ lsfA,
                                                        const float4* __restrict__ lsfB, const float* __restrict__ phT,
                                                        const float4* __restrict__ ciMap, float4* __restrict__ Vout,
-                                                       float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump) {
+                                                       float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump,
+                                                       SurrPhys sp) {
   __shared__ float sV[4 * TPS], sQ[4 * TPS], sA[2 * TPS], sP[TPS], sB[4 * TPS];
   Dims d = dims(u);
   const int tid = threadIdx.x;
@@ -1064,7 +979,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
   TAux ta{(const lds_float*)sA, aux, d, x0, y0, z};
   TPh tp{(const lds_float*)sP, phT, d, x0, y0, z};
   T2d tb{(const lds_float*)sB, x0, y0};
-  phys_point<DUMP>(u, d, BS, pv, pq, Q2in, ta, lsfA, tb, tp, ciMap, Vout, Q1o, Q2o, dump, x, y, z);
+  phys_point<MODE>(u, d, BS, pv, pq, Q2in, ta, lsfA, tb, tp, ciMap, Vout, Q1o, Q2o, dump, sp, x, y, z);
 }
 
 // ====================== DIV (model.js:910-925) ======================
@@ -1152,14 +1067,23 @@ void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, co
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys<false>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
-                     ciMap, Vout, Q1o, Q2o, none);
+  SurrPhys nos = {};
+  hipLaunchKernelGGL((k_phys<PHYS_PLAIN>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+                     lsfB, phT, ciMap, Vout, Q1o, Q2o, none, nos);
+}
+void launch_phys_surr(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                      const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                      const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const SurrPhys& sp) {
+  MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
+  hipLaunchKernelGGL((k_phys<PHYS_SURR>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+                     lsfB, phT, ciMap, Vout, Q1o, Q2o, none, sp);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
-  hipLaunchKernelGGL((k_phys<true>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT,
-                     ciMap, Vout, Q1o, Q2o, dump);
+  SurrPhys nos = {};
+  hipLaunchKernelGGL((k_phys<PHYS_DUMP>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+                     lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, nos);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);

@@ -14,6 +14,7 @@
 
 #include "../../include/outbreak_hip.h"
 #include "obl_kernels.h"
+#include "obl_mlp.h"
 
 using namespace obl;
 
@@ -415,6 +416,18 @@ struct obl_model {
   unsigned dump_cap = 0, dump_salt = 0;
   float dump_pa = 0.f, dump_pc = 0.f;
   bool dump_on = false;
+  // microphysics surrogate: attached network, point records, counters
+  obl_mlp* net = nullptr;
+  float* srec = nullptr;
+  unsigned* scount = nullptr;       // [0] points handed this step, [1] fallbacks this step
+  unsigned scap = 0;
+  unsigned long long* stally = nullptr;  // steps, points, fallbacks, overflow
+  // judge: the plain step's outputs, accumulated precipitation, sums
+  float4 *jV = nullptr, *jQ1 = nullptr, *jQ2 = nullptr;
+  double *jprs = nullptr, *jprp = nullptr, *jsums = nullptr;  // jsums: JUDGE_NSUM + 4 precip totals
+  unsigned long long* jmax = nullptr;
+  double judged = 0.0;
+  int sblocks = 256;  // persistent blocks of the surrogate kernel (one per CU)
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -439,7 +452,8 @@ extern "C" void obl_destroy(obl_model* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   void* bufs[] = {h->V[0], h->V[1], h->Vt, h->Q1[0], h->Q1[1], h->Q2[0], h->Q2[1], h->Q1t, h->Q2t, h->lsfA,
-                  h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI, h->dump_buf, h->dump_count};
+                  h->lsfB, h->ciMap, h->BS, h->aux, h->phT, h->phi, h->rhs, h->BD, h->CI, h->dump_buf, h->dump_count,
+                  h->srec, h->scount, h->stally, h->jV, h->jQ1, h->jQ2, h->jprs, h->jprp, h->jsums, h->jmax};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (h->solver) obl_poisson_destroy(h->solver);
@@ -604,6 +618,112 @@ extern "C" int obl_add_bubble(obl_model* h, double x, double y, double z, double
   return hip_ok(hipGetLastError(), &h->err, "bubble");
 }
 
+// judge buffers, allocated at the first judged step
+static int surr_judge_alloc(obl_model* h) {
+  if (h->jV) return 0;
+  size_t cols = (size_t)h->nx * h->ny;
+  if (hip_ok(hipMalloc(&h->jV, h->vcount() * 16), &h->err, "hipMalloc(judge V)") ||
+      hip_ok(hipMalloc(&h->jQ1, h->ccount() * 16), &h->err, "hipMalloc(judge Q1)") ||
+      hip_ok(hipMalloc(&h->jQ2, h->ccount() * 16), &h->err, "hipMalloc(judge Q2)") ||
+      hip_ok(hipMalloc(&h->jprs, cols * 8), &h->err, "hipMalloc(judge precip)") ||
+      hip_ok(hipMalloc(&h->jprp, cols * 8), &h->err, "hipMalloc(judge precip)") ||
+      hip_ok(hipMemsetAsync(h->jprs, 0, cols * 8, h->stream), &h->err, "clear") ||
+      hip_ok(hipMemsetAsync(h->jprp, 0, cols * 8, h->stream), &h->err, "clear"))
+    return -1;
+  return 0;
+}
+
+extern "C" int obl_set_surrogate(obl_model* h, obl_mlp* net, unsigned capacity) {
+  if (!h) return set_global("null handle");
+  if (hip_ok(hipStreamSynchronize(h->stream), &h->err, "sync")) return -1;
+  void* old[] = {h->srec, h->scount, h->stally, h->jV, h->jQ1, h->jQ2, h->jprs, h->jprp, h->jsums, h->jmax};
+  for (void* b : old)
+    if (b) hipFree(b);
+  h->srec = nullptr; h->scount = nullptr; h->stally = nullptr;
+  h->jV = h->jQ1 = h->jQ2 = nullptr;
+  h->jprs = h->jprp = h->jsums = nullptr;
+  h->jmax = nullptr;
+  h->judged = 0.0;
+  h->net = nullptr;
+  if (!net) return 0;
+  size_t cap = capacity ? (size_t)capacity : h->ccount() / 2;
+  if (cap > 0xffffffffull) cap = 0xffffffffull;
+  if (hip_ok(hipMalloc(&h->srec, cap * SURR_REC * 4), &h->err, "hipMalloc(surrogate records)") ||
+      hip_ok(hipMalloc(&h->scount, 2 * sizeof(unsigned)), &h->err, "hipMalloc(surrogate counters)") ||
+      hip_ok(hipMalloc(&h->stally, 4 * sizeof(unsigned long long)), &h->err, "hipMalloc(surrogate tally)") ||
+      hip_ok(hipMalloc(&h->jsums, (JUDGE_NSUM + 4) * sizeof(double)), &h->err, "hipMalloc(judge sums)") ||
+      hip_ok(hipMalloc(&h->jmax, JUDGE_NMAX * sizeof(unsigned long long)), &h->err, "hipMalloc(judge max)") ||
+      hip_ok(hipMemset(h->scount, 0, 2 * sizeof(unsigned)), &h->err, "clear") ||
+      hip_ok(hipMemset(h->stally, 0, 4 * sizeof(unsigned long long)), &h->err, "clear") ||
+      hip_ok(hipMemset(h->jsums, 0, (JUDGE_NSUM + 4) * sizeof(double)), &h->err, "clear") ||
+      hip_ok(hipMemset(h->jmax, 0, JUDGE_NMAX * sizeof(unsigned long long)), &h->err, "clear")) {
+    obl_set_surrogate(h, nullptr, 0);
+    return -1;
+  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    h->sblocks = cus;
+  h->scap = (unsigned)cap;
+  h->net = net;
+  return 0;
+}
+
+extern "C" int obl_surrogate_stats(obl_model* h, obl_surr_stats* out, int reset) {
+  if (!h) return set_global("null handle");
+  if (!out) return h->fail("null output");
+  memset(out, 0, sizeof *out);
+  if (!h->net) return h->fail("no surrogate attached");
+  hipStream_t s = h->stream;
+  unsigned long long tally[4] = {0, 0, 0, 0}, mx[JUDGE_NMAX] = {0, 0, 0};
+  double sums[JUDGE_NSUM + 4];
+  memset(sums, 0, sizeof sums);
+  if (h->jprs) {
+    hipMemsetAsync(h->jsums + JUDGE_NSUM, 0, 4 * sizeof(double), s);
+    launch_judge_precip(s, h->jprs, h->jprp, (size_t)h->nx * h->ny, h->jsums + JUDGE_NSUM);
+  }
+  if (hip_ok(hipMemcpyAsync(tally, h->stally, sizeof tally, hipMemcpyDeviceToHost, s), &h->err, "stats") ||
+      hip_ok(hipMemcpyAsync(sums, h->jsums, sizeof sums, hipMemcpyDeviceToHost, s), &h->err, "stats") ||
+      hip_ok(hipMemcpyAsync(mx, h->jmax, sizeof mx, hipMemcpyDeviceToHost, s), &h->err, "stats") ||
+      hip_ok(hipStreamSynchronize(s), &h->err, "stats"))
+    return -1;
+  out->steps = (double)tally[0];
+  out->points = (double)tally[1];
+  out->fallbacks = (double)tally[2];
+  out->overflow = (double)tally[3];
+  out->judged = h->judged;
+  for (int c = 0; c < 8; c++) {
+    out->d2[c] = sums[c];
+    out->r2[c] = sums[8 + c];
+  }
+  out->cc_d2 = sums[16];
+  out->cc_r2 = sums[17];
+  out->dbz_d2 = sums[18];
+  out->dbz_cols = sums[19];
+  double m[JUDGE_NMAX];
+  memcpy(m, mx, sizeof m);
+  out->cc_maxdiff = m[0];
+  out->cc_max = m[1];
+  out->dbz_maxdiff = m[2];
+  out->pr_d2 = sums[JUDGE_NSUM];
+  out->pr_r2 = sums[JUDGE_NSUM + 1];
+  out->pr_sum_s = sums[JUDGE_NSUM + 2];
+  out->pr_sum_p = sums[JUDGE_NSUM + 3];
+  if (reset) {
+    size_t cols = (size_t)h->nx * h->ny;
+    hipMemsetAsync(h->stally, 0, 4 * sizeof(unsigned long long), s);
+    hipMemsetAsync(h->jsums, 0, (JUDGE_NSUM + 4) * sizeof(double), s);
+    hipMemsetAsync(h->jmax, 0, JUDGE_NMAX * sizeof(unsigned long long), s);
+    if (h->jprs) {
+      hipMemsetAsync(h->jprs, 0, cols * 8, s);
+      hipMemsetAsync(h->jprp, 0, cols * 8, s);
+    }
+    h->judged = 0.0;
+    if (hip_ok(hipStreamSynchronize(s), &h->err, "stats reset")) return -1;
+  }
+  return 0;
+}
+
 extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags) {
   if (!h) return set_global("null handle");
   if (check_u(h, uniforms)) return -1;
@@ -620,7 +740,47 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
   TIMED(T, s, "m.advQ1", launch_advq1(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
   TIMED(T, s, "m.advQ2", launch_advq2(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t, h->Q1[1], h->Q2[1]));
-  if (h->dump_on) {
+  if (h->net) {
+    MlpView mv;
+    mlp_view(h->net, &mv);
+    if (flags & OBL_STEP_JUDGE) {
+      // the plain step from the same inputs, for the judge (not part of the surrogate step)
+      if (surr_judge_alloc(h)) return -1;
+      launch_phys(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT, h->ciMap, h->jV, h->jQ1,
+                  h->jQ2);
+    }
+    SurrPhys sp;
+    sp.rec = h->srec;
+    sp.count = h->scount;
+    sp.cap = h->scap;
+    for (int i = 0; i < 14; i++) {
+      sp.lo[i] = mv.lo[i];
+      sp.hi[i] = mv.hi[i];
+    }
+    SurrApply sa;
+    sa.rec = h->srec;
+    sa.count = h->scount;
+    sa.cap = h->scap;
+    sa.frag = mv.frag;
+    sa.bias = mv.bias;
+    for (int j = 0; j < 9; j++) {
+      sa.out_scale[j] = mv.out_scale[j];
+      sa.dead[j] = mv.dead_frac * mv.out_scale[j];
+    }
+    sa.Vout = Vc;
+    sa.Q1o = h->Q1[0];
+    sa.Q2o = h->Q2[0];
+    sa.nfallback = h->scount + 1;
+    TIMED(T, s, "m.phys", launch_phys_surr(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT,
+                                           h->ciMap, Vc, h->Q1[0], h->Q2[0], sp));
+    TIMED(T, s, "m.surr", launch_surr_apply(s, u, sa, h->sblocks));
+    if (flags & OBL_STEP_JUDGE) {
+      JudgeArgs ja = {h->BS, Vc, h->Q1[0], h->Q2[0], h->jV, h->jQ1, h->jQ2, h->jprs, h->jprp, h->jsums, h->jmax};
+      launch_judge_cols(s, u, d, ja);
+      h->judged += 1.0;
+    }
+    launch_surr_tally(s, h->scount, h->scap, h->stally);
+  } else if (h->dump_on) {
     MicroDump md = {h->dump_buf, h->dump_count, h->dump_cap, 0x9e3779b9u * (++h->dump_salt), h->dump_pa, h->dump_pc};
     TIMED(T, s, "m.phys", launch_phys_dump(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT,
                                            h->ciMap, Vc, h->Q1[0], h->Q2[0], md));

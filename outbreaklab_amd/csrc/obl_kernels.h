@@ -40,6 +40,48 @@ struct MicroDump {
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump);
+// ---- microphysics surrogate in the step (k_step.hip physics surrogate mode, k_surr.hip) ----
+// one record per point handed to the surrogate: cell index (bits), T, p0, rho0 | qv, qc, qi, qr |
+// qs, qg, Nr, w (cell centre) | sponge keep factor, pi, Q2 spare channel, 0
+constexpr int SURR_REC = 16;
+struct SurrPhys {
+  float* rec;
+  unsigned* count;
+  unsigned cap;
+  float lo[14], hi[14];  // training envelope of the 14 inputs
+};
+void launch_phys_surr(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                      const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                      const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const SurrPhys& sp);
+struct SurrApply {
+  const float* rec;
+  const unsigned* count;
+  unsigned cap;
+  const uint16_t* frag;  // packed BF16 weights (obl_mlp_infer.h)
+  const float* bias;
+  float out_scale[9];
+  float dead[9];  // creation thresholds of absent species (same units as the tendencies)
+  float4* Vout;
+  float4* Q1o;
+  float4* Q2o;
+  unsigned* nfallback;
+};
+void launch_surr_apply(hipStream_t s, const MU& u, const SurrApply& a, int blocks);
+void launch_surr_tally(hipStream_t s, unsigned* count, unsigned cap, unsigned long long* tally);
+constexpr int JUDGE_NSUM = 20;  // d2[8], r2[8] (qv qc qi qr qs qg Nr theta), cc d2, cc r2, dBZ d2, dBZ columns
+constexpr int JUDGE_NMAX = 3;   // max |d column condensate|, max column condensate, max |d dBZ|
+struct JudgeArgs {
+  const float4* BS;
+  const float4 *Vs, *Q1s, *Q2s;  // surrogate step
+  const float4 *Vp, *Q1p, *Q2p;  // plain step from the same inputs
+  double* prs;                   // accumulated surface precipitation, nx * ny (kg m^-2)
+  double* prp;
+  double* sums;
+  unsigned long long* maxes;     // non-negative doubles as bit patterns
+};
+void launch_judge_cols(hipStream_t s, const MU& u, const Dims& d, const JudgeArgs& a);
+void launch_judge_precip(hipStream_t s, const double* prs, const double* prp, size_t n, double* out);
+
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
 

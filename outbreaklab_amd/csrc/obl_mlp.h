@@ -1,6 +1,7 @@
 // The microphysics surrogate network: sizes, parameter layout, features, activation.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace obl {
 
@@ -62,8 +63,24 @@ struct MlpTrainArgs {
   int quant;            // round weights / activations to BF16 as inference does
 };
 
+// what the model step needs of a network: its inference form, output scales and envelope
+struct MlpView {
+  const uint16_t* frag;
+  const float* bias;
+  float out_scale[MLP_NOUT];
+  float lo[MLP_NIN], hi[MLP_NIN];
+  float dead_frac;  // dead zone of absent species, as a fraction of each output scale
+};
+
 void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a);
+void launch_mlp_eval(hipStream_t s, const uint16_t* frag, const float* bias, const float* rows, int stride, int n,
+                     float* y);
 void launch_adam(hipStream_t s, float* P, float* G, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                  int t);
 
 }  // namespace obl
+
+struct obl_mlp;
+namespace obl {
+int mlp_view(obl_mlp* m, MlpView* v);
+}

@@ -10,18 +10,22 @@
 
 #include "../../include/outbreak_hip.h"
 #include "obl_mlp.h"
+#include "obl_mlp_infer.h"
 
 using namespace obl;
 
 struct obl_mlp {
   hipStream_t stream = nullptr;
   float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr, *loss = nullptr;
+  uint16_t* frag = nullptr;  // inference form: packed BF16 weights and FP32 biases
+  float* bias = nullptr;
   float* rows = nullptr;  // rows of obl_mlp_load_rows
   size_t nrows = 0;
   int stride = 0;
   float out_scale[MLP_OUT];
   float in_lo[MLP_NIN], in_hi[MLP_NIN];
   int adam_t = 0;
+  float dead_frac = 0.02f;
   std::string err;
   int fail(const char* fmt, ...) {
     char buf[512];
@@ -40,12 +44,34 @@ struct obl_mlp {
 
 static std::string g_mlp_err;
 
+// refresh the inference form from the master parameters (after every change of them)
+static int repack(obl_mlp* m) {
+  launch_mlp_pack(m->stream, m->P, m->frag, m->bias);
+  if (m->ok(hipGetLastError(), "pack")) return -1;
+  return m->ok(hipStreamSynchronize(m->stream), "pack");
+}
+
+namespace obl {
+int mlp_view(obl_mlp* m, MlpView* v) {
+  if (!m || !v) return -1;
+  v->frag = m->frag;
+  v->bias = m->bias;
+  for (int j = 0; j < MLP_NOUT; j++) v->out_scale[j] = m->out_scale[j];
+  for (int i = 0; i < MLP_NIN; i++) {
+    v->lo[i] = m->in_lo[i];
+    v->hi[i] = m->in_hi[i];
+  }
+  v->dead_frac = m->dead_frac;
+  return 0;
+}
+}  // namespace obl
+
 extern "C" const char* obl_mlp_last_error(const obl_mlp* m) { return m ? m->err.c_str() : g_mlp_err.c_str(); }
 
 extern "C" void obl_mlp_destroy(obl_mlp* m) {
   if (!m) return;
   if (m->stream) hipStreamSynchronize(m->stream);
-  void* b[] = {m->P, m->G, m->M, m->V, m->loss, m->rows};
+  void* b[] = {m->P, m->G, m->M, m->V, m->loss, m->rows, m->frag, m->bias};
   for (void* p : b)
     if (p) hipFree(p);
   if (m->stream) hipStreamDestroy(m->stream);
@@ -69,9 +95,12 @@ extern "C" int obl_mlp_create(obl_mlp** out) {
   size_t n = (size_t)MLP_NPARAM * 4;
   if (m->ok(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream") || m->ok(hipMalloc(&m->P, n), "P") ||
       m->ok(hipMalloc(&m->G, n), "G") || m->ok(hipMalloc(&m->M, n), "M") || m->ok(hipMalloc(&m->V, n), "V") ||
-      m->ok(hipMalloc(&m->loss, 4), "loss") || m->ok(hipMemsetAsync(m->P, 0, n, m->stream), "clear") ||
+      m->ok(hipMalloc(&m->loss, 4), "loss") || m->ok(hipMalloc(&m->frag, (size_t)MLP_PACK_HALVES * 2), "frag") ||
+      m->ok(hipMalloc(&m->bias, (size_t)MLP_BIAS_FLOATS * 4), "bias") ||
+      m->ok(hipMemsetAsync(m->P, 0, n, m->stream), "clear") ||
       m->ok(hipMemsetAsync(m->G, 0, n, m->stream), "clear") || m->ok(hipMemsetAsync(m->M, 0, n, m->stream), "clear") ||
-      m->ok(hipMemsetAsync(m->V, 0, n, m->stream), "clear") || m->ok(hipStreamSynchronize(m->stream), "sync")) {
+      m->ok(hipMemsetAsync(m->V, 0, n, m->stream), "clear") || m->ok(hipStreamSynchronize(m->stream), "sync") ||
+      repack(m)) {
     g_mlp_err = m->err;
     obl_mlp_destroy(m);
     return -1;
@@ -90,7 +119,8 @@ extern "C" int obl_mlp_set_params(obl_mlp* m, const float* p, int n) {
       m->ok(hipMemsetAsync(m->M, 0, b, m->stream), "clear") || m->ok(hipMemsetAsync(m->V, 0, b, m->stream), "clear") ||
       m->ok(hipMemsetAsync(m->G, 0, b, m->stream), "clear"))
     return -1;
-  return m->ok(hipStreamSynchronize(m->stream), "set params");
+  if (m->ok(hipStreamSynchronize(m->stream), "set params")) return -1;
+  return repack(m);
 }
 
 extern "C" int obl_mlp_get_params(obl_mlp* m, float* p, int n) {
@@ -106,6 +136,13 @@ extern "C" int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const floa
     m->in_lo[i] = in_lo ? in_lo[i] : -3.0e38f;
     m->in_hi[i] = in_hi ? in_hi[i] : 3.0e38f;
   }
+  return 0;
+}
+
+extern "C" int obl_mlp_set_dead_zone(obl_mlp* m, float frac) {
+  if (!m) return -1;
+  if (!(frac >= 0.0f && frac < 1.0f)) return m->fail("dead zone fraction must be in [0, 1)");
+  m->dead_frac = frac;
   return 0;
 }
 
@@ -165,10 +202,33 @@ extern "C" int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, in
   if (m->ok(hipGetLastError(), "train launch")) return -1;
   float l = 0.f;
   if (m->ok(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream), "loss") ||
-      m->ok(hipStreamSynchronize(m->stream), "train"))
+      m->ok(hipStreamSynchronize(m->stream), "train") || repack(m))
     return -1;
   if (mean_loss) *mean_loss = l / ((float)steps * (float)batch * (float)MLP_NOUT);
   return 0;
+}
+
+extern "C" int obl_mlp_infer(obl_mlp* m, const float* rows_host, int n, int stride, float* y) {
+  if (!m) return -1;
+  if (!rows_host || !y || n < 1 || stride < MLP_NIN) return m->fail("infer: need n >= 1 rows of >= %d floats", MLP_NIN);
+  float *drows = nullptr, *dy = nullptr;
+  size_t rb = (size_t)n * stride * 4, yb = (size_t)n * MLP_OUT * 4;
+  if (m->ok(hipMalloc(&drows, rb), "rows") || m->ok(hipMalloc(&dy, yb), "y")) {
+    hipFree(drows);
+    hipFree(dy);
+    return -1;
+  }
+  int rc = m->ok(hipMemcpyAsync(drows, rows_host, rb, hipMemcpyHostToDevice, m->stream), "rows");
+  if (!rc) {
+    launch_mlp_eval(m->stream, m->frag, m->bias, drows, stride, n, dy);
+    rc = m->ok(hipGetLastError(), "infer launch");
+  }
+  if (!rc) rc = m->ok(hipMemcpyAsync(y, dy, yb, hipMemcpyDeviceToHost, m->stream), "y");
+  if (!rc) rc = m->ok(hipStreamSynchronize(m->stream), "infer");
+  hipStreamSynchronize(m->stream);
+  hipFree(drows);
+  hipFree(dy);
+  return rc;
 }
 
 extern "C" int obl_mlp_eval(obl_mlp* m, const float* rows_host, int n, int stride, int quant, float* loss, float* grads,

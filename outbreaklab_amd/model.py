@@ -209,10 +209,11 @@ class RegionalModel:
         """The reference's LSF schedule: at t = 0 and every >= 60 s of sim time (model.js:1461)."""
         return self.time - self.lsfT >= 60 or self.time == 0
 
-    def encStep(self, enc, dt):
-        """encStep (model.js:1442-1476): the state ends in V[parity], Q1[0], Q2[0]."""
+    def encStep(self, enc, dt, judge=False):
+        """encStep (model.js:1442-1476): the state ends in V[parity], Q1[0], Q2[0]. judge (with a
+        surrogate attached): also run the plain microphysics from the same inputs for the judge."""
         u = self.writeUniforms(dt)
-        flags = 0
+        flags = _lib.STEP_JUDGE if judge else 0
         if self.lsf_due():
             flags |= _lib.STEP_LSF
             self.lsfT = self.time
@@ -223,8 +224,8 @@ class RegionalModel:
         self.steps += 1
         self.phiScale = 1
 
-    def step(self, dt):
-        self.encStep(None, dt)
+    def step(self, dt, judge=False):
+        self.encStep(None, dt, judge)
 
     def sync(self):
         _lib.check(_lib.lib().obl_sync(self.h), self.h)
@@ -260,6 +261,30 @@ class RegionalModel:
         out = np.empty((n, 24), np.float32)
         if n:
             _lib.check(_lib.lib().obl_dump_read(self.h, _ptr(out), n, C.byref(rows), None), self.h)
+        return out
+
+    # ---------------- microphysics surrogate in the step (no reference counterpart) ----------------
+    def set_surrogate(self, net=None, capacity=0):
+        """Attach a trained Surrogate (or detach with None); see obl_set_surrogate."""
+        self._net = net
+        _lib.check(_lib.lib().obl_set_surrogate(self.h, None if net is None else net.h, int(capacity)), self.h)
+
+    def surrogate_stats(self, reset=False) -> dict:
+        """Surrogate usage and, over judged steps, its error against the plain block from the same
+        inputs: relative L2 per channel, column condensate, composite reflectivity, precipitation."""
+        st = _lib.SurrStats()
+        _lib.check(_lib.lib().obl_surrogate_stats(self.h, C.byref(st), 1 if reset else 0), self.h)
+        names = ["qv", "qc", "qi", "qr", "qs", "qg", "Nr", "theta"]
+        rel = lambda d, r: float(np.sqrt(d / r)) if r > 0 else (0.0 if d == 0 else float("inf"))  # noqa: E731
+        out = {"steps": st.steps, "points": st.points, "fallbacks": st.fallbacks, "overflow": st.overflow,
+               "judged": st.judged,
+               "rel_l2": {names[i]: rel(st.d2[i], st.r2[i]) for i in range(8)},
+               "column_condensate_rel_l2": rel(st.cc_d2, st.cc_r2),
+               "column_condensate_max_abs_diff": st.cc_maxdiff, "column_condensate_max": st.cc_max,
+               "dbz_rms": float(np.sqrt(st.dbz_d2 / st.dbz_cols)) if st.dbz_cols > 0 else 0.0,
+               "dbz_max_abs_diff": st.dbz_maxdiff, "dbz_columns": st.dbz_cols,
+               "precip_rel_l2": rel(st.pr_d2, st.pr_r2),
+               "precip_total_surrogate": st.pr_sum_s, "precip_total_plain": st.pr_sum_p}
         return out
 
     @property

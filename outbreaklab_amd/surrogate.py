@@ -73,6 +73,10 @@ class Surrogate:
         s = np.where(s > 0, s, 1.0)
         self.set_scales(s, rows[:, :NIN].min(0), rows[:, :NIN].max(0))
 
+    def set_dead_zone(self, frac):
+        """Absent species are created at inference only above frac * their output scale."""
+        _lib.check(_lib.lib().obl_mlp_set_dead_zone(self.h, float(frac)), self.h, mlp=True)
+
     def load_rows(self, rows):
         """Copy host rows (n x stride float32) to device memory owned by the network."""
         rows = np.ascontiguousarray(rows, np.float32)
@@ -97,6 +101,14 @@ class Surrogate:
                                            None if g is None else _p(g), _p(y)), self.h, mlp=True)
         return loss.value, g, y
 
+    def infer(self, rows):
+        """Raw outputs (n x 16, before output scaling) of the inference form the step uses."""
+        rows = np.ascontiguousarray(rows, np.float32)
+        n, stride = rows.shape
+        y = np.zeros((n, OUT), np.float32)
+        _lib.check(_lib.lib().obl_mlp_infer(self.h, _p(rows), n, stride, _p(y)), self.h, mlp=True)
+        return y
+
     def destroy(self):
         if getattr(self, "h", None) is not None:
             _lib.lib().obl_mlp_destroy(self.h)
@@ -107,3 +119,49 @@ class Surrogate:
             self.destroy()
         except Exception:
             pass
+
+
+def teach(model, dt, steps, capacity=2_000_000, p_active=0.5, p_clear=0.0):
+    """Run `steps` plain steps with teacher sampling of the microphysics block on; return the
+    sampled rows (n x 24: 14 inputs, 9 tendencies, level) and switch sampling off again."""
+    model.sample_micro(capacity, p_active, p_clear)
+    for _ in range(steps):
+        model.step(dt)
+    model.sync()
+    rows = model.micro_samples(copy=True)
+    model.sample_micro(0)
+    return rows
+
+
+NR_IN, NR_OUT = 9, NIN + 7  # Nr input and dNr output columns of a teacher row
+
+
+def targets(rows):
+    """Teacher rows with the network's targets: the number-concentration output becomes the log
+    ratio log((Nr + dNr + 1) / (Nr + 1)). Nr changes mostly multiplicatively (self-collection and
+    breakup, evaporation), over orders of magnitude; a linear dNr scaled to its largest values
+    leaves errors larger than small Nr themselves."""
+    t = np.array(rows, dtype=np.float32, copy=True)
+    n0 = np.maximum(t[:, NR_IN].astype(np.float64), 0.0)
+    n1 = np.maximum(n0 + t[:, NR_OUT], 0.0)
+    t[:, NR_OUT] = np.log((n1 + 1.0) / (n0 + 1.0))
+    return t
+
+
+def fit(net, rows, steps, batch=4096, lr=2e-3, seed=1, chunk=500, quant=True):
+    """Targets (see `targets`), scales and envelope from the teacher rows, then `steps` Adam steps on
+    the GPU with the learning rate halved for each of the last three quarters. Returns the mean
+    training loss of each chunk."""
+    rows = targets(rows)
+    net.fit_scales(rows)
+    net.load_rows(rows)
+    losses = []
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        frac = done / max(steps, 1)
+        rate = lr * 0.5 ** int(4 * frac)
+        losses.append(net.train(steps=k, batch=batch, lr=rate, seed=seed + done, quant=quant))
+        done += k
+    return losses
+
