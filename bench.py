@@ -97,6 +97,69 @@ def cpu_baseline(threads: int, dt: float, full_cells: int):
                       f"1024x1024x80; the reference's own WGSL on a CPU WebGPU adapter is unavailable here"}
 
 
+def timed(m, dt, steps, warm=2):
+    for _ in range(warm):
+        m.step(dt)
+    m.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.step(dt)
+    m.sync()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def kernel_times(m, dt, steps=3):
+    m.profile(True)
+    for _ in range(steps):
+        m.step(dt)
+    prof = m.profile_read()
+    m.profile(False)
+    return {k: round(v[0], 4) for k, v in prof.items()}
+
+
+def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
+    """The combined form on the same grid: the run continued until storms hold condensate, teacher
+    pairs from `teach_steps` plain steps, GPU training, then the plain FP32 step and the combined
+    form (surrogate + split-BF16 transforms + mixed FP32/FP64 Thomas) timed from the same developed
+    state, and one judged step (the surrogate against the plain block from the same inputs)."""
+    from outbreaklab_amd.surrogate import Surrogate, fit, teach
+
+    t0 = time.perf_counter()
+    for _ in range(spin):
+        m.step(dt)
+    rows = teach(m, dt, teach_steps, capacity=4_000_000, p_active=p_active)
+    net = Surrogate(seed=3)
+    losses = fit(net, rows, steps=train_steps, batch=8192, lr=2e-3)
+    prep = time.perf_counter() - t0
+    m.set_solver_mode("fp32", "mixed")
+    ms_plain = timed(m, dt, steps)
+    k_plain = kernel_times(m, dt)
+    m.set_surrogate(net)
+    m.set_solver_mode("bf16x3", "mixed")
+    ms_comb = timed(m, dt, steps)
+    k_comb = kernel_times(m, dt)
+    m.surrogate_stats(reset=True)
+    m.step(dt, judge=True)
+    st = m.surrogate_stats(reset=True)
+    m.set_surrogate(None)
+    m.set_solver_mode("fp32", "mixed")
+    net.destroy()
+    return {
+        "ms_per_step_plain_fp32": ms_plain, "ms_per_step_combined": ms_comb,
+        "sim_s_per_wall_s_combined": dt / (ms_comb * 1e-3), "speedup_vs_plain": ms_plain / ms_comb,
+        "state": f"after {m.steps} steps ({m.time:.0f} s simulated)",
+        "surrogate_points_per_step": st["points"] / max(st["steps"], 1), "fallbacks": st["fallbacks"],
+        "teacher_rows": int(rows.shape[0]), "training": f"{train_steps} Adam steps x 8192 rows, loss "
+                                                        f"{losses[0]:.4f} -> {losses[-1]:.4f}",
+        "prep_s": round(prep, 1),
+        "judge_one_step": {"rel_l2": {k: round(v, 6) for k, v in st["rel_l2"].items()},
+                           "column_condensate_rel_l2": st["column_condensate_rel_l2"],
+                           "dbz_rms": st["dbz_rms"], "dbz_max_abs_diff": st["dbz_max_abs_diff"],
+                           "precip_rel_l2": st["precip_rel_l2"]},
+        "kernels_plain_ms": k_plain, "kernels_combined_ms": k_comb,
+    }
+
+
 def calibrate_copy(mib: int):
     """Plain device-to-device copies of a known size: FETCH_SIZE / WRITE_SIZE of their dispatches
     give the scale from the HBM counters to bytes."""
@@ -123,6 +186,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--form", default="plain", choices=["plain", "c5"],
+                    help="plain: the plain FP32 step at 1024x1024x80 (and the combined form on the same grid); "
+                         "c5: the combined form at 1024x1024x100, 250 m")
+    ap.add_argument("--no-combined", action="store_true", help="skip the combined-form measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -133,6 +200,9 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.form == "c5":
+        bench_c5(args)
         return
 
     from outbreaklab_amd import RegionalModel
@@ -177,6 +247,13 @@ def main():
     import numpy as np
 
     finite = bool(np.isfinite(V).all() and np.isfinite(Q1).all())
+    comb = None
+    if not args.no_combined:
+        print("# combined form (surrogate + split-BF16) on the same grid ...", file=sys.stderr, flush=True)
+        try:
+            comb = combined_form(m, dt, args.steps, spin=750, teach_steps=40, p_active=0.02, train_steps=8000)
+        except Exception as e:  # reported, not fatal to the plain measurement
+            comb = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -201,8 +278,35 @@ def main():
         "roofline": roof,
         "kernels": kern,
         "cpu_baseline": cpu,
+        "combined_form": comb,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
+    }
+    print(json.dumps(out))
+    m.destroy()
+
+
+def bench_c5(args):
+    """The combined form at 1024 x 1024 x 100, 250 m (one outflow boundary, dt 1 s)."""
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_combined_1024
+
+    m = RegionalModel()
+    dt = setup_combined_1024(m, seed=1)
+    nx, ny, nz = m.nx, m.ny, m.nz
+    print(f"# workload {nx}x{ny}x{nz} dx 250 m, pad {m.pad}, Thompson, dt {dt} s", file=sys.stderr, flush=True)
+    c = combined_form(m, dt, args.steps, spin=900, teach_steps=60, p_active=0.02, train_steps=6000)
+    cells = nx * ny * nz
+    out = {
+        "metric": "sim-s per wall-s at 1024x1024x100 (surrogate + split-BF16)",
+        "value": c["sim_s_per_wall_s_combined"], "unit": "sim-s/wall-s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 2, "ms_per_step": c["ms_per_step_combined"],
+        "cell_updates_per_s": cells / (c["ms_per_step_combined"] * 1e-3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 state, bf16x3 transforms, bf16 network",
+        "data": "synthetic (WK sounding at 4000 J/kg, one seeded outflow boundary dT -6 K 1.5 km deep)",
+        "config": {"workload": f"combined form {nx}x{ny}x{nz} at 250 m, Thompson, dt {dt} s, solve "
+                               f"{m.nxp}x{m.nyp}x{nz}, surrogate microphysics, split-BF16 transforms, mixed Thomas"},
+        "combined_form": c,
     }
     print(json.dumps(out))
     m.destroy()

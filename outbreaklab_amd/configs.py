@@ -87,6 +87,32 @@ def setup_plain_1024(model, seed: int = 1, scheme: str = "thompson", n=(1024, 10
     return 2.0
 
 
+def setup_combined_1024(model, seed: int = 1, n=(1024, 1024, 100), cape: float = 4000.0):
+    """1024 x 1024 x 100 at 250 m (dz 200 m, pad 40): WK at 4000 J/kg SBCAPE with 25 m/s 0-6 km shear,
+    Thompson, one seeded outflow-type boundary (dT -6 K, 1.5 km deep, along-front variation 0.8 with
+    50 km hot spots) on a north-south line at 30 % of the domain moving east at 6 m/s, dt 1 s.
+    The boundary record follows bndModel (app.js:527-541) for an outflow boundary moving east."""
+    nx, ny, nz = n
+    model.allocate("custom", {"key": f"custom:250:{nx}x{ny}x{nz}", "dx": 250, "n": [nx, ny, nz]})
+    model.params["scheme"] = "thompson"
+    model.params["noise"] = 0.1
+    model.params["lat"] = 35.9
+    model.noise_seed = seed
+    model.setSounding(wk_text_for_cape(cape, 25.0))
+    x0 = 0.3 * model.Lx
+    pts = [[x0, model.Ly * (0.08 + 0.84 * i / 7.0)] for i in range(8)]
+    ws = model.baseWind(150.0)
+    # cold-pool wind 8 m/s from the west (windDir = dir + 180 for an outflow moving toward dir = 90)
+    bnd = {"pts": pts, "motion": [6.0, 0.0], "halfW": 4000.0, "dT": -6.0, "Hmax": 1500.0, "slope": 0.05,
+           "Hmin": 300.0, "dq": 0.1, "qRatio": 1.0, "du": 8.0 - ws[0], "dv": 0.0 - ws[1], "wRatio": 1.0,
+           "nudge": False, "dFree": 25000.0, "side": 1, "vary": 0.8, "seed": seed * 0.731 + 11.3, "varyL": 50000.0}
+    model.setBoundaries([bnd])
+    model.setCI([])
+    model.setSurface(None, None)
+    model.reset()
+    return 1.0
+
+
 def solve_rhs_1024(nx=1024, ny=1024, nz=80, seed=1, rms=1e-5):
     """Seeded zero-mean Gaussian divergence RHS with a k^-5/3 horizontal spectrum:
     in DST space g(l, m, k) kappa^(-4/3), kappa = sqrt((l+1)^2 + (m+1)^2), g ~ N(0, 1);

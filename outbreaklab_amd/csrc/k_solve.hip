@@ -223,52 +223,50 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
 // ---------------- split hi/lo BF16 form ----------------
 // Each FP32 operand x is split into hi = RNE_bf16(x) and lo = RNE_bf16(x - hi); the
 // product is hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo term, ~2^-18 relative,
-// is dropped). Sine matrices are split once on the host; data operands are split while
-// they are staged into LDS. v_mfma_f32_16x16x32_bf16, 128 x 128 block tiles, 64 x 64 per wave.
+// is dropped). v_mfma_f32_16x16x32_bf16, 128 x 128 block tiles, 64 x 64 per wave.
+// The sine matrices are split once on the host and staged as BF16 hi/lo tiles. The data tile is
+// staged in LDS as FP32 (even / odd folded), and each wave splits the fragments it reads with the
+// hardware's packed conversion (v_cvt_pk_bf16_f32), so the split overlaps its MFMAs instead of
+// sitting between the global loads and the barrier.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-constexpr int BBM = 128, BBN = 128, BBK = 32, BLD = BBK + 8;  // LDS row: 40 bf16 = 80 B
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+constexpr int BBM = 128, BBN = 128, BBK = 32;
+constexpr int BLD = BBK + 8;   // matrix LDS row: 40 bf16 = 80 B
+constexpr int DLD = BBK + 4;   // data LDS row: 36 floats = 144 B
 
-__device__ __forceinline__ void split8(const float* v, uint4& hi, uint4& lo) {
+// hi / lo BF16 halves of 8 consecutive floats (RNE, as the host split of the matrices)
+__device__ __forceinline__ void split8_hw(float4 v0, float4 v1, bf16x8& hi, bf16x8& lo) {
+  const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
   uint32_t h[4], l[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    short h0 = bf16_rne(v[2 * i]), h1 = bf16_rne(v[2 * i + 1]);
-    short l0 = bf16_rne(v[2 * i] - bf16_to_f(h0)), l1 = bf16_rne(v[2 * i + 1] - bf16_to_f(h1));
-    h[i] = (uint32_t)(uint16_t)h0 | ((uint32_t)(uint16_t)h1 << 16);
-    l[i] = (uint32_t)(uint16_t)l0 | ((uint32_t)(uint16_t)l1 << 16);
+    bf16x2 hb = __builtin_convertvector(f32x2{x[2 * i], x[2 * i + 1]}, bf16x2);
+    uint32_t hu = __builtin_bit_cast(uint32_t, hb);
+    float h0 = __uint_as_float(hu << 16), h1 = __uint_as_float(hu & 0xffff0000u);
+    bf16x2 lb = __builtin_convertvector(f32x2{x[2 * i] - h0, x[2 * i + 1] - h1}, bf16x2);
+    h[i] = hu;
+    l[i] = __builtin_bit_cast(uint32_t, lb);
   }
-  hi = make_uint4(h[0], h[1], h[2], h[3]);
-  lo = make_uint4(l[0], l[1], l[2], l[3]);
-}
-
-__device__ __forceinline__ void split4(const float* v, uint2& hi, uint2& lo) {
-  uint32_t h[2], l[2];
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    short h0 = bf16_rne(v[2 * i]), h1 = bf16_rne(v[2 * i + 1]);
-    short l0 = bf16_rne(v[2 * i] - bf16_to_f(h0)), l1 = bf16_rne(v[2 * i + 1] - bf16_to_f(h1));
-    h[i] = (uint32_t)(uint16_t)h0 | ((uint32_t)(uint16_t)h1 << 16);
-    l[i] = (uint32_t)(uint16_t)l0 | ((uint32_t)(uint16_t)l1 << 16);
-  }
-  hi = make_uint2(h[0], h[1]);
-  lo = make_uint2(l[0], l[1]);
+  hi = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
+  lo = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
-  // [parity][hi/lo][row][k]
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][2][BBM][BLD];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][2][BBN][BLD];
+__global__ void __launch_bounds__(256, 2) k_dst_bf16(DstArgs a) {
+  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
+  // data: [parity][row (X: data row; Y: output column n)][k] FP32; matrix: [parity][hi/lo][row][k]
+  __shared__ __attribute__((aligned(16))) float Ds[2][XMODE ? BBM : BBN][DLD];
+  __shared__ __attribute__((aligned(16))) uint16_t Ms[2][2][XMODE ? BBN : BBM][BLD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, g = lane >> 4;
   int mt, nt, plane;
-  dst_tile<MODE == DST_XF || MODE == DST_XI>(mt, nt, plane);
+  dst_tile<XMODE>(mt, nt, plane);
   const int m0 = mt * BBM, n0 = nt * BBN;
   const int N = a.N, Nh = a.Nh;
   const size_t planeSz = (size_t)N * (size_t)a.inner;
-  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
 
   v4f acc[2][4][4];
 #pragma unroll
@@ -284,54 +282,40 @@ __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
       int row = tid >> 1, half = tid & 1;
       int mrow = (XMODE ? n0 : m0) + row;
       const size_t wo = (size_t)mrow * a.KpB + k0 + 16 * half;
-      uint16_t(*M)[2][BBN][BLD] = XMODE ? Bs : As;
       const uint16_t* src[4] = {a.W0h, a.W0l, a.W1h, a.W1l};
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         const uint4* q = reinterpret_cast<const uint4*>(src[t] + wo);
         uint4 v0 = q[0], v1 = q[1];
-        *reinterpret_cast<uint4*>(&M[t >> 1][t & 1][row][16 * half]) = v0;
-        *reinterpret_cast<uint4*>(&M[t >> 1][t & 1][row][16 * half + 8]) = v1;
+        *reinterpret_cast<uint4*>(&Ms[t >> 1][t & 1][row][16 * half]) = v0;
+        *reinterpret_cast<uint4*>(&Ms[t >> 1][t & 1][row][16 * half + 8]) = v1;
       }
     }
-    // ---- data operand, split while staging ----
+    // ---- data operand, FP32, even / odd folded ----
     if (XMODE) {
       int row = tid >> 1, half = tid & 1;
-      int r = m0 + row, k = k0 + 16 * half;
-      float s0[16], s1[16];
+      int r = m0 + row;
+      const float* src = a.in + (size_t)r * N;
 #pragma unroll
-      for (int i = 0; i < 16; i++) { s0[i] = 0.f; s1[i] = 0.f; }
-      if (r < a.rows) {
-        const float* src = a.in + (size_t)r * N;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          int kk = k + 4 * c;
-          if (kk < Nh) {
-            float4 f = *reinterpret_cast<const float4*>(src + kk);
-            if (MODE == DST_XF) {
-              float4 gg = *reinterpret_cast<const float4*>(src + (N - 4 - kk));
-              s0[4 * c + 0] = f.x + gg.w; s0[4 * c + 1] = f.y + gg.z; s0[4 * c + 2] = f.z + gg.y; s0[4 * c + 3] = f.w + gg.x;
-              s1[4 * c + 0] = f.x - gg.w; s1[4 * c + 1] = f.y - gg.z; s1[4 * c + 2] = f.z - gg.y; s1[4 * c + 3] = f.w - gg.x;
-            } else {
-              float4 o = *reinterpret_cast<const float4*>(src + Nh + kk);
-              s0[4 * c + 0] = f.x; s0[4 * c + 1] = f.y; s0[4 * c + 2] = f.z; s0[4 * c + 3] = f.w;
-              s1[4 * c + 0] = o.x; s1[4 * c + 1] = o.y; s1[4 * c + 2] = o.z; s1[4 * c + 3] = o.w;
-            }
+      for (int c = 0; c < 4; c++) {
+        int kk = k0 + 16 * half + 4 * c;
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        if (r < a.rows && kk < Nh) {
+          float4 f = *reinterpret_cast<const float4*>(src + kk);
+          if (MODE == DST_XF) {
+            float4 gg = *reinterpret_cast<const float4*>(src + (N - 4 - kk));
+            s0 = make_float4(f.x + gg.w, f.y + gg.z, f.z + gg.y, f.w + gg.x);
+            s1 = make_float4(f.x - gg.w, f.y - gg.z, f.z - gg.y, f.w - gg.x);
+          } else {
+            s0 = f;
+            s1 = *reinterpret_cast<const float4*>(src + Nh + kk);
           }
         }
-      }
-      uint4 h, l;
-#pragma unroll
-      for (int c = 0; c < 2; c++) {
-        split8(s0 + 8 * c, h, l);
-        *reinterpret_cast<uint4*>(&As[0][0][row][16 * half + 8 * c]) = h;
-        *reinterpret_cast<uint4*>(&As[0][1][row][16 * half + 8 * c]) = l;
-        split8(s1 + 8 * c, h, l);
-        *reinterpret_cast<uint4*>(&As[1][0][row][16 * half + 8 * c]) = h;
-        *reinterpret_cast<uint4*>(&As[1][1][row][16 * half + 8 * c]) = l;
+        *reinterpret_cast<float4*>(&Ds[0][row][16 * half + 4 * c]) = s0;
+        *reinterpret_cast<float4*>(&Ds[1][row][16 * half + 4 * c]) = s1;
       }
     } else {
-      // 4 n x 4 k per thread: float4 rows in, packed 4 x bf16 (8 B) stores along k out
+      // 4 n x 4 k per thread: float4 rows in, transposed to [n][k] (float4 along k) out
       int ng = tid & 31, kq = tid >> 5;
       int gn = n0 + 4 * ng;
       const float* pl = a.in + (size_t)plane * planeSz;
@@ -343,9 +327,9 @@ __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
         if (k < Nh && gn < a.inner) {
           float4 f = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + gn);
           if (MODE == DST_YF) {
-            float4 m = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + gn);
-            f0 = make_float4(f.x + m.x, f.y + m.y, f.z + m.z, f.w + m.w);
-            f1 = make_float4(f.x - m.x, f.y - m.y, f.z - m.z, f.w - m.w);
+            float4 mm = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + gn);
+            f0 = make_float4(f.x + mm.x, f.y + mm.y, f.z + mm.z, f.w + mm.w);
+            f1 = make_float4(f.x - mm.x, f.y - mm.y, f.z - mm.z, f.w - mm.w);
           } else {
             f0 = f;
             f1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + gn);
@@ -356,36 +340,34 @@ __global__ void __launch_bounds__(256) k_dst_bf16(DstArgs a) {
       }
 #pragma unroll
       for (int nn = 0; nn < 4; nn++) {
-        float v0[4] = {s0[0][nn], s0[1][nn], s0[2][nn], s0[3][nn]};
-        float v1[4] = {s1[0][nn], s1[1][nn], s1[2][nn], s1[3][nn]};
-        uint2 h, l;
-        split4(v0, h, l);
-        *reinterpret_cast<uint2*>(&Bs[0][0][4 * ng + nn][4 * kq]) = h;
-        *reinterpret_cast<uint2*>(&Bs[0][1][4 * ng + nn][4 * kq]) = l;
-        split4(v1, h, l);
-        *reinterpret_cast<uint2*>(&Bs[1][0][4 * ng + nn][4 * kq]) = h;
-        *reinterpret_cast<uint2*>(&Bs[1][1][4 * ng + nn][4 * kq]) = l;
+        *reinterpret_cast<float4*>(&Ds[0][4 * ng + nn][4 * kq]) = make_float4(s0[0][nn], s0[1][nn], s0[2][nn], s0[3][nn]);
+        *reinterpret_cast<float4*>(&Ds[1][4 * ng + nn][4 * kq]) = make_float4(s1[0][nn], s1[1][nn], s1[2][nn], s1[3][nn]);
       }
     }
     __syncthreads();
     // ---- MFMA: lane group g feeds k = 8 g .. 8 g + 7 of both operands ----
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-      bf16x8 ah[4], al[4], bh[4], bl[4];
+      // data fragments (split here) and matrix fragments (pre-split)
+      bf16x8 dh[4], dl[4], mh[4], ml[4];
+      const int dbase = XMODE ? wm * 64 : wn * 64, mbase = XMODE ? wn * 64 : wm * 64;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        ah[i] = *reinterpret_cast<const bf16x8*>(&As[p][0][wm * 64 + i * 16 + lr][8 * g]);
-        al[i] = *reinterpret_cast<const bf16x8*>(&As[p][1][wm * 64 + i * 16 + lr][8 * g]);
-        bh[i] = *reinterpret_cast<const bf16x8*>(&Bs[p][0][wn * 64 + i * 16 + lr][8 * g]);
-        bl[i] = *reinterpret_cast<const bf16x8*>(&Bs[p][1][wn * 64 + i * 16 + lr][8 * g]);
+        const float* dp = &Ds[p][dbase + i * 16 + lr][8 * g];
+        split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), dh[i], dl[i]);
+        mh[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + i * 16 + lr][8 * g]);
+        ml[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + i * 16 + lr][8 * g]);
       }
+      // A = rows m (X: data, Y: matrix), B = columns n (X: matrix, Y: data)
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[p][i][j], 0, 0, 0);
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[p][i][j], 0, 0, 0);
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[p][i][j], 0, 0, 0);
+          const bf16x8 ah = XMODE ? dh[i] : mh[i], al = XMODE ? dl[i] : ml[i];
+          const bf16x8 bh = XMODE ? mh[j] : dh[j], bl = XMODE ? ml[j] : dl[j];
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[p][i][j], 0, 0, 0);
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[p][i][j], 0, 0, 0);
+          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[p][i][j], 0, 0, 0);
         }
     }
     __syncthreads();
