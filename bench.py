@@ -53,6 +53,18 @@ def kernel_roofline(name, ms, nx, ny, nz, nxp, nyp, gemm):
     return None
 
 
+def measured_traffic(kernel: str, size: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/hbm_traffic_<size>.md, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{size}.md")
+    try:
+        txt = open(path).read()
+        blk = txt.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(blk)[kernel]["traffic_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(threads: int, dt: float, full_cells: int):
     """The oracle's FP32 step (C, OpenMP) on a 256 x 256 x 80 slice of the workload at 500 m."""
     import numpy as np
@@ -242,7 +254,8 @@ def main():
     dom = max((k for k in kern if "bound" in kern[k]), key=lambda k: kern[k]["ms"])
     d = kern[dom]
     roof = {"kernel": dom, "bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
-            "frac": d["frac"], "traffic": None}
+            "frac": d["frac"], "traffic": measured_traffic(dom, f"{nx}x{ny}x{nz}"),
+            "algorithmic": d.get("algorithmic_bytes", d.get("algorithmic_flops"))}
     V, Q1, _ = m.state()
     import numpy as np
 
