@@ -45,6 +45,22 @@ typedef struct {
   int (*p_set_density)(obl_poisson*, const double*, const double*);
   int (*p_solve_host)(obl_poisson*, const float*, float*, int, int, double);
   const char* (*p_last_error)(const obl_poisson*);
+  /* microphysics surrogate (no reference counterpart) */
+  int (*dump_enable)(obl_model*, unsigned, float, float);
+  int (*dump_read)(obl_model*, float*, size_t, size_t*, void**);
+  int (*set_surrogate)(obl_model*, obl_mlp*, unsigned);
+  int (*surrogate_stats)(obl_model*, obl_surr_stats*, int);
+  int (*m_create)(obl_mlp**);
+  void (*m_destroy)(obl_mlp*);
+  const char* (*m_last_error)(const obl_mlp*);
+  int (*m_nparams)(void);
+  int (*m_set_params)(obl_mlp*, const float*, int);
+  int (*m_get_params)(obl_mlp*, float*, int);
+  int (*m_set_scales)(obl_mlp*, const float*, const float*, const float*);
+  int (*m_set_dead_zone)(obl_mlp*, float);
+  int (*m_load_rows)(obl_mlp*, const float*, size_t, int);
+  int (*m_train)(obl_mlp*, const float*, size_t, int, int, int, float, unsigned, int, float*);
+  int (*m_infer)(obl_mlp*, const float*, int, int, float*);
 } Api;
 
 static Api A;
@@ -98,6 +114,21 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(p_set_density, "obl_poisson_set_density");
   SYM(p_solve_host, "obl_poisson_solve_host");
   SYM(p_last_error, "obl_poisson_last_error");
+  SYM(dump_enable, "obl_dump_enable");
+  SYM(dump_read, "obl_dump_read");
+  SYM(set_surrogate, "obl_set_surrogate");
+  SYM(surrogate_stats, "obl_surrogate_stats");
+  SYM(m_create, "obl_mlp_create");
+  SYM(m_destroy, "obl_mlp_destroy");
+  SYM(m_last_error, "obl_mlp_last_error");
+  SYM(m_nparams, "obl_mlp_nparams");
+  SYM(m_set_params, "obl_mlp_set_params");
+  SYM(m_get_params, "obl_mlp_get_params");
+  SYM(m_set_scales, "obl_mlp_set_scales");
+  SYM(m_set_dead_zone, "obl_mlp_set_dead_zone");
+  SYM(m_load_rows, "obl_mlp_load_rows");
+  SYM(m_train, "obl_mlp_train");
+  SYM(m_infer, "obl_mlp_infer");
 #undef SYM
   A.so = so;
   return 0;
@@ -410,6 +441,169 @@ static napi_value js_p_solve(napi_env env, napi_callback_info info) {
   return u;
 }
 
+/* ---------------- microphysics surrogate (no reference counterpart) ---------------- */
+static napi_value undef(napi_env env) {
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+static napi_value mlp_ok(napi_env env, int rc, obl_mlp* m) {
+  if (rc) return throw_msg(env, A.m_last_error(m));
+  return undef(env);
+}
+static napi_value js_sample_micro(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  return ok_or_throw(env, A.dump_enable(h, (unsigned)num(env, argv[1]), (float)num(env, argv[2]), (float)num(env, argv[3])),
+                     h);
+}
+/* the sampled teacher rows as a Float32Array (rows x 24) */
+static napi_value js_micro_samples(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t rows = 0;
+  if (A.dump_read(h, NULL, 0, &rows, NULL)) return throw_msg(env, A.last_error(h));
+  napi_value ab, arr;
+  void* data = NULL;
+  CHECK_NAPI(napi_create_arraybuffer(env, rows * 24 * 4, &data, &ab));
+  if (rows && A.dump_read(h, (float*)data, rows, &rows, NULL)) return throw_msg(env, A.last_error(h));
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, rows * 24, ab, 0, &arr));
+  return arr;
+}
+static void mlp_finalize(napi_env env, void* data, void* hint) {
+  (void)env; (void)hint;
+  if (data && A.m_destroy) A.m_destroy((obl_mlp*)data);
+}
+static napi_value js_mlp_create(napi_env env, napi_callback_info info) {
+  (void)info;
+  if (!A.so) return throw_msg(env, "outbreak_hip library not loaded");
+  obl_mlp* m = NULL;
+  if (A.m_create(&m)) return throw_msg(env, A.m_last_error(NULL));
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, m, mlp_finalize, NULL, &out));
+  return out;
+}
+static napi_value js_mlp_nparams(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value v;
+  napi_create_double(env, (double)A.m_nparams(), &v);
+  return v;
+}
+static napi_value js_mlp_params(napi_env env, napi_callback_info info) {
+  /* (m, Float32Array): set when given with data, else get into it */
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  size_t b;
+  float* p = tarr(env, argv[1], &b);
+  if (!p) return throw_msg(env, "params: Float32Array expected");
+  bool set = false;
+  napi_get_value_bool(env, argv[2], &set);
+  return mlp_ok(env, set ? A.m_set_params(m, p, (int)(b / 4)) : A.m_get_params(m, p, (int)(b / 4)), m);
+}
+static napi_value js_mlp_set_scales(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  size_t b0, b1, b2;
+  float* os = tarr(env, argv[1], &b0);
+  float* lo = tarr(env, argv[2], &b1);
+  float* hi = tarr(env, argv[3], &b2);
+  if (!os || b0 < 36 || (lo && b1 < 56) || (hi && b2 < 56)) return throw_msg(env, "setScales: Float32Array(9), (14), (14)");
+  return mlp_ok(env, A.m_set_scales(m, os, lo, hi), m);
+}
+static napi_value js_mlp_set_dead_zone(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  return mlp_ok(env, A.m_set_dead_zone(m, (float)num(env, argv[1])), m);
+}
+static napi_value js_mlp_load_rows(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  size_t b;
+  float* r = tarr(env, argv[1], &b);
+  int stride = (int)num(env, argv[2]);
+  if (!r || stride < 23 || b < (size_t)stride * 4) return throw_msg(env, "loadRows: Float32Array of rows x stride expected");
+  return mlp_ok(env, A.m_load_rows(m, r, b / 4 / (size_t)stride, stride), m);
+}
+/* (m, steps, batch, lr, seed, quant) on the loaded rows -> mean loss */
+static napi_value js_mlp_train(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  get_args(env, info, 6, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  float loss = 0.0f;
+  if (A.m_train(m, NULL, 0, 0, (int)num(env, argv[1]), (int)num(env, argv[2]), (float)num(env, argv[3]),
+                (unsigned)num(env, argv[4]), (int)num(env, argv[5]), &loss))
+    return throw_msg(env, A.m_last_error(m));
+  napi_value v;
+  napi_create_double(env, (double)loss, &v);
+  return v;
+}
+/* (m, rows Float32Array, stride) -> Float32Array(n x 16) of raw outputs */
+static napi_value js_mlp_infer(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  size_t b;
+  float* r = tarr(env, argv[1], &b);
+  int stride = (int)num(env, argv[2]);
+  if (!r || stride < 14 || b < (size_t)stride * 4) return throw_msg(env, "infer: Float32Array of rows x stride expected");
+  int n = (int)(b / 4 / (size_t)stride);
+  napi_value ab, arr;
+  void* data = NULL;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)n * 16 * 4, &data, &ab));
+  if (A.m_infer(m, r, n, stride, (float*)data)) return throw_msg(env, A.m_last_error(m));
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, (size_t)n * 16, ab, 0, &arr));
+  return arr;
+}
+static napi_value js_set_surrogate(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  napi_valuetype t;
+  napi_typeof(env, argv[1], &t);
+  obl_mlp* m = t == napi_external ? (obl_mlp*)ext(env, argv[1]) : NULL;
+  return ok_or_throw(env, A.set_surrogate(h, m, (unsigned)num(env, argv[2])), h);
+}
+static void put(napi_env env, napi_value o, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, o, k, x);
+}
+static napi_value js_surrogate_stats(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_model* h = ext(env, argv[0]);
+  bool reset = false;
+  napi_get_value_bool(env, argv[1], &reset);
+  obl_surr_stats st;
+  if (A.surrogate_stats(h, &st, reset ? 1 : 0)) return throw_msg(env, A.last_error(h));
+  napi_value o, d2, r2;
+  CHECK_NAPI(napi_create_object(env, &o));
+  put(env, o, "steps", st.steps); put(env, o, "points", st.points); put(env, o, "fallbacks", st.fallbacks);
+  put(env, o, "overflow", st.overflow); put(env, o, "judged", st.judged);
+  CHECK_NAPI(napi_create_array_with_length(env, 8, &d2));
+  CHECK_NAPI(napi_create_array_with_length(env, 8, &r2));
+  for (uint32_t i = 0; i < 8; i++) {
+    napi_value a, b;
+    napi_create_double(env, st.d2[i], &a);
+    napi_create_double(env, st.r2[i], &b);
+    napi_set_element(env, d2, i, a);
+    napi_set_element(env, r2, i, b);
+  }
+  napi_set_named_property(env, o, "d2", d2);
+  napi_set_named_property(env, o, "r2", r2);
+  put(env, o, "ccD2", st.cc_d2); put(env, o, "ccR2", st.cc_r2); put(env, o, "ccMaxDiff", st.cc_maxdiff);
+  put(env, o, "ccMax", st.cc_max); put(env, o, "dbzD2", st.dbz_d2); put(env, o, "dbzCols", st.dbz_cols);
+  put(env, o, "dbzMaxDiff", st.dbz_maxdiff); put(env, o, "prD2", st.pr_d2); put(env, o, "prR2", st.pr_r2);
+  put(env, o, "prSumS", st.pr_sum_s); put(env, o, "prSumP", st.pr_sum_p);
+  return o;
+}
+
 #define EXPORT(name, fn)                                                      \
   do {                                                                        \
     napi_value f;                                                             \
@@ -437,6 +631,18 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("poissonCreate", js_p_create);
   EXPORT("poissonSetDensity", js_p_set_density);
   EXPORT("poissonSolve", js_p_solve);
+  EXPORT("sampleMicro", js_sample_micro);
+  EXPORT("microSamples", js_micro_samples);
+  EXPORT("setSurrogate", js_set_surrogate);
+  EXPORT("surrogateStats", js_surrogate_stats);
+  EXPORT("mlpCreate", js_mlp_create);
+  EXPORT("mlpNparams", js_mlp_nparams);
+  EXPORT("mlpParams", js_mlp_params);
+  EXPORT("mlpSetScales", js_mlp_set_scales);
+  EXPORT("mlpSetDeadZone", js_mlp_set_dead_zone);
+  EXPORT("mlpLoadRows", js_mlp_load_rows);
+  EXPORT("mlpTrain", js_mlp_train);
+  EXPORT("mlpInfer", js_mlp_infer);
   return exports;
 }
 

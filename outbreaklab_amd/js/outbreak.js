@@ -322,10 +322,11 @@ class RegionalModel {
   }
   // build-defined warm bubble (no reference counterpart), on both state copies
   addBubble(x, y, z, rh, rz, dth) { addon.addBubble(this.h, x, y, z, rh, rz, dth); }
-  encStep(enc, dt) {
+  // judge (with a surrogate attached): also run the plain microphysics from the same inputs
+  encStep(enc, dt, judge) {
     void enc;
     const u = this.writeUniforms(dt);
-    let flags = 0;
+    let flags = judge ? 2 : 0;
     if (this.time - this.lsfT >= 60 || this.time === 0) { flags |= 1; this.lsfT = this.time; }
     addon.step(this.h, u, this.parity, flags);
     this.parity = 1 - this.parity;
@@ -335,7 +336,70 @@ class RegionalModel {
   setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'fp32'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
   sync() { return addon.sync(this.h); }
   requestMaps() { return null; }  // the 2-D map kernel (model.js:1033-1100) is not part of this library yet
+  // ── microphysics surrogate (no reference counterpart) ──
+  // teacher sampling of the microphysics block during plain steps (capacity 0: off)
+  sampleMicro(capacity, pActive, pClear) { addon.sampleMicro(this.h, capacity, dflt(pActive, 1), dflt(pClear, 0)); }
+  microSamples() { return addon.microSamples(this.h); }  // Float32Array, rows x 24
+  // attach a trained Surrogate (null detaches); capacity 0 = half the cells
+  setSurrogate(net, capacity) { this._net = net || null; addon.setSurrogate(this.h, net ? net.h : null, capacity || 0); }
+  // usage and, over judged steps, the error against the plain block (relative L2 per channel, ...)
+  surrogateStats(reset) {
+    const s = addon.surrogateStats(this.h, !!reset);
+    const rel = (d, r) => (r > 0 ? Math.sqrt(d / r) : (d === 0 ? 0 : Infinity));
+    const names = ['qv', 'qc', 'qi', 'qr', 'qs', 'qg', 'Nr', 'theta'];
+    const relL2 = {};
+    names.forEach((n, i) => { relL2[n] = rel(s.d2[i], s.r2[i]); });
+    return { steps: s.steps, points: s.points, fallbacks: s.fallbacks, overflow: s.overflow, judged: s.judged, relL2,
+      columnCondensateRelL2: rel(s.ccD2, s.ccR2), dbzRms: s.dbzCols > 0 ? Math.sqrt(s.dbzD2 / s.dbzCols) : 0,
+      dbzMaxAbsDiff: s.dbzMaxDiff, precipRelL2: rel(s.prD2, s.prR2), precipTotals: [s.prSumS, s.prSumP] };
+  }
   get Vcur() { return this.V[this.parity]; }
 }
 
-module.exports = { RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };
+// ── the microphysics surrogate network (no reference counterpart): 14 inputs -> 3 x 128 GELU -> 9
+// outputs, trained on the GPU from the model's teacher rows (14 inputs, 9 tendencies, level) ──
+class Surrogate {
+  constructor() {
+    this.h = addon.mlpCreate();
+    this.n = addon.mlpNparams();
+  }
+  params() { const p = new Float32Array(this.n); addon.mlpParams(this.h, p, false); return p; }
+  setParams(p) { addon.mlpParams(this.h, Float32Array.from(p), true); }
+  setDeadZone(frac) { addon.mlpSetDeadZone(this.h, frac); }
+  // the network's targets: the Nr output as log((Nr + dNr + 1) / (Nr + 1)); scales (RMS of each target)
+  // and the input envelope (min / max) from the rows, as the Python host's `fit`
+  static targets(rows) {
+    const t = Float32Array.from(rows);
+    for (let r = 0; r < t.length / 24; r++) {
+      const n0 = Math.max(t[r * 24 + 9], 0), n1 = Math.max(n0 + t[r * 24 + 21], 0);
+      t[r * 24 + 21] = Math.log((n1 + 1) / (n0 + 1));
+    }
+    return t;
+  }
+  fitScales(t) {
+    const n = t.length / 24, s = new Float64Array(9), lo = new Float32Array(14).fill(Infinity), hi = new Float32Array(14).fill(-Infinity);
+    for (let r = 0; r < n; r++) {
+      for (let j = 0; j < 9; j++) { const v = t[r * 24 + 14 + j]; s[j] += v * v; }
+      for (let i = 0; i < 14; i++) { const v = t[r * 24 + i]; if (v < lo[i]) lo[i] = v; if (v > hi[i]) hi[i] = v; }
+    }
+    const sc = Float32Array.from(s, v => (v > 0 ? Math.sqrt(v / n) : 1));
+    addon.mlpSetScales(this.h, sc, lo, hi);
+    return { scale: sc, lo, hi };
+  }
+  // GPU training on the rows (Adam; learning rate halved for each of the last three quarters)
+  fit(rows, steps, opt) {
+    const o = opt || {};
+    const t = Surrogate.targets(rows);
+    this.fitScales(t);
+    addon.mlpLoadRows(this.h, t, 24);
+    const chunk = o.chunk || 500, batch = o.batch || 4096, lr = o.lr || 2e-3, losses = [];
+    for (let done = 0; done < steps; done += chunk) {
+      const k = Math.min(chunk, steps - done);
+      losses.push(addon.mlpTrain(this.h, k, batch, lr * Math.pow(0.5, Math.floor(4 * done / steps)), (o.seed || 1) + done, 1));
+    }
+    return losses;
+  }
+  infer(rows, stride) { return addon.mlpInfer(this.h, rows instanceof Float32Array ? rows : Float32Array.from(rows), stride || 24); }
+}
+
+module.exports = { Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

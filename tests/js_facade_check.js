@@ -24,5 +24,9 @@ m.rhoSfc = wk.B[2]; m.time = 42;
 const u = Array.from(m.writeUniforms(6));
 let perr = '';
 try { new O.PoissonDirect(30, 30, 10, 1000, 500); } catch (e) { perr = e.message; }
-console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr,
+// the surrogate's training targets for two teacher rows (Nr 0 -> 3e4, 5e3 -> 1e3)
+const rows = new Float32Array(48);
+rows[9] = 0; rows[21] = 3e4; rows[24 + 9] = 5e3; rows[24 + 21] = -4e3;
+const targets = Array.from(O.Surrogate.targets(rows));
+console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr, targets,
   levels: [O.levelsFor(2000), O.levelsFor(500), O.levelsFor(250)], pads: [O.gridPad(2000), O.gridPad(500), O.gridPad(250)] }));

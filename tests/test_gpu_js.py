@@ -1,0 +1,38 @@
+"""The Node facade on the GPU: 20 steps of the 128 x 128 x 40, 2 km Thompson workload driven through
+outbreak.js and the N-API addon equal, bit for bit, the same steps driven through the Python host
+(same library, same uniform blocks, deterministic kernels)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "outbreaklab_amd", "js", "obl_napi.node")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                 reason="node or the built addon is missing")]
+
+
+def test_node_facade_steps_equal_python_host(tmp_path):
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_plain_128
+
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    raw = np.fromfile(info["out"], np.float32)
+    nv, nq = info["nV"], info["nQ"]
+    Vj, Q1j, Q2j = raw[:nv], raw[nv:nv + nq], raw[nv + nq:]
+    m = RegionalModel()
+    dt = setup_plain_128(m, scheme="thompson")
+    for _ in range(20):
+        m.step(dt)
+    V, Q1, Q2 = m.state()
+    m.destroy()
+    assert info["parity"] == m.parity and info["time"] == m.time
+    assert np.array_equal(Vj, V.ravel()) and np.array_equal(Q1j, Q1.ravel()) and np.array_equal(Q2j, Q2.ravel())

@@ -26,7 +26,8 @@ def _run():
 
 def test_facade_loads_and_matches_golden():
     out = _run()
-    for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState"]:
+    for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState",
+                 "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer"]:
         assert name in out["exports"]
     assert out["mismatches"] == 0
     assert out["levels"] == [56, 80, 100] and out["pads"] == [8, 24, 40]
@@ -40,3 +41,13 @@ def test_facade_uniforms_equal_python_host():
     m.time = 42.0
     u = uniforms(m, 6.0)
     assert np.array_equal(np.array(out["uniforms"], np.float32), u)
+
+
+def test_facade_surrogate_targets_equal_python_host():
+    from outbreaklab_amd.surrogate import targets
+
+    out = _run()
+    rows = np.zeros((2, 24), np.float32)
+    rows[0, 9], rows[0, 21], rows[1, 9], rows[1, 21] = 0, 3e4, 5e3, -4e3
+    assert np.array_equal(np.array(out["targets"], np.float32).reshape(2, 24), targets(rows))
+
