@@ -536,14 +536,6 @@ __device__ __forceinline__ float spongeTop(const MU& u, const Dims& d, float z) 
   float s = sinf(1.5707963f * (z - u.v[U_SPONGE].z) / (ztop - u.v[U_SPONGE].z));
   return s * s / u.v[U_SPONGE].w;
 }
-template <class AV, class AQ>
-__device__ __forceinline__ float buoy(const AV& V, const AQ& Q1, const float4* Q2, const float4* BS, const Dims& d, int x,
-                                      int y, int z) {
-  float4 A = bsA(BS, z, d.nz);
-  float th = V.ldc(x, y, z, 3);
-  float4 q1 = Q1.ld4(x, y, z), q2 = ldQ(Q2, d, x, y, z);
-  return 9.81f * ((th - A.x) / A.x + 0.608f * (q1.x - A.y) - (q1.y + q1.z + q1.w + q2.x + q2.y));
-}
 __device__ __forceinline__ float2 ldA(const float2* aux, const Dims& d, int x, int y, int z) {
   return aux[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
 }
@@ -603,6 +595,18 @@ struct ZCol {
   }
 };
 
+// The values of a point's column that the vertical stencils and the w-face buoyancy need from
+// z - 1 / z + 1 (clamped texels), its Q2 texel and its 2-D map texels.
+struct PhysCol {
+  float4 Vm, Vp, Q1m, Q1p, Q2m, Q2c, LA, ci;
+  float2 am;
+  float pm;
+};
+// buoyancy of a cell (model.js:624-627) from its texels
+__device__ __forceinline__ float buoyv(float4 A, float th, float4 q1, float4 q2) {
+  return 9.81f * ((th - A.x) / A.x + 0.608f * (q1.x - A.y) - (q1.y + q1.z + q1.w + q2.x + q2.y));
+}
+
 enum { PHYS_PLAIN = 0, PHYS_DUMP = 1, PHYS_SURR = 2 };
 
 // Hand a point to the surrogate: one atomic per wave reserves consecutive record slots for the
@@ -621,9 +625,8 @@ __device__ __forceinline__ bool surr_slot(bool cand, unsigned* count, unsigned c
 
 template <int MODE, class AV, class AQ, class AX, class AP, class AB>
 __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const float4* __restrict__ BS, const AV& Vin,
-                                           const AQ& Q1in, const float4* __restrict__ Q2in, const AX& aux,
-                                           const float4* __restrict__ lsfA, const AB& lsfB,
-                                           const AP& phT, const float4* __restrict__ ciMap,
+                                           const AQ& Q1in, const PhysCol& col, const AX& aux, const AB& lsfB,
+                                           const AP& phT,
                                            float4* __restrict__ Vout, float4* __restrict__ Q1o, float4* __restrict__ Q2o,
                                            const MicroDump& dump, const SurrPhys& surr, int x, int y, int z) {
   int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -635,8 +638,8 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   bool drag = u.v[U_TURB].w > 0.5f;
   size_t vi = vidx(d, x, y, z);
   float4 V = Vin.ld4(x, y, z), o = V;
-  // the column's z neighbours, read once as whole texels (clamped)
-  const ZCol zcol{Vin.ld4(x, y, z - 1), V, Vin.ld4(x, y, z + 1), z};
+  // the column's z neighbours (clamped texels)
+  const ZCol zcol{col.Vm, V, col.Vp, z};
   // ---- u ----
   if (y < ny && z < nz) {
     float xmx = (float)x * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
@@ -682,14 +685,15 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   // ---- w ----
   if (x < nx && y < ny && z > 0 && z < nz) {
     float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = (float)z * dz;
-    float2 a0 = aux(x, y, z - 1), a1 = aux(x, y, z);
+    float2 a0 = col.am, a1 = aux(x, y, z);
     float nux = 0.5f * (a0.x + a1.x) + nu0, nuy = 0.5f * (a0.y + a1.y) + nu0;
     float lz = zcol.m.z + zcol.p.z - 2.0f * V.z;
     float dw = nux * lapH(Vin, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
-    dw += 0.5f * (buoy(zcol, Q1in, Q2in, BS, d, x, y, z - 1) + buoy(zcol, Q1in, Q2in, BS, d, x, y, z)) -
-          (phT(x, y, z) - phT(x, y, z - 1)) / dz;
+    dw += 0.5f * (buoyv(bsA(BS, z - 1, nz), col.Vm.w, col.Q1m, col.Q2m) +
+                  buoyv(bsA(BS, z, nz), V.w, Q1in.ld4(x, y, z), col.Q2c)) -
+          (phT(x, y, z) - col.pm) / dz;
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
-    float4 ci = ciMap[(size_t)x + (size_t)nx * y];
+    float4 ci = col.ci;
     if (ci.x > 0.0f) {
       float zr = fabsf(zz - ci.z) / ci.w;
       if (zr < 1.0f) {
@@ -705,7 +709,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   if (x < nx && y < ny && z < nz) {
     float xmx = ((float)x + 0.5f) * dx, xmy = ((float)y + 0.5f) * dx, zz = ((float)z + 0.5f) * dz;
     float4 A = bsA(BS, z, nz), B = bsB(BS, z, nz);
-    float4 LA = ld2d(lsfA, d, x, y);
+    float4 LA = col.LA;
     float nw = lsfB(x, y).w;
     float tth = tgtTh(BS, d, LA, z), tqv = tgtQv(BS, d, LA, z);
     float sl = spongeLat(u, d, xmx, xmy);
@@ -716,11 +720,11 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
                            nuy * lapZp(zcol, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
                            (relax + spongeTop(u, d, zz)) * (tth - V.w));
     size_t ci_ = cidx(d, x, y, z);
-    float4 q1 = Q1in.ld4(x, y, z), q2 = Q2in[ci_];
+    float4 q1 = Q1in.ld4(x, y, z), q2 = col.Q2c;
     float4 e1 = Q1in.ld4(x + 1, y, z), w1 = Q1in.ld4(x - 1, y, z);
     float4 n1 = Q1in.ld4(x, y + 1, z), s1 = Q1in.ld4(x, y - 1, z);
-    float4 dn = z == 0 ? q1 : Q1in.ld4(x, y, z - 1);
-    float4 up = z == nz - 1 ? q1 : Q1in.ld4(x, y, z + 1);
+    float4 dn = z == 0 ? q1 : col.Q1m;
+    float4 up = z == nz - 1 ? q1 : col.Q1p;
     float l1x = e1.x + w1.x + n1.x + s1.x - 4.0f * q1.x, l1y = e1.y + w1.y + n1.y + s1.y - 4.0f * q1.y;
     float l1zz = e1.z + w1.z + n1.z + s1.z - 4.0f * q1.z;
     float lzx = dn.x + up.x - 2.0f * q1.x, lzy = dn.y + up.y - 2.0f * q1.y, lzz = dn.z + up.z - 2.0f * q1.z;
@@ -935,11 +939,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
   }
 }
 
-// phys is compute-heavy (forces and microphysics) with few gathers: a flat 3-D grid keeps more
-// waves in flight than a z-marching block would, and its neighbour reads hit L1/L2
-// One block per 32 x 8 tile of one z-plane of the faces grid; the plane of V and of Q1 (with a
-// one-point halo) is staged in LDS, so the many x / y neighbour reads of the tendencies and
-// diffusion come from LDS; the z neighbours are read from global memory.
+// One block per 32 x 8 column tile of the faces grid, walking z upward like the advection
+// kernels: V and Q1 are three-plane LDS rings (the next plane prefetched into registers during a
+// level's compute), aux and phT the current plane, lsfB the 2-D tile; the vertical stencils read
+// z - 1 / z + 1 from the rings, and Q2 / aux / phT at z - 1 are carried from the level below. Every
+// field crosses HBM about once per step instead of once per neighbouring level.
 template <int MODE>
 __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
                                                        const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
@@ -948,38 +952,85 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
                                                        const float4* __restrict__ ciMap, float4* __restrict__ Vout,
                                                        float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump,
                                                        SurrPhys sp) {
-  __shared__ float sV[4 * TPS], sQ[4 * TPS], sA[2 * TPS], sP[TPS], sB[4 * TPS];
-  Dims d = dims(u);
-  const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z = blockIdx.z;
-  Plane pv{(lds_float*)sV, Vin, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, z};
-  Plane pq{(lds_float*)sQ, Q1in, d.nx, d.ny, d.nz, x0, y0, z};
-  pv.load(tid);
-  pq.load(tid);
-  {
-    // aux, phT (cells, this plane) and lsfB (2-D), clamped like ldA / ldP / ld2d
-    const int zz = clampi(z, 0, d.nz - 1);
-    for (int t = tid; t < TPS; t += TILE_THREADS) {
-      int i = clampi(x0 - 1 + t % TPW, 0, d.nx - 1), j = clampi(y0 - 1 + t / TPW, 0, d.ny - 1);
-      size_t c = (size_t)i + (size_t)d.nx * ((size_t)j + (size_t)d.ny * (size_t)zz);
-      float2 a = aux[c];
-      sA[t] = a.x;
-      sA[TPS + t] = a.y;
-      sP[t] = phT[c];
-      float4 b = lsfB[(size_t)i + (size_t)d.nx * (size_t)j];
-      sB[t] = b.x;
-      sB[TPS + t] = b.y;
-      sB[2 * TPS + t] = b.z;
-      sB[3 * TPS + t] = b.w;
+  __shared__ float lds[2 * RING_FLOATS];
+  __shared__ float sA[2 * TPS], sP[TPS], sB[4 * TPS];
+  TILE_SETUP
+  const bool act = x <= d.nx && y <= d.ny;
+  Ring rv{lds, Vin, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rq{lds + RING_FLOATS, Q1in, d.nx, d.ny, d.nz, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) { rv.load(k, tid); rq.load(k, tid); }
+  // aux / phT tile texels of this thread (clamped like ldA / ldP), and the 2-D lsfB tile
+  constexpr int NQ = (TPS + TILE_THREADS - 1) / TILE_THREADS;
+  size_t cof[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    int t = tid + q * TILE_THREADS;
+    int i = clampi(x0 - 1 + t % TPW, 0, d.nx - 1), j = clampi(y0 - 1 + t / TPW, 0, d.ny - 1);
+    cof[q] = (size_t)i + (size_t)d.nx * (size_t)j;
+    if (t < TPS) {
+      float4 b = lsfB[cof[q]];
+      sB[t] = b.x; sB[TPS + t] = b.y; sB[2 * TPS + t] = b.z; sB[3 * TPS + t] = b.w;
+      float2 a = aux[cof[q]];
+      sA[t] = a.x; sA[TPS + t] = a.y;
+      sP[t] = phT[cof[q]];
     }
   }
+  const size_t plane = (size_t)d.nx * d.ny;
+  PhysCol col;
+  col.LA = ld2d(lsfA, d, x, y);
+  col.ci = ciMap[(size_t)clampi(x, 0, d.nx - 1) + (size_t)d.nx * clampi(y, 0, d.ny - 1)];
+  col.Q2m = ldQ(Q2in, d, x, y, 0);
+  col.am = ldA(aux, d, x, y, 0);
+  col.pm = ldP(phT, d, x, y, 0);
   __syncthreads();
-  const int x = x0 + (tid % TX), y = y0 + (tid / TX);
-  if (x > d.nx || y > d.ny || z > d.nz) return;
-  TAux ta{(const lds_float*)sA, aux, d, x0, y0, z};
-  TPh tp{(const lds_float*)sP, phT, d, x0, y0, z};
-  T2d tb{(const lds_float*)sB, x0, y0};
-  phys_point<MODE>(u, d, BS, pv, pq, Q2in, ta, lsfA, tb, tp, ciMap, Vout, Q1o, Q2o, dump, sp, x, y, z);
+  const T2d tb{(const lds_float*)sB, x0, y0};
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv, pq;
+    float2 na[NQ];
+    float np[NQ];
+    const bool more = z < d.nz;
+    if (more) {
+      rv.fetch(z + 2, tid, pv);
+      rq.fetch(z + 2, tid, pq);
+      const size_t zo = plane * (size_t)clampi(z + 1, 0, d.nz - 1);
+#pragma unroll
+      for (int q = 0; q < NQ; q++)
+        if (tid + q * TILE_THREADS < TPS) {
+          na[q] = aux[cof[q] + zo];
+          np[q] = phT[cof[q] + zo];
+        }
+    }
+    col.Q2c = ldQ(Q2in, d, x, y, z);
+    rv.center(z);
+    rq.center(z);
+    const RingW wv = rv.w(), wq = rq.w();
+    if (act) {
+      col.Vm = wv.ld4(x, y, z - 1);
+      col.Vp = wv.ld4(x, y, z + 1);
+      col.Q1m = wq.ld4(x, y, z - 1);
+      col.Q1p = wq.ld4(x, y, z + 1);
+      const TAux ta{(const lds_float*)sA, aux, d, x0, y0, z};
+      const TPh tp{(const lds_float*)sP, phT, d, x0, y0, z};
+      phys_point<MODE>(u, d, BS, wv, wq, col, ta, tb, tp, Vout, Q1o, Q2o, dump, sp, x, y, z);
+      col.Q2m = col.Q2c;
+      col.am = ta(x, y, z);
+      col.pm = tp(x, y, z);
+    }
+    __syncthreads();
+    if (more) {
+      rv.store(z + 2, tid, pv);
+      rq.store(z + 2, tid, pq);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        int t = tid + q * TILE_THREADS;
+        if (t < TPS) {
+          sA[t] = na[q].x; sA[TPS + t] = na[q].y;
+          sP[t] = np[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ====================== DIV (model.js:910-925) ======================
@@ -1043,7 +1094,6 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
   hipLaunchKernelGGL(k_bubble, faces_grid(d), dim3(64, 4, 1), 0, s, u, V0, V1, x, y, z, rh, rz, dth);
 }
 static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
-static dim3 tface_grid3(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, d.nz + 1); }
 static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 float2* aux) {
@@ -1068,21 +1118,21 @@ void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, co
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
   SurrPhys nos = {};
-  hipLaunchKernelGGL((k_phys<PHYS_PLAIN>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+  hipLaunchKernelGGL((k_phys<PHYS_PLAIN>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
                      lsfB, phT, ciMap, Vout, Q1o, Q2o, none, nos);
 }
 void launch_phys_surr(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const SurrPhys& sp) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys<PHYS_SURR>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+  hipLaunchKernelGGL((k_phys<PHYS_SURR>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
                      lsfB, phT, ciMap, Vout, Q1o, Q2o, none, sp);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
   SurrPhys nos = {};
-  hipLaunchKernelGGL((k_phys<PHYS_DUMP>), tface_grid3(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
+  hipLaunchKernelGGL((k_phys<PHYS_DUMP>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
                      lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, nos);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
