@@ -4,8 +4,8 @@
 // Training (this file): one block takes 16 sample rows through the forward pass and back,
 // in FP32 arithmetic; with `quant` on, weights, features and hidden activations are rounded
 // to BF16 exactly where the inference kernel feeds them to BF16 MFMA (straight-through
-// gradients). Weight gradients of the block go to a global FP32 buffer with atomicAdd;
-// k_adam applies Adam to the FP32 master parameters.
+// gradients). Each block writes its own gradient and loss partials; k_mlp_reduce sums them in block
+// order, so training is reproducible run to run; k_adam applies Adam to the FP32 master parameters.
 //
 // Parameter layout (FP32): W1[16][128] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]
 // W4[128][16] b4[16] (W[k][c]: input k, output c).
@@ -31,12 +31,11 @@ __device__ __forceinline__ float qz(float x, bool q) { return q ? bf16_round(x) 
 __global__ void __launch_bounds__(256) k_mlp_grad(MlpTrainArgs a) {
   __shared__ float f[MR][MLP_IN], tg[MR][MLP_OUT], yv[MR][MLP_OUT];
   __shared__ float z1[MR][MLP_H], z2[MR][MLP_H], z3[MR][MLP_H], hb[MR][MLP_H], db[MR][MLP_H];
-  __shared__ float lsum;
+  __shared__ float sq[MR][MLP_OUT];
   const int tid = threadIdx.x;
   const int row0 = blockIdx.x * MR;
   const bool q = a.quant != 0;
   const float* P = a.P;
-  if (tid == 0) lsum = 0.0f;
   if (tid < MR) {
     int r = row0 + tid;
     bool ok = r < a.batch;
@@ -115,15 +114,20 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpTrainArgs a) {
     bool ok = row0 + r < a.batch && j < MLP_NOUT;
     float dd = ok ? acc - tg[r][j] : 0.0f;
     if (a.y_out && row0 + r < a.batch) a.y_out[(size_t)(row0 + r) * MLP_OUT + j] = acc;
-    atomicAdd(&lsum, dd * dd);
+    sq[r][j] = dd * dd;
     // dy, kept in yv
     __syncthreads();
     yv[r][j] = 2.0f * dd * a.loss_scale;
   }
   __syncthreads();
-  if (tid == 0 && a.loss) atomicAdd(a.loss, lsum);
+  if (tid == 0 && a.lpart) {
+    float l = 0.0f;
+    for (int r = 0; r < MR; r++)
+      for (int j = 0; j < MLP_OUT; j++) l += sq[r][j];
+    a.lpart[blockIdx.x] = l;
+  }
   if (!a.G) return;
-  float* G = a.G;
+  float* G = a.G + (size_t)blockIdx.x * MLP_NPARAM;  // this block's partial gradient
   // ---- backward ----
   // dW4[k][j] = sum_r h3[r][k] dy[r][j]; db4[j] = sum_r dy[r][j]
   for (int e = tid; e < MLP_H * MLP_OUT; e += 256) {
@@ -131,13 +135,13 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpTrainArgs a) {
     float s = 0.0f;
 #pragma unroll
     for (int r = 0; r < MR; r++) s += hb[r][k] * yv[r][j];
-    atomicAdd(&G[MLP_W4 + e], s);
+    G[MLP_W4 + e] = s;
   }
   if (tid < MLP_OUT) {
     float s = 0.0f;
 #pragma unroll
     for (int r = 0; r < MR; r++) s += yv[r][tid];
-    atomicAdd(&G[MLP_B4 + tid], s);
+    G[MLP_B4 + tid] = s;
   }
   // dh3[r][k] = sum_j dy[r][j] W4[k][j]; dz3 = dh3 * gelu'(z3)
 #pragma unroll
@@ -162,13 +166,13 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpTrainArgs a) {
       float s = 0.0f;
 #pragma unroll
       for (int r = 0; r < MR; r++) s += hb[r][k] * db[r][c];
-      atomicAdd(&G[woff[L] + k * MLP_H + c], s);
+      G[woff[L] + k * MLP_H + c] = s;
     }
     if (tid < MLP_H) {
       float s = 0.0f;
 #pragma unroll
       for (int r = 0; r < MR; r++) s += db[r][tid];
-      atomicAdd(&G[boff[L] + tid], s);
+      G[boff[L] + tid] = s;
     }
     float nd[8];
 #pragma unroll
@@ -189,13 +193,29 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpTrainArgs a) {
     float s = 0.0f;
 #pragma unroll
     for (int r = 0; r < MR; r++) s += f[r][k] * db[r][cc];
-    atomicAdd(&G[MLP_W1 + e], s);
+    G[MLP_W1 + e] = s;
   }
   if (tid < MLP_H) {
     float s = 0.0f;
 #pragma unroll
     for (int r = 0; r < MR; r++) s += db[r][tid];
-    atomicAdd(&G[MLP_B1 + tid], s);
+    G[MLP_B1 + tid] = s;
+  }
+}
+
+// G[p] = sum over blocks b of Gpart[b][p], in block order; loss = sum of the blocks' loss partials
+__global__ void k_mlp_reduce(const float* __restrict__ Gpart, const float* __restrict__ lpart, int nblk,
+                             float* __restrict__ G, float* __restrict__ loss) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < MLP_NPARAM && Gpart) {
+    float s = 0.0f;
+    for (int b = 0; b < nblk; b++) s += Gpart[(size_t)b * MLP_NPARAM + p];
+    G[p] = s;
+  }
+  if (p == 0 && loss) {
+    float l = 0.0f;
+    for (int b = 0; b < nblk; b++) l += lpart[b];
+    *loss += l;
   }
 }
 
@@ -271,8 +291,11 @@ void launch_mlp_eval(hipStream_t s, const uint16_t* frag, const float* bias, con
   hipLaunchKernelGGL(k_mlp_eval, dim3((n + 255) / 256), dim3(256), 0, s, frag, bias, rows, stride, n, y);
 }
 
-void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a) {
-  hipLaunchKernelGGL(k_mlp_grad, dim3((a.batch + MR - 1) / MR), dim3(256), 0, s, a);
+int mlp_grad_blocks(int batch) { return (batch + MR - 1) / MR; }
+void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a, float* G, float* loss) {
+  const int nblk = mlp_grad_blocks(a.batch);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, a.G, a.lpart, nblk, G, loss);
 }
 void launch_adam(hipStream_t s, float* P, float* G, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                  int t) {

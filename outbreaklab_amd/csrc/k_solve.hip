@@ -27,7 +27,7 @@ namespace obl {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 64, BN = 64, BK = 16, LDK = BK + 4;
+constexpr int BM = 64, BN = 64, BK = 16;
 
 // bf16 split of an fp32 value: hi = RNE(x), lo = RNE(x - hi)
 __device__ __forceinline__ short bf16_rne(float x) {
@@ -66,22 +66,28 @@ __device__ __forceinline__ void dst_tile(int& mt, int& nt, int& plane) {
   }
 }
 
+// LDS layout of the k-major tiles (rows of BK = 16 floats, no padding): the four 4-float chunks of
+// row r are stored in the order q ^ ((r >> 2) & 3). A wave's float4 stores (4 threads per row, 16
+// consecutive rows) and its fragment reads (16 rows, one chunk each) then fall in distinct banks.
+__device__ __forceinline__ int swz(int r, int q) { return r * BK + 4 * (q ^ ((r >> 2) & 3)); }
+constexpr int YLD = BN + 4;  // Y-mode data tile row ([k][n], n contiguous)
+
 template <int MODE, int PREC>
 __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
   // one LDS stage; the global loads of stage s + 1 go to registers before the MFMAs of
   // stage s, so their latency overlaps this wave's own MFMAs
-  __shared__ float As[2][BM][LDK];
-  __shared__ float Bs[2][BN][LDK];
+  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
+  __shared__ __attribute__((aligned(16))) float As[2][BM * BK];                 // k-major rows (swizzled)
+  __shared__ __attribute__((aligned(16))) float Bs[2][XMODE ? BN * BK : BK * YLD];  // X: k-major; Y: [k][n]
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, kq = lane >> 4;
   int mt, nt, plane;
-  dst_tile<MODE == DST_XF || MODE == DST_XI>(mt, nt, plane);
+  dst_tile<XMODE>(mt, nt, plane);
   const int m0 = mt * BM, n0 = nt * BN;
   const int N = a.N, Nh = a.Nh;
   const size_t planeSz = (size_t)N * (size_t)a.inner;
-  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
 
   v4f acc[2][2][2];
 #pragma unroll
@@ -131,16 +137,17 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
     }
   };
   auto stash = [&]() {
+    const int o = swz(row, q);
     if (XMODE) {
-      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = d0;
-      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = d1;
-      *reinterpret_cast<float4*>(&Bs[0][row][4 * q]) = w0;
-      *reinterpret_cast<float4*>(&Bs[1][row][4 * q]) = w1;
+      *reinterpret_cast<float4*>(&As[0][o]) = d0;
+      *reinterpret_cast<float4*>(&As[1][o]) = d1;
+      *reinterpret_cast<float4*>(&Bs[0][o]) = w0;
+      *reinterpret_cast<float4*>(&Bs[1][o]) = w1;
     } else {
-      *reinterpret_cast<float4*>(&As[0][row][4 * q]) = w0;
-      *reinterpret_cast<float4*>(&As[1][row][4 * q]) = w1;
-      Bs[0][4 * qq + 0][kk] = d0.x; Bs[0][4 * qq + 1][kk] = d0.y; Bs[0][4 * qq + 2][kk] = d0.z; Bs[0][4 * qq + 3][kk] = d0.w;
-      Bs[1][4 * qq + 0][kk] = d1.x; Bs[1][4 * qq + 1][kk] = d1.y; Bs[1][4 * qq + 2][kk] = d1.z; Bs[1][4 * qq + 3][kk] = d1.w;
+      *reinterpret_cast<float4*>(&As[0][o]) = w0;
+      *reinterpret_cast<float4*>(&As[1][o]) = w1;
+      *reinterpret_cast<float4*>(&Bs[0][kk * YLD + 4 * qq]) = d0;
+      *reinterpret_cast<float4*>(&Bs[1][kk * YLD + 4 * qq]) = d1;
     }
   };
 
@@ -156,9 +163,16 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
     for (int p = 0; p < 2; p++) {
       float4 af[2], bf[2];
 #pragma unroll
-      for (int i = 0; i < 2; i++) af[i] = *reinterpret_cast<const float4*>(&As[p][wm * 32 + i * 16 + lr][4 * kq]);
+      for (int i = 0; i < 2; i++) af[i] = *reinterpret_cast<const float4*>(&As[p][swz(wm * 32 + i * 16 + lr, kq)]);
 #pragma unroll
-      for (int j = 0; j < 2; j++) bf[j] = *reinterpret_cast<const float4*>(&Bs[p][wn * 32 + j * 16 + lr][4 * kq]);
+      for (int j = 0; j < 2; j++) {
+        if (XMODE) {
+          bf[j] = *reinterpret_cast<const float4*>(&Bs[p][swz(wn * 32 + j * 16 + lr, kq)]);
+        } else {
+          const float* bp = &Bs[p][(4 * kq) * YLD + wn * 32 + j * 16 + lr];
+          bf[j] = make_float4(bp[0], bp[YLD], bp[2 * YLD], bp[3 * YLD]);
+        }
+      }
       if (PREC == 0) {
         // k step outermost: four independent accumulators between two updates of one
 #pragma unroll

@@ -49,7 +49,8 @@ __device__ __forceinline__ float gelu_grad_f(float z) {
 
 struct MlpTrainArgs {
   const float* P;       // parameters
-  float* G;             // gradient accumulator (nullptr: forward / loss only)
+  float* G;             // per-block gradient partials [blocks][MLP_NPARAM] (nullptr: forward / loss only)
+  float* lpart;         // per-block loss partials [blocks]
   const float* rows;    // sample rows (14 inputs, 9 outputs, ...)
   int stride;           // floats per row
   size_t nrows;
@@ -58,7 +59,6 @@ struct MlpTrainArgs {
   int batch;
   float out_inv_scale[MLP_OUT];
   float loss_scale;     // 1 / (batch * MLP_NOUT)
-  float* loss;          // sum of squared errors (atomic)
   float* y_out;         // optional [batch][MLP_OUT] network outputs
   int quant;            // round weights / activations to BF16 as inference does
 };
@@ -72,7 +72,10 @@ struct MlpView {
   float dead_frac;  // dead zone of absent species, as a fraction of each output scale
 };
 
-void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a);
+int mlp_grad_blocks(int batch);
+// the blocks' partials of a.G / a.lpart, then their sums in block order into G (if a.G) and
+// added to *loss (if loss)
+void launch_mlp_grad(hipStream_t s, const MlpTrainArgs& a, float* G, float* loss);
 void launch_mlp_eval(hipStream_t s, const uint16_t* frag, const float* bias, const float* rows, int stride, int n,
                      float* y);
 void launch_adam(hipStream_t s, float* P, float* G, float* m, float* v, int n, float lr, float b1, float b2, float eps,

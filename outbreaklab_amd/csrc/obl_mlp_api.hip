@@ -17,6 +17,8 @@ using namespace obl;
 struct obl_mlp {
   hipStream_t stream = nullptr;
   float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr, *loss = nullptr;
+  float *Gpart = nullptr, *lpart = nullptr;  // per-block gradient and loss partials
+  int part_blocks = 0;
   uint16_t* frag = nullptr;  // inference form: packed BF16 weights and FP32 biases
   float* bias = nullptr;
   float* rows = nullptr;  // rows of obl_mlp_load_rows
@@ -71,7 +73,7 @@ extern "C" const char* obl_mlp_last_error(const obl_mlp* m) { return m ? m->err.
 extern "C" void obl_mlp_destroy(obl_mlp* m) {
   if (!m) return;
   if (m->stream) hipStreamSynchronize(m->stream);
-  void* b[] = {m->P, m->G, m->M, m->V, m->loss, m->rows, m->frag, m->bias};
+  void* b[] = {m->P, m->G, m->M, m->V, m->loss, m->rows, m->frag, m->bias, m->Gpart, m->lpart};
   for (void* p : b)
     if (p) hipFree(p);
   if (m->stream) hipStreamDestroy(m->stream);
@@ -146,6 +148,24 @@ extern "C" int obl_mlp_set_dead_zone(obl_mlp* m, float frac) {
   return 0;
 }
 
+// partial buffers for a batch (grown on demand)
+static int ensure_parts(obl_mlp* m, int batch) {
+  int nb = mlp_grad_blocks(batch);
+  if (nb <= m->part_blocks) return 0;
+  if (m->Gpart || m->lpart) {
+    hipStreamSynchronize(m->stream);
+    hipFree(m->Gpart);
+    hipFree(m->lpart);
+    m->Gpart = m->lpart = nullptr;
+    m->part_blocks = 0;
+  }
+  if (m->ok(hipMalloc(&m->Gpart, (size_t)nb * MLP_NPARAM * 4), "gradient partials") ||
+      m->ok(hipMalloc(&m->lpart, (size_t)nb * 4), "loss partials"))
+    return -1;
+  m->part_blocks = nb;
+  return 0;
+}
+
 static void fill_args(obl_mlp* m, MlpTrainArgs& a, const float* rows, size_t nrows, int stride, int batch, int quant) {
   memset(&a, 0, sizeof a);
   a.P = m->P;
@@ -155,7 +175,7 @@ static void fill_args(obl_mlp* m, MlpTrainArgs& a, const float* rows, size_t nro
   a.batch = batch;
   for (int j = 0; j < MLP_OUT; j++) a.out_inv_scale[j] = 1.0f / m->out_scale[j];
   a.loss_scale = 1.0f / ((float)batch * (float)MLP_NOUT);
-  a.loss = m->loss;
+  a.lpart = m->lpart;
   a.quant = quant;
 }
 
@@ -190,13 +210,14 @@ extern "C" int obl_mlp_train(obl_mlp* m, const float* rows_dev, size_t nrows, in
   }
   if (!rows_dev || nrows == 0 || stride < MLP_NIN + MLP_NOUT || batch < 1 || steps < 1)
     return m->fail("train: need rows (stride >= %d), batch >= 1, steps >= 1", MLP_NIN + MLP_NOUT);
+  if (ensure_parts(m, batch)) return -1;
   MlpTrainArgs a;
   fill_args(m, a, rows_dev, nrows, stride, batch, quant);
-  a.G = m->G;
+  a.G = m->Gpart;
   if (m->ok(hipMemsetAsync(m->loss, 0, 4, m->stream), "loss")) return -1;
   for (int s = 0; s < steps; s++) {
     a.seed = seed * 2654435761u + (unsigned)s;
-    launch_mlp_grad(m->stream, a);
+    launch_mlp_grad(m->stream, a, m->G, m->loss);
     launch_adam(m->stream, m->P, m->G, m->M, m->V, MLP_NPARAM, lr, 0.9f, 0.999f, 1e-8f, ++m->adam_t);
   }
   if (m->ok(hipGetLastError(), "train launch")) return -1;
@@ -248,20 +269,22 @@ extern "C" int obl_mlp_eval(obl_mlp* m, const float* rows_host, int n, int strid
   hipMemcpyAsync(drows, rows_host, rb, hipMemcpyHostToDevice, m->stream);
   hipMemcpyAsync(didx, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->stream);
   hipMemsetAsync(m->loss, 0, 4, m->stream);
-  if (grads) hipMemsetAsync(m->G, 0, (size_t)MLP_NPARAM * 4, m->stream);
+  if (ensure_parts(m, n)) {
+    hipFree(drows); hipFree(didx); hipFree(dy);
+    return -1;
+  }
   MlpTrainArgs a;
   fill_args(m, a, drows, (size_t)n, stride, n, quant);
   a.idx = didx;
-  a.G = grads ? m->G : nullptr;
+  a.G = grads ? m->Gpart : nullptr;
   a.y_out = dy;
-  launch_mlp_grad(m->stream, a);
+  launch_mlp_grad(m->stream, a, m->G, m->loss);
   float l = 0.f;
   int rc = m->ok(hipGetLastError(), "eval launch");
   if (!rc) rc = m->ok(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream), "loss");
   if (!rc && grads) rc = m->ok(hipMemcpyAsync(grads, m->G, (size_t)MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "grads");
   if (!rc && y) rc = m->ok(hipMemcpyAsync(y, dy, (size_t)n * MLP_OUT * 4, hipMemcpyDeviceToHost, m->stream), "y");
   if (!rc) rc = m->ok(hipStreamSynchronize(m->stream), "eval");
-  if (!rc && grads) hipMemsetAsync(m->G, 0, (size_t)MLP_NPARAM * 4, m->stream);
   hipStreamSynchronize(m->stream);
   hipFree(drows);
   hipFree(didx);

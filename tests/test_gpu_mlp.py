@@ -104,3 +104,20 @@ def test_adam_fits_smooth_map():
     print(f"loss {l0:.4e} -> {l1:.4e}")
     assert l1 < 0.05 * l0
     s.destroy()
+
+
+def test_training_is_reproducible():
+    """Gradients are summed in block order (no atomics), so the same rows, seeds and steps give
+    bit-identical parameters."""
+    from outbreaklab_amd.surrogate import Surrogate
+
+    rows = synth_rows(5000, 9)
+    out = []
+    for _ in range(2):
+        s = Surrogate(seed=4)
+        s.set_scales(np.ones(9, np.float32))
+        s.load_rows(rows)
+        s.train(steps=50, batch=1024, lr=1e-3, seed=7, quant=True)
+        out.append(s.params())
+        s.destroy()
+    assert np.array_equal(out[0], out[1])
