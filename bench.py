@@ -172,6 +172,45 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     }
 
 
+def solve_alone(nx=1024, ny=1024, nz=80, dx=500.0):
+    """The pressure solve alone on 1024 x 1024 x 80 (dz 250 m, no padding): a seeded k^-5/3 RHS,
+    FP32 and split-BF16 transforms with mixed Thomas, device time per solve and the relative error
+    against an FP64 separable solve (scipy DST-II, independent of this library)."""
+    import numpy as np
+    import scipy.fft
+
+    from outbreaklab_amd import PoissonDirect
+    from outbreaklab_amd.configs import solve_rhs_1024
+    from outbreaklab_amd.sounding import base_state, parse_sharppy, wk_sounding_text
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _ref_solve import solve_scipy  # noqa: E402
+
+    dz = 20000.0 / nz
+    B = base_state(parse_sharppy(wk_sounding_text()), nz, dz)["B"]
+    rc, rw = B[2:8 * nz:8].astype(np.float64)[:nz], B[3::8].astype(np.float64)[: nz + 1]
+    rhs = solve_rhs_1024(nx, ny, nz, seed=1)
+    with scipy.fft.set_workers(16):
+        ref = solve_scipy(rhs, dx, dz, rc, rw)
+    ps = PoissonDirect(nx, ny, nz, dx, dz)
+    ps.setDensity(rc, rw)
+    out = {"grid": f"{nx}x{ny}x{nz}, dx {dx:.0f} m, dz {dz:.0f} m", "thomas": "mixed (FP64 below condition 16)"}
+    r32 = rhs.astype(np.float32)
+    for gemm in ("fp32", "bf16x3"):
+        phi = ps.solve(r32, gemm=gemm, thomas="mixed")
+        ps.profile(True)
+        for _ in range(3):
+            ps.solve(r32, gemm=gemm, thomas="mixed")
+        prof = ps.profile_read()
+        ps.profile(False)
+        ms = sum(prof.values())
+        err = float(np.sqrt(np.sum((phi - ref) ** 2) / np.sum(ref ** 2)))
+        out[gemm] = {"ms_per_solve": round(ms, 4), "passes_ms": {k: round(v, 4) for k, v in prof.items()},
+                     "rel_err_vs_fp64": err}
+    ps.destroy()
+    return out
+
+
 def calibrate_copy(mib: int):
     """Plain device-to-device copies of a known size: FETCH_SIZE / WRITE_SIZE of their dispatches
     give the scale from the HBM counters to bytes."""
@@ -267,6 +306,13 @@ def main():
             comb = combined_form(m, dt, args.steps, spin=750, teach_steps=40, p_active=0.02, train_steps=8000)
         except Exception as e:  # reported, not fatal to the plain measurement
             comb = {"error": f"{type(e).__name__}: {e}"}
+    solve = None
+    if not args.no_combined:
+        print("# pressure solve alone (FP32 / split BF16 vs FP64) ...", file=sys.stderr, flush=True)
+        try:
+            solve = solve_alone()
+        except Exception as e:  # reported, not fatal to the plain measurement
+            solve = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -292,6 +338,7 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
         "combined_form": comb,
+        "solve_alone": solve,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
     }

@@ -268,6 +268,8 @@ __device__ __forceinline__ void split8_hw(float4 v0, float4 v1, bf16x8& hi, bf16
 
 template <int MODE>
 __global__ void __launch_bounds__(256, 2) k_dst_bf16(DstArgs a) {
+  // One LDS stage. The global loads of stage s + 1 are issued into registers before the MFMAs of
+  // stage s and written to LDS after them, so their latency overlaps this stage's MFMAs.
   constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
   // data: [parity][row (X: data row; Y: output column n)][k] FP32; matrix: [parity][hi/lo][row][k]
   __shared__ __attribute__((aligned(16))) float Ds[2][XMODE ? BBM : BBN][DLD];
@@ -290,64 +292,79 @@ __global__ void __launch_bounds__(256, 2) k_dst_bf16(DstArgs a) {
 #pragma unroll
       for (int j = 0; j < 4; j++) acc[p][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < Nh; k0 += BBK) {
-    // ---- matrix operand (pre-split bf16): 128 rows x 32 k per array ----
-    {
-      int row = tid >> 1, half = tid & 1;
-      int mrow = (XMODE ? n0 : m0) + row;
-      const size_t wo = (size_t)mrow * a.KpB + k0 + 16 * half;
-      const uint16_t* src[4] = {a.W0h, a.W0l, a.W1h, a.W1l};
+  // staging registers: matrix [array][half of the 32 k] and data (X: 4 float4 of row `row` at
+  // k, then 4 at the mirrored / odd position; Y: 4 k rows of 4 columns, then their partners)
+  uint4 w0a, w0b, w1a, w1b, w2a, w2b, w3a, w3b;
+  float4 pd[8];
+  const int row = tid >> 1, half = tid & 1;
+  const int ng = tid & 31, kq = tid >> 5;
+  const float* pl = a.in + (XMODE ? 0 : (size_t)plane * planeSz);
+  auto fetch = [&](int k0) {
+    const size_t wo = (size_t)((XMODE ? n0 : m0) + row) * a.KpB + k0 + 16 * half;
+    const uint4* q0 = reinterpret_cast<const uint4*>(a.W0h + wo);
+    const uint4* q1 = reinterpret_cast<const uint4*>(a.W0l + wo);
+    const uint4* q2 = reinterpret_cast<const uint4*>(a.W1h + wo);
+    const uint4* q3 = reinterpret_cast<const uint4*>(a.W1l + wo);
+    w0a = q0[0]; w0b = q0[1];
+    w1a = q1[0]; w1b = q1[1];
+    w2a = q2[0]; w2b = q2[1];
+    w3a = q3[0]; w3b = q3[1];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const uint4* q = reinterpret_cast<const uint4*>(src[t] + wo);
-        uint4 v0 = q[0], v1 = q[1];
-        *reinterpret_cast<uint4*>(&Ms[t >> 1][t & 1][row][16 * half]) = v0;
-        *reinterpret_cast<uint4*>(&Ms[t >> 1][t & 1][row][16 * half + 8]) = v1;
-      }
-    }
-    // ---- data operand, FP32, even / odd folded ----
+    for (int c = 0; c < 8; c++) pd[c] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (XMODE) {
-      int row = tid >> 1, half = tid & 1;
-      int r = m0 + row;
-      const float* src = a.in + (size_t)r * N;
+      const int r = m0 + row;
+      const float* sr = a.in + (size_t)r * N;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        int kk = k0 + 16 * half + 4 * c;
-        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        const int kk = k0 + 16 * half + 4 * c;
         if (r < a.rows && kk < Nh) {
-          float4 f = *reinterpret_cast<const float4*>(src + kk);
-          if (MODE == DST_XF) {
-            float4 gg = *reinterpret_cast<const float4*>(src + (N - 4 - kk));
-            s0 = make_float4(f.x + gg.w, f.y + gg.z, f.z + gg.y, f.w + gg.x);
-            s1 = make_float4(f.x - gg.w, f.y - gg.z, f.z - gg.y, f.w - gg.x);
-          } else {
-            s0 = f;
-            s1 = *reinterpret_cast<const float4*>(src + Nh + kk);
-          }
+          pd[c] = *reinterpret_cast<const float4*>(sr + kk);
+          pd[4 + c] = *reinterpret_cast<const float4*>(sr + (MODE == DST_XF ? N - 4 - kk : Nh + kk));
+        }
+      }
+    } else {
+      const int gn = n0 + 4 * ng;
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        const int k = k0 + 4 * kq + kk;
+        if (k < Nh && gn < a.inner) {
+          pd[kk] = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + gn);
+          pd[4 + kk] = *reinterpret_cast<const float4*>(pl + (size_t)(MODE == DST_YF ? N - 1 - k : Nh + k) * a.inner + gn);
+        }
+      }
+    }
+  };
+  auto stash = [&]() {
+    *reinterpret_cast<uint4*>(&Ms[0][0][row][16 * half]) = w0a;
+    *reinterpret_cast<uint4*>(&Ms[0][0][row][16 * half + 8]) = w0b;
+    *reinterpret_cast<uint4*>(&Ms[0][1][row][16 * half]) = w1a;
+    *reinterpret_cast<uint4*>(&Ms[0][1][row][16 * half + 8]) = w1b;
+    *reinterpret_cast<uint4*>(&Ms[1][0][row][16 * half]) = w2a;
+    *reinterpret_cast<uint4*>(&Ms[1][0][row][16 * half + 8]) = w2b;
+    *reinterpret_cast<uint4*>(&Ms[1][1][row][16 * half]) = w3a;
+    *reinterpret_cast<uint4*>(&Ms[1][1][row][16 * half + 8]) = w3b;
+    if (XMODE) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const float4 f = pd[c], h = pd[4 + c];
+        float4 s0 = f, s1 = h;
+        if (MODE == DST_XF) {
+          s0 = make_float4(f.x + h.w, f.y + h.z, f.z + h.y, f.w + h.x);
+          s1 = make_float4(f.x - h.w, f.y - h.z, f.z - h.y, f.w - h.x);
         }
         *reinterpret_cast<float4*>(&Ds[0][row][16 * half + 4 * c]) = s0;
         *reinterpret_cast<float4*>(&Ds[1][row][16 * half + 4 * c]) = s1;
       }
     } else {
-      // 4 n x 4 k per thread: float4 rows in, transposed to [n][k] (float4 along k) out
-      int ng = tid & 31, kq = tid >> 5;
-      int gn = n0 + 4 * ng;
-      const float* pl = a.in + (size_t)plane * planeSz;
-      float s0[4][4], s1[4][4];  // [k][n]
+      // 4 k x 4 n per thread, transposed to [n][k] (float4 along k)
+      float s0[4][4], s1[4][4];
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) {
-        int k = k0 + 4 * kq + kk;
-        float4 f0 = make_float4(0.f, 0.f, 0.f, 0.f), f1 = f0;
-        if (k < Nh && gn < a.inner) {
-          float4 f = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + gn);
-          if (MODE == DST_YF) {
-            float4 mm = *reinterpret_cast<const float4*>(pl + (size_t)(N - 1 - k) * a.inner + gn);
-            f0 = make_float4(f.x + mm.x, f.y + mm.y, f.z + mm.z, f.w + mm.w);
-            f1 = make_float4(f.x - mm.x, f.y - mm.y, f.z - mm.z, f.w - mm.w);
-          } else {
-            f0 = f;
-            f1 = *reinterpret_cast<const float4*>(pl + (size_t)(Nh + k) * a.inner + gn);
-          }
+        const float4 f = pd[kk], h = pd[4 + kk];
+        float4 f0 = f, f1 = h;
+        if (MODE == DST_YF) {
+          f0 = make_float4(f.x + h.x, f.y + h.y, f.z + h.z, f.w + h.w);
+          f1 = make_float4(f.x - h.x, f.y - h.y, f.z - h.z, f.w - h.w);
         }
         s0[kk][0] = f0.x; s0[kk][1] = f0.y; s0[kk][2] = f0.z; s0[kk][3] = f0.w;
         s1[kk][0] = f1.x; s1[kk][1] = f1.y; s1[kk][2] = f1.z; s1[kk][3] = f1.w;
@@ -358,31 +375,48 @@ __global__ void __launch_bounds__(256, 2) k_dst_bf16(DstArgs a) {
         *reinterpret_cast<float4*>(&Ds[1][4 * ng + nn][4 * kq]) = make_float4(s1[0][nn], s1[1][nn], s1[2][nn], s1[3][nn]);
       }
     }
+  };
+
+  fetch(0);
+  for (int k0 = 0; k0 < Nh; k0 += BBK) {
+    stash();
     __syncthreads();
+    if (k0 + BBK < Nh) fetch(k0 + BBK);
     // ---- MFMA: lane group g feeds k = 8 g .. 8 g + 7 of both operands ----
+    // A = rows m (X: data, Y: matrix), B = columns n (X: matrix, Y: data); the four A fragments
+    // are held for the parity, one B fragment at a time.
+    const int dbase = XMODE ? wm * 64 : wn * 64, mbase = XMODE ? wn * 64 : wm * 64;
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-      // data fragments (split here) and matrix fragments (pre-split)
-      bf16x8 dh[4], dl[4], mh[4], ml[4];
-      const int dbase = XMODE ? wm * 64 : wn * 64, mbase = XMODE ? wn * 64 : wm * 64;
+      bf16x8 ah[4], al[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const float* dp = &Ds[p][dbase + i * 16 + lr][8 * g];
-        split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), dh[i], dl[i]);
-        mh[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + i * 16 + lr][8 * g]);
-        ml[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + i * 16 + lr][8 * g]);
-      }
-      // A = rows m (X: data, Y: matrix), B = columns n (X: matrix, Y: data)
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const bf16x8 ah = XMODE ? dh[i] : mh[i], al = XMODE ? dl[i] : ml[i];
-          const bf16x8 bh = XMODE ? mh[j] : dh[j], bl = XMODE ? ml[j] : dl[j];
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[p][i][j], 0, 0, 0);
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[p][i][j], 0, 0, 0);
-          acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[p][i][j], 0, 0, 0);
+        if (XMODE) {
+          const float* dp = &Ds[p][dbase + i * 16 + lr][8 * g];
+          split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), ah[i], al[i]);
+        } else {
+          ah[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + i * 16 + lr][8 * g]);
+          al[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + i * 16 + lr][8 * g]);
         }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        bf16x8 bh, bl;
+        if (XMODE) {
+          bh = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + j * 16 + lr][8 * g]);
+          bl = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + j * 16 + lr][8 * g]);
+        } else {
+          const float* dp = &Ds[p][dbase + j * 16 + lr][8 * g];
+          split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), bh, bl);
+        }
+        // small terms first; four independent accumulators between two updates of one
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[p][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[p][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[p][i][j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
@@ -450,55 +484,106 @@ void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
 }
 
 // ---------------- Thomas ----------------
+// FP64 sweep of one mode (the near-singular ones, or every mode in the FP64 form): c' and d' go
+// through global scratch, as there are few such modes.
+__device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, size_t base, size_t sxy) {
+  double shift = a.lam64[sx] + a.lam64[a.nx + sy];
+  double cprev = 0.0, dprev = 0.0;
+  for (int k = 0; k < a.nz; k++) {
+    const double* o = a.zop64 + 4 * k;
+    size_t idx = base + sxy * k;
+    double den = o[2] + shift * o[3] - o[0] * cprev;
+    cprev = o[1] / den;
+    dprev = ((double)a.hat[idx] - o[0] * dprev) / den;
+    a.cp64[idx] = cprev;
+    a.dp64[idx] = dprev;
+  }
+  double above = a.dp64[base + sxy * (a.nz - 1)];
+  a.hat[base + sxy * (a.nz - 1)] = (float)above;
+  for (int k = a.nz - 2; k >= 0; k--) {
+    size_t idx = base + sxy * k;
+    above = a.dp64[idx] - a.cp64[idx] * above;
+    a.hat[idx] = (float)above;
+  }
+}
+
+// FP32 sweeps with c' and d' held in registers (nz <= NZMAX): the column of the mode is read once
+// (all loads issued before the recurrence needs them) and written once.
+template <int NZMAX>
 __global__ void __launch_bounds__(256) k_thomas(ThomasArgs a) {
   int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y * blockDim.y + threadIdx.y;
   if (sx >= a.nx || sy >= a.ny) return;
   const size_t sxy = (size_t)a.nx * a.ny, base = (size_t)sx + (size_t)a.nx * sy;
   bool use64 = a.mode == 1;
   if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
-  if (!use64) {
-    float shift = a.lam32[sx] + a.lam32[a.nx + sy];
-    float cprev = 0.0f, dprev = 0.0f;
-    for (int k = 0; k < a.nz; k++) {
+  if (use64) {
+    thomas64(a, sx, sy, base, sxy);
+    return;
+  }
+  const int nz = a.nz;
+  float d[NZMAX], c[NZMAX];
+#pragma unroll
+  for (int k = 0; k < NZMAX; k++)
+    if (k < nz) d[k] = a.hat[base + sxy * k];
+  const float shift = a.lam32[sx] + a.lam32[a.nx + sy];
+  float cprev = 0.0f, dprev = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NZMAX; k++) {
+    if (k < nz) {
       float4 o = a.zop32[k];
-      size_t idx = base + sxy * k;
       float den = o.z + shift * o.w - o.x * cprev;
       cprev = o.y / den;
-      dprev = (a.hat[idx] - o.x * dprev) / den;
-      a.cp32[idx] = cprev;
-      a.hat[idx] = dprev;
+      dprev = (d[k] - o.x * dprev) / den;
+      c[k] = cprev;
+      d[k] = dprev;
     }
-    float above = a.hat[base + sxy * (a.nz - 1)];
-    for (int k = a.nz - 2; k >= 0; k--) {
-      size_t idx = base + sxy * k;
-      above = a.hat[idx] - a.cp32[idx] * above;
-      a.hat[idx] = above;
+  }
+  float above = 0.0f;
+#pragma unroll
+  for (int k = NZMAX - 1; k >= 0; k--) {
+    if (k < nz) {
+      above = (k == nz - 1) ? d[k] : d[k] - c[k] * above;
+      a.hat[base + sxy * k] = above;
     }
-  } else {
-    double shift = a.lam64[sx] + a.lam64[a.nx + sy];
-    double cprev = 0.0, dprev = 0.0;
-    for (int k = 0; k < a.nz; k++) {
-      const double* o = a.zop64 + 4 * k;
-      size_t idx = base + sxy * k;
-      double den = o[2] + shift * o[3] - o[0] * cprev;
-      cprev = o[1] / den;
-      dprev = ((double)a.hat[idx] - o[0] * dprev) / den;
-      a.cp64[idx] = cprev;
-      a.dp64[idx] = dprev;
-    }
-    double above = a.dp64[base + sxy * (a.nz - 1)];
-    a.hat[base + sxy * (a.nz - 1)] = (float)above;
-    for (int k = a.nz - 2; k >= 0; k--) {
-      size_t idx = base + sxy * k;
-      above = a.dp64[idx] - a.cp64[idx] * above;
-      a.hat[idx] = (float)above;
-    }
+  }
+}
+
+// Any nz: c' goes through global scratch.
+__global__ void __launch_bounds__(256) k_thomas_any(ThomasArgs a) {
+  int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (sx >= a.nx || sy >= a.ny) return;
+  const size_t sxy = (size_t)a.nx * a.ny, base = (size_t)sx + (size_t)a.nx * sy;
+  bool use64 = a.mode == 1;
+  if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
+  if (use64) {
+    thomas64(a, sx, sy, base, sxy);
+    return;
+  }
+  float shift = a.lam32[sx] + a.lam32[a.nx + sy];
+  float cprev = 0.0f, dprev = 0.0f;
+  for (int k = 0; k < a.nz; k++) {
+    float4 o = a.zop32[k];
+    size_t idx = base + sxy * k;
+    float den = o.z + shift * o.w - o.x * cprev;
+    cprev = o.y / den;
+    dprev = (a.hat[idx] - o.x * dprev) / den;
+    a.cp32[idx] = cprev;
+    a.hat[idx] = dprev;
+  }
+  float above = a.hat[base + sxy * (a.nz - 1)];
+  for (int k = a.nz - 2; k >= 0; k--) {
+    size_t idx = base + sxy * k;
+    above = a.hat[idx] - a.cp32[idx] * above;
+    a.hat[idx] = above;
   }
 }
 
 void launch_thomas(hipStream_t s, const ThomasArgs& a) {
   dim3 blk(64, 4, 1), grid((a.nx + 63) / 64, (a.ny + 3) / 4, 1);
-  hipLaunchKernelGGL(k_thomas, grid, blk, 0, s, a);
+  if (a.nz <= 64) hipLaunchKernelGGL((k_thomas<64>), grid, blk, 0, s, a);
+  else if (a.nz <= 80) hipLaunchKernelGGL((k_thomas<80>), grid, blk, 0, s, a);
+  else if (a.nz <= 104) hipLaunchKernelGGL((k_thomas<104>), grid, blk, 0, s, a);
+  else hipLaunchKernelGGL(k_thomas_any, grid, blk, 0, s, a);
 }
 
 }  // namespace obl
