@@ -246,8 +246,8 @@ __device__ __forceinline__ float nut_wc(const A& V, int x, int y, int z) {
   return 0.5f * (V.ldc(x, y, z, 2) + V.ldc(x, y, z + 1, 2));
 }
 template <class A>
-__device__ __forceinline__ float2 nut_point(const MU& u, const Dims& d, const float4* BS, const A& V,
-                                            const float4* Q1, int x, int y, int z) {
+__device__ __forceinline__ float2 nut_point(const MU& u, const Dims& d, const float4* BS, const A& V, float4 q1,
+                                            int x, int y, int z) {
   float dx = d.dx, dz = d.dz;
   int nz = d.nz;
   float4 v0 = V.ld4(x, y, z);
@@ -264,7 +264,6 @@ __device__ __forceinline__ float2 nut_point(const MU& u, const Dims& d, const fl
   float S2 = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + 4.0f * (s12 * s12 + s13 * s13 + s23 * s23);
   int kz0 = max(z - 1, 0), kz1 = min(z + 1, nz - 1);
   float N2 = 9.81f / bsA(BS, z, nz).x * (V.ldc(x, y, kz1, 3) - V.ldc(x, y, kz0, 3)) / ((float)(kz1 - kz0) * dz);
-  float4 q1 = Q1[cidx(d, x, y, z)];
   if (u.v[U_COR].w > 0.5f && q1.y + q1.z > 1e-5f) N2 = fminf(N2, 0.0f);
   float Ri = N2 / fmaxf(S2, 1e-10f);
   float fst = sqrtf(fmaxf(0.0f, 1.0f - Ri / 0.33f));
@@ -297,8 +296,8 @@ __device__ __forceinline__ Loc localVel(const A& V, int x, int y, int z) {
 }
 // first-order departure (s = 1) / forward (s = -1) point, clamped to [0, N]
 __device__ __forceinline__ float3 traceV(const Dims& d, float3 k, float3 p, float3 v0, float s) {
-  return make_float3(clampf_(p.x - s * k.x * v0.x, 0.0f, (float)d.nx), clampf_(p.y - s * k.y * v0.y, 0.0f, (float)d.ny),
-                     clampf_(p.z - s * k.z * v0.z, 0.0f, (float)d.nz));
+  return make_float3(clampt_(p.x - s * k.x * v0.x, 0.0f, (float)d.nx), clampt_(p.y - s * k.y * v0.y, 0.0f, (float)d.ny),
+                     clampt_(p.z - s * k.z * v0.z, 0.0f, (float)d.nz));
 }
 // texU / texVv / texW / texC (model.js:411-414) in texel coordinates
 template <class A>
@@ -344,7 +343,7 @@ __device__ __forceinline__ float4 advv1_point(const Dims& d, const Ring& rv, int
 
 template <class A>
 __device__ __forceinline__ float2 mmV(const A& T, float3 p, float ox, float oy, float oz, int ch) {
-  return minmax1(T, (int)floorf(p.x - ox), (int)floorf(p.y - oy), (int)floorf(p.z - oz), ch);
+  return minmax1(T, floori_(p.x - ox), floori_(p.y - oy), floori_(p.z - oz), ch);
 }
 template <class A, class B>
 __device__ __forceinline__ float4 advv2_rest(const MU& u, const Dims& d, const A& V, const B& Vt, const Loc& L, float4 hat,
@@ -386,10 +385,21 @@ __device__ __forceinline__ float3 cellVel(const A& V, int x, int y, int z) {
   return make_float3(0.5f * (p.x + V.ldc(x + 1, y, z, 0)), 0.5f * (p.y + V.ldc(x, y + 1, z, 1)),
                      0.5f * (p.z + V.ldc(x, y, z + 1, 2)));
 }
+// the same in two parts: the face texels (issued as loads), then the averages
+struct CellVelTexels {
+  float4 p;
+  float xp, yp, zp;
+};
+__device__ __forceinline__ CellVelTexels cellVelTexels(const GAcc& V, int x, int y, int z) {
+  return CellVelTexels{V.ld4(x, y, z), V.ldc(x + 1, y, z, 0), V.ldc(x, y + 1, z, 1), V.ldc(x, y, z + 1, 2)};
+}
+__device__ __forceinline__ float3 cellVelOf(const CellVelTexels& t) {
+  return make_float3(0.5f * (t.p.x + t.xp), 0.5f * (t.p.y + t.yp), 0.5f * (t.p.z + t.zp));
+}
 __device__ __forceinline__ float3 traceQ(const Dims& d, float3 k, float3 p, float3 v0, float fall, float s) {
-  return make_float3(clampf_(p.x - s * k.x * v0.x, 0.5f, (float)d.nx - 0.5f),
-                     clampf_(p.y - s * k.y * v0.y, 0.5f, (float)d.ny - 0.5f),
-                     clampf_(p.z - s * k.z * (v0.z - fall), 0.5f, (float)d.nz - 0.5f));
+  return make_float3(clampt_(p.x - s * k.x * v0.x, 0.5f, (float)d.nx - 0.5f),
+                     clampt_(p.y - s * k.y * v0.y, 0.5f, (float)d.ny - 0.5f),
+                     clampt_(p.z - s * k.z * (v0.z - fall), 0.5f, (float)d.nz - 0.5f));
 }
 // texS(x) = x / N: texel coordinate x - 1/2
 template <class A>
@@ -398,7 +408,7 @@ template <class A>
 __device__ __forceinline__ float sampQ1(const A& T, float3 p, int ch) { return samp1(T, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch); }
 template <class A>
 __device__ __forceinline__ float2 mmQ1(const A& T, float3 p, int ch) {
-  return minmax1(T, (int)floorf(p.x - 0.5f), (int)floorf(p.y - 0.5f), (int)floorf(p.z - 0.5f), ch);
+  return minmax1(T, floori_(p.x - 0.5f), floori_(p.y - 0.5f), floori_(p.z - 0.5f), ch);
 }
 
 // Fall speeds of one cell (model.js:495-500): rain mass- and number-weighted, snow, graupel.
@@ -435,11 +445,10 @@ __device__ __forceinline__ void advq1_samples(const Dims& d, bool thm, const AQ&
   o1 = make_float4(a0, a1, a2, r);
   o2 = make_float4(s, g, nr, bw);
 }
-__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, const GAcc& V, const Ring& Q1,
+__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, float3 v, const Ring& Q1,
                                             const Ring& Q2, int x, int y, int z, float4& o1, float4& o2) {
   bool thm = u.v[U_MICRO2].z > 0.5f;
   RingW w1 = Q1.w(), w2 = Q2.w();
-  float3 v = cellVel(V, x, y, z);
   float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
   Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
   if (wave_all(near_traces_q(d, v, f))) {
@@ -485,12 +494,11 @@ __device__ __forceinline__ void advq2_rest(const MU& u, const Dims& d, const AQ&
   o1 = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
   o2 = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
 }
-__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, const GAcc& V, const Ring& Q1,
+__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, float3 v, const Ring& Q1,
                                             const Ring& Q2, const Ring& Q1t, const Ring& Q2t, int x, int y, int z,
                                             float4& o1, float4& o2) {
   bool thm = u.v[U_MICRO2].z > 0.5f;
   RingW w1 = Q1.w(), w2 = Q2.w(), w1t = Q1t.w(), w2t = Q2t.w();
-  float3 v = cellVel(V, x, y, z);
   float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
   float4 h1 = w1t.ld4(x, y, z), h2 = w2t.ld4(x, y, z);
   Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
@@ -537,10 +545,10 @@ __device__ __forceinline__ float spongeTop(const MU& u, const Dims& d, float z) 
   return s * s / u.v[U_SPONGE].w;
 }
 __device__ __forceinline__ float2 ldA(const float2* aux, const Dims& d, int x, int y, int z) {
-  return aux[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
+  return ldx(aux, cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1)));
 }
 __device__ __forceinline__ float ldP(const float* phT, const Dims& d, int x, int y, int z) {
-  return phT[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
+  return ldx(phT, cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1)));
 }
 
 // Accessors of phys_point's cell fields and 2-D map: global (clamped) reads, or reads of a block's
@@ -636,7 +644,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   float nu0 = u.v[U_TURB].y;
   bool thm = u.v[U_MICRO2].z > 0.5f;
   bool drag = u.v[U_TURB].w > 0.5f;
-  size_t vi = vidx(d, x, y, z);
+  const uint32_t vi = vidx(d, x, y, z);
   float4 V = Vin.ld4(x, y, z), o = V;
   // the column's z neighbours (clamped texels)
   const ZCol zcol{col.Vm, V, col.Vp, z};
@@ -719,7 +727,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float th = V.w + dt * (nux * lapH(Vin, x, y, z, 3, V.w) * ix2 +
                            nuy * lapZp(zcol, BS, d, x, y, z, 3, V.w, z == 0, z == nz - 1) * iz2 +
                            (relax + spongeTop(u, d, zz)) * (tth - V.w));
-    size_t ci_ = cidx(d, x, y, z);
+    const uint32_t ci_ = cidx(d, x, y, z);
     float4 q1 = Q1in.ld4(x, y, z), q2 = col.Q2c;
     float4 e1 = Q1in.ld4(x + 1, y, z), w1 = Q1in.ld4(x - 1, y, z);
     float4 n1 = Q1in.ld4(x, y + 1, z), s1 = Q1in.ld4(x, y - 1, z);
@@ -789,8 +797,8 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
       th = T / B.x;
       float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
       qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
-      Q1o[ci_] = make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f));
-      Q2o[ci_] = make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w);
+      stx(Q1o, ci_, make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f)));
+      stx(Q2o, ci_, make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w));
     }
     o.w = th;  // a handed point's theta is written by the surrogate
   }
@@ -804,7 +812,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float4 Bz = bsB(BS, z, nz);
     o = make_float4(Bz.z, Bz.w, 0.0f, bsA(BS, z, nz).x);
   }
-  Vout[vi] = o;
+  stx(Vout, vi, o);
 }
 
 
@@ -826,14 +834,19 @@ __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __re
   rv.load(-1, tid); rv.load(0, tid); rv.load(1, tid);
   __syncthreads();
   const bool act = x < d.nx && y < d.ny;
+  // Reads a level needs besides its rings are loaded one level ahead, like the ring planes: the
+  // loads a level issues are then only needed by later levels, and its compute starts without
+  // waiting for memory.
+  float4 q1n = ldQ(Q1, d, x, y, 0);
   for (int z = 0; z < d.nz; z++) {
     Ring::Pre pv;
-    const bool more = z + 1 < d.nz;
-    if (more) rv.fetch(z + 2, tid, pv);
+    const float4 q1 = q1n;
+    q1n = ldQ(Q1, d, x, y, z + 1);
+    rv.fetch(z + 2, tid, pv);
     rv.center(z);
-    if (act) aux[cidx(d, x, y, z)] = nut_point(u, d, BS, rv.w(), Q1, x, y, z);
+    if (act) stx(aux, cidx(d, x, y, z), nut_point(u, d, BS, rv.w(), q1, x, y, z));
     __syncthreads();
-    if (more) rv.store(z + 2, tid, pv);
+    rv.store(z + 2, tid, pv);
     __syncthreads();
   }
 }
@@ -850,7 +863,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advv1_t(MU u, const float4* __
     const bool more = z < d.nz;
     if (more) rv.fetch(z + 2, tid, pv);
     rv.center(z);
-    if (act) Vt[vidx(d, x, y, z)] = advv1_point(d, rv, x, y, z);
+    if (act) stx(Vt, vidx(d, x, y, z), advv1_point(d, rv, x, y, z));
     __syncthreads();
     if (more) rv.store(z + 2, tid, pv);
     __syncthreads();
@@ -871,7 +884,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advv2_t(MU u, const float4* __
     const bool more = z < d.nz;
     if (more) { rv.fetch(z + 2, tid, pv); rt.fetch(z + 2, tid, pt); }
     rv.center(z); rt.center(z);
-    if (act) Vout[vidx(d, x, y, z)] = advv2_point(u, d, rv, rt, x, y, z);
+    if (act) stx(Vout, vidx(d, x, y, z), advv2_point(u, d, rv, rt, x, y, z));
     __syncthreads();
     if (more) { rv.store(z + 2, tid, pv); rt.store(z + 2, tid, pt); }
     __syncthreads();
@@ -889,20 +902,23 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq1_t(MU u, const float4* __
   for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); }
   __syncthreads();
   const bool act = x < d.nx && y < d.ny;
+  // the cell velocity's texels are loaded one level ahead (see k_nut_t)
+  CellVelTexels vtn = cellVelTexels(gv, x, y, 0);
   for (int z = 0; z < d.nz; z++) {
     Ring::Pre p1, p2;
-    const bool more = z + 1 < d.nz;
-    if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); }
+    const CellVelTexels vt = vtn;
+    vtn = cellVelTexels(gv, x, y, z + 1);
+    r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2);
     r1.center(z); r2.center(z);
     if (act) {
       float4 o1, o2;
-      advq1_point(u, d, BS, gv, r1, r2, x, y, z, o1, o2);
-      size_t i = cidx(d, x, y, z);
-      Q1t[i] = o1;
-      Q2t[i] = o2;
+      advq1_point(u, d, BS, cellVelOf(vt), r1, r2, x, y, z, o1, o2);
+      const uint32_t i = cidx(d, x, y, z);
+      stx(Q1t, i, o1);
+      stx(Q2t, i, o2);
     }
     __syncthreads();
-    if (more) { r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); }
+    r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2);
     __syncthreads();
   }
 }
@@ -921,20 +937,23 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
   for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); r3.load(k, tid); r4.load(k, tid); }
   __syncthreads();
   const bool act = x < d.nx && y < d.ny;
+  // the cell velocity's texels are loaded one level ahead (see k_nut_t)
+  CellVelTexels vtn = cellVelTexels(gv, x, y, 0);
   for (int z = 0; z < d.nz; z++) {
     Ring::Pre p1, p2, p3, p4;
-    const bool more = z + 1 < d.nz;
-    if (more) { r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4); }
+    const CellVelTexels vt = vtn;
+    vtn = cellVelTexels(gv, x, y, z + 1);
+    r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4);
     r1.center(z); r2.center(z); r3.center(z); r4.center(z);
     if (act) {
       float4 o1, o2;
-      advq2_point(u, d, BS, gv, r1, r2, r3, r4, x, y, z, o1, o2);
-      size_t i = cidx(d, x, y, z);
-      Q1o[i] = o1;
-      Q2o[i] = o2;
+      advq2_point(u, d, BS, cellVelOf(vt), r1, r2, r3, r4, x, y, z, o1, o2);
+      const uint32_t i = cidx(d, x, y, z);
+      stx(Q1o, i, o1);
+      stx(Q2o, i, o2);
     }
     __syncthreads();
-    if (more) { r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4); }
+    r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4);
     __syncthreads();
   }
 }
@@ -961,24 +980,24 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
   for (int k = -1; k <= 1; k++) { rv.load(k, tid); rq.load(k, tid); }
   // aux / phT tile texels of this thread (clamped like ldA / ldP), and the 2-D lsfB tile
   constexpr int NQ = (TPS + TILE_THREADS - 1) / TILE_THREADS;
-  size_t cof[NQ];
+  uint32_t cof[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     int t = tid + q * TILE_THREADS;
     int i = clampi(x0 - 1 + t % TPW, 0, d.nx - 1), j = clampi(y0 - 1 + t / TPW, 0, d.ny - 1);
-    cof[q] = (size_t)i + (size_t)d.nx * (size_t)j;
+    cof[q] = (uint32_t)i + __umul24((uint32_t)d.nx, (uint32_t)j);
     if (t < TPS) {
-      float4 b = lsfB[cof[q]];
+      float4 b = ldx(lsfB, cof[q]);
       sB[t] = b.x; sB[TPS + t] = b.y; sB[2 * TPS + t] = b.z; sB[3 * TPS + t] = b.w;
-      float2 a = aux[cof[q]];
+      float2 a = ldx(aux, cof[q]);
       sA[t] = a.x; sA[TPS + t] = a.y;
-      sP[t] = phT[cof[q]];
+      sP[t] = ldx(phT, cof[q]);
     }
   }
-  const size_t plane = (size_t)d.nx * d.ny;
+  const uint32_t plane = __umul24((uint32_t)d.nx, (uint32_t)d.ny);
   PhysCol col;
   col.LA = ld2d(lsfA, d, x, y);
-  col.ci = ciMap[(size_t)clampi(x, 0, d.nx - 1) + (size_t)d.nx * clampi(y, 0, d.ny - 1)];
+  col.ci = ldx(ciMap, (uint32_t)clampi(x, 0, d.nx - 1) + __umul24((uint32_t)d.nx, (uint32_t)clampi(y, 0, d.ny - 1)));
   col.Q2m = ldQ(Q2in, d, x, y, 0);
   col.am = ldA(aux, d, x, y, 0);
   col.pm = ldP(phT, d, x, y, 0);
@@ -988,19 +1007,18 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
     Ring::Pre pv, pq;
     float2 na[NQ];
     float np[NQ];
-    const bool more = z < d.nz;
-    if (more) {
+    col.Q2c = ldQ(Q2in, d, x, y, z);
+    {
       rv.fetch(z + 2, tid, pv);
       rq.fetch(z + 2, tid, pq);
-      const size_t zo = plane * (size_t)clampi(z + 1, 0, d.nz - 1);
+      const uint32_t zo = __umul24(plane, (uint32_t)clampi(z + 1, 0, d.nz - 1));
+      // unconditional (cof is a clamped, valid offset for every lane)
 #pragma unroll
-      for (int q = 0; q < NQ; q++)
-        if (tid + q * TILE_THREADS < TPS) {
-          na[q] = aux[cof[q] + zo];
-          np[q] = phT[cof[q] + zo];
-        }
+      for (int q = 0; q < NQ; q++) {
+        na[q] = ldx(aux, cof[q] + zo);
+        np[q] = ldx(phT, cof[q] + zo);
+      }
     }
-    col.Q2c = ldQ(Q2in, d, x, y, z);
     rv.center(z);
     rq.center(z);
     const RingW wv = rv.w(), wq = rq.w();
@@ -1017,7 +1035,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
       col.pm = tp(x, y, z);
     }
     __syncthreads();
-    if (more) {
+    {
       rv.store(z + 2, tid, pv);
       rq.store(z + 2, tid, pq);
 #pragma unroll
@@ -1039,14 +1057,14 @@ __global__ void __launch_bounds__(256) k_div(MU u, const float4* __restrict__ BS
   Dims d = dims(u);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x >= d.nx || y >= d.ny || z >= d.nz) return;
-  float4 p = V[vidx(d, x, y, z)];
+  float4 p = ldx(V, vidx(d, x, y, z));
   float rc = bsA(BS, z, d.nz).z;
   float wt = z == d.nz - 1 ? 0.0f : ldVc(V, d, x, y, z + 1, 2);
   float wb = z == 0 ? 0.0f : p.z;
   float dd = rc * (ldVc(V, d, x + 1, y, z, 0) - p.x + ldVc(V, d, x, y + 1, z, 1) - p.y) / d.dx +
              (rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb) / d.dz;
   int pd = (int)u.v[U_PAD].x, nxp = (int)u.v[U_PAD].y, nyp = (int)u.v[U_PAD].z;
-  rhs[(size_t)(x + pd) + (size_t)nxp * ((size_t)(y + pd) + (size_t)nyp * z)] = dd / d.dt;
+  stx(rhs, off3(nxp, nyp, x + pd, y + pd, z), dd / d.dt);
 }
 
 // ====================== PROJ (model.js:928-946) ======================
@@ -1055,7 +1073,7 @@ __device__ __forceinline__ float Pphi(const float* phi, const MU& u, const Dims&
   x = clampi(x, -p, d.nx + p - 1);
   y = clampi(y, -p, d.ny + p - 1);
   z = clampi(z, 0, d.nz - 1);
-  return phi[(size_t)(x + p) + (size_t)nxp * ((size_t)(y + p) + (size_t)nyp * z)];
+  return ldx(phi, off3(nxp, nyp, x + p, y + p, z));
 }
 __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V, const float* __restrict__ phi,
                                               float4* __restrict__ Vout) {
@@ -1063,14 +1081,14 @@ __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > d.nx || y > d.ny || z > d.nz) return;
   float kx = d.dt / d.dx, kz = d.dt / d.dz;
-  size_t i = vidx(d, x, y, z);
-  float4 Vv = V[i], o = Vv;
+  const uint32_t i = vidx(d, x, y, z);
+  float4 Vv = ldx(V, i), o = Vv;
   float pc = Pphi(phi, u, d, x, y, z);
   if (y < d.ny && z < d.nz) o.x = Vv.x - kx * (pc - Pphi(phi, u, d, x - 1, y, z));
   if (x < d.nx && z < d.nz) o.y = Vv.y - kx * (pc - Pphi(phi, u, d, x, y - 1, z));
   if (x < d.nx && y < d.ny && z > 0 && z < d.nz) o.z = Vv.z - kz * (pc - Pphi(phi, u, d, x, y, z - 1));
   else o.z = 0.0f;
-  Vout[i] = o;
+  stx(Vout, i, o);
 }
 
 // ====================== launchers ======================

@@ -474,6 +474,11 @@ extern "C" int obl_create(const obl_grid* g, obl_model** out) {
   if (g->nx < 8 || g->ny < 8 || g->nz < 2 || g->nx % 8 || g->ny % 8)
     return set_global("allocate: nx and ny must be multiples of 8 and nz >= 2 (got %d x %d x %d)", g->nx, g->ny, g->nz);
   if (g->pad < 0 || g->pad % 4) return set_global("allocate: the solve padding must be a non-negative multiple of 4");
+  // the stencil kernels address fields with 32-bit texel offsets (24-bit factors)
+  if ((double)(g->nx + 1) * (g->ny + 1) * (g->nz + 1) >= (double)(1u << 28) || (double)(g->ny + 1) * (g->nz + 1) >= (double)(1u << 24) ||
+      (double)(g->nx + 1 + 2 * g->pad) * (g->ny + 1 + 2 * g->pad) >= (double)(1u << 24) ||
+      (double)(g->ny + 2 * g->pad) * g->nz >= (double)(1u << 24))
+    return set_global("allocate: %d x %d x %d exceeds 2^28 texels per field (4 GiB of float4)", g->nx, g->ny, g->nz);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_global("allocate: no ROCm GPU is available");
   obl_model* h = new obl_model();

@@ -41,6 +41,15 @@ __device__ __forceinline__ Dims dims(const MU& u) {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 // WGSL clamp(e, lo, hi) = min(max(e, lo), hi); mix(a, b, t) = a (1 - t) + b t
 __device__ __forceinline__ float clampf_(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
+// Clamp of a finite coordinate in one v_med3_f32 (the trace points of the advection kernels).
+__device__ __forceinline__ float clampt_(float e, float lo, float hi) { return __builtin_amdgcn_fmed3f(e, lo, hi); }
+// Integer floor in one v_cvt_flr_i32_f32 (the compiler emits v_floor + v_cvt); with
+// __builtin_amdgcn_fractf for the fraction, a texel coordinate splits in two instructions.
+__device__ __forceinline__ int floori_(float t) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(t));
+  return r;
+}
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ __forceinline__ float sstep(float a, float b, float x) {
   float t = clampf_((x - a) / (b - a), 0.0f, 1.0f);
@@ -52,23 +61,37 @@ __device__ __forceinline__ float4 bsA(const float4* __restrict__ BS, int k, int 
 __device__ __forceinline__ float4 bsB(const float4* __restrict__ BS, int k, int nz) { return BS[clampi(k, 0, nz - 1) * 2 + 1]; }
 __device__ __forceinline__ float rhoW(const float4* __restrict__ BS, int k, int nz) { return BS[clampi(k, 0, nz) * 2].w; }
 
-__device__ __forceinline__ size_t vidx(const Dims& d, int x, int y, int z) {
-  return (size_t)x + (size_t)(d.nx + 1) * ((size_t)y + (size_t)(d.ny + 1) * (size_t)z);
+// Texel offsets are 32-bit: a field holds fewer than 2^28 texels and every factor of an offset
+// is below 2^24 (checked when a model is created), so offsets are built with 24-bit multiplies and
+// an access is a uniform base plus a 32-bit byte offset instead of 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T ldx(const T* p, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
 }
-__device__ __forceinline__ size_t cidx(const Dims& d, int x, int y, int z) {
-  return (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * (size_t)z);
+template <class T>
+__device__ __forceinline__ void stx(T* p, uint32_t i, const T& v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (uint32_t)sizeof(T)) = v;
 }
+// channel ch of float4 texel i
+__device__ __forceinline__ float ldxc(const float4* p, uint32_t i, int ch) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + i * 16u + (uint32_t)ch * 4u);
+}
+__device__ __forceinline__ uint32_t off3(int W, int H, int x, int y, int z) {
+  return (uint32_t)x + __umul24((uint32_t)W, (uint32_t)y + __umul24((uint32_t)H, (uint32_t)z));
+}
+__device__ __forceinline__ uint32_t vidx(const Dims& d, int x, int y, int z) { return off3(d.nx + 1, d.ny + 1, x, y, z); }
+__device__ __forceinline__ uint32_t cidx(const Dims& d, int x, int y, int z) { return off3(d.nx, d.ny, x, y, z); }
 __device__ __forceinline__ float4 ldV(const float4* __restrict__ V, const Dims& d, int x, int y, int z) {
-  return V[vidx(d, clampi(x, 0, d.nx), clampi(y, 0, d.ny), clampi(z, 0, d.nz))];
+  return ldx(V, vidx(d, clampi(x, 0, d.nx), clampi(y, 0, d.ny), clampi(z, 0, d.nz)));
 }
 __device__ __forceinline__ float ldVc(const float4* __restrict__ V, const Dims& d, int x, int y, int z, int ch) {
-  return reinterpret_cast<const float*>(V)[4 * vidx(d, clampi(x, 0, d.nx), clampi(y, 0, d.ny), clampi(z, 0, d.nz)) + ch];
+  return ldxc(V, vidx(d, clampi(x, 0, d.nx), clampi(y, 0, d.ny), clampi(z, 0, d.nz)), ch);
 }
 __device__ __forceinline__ float4 ldQ(const float4* __restrict__ Q, const Dims& d, int x, int y, int z) {
-  return Q[cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1))];
+  return ldx(Q, cidx(d, clampi(x, 0, d.nx - 1), clampi(y, 0, d.ny - 1), clampi(z, 0, d.nz - 1)));
 }
 __device__ __forceinline__ float4 ld2d(const float4* __restrict__ M, const Dims& d, int i, int j) {
-  return M[(size_t)clampi(i, 0, d.nx - 1) + (size_t)d.nx * (size_t)clampi(j, 0, d.ny - 1)];
+  return ldx(M, (uint32_t)clampi(i, 0, d.nx - 1) + __umul24((uint32_t)d.nx, (uint32_t)clampi(j, 0, d.ny - 1)));
 }
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }

@@ -31,7 +31,7 @@ struct RingW {
   int om, oc, op;  // float offsets of planes zc-1, zc, zc+1
   __device__ __forceinline__ int plane(int k) const { return k < zc ? om : (k == zc ? oc : op); }
   __device__ __forceinline__ int sidx(int i, int j, int k, int ch) const {
-    return plane(k) + ch * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+    return plane(k) + ch * TPS + __mul24(j - y0 + 1, TPW) + (i - x0 + 1);
   }
   __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return s[sidx(i, j, k, ch)]; }
   __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
@@ -64,38 +64,43 @@ struct Ring {
     return (unsigned)(i - x0 + 1) < (unsigned)TPW && (unsigned)(j - y0 + 1) < (unsigned)TPH &&
            (unsigned)(k - zc + 1) < 3u;
   }
-  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
-    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  __device__ __forceinline__ uint32_t gidx(int i, int j, int k) const {
+    return off3(W, H, clampi(i, 0, W - 1), clampi(j, 0, H - 1), clampi(k, 0, D - 1));
   }
   __device__ __forceinline__ int sidx(int i, int j, int k, int ch) const {
-    return (slot(k) * 4 + ch) * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+    return (slot(k) * 4 + ch) * TPS + __mul24(j - y0 + 1, TPW) + (i - x0 + 1);
   }
   __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
     if (in(i, j, k)) return s[sidx(i, j, k, ch)];
-    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
+    return ldxc(g, gidx(i, j, k), ch);
   }
   __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
     if (in(i, j, k)) {
       int b = sidx(i, j, k, 0);
       return make_float4(s[b], s[b + TPS], s[b + 2 * TPS], s[b + 3 * TPS]);
     }
-    return g[gidx(i, j, k)];
+    return ldx(g, gidx(i, j, k));
+  }
+  // in-plane texel offset of halo slot t (clamped), and the offset of plane k (clamped)
+  __device__ __forceinline__ uint32_t tofs(int t) const {
+    return (uint32_t)clampi(x0 - 1 + t % TPW, 0, W - 1) + __umul24((uint32_t)W, (uint32_t)clampi(y0 - 1 + t / TPW, 0, H - 1));
+  }
+  __device__ __forceinline__ uint32_t pofs(int k) const {
+    return __umul24((uint32_t)clampi(k, 0, D - 1), (uint32_t)(W * H));
   }
   // software-pipelined plane load: fetch plane k (clamped) into registers early, store it to
   // its slot after the compute that still reads the slot's old plane
   struct Pre {
     float4 v[(TPS + TILE_THREADS - 1) / TILE_THREADS];
   };
+  // Every lane loads (lanes past the tile repeat its last texel): no divergent branch around the
+  // loads, so the wait for a level's own reads need not wait for these.
   __device__ __forceinline__ void fetch(int k, int tid, Pre& r) const {
-    int kk = clampi(k, 0, D - 1);
+    const uint32_t po = pofs(k);
 #pragma unroll
     for (int q = 0; q < (TPS + TILE_THREADS - 1) / TILE_THREADS; q++) {
-      int t = tid + q * TILE_THREADS;
-      if (t < TPS) {
-        int ti = t % TPW, tj = t / TPW;
-        r.v[q] = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
-                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
-      }
+      int t = min(tid + q * TILE_THREADS, TPS - 1);
+      r.v[q] = ldx(g, tofs(t) + po);
     }
   }
   __device__ __forceinline__ void store(int k, int tid, const Pre& r) {
@@ -114,12 +119,10 @@ struct Ring {
   }
   // cooperative load of plane k (clamped) into its slot
   __device__ __forceinline__ void load(int k, int tid) {
-    int kk = clampi(k, 0, D - 1);
+    const uint32_t po = pofs(k);
     int sl = slot(k);
     for (int t = tid; t < TPS; t += TILE_THREADS) {
-      int ti = t % TPW, tj = t / TPW;
-      float4 v = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
-                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
+      float4 v = ldx(g, tofs(t) + po);
       float* p = s + sl * 4 * TPS + t;
       p[0] = v.x;
       p[TPS] = v.y;
@@ -140,30 +143,29 @@ struct Plane {
   int W, H, D;
   int x0, y0, zc;
   __device__ __forceinline__ bool in(int, int, int k) const { return k == zc; }
-  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
-    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  __device__ __forceinline__ uint32_t gidx(int i, int j, int k) const {
+    return off3(W, H, clampi(i, 0, W - 1), clampi(j, 0, H - 1), clampi(k, 0, D - 1));
   }
   __device__ __forceinline__ int sidx(int i, int j, int, int ch) const {
-    return ch * TPS + (j - y0 + 1) * TPW + (i - x0 + 1);
+    return ch * TPS + __mul24(j - y0 + 1, TPW) + (i - x0 + 1);
   }
   __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
     if (in(i, j, k)) return s[sidx(i, j, k, ch)];
-    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
+    return ldxc(g, gidx(i, j, k), ch);
   }
   __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
     if (in(i, j, k)) {
       int b = sidx(i, j, k, 0);
       return make_float4(s[b], s[b + TPS], s[b + 2 * TPS], s[b + 3 * TPS]);
     }
-    return g[gidx(i, j, k)];
+    return ldx(g, gidx(i, j, k));
   }
   // cooperative load of the plane (clamped texels in the halo); the caller synchronises
   __device__ __forceinline__ void load(int tid) {
     int kk = clampi(zc, 0, D - 1);
     for (int t = tid; t < TPS; t += TILE_THREADS) {
       int ti = t % TPW, tj = t / TPW;
-      float4 v = g[(size_t)clampi(x0 - 1 + ti, 0, W - 1) +
-                   (size_t)W * ((size_t)clampi(y0 - 1 + tj, 0, H - 1) + (size_t)H * (size_t)kk)];
+      float4 v = ldx(g, off3(W, H, clampi(x0 - 1 + ti, 0, W - 1), clampi(y0 - 1 + tj, 0, H - 1), kk));
       s[t] = v.x;
       s[TPS + t] = v.y;
       s[2 * TPS + t] = v.z;
@@ -180,38 +182,32 @@ struct GAcc {
   float* s;
   __device__ __forceinline__ bool in(int, int, int) const { return false; }
   __device__ __forceinline__ int sidx(int, int, int, int) const { return 0; }
-  __device__ __forceinline__ size_t gidx(int i, int j, int k) const {
-    return (size_t)clampi(i, 0, W - 1) + (size_t)W * ((size_t)clampi(j, 0, H - 1) + (size_t)H * (size_t)clampi(k, 0, D - 1));
+  __device__ __forceinline__ uint32_t gidx(int i, int j, int k) const {
+    return off3(W, H, clampi(i, 0, W - 1), clampi(j, 0, H - 1), clampi(k, 0, D - 1));
   }
-  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const {
-    return reinterpret_cast<const float*>(g)[4 * gidx(i, j, k) + ch];
-  }
-  __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return g[gidx(i, j, k)]; }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return ldxc(g, gidx(i, j, k), ch); }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return ldx(g, gidx(i, j, k)); }
 };
 
 // Out-of-window paths, kept out of line: they are rare, and inlined they would hold registers
 // across every sample of the fast path.
 __device__ __noinline__ static float samp1_global(const float4* g4, int W, int H, int D, int i0, int j0, int k0, float ax,
                                                   float ay, float az, int ch) {
-  const float* g = reinterpret_cast<const float*>(g4);
   int x0 = clampi(i0, 0, W - 1), x1 = clampi(i0 + 1, 0, W - 1);
   int y0 = clampi(j0, 0, H - 1), y1 = clampi(j0 + 1, 0, H - 1);
   int z0 = clampi(k0, 0, D - 1), z1 = clampi(k0 + 1, 0, D - 1);
-  size_t r00 = (size_t)W * ((size_t)y0 + (size_t)H * z0), r10 = (size_t)W * ((size_t)y1 + (size_t)H * z0);
-  size_t r01 = (size_t)W * ((size_t)y0 + (size_t)H * z1), r11 = (size_t)W * ((size_t)y1 + (size_t)H * z1);
-  float c00 = lerpf(g[4 * (r00 + x0) + ch], g[4 * (r00 + x1) + ch], ax);
-  float c10 = lerpf(g[4 * (r10 + x0) + ch], g[4 * (r10 + x1) + ch], ax);
-  float c01 = lerpf(g[4 * (r01 + x0) + ch], g[4 * (r01 + x1) + ch], ax);
-  float c11 = lerpf(g[4 * (r11 + x0) + ch], g[4 * (r11 + x1) + ch], ax);
+  float c00 = lerpf(ldxc(g4, off3(W, H, x0, y0, z0), ch), ldxc(g4, off3(W, H, x1, y0, z0), ch), ax);
+  float c10 = lerpf(ldxc(g4, off3(W, H, x0, y1, z0), ch), ldxc(g4, off3(W, H, x1, y1, z0), ch), ax);
+  float c01 = lerpf(ldxc(g4, off3(W, H, x0, y0, z1), ch), ldxc(g4, off3(W, H, x1, y0, z1), ch), ax);
+  float c11 = lerpf(ldxc(g4, off3(W, H, x0, y1, z1), ch), ldxc(g4, off3(W, H, x1, y1, z1), ch), ax);
   return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
 }
 __device__ __noinline__ static float2 minmax1_global(const float4* g4, int W, int H, int D, int i0, int j0, int k0,
                                                      int ch) {
-  const float* g = reinterpret_cast<const float*>(g4);
   float lo = 1e30f, hi = -1e30f;
   for (int q = 0; q < 8; q++) {
     int i = clampi(i0 + (q & 1), 0, W - 1), j = clampi(j0 + ((q >> 1) & 1), 0, H - 1), k = clampi(k0 + (q >> 2), 0, D - 1);
-    float s = g[4 * ((size_t)i + (size_t)W * ((size_t)j + (size_t)H * (size_t)k)) + ch];
+    float s = ldxc(g4, off3(W, H, i, j, k), ch);
     lo = fminf(lo, s);
     hi = fmaxf(hi, s);
   }
@@ -221,11 +217,10 @@ __device__ __noinline__ static float2 minmax1_global(const float4* g4, int W, in
 // Window-guaranteed forms: the 2 x 2 x 2 texel box of the sample lies in the window, so
 // k0 is zc - 1 or zc.
 __device__ __forceinline__ float samp1(const RingW& a, float tx, float ty, float tz, int ch) {
-  float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
-  float ax = tx - fx, ay = ty - fy, az = tz - fz;
-  int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+  float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
   bool lo = k0 < a.zc;
-  int b = (lo ? a.om : a.oc) + ch * TPS + (j0 - a.y0 + 1) * TPW + (i0 - a.x0 + 1);
+  int b = (lo ? a.om : a.oc) + ch * TPS + __mul24(j0 - a.y0 + 1, TPW) + (i0 - a.x0 + 1);
   int dz = lo ? a.oc - a.om : a.op - a.oc;
   float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + TPW], c110 = a.s[b + TPW + 1];
   float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + TPW], c111 = a.s[b + dz + TPW + 1];
@@ -235,7 +230,7 @@ __device__ __forceinline__ float samp1(const RingW& a, float tx, float ty, float
 }
 __device__ __forceinline__ float2 minmax1(const RingW& a, int i0, int j0, int k0, int ch) {
   bool lo = k0 < a.zc;
-  int b = (lo ? a.om : a.oc) + ch * TPS + (j0 - a.y0 + 1) * TPW + (i0 - a.x0 + 1);
+  int b = (lo ? a.om : a.oc) + ch * TPS + __mul24(j0 - a.y0 + 1, TPW) + (i0 - a.x0 + 1);
   int dz = lo ? a.oc - a.om : a.op - a.oc;
   float v0 = a.s[b], v1 = a.s[b + 1], v2 = a.s[b + TPW], v3 = a.s[b + TPW + 1];
   float v4 = a.s[b + dz], v5 = a.s[b + dz + 1], v6 = a.s[b + dz + TPW], v7 = a.s[b + dz + TPW + 1];
@@ -248,9 +243,8 @@ __device__ __forceinline__ float2 minmax1(const RingW& a, int i0, int j0, int k0
 // applied by the caller); x, then y, then z lerps as a + t (b - a).
 template <class A>
 __device__ __forceinline__ float samp1(const A& a, float tx, float ty, float tz, int ch) {
-  float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
-  float ax = tx - fx, ay = ty - fy, az = tz - fz;
-  int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+  float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
   if (!(a.in(i0, j0, k0) && a.in(i0 + 1, j0 + 1, k0 + 1))) return samp1_global(a.g, a.W, a.H, a.D, i0, j0, k0, ax, ay, az, ch);
   int b = a.sidx(i0, j0, k0, ch);
   int dz = a.sidx(i0, j0, k0 + 1, ch) - b;
