@@ -363,10 +363,10 @@ __device__ __forceinline__ float4 advv2_rest(const MU& u, const Dims& d, const A
   float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
                          hat.w + sm * (old.w - b.w));
   float2 m;
-  m = mmV(V, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = clampf_(r.x, m.x, m.y);
-  m = mmV(V, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = clampf_(r.y, m.x, m.y);
-  m = mmV(V, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = clampf_(r.z, m.x, m.y);
-  m = mmV(V, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = clampf_(r.w, m.x, m.y);
+  m = mmV(V, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = vclamp_(r.x, m);
+  m = mmV(V, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
+  m = mmV(V, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
+  m = mmV(V, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
   return r;
 }
 __device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const Ring& rv, const Ring& rt, int x, int y,
@@ -483,16 +483,16 @@ __device__ __forceinline__ void advq2_rest(const MU& u, const Dims& d, const AQ&
                           h2.w + sm * (q2.w - b2w));
   float3 d0 = traceQ(d, k, p, v, 0.0f, 1.0f);
   float2 m;
-  m = mmQ1(Q1, d0, 0); r1.x = clampf_(r1.x, m.x, m.y);
-  m = mmQ1(Q1, d0, 1); r1.y = clampf_(r1.y, m.x, m.y);
-  m = mmQ1(Q1, d0, 2); r1.z = clampf_(r1.z, m.x, m.y);
-  m = mmQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3); r1.w = clampf_(r1.w, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0); r2.x = clampf_(r2.x, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1); r2.y = clampf_(r2.y, m.x, m.y);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2); r2.z = clampf_(r2.z, m.x, m.y);
-  m = mmQ1(Q2, d0, 3); r2.w = clampf_(r2.w, m.x, m.y);
-  o1 = make_float4(fmaxf(r1.x, 0.0f), fmaxf(r1.y, 0.0f), fmaxf(r1.z, 0.0f), fmaxf(r1.w, 0.0f));
-  o2 = make_float4(fmaxf(r2.x, 0.0f), fmaxf(r2.y, 0.0f), fmaxf(r2.z, 0.0f), fmaxf(r2.w, 0.0f));
+  m = mmQ1(Q1, d0, 0); r1.x = vclamp_(r1.x, m);
+  m = mmQ1(Q1, d0, 1); r1.y = vclamp_(r1.y, m);
+  m = mmQ1(Q1, d0, 2); r1.z = vclamp_(r1.z, m);
+  m = mmQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3); r1.w = vclamp_(r1.w, m);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0); r2.x = vclamp_(r2.x, m);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1); r2.y = vclamp_(r2.y, m);
+  m = mmQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2); r2.z = vclamp_(r2.z, m);
+  m = mmQ1(Q2, d0, 3); r2.w = vclamp_(r2.w, m);
+  o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
+  o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
 }
 __device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, float3 v, const Ring& Q1,
                                             const Ring& Q2, const Ring& Q1t, const Ring& Q2t, int x, int y, int z,

@@ -50,6 +50,35 @@ __device__ __forceinline__ int floori_(float t) {
   asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(t));
   return r;
 }
+// min / max of texels read from LDS, as the bare VALU instructions: fminf / fmaxf on loaded values
+// first quiet possible signalling NaNs (one extra v_max_f32 per operand), which the limiter of the
+// advection kernels, eight texels per channel, does not need.
+__device__ __forceinline__ float vmin_(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float vmax_(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float vmin3_(float a, float b, float c) {
+  float r;
+  asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+__device__ __forceinline__ float vmax3_(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// clamp to a limiter's [min, max] (lo <= hi): one v_med3_f32
+__device__ __forceinline__ float vclamp_(float x, float2 m) {
+  float r;
+  asm("v_med3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(m.x), "v"(m.y));
+  return r;
+}
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ __forceinline__ float sstep(float a, float b, float x) {
   float t = clampf_((x - a) / (b - a), 0.0f, 1.0f);

@@ -234,9 +234,8 @@ __device__ __forceinline__ float2 minmax1(const RingW& a, int i0, int j0, int k0
   int dz = lo ? a.oc - a.om : a.op - a.oc;
   float v0 = a.s[b], v1 = a.s[b + 1], v2 = a.s[b + TPW], v3 = a.s[b + TPW + 1];
   float v4 = a.s[b + dz], v5 = a.s[b + dz + 1], v6 = a.s[b + dz + TPW], v7 = a.s[b + dz + TPW + 1];
-  float lo_ = fminf(fminf(fminf(v0, v1), fminf(v2, v3)), fminf(fminf(v4, v5), fminf(v6, v7)));
-  float hi_ = fmaxf(fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)), fmaxf(fmaxf(v4, v5), fmaxf(v6, v7)));
-  return make_float2(lo_, hi_);
+  return make_float2(vmin3_(vmin3_(v0, v1, v2), vmin3_(v3, v4, v5), vmin_(v6, v7)),
+                     vmax3_(vmax3_(v0, v1, v2), vmax3_(v3, v4, v5), vmax_(v6, v7)));
 }
 
 // Trilinear, clamp-to-edge sample of one channel at texel coordinates (t - 1/2 convention
