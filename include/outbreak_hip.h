@@ -201,7 +201,8 @@ typedef struct {
   double d2[8], r2[8];
   /* column condensate sum_k rho0 (qc + qi + qr + qs + qg) dz (kg m^-2), per column */
   double cc_d2, cc_r2, cc_maxdiff, cc_max;
-  /* composite reflectivity (dBZ, model.js:1042-1050) over columns above 0 dBZ in either */
+  /* composite reflectivity (dBZ, model.js:1042-1050), both floored at 0 dBZ (echo weaker than
+   * that counts as none), over columns above 0 dBZ in either */
   double dbz_d2, dbz_cols, dbz_maxdiff;
   /* surface precipitation accumulated over judged steps (kg m^-2), per column */
   double pr_d2, pr_r2, pr_sum_s, pr_sum_p;

@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(256) k_judge_cols(MU u, JudgeArgs a) {
     mx[0] = fabs(dc);
     mx[1] = (double)ccp;
     if (zs > 0.0f || zp > 0.0f) {
-      double dz_ = (double)zs - (double)zp;
+      // echo below 0 dBZ counts as none: a -30 dBZ "no echo" against a 2 dBZ trace is not a 32 dB error
+      double dz_ = (double)fmaxf(zs, 0.0f) - (double)fmaxf(zp, 0.0f);
       acc[18] += dz_ * dz_;
       acc[19] += 1.0;
       mx[2] = fabs(dz_);
