@@ -299,6 +299,18 @@ def main():
     import numpy as np
 
     finite = bool(np.isfinite(V).all() and np.isfinite(Q1).all())
+    # requestMaps of the current state (the step's display consumer): the map kernel's HIP-event time
+    m.profile(True)
+    for _ in range(3):
+        m.requestMaps()()
+    mprof = m.profile_read()
+    m.profile(False)
+    maps = None
+    if "m.map" in mprof:
+        ms_map = mprof["m.map"][0]
+        alg = 48 * cells  # V, Q1, Q2 once per cell (a clear column's brightness temperature visits every level)
+        maps = {"kernel_ms": round(ms_map, 4), "algorithmic_bytes": alg, "achieved_GBps": round(alg / (ms_map * 1e-3) / 1e9, 1),
+                "note": "2-D maps (model.js:1028-1100), one work-item per column; host readback not included"}
     comb = None
     if not args.no_combined:
         print("# combined form (surrogate + split-BF16) on the same grid ...", file=sys.stderr, flush=True)
@@ -339,6 +351,7 @@ def main():
         "cpu_baseline": cpu,
         "combined_form": comb,
         "solve_alone": solve,
+        "maps": maps,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
     }

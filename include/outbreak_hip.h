@@ -116,6 +116,19 @@ int obl_reset(obl_model* h, const float* uniforms, int seed);
  * state moves from V[parity] to V[1-parity]. The caller keeps time/steps/parity
  * as the reference does and passes parity explicitly. */
 int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags);
+/* requestMaps (model.js:1478-1494) with the map kernel (model.js:1028-1100): enqueue the 2-D maps of
+ * the current state (V[parity] -- the copy the next obl_step would read --, Q1[0], Q2[0] and phi of the
+ * last solve) and their copy into one of two pinned host slots. *slot receives 0 or 1, or -1 when both
+ * slots still await collection (the reference returns null then). The maps are m0..m3, each nx*ny
+ * float4 (x fastest), stored one after the other (16*nx*ny floats), as the reference's map buffer:
+ *   m0 = (composite dBZ, dBZ at ~1 km, lowest-level theta - theta0(0), zeta at ~1 km 1/s)
+ *   m1 = (max w below 12 km, 2-5 km updraft helicity m^2/s^2, lowest-level convergence 1e-3/s,
+ *         lowest-level wind speed)
+ *   m2 = (u, v at the lowest level, u, v at ~1 km)
+ *   m3 = (surface pressure perturbation hPa, 10.7 um brightness temperature K, ~10-m wind, zeta at ~5 km) */
+int obl_maps_request(obl_model* h, const float* uniforms, int parity, int* slot);
+/* Wait for a requested slot's maps and copy them (16*nx*ny floats) to dst; the slot is free again. */
+int obl_maps_collect(obl_model* h, int slot, float* dst);
 /* Warm bubble, theta += dth cos^2(pi r/2) (no reference counterpart: the configs
  * start storms with bubbles instead of CI towers), on both V copies. */
 int obl_add_bubble(obl_model* h, double x, double y, double z, double rh, double rz, double dth);

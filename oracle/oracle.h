@@ -49,7 +49,9 @@ extern "C" {
   void orc_div_##S(const float* U, const float* BS, const R* V, R* rhs);                            \
   void orc_proj_##S(const float* U, const float* BS, const R* V, const R* phi, R* Vout);            \
   void orc_bubble_##S(const float* U, R* V, double x, double y, double z, double rh, double rz,      \
-                      double dth);
+                      double dth);                                                                  \
+  void orc_maps_##S(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2,          \
+                    const R* phi, R* maps);
 
 ORC_DECL(float, f32)
 ORC_DECL(double, f64)

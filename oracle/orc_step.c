@@ -22,6 +22,10 @@
  *   phys forces    linear fields have zero Laplacian, closed-form buoyancy
  *   microphysics   total-water conservation, positivity, branch closed forms
  *   div/proj       div(proj(V)) ~ 0 with the solved phi
+ *   maps           zero wind gives zero vorticity, helicity, convergence; solid-body rotation
+ *                  gives zeta = 2 Omega; a uniform divergence gives its convergence; a cloud-free
+ *                  column's brightness temperature is the lowest level's; one opaque layer's is its
+ *                  own; the Lin rain reflectivity law in closed form
  *   solve          scipy DST-II, manufactured solution, sparse LU
  */
 #include <math.h>
@@ -42,6 +46,7 @@ typedef double R;
 #define COS cos
 #define ATAN2 atan2
 #define FABS fabs
+#define LOG log
 #else
 typedef float R;
 #define S_(n) n##_f32
@@ -53,6 +58,7 @@ typedef float R;
 #define COS cosf
 #define ATAN2 atan2f
 #define FABS fabsf
+#define LOG logf
 #endif
 
 typedef struct { R x, y, z, w; } R4;
@@ -1070,4 +1076,85 @@ void S_(orc_proj)(const float* U, const float* BS, const R* V, const R* phi, R* 
         else o.z = 0;
         put4(Vout, vidx(c, x, y, z), o);
       }
+}
+
+/* ====================== 2-D maps (model.js:1028-1100) ======================
+ * reflectivity helpers zRain / zGrau (model.js:94-106), outbreak mode (no maritime graupel) */
+static R zRain(const Ctx* c, R qr, R nr, R rho) {
+  if (!thompson(c)) return (R)3.63e9 * POW(maxr(rho * qr, 0), (R)1.75);
+  if (qr < (R)1e-9) return 0;
+  R l = lamR(qr, nr);
+  R nb = qr * l * l * l / (R)3141.59;
+  return (R)720e18 * rho * nb / (l * l * l * l * l * l);
+}
+static R zGrau(const Ctx* c, R qg, R rho, R T) {
+  if (!thompson(c)) return (T < T0K ? (R)1.2e10 : (R)4.33e10) * POW(maxr(rho * qg, 0), (R)1.75);
+  if (qg < (R)1e-9) return 0;
+  R l = lamG(qg, rho), l2 = l * l;
+  return (R)720e18 * n0G(qg, rho) / (l2 * l2 * l2 * l) * (T < T0K ? (R)0.0666 : (R)0.24);
+}
+typedef struct { R x, y, z; } MV3;
+static MV3 map_cvel(const Ctx* c, const R* V, int i, int j, int k) {
+  R4 p = ldV(c, V, i, j, k);
+  MV3 r = {(R)0.5 * (p.x + ldV(c, V, i + 1, j, k).x), (R)0.5 * (p.y + ldV(c, V, i, j + 1, k).y),
+           (R)0.5 * (p.z + ldV(c, V, i, j, k + 1).z)};
+  return r;
+}
+static R map_dbz(const Ctx* c, const R* V, const R* Q1, const R* Q2, int i, int j, int k) {
+  R4 q1 = get4(Q1, cidx(c, i, j, k)), q2 = get4(Q2, cidx(c, i, j, k));
+  R rho = bsA(c, k).z, T = ldV(c, V, i, j, k).w * bsB(c, k).x;
+  R Z = zRain(c, q1.w, q2.z, rho) + (R)9.80e8 * POW(maxr(rho * q2.x, 0), (R)1.75) * (R)0.224 * (R)4 + zGrau(c, q2.y, rho, T);
+  return Z > (R)1e-3 ? (R)10 * LOG(Z) / (R)2.302585 : (R)-30;
+}
+static R map_zeta(const Ctx* c, const R* V, int i, int j, int k) {
+  return (map_cvel(c, V, i + 1, j, k).y - map_cvel(c, V, i - 1, j, k).y - map_cvel(c, V, i, j + 1, k).x +
+          map_cvel(c, V, i, j - 1, k).x) / ((R)2 * c->dx);
+}
+/* maps: 4 planar maps of nx*ny x 4 (m0..m3 as the reference's map buffer); phi padded as the solve's */
+void S_(orc_maps)(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, const R* phi, R* maps) {
+  Ctx C = mkctx(U, BS), *c = &C;
+  const size_t n2 = (size_t)c->nx * c->ny;
+  const R dz = c->dz;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < c->ny; j++)
+    for (int i = 0; i < c->nx; i++) {
+      R comp = -30, wmx = 0, uh = 0;
+      int k12 = (int)((R)12000 / dz);
+      if (k12 > c->nz - 1) k12 = c->nz - 1;
+      for (int k = 0; k < k12; k++) {
+        R w = map_cvel(c, V, i, j, k).z;
+        wmx = maxr(wmx, w);
+        comp = maxr(comp, map_dbz(c, V, Q1, Q2, i, j, k));
+        R z = ((R)k + (R)0.5) * dz;
+        if (z > (R)2000 && z < (R)5000 && w > 0) uh += w * maxr(map_zeta(c, V, i, j, k), 0) * dz;
+      }
+      int k1 = clampi((int)((R)1000 / dz), 1, c->nz - 2);
+      MV3 v0 = map_cvel(c, V, i, j, 0);
+      R4 s0 = ldV(c, V, i, j, 0);
+      R conv = -((ldV(c, V, i + 1, j, 0).x - s0.x + ldV(c, V, i, j + 1, 0).y - s0.y) / c->dx) * (R)1000;
+      size_t idx = (size_t)i + (size_t)c->nx * j;
+      R spd = SQRT(v0.x * v0.x + v0.y * v0.y);
+      R4 m0 = {comp, map_dbz(c, V, Q1, Q2, i, j, k1), s0.w - bsA(c, 0).x, map_zeta(c, V, i, j, k1)};
+      R4 m1 = {wmx, uh, conv, spd};
+      MV3 v1 = map_cvel(c, V, i, j, k1);
+      R4 m2 = {v0.x, v0.y, v1.x, v1.y};
+      R tr = 1, bt = 0;
+      for (int k = c->nz - 1; k >= 0; k--) {
+        R4 q1 = get4(Q1, cidx(c, i, j, k)), q2 = get4(Q2, cidx(c, i, j, k));
+        R dtau = ((R)80 * q1.y + (R)60 * q1.z + (R)25 * q2.x + (R)4 * q2.y + (R)1 * q1.w) * bsA(c, k).z * dz;
+        R a = (R)1 - EXP(-dtau);
+        bt += tr * a * ldV(c, V, i, j, k).w * bsB(c, k).x;
+        tr *= (R)1 - a;
+        if (tr < (R)1e-3) break;
+      }
+      bt += tr * (s0.w * bsB(c, 0).x);  /* no sea surface in outbreak mode */
+      int p = (int)c->U[U_PAD], nxp = (int)c->U[U_PAD + 1];
+      R ps = bsA(c, 0).z * UF(c, U_CN + 2) * phi[(size_t)(i + p) + (size_t)nxp * (size_t)(j + p)] / (R)100;
+      int k5 = clampi((int)((R)5000 / dz), 1, c->nz - 2);
+      R4 m3 = {ps, bt, UF(c, U_HUR2 + 3) * spd, map_zeta(c, V, i, j, k5)};
+      put4(maps, idx, m0);
+      put4(maps, idx + n2, m1);
+      put4(maps, idx + 2 * n2, m2);
+      put4(maps, idx + 3 * n2, m3);
+    }
 }

@@ -43,6 +43,8 @@ _SIGS = {
     "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
     "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
+    "obl_maps_request": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "obl_maps_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "obl_add_bubble": (C.c_int, [C.c_void_p] + [C.c_double] * 6),
     "obl_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),
     "obl_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),

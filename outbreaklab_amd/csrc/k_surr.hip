@@ -32,6 +32,7 @@
 
 #include "obl_device.h"
 #include "obl_kernels.h"
+#include "obl_refl.h"
 #include "obl_micro.h"
 #include "obl_mlp_infer.h"
 
@@ -155,25 +156,7 @@ void launch_surr_apply(hipStream_t s, const MU& u, const SurrApply& a, int block
 }
 
 // ---------------- judge ----------------
-// composite-reflectivity helpers (model.js:94-106, 1042-1050), outbreak mode (no maritime graupel)
-__device__ __forceinline__ float z_rain(bool thm, float qr, float nr, float rho) {
-  if (!thm) return 3.63e9f * pw(fmaxf(rho * qr, 0.0f), 1.75f);
-  if (qr < 1e-9f) return 0.0f;
-  float l = lamR(qr, nr);
-  float nb = qr * l * l * l / 3141.59f;
-  return 720e18f * rho * nb / (l * l * l * l * l * l);
-}
-__device__ __forceinline__ float z_grau(bool thm, float qg, float rho, float T) {
-  if (!thm) return (T < OBL_T0K ? 1.2e10f : 4.33e10f) * pw(fmaxf(rho * qg, 0.0f), 1.75f);
-  if (qg < 1e-9f) return 0.0f;
-  float l = lamG(qg, rho), l2 = l * l;
-  return 720e18f * n0G(qg, rho) / (l2 * l2 * l2 * l) * (T < OBL_T0K ? 0.0666f : 0.24f);
-}
-__device__ __forceinline__ float dbz_of(bool thm, float4 q1, float4 q2, float rho, float T) {
-  float Z = z_rain(thm, q1.w, q2.z, rho) + 9.80e8f * pw(fmaxf(rho * q2.x, 0.0f), 1.75f) * 0.224f * 4.0f +
-            z_grau(thm, q2.y, rho, T);
-  return Z > 1e-3f ? 10.0f * __builtin_amdgcn_logf(Z) * 0.30103f : -30.0f;
-}
+// reflectivity helpers: obl_refl.h
 
 // One thread per column (x fastest): per-channel squared differences against the plain step,
 // column condensate, composite reflectivity and the surface precipitation rate, accumulated

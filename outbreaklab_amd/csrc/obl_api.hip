@@ -428,6 +428,11 @@ struct obl_model {
   unsigned long long* jmax = nullptr;
   double judged = 0.0;
   int sblocks = 256;  // persistent blocks of the surrogate kernel (one per CU)
+  // requestMaps: one device map buffer, two pinned host slots with their copy events
+  float4* mapd = nullptr;
+  float* maph[2] = {nullptr, nullptr};
+  hipEvent_t mapev[2] = {nullptr, nullptr};
+  bool mapbusy[2] = {false, false};
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -456,6 +461,11 @@ extern "C" void obl_destroy(obl_model* h) {
                   h->srec, h->scount, h->stally, h->jV, h->jQ1, h->jQ2, h->jprs, h->jprp, h->jsums, h->jmax};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (h->mapd) hipFree(h->mapd);
+  for (int q = 0; q < 2; q++) {
+    if (h->maph[q]) hipHostFree(h->maph[q]);
+    if (h->mapev[q]) hipEventDestroy(h->mapev[q]);
+  }
   if (h->solver) obl_poisson_destroy(h->solver);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -726,6 +736,43 @@ extern "C" int obl_surrogate_stats(obl_model* h, obl_surr_stats* out, int reset)
     h->judged = 0.0;
     if (hip_ok(hipStreamSynchronize(s), &h->err, "stats reset")) return -1;
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------- requestMaps
+extern "C" int obl_maps_request(obl_model* h, const float* uniforms, int parity, int* slot) {
+  if (!h) return set_global("null handle");
+  if (!slot) return h->fail("requestMaps: null slot");
+  *slot = -1;
+  if (check_u(h, uniforms)) return -1;
+  if (parity != 0 && parity != 1) return h->fail("parity must be 0 or 1");
+  const int q = h->mapbusy[0] ? (h->mapbusy[1] ? -1 : 1) : 0;
+  if (q < 0) return 0;  // both slots await collection (the reference returns null)
+  const size_t bytes = (size_t)h->nx * h->ny * 64;
+  if (!h->mapd && hip_ok(hipMalloc(&h->mapd, bytes), &h->err, "hipMalloc(maps)")) return -1;
+  if (!h->maph[q] && hip_ok(hipHostMalloc((void**)&h->maph[q], bytes, hipHostMallocDefault), &h->err, "hipHostMalloc(maps)"))
+    return -1;
+  if (!h->mapev[q] && hip_ok(hipEventCreateWithFlags(&h->mapev[q], hipEventDisableTiming), &h->err, "hipEventCreate(maps)"))
+    return -1;
+  MU u = to_mu(uniforms);
+  Dims d = dims_of(h, u);
+  hipStream_t s = h->stream;
+  TIMED(h->timer, s, "m.map", launch_maps(s, u, d, h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->phi, h->mapd));
+  if (hip_ok(hipMemcpyAsync(h->maph[q], h->mapd, bytes, hipMemcpyDeviceToHost, s), &h->err, "requestMaps copy") ||
+      hip_ok(hipEventRecord(h->mapev[q], s), &h->err, "requestMaps event"))
+    return -1;
+  h->mapbusy[q] = true;
+  *slot = q;
+  return 0;
+}
+
+extern "C" int obl_maps_collect(obl_model* h, int slot, float* dst) {
+  if (!h) return set_global("null handle");
+  if (slot != 0 && slot != 1 || !h->mapbusy[slot]) return h->fail("collectMaps: slot %d holds no requested maps", slot);
+  if (!dst) return h->fail("collectMaps: null destination");
+  if (hip_ok(hipEventSynchronize(h->mapev[slot]), &h->err, "collectMaps")) return -1;
+  memcpy(dst, h->maph[slot], (size_t)h->nx * h->ny * 64);
+  h->mapbusy[slot] = false;
   return 0;
 }
 

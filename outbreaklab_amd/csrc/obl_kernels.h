@@ -84,6 +84,9 @@ void launch_judge_precip(hipStream_t s, const double* prs, const double* prp, si
 
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
+// 2-D maps of the current state (model.js:1028-1100): out = 4 planar maps of nx*ny float4
+void launch_maps(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                 const float4* Q2, const float* phi, float4* out);
 
 // ---- pressure solve (k_solve.hip) ----
 struct DstArgs {

@@ -32,6 +32,8 @@ typedef struct {
   int (*set_surface)(obl_model*, const float*);
   int (*reset)(obl_model*, const float*, int);
   int (*step)(obl_model*, const float*, int, unsigned);
+  int (*maps_request)(obl_model*, const float*, int, int*);
+  int (*maps_collect)(obl_model*, int, float*);
   int (*add_bubble)(obl_model*, double, double, double, double, double, double);
   int (*upload)(obl_model*, int, int, const void*, size_t, size_t);
   int (*download)(obl_model*, int, int, void*, size_t, size_t);
@@ -101,6 +103,8 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(set_surface, "obl_set_surface");
   SYM(reset, "obl_reset");
   SYM(step, "obl_step");
+  SYM(maps_request, "obl_maps_request");
+  SYM(maps_collect, "obl_maps_collect");
   SYM(add_bubble, "obl_add_bubble");
   SYM(upload, "obl_upload");
   SYM(download, "obl_download");
@@ -271,6 +275,32 @@ static napi_value js_step(napi_env env, napi_callback_info info) {
   float* u = tarr(env, argv[1], &b);
   if (!u || b < OBL_UNIFORM_FLOATS * 4) return throw_msg(env, "step: Float32Array(76) uniform block expected");
   return ok_or_throw(env, A.step(h, u, (int)num(env, argv[2]), (unsigned)num(env, argv[3])), h);
+}
+
+/* mapsRequest(h, uniforms, parity) -> slot (0 / 1), or -1 while both slots await collection */
+static napi_value js_maps_request(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* u = tarr(env, argv[1], &b);
+  if (!u || b < OBL_UNIFORM_FLOATS * 4) return throw_msg(env, "requestMaps: Float32Array(76) uniform block expected");
+  int slot = -1;
+  if (A.maps_request(h, u, (int)num(env, argv[2]), &slot)) return throw_msg(env, A.last_error(h));
+  napi_value r;
+  CHECK_NAPI(napi_create_int32(env, slot, &r));
+  return r;
+}
+
+/* mapsCollect(h, slot, Float32Array(16 nx ny)) */
+static napi_value js_maps_collect(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* dst = tarr(env, argv[2], &b);
+  if (!dst) return throw_msg(env, "collectMaps: Float32Array expected");
+  return ok_or_throw(env, A.maps_collect(h, (int)num(env, argv[1]), dst), h);
 }
 
 static napi_value js_add_bubble(napi_env env, napi_callback_info info) {
@@ -620,6 +650,8 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("setCI", js_set_ci);
   EXPORT("reset", js_reset);
   EXPORT("step", js_step);
+  EXPORT("mapsRequest", js_maps_request);
+  EXPORT("mapsCollect", js_maps_collect);
   EXPORT("addBubble", js_add_bubble);
   EXPORT("download", js_download);
   EXPORT("upload", js_upload);

@@ -335,7 +335,22 @@ class RegionalModel {
   }
   setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'fp32'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
   sync() { return addon.sync(this.h); }
-  requestMaps() { return null; }  // the 2-D map kernel (model.js:1033-1100) is not part of this library yet
+  // requestMaps (model.js:1478-1494): the 2-D maps of the current state from the uniform block last
+  // written; returns () => Promise<{m0, m1, m2, m3: Float32Array(nx*ny*4), t}>, or null while both
+  // readback slots are pending
+  requestMaps(enc) {
+    void enc;
+    const slot = addon.mapsRequest(this.h, this.U.f, this.parity);
+    if (slot < 0) return null;
+    const t = this.time, n = this.nx * this.ny, h = this.h;
+    return () => new Promise((resolve, reject) => setImmediate(() => {
+      try {
+        const a = new Float32Array(16 * n);
+        addon.mapsCollect(h, slot, a);
+        resolve({ m0: a.subarray(0, 4 * n), m1: a.subarray(4 * n, 8 * n), m2: a.subarray(8 * n, 12 * n), m3: a.subarray(12 * n, 16 * n), t });
+      } catch (e) { reject(e); }
+    }));
+  }
   // ── microphysics surrogate (no reference counterpart) ──
   // teacher sampling of the microphysics block during plain steps (capacity 0: off)
   sampleMicro(capacity, pActive, pClear) { addon.sampleMicro(this.h, capacity, dflt(pActive, 1), dflt(pClear, 0)); }

@@ -287,6 +287,26 @@ class RegionalModel:
                "precip_total_surrogate": st.pr_sum_s, "precip_total_plain": st.pr_sum_p}
         return out
 
+    def requestMaps(self, enc=None):
+        """requestMaps (model.js:1478-1494): enqueue the 2-D maps (model.js:1028-1100) of the current state.
+        Returns a callable that waits and returns {"m0".."m3": float32 arrays of nx*ny*4, "t": model time
+        at the request}, or None while both readback slots are still pending."""
+        u = getattr(self, "_u_last", None)
+        if u is None:
+            u = self.writeUniforms(1.0)
+        slot = C.c_int(-1)
+        _lib.check(_lib.lib().obl_maps_request(self.h, _ptr(u), self.parity, C.byref(slot)), self.h)
+        if slot.value < 0:
+            return None
+        t, n, q = self.time, self.nx * self.ny, slot.value
+
+        def collect():
+            a = np.empty(16 * n, np.float32)
+            _lib.check(_lib.lib().obl_maps_collect(self.h, q, _ptr(a)), self.h)
+            return {"m0": a[:4 * n], "m1": a[4 * n:8 * n], "m2": a[8 * n:12 * n], "m3": a[12 * n:], "t": t}
+
+        return collect
+
     @property
     def Vcur(self):
         return self.V[self.parity]

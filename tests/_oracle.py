@@ -144,6 +144,12 @@ class Oracle:
         self._f("proj")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(phi)), P(Vo))
         return Vo
 
+    def maps(self, V, Q1, Q2, phi):
+        """The four 2-D maps (model.js:1028-1100), shape (4, ny, nx, 4): m0..m3."""
+        M = np.zeros((4, self.ny, self.nx, 4), self.dt)
+        self._f("maps")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.a(Q2)), P(self.a(phi)), P(M))
+        return M
+
     def densities(self):
         rc = self.BS[2:8 * self.nz:8].astype(np.float64)[: self.nz]
         rw = self.BS[3::8].astype(np.float64)[: self.nz + 1]

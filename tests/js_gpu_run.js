@@ -21,5 +21,9 @@ const O = require(path.join(__dirname, '..', 'outbreaklab_amd', 'js', 'outbreak.
   const out = path.join(process.argv[2] || '.', 'js_state.bin');
   const V = m.Vcur.download(), Q1 = m.Q1[0].download(), Q2 = m.Q2[0].download();
   fs.writeFileSync(out, Buffer.concat([Buffer.from(V.buffer), Buffer.from(Q1.buffer), Buffer.from(Q2.buffer)]));
-  console.log(JSON.stringify({ out, parity: m.parity, time: m.time, nV: V.length, nQ: Q1.length }));
+  // requestMaps of the final state (model.js:1478-1494)
+  const maps = await m.requestMaps(null)();
+  const mout = path.join(process.argv[2] || '.', 'js_maps.bin');
+  fs.writeFileSync(mout, Buffer.concat([0, 1, 2, 3].map(q => Buffer.from(maps['m' + q].slice().buffer))));
+  console.log(JSON.stringify({ out, mout, mapsT: maps.t, parity: m.parity, time: m.time, nV: V.length, nQ: Q1.length }));
 })().catch(e => { console.error(e.stack || e); process.exit(1); });

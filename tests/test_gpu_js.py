@@ -1,6 +1,6 @@
 """The Node facade on the GPU: 20 steps of the 128 x 128 x 40, 2 km Thompson workload driven through
 outbreak.js and the N-API addon equal, bit for bit, the same steps driven through the Python host
-(same library, same uniform blocks, deterministic kernels)."""
+(same library, same uniform blocks, deterministic kernels), and so do the requestMaps maps."""
 import json
 import os
 import shutil
@@ -33,6 +33,11 @@ def test_node_facade_steps_equal_python_host(tmp_path):
     for _ in range(20):
         m.step(dt)
     V, Q1, Q2 = m.state()
+    maps = m.requestMaps()()
     m.destroy()
     assert info["parity"] == m.parity and info["time"] == m.time
     assert np.array_equal(Vj, V.ravel()) and np.array_equal(Q1j, Q1.ravel()) and np.array_equal(Q2j, Q2.ravel())
+    # requestMaps through the Node facade equals the Python host's
+    mj = np.fromfile(info["mout"], np.float32)
+    assert info["mapsT"] == maps["t"]
+    assert np.array_equal(mj, np.concatenate([maps[f"m{q}"] for q in range(4)]))

@@ -27,7 +27,8 @@ def _run():
 def test_facade_loads_and_matches_golden():
     out = _run()
     for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState",
-                 "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer"]:
+                 "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer",
+                 "mapsRequest", "mapsCollect"]:
         assert name in out["exports"]
     assert out["mismatches"] == 0
     assert out["levels"] == [56, 80, 100] and out["pads"] == [8, 24, 40]
