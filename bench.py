@@ -172,6 +172,40 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     }
 
 
+def radar_volume(m, volumes=2):
+    """One VCP 212 volume (17 cuts, six moments) of a radar 60 km west / 40 km south of the domain
+    centre, sampled from the current state: the radar kernels' HIP-event time per volume."""
+    from outbreaklab_amd.radar import Geo, RadarSim
+
+    geo = Geo(35.9, -97.0, m.Lx, m.Ly, 0.0)
+    sim = RadarSim(m, geo)
+    try:
+        lat, lon = geo.fromModel(m.Lx / 2 - 60e3, m.Ly / 2 - 40e3)
+        R = sim.addRadar({"id": "KTST", "lat": lat, "lon": lon, "h": 0.0}, {"vcp": 212, "seed": 1})
+        R["phase"] = 0.0
+        dur = R["vcp"]["dur"]
+        for d in sim.encDue(None, dur - 1.0):
+            sim.collect(d)
+        m.profile(True)
+        t0 = time.perf_counter()
+        for v in range(volumes):
+            for d in sim.encDue(None, dur * (v + 1) + dur - 1.0):
+                sim.collect(d)
+        wall = (time.perf_counter() - t0) * 1e3 / volumes
+        prof = m.profile_read()
+        m.profile(False)
+        ms, cnt = prof["r.cut"]
+        vol_ms = ms * cnt / volumes
+        gates = sum(c["nrad"] * R["ngRef"] for c in R["cuts"])
+        return {"volume_kernel_ms": round(vol_ms, 3), "volume_wall_ms": round(wall, 3), "cuts": len(R["cuts"]),
+                "gates_per_volume": gates, "ref_gates": R["ngRef"], "dop_gates": R["ngDop"], "bytes": R["bytes"],
+                "gates_per_s": round(gates / (vol_ms * 1e-3), 1),
+                "note": "virtual WSR-88D (radar.js:41-413), one work-item per 4 gates; wall includes the pinned "
+                        "readback and the host"}
+    finally:
+        sim.destroy()
+
+
 def solve_alone(nx=1024, ny=1024, nz=80, dx=500.0):
     """The pressure solve alone on 1024 x 1024 x 80 (dz 250 m, no padding): a seeded k^-5/3 RHS,
     FP32 and split-BF16 transforms with mixed Thomas, device time per solve and the relative error
@@ -311,6 +345,11 @@ def main():
         alg = 48 * cells  # V, Q1, Q2 once per cell (a clear column's brightness temperature visits every level)
         maps = {"kernel_ms": round(ms_map, 4), "algorithmic_bytes": alg, "achieved_GBps": round(alg / (ms_map * 1e-3) / 1e9, 1),
                 "note": "2-D maps (model.js:1028-1100), one work-item per column; host readback not included"}
+    print("# radar volume ...", file=sys.stderr, flush=True)
+    try:
+        radar = radar_volume(m)
+    except Exception as e:  # reported, not fatal to the plain measurement
+        radar = {"error": f"{type(e).__name__}: {e}"}
     comb = None
     if not args.no_combined:
         print("# combined form (surrogate + split-BF16) on the same grid ...", file=sys.stderr, flush=True)
@@ -352,6 +391,7 @@ def main():
         "combined_form": comb,
         "solve_alone": solve,
         "maps": maps,
+        "radar": radar,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
     }

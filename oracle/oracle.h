@@ -56,6 +56,11 @@ extern "C" {
 ORC_DECL(float, f32)
 ORC_DECL(double, f64)
 
+/* Virtual radar cut (radar.js:41-268), FP32: both passes of one cut, and the beam geometry of a gate. */
+void orc_radar_cut(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2, const float* aux,
+                   const float* tor, uint32_t* out, float* kdp);
+void orc_radar_geom(const float* ru, int ri, int gi, float* hs_xy);
+
 /* Noise hash (noise.js:5-9), integer-exact. */
 uint32_t orc_pcg(uint32_t v);
 uint32_t orc_hash3i(int32_t x, int32_t y, int32_t z);

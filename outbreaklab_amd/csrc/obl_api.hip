@@ -14,6 +14,7 @@
 
 #include "../../include/outbreak_hip.h"
 #include "obl_kernels.h"
+#include "obl_radar.h"
 #include "obl_mlp.h"
 
 using namespace obl;
@@ -433,6 +434,9 @@ struct obl_model {
   float* maph[2] = {nullptr, nullptr};
   hipEvent_t mapev[2] = {nullptr, nullptr};
   bool mapbusy[2] = {false, false};
+  // virtual radars: the sub-grid tornado list shared by every radar of the model
+  float4* tor = nullptr;
+  int ntor = 0;
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -462,6 +466,7 @@ extern "C" void obl_destroy(obl_model* h) {
   for (void* b : bufs)
     if (b) hipFree(b);
   if (h->mapd) hipFree(h->mapd);
+  if (h->tor) hipFree(h->tor);
   for (int q = 0; q < 2; q++) {
     if (h->maph[q]) hipHostFree(h->maph[q]);
     if (h->mapev[q]) hipEventDestroy(h->mapev[q]);
@@ -773,6 +778,124 @@ extern "C" int obl_maps_collect(obl_model* h, int slot, float* dst) {
   if (hip_ok(hipEventSynchronize(h->mapev[slot]), &h->err, "collectMaps")) return -1;
   memcpy(dst, h->maph[slot], (size_t)h->nx * h->ny * 64);
   h->mapbusy[slot] = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------- virtual radars
+struct obl_radar {
+  obl_model* m = nullptr;
+  uint32_t* out = nullptr;
+  size_t bytes = 0;
+  float* kdp = nullptr;
+  size_t kdp_floats = 0;
+  uint8_t* host[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};
+};
+
+extern "C" int obl_radar_create(obl_model* h, size_t bytes, size_t kdp_floats, obl_radar** out) {
+  if (!h || !out) return set_global("obl_radar_create: null argument");
+  *out = nullptr;
+  if (bytes == 0 || bytes % 4 || kdp_floats == 0) return h->fail("radar: volume bytes must be a positive multiple of 4");
+  obl_radar* r = new obl_radar();
+  r->m = h;
+  r->bytes = bytes;
+  r->kdp_floats = kdp_floats;
+  if (hip_ok(hipMalloc(&r->out, bytes), &h->err, "hipMalloc(radar volume)") ||
+      hip_ok(hipMemsetAsync(r->out, 0, bytes, h->stream), &h->err, "radar clear") ||
+      hip_ok(hipMalloc(&r->kdp, kdp_floats * 4), &h->err, "hipMalloc(radar kdp)")) {
+    obl_radar_destroy(r);
+    return -1;
+  }
+  for (int q = 0; q < 2; q++)
+    if (hip_ok(hipHostMalloc((void**)&r->host[q], bytes, hipHostMallocDefault), &h->err, "hipHostMalloc(radar)") ||
+        hip_ok(hipEventCreateWithFlags(&r->ev[q], hipEventDisableTiming), &h->err, "hipEventCreate(radar)")) {
+      obl_radar_destroy(r);
+      return -1;
+    }
+  if (!h->tor) {
+    if (hip_ok(hipMalloc(&h->tor, RADAR_MAX_TOR * 2 * sizeof(float4)), &h->err, "hipMalloc(tornadoes)") ||
+        hip_ok(hipMemsetAsync(h->tor, 0, RADAR_MAX_TOR * 2 * sizeof(float4), h->stream), &h->err, "tornado clear")) {
+      obl_radar_destroy(r);
+      return -1;
+    }
+  }
+  *out = r;
+  return 0;
+}
+
+extern "C" void obl_radar_destroy(obl_radar* r) {
+  if (!r) return;
+  if (r->m && r->m->stream) hipStreamSynchronize(r->m->stream);
+  if (r->out) hipFree(r->out);
+  if (r->kdp) hipFree(r->kdp);
+  for (int q = 0; q < 2; q++) {
+    if (r->host[q]) hipHostFree(r->host[q]);
+    if (r->ev[q]) hipEventDestroy(r->ev[q]);
+  }
+  delete r;
+}
+
+extern "C" int obl_radar_set_tornadoes(obl_model* h, const float* f, int n) {
+  if (!h) return set_global("null handle");
+  if (n < 0 || n > RADAR_MAX_TOR || (n > 0 && !f)) return h->fail("setTornadoes: 0..%d tornadoes of 8 floats", RADAR_MAX_TOR);
+  if (!h->tor && hip_ok(hipMalloc(&h->tor, RADAR_MAX_TOR * 2 * sizeof(float4)), &h->err, "hipMalloc(tornadoes)")) return -1;
+  if (n > 0 && hip_ok(hipMemcpyAsync(h->tor, f, (size_t)n * 8 * 4, hipMemcpyHostToDevice, h->stream), &h->err, "setTornadoes"))
+    return -1;
+  h->ntor = n;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setTornadoes");
+}
+
+extern "C" int obl_radar_cut(obl_radar* r, const float* ru, int parity) {
+  if (!r) return set_global("null radar");
+  obl_model* h = r->m;
+  if (!ru) return h->fail("radar cut: null uniform block");
+  if (parity != 0 && parity != 1) return h->fail("parity must be 0 or 1");
+  RadarU R;
+  memcpy(&R, ru, sizeof(R));
+  // the block must describe this model and stay inside this radar's buffers
+  if ((int)R.mdl.x != h->nx || (int)R.mdl.y != h->ny || (int)R.mdl.z != h->nz)
+    return h->fail("radar cut: mdl (%g, %g, %g) is not the model's %d x %d x %d", R.mdl.x, R.mdl.y, R.mdl.z, h->nx, h->ny, h->nz);
+  const long nrad = (long)R.cut.w, ngr = (long)R.gates.x, ngd = (long)R.gates.y;
+  const unsigned mask = (unsigned)R.offB.w;
+  if (nrad < 1 || ngr < 4 || ngr % 4 || ngd < 0 || ngd % 4 || ngd > ngr || R.phys.w < 0 || (int)R.phys.w > h->ntor)
+    return h->fail("radar cut: %ld radials, %ld / %ld gates, %g tornadoes (%d set)", nrad, ngr, ngd, R.phys.w, h->ntor);
+  const size_t words = r->bytes / 4;
+  auto fits = [&](float off, long n) { return off >= 0 && (size_t)off + (size_t)n <= words; };
+  bool ok = fits(R.offA.x, nrad * ngr / 4) && (size_t)(nrad * ngd) <= r->kdp_floats;
+  if (mask & 2u) ok = ok && fits(R.offA.y, nrad * ngd / 4);
+  if (mask & 4u) ok = ok && fits(R.offA.z, nrad * ngd / 4);
+  if (mask & 8u) ok = ok && fits(R.offA.w, nrad * ngd / 4);
+  if (mask & 16u) ok = ok && fits(R.offB.x, nrad * ngd / 4);
+  if (mask & 32u) ok = ok && fits(R.offB.y, nrad * ngd / 4);
+  if (!ok) return h->fail("radar cut: the cut's moments do not fit the %zu-byte volume", r->bytes);
+  RadarArgs a{h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->aux, h->tor, r->out, r->kdp};
+  TIMED(h->timer, h->stream, "r.cut", launch_radar_cut(h->stream, R, a));
+  return hip_ok(hipGetLastError(), &h->err, "radar cut launch");
+}
+
+extern "C" int obl_radar_readback(obl_radar* r, int* slot) {
+  if (!r) return set_global("null radar");
+  obl_model* h = r->m;
+  if (!slot) return h->fail("radar readback: null slot");
+  *slot = r->busy[0] ? (r->busy[1] ? -1 : 1) : 0;
+  if (*slot < 0) return 0;
+  const int q = *slot;
+  if (hip_ok(hipMemcpyAsync(r->host[q], r->out, r->bytes, hipMemcpyDeviceToHost, h->stream), &h->err, "radar readback") ||
+      hip_ok(hipEventRecord(r->ev[q], h->stream), &h->err, "radar readback event"))
+    return -1;
+  r->busy[q] = true;
+  return 0;
+}
+
+extern "C" int obl_radar_collect(obl_radar* r, int slot, void* dst) {
+  if (!r) return set_global("null radar");
+  obl_model* h = r->m;
+  if (slot != 0 && slot != 1 || !r->busy[slot]) return h->fail("radar collect: slot %d holds no volume", slot);
+  if (!dst) return h->fail("radar collect: null destination");
+  if (hip_ok(hipEventSynchronize(r->ev[slot]), &h->err, "radar collect")) return -1;
+  memcpy(dst, r->host[slot], r->bytes);
+  r->busy[slot] = false;
   return 0;
 }
 

@@ -211,3 +211,26 @@ def step_f32(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, ph
     f(P(np.ascontiguousarray(U, np.float32)), P(np.ascontiguousarray(BS, np.float32)),
       P(np.ascontiguousarray(BD, np.float32)), P(ci), int(run_lsf), int(parity), *[P(a) for a in arrs],
       threads or min(8, os.cpu_count() or 1))
+
+
+def radar_cut(ru, BS, V, Q1, Q2, aux, tor, words, kdp_len, out=None):
+    """One radar cut (radar.js:41-268) into a uint32 volume of ``words`` words (updated in place when
+    ``out`` is given); returns (out, kdp)."""
+    out = np.zeros(words, np.uint32) if out is None else out
+    kdp = np.zeros(kdp_len, np.float32)
+    f = lib().orc_radar_cut
+    f.restype = None
+    a32 = lambda x: np.ascontiguousarray(x, np.float32)  # noqa: E731
+    t = a32(tor if tor is not None and len(tor) else np.zeros(8, np.float32))
+    f(P(a32(ru)), P(a32(BS)), P(a32(V)), P(a32(Q1)), P(a32(Q2)), P(a32(aux)), P(t), P(out), P(kdp))
+    return out, kdp
+
+
+def radar_geom(ru, ri, gi):
+    """(beam height above the antenna, ground distance, model x, model y) of gate gi on radial ri."""
+    o = np.zeros(4, np.float32)
+    f = lib().orc_radar_geom
+    f.restype = None
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    f(P(np.ascontiguousarray(ru, np.float32)), ri, gi, P(o))
+    return o
