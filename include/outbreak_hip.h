@@ -127,15 +127,18 @@ int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags);
  *   m2 = (u, v at the lowest level, u, v at ~1 km)
  *   m3 = (surface pressure perturbation hPa, 10.7 um brightness temperature K, ~10-m wind, zeta at ~5 km) */
 int obl_maps_request(obl_model* h, const float* uniforms, int parity, int* slot);
-/* Wait for a requested slot's maps and copy them (16*nx*ny floats) to dst; the slot is free again. */
-int obl_maps_collect(obl_model* h, int slot, float* dst);
+/* Wait for a requested slot's maps and copy them (16*nx*ny floats; dst_floats at least that) to dst;
+ * the slot is free again. */
+int obl_maps_collect(obl_model* h, int slot, float* dst, size_t dst_floats);
 
 /* ---------------- virtual WSR-88D radars ----------------
  * The reference's RadarSim (radar.js:271-413) keeps the VCP layout, scan schedule and uniform
  * packing on the host; these calls are its GPU side (radar.js:41-268). A radar owns a volume buffer
  * of `bytes` (8-bit Archive II codes, 4 gates per 32-bit word, each cut's moments radial-major at
  * the word offsets its uniform block names), a KDP scratch of `kdp_floats` (>= radials x Doppler
- * gates of any cut) and two pinned readback slots. Errors are reported on the model's handle. */
+ * gates of any cut) and two pinned readback slots. Errors are reported on the model's handle.
+ * The model owns its radars: obl_destroy frees any not yet destroyed, after which their handles
+ * are dead. */
 typedef struct obl_radar obl_radar;
 int obl_radar_create(obl_model* h, size_t bytes, size_t kdp_floats, obl_radar** out);
 void obl_radar_destroy(obl_radar* r);
@@ -149,8 +152,8 @@ int obl_radar_cut(obl_radar* r, const float* ru, int parity);
 /* the volume's copy into a pinned slot (the reference's copyBufferToBuffer into read[slot]):
  * *slot = 0 / 1, or -1 while both slots await collection */
 int obl_radar_readback(obl_radar* r, int* slot);
-/* wait for a slot and copy its `bytes` out; the slot is free again */
-int obl_radar_collect(obl_radar* r, int slot, void* dst);
+/* wait for a slot and copy its `bytes` out into dst (dst_bytes >= bytes); the slot is free again */
+int obl_radar_collect(obl_radar* r, int slot, void* dst, size_t dst_bytes);
 /* Warm bubble, theta += dth cos^2(pi r/2) (no reference counterpart: the configs
  * start storms with bubbles instead of CI towers), on both V copies. */
 int obl_add_bubble(obl_model* h, double x, double y, double z, double rh, double rz, double dth);

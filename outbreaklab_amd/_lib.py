@@ -44,13 +44,13 @@ _SIGS = {
     "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
     "obl_maps_request": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
-    "obl_maps_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "obl_maps_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "obl_radar_create": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.POINTER(C.c_void_p)]),
     "obl_radar_destroy": (None, [C.c_void_p]),
     "obl_radar_set_tornadoes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_radar_cut": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_radar_readback": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
-    "obl_radar_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "obl_radar_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "obl_add_bubble": (C.c_int, [C.c_void_p] + [C.c_double] * 6),
     "obl_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),
     "obl_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t]),

@@ -437,6 +437,7 @@ struct obl_model {
   // virtual radars: the sub-grid tornado list shared by every radar of the model
   float4* tor = nullptr;
   int ntor = 0;
+  std::vector<obl_radar*> radars;  // owned: freed with the model
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -465,6 +466,7 @@ extern "C" void obl_destroy(obl_model* h) {
                   h->srec, h->scount, h->stally, h->jV, h->jQ1, h->jQ2, h->jprs, h->jprp, h->jsums, h->jmax};
   for (void* b : bufs)
     if (b) hipFree(b);
+  while (!h->radars.empty()) obl_radar_destroy(h->radars.back());
   if (h->mapd) hipFree(h->mapd);
   if (h->tor) hipFree(h->tor);
   for (int q = 0; q < 2; q++) {
@@ -771,10 +773,11 @@ extern "C" int obl_maps_request(obl_model* h, const float* uniforms, int parity,
   return 0;
 }
 
-extern "C" int obl_maps_collect(obl_model* h, int slot, float* dst) {
+extern "C" int obl_maps_collect(obl_model* h, int slot, float* dst, size_t dst_floats) {
   if (!h) return set_global("null handle");
-  if (slot != 0 && slot != 1 || !h->mapbusy[slot]) return h->fail("collectMaps: slot %d holds no requested maps", slot);
-  if (!dst) return h->fail("collectMaps: null destination");
+  if ((slot != 0 && slot != 1) || !h->mapbusy[slot]) return h->fail("collectMaps: slot %d holds no requested maps", slot);
+  if (!dst || dst_floats < (size_t)h->nx * h->ny * 16)
+    return h->fail("collectMaps: the destination holds %zu of %zu floats", dst_floats, (size_t)h->nx * h->ny * 16);
   if (hip_ok(hipEventSynchronize(h->mapev[slot]), &h->err, "collectMaps")) return -1;
   memcpy(dst, h->maph[slot], (size_t)h->nx * h->ny * 64);
   h->mapbusy[slot] = false;
@@ -820,13 +823,22 @@ extern "C" int obl_radar_create(obl_model* h, size_t bytes, size_t kdp_floats, o
       return -1;
     }
   }
+  h->radars.push_back(r);
   *out = r;
   return 0;
 }
 
 extern "C" void obl_radar_destroy(obl_radar* r) {
   if (!r) return;
-  if (r->m && r->m->stream) hipStreamSynchronize(r->m->stream);
+  if (r->m) {
+    if (r->m->stream) hipStreamSynchronize(r->m->stream);
+    auto& v = r->m->radars;
+    for (size_t i = 0; i < v.size(); i++)
+      if (v[i] == r) {
+        v.erase(v.begin() + (long)i);
+        break;
+      }
+  }
   if (r->out) hipFree(r->out);
   if (r->kdp) hipFree(r->kdp);
   for (int q = 0; q < 2; q++) {
@@ -888,11 +900,11 @@ extern "C" int obl_radar_readback(obl_radar* r, int* slot) {
   return 0;
 }
 
-extern "C" int obl_radar_collect(obl_radar* r, int slot, void* dst) {
+extern "C" int obl_radar_collect(obl_radar* r, int slot, void* dst, size_t dst_bytes) {
   if (!r) return set_global("null radar");
   obl_model* h = r->m;
-  if (slot != 0 && slot != 1 || !r->busy[slot]) return h->fail("radar collect: slot %d holds no volume", slot);
-  if (!dst) return h->fail("radar collect: null destination");
+  if ((slot != 0 && slot != 1) || !r->busy[slot]) return h->fail("radar collect: slot %d holds no volume", slot);
+  if (!dst || dst_bytes < r->bytes) return h->fail("radar collect: the destination holds %zu of %zu bytes", dst_bytes, r->bytes);
   if (hip_ok(hipEventSynchronize(r->ev[slot]), &h->err, "radar collect")) return -1;
   memcpy(dst, r->host[slot], r->bytes);
   r->busy[slot] = false;

@@ -33,7 +33,13 @@ typedef struct {
   int (*reset)(obl_model*, const float*, int);
   int (*step)(obl_model*, const float*, int, unsigned);
   int (*maps_request)(obl_model*, const float*, int, int*);
-  int (*maps_collect)(obl_model*, int, float*);
+  int (*maps_collect)(obl_model*, int, float*, size_t);
+  int (*r_create)(obl_model*, size_t, size_t, obl_radar**);
+  void (*r_destroy)(obl_radar*);
+  int (*r_set_tornadoes)(obl_model*, const float*, int);
+  int (*r_cut)(obl_radar*, const float*, int);
+  int (*r_readback)(obl_radar*, int*);
+  int (*r_collect)(obl_radar*, int, void*, size_t);
   int (*add_bubble)(obl_model*, double, double, double, double, double, double);
   int (*upload)(obl_model*, int, int, const void*, size_t, size_t);
   int (*download)(obl_model*, int, int, void*, size_t, size_t);
@@ -105,6 +111,12 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(step, "obl_step");
   SYM(maps_request, "obl_maps_request");
   SYM(maps_collect, "obl_maps_collect");
+  SYM(r_create, "obl_radar_create");
+  SYM(r_destroy, "obl_radar_destroy");
+  SYM(r_set_tornadoes, "obl_radar_set_tornadoes");
+  SYM(r_cut, "obl_radar_cut");
+  SYM(r_readback, "obl_radar_readback");
+  SYM(r_collect, "obl_radar_collect");
   SYM(add_bubble, "obl_add_bubble");
   SYM(upload, "obl_upload");
   SYM(download, "obl_download");
@@ -300,7 +312,72 @@ static napi_value js_maps_collect(napi_env env, napi_callback_info info) {
   size_t b;
   float* dst = tarr(env, argv[2], &b);
   if (!dst) return throw_msg(env, "collectMaps: Float32Array expected");
-  return ok_or_throw(env, A.maps_collect(h, (int)num(env, argv[1]), dst), h);
+  return ok_or_throw(env, A.maps_collect(h, (int)num(env, argv[1]), dst, b / 4), h);
+}
+
+/* ---- virtual radars (radar.js:271-413); the model owns them, so the external has no finalizer ---- */
+/* radarCreate(h, bytes, kdpFloats) -> radar */
+static napi_value js_radar_create(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  obl_radar* r = NULL;
+  if (A.r_create(h, (size_t)num(env, argv[1]), (size_t)num(env, argv[2]), &r))
+    return throw_msg(env, h ? A.last_error(h) : A.global_error());
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, r, NULL, NULL, &out));
+  return out;
+}
+/* radarDestroy(radar) */
+static napi_value js_radar_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  get_args(env, info, 1, argv);
+  A.r_destroy((obl_radar*)ext(env, argv[0]));
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+/* radarSetTornadoes(h, Float32Array(8 n), n) */
+static napi_value js_radar_set_tornadoes(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* f = tarr(env, argv[1], &b);
+  int n = (int)num(env, argv[2]);
+  if (n > 0 && (!f || b < (size_t)n * 32)) return throw_msg(env, "setTornadoes: Float32Array(8 n) expected");
+  return ok_or_throw(env, A.r_set_tornadoes(h, f, n), h);
+}
+/* radarCut(h, radar, Float32Array(44), parity) */
+static napi_value js_radar_cut(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* u = tarr(env, argv[2], &b);
+  if (!u || b < 44 * 4) return throw_msg(env, "encCut: Float32Array(44) RADAR_FIELDS block expected");
+  return ok_or_throw(env, A.r_cut((obl_radar*)ext(env, argv[1]), u, (int)num(env, argv[3])), h);
+}
+/* radarReadback(h, radar) -> slot (0 / 1), or -1 while both slots await collection */
+static napi_value js_radar_readback(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_model* h = ext(env, argv[0]);
+  int slot = -1;
+  if (A.r_readback((obl_radar*)ext(env, argv[1]), &slot)) return throw_msg(env, A.last_error(h));
+  napi_value r;
+  CHECK_NAPI(napi_create_int32(env, slot, &r));
+  return r;
+}
+/* radarCollect(h, radar, slot, Uint8Array(bytes)) */
+static napi_value js_radar_collect(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  get_args(env, info, 4, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  void* dst = tarr(env, argv[3], &b);
+  if (!dst) return throw_msg(env, "collect: Uint8Array expected");
+  return ok_or_throw(env, A.r_collect((obl_radar*)ext(env, argv[1]), (int)num(env, argv[2]), dst, b), h);
 }
 
 static napi_value js_add_bubble(napi_env env, napi_callback_info info) {
@@ -652,6 +729,12 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("step", js_step);
   EXPORT("mapsRequest", js_maps_request);
   EXPORT("mapsCollect", js_maps_collect);
+  EXPORT("radarCreate", js_radar_create);
+  EXPORT("radarDestroy", js_radar_destroy);
+  EXPORT("radarSetTornadoes", js_radar_set_tornadoes);
+  EXPORT("radarCut", js_radar_cut);
+  EXPORT("radarReadback", js_radar_readback);
+  EXPORT("radarCollect", js_radar_collect);
   EXPORT("addBubble", js_add_bubble);
   EXPORT("download", js_download);
   EXPORT("upload", js_upload);

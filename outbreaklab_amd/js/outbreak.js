@@ -417,4 +417,197 @@ class Surrogate {
   infer(rows, stride) { return addon.mlpInfer(this.h, rows instanceof Float32Array ? rows : Float32Array.from(rows), stride || 24); }
 }
 
-module.exports = { Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };
+// ── virtual WSR-88D radars (radar.js:18-413), azimuthal-equidistant projection (geo.js:6-29) ──
+const RE_M = 6371000, KE = 4 / 3;
+const GATE0 = 2125, GATEDR = 250, MAX_REF_GATES = 1832, MAX_DOP_GATES = 1192;
+const L2_SCALE = { REF: [2, 66], VEL: [2, 129], SW: [2, 129], ZDR: [16, 128], PHI: [253 / 360, 2], RHO: [300, -60.5] };
+const CUT_MOMENTS = { surv: ['REF', 'ZDR', 'PHI', 'RHO'], dop: ['REF', 'VEL', 'SW'], both: ['REF', 'VEL', 'SW', 'ZDR', 'PHI', 'RHO'] };
+const MOMENT_ORDER = ['REF', 'VEL', 'SW', 'ZDR', 'PHI', 'RHO'];
+function splitVcp(split, rest, nyqSplit, nyqRest) {
+  const cuts = [];
+  split.forEach(a => {
+    cuts.push({ angle: a, kind: 'surv', superRes: true, nyquist: 8.9 });
+    cuts.push({ angle: a, kind: 'dop', superRes: true, nyquist: nyqSplit });
+  });
+  rest.forEach(a => cuts.push({ angle: a, kind: 'both', superRes: false, nyquist: a < 7 ? nyqRest : 32.3 }));
+  return cuts;
+}
+const REST12 = [1.8, 2.4, 3.1, 4.0, 5.1, 6.4, 8.0, 10.0, 12.5, 15.6, 19.5];
+const VCPS = {
+  212: { label: 'VCP 212', dur: 270, cuts: splitVcp([0.5, 0.9, 1.3], REST12, 28.4, 29.4) },
+  12: { label: 'VCP 12', dur: 250, cuts: splitVcp([0.5, 0.9, 1.3], REST12, 28.4, 29.4) },
+  215: { label: 'VCP 215', dur: 360, cuts: splitVcp([0.5, 0.9, 1.3], [1.8, 2.4, 3.1, 4.0, 5.1, 6.4, 8.0, 10.0, 12.0, 14.0, 16.7, 19.5], 28.4, 29.4) },
+  35: { label: 'VCP 35', dur: 420, cuts: splitVcp([0.5, 0.9, 1.3], [1.8, 2.4, 3.1, 4.0, 5.1, 6.4], 26.0, 27.0) },
+};
+const RADAR_FIELDS = ['site', 'geo', 'cut', 'gates', 'nyq', 'offA', 'offB', 'mdl', 'mdl2', 'phys', 'phys2'];
+
+// the reference's seeded generator (rng.js:3-27): xmur3 of the seed's text, then mulberry32
+class Rng {
+  constructor(seed) {
+    const str = String(seed);
+    let h = 1779033703 ^ str.length;
+    for (let i = 0; i < str.length; i++) {
+      h = Math.imul(h ^ str.charCodeAt(i), 3432918353);
+      h = (h << 13) | (h >>> 19);
+    }
+    h = Math.imul(h ^ (h >>> 16), 2246822507);
+    h = Math.imul(h ^ (h >>> 13), 3266489909);
+    this.state = (h ^ (h >>> 16)) >>> 0;
+  }
+  u() {
+    this.state = (this.state + 0x6D2B79F5) >>> 0;
+    let t = this.state;
+    t = Math.imul(t ^ (t >>> 15), t | 1);
+    t ^= t + Math.imul(t ^ (t >>> 7), t | 61);
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  }
+}
+
+class Geo {
+  constructor(lat0, lon0, Lx, Ly, groundMSL) {
+    Object.assign(this, { lat0, lon0, Lx: Lx || 0, Ly: Ly || 0, groundMSL: groundMSL || 0 });
+    this.p0 = lat0 * DEG; this.l0 = lon0 * DEG; this.sp0 = Math.sin(this.p0); this.cp0 = Math.cos(this.p0);
+  }
+  fwd(lat, lon) {
+    const p = lat * DEG, dl = lon * DEG - this.l0, sp = Math.sin(p), cp = Math.cos(p), cdl = Math.cos(dl);
+    const c = Math.acos(clamp(this.sp0 * sp + this.cp0 * cp * cdl, -1, 1));
+    const k = c < 1e-9 ? 1 : c / Math.sin(c);
+    return [RE_M * k * cp * Math.sin(dl), RE_M * k * (this.cp0 * sp - this.sp0 * cp * cdl)];
+  }
+  inv(x, y) {
+    const rho = Math.hypot(x, y);
+    if (rho < 1e-6) return [this.lat0, this.lon0];
+    const c = rho / RE_M, sc = Math.sin(c), cc = Math.cos(c);
+    const p = Math.asin(cc * this.sp0 + y * sc * this.cp0 / rho);
+    const l = this.l0 + Math.atan2(x * sc, rho * this.cp0 * cc - y * this.sp0 * sc);
+    return [p / DEG, ((l / DEG + 540) % 360) - 180];
+  }
+  toModel(lat, lon) { const xy = this.fwd(lat, lon); return [xy[0] + this.Lx / 2, xy[1] + this.Ly / 2]; }
+  fromModel(mx, my) { return this.inv(mx - this.Lx / 2, my - this.Ly / 2); }
+}
+
+class RadarSim {
+  constructor(model, geo) { this.m = model; this.geo = geo; this.radars = []; this.nTor = 0; }
+  // tornadoes: [{x, y, R, v, debrisTop, debrisDbz, debris}] model metres
+  setTornadoes(list) {
+    const act = list.slice(0, 64), f = new Float32Array(Math.max(1, act.length) * 8);
+    act.forEach((t, i) => f.set([t.x, t.y, t.R, t.v, t.debrisTop, t.debrisDbz, t.debris, 0], i * 8));
+    addon.radarSetTornadoes(this.m.h, f, act.length);
+    this.nTor = act.length;
+  }
+  // site: {id, lat, lon, h (ground m MSL)}; opts: {vcp, aliasing, minDbz, seed}
+  addRadar(site, opts) {
+    const R = RadarSim.plan(this.m, this.geo, site, opts);
+    R.h = addon.radarCreate(this.m.h, R.bytes, 720 * R.ngDop);
+    this.radars.push(R);
+    return R;
+  }
+  // addRadar's layout without the device buffers (radar.js:288-321)
+  static plan(m, g, site, opts) {
+    opts = opts || {};
+    const vcp = VCPS[opts.vcp] || VCPS[212];
+    const sxy = g.toModel(site.lat, site.lon), sx = sxy[0], sy = sxy[1];
+    let far = 0;
+    [[0, 0], [m.Lx, 0], [0, m.Ly], [m.Lx, m.Ly]].forEach(c => { far = Math.max(far, Math.hypot(c[0] - sx, c[1] - sy)); });
+    const rng = Math.min(460000, far + 10000);
+    let ngRef = Math.min(MAX_REF_GATES, Math.ceil((rng - GATE0) / GATEDR));
+    ngRef = Math.max(64, Math.ceil(ngRef / 4) * 4);
+    const ngDop = Math.min(MAX_DOP_GATES, ngRef);
+    let off = 0;
+    const cuts = vcp.cuts.map((c, ci) => {
+      const nrad = c.superRes ? 720 : 360;
+      const mom = CUT_MOMENTS[c.kind].map(name => {
+        const ng = name === 'REF' ? ngRef : ngDop, o = { name, ngates: ng, off };
+        off += nrad * ng;
+        return o;
+      });
+      return Object.assign({}, c, { index: ci, nrad, mom });
+    });
+    const wsum = cuts.reduce((s, c) => s + (c.superRes ? 2 : 1), 0);
+    let acc = 0;
+    cuts.forEach(c => { const w = c.superRes ? 2 : 1; c.t0 = acc / wsum * vcp.dur; acc += w; c.dur = w / wsum * vcp.dur; });
+    const rand = opts.seed !== undefined ? (r => () => r.u())(new Rng(opts.seed)) : Math.random;
+    return {
+      site, opts, vcp, vcpId: +opts.vcp || 212, cuts, bytes: off, ngRef, ngDop, sx, sy, rand, U: new UBlock(RADAR_FIELDS),
+      h: null, readBusy: [false, false], phase: rand() * vcp.dur, vol: -1, next: 0, volStart: 0, az0: 0,
+    };
+  }
+  destroy() {
+    if (this.m.h) this.radars.forEach(R => addon.radarDestroy(R.h));  // else the model freed them
+    this.radars = [];
+  }
+  // encode every cut whose scan time has been reached; returns the volumes that completed
+  encDue(enc, simT) {
+    const done = [];
+    for (const R of this.radars) {
+      for (;;) {
+        if (R.vol < 0 || R.next >= R.cuts.length) {
+          const vstart = R.vol < 0 ? R.phase : R.volStart + R.vcp.dur;
+          if (simT < vstart || (R.readBusy[0] && R.readBusy[1])) break;
+          R.vol++; R.volStart = vstart; R.next = 0; R.az0 = R.rand();
+        }
+        const c = R.cuts[R.next];
+        if (simT < R.volStart + c.t0) break;
+        this.encCut(enc, R, c, simT);
+        R.next++;
+        if (R.next >= R.cuts.length) {
+          const slot = addon.radarReadback(this.m.h, R.h);
+          if (slot < 0) throw new Error('radar ' + R.site.id + ': both readback slots are pending');
+          R.readBusy[slot] = true;
+          done.push({ R, slot, vol: R.vol, volStart: R.volStart, az0: R.az0 });
+        }
+      }
+    }
+    return done;
+  }
+  // the cut's RADAR_FIELDS block (radar.js:366-380)
+  cutBlock(R, c, simT) { return RadarSim.block(this.m, this.geo, R, c, simT, this.nTor); }
+  static block(m, g, R, c, simT, nTor) {
+    const U = R.U, off = {};
+    c.mom.forEach(o => { off[o.name] = o.off / 4; });
+    let mask = 0;
+    MOMENT_ORDER.forEach((n, i) => { if (off[n] !== undefined) mask |= 1 << i; });
+    const o = n => dflt(off[n], 0);
+    U.set('site', R.sx, R.sy, R.site.h + 20, KE * RE_M);
+    U.set('geo', g.lat0 * DEG, g.lon0 * DEG, R.site.lat * DEG, R.site.lon * DEG);
+    U.set('cut', c.angle * DEG, R.az0, 360 / c.nrad, c.nrad);
+    U.set('gates', R.ngRef, R.ngDop, GATE0, GATEDR);
+    U.set('nyq', R.opts.aliasing === false ? 63.5 : c.nyquist, simT, (R.vol * 97 + c.index * 13) % 100000, 25);
+    U.set('offA', o('REF'), o('VEL'), o('SW'), o('ZDR'));
+    U.set('offB', o('PHI'), o('RHO'), 0, mask);
+    U.set('mdl', m.nx, m.ny, m.nz, m.dx);
+    U.set('mdl2', m.dz, g.groundMSL, -m.Lx / 2, -m.Ly / 2);
+    U.set('phys', m.rhoSfc || 1.15, dflt(R.opts.minDbz, 0), m.params.scheme === 'thompson' ? 1 : 0, nTor || 0);
+    U.set('phys2', m.maritime ? 1 : 0, 0, 0, 0);
+    return U.f;
+  }
+  encCut(enc, R, c, simT) {
+    void enc;
+    addon.radarCut(this.m.h, R.h, this.cutBlock(R, c, simT), this.m.parity);
+    c.simT = simT;
+  }
+  // a finished volume's bytes and the header the server reads (radar.js:390-412)
+  collect(d, run, epoch0) {
+    const R = d.R;
+    return new Promise((resolve, reject) => setImmediate(() => {
+      try {
+        const bytes = new Uint8Array(R.bytes);
+        addon.radarCollect(this.m.h, R.h, d.slot, bytes);  // a failed collect leaves the slot pending
+        R.readBusy[d.slot] = false;
+        const t0 = dflt(epoch0, 0) + d.volStart * 1000;
+        const header = {
+          run, icao: R.site.id, lat: R.site.lat, lon: R.site.lon, elev: R.site.h + 20, vcp: R.vcpId, epoch_ms: Math.round(t0),
+          sweeps: R.cuts.map(c => ({
+            angle: c.angle, superRes: c.superRes, kind: c.kind, az0: d.az0, nrad: c.nrad,
+            nyquist: R.opts.aliasing === false ? 63.5 : c.nyquist, unamb_km: c.kind === 'surv' ? 466 : 300,
+            start_ms: Math.round(t0 + c.t0 * 1000), dur_ms: Math.round(c.dur * 1000),
+            moments: c.mom.map(o => ({ name: o.name, ngates: o.ngates, off: o.off })),
+          })),
+        };
+        resolve({ R, header, bytes });
+      } catch (e) { reject(e); }
+    }));
+  }
+}
+
+module.exports = { RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

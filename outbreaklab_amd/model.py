@@ -302,7 +302,7 @@ class RegionalModel:
 
         def collect():
             a = np.empty(16 * n, np.float32)
-            _lib.check(_lib.lib().obl_maps_collect(self.h, q, _ptr(a)), self.h)
+            _lib.check(_lib.lib().obl_maps_collect(self.h, q, _ptr(a), a.size), self.h)
             return {"m0": a[:4 * n], "m1": a[4 * n:8 * n], "m2": a[8 * n:12 * n], "m3": a[12 * n:], "t": t}
 
         return collect

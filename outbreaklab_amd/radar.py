@@ -5,8 +5,9 @@
 The VCP tables (radar.js:24-31), the volume layout (cuts x moments, radial-major, 8-bit Archive II
 codes; radar.js:289-312), the scan schedule (radar.js:314-318, 339-364) and the RADAR_FIELDS block
 (radar.js:33-35, 366-380) follow the reference; the GPU passes are `k_radar_cut` / `k_radar_phi`.
-The reference draws each site's volume phase and each volume's first azimuth from Math.random; here
-they come from a seeded generator (`opts["seed"]`) so runs reproduce.
+The reference draws each site's volume phase and each volume's first azimuth from Math.random; with
+`opts["seed"]` they come from the reference's own seeded generator (rng.js:3-27), so runs reproduce and
+the Node host draws the same numbers.
 """
 from __future__ import annotations
 
@@ -27,6 +28,40 @@ CUT_MOMENTS = {"surv": ["REF", "ZDR", "PHI", "RHO"], "dop": ["REF", "VEL", "SW"]
 MOMENT_BITS = {n: 1 << i for i, n in enumerate(["REF", "VEL", "SW", "ZDR", "PHI", "RHO"])}
 RADAR_FIELDS = ["site", "geo", "cut", "gates", "nyq", "offA", "offB", "mdl", "mdl2", "phys", "phys2"]
 DEG = math.pi / 180
+
+
+_M32 = 0xFFFFFFFF
+
+
+def _imul(a, b):
+    return (a * b) & _M32
+
+
+class Rng:
+    """The reference's seeded generator (rng.js:3-27): the xmur3 hash of the seed's decimal text gives
+    the state of a mulberry32 stream; ``u()`` is uniform on [0, 1) in steps of 2^-32."""
+
+    def __init__(self, seed):
+        text = str(seed)
+        h = (1779033703 ^ len(text)) & _M32
+        for ch in text:
+            h = _imul(h ^ ord(ch), 3432918353)
+            h = ((h << 13) | (h >> 19)) & _M32
+        h = _imul(h ^ (h >> 16), 2246822507)
+        h = _imul(h ^ (h >> 13), 3266489909)
+        self.state = (h ^ (h >> 16)) & _M32
+
+    def u(self):
+        self.state = (self.state + 0x6D2B79F5) & _M32
+        t = self.state
+        t = _imul(t ^ (t >> 15), t | 1)
+        t ^= (t + _imul(t ^ (t >> 7), t | 61)) & _M32
+        return ((t ^ (t >> 14)) & _M32) / 4294967296.0
+
+
+def _jsround(x):
+    """Math.round: halves go up."""
+    return int(math.floor(x + 0.5))
 
 
 def split_vcp(split, rest, nyq_split, nyq_rest):
@@ -109,10 +144,10 @@ def plan_radar(m, geo, site, opts=None):
         c["t0"] = acc / wsum * vcp["dur"]
         acc += w
         c["dur"] = w / wsum * vcp["dur"]
-    rand = random.Random(opts.get("seed", 0))
+    rand = Rng(opts["seed"]).u if "seed" in opts else random.random
     return {"site": site, "opts": opts, "vcp": vcp, "vcpId": int(opts.get("vcp", 212)), "cuts": cuts, "bytes": off,
             "ngRef": ng_ref, "ngDop": ng_dop, "sx": sx, "sy": sy, "h": None, "U": UBlock(RADAR_FIELDS), "rand": rand,
-            "phase": rand.random() * vcp["dur"], "vol": -1, "next": 0, "volStart": 0.0, "az0": 0.0, "busy": [False, False]}
+            "phase": rand() * vcp["dur"], "vol": -1, "next": 0, "volStart": 0.0, "az0": 0.0, "busy": [False, False]}
 
 
 def cut_block(m, geo, R, c, simT, n_tor=0):
@@ -161,8 +196,10 @@ class RadarSim:
         return R
 
     def destroy(self):
-        for R in self.radars:
-            _lib.lib().obl_radar_destroy(R["h"])
+        # the model frees its radars itself when it is destroyed first
+        if self.m.h is not None:
+            for R in self.radars:
+                _lib.lib().obl_radar_destroy(R["h"])
         self.radars = []
 
     def encDue(self, enc, simT):
@@ -175,7 +212,7 @@ class RadarSim:
                     if simT < vstart or (R["busy"][0] and R["busy"][1]):
                         break
                     R["vol"] += 1
-                    R["volStart"], R["next"], R["az0"] = vstart, 0, R["rand"].random()
+                    R["volStart"], R["next"], R["az0"] = vstart, 0, R["rand"]()
                 c = R["cuts"][R["next"]]
                 if simT < R["volStart"] + c["t0"]:
                     break
@@ -184,6 +221,8 @@ class RadarSim:
                 if R["next"] >= len(R["cuts"]):
                     slot = C.c_int(-1)
                     _lib.check(_lib.lib().obl_radar_readback(R["h"], C.byref(slot)), self.m.h)
+                    if slot.value < 0:
+                        raise _lib.OblError(f"radar {R['site']['id']}: both readback slots are pending")
                     R["busy"][slot.value] = True
                     done.append({"R": R, "slot": slot.value, "vol": R["vol"], "volStart": R["volStart"], "az0": R["az0"]})
         return done
@@ -197,15 +236,15 @@ class RadarSim:
         """collect (radar.js:390-412): the finished volume's bytes and the header the server reads."""
         R = d["R"]
         buf = np.empty(R["bytes"], np.uint8)
-        _lib.check(_lib.lib().obl_radar_collect(R["h"], d["slot"], buf.ctypes.data_as(C.c_void_p)), self.m.h)
+        _lib.check(_lib.lib().obl_radar_collect(R["h"], d["slot"], buf.ctypes.data_as(C.c_void_p), buf.nbytes), self.m.h)
         R["busy"][d["slot"]] = False
         t0 = epoch0 + d["volStart"] * 1000
         nyq = lambda c: 63.5 if R["opts"].get("aliasing") is False else c["nyquist"]  # noqa: E731
         header = {"run": run, "icao": R["site"]["id"], "lat": R["site"]["lat"], "lon": R["site"]["lon"],
-                  "elev": R["site"]["h"] + 20, "vcp": R["vcpId"], "epoch_ms": round(t0),
+                  "elev": R["site"]["h"] + 20, "vcp": R["vcpId"], "epoch_ms": _jsround(t0),
                   "sweeps": [{"angle": c["angle"], "superRes": c["superRes"], "kind": c["kind"], "az0": d["az0"],
                               "nrad": c["nrad"], "nyquist": nyq(c), "unamb_km": 466 if c["kind"] == "surv" else 300,
-                              "start_ms": round(t0 + c["t0"] * 1000), "dur_ms": round(c["dur"] * 1000),
+                              "start_ms": _jsround(t0 + c["t0"] * 1000), "dur_ms": _jsround(c["dur"] * 1000),
                               "moments": [{"name": o["name"], "ngates": o["ngates"], "off": o["off"]} for o in c["mom"]]}
                              for c in R["cuts"]]}
         return {"R": R, "header": header, "bytes": buf}

@@ -28,5 +28,13 @@ try { new O.PoissonDirect(30, 30, 10, 1000, 500); } catch (e) { perr = e.message
 const rows = new Float32Array(48);
 rows[9] = 0; rows[21] = 3e4; rows[24 + 9] = 5e3; rows[24 + 21] = -4e3;
 const targets = Array.from(O.Surrogate.targets(rows));
-console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr, targets,
+// the radar host: seeded draws, a radar's layout and a cut's RADAR_FIELDS block on the same model
+const rng = [7, 'KTLX', 123456789].map(sd => { const r = new O.Rng(sd); return [r.u(), r.u(), r.u()]; });
+const geo = new O.Geo(35.9, -97.0, m.Lx, m.Ly, 300);
+const ll = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3);
+const R = O.RadarSim.plan(m, geo, { id: 'KTST', lat: ll[0], lon: ll[1], h: 350 }, { vcp: 212, seed: 7 });
+R.vol = 3; R.az0 = R.rand();
+const radar = { ll, xy: geo.fwd(36.4, -96.1), phase: R.phase, az0: R.az0, bytes: R.bytes, ngRef: R.ngRef, ngDop: R.ngDop,
+  t0: R.cuts.map(c => c.t0), block: Array.from(O.RadarSim.block(m, geo, R, R.cuts[5], 123.5, 2)) };
+console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr, targets, rng, radar,
   levels: [O.levelsFor(2000), O.levelsFor(500), O.levelsFor(250)], pads: [O.gridPad(2000), O.gridPad(500), O.gridPad(250)] }));

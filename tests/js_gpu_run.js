@@ -25,5 +25,17 @@ const O = require(path.join(__dirname, '..', 'outbreaklab_amd', 'js', 'outbreak.
   const maps = await m.requestMaps(null)();
   const mout = path.join(process.argv[2] || '.', 'js_maps.bin');
   fs.writeFileSync(mout, Buffer.concat([0, 1, 2, 3].map(q => Buffer.from(maps['m' + q].slice().buffer))));
-  console.log(JSON.stringify({ out, mout, mapsT: maps.t, parity: m.parity, time: m.time, nV: V.length, nQ: Q1.length }));
+  // one seeded VCP 35 volume of a radar 40 km west / 30 km south of the centre, one sub-grid tornado
+  const geo = new O.Geo(35.9, -97.0, m.Lx, m.Ly, 0);
+  const sim = new O.RadarSim(m, geo);
+  sim.setTornadoes([{ x: 0.5 * m.Lx, y: 0.5 * m.Ly, R: 150, v: 60, debrisTop: 1200, debrisDbz: 55, debris: 1 }]);
+  const ll = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3);
+  const R = sim.addRadar({ id: 'KTST', lat: ll[0], lon: ll[1], h: 0 }, { vcp: 35, seed: 11 });
+  const done = sim.encDue(null, R.phase + R.vcp.dur - 1);
+  const vol = await sim.collect(done[0], 'js', 0);
+  const rout = path.join(process.argv[2] || '.', 'js_radar.bin');
+  fs.writeFileSync(rout, Buffer.from(vol.bytes.buffer));
+  sim.destroy();
+  console.log(JSON.stringify({ out, mout, rout, header: vol.header, volumes: done.length, mapsT: maps.t, parity: m.parity,
+    time: m.time, nV: V.length, nQ: Q1.length }));
 })().catch(e => { console.error(e.stack || e); process.exit(1); });

@@ -1,6 +1,7 @@
 """The Node facade on the GPU: 20 steps of the 128 x 128 x 40, 2 km Thompson workload driven through
 outbreak.js and the N-API addon equal, bit for bit, the same steps driven through the Python host
-(same library, same uniform blocks, deterministic kernels), and so do the requestMaps maps."""
+(same library, same uniform blocks, deterministic kernels), and so do the requestMaps maps and a seeded
+radar volume with its header."""
 import json
 import os
 import shutil
@@ -34,6 +35,17 @@ def test_node_facade_steps_equal_python_host(tmp_path):
         m.step(dt)
     V, Q1, Q2 = m.state()
     maps = m.requestMaps()()
+    from outbreaklab_amd.radar import Geo, RadarSim
+
+    geo = Geo(35.9, -97.0, m.Lx, m.Ly, 0.0)
+    sim = RadarSim(m, geo)
+    sim.setTornadoes([{"x": 0.5 * m.Lx, "y": 0.5 * m.Ly, "R": 150.0, "v": 60.0, "debrisTop": 1200.0,
+                       "debrisDbz": 55.0, "debris": 1.0}])
+    lat, lon = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3)
+    R = sim.addRadar({"id": "KTST", "lat": lat, "lon": lon, "h": 0.0}, {"vcp": 35, "seed": 11})
+    done = sim.encDue(None, R["phase"] + R["vcp"]["dur"] - 1)
+    vol = sim.collect(done[0], "js", 0.0)
+    sim.destroy()
     m.destroy()
     assert info["parity"] == m.parity and info["time"] == m.time
     assert np.array_equal(Vj, V.ravel()) and np.array_equal(Q1j, Q1.ravel()) and np.array_equal(Q2j, Q2.ravel())
@@ -41,3 +53,7 @@ def test_node_facade_steps_equal_python_host(tmp_path):
     mj = np.fromfile(info["mout"], np.float32)
     assert info["mapsT"] == maps["t"]
     assert np.array_equal(mj, np.concatenate([maps[f"m{q}"] for q in range(4)]))
+    # a seeded radar volume (rng.js draws, RADAR_FIELDS blocks, kernels) through either host
+    assert info["volumes"] == len(done) == 1
+    assert np.array_equal(np.fromfile(info["rout"], np.uint8), vol["bytes"])
+    assert info["header"] == vol["header"]

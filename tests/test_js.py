@@ -28,7 +28,8 @@ def test_facade_loads_and_matches_golden():
     out = _run()
     for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState",
                  "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer",
-                 "mapsRequest", "mapsCollect"]:
+                 "mapsRequest", "mapsCollect", "radarCreate", "radarDestroy", "radarSetTornadoes", "radarCut",
+                 "radarReadback", "radarCollect"]:
         assert name in out["exports"]
     assert out["mismatches"] == 0
     assert out["levels"] == [56, 80, 100] and out["pads"] == [8, 24, 40]
@@ -52,3 +53,37 @@ def test_facade_surrogate_targets_equal_python_host():
     rows[0, 9], rows[0, 21], rows[1, 9], rows[1, 21] = 0, 3e4, 5e3, -4e3
     assert np.array_equal(np.array(out["targets"], np.float32).reshape(2, 24), targets(rows))
 
+
+
+def test_facade_radar_host_equals_python_host():
+    """Seeded draws (rng.js:3-27), the volume layout and a cut's RADAR_FIELDS block are the same in both
+    hosts, so a seeded radar scans the same azimuths at the same times."""
+    from outbreaklab_amd.radar import Geo, Rng, cut_block, plan_radar
+
+    out = _run()
+    for sd, got in zip([7, "KTLX", 123456789], out["rng"]):
+        r = Rng(sd)
+        assert got == [r.u(), r.u(), r.u()]
+    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(os.path.join(ROOT, "tests", "golden",
+                                                                                  "wk_sounding.txt")).read())
+    geo = Geo(35.9, -97.0, m.Lx, m.Ly, 300.0)
+    rd = out["radar"]
+    lat, lon = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3)
+    assert [lat, lon] == rd["ll"] and list(geo.fwd(36.4, -96.1)) == rd["xy"]
+    R = plan_radar(m, geo, {"id": "KTST", "lat": lat, "lon": lon, "h": 350.0}, {"vcp": 212, "seed": 7})
+    R["vol"], R["az0"] = 3, R["rand"]()
+    assert R["phase"] == rd["phase"] and R["az0"] == rd["az0"]
+    assert (R["bytes"], R["ngRef"], R["ngDop"]) == (rd["bytes"], rd["ngRef"], rd["ngDop"])
+    assert [c["t0"] for c in R["cuts"]] == rd["t0"]
+    assert np.array_equal(cut_block(m, geo, R, R["cuts"][5], 123.5, 2), np.array(rd["block"], np.float32))
+
+
+def test_rng_is_uniform():
+    from outbreaklab_amd.radar import Rng
+
+    r = Rng(2024)
+    x = np.array([r.u() for _ in range(100000)])
+    assert x.min() >= 0 and x.max() < 1 and abs(x.mean() - 0.5) < 0.005 and abs(x.var() - 1 / 12) < 0.002
+    # equidistribution over 16 bins
+    h = np.bincount((x * 16).astype(int), minlength=16)
+    assert h.min() > 0.95 * 100000 / 16 and h.max() < 1.05 * 100000 / 16
