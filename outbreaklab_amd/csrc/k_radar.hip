@@ -2,9 +2,9 @@
 // (radar.js:41-268): pass 1 samples the state along 4/3-earth beams at real great-circle azimuths
 // and quantises REF / VEL / SW / ZDR / RHO to Archive II codes (4 gates per 32-bit word), writing
 // KDP per Doppler gate to a scratch radial; pass 2 accumulates PHIDP (two-way) along each radial.
-// One work-item per (radial, 4 gates) in pass 1 and per radial in pass 2. The state is sampled
-// trilinearly with clamp-to-edge at texel coordinate u*S - 1/2 (the reference's linear sampler),
-// with exact FP32 weights.
+// One work-item per gate in pass 1 (four lanes pack a word) and one wave per radial in pass 2.
+// The state is sampled trilinearly with clamp-to-edge at texel coordinate u*S - 1/2 (the
+// reference's linear sampler), with exact FP32 weights.
 #include <hip/hip_runtime.h>
 
 #include "obl_device.h"
@@ -236,29 +236,32 @@ __device__ Gate gate(const RadarCtx& C, const float4* tor, int ri, int gi) {
 }
 }  // namespace
 
+// one work-item per gate, consecutive lanes along the beam: at one instruction the wave's samples
+// lie 250 m apart, a cell or two, so their texels share cache lines. Four lanes make a word; the
+// radial's gate count is a multiple of 4, so a word never straddles two radials.
+__device__ __forceinline__ uint32_t pack4(uint32_t b) {
+  return b | (__shfl_down(b, 1) << 8) | (__shfl_down(b, 2) << 16) | (__shfl_down(b, 3) << 24);
+}
+
 __global__ void __launch_bounds__(64) k_radar_cut(RadarU R, RadarArgs a) {
-  const int nq = (int)R.gates.x / 4;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ngr = (int)R.gates.x, nq = ngr / 4;
   const int nrad = (int)R.cut.w;
-  if (t >= nrad * nq) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < nrad * ngr;  // whole 4-lane groups are live or not
   const int nx = (int)R.mdl.x, ny = (int)R.mdl.y, nz = (int)R.mdl.z;
   const RadarCtx C{R, a.BS, Tex{reinterpret_cast<const float*>(a.V), nx + 1, ny + 1, nz + 1, 4},
                    Tex{reinterpret_cast<const float*>(a.Q1), nx, ny, nz, 4}, Tex{reinterpret_cast<const float*>(a.Q2), nx, ny, nz, 4},
                    Tex{reinterpret_cast<const float*>(a.aux), nx, ny, nz, 2}};
-  const int ri = t / nq, q = t % nq;
-  const uint32_t mask = (uint32_t)R.offB.w;
-  uint32_t pr = 0u, pv = 0u, ps = 0u, pz = 0u, ph = 0u;
+  const int ri = live ? t / ngr : 0, gi = live ? t % ngr : 0;
   const int ng = (int)R.gates.y, ndq = ng / 4;
-  for (int k = 0; k < 4; k++) {
-    const int gi = q * 4 + k;
-    const Gate G = gate(C, a.tor, ri, gi);
-    pr |= G.zref << (8u * k);
-    pv |= G.vel << (8u * k);
-    ps |= G.sw << (8u * k);
-    pz |= G.zdr << (8u * k);
-    ph |= G.rho << (8u * k);
-    if (gi < ng) a.kdp[ri * ng + gi] = G.kdp;
-  }
+  Gate G{0u, 0u, 0u, 0u, 0u, -1.0f};
+  if (live) G = gate(C, a.tor, ri, gi);
+  if (live && gi < ng) a.kdp[ri * ng + gi] = G.kdp;
+  // all lanes take part in the shuffles
+  const uint32_t pr = pack4(G.zref), pv = pack4(G.vel), ps = pack4(G.sw), pz = pack4(G.zdr), ph = pack4(G.rho);
+  if (!live || (gi & 3)) return;
+  const int q = gi >> 2;
+  const uint32_t mask = (uint32_t)R.offB.w;
   a.out[(uint32_t)R.offA.x + (uint32_t)(ri * nq + q)] = pr;
   if (q < ndq) {
     const uint32_t o = (uint32_t)(ri * ndq + q);
@@ -269,31 +272,43 @@ __global__ void __launch_bounds__(64) k_radar_cut(RadarU R, RadarArgs a) {
   }
 }
 
-// PHIDP accumulated along each radial from KDP (two-way), from the system offset
+// PHIDP accumulated along each radial from KDP (two-way), from the system offset: one wave per
+// radial. The running sum keeps the reference's serial FP32 order (one add per gate, in gate order,
+// each increment rounded before the add): a chunk of 64 gates is loaded coalesced, the increments are
+// formed lane-parallel, and the chain walks the lanes with readlane; the codes are formed
+// lane-parallel again and packed four to a word.
 __global__ void __launch_bounds__(64) k_radar_phi(RadarU R, RadarArgs a) {
-  const int ri = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ri >= (int)R.cut.w) return;
+  const int ri = blockIdx.x;
+  const int lane = threadIdx.x;
   const int ng = (int)R.gates.y;
+  const float gw = R.gates.w;
+  const float* kr = a.kdp + (size_t)ri * (size_t)ng;
+  uint32_t* dst = a.out + (uint32_t)R.offB.x + (uint32_t)(ri * (ng / 4));
   float acc = R.nyq.w;
-  for (int q = 0; q < ng / 4; q++) {
-    uint32_t w = 0u;
-    for (int k = 0; k < 4; k++) {
-      const int gi = q * 4 + k;
-      const float kd = a.kdp[ri * ng + gi];
-      if (kd >= 0.0f) {
-        acc += 2.0f * kd * R.gates.w * 0.001f;
-        const float phi = acc - 360.0f * floorf(acc / 360.0f);
-        w |= (uint32_t)clampf_(rintf(phi * 253.0f / 360.0f + 2.0f), 2.0f, 255.0f) << (8u * k);
-      }
+  for (int base = 0; base < ng; base += 64) {
+    const int g = base + lane;
+    const float kd = g < ng ? kr[g] : -1.0f;
+    const float inc = kd >= 0.0f ? 2.0f * kd * gw * 0.001f : 0.0f;  // + 0 leaves acc (>= 25) unchanged
+    float mine = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 64; l++) {
+      acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), l));
+      mine = lane == l ? acc : mine;
     }
-    a.out[(uint32_t)R.offB.x + (uint32_t)(ri * (ng / 4) + q)] = w;
+    uint32_t b = 0u;
+    if (kd >= 0.0f) {
+      const float phi = mine - 360.0f * floorf(mine / 360.0f);
+      b = (uint32_t)clampf_(rintf(phi * 253.0f / 360.0f + 2.0f), 2.0f, 255.0f);
+    }
+    const uint32_t w = b | (__shfl_down(b, 1) << 8) | (__shfl_down(b, 2) << 16) | (__shfl_down(b, 3) << 24);
+    if ((lane & 3) == 0 && g < ng) dst[g >> 2] = w;  // ng is a multiple of 4: the whole word is in range
   }
 }
 
 void launch_radar_cut(hipStream_t s, const RadarU& R, const RadarArgs& a) {
-  const int nthreads = (int)R.cut.w * ((int)R.gates.x / 4);
+  const int nthreads = (int)R.cut.w * (int)R.gates.x;
   hipLaunchKernelGGL(k_radar_cut, dim3((nthreads + 63) / 64), dim3(64), 0, s, R, a);
-  if (((uint32_t)R.offB.w) & 16u) hipLaunchKernelGGL(k_radar_phi, dim3(((int)R.cut.w + 63) / 64), dim3(64), 0, s, R, a);
+  if (((uint32_t)R.offB.w) & 16u) hipLaunchKernelGGL(k_radar_phi, dim3((int)R.cut.w), dim3(64), 0, s, R, a);
 }
 
 }  // namespace obl
