@@ -200,7 +200,7 @@ def radar_volume(m, volumes=2):
         return {"volume_kernel_ms": round(vol_ms, 3), "volume_wall_ms": round(wall, 3), "cuts": len(R["cuts"]),
                 "gates_per_volume": gates, "ref_gates": R["ngRef"], "dop_gates": R["ngDop"], "bytes": R["bytes"],
                 "gates_per_s": round(gates / (vol_ms * 1e-3), 1),
-                "note": "virtual WSR-88D (radar.js:41-413), one work-item per 4 gates; wall includes the pinned "
+                "note": "virtual WSR-88D (radar.js:41-418), one work-item per gate; wall includes the pinned "
                         "readback and the host"}
     finally:
         sim.destroy()

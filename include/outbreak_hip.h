@@ -133,7 +133,7 @@ int obl_maps_collect(obl_model* h, int slot, float* dst, size_t dst_floats);
 
 /* ---------------- virtual WSR-88D radars ----------------
  * The reference's RadarSim (radar.js:271-413) keeps the VCP layout, scan schedule and uniform
- * packing on the host; these calls are its GPU side (radar.js:41-268). A radar owns a volume buffer
+ * packing on the host; these calls are its GPU side (radar.js:41-268, 274-275, 318-328). A radar owns a volume buffer
  * of `bytes` (8-bit Archive II codes, 4 gates per 32-bit word, each cut's moments radial-major at
  * the word offsets its uniform block names), a KDP scratch of `kdp_floats` (>= radials x Doppler
  * gates of any cut) and two pinned readback slots. Errors are reported on the model's handle.
@@ -145,7 +145,7 @@ void obl_radar_destroy(obl_radar* r);
 /* setTornadoes (radar.js:282-287): up to 64 sub-grid tornadoes of 8 floats (x, y, core radius, Vmax,
  * debris top, debris dBZ, debris 0-1, 0), model metres; shared by every radar of the model */
 int obl_radar_set_tornadoes(obl_model* h, const float* f, int n);
-/* encCut (radar.js:368-388): enqueue one cut from its RADAR_FIELDS block (44 floats, radar.js:33-35)
+/* encCut (radar.js:369-390): enqueue one cut from its RADAR_FIELDS block (44 floats, radar.js:37-39)
  * on the current state (V[parity], Q1[0], Q2[0], aux); PHIDP's pass runs when the moment mask holds
  * it. The block must name this model's dimensions and fit the radar's buffers. */
 int obl_radar_cut(obl_radar* r, const float* ru, int parity);

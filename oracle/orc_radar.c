@@ -59,7 +59,7 @@ typedef struct {
 static F4 bsA_(const Ctx* c, int k) { const float* p = c->BS + clampi_(k, 0, (int)c->R->mdl.z - 1) * 8; F4 r = {p[0], p[1], p[2], p[3]}; return r; }
 static F4 bsB_(const Ctx* c, int k) { const float* p = c->BS + clampi_(k, 0, (int)c->R->mdl.z - 1) * 8 + 4; F4 r = {p[0], p[1], p[2], p[3]}; return r; }
 
-/* radar.js:52-68 */
+/* radar.js:57-69 */
 static F2 gatePos(const Ctx* c, float a, float s) {
   const RU* R = c->R;
   float p1 = R->geo.z, l1 = R->geo.w, d = s / RE_;
@@ -73,7 +73,7 @@ static F2 gatePos(const Ctx* c, float a, float s) {
   F2 r = {RE_ * k * cosf(p2) * sinf(dl) - R->mdl2.z, RE_ * k * (cosf(p0) * sinf(p2) - sinf(p0) * cosf(p2) * cosf(dl)) - R->mdl2.w};
   return r;
 }
-/* radar.js:70-76 */
+/* radar.js:71-77 */
 static F2 beam(const Ctx* c, float r, float el) {
   float a = c->R->site.w;
   float q = r * r + 2.0f * r * a * sinf(el);
@@ -84,7 +84,7 @@ static F2 beam(const Ctx* c, float r, float el) {
 }
 
 typedef struct { float zr, zs, zg, vt; F3 u; float T, qr, rho, nu, inside, lr; } Smp;
-/* radar.js:94-143 */
+/* radar.js:95-139 */
 static Smp sample(const Ctx* c, F2 xy, float z) {
   const RU* R = c->R;
   Smp o;
@@ -147,7 +147,7 @@ static Smp sample(const Ctx* c, F2 xy, float z) {
 }
 
 typedef struct { uint32_t zref, vel, sw, zdr, rho; float kdp; } Gate;
-/* radar.js:148-233 */
+/* radar.js:144-214 */
 static Gate gate(const Ctx* c, const float* tor, int ri, int gi) {
   const RU* R = c->R;
   Gate G = {0, 0, 0, 0, 0, -1.0f};
@@ -223,7 +223,8 @@ static Gate gate(const Ctx* c, const float* tor, int ri, int gi) {
   return G;
 }
 
-/* one cut (both passes): ru = the 44-float RADAR_FIELDS block; out and kdp as the GPU's buffers */
+/* one cut (both passes, radar.js:215-240 and 243-267): ru = the 44-float RADAR_FIELDS block; out
+   and kdp as the GPU's buffers */
 void orc_radar_cut(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2, const float* aux,
                    const float* tor, uint32_t* out, float* kdp) {
   RU R;
@@ -253,7 +254,7 @@ void orc_radar_cut(const float* ru, const float* BS, const float* V, const float
       }
     }
   if (!(mask & 16u)) return;
-  /* radar.js:243-265 */
+  /* radar.js:243-267 */
   for (int ri = 0; ri < nrad; ri++) {
     float acc = R.nyq.w;
     for (int q = 0; q < ng / 4; q++) {

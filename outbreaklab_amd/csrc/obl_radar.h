@@ -5,7 +5,7 @@
 
 namespace obl {
 
-// RADAR_FIELDS (radar.js:33-35), 11 vec4f, in the reference's order:
+// RADAR_FIELDS (radar.js:37-39), 11 vec4f, in the reference's order:
 //   site = (model x, model y, antenna height MSL, ke R); geo = (lat0, lon0, site lat, site lon) radians;
 //   cut = (elevation rad, az0 deg, d az deg, nrad); gates = (REF gates, Doppler gates, first gate m, gate m);
 //   nyq = (Nyquist m/s, time s, noise seed, system PHIDP deg); offA = word offsets (REF, VEL, SW, ZDR);

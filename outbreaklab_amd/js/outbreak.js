@@ -417,7 +417,7 @@ class Surrogate {
   infer(rows, stride) { return addon.mlpInfer(this.h, rows instanceof Float32Array ? rows : Float32Array.from(rows), stride || 24); }
 }
 
-// ── virtual WSR-88D radars (radar.js:18-413), azimuthal-equidistant projection (geo.js:6-29) ──
+// ── virtual WSR-88D radars (radar.js:18-418), azimuthal-equidistant projection (geo.js:6-29) ──
 const RE_M = 6371000, KE = 4 / 3;
 const GATE0 = 2125, GATEDR = 250, MAX_REF_GATES = 1832, MAX_DOP_GATES = 1192;
 const L2_SCALE = { REF: [2, 66], VEL: [2, 129], SW: [2, 129], ZDR: [16, 128], PHI: [253 / 360, 2], RHO: [300, -60.5] };
@@ -502,7 +502,7 @@ class RadarSim {
     this.radars.push(R);
     return R;
   }
-  // addRadar's layout without the device buffers (radar.js:288-321)
+  // addRadar's layout without the device buffers (radar.js:289-316)
   static plan(m, g, site, opts) {
     opts = opts || {};
     const vcp = VCPS[opts.vcp] || VCPS[212];
@@ -529,7 +529,8 @@ class RadarSim {
     const rand = opts.seed !== undefined ? (r => () => r.u())(new Rng(opts.seed)) : Math.random;
     return {
       site, opts, vcp, vcpId: +opts.vcp || 212, cuts, bytes: off, ngRef, ngDop, sx, sy, rand, U: new UBlock(RADAR_FIELDS),
-      h: null, readBusy: [false, false], phase: rand() * vcp.dur, vol: -1, next: 0, volStart: 0, az0: 0,
+      h: null, readBusy: [false, false], phase: rand() * vcp.dur, vol: -1, next: 0, volStart: 0, cutsDone: 0, az0: 0,
+      done: 0, failed: 0,
     };
   }
   destroy() {
@@ -560,7 +561,7 @@ class RadarSim {
     }
     return done;
   }
-  // the cut's RADAR_FIELDS block (radar.js:366-380)
+  // the cut's RADAR_FIELDS block (radar.js:369-382)
   cutBlock(R, c, simT) { return RadarSim.block(this.m, this.geo, R, c, simT, this.nTor); }
   static block(m, g, R, c, simT, nTor) {
     const U = R.U, off = {};
@@ -586,7 +587,7 @@ class RadarSim {
     addon.radarCut(this.m.h, R.h, this.cutBlock(R, c, simT), this.m.parity);
     c.simT = simT;
   }
-  // a finished volume's bytes and the header the server reads (radar.js:390-412)
+  // a finished volume's bytes and the header the server reads (radar.js:394-416)
   collect(d, run, epoch0) {
     const R = d.R;
     return new Promise((resolve, reject) => setImmediate(() => {

@@ -1,10 +1,10 @@
 """Virtual WSR-88D radars on the MI355X model state: the host side of the reference's RadarSim
-(radar.js:271-413) over the C ABI's radar calls, and the azimuthal-equidistant projection it uses
+(radar.js:271-418) over the C ABI's radar calls, and the azimuthal-equidistant projection it uses
 (geo.js:6-29).
 
-The VCP tables (radar.js:24-31), the volume layout (cuts x moments, radial-major, 8-bit Archive II
-codes; radar.js:289-312), the scan schedule (radar.js:314-318, 339-364) and the RADAR_FIELDS block
-(radar.js:33-35, 366-380) follow the reference; the GPU passes are `k_radar_cut` / `k_radar_phi`.
+The VCP tables (radar.js:24-35), the volume layout (cuts x moments, radial-major, 8-bit Archive II
+codes; radar.js:289-313), the scan schedule (radar.js:314-316, 344-367) and the RADAR_FIELDS block
+(radar.js:37-39, 369-382) follow the reference; the GPU passes are `k_radar_cut` / `k_radar_phi`.
 The reference draws each site's volume phase and each volume's first azimuth from Math.random; with
 `opts["seed"]` they come from the reference's own seeded generator (rng.js:3-27), so runs reproduce and
 the Node host draws the same numbers.
@@ -118,7 +118,7 @@ class Geo:
 
 
 def plan_radar(m, geo, site, opts=None):
-    """addRadar's layout (radar.js:288-321) without the device buffers: gate counts from the reach of
+    """addRadar's layout (radar.js:289-316) without the device buffers: gate counts from the reach of
     the model, each cut's moments at byte offsets (radial-major), the scan time of each cut."""
     opts = dict(opts or {})
     vcp = VCPS.get(opts.get("vcp", 212), VCPS[212])
@@ -151,7 +151,7 @@ def plan_radar(m, geo, site, opts=None):
 
 
 def cut_block(m, geo, R, c, simT, n_tor=0):
-    """The cut's RADAR_FIELDS block (radar.js:366-380): 44 float32."""
+    """The cut's RADAR_FIELDS block (radar.js:369-382): 44 float32."""
     U = R["U"]
     off = {o["name"]: o["off"] // 4 for o in c["mom"]}
     mask = sum(MOMENT_BITS[n] for n in off)
@@ -172,7 +172,7 @@ def cut_block(m, geo, R, c, simT, n_tor=0):
 
 
 class RadarSim:
-    """RadarSim (radar.js:271-413) over `obl_radar_*`."""
+    """RadarSim (radar.js:271-418) over `obl_radar_*`."""
 
     def __init__(self, model, geo):
         self.m, self.geo = model, geo
@@ -203,7 +203,7 @@ class RadarSim:
         self.radars = []
 
     def encDue(self, enc, simT):
-        """encDue (radar.js:339-364): encode every cut whose scan time has come; returns finished volumes."""
+        """encDue (radar.js:344-367): encode every cut whose scan time has come; returns finished volumes."""
         done = []
         for R in self.radars:
             while True:
@@ -233,7 +233,7 @@ class RadarSim:
         c["simT"] = simT
 
     def collect(self, d, run="run", epoch0=0.0):
-        """collect (radar.js:390-412): the finished volume's bytes and the header the server reads."""
+        """collect (radar.js:394-416): the finished volume's bytes and the header the server reads."""
         R = d["R"]
         buf = np.empty(R["bytes"], np.uint8)
         _lib.check(_lib.lib().obl_radar_collect(R["h"], d["slot"], buf.ctypes.data_as(C.c_void_p), buf.nbytes), self.m.h)

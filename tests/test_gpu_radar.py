@@ -1,4 +1,4 @@
-"""Virtual WSR-88D volumes (radar.js:41-413) on the GPU against the FP32 oracle.
+"""Virtual WSR-88D volumes (radar.js:41-418) on the GPU against the FP32 oracle.
 
 The 128 x 128 x 40 Thompson workload after 300 steps (30 min), one radar 40 km west / 30 km south of
 the domain centre, VCP 212 (17 cuts, all six moments), one sub-grid tornado with debris at the storm's
@@ -91,7 +91,7 @@ def test_volume_matches_oracle(volume):
                 n_echo += int((b >= 66 + 2 * 35).sum())
             echo = max(int((b > 0).sum()), 1)
             # VEL: the radial direction is the difference of two FP32 gate positions 200 m apart
-            # (radar.js:154), so an ulp of sin/cos moves V . e by up to ~0.1 m/s
+            # (radar.js:152), so an ulp of sin/cos moves V . e by up to ~0.1 m/s
             lim1 = 0.04 if o["name"] == "VEL" else 0.01
             assert off1 <= lim1 * echo and big <= max(2, 1e-3 * echo), (c["index"], o["name"], off1, big, echo)
     assert n_echo > 1000, "the storm shows no echo above 35 dBZ"

@@ -1,5 +1,5 @@
 """Pins of the virtual-radar oracle (radar.js:41-268) and of the radar host's layout and projection
-(radar.js:271-413, geo.js:6-29), on the CPU.
+(radar.js:271-418, geo.js:6-29), on the CPU.
 
 Each check is fixed by something other than the restated formula:
 - the beam height and ground distance against the geometry of a straight ray in an earth of radius
@@ -276,7 +276,7 @@ def test_uniform_wind_radial_velocity(lab_c1):
         if abs(abs(vr) - nyq) < 0.5:
             continue
         row = vel[ri][lab_c1[ri] == 1]
-        # the radial's direction comes from two FP32 gate positions 200 m apart (radar.js:154): up to
+        # the radial's direction comes from two FP32 gate positions 200 m apart (radar.js:152): up to
         # ~0.2 m/s (0.4 code) of rounding noise in V . e
         assert np.all(np.abs(row - f) < 0.9), (ri, np.unique(row), f)
         hits += np.count_nonzero(row == round(f))
