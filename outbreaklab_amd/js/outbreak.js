@@ -15,6 +15,7 @@
 //  Written in ES2019 (no ?? or ?.), so Node 12 runs it.
 // ─────────────────────────────────────────────────────────────────────────────
 const path = require('path');
+const zlib = require('zlib');
 const addon = require('./obl_napi.node');
 addon.load(process.env.OBL_LIB || path.join(__dirname, '..', 'liboutbreak_hip.so'));
 
@@ -611,4 +612,13 @@ class RadarSim {
   }
 }
 
-module.exports = { RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };
+// the /api/volume request body of a collected volume (app.js:1743-1750): u32-LE JSON length, the header's
+// JSON, the codes; gzip (sent with X-OB-Gzip: 1)
+function volumeBody(header, bytes) {
+  const hj = Buffer.from(JSON.stringify(header), 'utf8');
+  const pre = Buffer.alloc(4);
+  pre.writeUInt32LE(hj.length, 0);
+  return zlib.gzipSync(Buffer.concat([pre, hj, Buffer.from(bytes.buffer, bytes.byteOffset, bytes.byteLength)]));
+}
+
+module.exports = { volumeBody, RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

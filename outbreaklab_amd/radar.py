@@ -12,8 +12,11 @@ the Node host draws the same numbers.
 from __future__ import annotations
 
 import ctypes as C
+import gzip
+import json
 import math
 import random
+import struct
 
 import numpy as np
 
@@ -248,3 +251,26 @@ class RadarSim:
                               "moments": [{"name": o["name"], "ngates": o["ngates"], "off": o["off"]} for o in c["mom"]]}
                              for c in R["cuts"]]}
         return {"R": R, "header": header, "bytes": buf}
+
+
+# ---------------- the volume upload (app.js:1735-1750) ----------------
+def volume_body(header, codes, compress=True):
+    """The /api/volume request body for a collected volume (app.js:1743-1750): the header's JSON length as a
+    little-endian u32, the JSON (UTF-8), then the volume's codes, gzip-compressed (sent with X-OB-Gzip: 1)."""
+    hj = json.dumps(header, separators=(",", ":"), ensure_ascii=False).encode("utf-8")
+    raw = struct.pack("<I", len(hj)) + hj + np.ascontiguousarray(codes, np.uint8).tobytes()
+    return gzip.compress(raw, compresslevel=6, mtime=0) if compress else raw
+
+
+def level2_sweeps(header, codes):
+    """The arguments the Archive II writer takes for a collected volume, as the server builds them from an
+    upload (server.py:256-266): (icao, lat, lon, elev, epoch_ms, sweeps, vcp), each sweep's moments as
+    {name: (codes of nrad x ngates, ngates)}."""
+    b = memoryview(np.ascontiguousarray(codes, np.uint8).tobytes())
+    sweeps = []
+    for s in header["sweeps"]:
+        mom = {m["name"]: (b[m["off"]:m["off"] + int(s["nrad"]) * int(m["ngates"])], int(m["ngates"]))
+               for m in s["moments"]}
+        sweeps.append({**s, "moments": mom})
+    return (header["icao"].upper(), float(header["lat"]), float(header["lon"]), float(header["elev"]),
+            int(header["epoch_ms"]), sweeps, int(header.get("vcp", 212)))

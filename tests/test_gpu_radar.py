@@ -135,3 +135,21 @@ def test_readback_slots_and_schedule(storm):
     assert v1["bytes"].size == v2["bytes"].size == v3["bytes"].size == R["bytes"]
     assert v1["header"]["epoch_ms"] == 10000 and v2["header"]["epoch_ms"] == 430000
     sim.destroy()
+
+
+def test_upload_body_of_a_gpu_volume(volume):
+    """The volume the GPU made, as the upload body (app.js:1743-1750) parsed the way the server parses it
+    (server.py:250-266): the header and every moment's codes come back unchanged."""
+    import json
+    import struct
+    import zlib
+
+    from outbreaklab_amd.radar import volume_body
+
+    m, geo, sim, R, tor, vol = volume
+    body = zlib.decompress(volume_body(vol["header"], vol["bytes"]), 16 + zlib.MAX_WBITS)
+    (hl,) = struct.unpack("<I", body[:4])
+    H = json.loads(body[4:4 + hl])
+    assert H == json.loads(json.dumps(vol["header"]))
+    payload = np.frombuffer(body, np.uint8, offset=4 + hl)
+    assert np.array_equal(payload, vol["bytes"])
