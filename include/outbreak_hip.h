@@ -69,11 +69,14 @@ enum {
   OBL_FIELD_CIMAP = 10,
   OBL_FIELD_PHI = 11,
   OBL_FIELD_RHS = 12,
-  OBL_FIELD_BS = 13
+  OBL_FIELD_BS = 13,
+  OBL_FIELD_SFC = 14, /* hurricane mode: sea surface map, nx*ny float4 (land fraction, SST K, 0, 0) */
+  OBL_FIELD_VP = 15   /* hurricane mode: the winds the advection carries (WGSL_M_PREP), as V */
 };
 
 /* obl_step flags */
 #define OBL_STEP_LSF 1u   /* run the large-scale forcing this step (model.js:1461) */
+#define OBL_STEP_FILT 4u  /* hurricane mode: run the 2-dx filter before the step (model.js:1447-1455) */
 #define OBL_STEP_JUDGE 2u /* with a surrogate attached: also run the plain microphysics from the same
                              inputs and add the difference to the judge statistics (not part of the
                              step's result; no reference counterpart) */
@@ -106,9 +109,16 @@ int obl_set_base_state(obl_model* h, const float* B, int nz);
 int obl_set_boundaries(obl_model* h, const float* f, int count);
 /* setCI (model.js:1383-1389): count towers of 12 floats */
 int obl_set_ci(obl_model* h, const float* f, int count);
-/* setSurface (model.js:1298-1303): NULL = outbreak mode (no sea). Hurricane
- * surfaces are outside this library's scope and are rejected. */
+/* setSurface (model.js:1298-1303): surf = nx*ny float4 (land fraction, SST K, 0, 0) per column, the
+ * sea that hurricane mode (hur.x = 1 in the uniform block) reads; NULL = outbreak mode (no sea). */
 int obl_set_surface(obl_model* h, const float* surf);
+/* setPrepField (model.js:1290-1296): allocate (on) or free the Vp field of hurricane runs; while it
+ * exists every step first runs WGSL_M_PREP and the wind advection carries Vp's values. */
+int obl_set_prep_field(obl_model* h, int on);
+/* setVortex (model.js:1318-1365), its device half: the table the init kernel reads, nz x nr float4
+ * (tangential wind m/s, theta' K, 0, 0) at radii i * dr, built on the host; used by obl_reset when
+ * vtx2.z = 1, with vtx = (x, y, dr, nr). NULL / nr 0 drops it. */
+int obl_set_vortex(obl_model* h, const float* table, int nr);
 /* reset (model.js:1427-1440): lsf + init with the given uniform block (written
  * for dt = 1), clears phi/rhs, copies V0 -> V1. seed 0 = the reference's noise. */
 int obl_reset(obl_model* h, const float* uniforms, int seed);

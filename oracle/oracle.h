@@ -51,7 +51,25 @@ extern "C" {
   void orc_bubble_##S(const float* U, R* V, double x, double y, double z, double rh, double rz,      \
                       double dth);                                                                  \
   void orc_maps_##S(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2,          \
-                    const R* phi, R* maps);
+                    const R* phi, R* maps);                                                         \
+  /* hurricane mode (model.js:88-119, 284-389, 392-576, 579-907, 948-1026): sfc = nx*ny x 4 float    \
+     (land fraction, SST K, 0, 0) or NULL, VX = the vortex table nz*nr x 4 float or NULL, Vval = the \
+     winds the V advection carries, phi = the padded solve */                                        \
+  void orc_init2_##S(const float* U, const float* BS, const R* lsfA, const R* lsfB, const float* VX, \
+                     R* V, R* Q1, R* Q2, int seed);                                                 \
+  void orc_nut2_##S(const float* U, const float* BS, const R* V, const R* Q1, const float* sfc,      \
+                    R* aux);                                                                        \
+  void orc_advv1_2_##S(const float* U, const float* BS, const R* V, const R* Vval, R* Vt);          \
+  void orc_advv2_2_##S(const float* U, const float* BS, const R* V, const R* Vval, const R* Vt,     \
+                       R* Vout);                                                                    \
+  void orc_phys2_##S(const float* U, const float* BS, const R* Vin, const R* Q1in, const R* Q2in,   \
+                     const R* aux, const R* lsfA, const R* lsfB, const R* phT, const R* ciMap,      \
+                     const float* sfc, const R* phi, R* Vout, R* Q1o, R* Q2o);                      \
+  void orc_maps2_##S(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2,         \
+                     const R* phi, const float* sfc, R* maps);                                      \
+  void orc_prep_##S(const float* U, const float* BS, const R* V, const R* phi, R* Vp);              \
+  void orc_filt_##S(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, R* Vo,   \
+                    R* Q1o, R* Q2o);
 
 ORC_DECL(float, f32)
 ORC_DECL(double, f64)

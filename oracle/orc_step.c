@@ -248,7 +248,29 @@ static R2 vtRain(const Ctx* c, R qr, R nr, R rho) {
   return r;
 }
 /* outbreak mode: maritime() is false (hur3.w only set in hurricane mode) */
-static R vtGrau(const Ctx* c, R qg, R rho) { return thompson(c) ? vtGT(c, qg, rho) : vtG(c, qg, rho); }
+/* hurricane mode (model.js:88-119): a warm sea under the domain */
+static int hurOn(const Ctx* c) { return c->U[U_HUR] > 0.5f; }
+static int maritime(const Ctx* c) { return c->U[U_HUR3 + 3] > 0.5f && !thompson(c); }
+static R vtGM(const Ctx* c, R q, R rho) {
+  return clampr((R)5 * POW(maxr((R)1000 * rho * q, (R)1e-6), (R)0.25), 1, 9) * SQRT(UF(c, U_STEP + 2) / rho);
+}
+static R vtGrau(const Ctx* c, R qg, R rho) {
+  if (maritime(c)) return vtGM(c, qg, rho);
+  return thompson(c) ? vtGT(c, qg, rho) : vtG(c, qg, rho);
+}
+static R vtI(const Ctx* c, R q, R rho) {
+  return (hurOn(c) && q > (R)1e-9) ? minr((R)3.29 * POW(maxr(rho * q, (R)1e-12), (R)0.16), (R)1.5) : (R)0;
+}
+static R cdSfc(const Ctx* c, R u1, R land) {
+  R f = UF(c, U_HUR2 + 3);
+  R cs = minr(((R)0.75 + (R)0.067 * f * u1) * (R)1e-3, UF(c, U_HUR5)) * f * f;
+  return mixr(cs, UF(c, U_HUR3), clampr(land, 0, 1));
+}
+/* the sea surface map (land fraction, SST K, 0, 0) per column; NULL = no sea (outbreak mode) */
+static R sfcAt(const Ctx* c, const float* sfc, int i, int j, int ch) {
+  if (!sfc) return 0;
+  return (R)sfc[4 * ((size_t)clampi(i, 0, c->nx - 1) + (size_t)c->nx * clampi(j, 0, c->ny - 1)) + ch];
+}
 
 /* ---------------- large-scale target (model.js:125-134) ---------------- */
 static R tprof(R z, R H) { return (R)1 - sstep((R)0.7 * H, (R)1.15 * H, z); }
@@ -412,10 +434,25 @@ void S_(orc_ci)(const float* U, const float* BS, const float* CI, R* ciMap) {
     }
 }
 
-/* ====================== INIT (model.js:284-339), outbreak mode ====================== */
-void S_(orc_init)(const float* U, const float* BS, const R* lsfA, const R* lsfB, R* V, R* Q1, R* Q2, int seed) {
+/* ====================== INIT (model.js:284-339) ====================== */
+/* vxAt (model.js:292-297): the vortex table's (tangential wind, theta') at radius r, level k */
+static R2 vxAt(const Ctx* c, const float* VX, R r, int k) {
+  R2 z0 = {0, 0};
+  int nr = (int)c->U[U_VTX + 3];
+  R fr = r / UF(c, U_VTX + 2);
+  int i0 = (int)FLOOR(fr);
+  if (!VX || i0 >= nr - 1) return z0;
+  size_t o = (size_t)clampi(k, 0, c->nz - 1) * (size_t)nr + (size_t)i0;
+  R t = fr - (R)i0;
+  R2 v = {mixr((R)VX[4 * o], (R)VX[4 * (o + 1)], t), mixr((R)VX[4 * o + 1], (R)VX[4 * (o + 1) + 1], t)};
+  return v;
+}
+void S_(orc_init2)(const float* U, const float* BS, const R* lsfA, const R* lsfB, const float* VX, R* V, R* Q1, R* Q2,
+                   int seed) {
   Ctx C = mkctx(U, BS), *c = &C;
   int nx = c->nx, ny = c->ny, nz = c->nz;
+  int vortex = c->U[U_VTX2 + 2] > 0.5f;
+  R xc = UF(c, U_VTX), yc = UF(c, U_VTX + 1), dx = c->dx;
 #pragma omp parallel for schedule(static)
   for (int z = 0; z <= nz; z++)
     for (int y = 0; y <= ny; y++)
@@ -424,12 +461,35 @@ void S_(orc_init)(const float* U, const float* BS, const R* lsfA, const R* lsfB,
         R4 A = ld2d(c, lsfA, x, y);
         R zz = ((R)k + (R)0.5) * c->dz;
         R nz0 = sstep((R)800, (R)1500, zz);
+        if (vortex) {
+          R px = ((R)x + (R)0.5) * dx - xc, py = ((R)y + (R)0.5) * dx - yc;
+          nz0 = SQRT(px * px + py * py) < UF(c, U_VTX2) ? sstep((R)6000, (R)8000, zz) : (R)1;
+        }
         R noise = UF(c, U_MISC + 2) * (rnd3(x + 17, y + 3 + 65536 * seed, z + 5) - (R)0.5) * (R)2 * ((R)1 - nz0);
         R th = tgtTh(c, A, k) + noise;
         R u = (y < ny && z < nz) ? tgtU(c, lsfB, x, y, k) : (R)0;
         R v = (x < nx && z < nz) ? tgtV(c, lsfB, x, y, k) : (R)0;
+        if (vortex) {
+          /* the balanced warm-core vortex built on the host (setVortex, model.js:1318-1365) */
+          R pux = (R)x * dx - xc, puy = ((R)y + (R)0.5) * dx - yc, ru = maxr(SQRT(pux * pux + puy * puy), 1);
+          R pvx = ((R)x + (R)0.5) * dx - xc, pvy = (R)y * dx - yc, rv = maxr(SQRT(pvx * pvx + pvy * pvy), 1);
+          R pcx = ((R)x + (R)0.5) * dx - xc, pcy = ((R)y + (R)0.5) * dx - yc, rc = SQRT(pcx * pcx + pcy * pcy);
+          u -= vxAt(c, VX, ru, k).x * puy / ru;
+          v += vxAt(c, VX, rv, k).x * pvx / rv;
+          th += vxAt(c, VX, rc, k).y;
+        }
         R T = th * bsB(c, k).x;
         R qv = minr(tgtQv(c, A, k), (R)0.95 * qvsat(T, bsB(c, k).y, fIce(T)));
+        if (vortex) {
+          /* a nearly saturated core (model.js:328-336) */
+          R pcx = ((R)x + (R)0.5) * dx - xc, pcy = ((R)y + (R)0.5) * dx - yc;
+          R f = ((R)1 - sstep((R)0.6, (R)1, SQRT(pcx * pcx + pcy * pcy) / UF(c, U_VTX2))) * sstep((R)500, (R)1500, zz) *
+                ((R)1 - sstep((R)7000, (R)10000, zz));
+          R q2 = maxr(qv, f * UF(c, U_VTX2 + 1) * qvsat(T, bsB(c, k).y, fIce(T)));
+          R dqv = q2 - qv;
+          qv = q2;
+          th -= (R)0.608 * dqv * bsA(c, k).x;
+        }
         R4 o = {u, v, 0, th};
         put4(V, vidx(c, x, y, z), o);
         if (x < nx && y < ny && z < nz) {
@@ -438,6 +498,9 @@ void S_(orc_init)(const float* U, const float* BS, const R* lsfA, const R* lsfB,
           put4(Q2, cidx(c, x, y, z), q2);
         }
       }
+}
+void S_(orc_init)(const float* U, const float* BS, const R* lsfA, const R* lsfB, R* V, R* Q1, R* Q2, int seed) {
+  S_(orc_init2)(U, BS, lsfA, lsfB, NULL, V, Q1, Q2, seed);
 }
 
 /* Build-defined warm bubble (no reference counterpart): theta += dth cos^2(pi r / 2),
@@ -470,7 +533,7 @@ static R nut_vc(const Ctx* c, const R* V, int x, int y, int z) {
 static R nut_wc(const Ctx* c, const R* V, int x, int y, int z) {
   return (R)0.5 * (ldV(c, V, x, y, z).z + ldV(c, V, x, y, z + 1).z);
 }
-void S_(orc_nut)(const float* U, const float* BS, const R* V, const R* Q1, R* aux) {
+void S_(orc_nut2)(const float* U, const float* BS, const R* V, const R* Q1, const float* sfc, R* aux) {
   Ctx C = mkctx(U, BS), *c = &C;
   R dx = c->dx, dz = c->dz;
 #pragma omp parallel for schedule(static)
@@ -499,10 +562,24 @@ void S_(orc_nut)(const float* U, const float* BS, const R* V, const R* Q1, R* au
         R lh = UF(c, U_TURB) * dx, lv = (R)0.18 * dz;
         R nuh = minr(lh * lh * sS * fst, (R)0.08 * dx * dx / maxr(c->dt, (R)0.1));
         R nuv = minr(lv * lv * sS * fst, (R)0.15 * dz * dz / maxr(c->dt, (R)0.1));
+        /* hurricane mode: the K-profile boundary layer, K = kappa u* z (1 - z/h)^2 (model.js:377-386) */
+        R zk = ((R)z + (R)0.5) * dz, hb = UF(c, U_HUR2 + 2);
+        if (hurOn(c) && zk < hb) {
+          R4 s0 = ldV(c, V, x, y, 0);
+          R ua = (R)0.5 * (s0.x + ldV(c, V, x + 1, y, 0).x), va = (R)0.5 * (s0.y + ldV(c, V, x, y + 1, 0).y);
+          R u1 = SQRT(ua * ua + va * va);
+          R ust = SQRT(cdSfc(c, u1, sfcAt(c, sfc, x, y, 0))) * maxr(u1, UF(c, U_HUR3 + 1));
+          R q = (R)1 - zk / hb;
+          nuv = maxr(nuv, minr((R)0.4 * ust * zk * q * q, (R)0.15 * dz * dz / maxr(c->dt, (R)0.1)));
+        }
         size_t i = cidx(c, x, y, z);
         aux[2 * i] = nuh;
         aux[2 * i + 1] = nuv;
       }
+}
+
+void S_(orc_nut)(const float* U, const float* BS, const R* V, const R* Q1, R* aux) {
+  S_(orc_nut2)(U, BS, V, Q1, NULL, aux);
 }
 
 /* ====================== advection of V (model.js:392-481) ====================== */
@@ -524,22 +601,35 @@ static Loc localVel(const Ctx* c, const R* V, int x, int y, int z) {
   return L;
 }
 static R3 kvec(const Ctx* c) { R3 k = {c->dt / c->dx, c->dt / c->dx, c->dt / c->dz}; return k; }
-/* first-order trace, clamped to [0, N] (cn.w = 0 in outbreak mode) */
-static R3 trace(const Ctx* c, R3 p, R3 v0, R s) {
-  R3 k = kvec(c), r;
-  r.x = clampr(p.x - s * k.x * v0.x, 0, (R)c->nx);
-  r.y = clampr(p.y - s * k.y * v0.y, 0, (R)c->ny);
-  r.z = clampr(p.z - s * k.z * v0.z, 0, (R)c->nz);
-  return r;
-}
 #define VSAMP(T, tx, ty, tz) sample3(T, c->nx + 1, c->ny + 1, c->nz + 1, tx, ty, tz)
 /* texU/texVv/texW/texC (model.js:411-414) in texel coordinates */
 static R sampU(const Ctx* c, const R* T, R3 p) { return VSAMP(T, p.x, p.y - (R)0.5, p.z - (R)0.5).x; }
 static R sampV(const Ctx* c, const R* T, R3 p) { return VSAMP(T, p.x - (R)0.5, p.y, p.z - (R)0.5).y; }
 static R sampW(const Ctx* c, const R* T, R3 p) { return VSAMP(T, p.x - (R)0.5, p.y - (R)0.5, p.z).z; }
 static R sampC(const Ctx* c, const R* T, R3 p) { return VSAMP(T, p.x - (R)0.5, p.y - (R)0.5, p.z - (R)0.5).w; }
+/* velAt / velQ (model.js:416-418, 488-492): the face winds sampled at p */
+static R3 velAt(const Ctx* c, const R* V, R3 p) { R3 v = {sampU(c, V, p), sampV(c, V, p), sampW(c, V, p)}; return v; }
+static R3 clampX(const Ctx* c, R3 p) {
+  R3 r = {clampr(p.x, 0, (R)c->nx), clampr(p.y, 0, (R)c->ny), clampr(p.z, 0, (R)c->nz)};
+  return r;
+}
+/* departure (s = 1) / forward (s = -1) point, clamped to [0, N] (model.js:420-430): first order, or with the wind
+ * at the trajectory's midpoint when cn.w is set (hurricane mode) */
+static R3 trace(const Ctx* c, const R* V, R3 p, R3 v0, R s) {
+  R3 k = kvec(c), r;
+  if (c->U[U_CN + 3] < 0.5f) {
+    r.x = p.x - s * k.x * v0.x; r.y = p.y - s * k.y * v0.y; r.z = p.z - s * k.z * v0.z;
+    return clampX(c, r);
+  }
+  R3 m = {p.x - (R)0.5 * s * k.x * v0.x, p.y - (R)0.5 * s * k.y * v0.y, p.z - (R)0.5 * s * k.z * v0.z};
+  R3 vm = velAt(c, V, clampX(c, m));
+  r.x = p.x - s * k.x * vm.x; r.y = p.y - s * k.y * vm.y; r.z = p.z - s * k.z * vm.z;
+  return clampX(c, r);
+}
 
-void S_(orc_advv1)(const float* U, const float* BS, const R* V, R* Vt) {
+/* V carries the trajectories; Vval the values carried (hurricane mode: Vin less 40 % of the last pressure force,
+ * WGSL_M_PREP; otherwise V itself) */
+void S_(orc_advv1_2)(const float* U, const float* BS, const R* V, const R* Vval, R* Vt) {
   Ctx C = mkctx(U, BS), *c = &C;
 #pragma omp parallel for schedule(static)
   for (int z = 0; z <= c->nz; z++)
@@ -549,13 +639,14 @@ void S_(orc_advv1)(const float* U, const float* BS, const R* V, R* Vt) {
         R3 pu = {(R)x, (R)y + (R)0.5, (R)z + (R)0.5}, pv = {(R)x + (R)0.5, (R)y, (R)z + (R)0.5};
         R3 pw = {(R)x + (R)0.5, (R)y + (R)0.5, (R)z}, pc = {(R)x + (R)0.5, (R)y + (R)0.5, (R)z + (R)0.5};
         R4 o;
-        o.x = sampU(c, V, trace(c, pu, L.vu, 1));
-        o.y = sampV(c, V, trace(c, pv, L.vv, 1));
-        o.z = sampW(c, V, trace(c, pw, L.vw, 1));
-        o.w = sampC(c, V, trace(c, pc, L.vc, 1));
+        o.x = sampU(c, Vval, trace(c, V, pu, L.vu, 1));
+        o.y = sampV(c, Vval, trace(c, V, pv, L.vv, 1));
+        o.z = sampW(c, Vval, trace(c, V, pw, L.vw, 1));
+        o.w = sampC(c, Vval, trace(c, V, pc, L.vc, 1));
         put4(Vt, vidx(c, x, y, z), o);
       }
 }
+void S_(orc_advv1)(const float* U, const float* BS, const R* V, R* Vt) { S_(orc_advv1_2)(U, BS, V, V, Vt); }
 static void mmV(const Ctx* c, const R* T, R3 p, R ox, R oy, R oz, int ch, R* lo, R* hi) {
   int i0 = (int)FLOOR(p.x - ox), j0 = (int)FLOOR(p.y - oy), k0 = (int)FLOOR(p.z - oz);
   R a = (R)1e30, b = (R)-1e30;
@@ -567,7 +658,7 @@ static void mmV(const Ctx* c, const R* T, R3 p, R ox, R oy, R oz, int ch, R* lo,
   *lo = a;
   *hi = b;
 }
-void S_(orc_advv2)(const float* U, const float* BS, const R* V, const R* Vt, R* Vout) {
+void S_(orc_advv2_2)(const float* U, const float* BS, const R* V, const R* Vval, const R* Vt, R* Vout) {
   Ctx C = mkctx(U, BS), *c = &C;
   R3 k = kvec(c);
   R hmc = (R)0.5 * UF(c, U_MISC + 1);
@@ -579,23 +670,26 @@ void S_(orc_advv2)(const float* U, const float* BS, const R* V, const R* Vt, R* 
         R3 pu = {(R)x, (R)y + (R)0.5, (R)z + (R)0.5}, pv = {(R)x + (R)0.5, (R)y, (R)z + (R)0.5};
         R3 pw = {(R)x + (R)0.5, (R)y + (R)0.5, (R)z}, pc = {(R)x + (R)0.5, (R)y + (R)0.5, (R)z + (R)0.5};
         R4 hat = get4(Vt, vidx(c, x, y, z));
-        R4 old = ldV(c, V, x, y, z);
+        R4 old = ldV(c, Vval, x, y, z);
         R4 b;
-        b.x = sampU(c, Vt, trace(c, pu, L.vu, -1));
-        b.y = sampV(c, Vt, trace(c, pv, L.vv, -1));
-        b.z = sampW(c, Vt, trace(c, pw, L.vw, -1));
-        b.w = sampC(c, Vt, trace(c, pc, L.vc, -1));
+        b.x = sampU(c, Vt, trace(c, V, pu, L.vu, -1));
+        b.y = sampV(c, Vt, trace(c, V, pv, L.vv, -1));
+        b.z = sampW(c, Vt, trace(c, V, pw, L.vw, -1));
+        b.w = sampC(c, Vt, trace(c, V, pc, L.vc, -1));
         R cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
         R cour = SQRT(cx * cx + cy * cy + cz * cz);
         R sm = hmc * clampr(cour / (R)0.35, UF(c, U_DZV + 3), 1);
         R4 r = {hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z), hat.w + sm * (old.w - b.w)};
         R lo, hi;
-        mmV(c, V, trace(c, pu, L.vu, 1), 0, (R)0.5, (R)0.5, 0, &lo, &hi); r.x = clampr(r.x, lo, hi);
-        mmV(c, V, trace(c, pv, L.vv, 1), (R)0.5, 0, (R)0.5, 1, &lo, &hi); r.y = clampr(r.y, lo, hi);
-        mmV(c, V, trace(c, pw, L.vw, 1), (R)0.5, (R)0.5, 0, 2, &lo, &hi); r.z = clampr(r.z, lo, hi);
-        mmV(c, V, trace(c, pc, L.vc, 1), (R)0.5, (R)0.5, (R)0.5, 3, &lo, &hi); r.w = clampr(r.w, lo, hi);
+        mmV(c, Vval, trace(c, V, pu, L.vu, 1), 0, (R)0.5, (R)0.5, 0, &lo, &hi); r.x = clampr(r.x, lo, hi);
+        mmV(c, Vval, trace(c, V, pv, L.vv, 1), (R)0.5, 0, (R)0.5, 1, &lo, &hi); r.y = clampr(r.y, lo, hi);
+        mmV(c, Vval, trace(c, V, pw, L.vw, 1), (R)0.5, (R)0.5, 0, 2, &lo, &hi); r.z = clampr(r.z, lo, hi);
+        mmV(c, Vval, trace(c, V, pc, L.vc, 1), (R)0.5, (R)0.5, (R)0.5, 3, &lo, &hi); r.w = clampr(r.w, lo, hi);
         put4(Vout, vidx(c, x, y, z), r);
       }
+}
+void S_(orc_advv2)(const float* U, const float* BS, const R* V, const R* Vt, R* Vout) {
+  S_(orc_advv2_2)(U, BS, V, V, Vt, Vout);
 }
 
 /* ====================== advection of moisture (model.js:482-576) ====================== */
@@ -605,12 +699,18 @@ static R3 cellVel(const Ctx* c, const R* V, int x, int y, int z) {
           (R)0.5 * (p.z + ldV(c, V, x, y, z + 1).z)};
   return v;
 }
-/* first-order trace of something falling at `fall`, clamped to [1/2, N - 1/2] */
-static R3 traceQ(const Ctx* c, R3 p, R3 v0, R fall, R s) {
-  R3 k = kvec(c), r;
-  r.x = clampr(p.x - s * k.x * v0.x, (R)0.5, (R)c->nx - (R)0.5);
-  r.y = clampr(p.y - s * k.y * v0.y, (R)0.5, (R)c->ny - (R)0.5);
-  r.z = clampr(p.z - s * k.z * (v0.z - fall), (R)0.5, (R)c->nz - (R)0.5);
+/* trace of something falling at `fall`, clamped to [1/2, N - 1/2] (model.js:494-499): first order, or (cn.w set,
+ * hurricane mode) with the wind sampled at the midpoint, clamped to [0, N] */
+static R3 traceQ(const Ctx* c, const R* V, R3 p, R3 v0, R fall, R s) {
+  R3 k = kvec(c), r, vm = {v0.x, v0.y, v0.z - fall};
+  if (c->U[U_CN + 3] > 0.5f) {
+    R3 m = {p.x - (R)0.5 * s * k.x * vm.x, p.y - (R)0.5 * s * k.y * vm.y, p.z - (R)0.5 * s * k.z * vm.z};
+    R3 w = velAt(c, V, clampX(c, m));
+    vm.x = w.x; vm.y = w.y; vm.z = w.z - fall;
+  }
+  r.x = clampr(p.x - s * k.x * vm.x, (R)0.5, (R)c->nx - (R)0.5);
+  r.y = clampr(p.y - s * k.y * vm.y, (R)0.5, (R)c->ny - (R)0.5);
+  r.z = clampr(p.z - s * k.z * vm.z, (R)0.5, (R)c->nz - (R)0.5);
   return r;
 }
 /* texS(x) = x / N: texel coordinate x - 1/2 */
@@ -641,14 +741,15 @@ void S_(orc_advq1)(const float* U, const float* BS, const R* V, const R* Q1, con
         R4 q1 = get4(Q1, i), q2 = get4(Q2, i);
         R rho = bsA(c, z).z;
         R2 fv = vtRain(c, q1.w, q2.z, rho);
-        R fs = vtS(c, q2.x, rho), fg = vtGrau(c, q2.y, rho);
-        R4 a = sampQ(c, Q1, traceQ(c, p, v, 0, 1));
-        R r = sampQ(c, Q1, traceQ(c, p, v, fv.x, 1)).w;
-        R4 b = sampQ(c, Q2, traceQ(c, p, v, 0, 1));
-        R s = sampQ(c, Q2, traceQ(c, p, v, fs, 1)).x;
-        R g = sampQ(c, Q2, traceQ(c, p, v, fg, 1)).y;
+        R fs = vtS(c, q2.x, rho), fg = vtGrau(c, q2.y, rho), fi = vtI(c, q1.z, rho);
+        R4 a = sampQ(c, Q1, traceQ(c, V, p, v, 0, 1));
+        if (fi > 0) a.z = sampQ(c, Q1, traceQ(c, V, p, v, fi, 1)).z;
+        R r = sampQ(c, Q1, traceQ(c, V, p, v, fv.x, 1)).w;
+        R4 b = sampQ(c, Q2, traceQ(c, V, p, v, 0, 1));
+        R s = sampQ(c, Q2, traceQ(c, V, p, v, fs, 1)).x;
+        R g = sampQ(c, Q2, traceQ(c, V, p, v, fg, 1)).y;
         R nr = b.z;
-        if (thm) nr = sampQ(c, Q2, traceQ(c, p, v, fv.y, 1)).z;
+        if (thm) nr = sampQ(c, Q2, traceQ(c, V, p, v, fv.y, 1)).z;
         R4 o1 = {a.x, a.y, a.z, r}, o2 = {s, g, nr, b.w};
         put4(Q1t, i, o1);
         put4(Q2t, i, o2);
@@ -670,33 +771,34 @@ void S_(orc_advq2)(const float* U, const float* BS, const R* V, const R* Q1, con
         R4 q1 = get4(Q1, i), q2 = get4(Q2, i);
         R rho = bsA(c, z).z;
         R2 fv = vtRain(c, q1.w, q2.z, rho);
-        R fr = fv.x, fs = vtS(c, q2.x, rho), fg = vtGrau(c, q2.y, rho);
+        R fr = fv.x, fs = vtS(c, q2.x, rho), fg = vtGrau(c, q2.y, rho), fi = vtI(c, q1.z, rho);
         R4 h1 = get4(Q1t, i), h2 = get4(Q2t, i);
         R cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
         R cour = SQRT(cx * cx + cy * cy + cz * cz);
         R sm = hmc * clampr(cour / (R)0.35, UF(c, U_DZV + 3), 1);
-        R4 b1 = sampQ(c, Q1t, traceQ(c, p, v, 0, -1));
-        R br = sampQ(c, Q1t, traceQ(c, p, v, fr, -1)).w;
-        R4 b2 = sampQ(c, Q2t, traceQ(c, p, v, 0, -1));
-        R bs = sampQ(c, Q2t, traceQ(c, p, v, fs, -1)).x;
-        R bg = sampQ(c, Q2t, traceQ(c, p, v, fg, -1)).y;
+        R4 b1 = sampQ(c, Q1t, traceQ(c, V, p, v, 0, -1));
+        R br = sampQ(c, Q1t, traceQ(c, V, p, v, fr, -1)).w;
+        R4 b2 = sampQ(c, Q2t, traceQ(c, V, p, v, 0, -1));
+        R bs = sampQ(c, Q2t, traceQ(c, V, p, v, fs, -1)).x;
+        R bg = sampQ(c, Q2t, traceQ(c, V, p, v, fg, -1)).y;
         R fn = thm ? fv.y : (R)0;
-        R bn = sampQ(c, Q2t, traceQ(c, p, v, fn, -1)).z;
-        R bi = b1.z; /* vtI = 0 outside hurricane mode */
+        R bn = sampQ(c, Q2t, traceQ(c, V, p, v, fn, -1)).z;
+        R bi = fi > 0 ? sampQ(c, Q1t, traceQ(c, V, p, v, fi, -1)).z : b1.z;
         R4 r1 = {h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - bi), h1.w + sm * (q1.w - br)};
         R4 r2 = {h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn), h2.w + sm * (q2.w - b2.w)};
         R4 l1, u1, lr, ur, l2, u2, ls, us, lg, ug, ln, un;
-        mmQ(c, Q1, traceQ(c, p, v, 0, 1), &l1, &u1);
-        mmQ(c, Q1, traceQ(c, p, v, fr, 1), &lr, &ur);
-        mmQ(c, Q2, traceQ(c, p, v, 0, 1), &l2, &u2);
-        mmQ(c, Q2, traceQ(c, p, v, fs, 1), &ls, &us);
-        mmQ(c, Q2, traceQ(c, p, v, fg, 1), &lg, &ug);
-        /* mi = mm1 of the fi = 0 trace, i.e. the same as m1 */
+        mmQ(c, Q1, traceQ(c, V, p, v, 0, 1), &l1, &u1);
+        mmQ(c, Q1, traceQ(c, V, p, v, fr, 1), &lr, &ur);
+        mmQ(c, Q2, traceQ(c, V, p, v, 0, 1), &l2, &u2);
+        mmQ(c, Q2, traceQ(c, V, p, v, fs, 1), &ls, &us);
+        mmQ(c, Q2, traceQ(c, V, p, v, fg, 1), &lg, &ug);
+        R4 li, ui;
+        mmQ(c, Q1, traceQ(c, V, p, v, fi, 1), &li, &ui);
         r1.x = clampr(r1.x, l1.x, u1.x);
         r1.y = clampr(r1.y, l1.y, u1.y);
-        r1.z = clampr(r1.z, l1.z, u1.z);
+        r1.z = clampr(r1.z, li.z, ui.z);
         r1.w = clampr(r1.w, lr.w, ur.w);
-        mmQ(c, Q2, traceQ(c, p, v, fn, 1), &ln, &un);
+        mmQ(c, Q2, traceQ(c, V, p, v, fn, 1), &ln, &un);
         r2.x = clampr(r2.x, ls.x, us.x);
         r2.y = clampr(r2.y, lg.y, ug.y);
         r2.z = clampr(r2.z, ln.z, un.z);
@@ -900,9 +1002,14 @@ void S_(orc_micro_point)(const float* U, const float* BS, int k, R* st) {
   st[0] = m.T / pi0; st[1] = m.qv; st[2] = m.qc; st[3] = m.qi; st[4] = m.qr; st[5] = m.qs; st[6] = m.qg; st[7] = m.nr;
 }
 
-void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, const R* Q2in, const R* aux,
-                  const R* lsfA, const R* lsfB, const R* phT, const R* ciMap, R* Vout, R* Q1o, R* Q2o) {
+static size_t pidx(const Ctx* c, int x, int y, int z);
+/* sfc: the sea surface map (NULL outside hurricane mode); phi: the last solve's padded phi (read by the sea
+ * fluxes' surface pressure in hurricane mode only) */
+void S_(orc_phys2)(const float* U, const float* BS, const R* Vin, const R* Q1in, const R* Q2in, const R* aux,
+                   const R* lsfA, const R* lsfB, const R* phT, const R* ciMap, const float* sfc, const R* phi, R* Vout,
+                   R* Q1o, R* Q2o) {
   Ctx C = mkctx(U, BS), *c = &C;
+  int hur = hurOn(c);
   int nx = c->nx, ny = c->ny, nz = c->nz;
   R dx = c->dx, dz = c->dz, dt = c->dt;
   R ix2 = (R)1 / (dx * dx), iz2 = (R)1 / (dz * dz);
@@ -929,7 +1036,8 @@ void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, 
           if (x > 0 && x < nx) du -= (ldP(c, phT, x, y, z) - ldP(c, phT, x - 1, y, z)) / dx;
           if (z == 0 && c->U[U_TURB + 3] > 0.5f) {
             R lg = SQRT(V.x * V.x + vbar * vbar), lg0 = SQRT(ut * ut + vtb * vtb);
-            R cd = UF(c, U_MICRO + 3);
+            R cd = hur ? cdSfc(c, lg, (R)0.5 * (sfcAt(c, sfc, x - 1 > 0 ? x - 1 : 0, y, 0) + sfcAt(c, sfc, x, y, 0)))
+                       : UF(c, U_MICRO + 3);
             du += cd * (-lg * V.x + lg0 * ut) / dz;
           }
           o.x = V.x + dt * du;
@@ -951,7 +1059,8 @@ void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, 
           if (y > 0 && y < ny) dv -= (ldP(c, phT, x, y, z) - ldP(c, phT, x, y - 1, z)) / dx;
           if (z == 0 && c->U[U_TURB + 3] > 0.5f) {
             R lg = SQRT(ubar * ubar + V.y * V.y), lg0 = SQRT(utb * utb + vt * vt);
-            R cd = UF(c, U_MICRO + 3);
+            R cd = hur ? cdSfc(c, lg, (R)0.5 * (sfcAt(c, sfc, x, y - 1 > 0 ? y - 1 : 0, 0) + sfcAt(c, sfc, x, y, 0)))
+                       : UF(c, U_MICRO + 3);
             dv += cd * (-lg * V.y + lg0 * vt) / dz;
           }
           o.y = V.y + dt * dv;
@@ -968,7 +1077,7 @@ void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, 
                 (ldP(c, phT, x, y, z) - ldP(c, phT, x, y, z - 1)) / dz;
           dw += (spongeLat(c, xmx, xmy) + spongeTop(c, zz)) * ((R)0 - V.z);
           R4 ci = get4(ciMap, (size_t)x + (size_t)nx * y);
-          if (ci.x > 0) {
+          if (ci.x > 0 && !hur) {
             R zr = FABS(zz - ci.z) / ci.w;
             if (zr < 1) {
               R cv = COS((R)1.5707963 * zr);
@@ -1006,13 +1115,63 @@ void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, 
           q1.y = q1.y + dt * (nux * l1.y * ix2 + nuy * l1z.y * iz2);
           q1.z = q1.z + dt * (nux * l1.z * ix2 + nuy * l1z.z * iz2);
           q1.x += dt * relax * (tqv - q1.x);
+          if (hur) {
+            /* hurricane mode (model.js:713-774) */
+            th += dt * clampr((tth - V.w) / UF(c, U_HUR2), -UF(c, U_HUR2 + 1), UF(c, U_HUR2 + 1));
+            R4 cs = get4(ciMap, (size_t)x + (size_t)nx * y);
+            if (cs.x > 0) {
+              R zr = FABS(zz - cs.z) / cs.w;
+              if (zr < 1) {
+                R f = cs.x * COS((R)1.5707963 * zr) * cs.y / (R)8;
+                th += dt * (R)1.2e-3 * f;
+                q1.x += dt * f / (R)600 * maxr((R)0.97 * qvsat(V.w * B.x, B.y, 0) - q1.x, 0);
+              }
+            }
+            R ex = xmx - UF(c, U_HUR4), ey = xmy - UF(c, U_HUR4 + 1);
+            R rS = SQRT(ex * ex + ey * ey);
+            R wEnv = sstep(UF(c, U_HUR4 + 3), (R)1.5 * UF(c, U_HUR4 + 3), rS) / (R)3600;
+            th += dt * wEnv * (tth - V.w);
+            q1.x += dt * wEnv * (tqv - q1.x);
+            if (zz < UF(c, U_HUR2 + 2)) {
+              R4 s0 = ldV(c, Vin, x, y, 0);
+              R ua = (R)0.5 * (s0.x + ldV(c, Vin, x + 1, y, 0).x), va = (R)0.5 * (s0.y + ldV(c, Vin, x, y + 1, 0).y);
+              R u1 = SQRT(ua * ua + va * va);
+              R e = UF(c, U_STEP + 1) / (R)900, e0 = FLOOR(e), qx = xmx / (R)6000, qy = xmy / (R)6000;
+              R ga = gnoise2(qx + (R)37.1 * e0, qy + (R)11.3 * e0);
+              R gb = gnoise2(qx + (R)37.1 * (e0 + (R)1), qy + (R)11.3 * (e0 + (R)1));
+              R rr = (R)0.5 * mixr(ga, gb, sstep(0, 1, fractr(e)));
+              R fu = minr(u1 * u1 / (R)225, 1) * ((R)1 - sstep((R)20, (R)32, u1)) *
+                     ((R)1 - sstep(UF(c, U_HUR4 + 2), (R)1.5 * UF(c, U_HUR4 + 2), rS));
+              th += dt * UF(c, U_HUR3 + 2) * (R)2 * rr * fu * ((R)1 - zz / UF(c, U_HUR2 + 2));
+            }
+            if (z == 0) {
+              R land = clampr(sfcAt(c, sfc, x, y, 0), 0, 1), sst = sfcAt(c, sfc, x, y, 1);
+              R ua = (R)0.5 * (V.x + ldV(c, Vin, x + 1, y, z).x), va = (R)0.5 * (V.y + ldV(c, Vin, x, y + 1, z).y);
+              R u1 = SQRT(ua * ua + va * va);
+              R ch = UF(c, U_HUR + 1) * UF(c, U_HUR2 + 3) * maxr(u1, UF(c, U_HUR3 + 1)) / dz;
+              R ps = maxr(UF(c, U_HUR + 3) + A.z * UF(c, U_CN + 1) * phi[pidx(c, x, y, z)], (R)50000);
+              R ths = sst * POW((R)1.0e5 / ps, (R)0.2854);
+              th += dt * ch * ((R)1 - land) * (ths - V.w);
+              if (c->U[U_HUR5 + 1] > 0.5f) {
+                R cdh = cdSfc(c, u1, land);
+                th += dt * cdh * u1 * u1 * u1 / (CPD * dz) / B.x;
+              }
+              R qsS = qvsat(sst, ps, 0), qsL = qvsat(V.w * B.x, ps, 0);
+              q1.x += dt * ch * (((R)1 - land) * (qsS - q1.x) + land * UF(c, U_HUR + 2) * maxr(qsL - q1.x, 0));
+            }
+          }
           q1.x = maxr(q1.x, 0); q1.y = maxr(q1.y, 0); q1.z = maxr(q1.z, 0); q1.w = maxr(q1.w, 0);
           q2.x = maxr(q2.x, 0); q2.y = maxr(q2.y, 0); q2.z = maxr(q2.z, 0); q2.w = maxr(q2.w, 0);
           R pi0 = B.x;
           MState m = {th * pi0, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, maxr(q2.z, 0)};
           micro_block(c, z, &m);
           th = m.T / pi0;
-          R keep = (R)1 / ((R)1 + (R)2 * dt * spongeLat(c, xmx, xmy) + dt * (R)0);
+          R envDecay = 0;
+          if (hur) {
+            R ex = xmx - UF(c, U_HUR4), ey = xmy - UF(c, U_HUR4 + 1);
+            envDecay = sstep(UF(c, U_HUR4 + 3), (R)1.5 * UF(c, U_HUR4 + 3), SQRT(ex * ex + ey * ey)) / (R)1800;
+          }
+          R keep = (R)1 / ((R)1 + (R)2 * dt * spongeLat(c, xmx, xmy) + dt * envDecay);
           m.qc *= keep; m.qi *= keep; m.qr *= keep; m.qs *= keep; m.qg *= keep; m.nr *= keep;
           size_t ii = cidx(c, x, y, z);
           R4 o1 = {maxr(m.qv, 0), maxr(m.qc, 0), maxr(m.qi, 0), maxr(m.qr, 0)};
@@ -1033,6 +1192,11 @@ void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, 
         }
         put4(Vout, vidx(c, x, y, z), o);
       }
+}
+
+void S_(orc_phys)(const float* U, const float* BS, const R* Vin, const R* Q1in, const R* Q2in, const R* aux,
+                  const R* lsfA, const R* lsfB, const R* phT, const R* ciMap, R* Vout, R* Q1o, R* Q2o) {
+  S_(orc_phys2)(U, BS, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT, ciMap, NULL, NULL, Vout, Q1o, Q2o);
 }
 
 /* ====================== DIV (model.js:910-925) ====================== */
@@ -1079,7 +1243,7 @@ void S_(orc_proj)(const float* U, const float* BS, const R* V, const R* phi, R* 
 }
 
 /* ====================== 2-D maps (model.js:1028-1100) ======================
- * reflectivity helpers zRain / zGrau (model.js:94-106), outbreak mode (no maritime graupel) */
+ * reflectivity helpers zRain / zGrau (model.js:94-106) */
 static R zRain(const Ctx* c, R qr, R nr, R rho) {
   if (!thompson(c)) return (R)3.63e9 * POW(maxr(rho * qr, 0), (R)1.75);
   if (qr < (R)1e-9) return 0;
@@ -1088,6 +1252,7 @@ static R zRain(const Ctx* c, R qr, R nr, R rho) {
   return (R)720e18 * rho * nb / (l * l * l * l * l * l);
 }
 static R zGrau(const Ctx* c, R qg, R rho, R T) {
+  if (maritime(c)) return (T < T0K ? (R)1.3e9 : (R)1.0e10) * POW(maxr(rho * qg, 0), (R)1.75);
   if (!thompson(c)) return (T < T0K ? (R)1.2e10 : (R)4.33e10) * POW(maxr(rho * qg, 0), (R)1.75);
   if (qg < (R)1e-9) return 0;
   R l = lamG(qg, rho), l2 = l * l;
@@ -1110,8 +1275,10 @@ static R map_zeta(const Ctx* c, const R* V, int i, int j, int k) {
   return (map_cvel(c, V, i + 1, j, k).y - map_cvel(c, V, i - 1, j, k).y - map_cvel(c, V, i, j + 1, k).x +
           map_cvel(c, V, i, j - 1, k).x) / ((R)2 * c->dx);
 }
-/* maps: 4 planar maps of nx*ny x 4 (m0..m3 as the reference's map buffer); phi padded as the solve's */
-void S_(orc_maps)(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, const R* phi, R* maps) {
+/* maps: 4 planar maps of nx*ny x 4 (m0..m3 as the reference's map buffer); phi padded as the solve's; sfc the
+ * sea surface map or NULL */
+void S_(orc_maps2)(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, const R* phi, const float* sfc,
+                   R* maps) {
   Ctx C = mkctx(U, BS), *c = &C;
   const size_t n2 = (size_t)c->nx * c->ny;
   const R dz = c->dz;
@@ -1147,7 +1314,8 @@ void S_(orc_maps)(const float* U, const float* BS, const R* V, const R* Q1, cons
         tr *= (R)1 - a;
         if (tr < (R)1e-3) break;
       }
-      bt += tr * (s0.w * bsB(c, 0).x);  /* no sea surface in outbreak mode */
+      R sst = sfcAt(c, sfc, i, j, 1);
+      bt += tr * (sst > (R)200 ? sst : s0.w * bsB(c, 0).x);
       int p = (int)c->U[U_PAD], nxp = (int)c->U[U_PAD + 1];
       R ps = bsA(c, 0).z * UF(c, U_CN + 2) * phi[(size_t)(i + p) + (size_t)nxp * (size_t)(j + p)] / (R)100;
       int k5 = clampi((int)((R)5000 / dz), 1, c->nz - 2);
@@ -1157,4 +1325,61 @@ void S_(orc_maps)(const float* U, const float* BS, const R* V, const R* Q1, cons
       put4(maps, idx + 2 * n2, m2);
       put4(maps, idx + 3 * n2, m3);
     }
+}
+void S_(orc_maps)(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, const R* phi, R* maps) {
+  S_(orc_maps2)(U, BS, V, Q1, Q2, phi, NULL, maps);
+}
+
+/* ====================== hurricane mode: PREP (model.js:996-1026) ======================
+ * the winds the advection carries: V less cn.x of the last step's pressure force (phi scaled by cn.y) */
+void S_(orc_prep)(const float* U, const float* BS, const R* V, const R* phi, R* Vp) {
+  Ctx C = mkctx(U, BS), *c = &C;
+  R f = UF(c, U_CN) * UF(c, U_CN + 1);
+  R kx = f * c->dt / c->dx, kz = f * c->dt / c->dz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z <= c->nz; z++)
+    for (int y = 0; y <= c->ny; y++)
+      for (int x = 0; x <= c->nx; x++) {
+        R4 Vv = ldV(c, V, x, y, z), o = Vv;
+        R pc = Pphi(c, phi, x, y, z);
+        if (y < c->ny && z < c->nz) o.x = Vv.x - kx * (pc - Pphi(c, phi, x - 1, y, z));
+        if (x < c->nx && z < c->nz) o.y = Vv.y - kx * (pc - Pphi(c, phi, x, y - 1, z));
+        if (x < c->nx && y < c->ny && z > 0 && z < c->nz) o.z = Vv.z - kz * (pc - Pphi(c, phi, x, y, z - 1));
+        put4(Vp, vidx(c, x, y, z), o);
+      }
+}
+
+/* ====================== hurricane mode: the 2-dx filter FILT (model.js:948-994) ====================== */
+static R4 bih(R4 c0, R4 xp, R4 xm, R4 yp, R4 ym, R4 pp, R4 pm, R4 mp, R4 mm, R4 x2p, R4 x2m, R4 y2p, R4 y2m) {
+  R4 r;
+#define BIH(ch) ((R)20 * c0.ch - (R)8 * (xp.ch + xm.ch + yp.ch + ym.ch) + (R)2 * (pp.ch + pm.ch + mp.ch + mm.ch) + x2p.ch + \
+                 x2m.ch + y2p.ch + y2m.ch)
+  r.x = BIH(x); r.y = BIH(y); r.z = BIH(z); r.w = BIH(w);
+#undef BIH
+  return r;
+}
+void S_(orc_filt)(const float* U, const float* BS, const R* V, const R* Q1, const R* Q2, R* Vo, R* Q1o, R* Q2o) {
+  Ctx C = mkctx(U, BS), *c = &C;
+  R a = UF(c, U_HUR5 + 2);
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z <= c->nz; z++)
+    for (int y = 0; y <= c->ny; y++)
+      for (int x = 0; x <= c->nx; x++) {
+#define NB(L, F) L(c, F, x, y, z), L(c, F, x + 1, y, z), L(c, F, x - 1, y, z), L(c, F, x, y + 1, z), L(c, F, x, y - 1, z), \
+                 L(c, F, x + 1, y + 1, z), L(c, F, x + 1, y - 1, z), L(c, F, x - 1, y + 1, z), L(c, F, x - 1, y - 1, z),      \
+                 L(c, F, x + 2, y, z), L(c, F, x - 2, y, z), L(c, F, x, y + 2, z), L(c, F, x, y - 2, z)
+        R4 v = ldV(c, V, x, y, z), b = bih(NB(ldV, V));
+        R4 o = {v.x - a * b.x, v.y - a * b.y, v.z - a * b.z, v.w - a * b.w};
+        if (z == 0 || z >= c->nz) o.z = 0;
+        put4(Vo, vidx(c, x, y, z), o);
+        if (x < c->nx && y < c->ny && z < c->nz) {
+          R4 q = ldQ(c, Q1, x, y, z), bq = bih(NB(ldQ, Q1));
+          R4 o1 = {maxr(q.x - a * bq.x, 0), maxr(q.y - a * bq.y, 0), maxr(q.z - a * bq.z, 0), maxr(q.w - a * bq.w, 0)};
+          R4 r = ldQ(c, Q2, x, y, z), br = bih(NB(ldQ, Q2));
+          R4 o2 = {maxr(r.x - a * br.x, 0), maxr(r.y - a * br.y, 0), maxr(r.z - a * br.z, 0), maxr(r.w - a * br.w, 0)};
+          put4(Q1o, cidx(c, x, y, z), o1);
+          put4(Q2o, cidx(c, x, y, z), o2);
+        }
+#undef NB
+      }
 }

@@ -41,6 +41,8 @@ _SIGS = {
     "obl_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_set_ci": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "obl_set_prep_field": (C.c_int, [C.c_void_p, C.c_int]),
+    "obl_set_vortex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
     "obl_maps_request": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
@@ -88,9 +90,10 @@ _SIGS = {
 }
 
 FIELD = {"V": 0, "Q1": 1, "Q2": 2, "Vt": 3, "Q1t": 4, "Q2t": 5, "aux": 6, "phT": 7, "lsfA": 8, "lsfB": 9,
-         "ciMap": 10, "phi": 11, "rhs": 12, "bs": 13}
+         "ciMap": 10, "phi": 11, "rhs": 12, "bs": 13, "sfc": 14, "Vp": 15}
 STEP_LSF = 1
 STEP_JUDGE = 2
+STEP_FILT = 4
 GEMM = {"fp32": 0, "bf16x3": 1}
 THOMAS = {"fp32": 0, "fp64": 1, "mixed": 2}
 

@@ -128,3 +128,38 @@ def solve_rhs_1024(nx=1024, ny=1024, nz=80, seed=1, rms=1e-5):
     f -= f.mean(axis=(1, 2), keepdims=True)
     f *= rms / np.sqrt(np.mean(f * f))
     return f
+
+
+# the hurricane-mode settings app.js passes to setSurface (app.js:1631), for a storm at the domain centre
+HURRICANE_SETTINGS = {"ck": 1.3e-3, "cdCap": 2.0e-3, "dissHeat": True, "thermalFade": True, "landMoist": 0.3,
+                      "tauRad": 43200, "radCap": 2 / 86400, "blDepth": 1000, "u10f": 0.8, "gust": 1.5,
+                      "thermals": 0.4 / 300, "maritime": True, "thermalR": 200000}
+# hurricane-mode parameter overrides of startRun (app.js:1613)
+HURRICANE_OVERRIDES = {"qc0": 1e-3, "dampZ": 16500, "dampTau": 300, "mcFloor": 1.0, "evap": 0.10}
+
+
+def setup_hurricane_128(model, seed: int = 0, scheme: str = "lin", land_from_km: float = 200.0):
+    """128 x 128 x 40, 2 km, hurricane mode: a 302 K sea (land east of ``land_from_km``), the app's hurricane
+    settings (environment hold from 100 km), a 30 m/s warm-core vortex (rmw 30 km, r0 120 km, moist core
+    60 km) at the centre, one storm-start tower, lat 20, dt 6 s."""
+    model.allocate("custom", {"key": "custom:2000:128x128x40", "dx": 2000, "n": [128, 128, 40]})
+    model.params.update(HURRICANE_OVERRIDES)
+    model.params["scheme"] = scheme
+    model.params["noise"] = 0.1
+    model.params["lat"] = 20.0
+    model.noise_seed = seed
+    model.setSounding(wk_sounding_text(th0=300.0, thtr=343.0, qv0=0.016, shear6=5.0, hodo_r=2.0))
+    model.setBoundaries([])
+    c = [0.5 * model.Lx, 0.5 * model.Ly]
+    model.setCI([{"x": c[0] + 40000, "y": c[1], "zc": 2000, "rh": 10000, "rz": 1500, "wt": 8, "t0": 0, "t1": 1800,
+                  "ramp": 60}])
+    surf = np.zeros((model.ny, model.nx, 4), np.float32)
+    surf[..., 1] = 302.0
+    xs = (np.arange(model.nx) + 0.5) * model.dx
+    surf[:, xs > land_from_km * 1000, 0] = 1.0
+    hur = dict(HURRICANE_SETTINGS, sfcP=101000.0, centre=c, envR=100000)
+    model.setSurface(surf, hur)
+    model.setVortex({"x": c[0], "y": c[1], "vmax": 30.0, "rmw": 30000.0, "r0": 120000.0, "ztop": 16000.0,
+                     "moistR": 60000.0, "moistRH": 0.9})
+    model.reset()
+    return 6.0

@@ -35,16 +35,17 @@ struct MapCtx {
 
 __global__ void __launch_bounds__(256) k_maps(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
                                               const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                              const float* __restrict__ phi, float4* __restrict__ out) {
+                                              const float* __restrict__ phi, const float4* __restrict__ sfc,
+                                              float4* __restrict__ out) {
   Dims d = dims(u);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= d.nx || j >= d.ny) return;
-  const bool thm = u.v[U_MICRO2].z > 0.5f;
+  const bool thm = u.v[U_MICRO2].z > 0.5f, mar = maritime_of(u);
   const MapCtx m{V, d.nx, d.ny, d.nz};
   const float dz = d.dz, dx = d.dx;
   auto dbz = [&](int k) {
     const uint32_t c = cidx(d, i, j, k);
-    return dbz_of(thm, ldx(Q1, c), ldx(Q2, c), bsA(BS, k, d.nz).z, m.ldV(i, j, k).w * bsB(BS, k, d.nz).x);
+    return dbz_of(thm, ldx(Q1, c), ldx(Q2, c), bsA(BS, k, d.nz).z, m.ldV(i, j, k).w * bsB(BS, k, d.nz).x, mar);
   };
   float comp = -30.0f, wmx = 0.0f, uh = 0.0f;
   const int k12 = min((int)(12000.0f / dz), d.nz - 1);
@@ -66,8 +67,8 @@ __global__ void __launch_bounds__(256) k_maps(MU u, const float4* __restrict__ B
   const float3 v1 = m.cvel(i, j, k1);
   stx(out, idx + 2 * n2, make_float4(v0.x, v0.y, v1.x, v1.y));
   // brightness temperature: each level emits at its temperature in proportion to the condensate's
-  // infrared absorption, seen through the cloud above it; the rest comes from the lowest level
-  // (outbreak mode: no sea surface)
+  // infrared absorption, seen through the cloud above it; the rest comes from the sea surface where there
+  // is one (hurricane mode), else from the lowest level
   float tr = 1.0f, bt = 0.0f;
   for (int k = d.nz - 1; k >= 0; k--) {
     const uint32_t c = cidx(d, i, j, k);
@@ -78,7 +79,8 @@ __global__ void __launch_bounds__(256) k_maps(MU u, const float4* __restrict__ B
     tr *= 1.0f - a;
     if (tr < 1e-3f) break;
   }
-  bt += tr * (s0.w * bsB(BS, 0, d.nz).x);
+  const float sst = sfc ? ldx(sfc, idx).y : 0.0f;
+  bt += tr * (sst > 200.0f ? sst : s0.w * bsB(BS, 0, d.nz).x);
   const int p = (int)u.v[U_PAD].x, nxp = (int)u.v[U_PAD].y;
   const float ps = bsA(BS, 0, d.nz).z * u.v[U_CN].z * ldx(phi, (uint32_t)(i + p) + __umul24((uint32_t)nxp, (uint32_t)(j + p))) / 100.0f;
   const int k5 = clampi((int)(5000.0f / dz), 1, d.nz - 2);
@@ -86,9 +88,9 @@ __global__ void __launch_bounds__(256) k_maps(MU u, const float4* __restrict__ B
 }
 
 void launch_maps(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 const float4* Q2, const float* phi, float4* out) {
+                 const float4* Q2, const float* phi, const float4* sfc, float4* out) {
   dim3 blk(64, 4), grid((d.nx + 63) / 64, (d.ny + 3) / 4);
-  hipLaunchKernelGGL(k_maps, grid, blk, 0, s, u, BS, V, Q1, Q2, phi, out);
+  hipLaunchKernelGGL(k_maps, grid, blk, 0, s, u, BS, V, Q1, Q2, phi, sfc, out);
 }
 
 }  // namespace obl

@@ -188,10 +188,22 @@ __device__ __forceinline__ float tgtV(const float4* BS, const float4* lsfB, cons
   return tgtV(BS, G2d{lsfB, d}, d, x, y, z);
 }
 
-// ====================== INIT (model.js:284-339), outbreak mode ======================
+// ====================== INIT (model.js:284-339) ======================
+// the vortex table's (tangential wind, theta') at radius r, level k (vxAt, model.js:292-297)
+__device__ __forceinline__ float2 vxAt(const MU& u, const float4* VX, int nz, float r, int k) {
+  int nr = (int)u.v[U_VTX].w;
+  float fr = r / u.v[U_VTX].z;
+  int i0 = (int)floorf(fr);
+  if (i0 >= nr - 1) return make_float2(0.0f, 0.0f);
+  int o = clampi(k, 0, nz - 1) * nr + i0;
+  float t = fr - (float)i0;
+  float4 a = VX[o], b = VX[o + 1];
+  return make_float2(a.x * (1.0f - t) + b.x * t, a.y * (1.0f - t) + b.y * t);
+}
 __global__ void __launch_bounds__(256) k_init(MU u, const float4* __restrict__ BS, const float4* __restrict__ lsfA,
-                                              const float4* __restrict__ lsfB, float4* __restrict__ V,
-                                              float4* __restrict__ Q1, float4* __restrict__ Q2, int seed) {
+                                              const float4* __restrict__ lsfB, const float4* __restrict__ VX,
+                                              float4* __restrict__ V, float4* __restrict__ Q1, float4* __restrict__ Q2,
+                                              int seed) {
   Dims d = dims(u);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > d.nx || y > d.ny || z > d.nz) return;
@@ -199,13 +211,33 @@ __global__ void __launch_bounds__(256) k_init(MU u, const float4* __restrict__ B
   float4 A = ld2d(lsfA, d, x, y);
   float zz = ((float)k + 0.5f) * d.dz;
   float nz0 = sstep(800.0f, 1500.0f, zz);
+  // hurricane mode: the initial vortex (setVortex, model.js:1318-1365) and its noise reaching up through the core
+  const bool vortex = u.v[U_VTX2].z > 0.5f && VX;
+  const float xc = u.v[U_VTX].x, yc = u.v[U_VTX].y, dx = d.dx;
+  const float pcx = ((float)x + 0.5f) * dx - xc, pcy = ((float)y + 0.5f) * dx - yc;
+  if (vortex) nz0 = sqrtf(pcx * pcx + pcy * pcy) < u.v[U_VTX2].x ? sstep(6000.0f, 8000.0f, zz) : 1.0f;
   float noise = u.v[U_MISC].z * (rnd3(x + 17, y + 3 + 65536 * seed, z + 5) - 0.5f) * 2.0f * (1.0f - nz0);
   float th = tgtTh(BS, d, A, k) + noise;
   float uu = (y < d.ny && z < d.nz) ? tgtU(BS, lsfB, d, x, y, k) : 0.0f;
   float vv = (x < d.nx && z < d.nz) ? tgtV(BS, lsfB, d, x, y, k) : 0.0f;
+  if (vortex) {
+    const float pux = (float)x * dx - xc, puy = ((float)y + 0.5f) * dx - yc, ru = fmaxf(sqrtf(pux * pux + puy * puy), 1.0f);
+    const float pvx = ((float)x + 0.5f) * dx - xc, pvy = (float)y * dx - yc, rv = fmaxf(sqrtf(pvx * pvx + pvy * pvy), 1.0f);
+    uu -= vxAt(u, VX, d.nz, ru, k).x * puy / ru;
+    vv += vxAt(u, VX, d.nz, rv, k).x * pvx / rv;
+    th += vxAt(u, VX, d.nz, sqrtf(pcx * pcx + pcy * pcy), k).y;
+  }
   float4 B = bsB(BS, k, d.nz);
   float T = th * B.x;
   float qv = fminf(tgtQv(BS, d, A, k), 0.95f * qvsat(T, B.y, fIce(T)));
+  if (vortex) {  // a nearly saturated core of the same density
+    const float f = (1.0f - sstep(0.6f, 1.0f, sqrtf(pcx * pcx + pcy * pcy) / u.v[U_VTX2].x)) * sstep(500.0f, 1500.0f, zz) *
+                    (1.0f - sstep(7000.0f, 10000.0f, zz));
+    const float q2 = fmaxf(qv, f * u.v[U_VTX2].y * qvsat(T, B.y, fIce(T)));
+    const float dqv = q2 - qv;
+    qv = q2;
+    th -= 0.608f * dqv * bsA(BS, k, d.nz).x;
+  }
   V[vidx(d, x, y, z)] = make_float4(uu, vv, 0.0f, th);
   if (x < d.nx && y < d.ny && z < d.nz) {
     Q1[cidx(d, x, y, z)] = make_float4(qv, 0.0f, 0.0f, 0.0f);
@@ -631,12 +663,23 @@ __device__ __forceinline__ bool surr_slot(bool cand, unsigned* count, unsigned c
   return slot < cap;
 }
 
-template <int MODE, class AV, class AQ, class AX, class AP, class AB>
+// hurricane mode: the sea surface map's column (clamped), and the lowest-level wind speed at a cell
+__device__ __forceinline__ float4 sfcAt(const HurArgs& hu, const Dims& d, int i, int j) {
+  return ldx(hu.sfc, (uint32_t)clampi(i, 0, d.nx - 1) + __umul24((uint32_t)d.nx, (uint32_t)clampi(j, 0, d.ny - 1)));
+}
+__device__ __forceinline__ float u1At(const HurArgs& hu, const Dims& d, int x, int y) {
+  const float4 s0 = ldx(hu.V, vidx(d, x, y, 0));
+  const float ua = 0.5f * (s0.x + ldxc(hu.V, vidx(d, x + 1, y, 0), 0)), va = 0.5f * (s0.y + ldxc(hu.V, vidx(d, x, y + 1, 0), 1));
+  return sqrtf(ua * ua + va * va);
+}
+
+template <int MODE, bool HUR, class AV, class AQ, class AX, class AP, class AB>
 __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const float4* __restrict__ BS, const AV& Vin,
                                            const AQ& Q1in, const PhysCol& col, const AX& aux, const AB& lsfB,
                                            const AP& phT,
                                            float4* __restrict__ Vout, float4* __restrict__ Q1o, float4* __restrict__ Q2o,
-                                           const MicroDump& dump, const SurrPhys& surr, int x, int y, int z) {
+                                           const MicroDump& dump, const SurrPhys& surr, const HurArgs& hu, int x, int y,
+                                           int z) {
   int nx = d.nx, ny = d.ny, nz = d.nz;
   float dx = d.dx, dz = d.dz, dt = d.dt;
   float ix2 = 1.0f / (dx * dx), iz2 = 1.0f / (dz * dz);
@@ -663,7 +706,9 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     if (x > 0 && x < nx) du -= (phT(x, y, z) - phT(x - 1, y, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(V.x * V.x + vbar * vbar), lg0 = sqrtf(ut * ut + vtb * vtb);
-      du += u.v[U_MICRO].w * (-lg * V.x + lg0 * ut) / dz;
+      float cd = u.v[U_MICRO].w;
+      if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, max(x - 1, 0), y).x + sfcAt(hu, d, x, y).x));
+      du += cd * (-lg * V.x + lg0 * ut) / dz;
     }
     o.x = V.x + dt * du;
   } else {
@@ -684,7 +729,9 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     if (y > 0 && y < ny) dv -= (phT(x, y, z) - phT(x, y - 1, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(ubar * ubar + V.y * V.y), lg0 = sqrtf(utb * utb + vt * vt);
-      dv += u.v[U_MICRO].w * (-lg * V.y + lg0 * vt) / dz;
+      float cd = u.v[U_MICRO].w;
+      if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, x, max(y - 1, 0)).x + sfcAt(hu, d, x, y).x));
+      dv += cd * (-lg * V.y + lg0 * vt) / dz;
     }
     o.y = V.y + dt * dv;
   } else {
@@ -702,7 +749,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
           (phT(x, y, z) - col.pm) / dz;
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
     float4 ci = col.ci;
-    if (ci.x > 0.0f) {
+    if (ci.x > 0.0f && !HUR) {  // hurricane mode starts its storms as warm, moist bubbles instead (theta below)
       float zr = fabsf(zz - ci.z) / ci.w;
       if (zr < 1.0f) {
         float cv = cosf(1.5707963f * zr);
@@ -742,6 +789,52 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     q1.y = q1.y + dt * (nux * l1y * ix2 + nuy * lzy * iz2);
     q1.z = q1.z + dt * (nux * l1zz * ix2 + nuy * lzz * iz2);
     q1.x += dt * relax * (tqv - q1.x);
+    float envDecay = 0.0f;
+    if (HUR) {
+      // hurricane mode (model.js:713-774): radiative relaxation, storm-start bubbles, the far field held to the
+      // sounding, boundary-layer thermals under the storm, the sea's heat and moisture
+      th += dt * clampf_((tth - V.w) / u.v[U_HUR2].x, -u.v[U_HUR2].y, u.v[U_HUR2].y);
+      const float4 cs = col.ci;
+      if (cs.x > 0.0f) {
+        const float zr = fabsf(zz - cs.z) / cs.w;
+        if (zr < 1.0f) {
+          const float f = cs.x * cosf(1.5707963f * zr) * cs.y / 8.0f;
+          th += dt * 1.2e-3f * f;
+          q1.x += dt * f / 600.0f * fmaxf(0.97f * qvsat(V.w * B.x, B.y, 0.0f) - q1.x, 0.0f);
+        }
+      }
+      const float ex = xmx - u.v[U_HUR4].x, ey = xmy - u.v[U_HUR4].y;
+      const float rS = sqrtf(ex * ex + ey * ey);
+      const float wEnv = sstep(u.v[U_HUR4].w, 1.5f * u.v[U_HUR4].w, rS) / 3600.0f;
+      th += dt * wEnv * (tth - V.w);
+      q1.x += dt * wEnv * (tqv - q1.x);
+      if (zz < u.v[U_HUR2].z) {
+        const float u1 = u1At(hu, d, x, y);
+        const float e = u.v[U_STEP].y / 900.0f, e0 = floorf(e), qx = xmx / 6000.0f, qy = xmy / 6000.0f;
+        const float ga = gnoise2(qx + 37.1f * e0, qy + 11.3f * e0);
+        const float gb = gnoise2(qx + 37.1f * (e0 + 1.0f), qy + 11.3f * (e0 + 1.0f));
+        const float t = sstep(0.0f, 1.0f, e - floorf(e));
+        const float rr = 0.5f * (ga * (1.0f - t) + gb * t);
+        const float fu = fminf(u1 * u1 / 225.0f, 1.0f) * (1.0f - sstep(20.0f, 32.0f, u1)) *
+                         (1.0f - sstep(u.v[U_HUR4].z, 1.5f * u.v[U_HUR4].z, rS));
+        th += dt * u.v[U_HUR3].z * 2.0f * rr * fu * (1.0f - zz / u.v[U_HUR2].z);
+      }
+      if (z == 0) {
+        const float4 sf = sfcAt(hu, d, x, y);
+        const float land = clampf_(sf.x, 0.0f, 1.0f);
+        const float u1 = u1At(hu, d, x, y);
+        const float ch = u.v[U_HUR].y * u.v[U_HUR2].w * fmaxf(u1, u.v[U_HUR3].y) / dz;
+        const int pd = (int)u.v[U_PAD].x;
+        const float phc = ldx(hu.phi, (uint32_t)(x + pd) + __umul24((uint32_t)u.v[U_PAD].y, (uint32_t)(y + pd)));
+        const float ps = fmaxf(u.v[U_HUR].w + A.z * u.v[U_CN].y * phc, 50000.0f);
+        const float ths = sf.y * powf(1.0e5f / ps, 0.2854f);
+        th += dt * ch * (1.0f - land) * (ths - V.w);
+        if (u.v[U_HUR5].y > 0.5f) th += dt * cdSfc(u, u1, land) * u1 * u1 * u1 / (OBL_CPD * dz) / B.x;
+        const float qsS = qvsat(sf.y, ps, 0.0f), qsL = qvsat(V.w * B.x, ps, 0.0f);
+        q1.x += dt * ch * ((1.0f - land) * (qsS - q1.x) + land * u.v[U_HUR].z * fmaxf(qsL - q1.x, 0.0f));
+      }
+      envDecay = sstep(u.v[U_HUR4].w, 1.5f * u.v[U_HUR4].w, rS) / 1800.0f;
+    }
     q1.x = fmaxf(q1.x, 0.0f); q1.y = fmaxf(q1.y, 0.0f); q1.z = fmaxf(q1.z, 0.0f); q1.w = fmaxf(q1.w, 0.0f);
     q2.x = fmaxf(q2.x, 0.0f); q2.y = fmaxf(q2.y, 0.0f); q2.z = fmaxf(q2.z, 0.0f); q2.w = fmaxf(q2.w, 0.0f);
     float T = th * B.x;
@@ -795,7 +888,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     }
     if (!handed) {
       th = T / B.x;
-      float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * 0.0f);
+      float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * envDecay);
       qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
       stx(Q1o, ci_, make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f)));
       stx(Q2o, ci_, make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w));
@@ -826,14 +919,25 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;     \
   const int x = x0 + tid % TX, y = y0 + tid / TX;
 
+template <bool HUR>
 __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                        const float4* __restrict__ Q1, float2* __restrict__ aux) {
+                                                        const float4* __restrict__ Q1, const float4* __restrict__ sfc,
+                                                        float2* __restrict__ aux) {
   __shared__ float lds[RING_FLOATS];
   TILE_SETUP
   Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
   rv.load(-1, tid); rv.load(0, tid); rv.load(1, tid);
   __syncthreads();
   const bool act = x < d.nx && y < d.ny;
+  // hurricane mode: the K-profile boundary layer's friction velocity, from the column's lowest-level wind
+  // (model.js:377-386)
+  float ust = 0.0f;
+  if (HUR && act) {
+    const float4 s0 = ldx(V, vidx(d, x, y, 0));
+    const float ua = 0.5f * (s0.x + ldxc(V, vidx(d, x + 1, y, 0), 0)), va = 0.5f * (s0.y + ldxc(V, vidx(d, x, y + 1, 0), 1));
+    const float u1 = sqrtf(ua * ua + va * va);
+    ust = sqrtf(cdSfc(u, u1, ldx(sfc, (uint32_t)x + __umul24((uint32_t)d.nx, (uint32_t)y)).x)) * fmaxf(u1, u.v[U_HUR3].y);
+  }
   // Reads a level needs besides its rings are loaded one level ahead, like the ring planes: the
   // loads a level issues are then only needed by later levels, and its compute starts without
   // waiting for memory.
@@ -844,7 +948,17 @@ __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __re
     q1n = ldQ(Q1, d, x, y, z + 1);
     rv.fetch(z + 2, tid, pv);
     rv.center(z);
-    if (act) stx(aux, cidx(d, x, y, z), nut_point(u, d, BS, rv.w(), q1, x, y, z));
+    if (act) {
+      float2 nu = nut_point(u, d, BS, rv.w(), q1, x, y, z);
+      if (HUR) {
+        const float zk = ((float)z + 0.5f) * d.dz, hb = u.v[U_HUR2].z;
+        if (zk < hb) {
+          const float q = 1.0f - zk / hb;
+          nu.y = fmaxf(nu.y, fminf(0.4f * ust * zk * q * q, 0.15f * d.dz * d.dz / fmaxf(d.dt, 0.1f)));
+        }
+      }
+      stx(aux, cidx(d, x, y, z), nu);
+    }
     __syncthreads();
     rv.store(z + 2, tid, pv);
     __syncthreads();
@@ -963,14 +1077,14 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
 // level's compute), aux and phT the current plane, lsfB the 2-D tile; the vertical stencils read
 // z - 1 / z + 1 from the rings, and Q2 / aux / phT at z - 1 are carried from the level below. Every
 // field crosses HBM about once per step instead of once per neighbouring level.
-template <int MODE>
+template <int MODE, bool HUR>
 __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __restrict__ BS, const float4* __restrict__ Vin,
                                                        const float4* __restrict__ Q1in, const float4* __restrict__ Q2in,
                                                        const float2* __restrict__ aux, const float4* __restrict__ lsfA,
                                                        const float4* __restrict__ lsfB, const float* __restrict__ phT,
                                                        const float4* __restrict__ ciMap, float4* __restrict__ Vout,
                                                        float4* __restrict__ Q1o, float4* __restrict__ Q2o, MicroDump dump,
-                                                       SurrPhys sp) {
+                                                       SurrPhys sp, HurArgs hu) {
   __shared__ float lds[2 * RING_FLOATS];
   __shared__ float sA[2 * TPS], sP[TPS], sB[4 * TPS];
   TILE_SETUP
@@ -1029,7 +1143,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
       col.Q1p = wq.ld4(x, y, z + 1);
       const TAux ta{(const lds_float*)sA, aux, d, x0, y0, z};
       const TPh tp{(const lds_float*)sP, phT, d, x0, y0, z};
-      phys_point<MODE>(u, d, BS, wv, wq, col, ta, tb, tp, Vout, Q1o, Q2o, dump, sp, x, y, z);
+      phys_point<MODE, HUR>(u, d, BS, wv, wq, col, ta, tb, tp, Vout, Q1o, Q2o, dump, sp, hu, x, y, z);
       col.Q2m = col.Q2c;
       col.am = ta(x, y, z);
       col.pm = tp(x, y, z);
@@ -1091,6 +1205,201 @@ __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V
   stx(Vout, i, o);
 }
 
+// ====================== hurricane mode: advection, PREP, FILT ======================
+// Hurricane runs trace with the wind at the trajectory's midpoint (cn.w, model.js:420-430, 494-499) and carry the
+// values of Vp (V less part of the last pressure force, WGSL_M_PREP) in the wind advection; ice settles (vtI) and
+// Lin graupel is maritime. These kernels read every field from global memory, one work-item per point.
+template <class A>
+__device__ __forceinline__ float3 velAt(const A& V, float3 p) { return make_float3(sampU(V, p), sampV(V, p), sampW(V, p)); }
+__device__ __forceinline__ float3 clampX3(const Dims& d, float3 p) {
+  return make_float3(clampt_(p.x, 0.0f, (float)d.nx), clampt_(p.y, 0.0f, (float)d.ny), clampt_(p.z, 0.0f, (float)d.nz));
+}
+template <class A>
+__device__ __forceinline__ float3 traceM(const Dims& d, const A& V, float3 k, float3 p, float3 v0, float s, bool mid) {
+  if (!mid) return traceV(d, k, p, v0, s);
+  float3 m = make_float3(p.x - 0.5f * s * k.x * v0.x, p.y - 0.5f * s * k.y * v0.y, p.z - 0.5f * s * k.z * v0.z);
+  return traceV(d, k, p, velAt(V, clampX3(d, m)), s);
+}
+template <class A>
+__device__ __forceinline__ float3 traceQM(const Dims& d, const A& V, float3 k, float3 p, float3 v0, float fall, float s,
+                                          bool mid) {
+  if (!mid) return traceQ(d, k, p, v0, fall, s);
+  float3 m = make_float3(p.x - 0.5f * s * k.x * v0.x, p.y - 0.5f * s * k.y * v0.y, p.z - 0.5f * s * k.z * (v0.z - fall));
+  return traceQ(d, k, p, velAt(V, clampX3(d, m)), fall, s);
+}
+#define HUR_FACE                                                                                     \
+  Dims d = dims(u);                                                                                  \
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
+__global__ void __launch_bounds__(256) k_advv1_h(MU u, const float4* __restrict__ V, const float4* __restrict__ Vval,
+                                                 float4* __restrict__ Vt) {
+  HUR_FACE
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const bool mid = u.v[U_CN].w > 0.5f;
+  const Loc L = localVel(gv, x, y, z);
+  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  float4 o;
+  o.x = sampU(gq, traceM(d, gv, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f, mid));
+  o.y = sampV(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f, mid));
+  o.z = sampW(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f, mid));
+  o.w = sampC(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f, mid));
+  stx(Vt, vidx(d, x, y, z), o);
+}
+__global__ void __launch_bounds__(256) k_advv2_h(MU u, const float4* __restrict__ V, const float4* __restrict__ Vval,
+                                                 const float4* __restrict__ Vt, float4* __restrict__ Vout) {
+  HUR_FACE
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const GAcc gt{Vt, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const bool mid = u.v[U_CN].w > 0.5f;
+  const Loc L = localVel(gv, x, y, z);
+  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  const float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
+  const float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
+  const float4 hat = gt.ld4(x, y, z), old = gq.ld4(x, y, z);
+  float4 b;
+  b.x = sampU(gt, traceM(d, gv, k, pu, L.vu, -1.0f, mid));
+  b.y = sampV(gt, traceM(d, gv, k, pv, L.vv, -1.0f, mid));
+  b.z = sampW(gt, traceM(d, gv, k, pw, L.vw, -1.0f, mid));
+  b.w = sampC(gt, traceM(d, gv, k, pc, L.vc, -1.0f, mid));
+  const float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
+  const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
+                         hat.w + sm * (old.w - b.w));
+  float2 m;
+  m = mmV(gq, traceM(d, gv, k, pu, L.vu, 1.0f, mid), 0.0f, 0.5f, 0.5f, 0); r.x = vclamp_(r.x, m);
+  m = mmV(gq, traceM(d, gv, k, pv, L.vv, 1.0f, mid), 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
+  m = mmV(gq, traceM(d, gv, k, pw, L.vw, 1.0f, mid), 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
+  m = mmV(gq, traceM(d, gv, k, pc, L.vc, 1.0f, mid), 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
+  stx(Vout, vidx(d, x, y, z), r);
+}
+struct FallH {
+  float r, n, s, g, i;
+};
+__device__ __forceinline__ FallH fall_h(const MU& u, float4 q1, float4 q2, float rho) {
+  const bool thm = u.v[U_MICRO2].z > 0.5f;
+  const float rs = u.v[U_STEP].z;
+  const float2 fv = vtRain(thm, q1.w, q2.z, rho, rs);
+  return FallH{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rs), vtGrauM(thm, maritime_of(u), q2.y, rho, rs), vtI(q1.z, rho)};
+}
+__global__ void __launch_bounds__(256) k_advq1_h(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                 const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                 float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
+  HUR_FACE
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+  const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
+  const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  const uint32_t i = cidx(d, x, y, z);
+  const float4 q1 = ldx(Q1, i), q2 = ldx(Q2, i);
+  const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
+  float4 a = sampQ(g1, traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid));
+  if (f.i > 0.0f) a.z = sampQ1(g1, traceQM(d, gv, k, p, v, f.i, 1.0f, mid), 2);
+  const float r = sampQ1(g1, traceQM(d, gv, k, p, v, f.r, 1.0f, mid), 3);
+  const float4 b = sampQ(g2, traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid));
+  const float sv = sampQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0);
+  const float g = sampQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1);
+  const float nr = thm ? sampQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2) : b.z;
+  stx(Q1t, i, make_float4(a.x, a.y, a.z, r));
+  stx(Q2t, i, make_float4(sv, g, nr, b.w));
+}
+__global__ void __launch_bounds__(256) k_advq2_h(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                 const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                 const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
+                                                 float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  HUR_FACE
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+  const GAcc g1t{Q1t, d.nx, d.ny, d.nz, nullptr}, g2t{Q2t, d.nx, d.ny, d.nz, nullptr};
+  const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
+  const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  const uint32_t i = cidx(d, x, y, z);
+  const float4 q1 = ldx(Q1, i), q2 = ldx(Q2, i), h1 = ldx(Q1t, i), h2 = ldx(Q2t, i);
+  const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
+  const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
+  const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  const float4 b1 = sampQ(g1t, traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid));
+  const float br = sampQ1(g1t, traceQM(d, gv, k, p, v, f.r, -1.0f, mid), 3);
+  const float4 b2 = sampQ(g2t, traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid));
+  const float bs = sampQ1(g2t, traceQM(d, gv, k, p, v, f.s, -1.0f, mid), 0);
+  const float bg = sampQ1(g2t, traceQM(d, gv, k, p, v, f.g, -1.0f, mid), 1);
+  const float bn = sampQ1(g2t, traceQM(d, gv, k, p, v, f.n, -1.0f, mid), 2);
+  const float bi = f.i > 0.0f ? sampQ1(g1t, traceQM(d, gv, k, p, v, f.i, -1.0f, mid), 2) : b1.z;
+  float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - bi),
+                          h1.w + sm * (q1.w - br));
+  float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
+                          h2.w + sm * (q2.w - b2.w));
+  const float3 d0 = traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid);
+  float2 m;
+  m = mmQ1(g1, d0, 0); r1.x = vclamp_(r1.x, m);
+  m = mmQ1(g1, d0, 1); r1.y = vclamp_(r1.y, m);
+  m = mmQ1(g1, traceQM(d, gv, k, p, v, f.i, 1.0f, mid), 2); r1.z = vclamp_(r1.z, m);
+  m = mmQ1(g1, traceQM(d, gv, k, p, v, f.r, 1.0f, mid), 3); r1.w = vclamp_(r1.w, m);
+  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0); r2.x = vclamp_(r2.x, m);
+  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1); r2.y = vclamp_(r2.y, m);
+  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2); r2.z = vclamp_(r2.z, m);
+  m = mmQ1(g2, d0, 3); r2.w = vclamp_(r2.w, m);
+  stx(Q1o, i, make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f)));
+  stx(Q2o, i, make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f)));
+}
+// PREP (model.js:996-1026): the winds the advection carries, V less cn.x of the last pressure force
+__global__ void __launch_bounds__(256) k_prep(MU u, const float4* __restrict__ V, const float* __restrict__ phi,
+                                              float4* __restrict__ Vp) {
+  HUR_FACE
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  const uint32_t vi = vidx(d, x, y, z);
+  const float4 Vv = ldx(V, vi);
+  float4 o = Vv;
+  const float f = u.v[U_CN].x * u.v[U_CN].y;
+  const float kx = f * d.dt / d.dx, kz = f * d.dt / d.dz;
+  const float pc = Pphi(phi, u, d, x, y, z);
+  if (y < d.ny && z < d.nz) o.x = Vv.x - kx * (pc - Pphi(phi, u, d, x - 1, y, z));
+  if (x < d.nx && z < d.nz) o.y = Vv.y - kx * (pc - Pphi(phi, u, d, x, y - 1, z));
+  if (x < d.nx && y < d.ny && z > 0 && z < d.nz) o.z = Vv.z - kz * (pc - Pphi(phi, u, d, x, y, z - 1));
+  stx(Vp, vi, o);
+}
+// FILT (model.js:948-994): a weak 4th-order horizontal hyperdiffusion, every 4th step at 4x strength
+__device__ __forceinline__ float4 bih4(const GAcc& g, int x, int y, int z) {
+  const float4 c0 = g.ld4(x, y, z);
+  const float4 xp = g.ld4(x + 1, y, z), xm = g.ld4(x - 1, y, z), yp = g.ld4(x, y + 1, z), ym = g.ld4(x, y - 1, z);
+  const float4 pp = g.ld4(x + 1, y + 1, z), pm = g.ld4(x + 1, y - 1, z), mp = g.ld4(x - 1, y + 1, z), mm = g.ld4(x - 1, y - 1, z);
+  const float4 x2p = g.ld4(x + 2, y, z), x2m = g.ld4(x - 2, y, z), y2p = g.ld4(x, y + 2, z), y2m = g.ld4(x, y - 2, z);
+#define OBL_BIH(ch)                                                                                                   \
+  (20.0f * c0.ch - 8.0f * (xp.ch + xm.ch + yp.ch + ym.ch) + 2.0f * (pp.ch + pm.ch + mp.ch + mm.ch) + x2p.ch + x2m.ch + \
+   y2p.ch + y2m.ch)
+  const float4 r = make_float4(OBL_BIH(x), OBL_BIH(y), OBL_BIH(z), OBL_BIH(w));
+#undef OBL_BIH
+  return r;
+}
+__global__ void __launch_bounds__(256) k_filt(MU u, const float4* __restrict__ V, const float4* __restrict__ Q1,
+                                              const float4* __restrict__ Q2, float4* __restrict__ Vo,
+                                              float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  HUR_FACE
+  if (x > d.nx || y > d.ny || z > d.nz) return;
+  const float a = u.v[U_HUR5].z;
+  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const float4 v = gv.ld4(x, y, z), bv = bih4(gv, x, y, z);
+  float4 o = make_float4(v.x - a * bv.x, v.y - a * bv.y, v.z - a * bv.z, v.w - a * bv.w);
+  if (z == 0 || z >= d.nz) o.z = 0.0f;  // the lid / ground faces of w stay zero
+  stx(Vo, vidx(d, x, y, z), o);
+  if (x < d.nx && y < d.ny && z < d.nz) {
+    const GAcc g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+    const uint32_t i = cidx(d, x, y, z);
+    const float4 q = ldx(Q1, i), bq = bih4(g1, x, y, z), r = ldx(Q2, i), br = bih4(g2, x, y, z);
+    stx(Q1o, i, make_float4(fmaxf(q.x - a * bq.x, 0.0f), fmaxf(q.y - a * bq.y, 0.0f), fmaxf(q.z - a * bq.z, 0.0f),
+                            fmaxf(q.w - a * bq.w, 0.0f)));
+    stx(Q2o, i, make_float4(fmaxf(r.x - a * br.x, 0.0f), fmaxf(r.y - a * br.y, 0.0f), fmaxf(r.z - a * br.z, 0.0f),
+                            fmaxf(r.w - a * br.w, 0.0f)));
+  }
+}
+#undef HUR_FACE
+
 // ====================== launchers ======================
 static dim3 faces_grid(const Dims& d) { return dim3((d.nx + 1 + 63) / 64, (d.ny + 1 + 3) / 4, d.nz + 1); }
 static dim3 cells_grid(const Dims& d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
@@ -1104,8 +1413,8 @@ void launch_ci(hipStream_t s, const MU& u, const Dims& d, const float* CI, float
   hipLaunchKernelGGL(k_ci, cols_grid(d), dim3(16, 16, 1), 0, s, u, CI, ciMap);
 }
 void launch_init(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* lsfA, const float4* lsfB,
-                 float4* V, float4* Q1, float4* Q2, int seed) {
-  hipLaunchKernelGGL(k_init, faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, lsfA, lsfB, V, Q1, Q2, seed);
+                 const float4* VX, float4* V, float4* Q1, float4* Q2, int seed) {
+  hipLaunchKernelGGL(k_init, faces_grid(d), dim3(64, 4, 1), 0, s, u, BS, lsfA, lsfB, VX, V, Q1, Q2, seed);
 }
 void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4* V1, double x, double y, double z,
                    double rh, double rz, double dth) {
@@ -1114,8 +1423,11 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
 static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
 static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                float2* aux) {
-  hipLaunchKernelGGL(k_nut_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, aux);
+                const float4* sfc, float2* aux) {
+  if (sfc && hur_on(u))
+    hipLaunchKernelGGL(k_nut_t<true>, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, sfc, aux);
+  else
+    hipLaunchKernelGGL(k_nut_t<false>, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, sfc, aux);
 }
 void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt) {
   hipLaunchKernelGGL(k_advv1_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt);
@@ -1136,28 +1448,62 @@ void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, co
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
   SurrPhys nos = {};
-  hipLaunchKernelGGL((k_phys<PHYS_PLAIN>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
-                     lsfB, phT, ciMap, Vout, Q1o, Q2o, none, nos);
+  HurArgs noh = {nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL((k_phys<PHYS_PLAIN, false>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux,
+                     lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, none, nos, noh);
+}
+void launch_phys_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                     const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                     const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const HurArgs& hu) {
+  MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
+  SurrPhys nos = {};
+  hipLaunchKernelGGL((k_phys<PHYS_PLAIN, true>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux,
+                     lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, none, nos, hu);
 }
 void launch_phys_surr(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const SurrPhys& sp) {
   MicroDump none = {nullptr, nullptr, 0, 0u, 0.0f, 0.0f};
-  hipLaunchKernelGGL((k_phys<PHYS_SURR>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
-                     lsfB, phT, ciMap, Vout, Q1o, Q2o, none, sp);
+  HurArgs noh = {nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL((k_phys<PHYS_SURR, false>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux,
+                     lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, none, sp, noh);
 }
 void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                       const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const MicroDump& dump) {
   SurrPhys nos = {};
-  hipLaunchKernelGGL((k_phys<PHYS_DUMP>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux, lsfA,
-                     lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, nos);
+  HurArgs noh = {nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL((k_phys<PHYS_DUMP, false>), tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, Vin, Q1in, Q2in, aux,
+                     lsfA, lsfB, phT, ciMap, Vout, Q1o, Q2o, dump, nos, noh);
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, rhs);
 }
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout) {
   hipLaunchKernelGGL(k_proj, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, phi, Vout);
+}
+
+void launch_advv1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, float4* Vt) {
+  hipLaunchKernelGGL(k_advv1_h, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vval, Vt);
+}
+void launch_advv2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, const float4* Vt,
+                      float4* Vout) {
+  hipLaunchKernelGGL(k_advv2_h, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vval, Vt, Vout);
+}
+void launch_advq1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                      const float4* Q2, float4* Q1t, float4* Q2t) {
+  hipLaunchKernelGGL(k_advq1_h, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
+}
+void launch_advq2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                      const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o) {
+  hipLaunchKernelGGL(k_advq2_h, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
+}
+void launch_prep(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vp) {
+  hipLaunchKernelGGL(k_prep, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, phi, Vp);
+}
+void launch_filt(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Q1, const float4* Q2,
+                 float4* Vo, float4* Q1o, float4* Q2o) {
+  hipLaunchKernelGGL(k_filt, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Q1, Q2, Vo, Q1o, Q2o);
 }
 
 }  // namespace obl

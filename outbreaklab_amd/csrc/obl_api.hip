@@ -438,6 +438,11 @@ struct obl_model {
   float4* tor = nullptr;
   int ntor = 0;
   std::vector<obl_radar*> radars;  // owned: freed with the model
+  // hurricane mode
+  float4* sfc = nullptr;  // sea surface map, nx*ny
+  float4* Vp = nullptr;   // the winds the advection carries (prep field)
+  float4* vx = nullptr;   // initial vortex table, nz*vx_nr
+  int vx_nr = 0;
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -467,6 +472,8 @@ extern "C" void obl_destroy(obl_model* h) {
   for (void* b : bufs)
     if (b) hipFree(b);
   while (!h->radars.empty()) obl_radar_destroy(h->radars.back());
+  for (void* b : {(void*)h->sfc, (void*)h->Vp, (void*)h->vx})
+    if (b) hipFree(b);
   if (h->mapd) hipFree(h->mapd);
   if (h->tor) hipFree(h->tor);
   for (int q = 0; q < 2; q++) {
@@ -555,9 +562,8 @@ static int check_u(obl_model* h, const float* u) {
   if ((int)u[36] != h->pad || (int)u[37] != h->nxp || (int)u[38] != h->nyp)
     return h->fail("uniform pad block (%g, %g, %g) does not match pad %d (%d x %d)", u[36], u[37], u[38], h->pad, h->nxp,
                    h->nyp);
-  if (u[44] > 0.5f) return h->fail("hurricane mode (hur.x = 1) is not supported by this library");
-  if (u[70] > 0.5f) return h->fail("initial vortex (vtx2.z = 1) is not supported by this library");
-  if (u[75] > 0.5f) return h->fail("midpoint trajectories (cn.w = 1) are not supported by this library");
+  if (u[44] > 0.5f && !h->sfc) return h->fail("hurricane mode (hur.x = 1) needs a sea surface: setSurface first");
+  if (u[75] > 0.5f && !(u[44] > 0.5f)) return h->fail("midpoint trajectories (cn.w = 1) are a hurricane-mode setting");
   if (!(u[8] > 0.0f)) return h->fail("dt must be positive");
   return 0;
 }
@@ -602,8 +608,50 @@ extern "C" int obl_set_ci(obl_model* h, const float* f, int count) {
 
 extern "C" int obl_set_surface(obl_model* h, const float* surf) {
   if (!h) return set_global("null handle");
-  if (surf) return h->fail("setSurface: hurricane-mode surfaces are not supported by this library");
-  return 0;
+  if (!surf) {
+    if (h->sfc) {
+      hipStreamSynchronize(h->stream);
+      hipFree(h->sfc);
+      h->sfc = nullptr;
+    }
+    return 0;
+  }
+  const size_t b = (size_t)h->nx * h->ny * 16;
+  if (!h->sfc && hip_ok(hipMalloc(&h->sfc, b), &h->err, "hipMalloc(surface)")) return -1;
+  if (hip_ok(hipMemcpyAsync(h->sfc, surf, b, hipMemcpyHostToDevice, h->stream), &h->err, "setSurface")) return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setSurface");
+}
+
+extern "C" int obl_set_prep_field(obl_model* h, int on) {
+  if (!h) return set_global("null handle");
+  if (!on) {
+    if (h->Vp) {
+      hipStreamSynchronize(h->stream);
+      hipFree(h->Vp);
+      h->Vp = nullptr;
+    }
+    return 0;
+  }
+  if (h->Vp) return 0;
+  if (hip_ok(hipMalloc(&h->Vp, h->vcount() * 16), &h->err, "hipMalloc(prep field)")) return -1;
+  return hip_ok(hipMemsetAsync(h->Vp, 0, h->vcount() * 16, h->stream), &h->err, "prep field clear");
+}
+
+extern "C" int obl_set_vortex(obl_model* h, const float* table, int nr) {
+  if (!h) return set_global("null handle");
+  if (h->vx) {
+    hipStreamSynchronize(h->stream);
+    hipFree(h->vx);
+    h->vx = nullptr;
+    h->vx_nr = 0;
+  }
+  if (!table || nr == 0) return 0;
+  if (nr < 2 || nr > 65536) return h->fail("setVortex: the table needs 2..65536 radii (got %d)", nr);
+  const size_t b = (size_t)h->nz * nr * 16;
+  if (hip_ok(hipMalloc(&h->vx, b), &h->err, "hipMalloc(vortex)")) return -1;
+  h->vx_nr = nr;
+  if (hip_ok(hipMemcpyAsync(h->vx, table, b, hipMemcpyHostToDevice, h->stream), &h->err, "setVortex")) return -1;
+  return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setVortex");
 }
 
 static Dims dims_of(const obl_model* h, const MU& u) {
@@ -620,7 +668,13 @@ extern "C" int obl_reset(obl_model* h, const float* uniforms, int seed) {
   Dims d = dims_of(h, u);
   hipStream_t s = h->stream;
   launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT);
-  launch_init(s, u, d, h->BS, h->lsfA, h->lsfB, h->V[0], h->Q1[0], h->Q2[0], seed);
+  const float4* VX = nullptr;
+  if (u.v[U_VTX2].z > 0.5f) {
+    if (!h->vx || (int)u.v[U_VTX].w != h->vx_nr)
+      return h->fail("reset: vtx2.z = 1 needs the vortex table of %g radii (setVortex; %d loaded)", u.v[U_VTX].w, h->vx_nr);
+    VX = h->vx;
+  }
+  launch_init(s, u, d, h->BS, h->lsfA, h->lsfB, VX, h->V[0], h->Q1[0], h->Q2[0], seed);
   hipMemsetAsync(h->phi, 0, h->pcount() * 4, s);
   hipMemsetAsync(h->rhs, 0, h->pcount() * 4, s);
   hipMemcpyAsync(h->V[1], h->V[0], h->vcount() * 16, hipMemcpyDeviceToDevice, s);
@@ -764,7 +818,7 @@ extern "C" int obl_maps_request(obl_model* h, const float* uniforms, int parity,
   MU u = to_mu(uniforms);
   Dims d = dims_of(h, u);
   hipStream_t s = h->stream;
-  TIMED(h->timer, s, "m.map", launch_maps(s, u, d, h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->phi, h->mapd));
+  TIMED(h->timer, s, "m.map", launch_maps(s, u, d, h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->phi, h->sfc, h->mapd));
   if (hip_ok(hipMemcpyAsync(h->maph[q], h->mapd, bytes, hipMemcpyDeviceToHost, s), &h->err, "requestMaps copy") ||
       hip_ok(hipEventRecord(h->mapev[q], s), &h->err, "requestMaps event"))
     return -1;
@@ -911,6 +965,45 @@ extern "C" int obl_radar_collect(obl_radar* r, int slot, void* dst, size_t dst_b
   return 0;
 }
 
+// The hurricane-mode step (model.js:1442-1476 with this.hur): the 2-dx filter every 4th step (state ->
+// the spare copies -> back), PREP while the prep field exists, then the step with the hurricane kernels.
+// The surrogate, its judge and teacher sampling are outbreak-mode tools and are refused here.
+static int step_hurricane(obl_model* h, const MU& u, const Dims& d, int parity, unsigned flags) {
+  hipStream_t s = h->stream;
+  Timer& T = h->timer;
+  float4 *Vc = h->V[parity], *Vn = h->V[1 - parity];
+  if (h->net || h->dump_on || (flags & OBL_STEP_JUDGE))
+    return h->fail("hurricane mode runs the plain microphysics: detach the surrogate and teacher sampling");
+  if (flags & OBL_STEP_FILT) {
+    TIMED(T, s, "m.filt", launch_filt(s, u, d, Vc, h->Q1[0], h->Q2[0], Vn, h->Q1[1], h->Q2[1]));
+    if (hip_ok(hipMemcpyAsync(Vc, Vn, h->vcount() * 16, hipMemcpyDeviceToDevice, s), &h->err, "filter copy") ||
+        hip_ok(hipMemcpyAsync(h->Q1[0], h->Q1[1], h->ccount() * 16, hipMemcpyDeviceToDevice, s), &h->err, "filter copy") ||
+        hip_ok(hipMemcpyAsync(h->Q2[0], h->Q2[1], h->ccount() * 16, hipMemcpyDeviceToDevice, s), &h->err, "filter copy"))
+      return -1;
+  }
+  const float4* Vval = Vc;
+  if (h->Vp) {
+    TIMED(T, s, "m.prep", launch_prep(s, u, d, Vc, h->phi, h->Vp));
+    Vval = h->Vp;
+  }
+  if (flags & OBL_STEP_LSF) TIMED(T, s, "m.lsf", launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT));
+  TIMED(T, s, "m.ci", launch_ci(s, u, d, h->CI, h->ciMap));
+  TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], h->sfc, h->aux));
+  TIMED(T, s, "m.advV1", launch_advv1_hur(s, u, d, Vc, Vval, h->Vt));
+  TIMED(T, s, "m.advV2", launch_advv2_hur(s, u, d, Vc, Vval, h->Vt, Vn));
+  TIMED(T, s, "m.advQ1", launch_advq1_hur(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
+  TIMED(T, s, "m.advQ2", launch_advq2_hur(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t, h->Q1[1], h->Q2[1]));
+  HurArgs hu = {h->sfc, h->phi, Vn};
+  TIMED(T, s, "m.phys", launch_phys_hur(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT,
+                                        h->ciMap, Vc, h->Q1[0], h->Q2[0], hu));
+  TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
+  h->solver->timer.on = T.on;
+  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s))
+    return h->fail("%s", h->solver->err.c_str());
+  TIMED(T, s, "m.proj", launch_proj(s, u, d, Vc, h->phi, Vn));
+  return hip_ok(hipGetLastError(), &h->err, "encStep");
+}
+
 extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigned flags) {
   if (!h) return set_global("null handle");
   if (check_u(h, uniforms)) return -1;
@@ -920,9 +1013,11 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   hipStream_t s = h->stream;
   float4 *Vc = h->V[parity], *Vn = h->V[1 - parity];
   Timer& T = h->timer;
+  if (hur_on(u)) return step_hurricane(h, u, d, parity, flags);
+  if (flags & OBL_STEP_FILT) return h->fail("the 2-dx filter (OBL_STEP_FILT) is a hurricane-mode step");
   if (flags & OBL_STEP_LSF) TIMED(T, s, "m.lsf", launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT));
   TIMED(T, s, "m.ci", launch_ci(s, u, d, h->CI, h->ciMap));
-  TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], h->aux));
+  TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], nullptr, h->aux));
   TIMED(T, s, "m.advV1", launch_advv1(s, u, d, Vc, h->Vt));
   TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
   TIMED(T, s, "m.advQ1", launch_advq1(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
@@ -1003,6 +1098,12 @@ static int field_info(obl_model* h, int field, int copy, void** ptr, size_t* byt
     case OBL_FIELD_PHI: *ptr = h->phi; *bytes = pb; return 0;
     case OBL_FIELD_RHS: *ptr = h->rhs; *bytes = pb; return 0;
     case OBL_FIELD_BS: *ptr = h->BS; *bytes = (size_t)(h->nz + 1) * 32; return 0;
+    case OBL_FIELD_SFC:
+      if (!h->sfc) return h->fail("no sea surface (setSurface)");
+      *ptr = h->sfc; *bytes = mb; return 0;
+    case OBL_FIELD_VP:
+      if (!h->Vp) return h->fail("no prep field (setPrepField)");
+      *ptr = h->Vp; *bytes = vb; return 0;
   }
   return h->fail("unknown field id %d", field);
 }

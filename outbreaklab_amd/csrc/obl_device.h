@@ -259,6 +259,27 @@ __device__ __forceinline__ float2 vtRain(bool thm, float qr, float nr, float rho
 __device__ __forceinline__ float vtGrau(bool thm, float qg, float rho, float rsfc) {
   return thm ? vtGT(qg, rho, rsfc) : vtG(qg, rho, rsfc);
 }
+// ---- hurricane mode (model.js:88-119): a warm sea under the domain ----
+__host__ __device__ __forceinline__ bool hur_on(const MU& u) { return u.v[U_HUR].x > 0.5f; }
+// maritime (soft, low-density) graupel with the Lin scheme
+__device__ __forceinline__ bool maritime_of(const MU& u) { return u.v[U_HUR3].w > 0.5f && !(u.v[U_MICRO2].z > 0.5f); }
+__device__ __forceinline__ float vtGM(float q, float rho, float rsfc) {
+  return clampf_(5.0f * pw(fmaxf(1000.0f * rho * q, 1e-6f), 0.25f), 1.0f, 9.0f) * sqrtf(rsfc / rho);
+}
+__device__ __forceinline__ float vtGrauM(bool thm, bool mar, float qg, float rho, float rsfc) {
+  return mar ? vtGM(qg, rho, rsfc) : vtGrau(thm, qg, rho, rsfc);
+}
+// settling cloud ice (hurricane mode only)
+__device__ __forceinline__ float vtI(float q, float rho) {
+  return q > 1e-9f ? fminf(3.29f * pw(fmaxf(rho * q, 1e-12f), 0.16f), 1.5f) : 0.0f;
+}
+// drag coefficient of the lowest-level wind u1: over the sea rising with the wind up to a cap, over land the log law
+__device__ __forceinline__ float cdSfc(const MU& u, float u1, float land) {
+  float f = u.v[U_HUR2].w;
+  float cs = fminf((0.75f + 0.067f * f * u1) * 1e-3f, u.v[U_HUR5].x) * f * f;
+  float t = clampf_(land, 0.0f, 1.0f);
+  return cs * (1.0f - t) + u.v[U_HUR3].x * t;
+}
 
 // ---------------- large-scale target ----------------
 __device__ __forceinline__ float tprof(float z, float H) { return 1.0f - sstep(0.7f * H, 1.15f * H, z); }

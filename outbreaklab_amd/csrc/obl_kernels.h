@@ -10,12 +10,14 @@ namespace obl {
 void launch_lsf(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float* BD, float4* lsfA,
                 float4* lsfB, float* phT);
 void launch_ci(hipStream_t s, const MU& u, const Dims& d, const float* CI, float4* ciMap);
+// VX: the initial vortex's table, nz x nr float4 (tangential wind, theta', 0, 0), or null
 void launch_init(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* lsfA, const float4* lsfB,
-                 float4* V, float4* Q1, float4* Q2, int seed);
+                 const float4* VX, float4* V, float4* Q1, float4* Q2, int seed);
 void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4* V1, double x, double y, double z,
                    double rh, double rz, double dth);
+// sfc: the sea surface map (nx * ny float4: land fraction, SST K, 0, 0); null outside hurricane mode
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                float2* aux);
+                const float4* sfc, float2* aux);
 void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt);
 void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout);
 void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
@@ -82,11 +84,31 @@ struct JudgeArgs {
 void launch_judge_cols(hipStream_t s, const MU& u, const Dims& d, const JudgeArgs& a);
 void launch_judge_precip(hipStream_t s, const double* prs, const double* prp, size_t n, double* out);
 
+// ---- hurricane mode (model.js:88-119 and the hurOn() / cn.w branches, WGSL_M_PREP, WGSL_M_FILT) ----
+struct HurArgs {
+  const float4* sfc;  // sea surface map, nx * ny float4 (land fraction, SST K, 0, 0)
+  const float* phi;   // the last solve's padded phi (the sea fluxes' surface pressure)
+  const float4* V;    // the physics kernel's input winds (lowest-level wind speed)
+};
+void launch_phys_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
+                     const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
+                     const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o, const HurArgs& hu);
+void launch_advv1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, float4* Vt);
+void launch_advv2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, const float4* Vt,
+                      float4* Vout);
+void launch_advq1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                      const float4* Q2, float4* Q1t, float4* Q2t);
+void launch_advq2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                      const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o);
+void launch_prep(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vp);
+void launch_filt(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Q1, const float4* Q2,
+                 float4* Vo, float4* Q1o, float4* Q2o);
+
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
 // 2-D maps of the current state (model.js:1028-1100): out = 4 planar maps of nx*ny float4
 void launch_maps(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 const float4* Q2, const float* phi, float4* out);
+                 const float4* Q2, const float* phi, const float4* sfc, float4* out);
 
 // ---- pressure solve (k_solve.hip) ----
 struct DstArgs {

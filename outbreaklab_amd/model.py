@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from .grid import GRID_PRESETS, grid_bytes, grid_pad
 from .sounding import analyze, base_state, parse_sharppy
-from .uniforms import OUTBREAK_OVERRIDES, UBlock, default_params, write_uniforms
+from .uniforms import CN_OLD, DEG, OUTBREAK_OVERRIDES, UBlock, default_params, write_uniforms
 
 MAX_BOUNDS, MAX_BPTS, MAX_CI = 8, 32, 256
 
@@ -52,6 +52,80 @@ class Field:
                    self.model.h)
 
 
+VX_MAXR = 2048  # radii of the vortex table (model.js:38)
+
+
+def _smooth(a, b, x):
+    t = min(max((x - a) / (b - a), 0.0), 1.0)
+    return t * t * (3 - 2 * t)
+
+
+def vortex_table(nz, dz, dx, lat, B, v):
+    """setVortex's table (model.js:1318-1365) in FP64, stored FP32: tangential wind v(r, z) = Vs(r) F(z) with Vs a
+    Rankine-type profile vmax (1+a) x / (a + x^(1+a)), x = r / rmw, a = 0.6, tapered to r0 and held inertially
+    stable (the absolute angular momentum never decreases outward); phi(r) integrated inward from the gradient-wind
+    balance d phi / dr = v^2 / r + f v; theta' = theta0 (d phi / dz) / g. Returns (table nz x nr x 4, info)."""
+    f = 2 * 7.292e-5 * math.sin(lat * DEG)
+    dr = max(500.0, dx / 2)
+    nr = min(VX_MAXR, math.ceil(v["r0"] * 2 / dr) + 3)
+    sg = -1 if f < 0 else 1
+    fa, al = abs(f), 0.6
+    vs = np.zeros(nr)
+    Mp = 0.0
+    for i in range(nr):
+        r = i * dr
+        x = r / v["rmw"]
+        s = v["vmax"] * (1 + al) * x / (al + x ** (1 + al)) * (1 - _smooth(0.5 * v["r0"], v["r0"], r))
+        if r > 0:
+            s = max(s, (Mp - 0.5 * fa * r * r) / r, 0.0)
+        vs[i] = s
+        Mp = r * s + 0.5 * fa * r * r
+
+    def Vs(r):
+        return sg * vs[min(nr - 1, _jsround(r / dr))]
+
+    if v.get("shape") == "mid":
+        def Fz(z):
+            return math.exp(-((z - 5000) / 2500) ** 2)
+    else:
+        def Fz(z):
+            return 0.0 if z >= v["ztop"] else 1 - (z / v["ztop"] - math.sin(2 * math.pi * z / v["ztop"]) / (2 * math.pi))
+    vt, ph = [], []
+    for k in range(nz):
+        F = Fz((k + 0.5) * dz)
+        vk = np.array([Vs(i * dr) * F for i in range(nr)])
+        g = np.array([0.0] + [vk[i] * vk[i] / (i * dr) + f * vk[i] for i in range(1, nr)])
+        pk = np.zeros(nr)
+        for i in range(nr - 2, -1, -1):
+            pk[i] = pk[i + 1] - 0.5 * (g[i] + g[i + 1]) * dr
+        vt.append(vk)
+        ph.append(pk)
+    tab = np.zeros((nz, nr, 4), np.float32)
+    for k in range(nz):
+        k0, k1 = max(k - 1, 0), min(k + 1, nz - 1)
+        b = (ph[k1] - ph[k0]) / ((k1 - k0) * dz)
+        tab[k, :, 0] = vt[k]
+        tab[k, :, 1] = float(B[k * 8]) * b / 9.81
+    min_stab, warm = 1e9, 0.0
+    for k in range(1, nz):
+        z = k * dz
+        if z < 2000 or z > v["ztop"]:
+            continue
+        for i in range(min(nr, math.ceil(2 * v["rmw"] / dr))):
+            a = float(B[(k - 1) * 8]) + float(tab[k - 1, i, 1])
+            bb = float(B[k * 8]) + float(tab[k, i, 1])
+            min_stab = min(min_stab, (bb - a) / dz * 1000)
+        warm = max(warm, float(tab[k, 0, 1]))
+    mr = v.get("moistR") or 0
+    info = {"x": v["x"], "y": v["y"], "dr": dr, "nr": nr, "moistR": mr or 1, "moistRH": v.get("moistRH", 0) if mr else 0,
+            "pc": ph[0][0] * float(B[2]) / 100, "minStab": min_stab, "warm": warm}
+    return tab, info
+
+
+def _jsround(x):
+    return int(math.floor(x + 0.5))
+
+
 class RegionalModel:
     """RegionalModel (model.js:1156-1496) on MI355X."""
 
@@ -67,6 +141,11 @@ class RegionalModel:
         self.ciActive = 0
         self.rhoSfc = None
         self.phiScale = 1
+        self.hur = None
+        self.prep = False
+        self.vortex = None
+        self.filtNow = False
+        self.cnNow = False
         self.h = None
         self.key = None
         self.base = None
@@ -172,18 +251,41 @@ class RegionalModel:
         self._ci = f
         _lib.check(_lib.lib().obl_set_ci(self.h, _ptr(f), len(act)), self.h)
 
+    def setPrepField(self, on):
+        """setPrepField (model.js:1290-1296): the extra wind field hurricane runs carry."""
+        _lib.check(_lib.lib().obl_set_prep_field(self.h, 1 if on else 0), self.h)
+        self.prep = bool(on)
+        if on:
+            self.Vp = Field(self, "Vp", 0, self.V[0].shape)
+
     def setSurface(self, surf=None, hur=None):
-        """setSurface (model.js:1298-1303): outbreak mode only (surf None)."""
-        if surf is not None or hur is not None:
-            raise NotImplementedError("hurricane mode is outside this library's scope")
-        _lib.check(_lib.lib().obl_set_surface(self.h, None), self.h)
-        self.hur = None
+        """setSurface (model.js:1298-1303). surf: nx*ny*4 float32 (land fraction, SST K, 0, 0) per column, or
+        None (no sea: outbreak mode); hur: {ck, landMoist, sfcP Pa, tauRad s, radCap K/s, blDepth m, u10f, gust m/s,
+        ...} or None."""
+        a = None if surf is None else np.ascontiguousarray(surf, np.float32).reshape(-1)
+        if a is not None and a.size != self.nx * self.ny * 4:
+            raise ValueError(f"setSurface: {self.nx * self.ny * 4} floats expected, got {a.size}")
+        if a is None and hur is not None:
+            a = np.zeros(self.nx * self.ny * 4, np.float32)
+        _lib.check(_lib.lib().obl_set_surface(self.h, None if a is None else _ptr(a)), self.h)
+        self.sfc = None if a is None else Field(self, "sfc", 0, (self.ny, self.nx, 4))
+        self.hur = dict(hur) if hur else None
+        self.setPrepField(bool(hur))
 
     def setVortex(self, v=None):
-        if v is not None:
-            raise NotImplementedError("the initial vortex (hurricane mode) is outside this library's scope")
-        self.vortex = None
-        return None
+        """setVortex (model.js:1304-1365): the initial warm-core vortex, v = {x, y (model m), vmax m/s, rmw m, r0 m,
+        ztop m, moistR m (0 = no moist core), moistRH, shape ('mid' for a mid-level vortex)} or None. Builds the
+        (tangential wind, theta') table on the host and hands it to the library for reset; returns the vortex info
+        (table spacing, size, central pressure deficit hPa, least core stability K/km, warm-core peak K)."""
+        if not v:
+            self.vortex = None
+            _lib.check(_lib.lib().obl_set_vortex(self.h, None, 0), self.h)
+            return None
+        tab, info = vortex_table(self.nz, self.dz, self.dx, self.params["lat"], self.base["B"], v)
+        _lib.check(_lib.lib().obl_set_vortex(self.h, _ptr(tab), info["nr"]), self.h)
+        self.vortex = info
+        self._vx = tab
+        return info
 
     def baseWind(self, z):
         k = min(max(int(math.floor(z / self.dz)), 0), self.nz - 1)
@@ -217,12 +319,15 @@ class RegionalModel:
         if self.lsf_due():
             flags |= _lib.STEP_LSF
             self.lsfT = self.time
+        if getattr(self, "hur", None) and self.filtNow:
+            flags |= _lib.STEP_FILT
         self._u_last = u
         _lib.check(_lib.lib().obl_step(self.h, _ptr(u), self.parity, flags), self.h)
         self.parity = 1 - self.parity
         self.time += dt
         self.steps += 1
-        self.phiScale = 1
+        cn_old = CN_OLD if getattr(self, "cnOld", None) is None else self.cnOld
+        self.phiScale = 1 / (1 - cn_old) if self.cnNow else 1
 
     def step(self, dt, judge=False):
         self.encStep(None, dt, judge)

@@ -144,6 +144,62 @@ class Oracle:
         self._f("proj")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(phi)), P(Vo))
         return Vo
 
+    # ---- hurricane mode ----
+    @staticmethod
+    def f32(x):
+        return None if x is None else np.ascontiguousarray(x, np.float32)
+
+    def init2(self, lsfA, lsfB, VX, seed=0):
+        V = np.zeros(self.vshape(), self.dt)
+        Q1 = np.zeros(self.cshape(), self.dt)
+        Q2 = np.zeros(self.cshape(), self.dt)
+        f = self._f("init2")
+        f.argtypes = [C.c_void_p] * 8 + [C.c_int]
+        f(P(self.U), P(self.BS), P(self.a(lsfA)), P(self.a(lsfB)), P(self.f32(VX)), P(V), P(Q1), P(Q2), seed)
+        return V, Q1, Q2
+
+    def nut2(self, V, Q1, sfc):
+        aux = np.zeros(self.cshape(2), self.dt)
+        self._f("nut2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.f32(sfc)), P(aux))
+        return aux
+
+    def advv1_2(self, V, Vval):
+        Vt = np.zeros(self.vshape(), self.dt)
+        self._f("advv1_2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Vval)), P(Vt))
+        return Vt
+
+    def advv2_2(self, V, Vval, Vt):
+        Vo = np.zeros(self.vshape(), self.dt)
+        self._f("advv2_2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Vval)), P(self.a(Vt)), P(Vo))
+        return Vo
+
+    def phys2(self, Vin, Q1in, Q2in, aux, lsfA, lsfB, phT, ciMap, sfc, phi):
+        Vo = np.zeros(self.vshape(), self.dt)
+        Q1 = np.zeros(self.cshape(), self.dt)
+        Q2 = np.zeros(self.cshape(), self.dt)
+        self._f("phys2")(P(self.U), P(self.BS), P(self.a(Vin)), P(self.a(Q1in)), P(self.a(Q2in)), P(self.a(aux)),
+                         P(self.a(lsfA)), P(self.a(lsfB)), P(self.a(phT)), P(self.a(ciMap)), P(self.f32(sfc)),
+                         P(None if phi is None else self.a(phi)), P(Vo), P(Q1), P(Q2))
+        return Vo, Q1, Q2
+
+    def maps2(self, V, Q1, Q2, phi, sfc):
+        M = np.zeros((4, self.ny, self.nx, 4), self.dt)
+        self._f("maps2")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.a(Q2)), P(self.a(phi)),
+                         P(self.f32(sfc)), P(M))
+        return M
+
+    def prep(self, V, phi):
+        Vp = np.zeros(self.vshape(), self.dt)
+        self._f("prep")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(phi)), P(Vp))
+        return Vp
+
+    def filt(self, V, Q1, Q2):
+        Vo = np.zeros(self.vshape(), self.dt)
+        Q1o = np.zeros(self.cshape(), self.dt)
+        Q2o = np.zeros(self.cshape(), self.dt)
+        self._f("filt")(P(self.U), P(self.BS), P(self.a(V)), P(self.a(Q1)), P(self.a(Q2)), P(Vo), P(Q1o), P(Q2o))
+        return Vo, Q1o, Q2o
+
     def maps(self, V, Q1, Q2, phi):
         """The four 2-D maps (model.js:1028-1100), shape (4, ny, nx, 4): m0..m3."""
         M = np.zeros((4, self.ny, self.nx, 4), self.dt)

@@ -1,0 +1,153 @@
+"""Hurricane mode on the GPU against the FP32 oracle (model.js:88-119, 284-389, 392-576, 713-774, 895,
+948-1026, 1033-1098).
+
+The 128 x 128 x 40, 2 km hurricane workload (a 302 K sea with land in the east, the app's hurricane settings,
+a 30 m/s balanced warm-core vortex with a moist core, a storm-start tower) after 43 steps. The initial state
+is checked against the oracle's init with the same vortex table; then one filter step (the 2-dx filter, PREP,
+the K-profile nut, the midpoint-trajectory advections carrying Vp, the hurricane physics) and the plain
+hurricane step after it are checked kernel by kernel, each oracle kernel fed the GPU's own inputs, at the
+step's FP32 tolerance (relative L2 <= 1e-5 per channel, theta' RMS <= 1e-4 K); the maps over the sea too.
+"""
+import numpy as np
+import pytest
+
+from _oracle import Oracle, solve as oracle_solve
+from _util import rel_l2, theta_pert
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+THETA_RMS_K = 1e-4
+PHI_TOL = 1e-4
+V_CH = ["u", "v", "w", "theta"]
+Q1_CH = ["qv", "qc", "qi", "qr"]
+Q2_CH = ["qs", "qg", "Nr", "spare"]
+
+
+def _cmp(name, gpu, ref, chans, errs, BS=None, nz=None):
+    for i, c in enumerate(chans):
+        errs[f"{name}.{c}"] = rel_l2(gpu[..., i], ref[..., i])
+        if c == "theta":
+            d = theta_pert(gpu, BS, nz) - theta_pert(ref, BS, nz)
+            errs[f"{name}.theta_rms_K"] = float(np.sqrt(np.mean(d * d)))
+
+
+def _check(errs):
+    bad = {}
+    for k, v in errs.items():
+        lim = THETA_RMS_K if k.endswith("_rms_K") else PHI_TOL if k == "solve" else TOL
+        if not v <= lim:
+            bad[k] = v
+    return bad
+
+
+def test_hurricane_init_matches_oracle():
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_hurricane_128
+
+    m = RegionalModel()
+    setup_hurricane_128(m)
+    o = Oracle(m._u_reset, m._B)
+    A, B, _ = o.lsf(np.zeros(8 * 148, np.float32))
+    V, Q1, Q2 = o.init2(A, B, m._vx)
+    Vg = m.V[0].download()
+    assert np.array_equal(Vg, m.V[1].download())
+    spd = np.hypot(Vg[2, ..., 0], Vg[2, ..., 1])
+    print(f"vortex: max wind {spd.max():.1f} m/s, central pressure deficit {m.vortex['pc']:.1f} hPa")
+    assert spd.max() > 20
+    assert np.max(np.abs(Vg - V)) <= 1e-4
+    assert rel_l2(m.Q1[0].download(), Q1) <= 1e-6
+    m.destroy()
+
+
+@pytest.fixture(scope="module")
+def hurricane():
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_hurricane_128
+
+    m = RegionalModel()
+    dt = setup_hurricane_128(m)
+    for _ in range(43):
+        m.step(dt)
+    m.sync()
+    yield m, dt
+    m.destroy()
+
+
+def _step_kernels(m, dt):
+    p = m.parity
+    V0, Q10, Q20, phi0 = m.V[p].download(), m.Q1[0].download(), m.Q2[0].download(), m.phi.download()
+    sfc = m.sfc.download()
+    m.step(dt)
+    m.sync()
+    filt = m.filtNow
+    U = m._u_last
+    o = Oracle(U, m._B)
+    lsfA, lsfB, phT, ciMap = m.lsfA.download(), m.lsfB.download(), m.phT.download(), m.ciMap.download()
+    errs = {}
+    Vf, Q1f, Q2f = o.filt(V0, Q10, Q20) if filt else (V0, Q10, Q20)
+    Vp = m.Vp.download()
+    _cmp("prep", Vp, o.prep(Vf, phi0), V_CH, errs, m._B, m.nz)
+    aux = m.aux.download()
+    _cmp("nut", aux, o.nut2(Vf, Q1f, sfc), ["nuh", "nuv"], errs)
+    Vt = m.Vt.download()
+    _cmp("advV1", Vt, o.advv1_2(Vf, Vp), V_CH, errs, m._B, m.nz)
+    Q1t, Q2t = m.Q1t.download(), m.Q2t.download()
+    r1, r2 = o.advq1(Vf, Q1f, Q2f)
+    _cmp("advQ1.q1", Q1t, r1, Q1_CH, errs)
+    _cmp("advQ1.q2", Q2t, r2, Q2_CH[:3], errs)
+    Q1a, Q2a = m.Q1[1].download(), m.Q2[1].download()
+    r1, r2 = o.advq2(Vf, Q1f, Q2f, Q1t, Q2t)
+    _cmp("advQ2.q1", Q1a, r1, Q1_CH, errs)
+    _cmp("advQ2.q2", Q2a, r2, Q2_CH[:3], errs)
+    Vadv = o.advv2_2(Vf, Vp, Vt)
+    Vphys_g = m.V[p].download()
+    Vph, Q1p, Q2p = o.phys2(Vadv, Q1a, Q2a, aux, lsfA, lsfB, phT, ciMap, sfc, phi0)
+    _cmp("advV2+phys", Vphys_g, Vph, V_CH, errs, m._B, m.nz)
+    _cmp("phys.q1", m.Q1[0].download(), Q1p, Q1_CH, errs)
+    _cmp("phys.q2", m.Q2[0].download(), Q2p, Q2_CH[:3], errs)
+    rhs_g = m.rhs.download()
+    errs["div"] = rel_l2(rhs_g, o.div(Vphys_g))
+    phi_g = m.phi.download()
+    errs["solve"] = rel_l2(phi_g, oracle_solve(m.nxp, m.nyp, m.nz, m.dx, m.dz, *o.densities(),
+                                                rhs_g.astype(np.float64), "f64"))
+    _cmp("proj", m.V[1 - p].download(), o.proj(Vphys_g, phi_g), V_CH, errs, m._B, m.nz)
+    return filt, U, errs
+
+
+def test_hurricane_filter_step_kernels(hurricane):
+    m, dt = hurricane
+    assert m.steps % 4 == 3
+    filt, U, errs = _step_kernels(m, dt)
+    assert filt and U[75] == 1.0 and U[72] > 0  # the filter, midpoint trajectories, 40 % of the old force
+    for k, v in errs.items():
+        print(f"filter step {k:16s} {v:.3e}")
+    bad = _check(errs)
+    assert not bad, bad
+
+
+def test_hurricane_step_kernels(hurricane):
+    m, dt = hurricane
+    filt, U, errs = _step_kernels(m, dt)
+    assert not filt
+    for k, v in errs.items():
+        print(f"step {k:16s} {v:.3e}")
+    bad = _check(errs)
+    assert not bad, bad
+
+
+def test_hurricane_maps(hurricane):
+    m, dt = hurricane
+    got = m.requestMaps()()
+    G = np.stack([got[f"m{q}"].reshape(m.ny, m.nx, 4) for q in range(4)])
+    V, Q1, Q2 = m.state()
+    o = Oracle(m._u_last, m._B)
+    R = o.maps2(V, Q1, Q2, m.phi.download(), m.sfc.download())
+    for q in range(4):
+        for ch in range(4):
+            if (q, ch) in ((0, 0), (0, 1)):
+                assert np.abs(G[q, ..., ch] - R[q, ..., ch]).max() <= 1e-3
+            else:
+                assert rel_l2(G[q, ..., ch], R[q, ..., ch]) <= 1e-5, (q, ch)
+    # the clear sea's brightness temperature is the SST
+    assert np.isclose(G[3, ..., 1], 302.0, atol=0.5).mean() > 0.2
