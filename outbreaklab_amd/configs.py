@@ -138,17 +138,18 @@ HURRICANE_SETTINGS = {"ck": 1.3e-3, "cdCap": 2.0e-3, "dissHeat": True, "thermalF
 HURRICANE_OVERRIDES = {"qc0": 1e-3, "dampZ": 16500, "dampTau": 300, "mcFloor": 1.0, "evap": 0.10}
 
 
-def setup_hurricane_128(model, seed: int = 0, scheme: str = "lin", land_from_km: float = 200.0):
+def setup_hurricane_128(model, seed: int = 0, scheme: str = "lin", land_from_km: float = 200.0, text=None):
     """128 x 128 x 40, 2 km, hurricane mode: a 302 K sea (land east of ``land_from_km``), the app's hurricane
     settings (environment hold from 100 km), a 30 m/s warm-core vortex (rmw 30 km, r0 120 km, moist core
-    60 km) at the centre, one storm-start tower, lat 20, dt 6 s."""
+    60 km) at the centre, one storm-start tower, lat 20, dt 6 s. ``text``: another SHARPpy sounding instead of
+    the moist, weakly sheared WK one."""
     model.allocate("custom", {"key": "custom:2000:128x128x40", "dx": 2000, "n": [128, 128, 40]})
     model.params.update(HURRICANE_OVERRIDES)
     model.params["scheme"] = scheme
     model.params["noise"] = 0.1
     model.params["lat"] = 20.0
     model.noise_seed = seed
-    model.setSounding(wk_sounding_text(th0=300.0, thtr=343.0, qv0=0.016, shear6=5.0, hodo_r=2.0))
+    model.setSounding(text or wk_sounding_text(th0=300.0, thtr=343.0, qv0=0.016, shear6=5.0, hodo_r=2.0))
     model.setBoundaries([])
     c = [0.5 * model.Lx, 0.5 * model.Ly]
     model.setCI([{"x": c[0] + 40000, "y": c[1], "zc": 2000, "rh": 10000, "rz": 1500, "wt": 8, "t0": 0, "t1": 1800,

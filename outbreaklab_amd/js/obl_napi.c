@@ -30,6 +30,8 @@ typedef struct {
   int (*set_boundaries)(obl_model*, const float*, int);
   int (*set_ci)(obl_model*, const float*, int);
   int (*set_surface)(obl_model*, const float*);
+  int (*set_prep_field)(obl_model*, int);
+  int (*set_vortex)(obl_model*, const float*, int);
   int (*reset)(obl_model*, const float*, int);
   int (*step)(obl_model*, const float*, int, unsigned);
   int (*maps_request)(obl_model*, const float*, int, int*);
@@ -107,6 +109,8 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(set_boundaries, "obl_set_boundaries");
   SYM(set_ci, "obl_set_ci");
   SYM(set_surface, "obl_set_surface");
+  SYM(set_prep_field, "obl_set_prep_field");
+  SYM(set_vortex, "obl_set_vortex");
   SYM(reset, "obl_reset");
   SYM(step, "obl_step");
   SYM(maps_request, "obl_maps_request");
@@ -267,6 +271,34 @@ static napi_value js_set_ci(napi_env env, napi_callback_info info) {
   int count = (int)num(env, argv[2]);
   if (count && (!f || b < (size_t)count * OBL_CI_FLOATS * 4)) return throw_msg(env, "setCI: short array");
   return ok_or_throw(env, A.set_ci(h, f, count), h);
+}
+
+/* setSurface(h, Float32Array(4 nx ny) | null) */
+static napi_value js_set_surface(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* f = tarr(env, argv[1], &b);
+  return ok_or_throw(env, A.set_surface(h, f), h);
+}
+/* setPrepField(h, on) */
+static napi_value js_set_prep_field(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_model* h = ext(env, argv[0]);
+  return ok_or_throw(env, A.set_prep_field(h, num(env, argv[1]) != 0.0 ? 1 : 0), h);
+}
+/* setVortex(h, Float32Array(4 nz nr) | null, nr) */
+static napi_value js_set_vortex(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  obl_model* h = ext(env, argv[0]);
+  size_t b;
+  float* f = tarr(env, argv[1], &b);
+  int nr = (int)num(env, argv[2]);
+  if (f && b < (size_t)nr * 16) return throw_msg(env, "setVortex: short table");
+  return ok_or_throw(env, A.set_vortex(h, f, f ? nr : 0), h);
 }
 
 static napi_value js_reset(napi_env env, napi_callback_info info) {
@@ -726,6 +758,9 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("setBoundaries", js_set_boundaries);
   EXPORT("setCI", js_set_ci);
   EXPORT("reset", js_reset);
+  EXPORT("setSurface", js_set_surface);
+  EXPORT("setPrepField", js_set_prep_field);
+  EXPORT("setVortex", js_set_vortex);
   EXPORT("step", js_step);
   EXPORT("mapsRequest", js_maps_request);
   EXPORT("mapsCollect", js_maps_collect);

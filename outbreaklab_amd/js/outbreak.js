@@ -23,10 +23,12 @@ const DEG = Math.PI / 180;
 const clamp = (v, a, b) => Math.min(Math.max(v, a), b);
 const lerp = (a, b, t) => a + (b - a) * t;
 const dflt = (v, d) => (v === undefined || v === null ? d : v);
-const FIELD = { V: 0, Q1: 1, Q2: 2, Vt: 3, Q1t: 4, Q2t: 5, aux: 6, phT: 7, lsfA: 8, lsfB: 9, ciMap: 10, phi: 11, rhs: 12, bs: 13 };
+const FIELD = { V: 0, Q1: 1, Q2: 2, Vt: 3, Q1t: 4, Q2t: 5, aux: 6, phT: 7, lsfA: 8, lsfB: 9, ciMap: 10, phi: 11, rhs: 12, bs: 13, sfc: 14, Vp: 15 };
 const GEMM = { fp32: 0, bf16x3: 1 };
 const THOMAS = { fp32: 0, fp64: 1, mixed: 2 };
-const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256;
+const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256, VX_MAXR = 2048;
+const CN_OLD = 0.4;  // hurricane mode: the share of the last step's pressure force applied before advection
+const smoothstep = (a, b, x) => { const t = clamp((x - a) / (b - a), 0, 1); return t * t * (3 - 2 * t); };
 
 // ── the uniform block: MODEL_FIELDS (model.js:23-37), UBlock (util.js:83-98) ──
 const MODEL_FIELDS = ['dims', 'dzv', 'step', 'sponge', 'turb', 'micro', 'misc', 'cor', 'ci', 'pad', 'micro2',
@@ -213,6 +215,48 @@ class Field {
   get ptr() { return addon.fieldPtr(this.m.h, FIELD[this.name], this.copy); }
 }
 
+// setVortex's table (model.js:1318-1365): v(r, z) = Vs(r) F(z), Vs a Rankine-type profile made inertially stable,
+// phi from the gradient-wind balance, theta' = theta0 (d phi / dz) / g; the same arithmetic as the Python host
+function vortexTable(nz, dz, dx, lat, B, v) {
+  const f = 2 * 7.292e-5 * Math.sin(lat * DEG), dr = Math.max(500, dx / 2), nr = Math.min(VX_MAXR, Math.ceil(v.r0 * 2 / dr) + 3);
+  const sg = f < 0 ? -1 : 1, fa = Math.abs(f), al = 0.6, vs = new Float64Array(nr);
+  let Mp = 0;
+  for (let i = 0; i < nr; i++) {
+    const r = i * dr, x = r / v.rmw;
+    let s = v.vmax * (1 + al) * x / (al + Math.pow(x, 1 + al)) * (1 - smoothstep(0.5 * v.r0, v.r0, r));
+    if (r > 0) s = Math.max(s, (Mp - 0.5 * fa * r * r) / r, 0);
+    vs[i] = s; Mp = r * s + 0.5 * fa * r * r;
+  }
+  const Vs = r => sg * vs[Math.min(nr - 1, Math.round(r / dr))];
+  const Fz = v.shape === 'mid' ? z => Math.exp(-Math.pow((z - 5000) / 2500, 2))
+    : z => (z >= v.ztop ? 0 : 1 - (z / v.ztop - Math.sin(2 * Math.PI * z / v.ztop) / (2 * Math.PI)));
+  const vt = [], ph = [];
+  for (let k = 0; k < nz; k++) {
+    const F = Fz((k + 0.5) * dz), vk = new Float64Array(nr), pk = new Float64Array(nr);
+    for (let i = 0; i < nr; i++) vk[i] = Vs(i * dr) * F;
+    const g = i => (i === 0 ? 0 : vk[i] * vk[i] / (i * dr) + f * vk[i]);
+    for (let i = nr - 2; i >= 0; i--) pk[i] = pk[i + 1] - 0.5 * (g(i) + g(i + 1)) * dr;
+    vt.push(vk); ph.push(pk);
+  }
+  const tab = new Float32Array(nz * nr * 4);
+  for (let k = 0; k < nz; k++) for (let i = 0; i < nr; i++) {
+    const k0 = Math.max(k - 1, 0), k1 = Math.min(k + 1, nz - 1), b = (ph[k1][i] - ph[k0][i]) / ((k1 - k0) * dz);
+    tab[(k * nr + i) * 4] = vt[k][i]; tab[(k * nr + i) * 4 + 1] = B[k * 8] * b / 9.81;
+  }
+  let minStab = 1e9, warm = 0;
+  for (let k = 1; k < nz; k++) {
+    const z = k * dz;
+    if (z < 2000 || z > v.ztop) continue;
+    for (let i = 0; i < Math.min(nr, Math.ceil(2 * v.rmw / dr)); i++) {
+      const a = B[(k - 1) * 8] + tab[((k - 1) * nr + i) * 4 + 1], b = B[k * 8] + tab[(k * nr + i) * 4 + 1];
+      minStab = Math.min(minStab, (b - a) / dz * 1000);
+    }
+    warm = Math.max(warm, tab[(k * nr) * 4 + 1]);
+  }
+  const info = { x: v.x, y: v.y, dr, nr, moistR: v.moistR || 1, moistRH: v.moistR ? v.moistRH : 0, pc: ph[0][0] * B[2] / 100, minStab, warm };
+  return { tab, info };
+}
+
 class RegionalModel {
   constructor() {
     this.U = new UBlock(MODEL_FIELDS);
@@ -270,15 +314,29 @@ class RegionalModel {
     addon.setBaseState(this.h, bsx.B, this.nz);
     return { snd, windAt: Sounding.winds(snd) };
   }
-  setPrepField(on) { if (on) throw new Error('the trajectory-averaged pressure force (hurricane mode) is not supported on this GPU path'); }
-  setSurface(surf, hur) {
-    if (surf || hur) throw new Error('hurricane mode is not supported on this GPU path');
-    this.hur = null;
+  // the extra wind field the trajectory-averaged pressure force needs (hurricane runs only)
+  setPrepField(on) {
+    addon.setPrepField(this.h, on ? 1 : 0);
+    this.prep = !!on;
+    this.Vp = on ? new Field(this, 'Vp', 0, (this.nx + 1) * (this.ny + 1) * (this.nz + 1) * 4) : null;
   }
+  // hurricane mode. surf: Float32Array(nx*ny*4) = (land fraction, SST K, 0, 0) per column (null = no sea);
+  // hur: {ck, landMoist, sfcP Pa, tauRad s, radCap K/s, blDepth m, u10f, gust m/s, ...} or null
+  setSurface(surf, hur) {
+    const a = surf ? Float32Array.from(surf) : (hur ? new Float32Array(this.nx * this.ny * 4) : null);
+    if (a && a.length !== this.nx * this.ny * 4) throw new Error('setSurface: ' + this.nx * this.ny * 4 + ' floats expected');
+    addon.setSurface(this.h, a);
+    this.hur = hur || null;
+    this.setPrepField(!!hur);
+  }
+  // the initial warm-core vortex: v = {x, y (model m), vmax m/s, rmw m, r0 m, ztop m, moistR m, moistRH, shape};
+  // null = none. The (tangential wind, theta') table is built here and handed to the library for reset.
   setVortex(v) {
-    if (v) throw new Error('the initial vortex (hurricane mode) is not supported on this GPU path');
-    this.vortex = null;
-    return null;
+    if (!v) { this.vortex = null; addon.setVortex(this.h, null, 0); return null; }
+    const r = vortexTable(this.nz, this.dz, this.dx, this.params.lat, this.base.B, v);
+    addon.setVortex(this.h, r.tab, r.info.nr);
+    this.vortex = r.info;
+    return this.vortex;
   }
   baseWind(z) { const k = clamp(Math.floor(z / this.dz), 0, this.nz - 1), B = this.base.B; return [B[k * 8 + 6], B[k * 8 + 7]]; }
   setBoundaries(bnds) {
@@ -310,9 +368,26 @@ class RegionalModel {
     U.set('cor', 2 * 7.292e-5 * Math.sin(P.lat * DEG), P.nudgeTau, this.bounds.length, dflt(P.moistMix, 0));
     U.set('ci', this.ciActive || 0, 0, 0, 0);
     U.set('micro2', 0, dflt(P.sortQ, 5e-4), P.scheme === 'thompson' ? 1 : 0, dflt(P.evapT, 0.3));
-    U.set('hur', 0, 0, 0, 0); U.set('hur2', 1, 0, 1, 0.8); U.set('hur3', 0, 1, 0, 0); U.set('hur4', 0, 0, 1, 1e9); U.set('hur5', 2.4e-3, 0, 0, 0);
-    U.set('cn', 0, this.phiScale || 1, 1, 0);
-    U.set('vtx', 0, 0, 1, 0); U.set('vtx2', 1, 0, 0, 0);
+    const H = this.hur;
+    if (H) {
+      U.set('hur', 1, H.ck, H.landMoist, H.sfcP);
+      U.set('hur2', H.tauRad, H.radCap, H.blDepth, H.u10f);
+      U.set('hur3', Math.pow(0.41 / Math.log(0.5 * this.dz / 0.3 + 1), 2), H.gust, dflt(H.thermals, 0) * dflt(H.thermalScale, 1), H.maritime ? 1 : 0);
+      const f4 = dflt(H.filter4, 0.003);
+      this.filtNow = this.steps % 4 === 3 && f4 > 0;
+      U.set('hur5', dflt(H.cdCap, 2.4e-3), H.dissHeat ? 1 : 0, 4 * f4, 0);
+      const C = H.centre || [this.Lx / 2, this.Ly / 2];
+      U.set('hur4', C[0], C[1], H.thermalR || 200000, H.envR || 400000);
+    } else {
+      this.filtNow = false;
+      U.set('hur', 0, 0, 0, 0); U.set('hur2', 1, 0, 1, 0.8); U.set('hur3', 0, 1, 0, 0); U.set('hur4', 0, 0, 1, 1e9); U.set('hur5', 2.4e-3, 0, 0, 0);
+    }
+    const cnOld = dflt(this.cnOld, CN_OLD), cn = !!H && !!this.prep && this.steps > 0 && cnOld > 0;
+    this.cnNow = cn;
+    U.set('cn', cn ? cnOld : 0, this.phiScale || 1, cn ? 1 / (1 - cnOld) : 1, H && dflt(this.traj2, true) ? 1 : 0);
+    const X = this.vortex;
+    if (X) { U.set('vtx', X.x, X.y, X.dr, X.nr); U.set('vtx2', X.moistR, X.moistRH, 1, 0); }
+    else { U.set('vtx', 0, 0, 1, 0); U.set('vtx2', 1, 0, 0, 0); }
     U.set('dzv', this.dz, 1 / this.dx, 1 / this.dz, dflt(P.mcFloor, 0.25));
     return U.f;
   }
@@ -329,10 +404,11 @@ class RegionalModel {
     const u = this.writeUniforms(dt);
     let flags = judge ? 2 : 0;
     if (this.time - this.lsfT >= 60 || this.time === 0) { flags |= 1; this.lsfT = this.time; }
+    if (this.hur && this.filtNow) flags |= 4;  // the 2-dx filter first (hurricane mode, every 4th step)
     addon.step(this.h, u, this.parity, flags);
     this.parity = 1 - this.parity;
     this.time += dt; this.steps++;
-    this.phiScale = 1;
+    this.phiScale = this.cnNow ? 1 / (1 - dflt(this.cnOld, CN_OLD)) : 1;
   }
   setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'fp32'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
   sync() { return addon.sync(this.h); }
@@ -621,4 +697,4 @@ function volumeBody(header, bytes) {
   return zlib.gzipSync(Buffer.concat([pre, hj, Buffer.from(bytes.buffer, bytes.byteOffset, bytes.byteLength)]));
 }
 
-module.exports = { volumeBody, RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };
+module.exports = { vortexTable, volumeBody, RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

@@ -34,7 +34,15 @@ const geo = new O.Geo(35.9, -97.0, m.Lx, m.Ly, 300);
 const ll = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3);
 const R = O.RadarSim.plan(m, geo, { id: 'KTST', lat: ll[0], lon: ll[1], h: 350 }, { vcp: 212, seed: 7 });
 R.vol = 3; R.az0 = R.rand();
+// hurricane mode on the same model: the uniform block after three steps, and the vortex table
+Object.assign(m, { hur: { ck: 1.3e-3, cdCap: 2.0e-3, dissHeat: true, landMoist: 0.3, sfcP: 101000, tauRad: 43200, radCap: 2 / 86400,
+  blDepth: 1000, u10f: 0.8, gust: 1.5, thermals: 0.4 / 300, maritime: true, centre: [128000, 128000], thermalR: 200000, envR: 100000 },
+  prep: true, steps: 3, phiScale: 1 / 0.6, base: wk });
+m.params.lat = 20;
+const hurU = Array.from(m.writeUniforms(6));
+const vx = O.vortexTable(40, 500, 2000, 20, wk.B, { x: 128000, y: 128000, vmax: 30, rmw: 30000, r0: 120000, ztop: 16000, moistR: 60000, moistRH: 0.9 });
+const vortex = { info: vx.info, tab: Array.from(vx.tab) };
 const radar = { ll, xy: geo.fwd(36.4, -96.1), phase: R.phase, az0: R.az0, bytes: R.bytes, ngRef: R.ngRef, ngDop: R.ngDop,
   t0: R.cuts.map(c => c.t0), block: Array.from(O.RadarSim.block(m, geo, R, R.cuts[5], 123.5, 2)) };
-console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr, targets, rng, radar,
+console.log(JSON.stringify({ exports: Object.keys(O.addon).sort(), mismatches, uniforms: u, poissonError: perr, targets, rng, radar, hurU, vortex,
   levels: [O.levelsFor(2000), O.levelsFor(500), O.levelsFor(250)], pads: [O.gridPad(2000), O.gridPad(500), O.gridPad(250)] }));

@@ -2,7 +2,8 @@
 948-1026, 1033-1098).
 
 The 128 x 128 x 40, 2 km hurricane workload (a 302 K sea with land in the east, the app's hurricane settings,
-a 30 m/s balanced warm-core vortex with a moist core, a storm-start tower) after 43 steps. The initial state
+a 30 m/s balanced warm-core vortex with a moist core, a storm-start tower) after 603 steps (1 h), when it
+has cloud, rain and ice. The initial state
 is checked against the oracle's init with the same vortex table; then one filter step (the 2-dx filter, PREP,
 the K-profile nut, the midpoint-trajectory advections carrying Vp, the hurricane physics) and the plain
 hurricane step after it are checked kernel by kernel, each oracle kernel fed the GPU's own inputs, at the
@@ -67,9 +68,13 @@ def hurricane():
 
     m = RegionalModel()
     dt = setup_hurricane_128(m)
-    for _ in range(43):
+    for _ in range(603):
         m.step(dt)
     m.sync()
+    Q1, Q2 = m.Q1[0].download(), m.Q2[0].download()
+    print(f"condensate maxima: qc {Q1[..., 1].max():.2e} qi {Q1[..., 2].max():.2e} qr {Q1[..., 3].max():.2e} "
+          f"qs {Q2[..., 0].max():.2e} qg {Q2[..., 1].max():.2e}")
+    assert Q1[..., 1].max() > 1e-4 and Q1[..., 3].max() > 1e-5, "the hurricane workload should have cloud and rain"
     yield m, dt
     m.destroy()
 

@@ -57,3 +57,29 @@ def test_node_facade_steps_equal_python_host(tmp_path):
     assert info["volumes"] == len(done) == 1
     assert np.array_equal(np.fromfile(info["rout"], np.uint8), vol["bytes"])
     assert info["header"] == vol["header"]
+
+
+def test_node_hurricane_run_equals_python_host(tmp_path):
+    """Hurricane mode through the Node host (setSurface, setVortex and its table, the filter schedule, PREP and
+    the phi scale) equals the Python host's run of the same workload."""
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_hurricane_128
+    from _util import rel_l2
+
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])["hinfo"]
+    m = RegionalModel()
+    dt = setup_hurricane_128(m, text=open(os.path.join(ROOT, "tests", "golden", "wk_sounding.txt")).read())
+    for _ in range(9):
+        m.step(dt)
+    V, Q1, Q2 = m.state()
+    m.destroy()
+    assert (info["hparity"], info["hsteps"]) == (m.parity, m.steps) and abs(info["hphiScale"] - m.phiScale) < 1e-12
+    raw = np.fromfile(info["hout"], np.float32)
+    nv, nq = V.size, Q1.size
+    same = np.array_equal(raw[:nv], V.ravel()) and np.array_equal(raw[nv:nv + nq], Q1.ravel())
+    print("bit-identical" if same else "not bit-identical")
+    assert rel_l2(raw[:nv], V.ravel()) <= 1e-6 and rel_l2(raw[nv:nv + nq], Q1.ravel()) <= 1e-5
+    assert rel_l2(raw[nv + nq:], Q2.ravel()) <= 1e-5 or not np.any(Q2)

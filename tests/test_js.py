@@ -29,7 +29,7 @@ def test_facade_loads_and_matches_golden():
     for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState",
                  "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer",
                  "mapsRequest", "mapsCollect", "radarCreate", "radarDestroy", "radarSetTornadoes", "radarCut",
-                 "radarReadback", "radarCollect"]:
+                 "radarReadback", "radarCollect", "setSurface", "setPrepField", "setVortex"]:
         assert name in out["exports"]
     assert out["mismatches"] == 0
     assert out["levels"] == [56, 80, 100] and out["pads"] == [8, 24, 40]
@@ -87,3 +87,30 @@ def test_rng_is_uniform():
     # equidistribution over 16 bins
     h = np.bincount((x * 16).astype(int), minlength=16)
     assert h.min() > 0.95 * 100000 / 16 and h.max() < 1.05 * 100000 / 16
+
+
+def test_facade_hurricane_host_equals_python_host():
+    """Hurricane mode: the JS host packs the same uniform block (hur..hur5, cn, filter schedule) and builds the
+    same vortex table as the Python host (model.js:1318-1365, 1391-1425)."""
+    from outbreaklab_amd.configs import HURRICANE_SETTINGS
+    from outbreaklab_amd.model import vortex_table
+
+    out = _run()
+    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(os.path.join(ROOT, "tests", "golden",
+                                                                                  "wk_sounding.txt")).read())
+    m.time = 42.0
+    m.params["lat"] = 20.0
+    m.hur = dict(HURRICANE_SETTINGS, sfcP=101000.0, centre=[128000.0, 128000.0], envR=100000)
+    m.hur.pop("thermalFade", None)
+    m.prep, m.steps, m.phiScale, m.vortex = True, 3, 1 / 0.6, None
+    u = uniforms(m, 6.0)
+    assert np.array_equal(np.array(out["hurU"], np.float32), u)
+    assert m.filtNow and m.cnNow
+    tab, info = vortex_table(40, 500.0, 2000.0, 20.0, m.B, {"x": 128000.0, "y": 128000.0, "vmax": 30.0,
+                                                            "rmw": 30000.0, "r0": 120000.0, "ztop": 16000.0,
+                                                            "moistR": 60000.0, "moistRH": 0.9})
+    jt = np.array(out["vortex"]["tab"], np.float32).reshape(tab.shape)
+    assert np.abs(jt - tab).max() <= 1e-6 * np.abs(tab).max()
+    for k in ("dr", "nr", "moistR", "moistRH"):
+        assert out["vortex"]["info"][k] == info[k]
+    assert abs(out["vortex"]["info"]["pc"] - info["pc"]) < 1e-9 * abs(info["pc"])
