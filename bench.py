@@ -172,6 +172,39 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     }
 
 
+def hurricane_step(key="h2", steps=20, warmup=8):
+    """The hurricane-mode step (filter every 4th step, PREP, midpoint trajectories, sea fluxes) on the reference's
+    hurricane grid ``key``: ms per step over ``steps`` steps after ``warmup``, sim-s per wall-s, per-kernel times."""
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_hurricane_preset
+
+    m = RegionalModel()
+    try:
+        dt = setup_hurricane_preset(m, key)
+        for _ in range(warmup):
+            m.step(dt)
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.step(dt)
+        m.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        m.profile(True)
+        for _ in range(4):
+            m.step(dt)
+        prof = m.profile_read()
+        m.profile(False)
+        cells = m.nx * m.ny * m.nz
+        return {"workload": f"hurricane {key} {m.nx}x{m.ny}x{m.nz} at {m.dx:.0f} m, dt {dt:.0f} s, 35 m/s vortex, "
+                            f"302 K sea", "ms_per_step": round(ms, 3), "sim_s_per_wall_s": round(dt / (ms * 1e-3), 1),
+                "cell_updates_per_s": round(cells / (ms * 1e-3), 1),
+                "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
+                "note": "filter launches every 4th step (its per-launch time is shown); the hurricane advections "
+                        "read from global memory, one work-item per point"}
+    finally:
+        m.destroy()
+
+
 def radar_volume(m, volumes=2):
     """One VCP 212 volume (17 cuts, six moments) of a radar 60 km west / 40 km south of the domain
     centre, sampled from the current state: the radar kernels' HIP-event time per volume."""
@@ -350,6 +383,13 @@ def main():
         radar = radar_volume(m)
     except Exception as e:  # reported, not fatal to the plain measurement
         radar = {"error": f"{type(e).__name__}: {e}"}
+    hur = None
+    if not args.no_combined:
+        print("# hurricane step (h2) ...", file=sys.stderr, flush=True)
+        try:
+            hur = hurricane_step()
+        except Exception as e:  # reported, not fatal to the plain measurement
+            hur = {"error": f"{type(e).__name__}: {e}"}
     comb = None
     if not args.no_combined:
         print("# combined form (surrogate + split-BF16) on the same grid ...", file=sys.stderr, flush=True)
@@ -392,6 +432,7 @@ def main():
         "solve_alone": solve,
         "maps": maps,
         "radar": radar,
+        "hurricane": hur,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
     }

@@ -164,3 +164,27 @@ def setup_hurricane_128(model, seed: int = 0, scheme: str = "lin", land_from_km:
                      "moistR": 60000.0, "moistRH": 0.9})
     model.reset()
     return 6.0
+
+
+def setup_hurricane_preset(model, key: str = "h2", seed: int = 0, sst: float = 302.0):
+    """A hurricane on one of the reference's hurricane grids (h2: 640 x 640 x 56 at 2 km; model.js:1150-1153):
+    an open 302 K sea, the app's hurricane settings with the environment held beyond 300 km, a 35 m/s balanced
+    warm-core vortex (rmw 40 km, r0 250 km, moist core 100 km) at the centre on a moist, weakly sheared WK
+    sounding, lat 20, dt 6 dx_km s (app.js:1666)."""
+    model.allocate(key)
+    model.params.update(HURRICANE_OVERRIDES)
+    model.params["scheme"] = "lin"
+    model.params["noise"] = 0.1
+    model.params["lat"] = 20.0
+    model.noise_seed = seed
+    model.setSounding(wk_sounding_text(th0=300.0, thtr=343.0, qv0=0.016, shear6=5.0, hodo_r=2.0))
+    model.setBoundaries([])
+    model.setCI([])
+    c = [0.5 * model.Lx, 0.5 * model.Ly]
+    surf = np.zeros((model.ny, model.nx, 4), np.float32)
+    surf[..., 1] = sst
+    model.setSurface(surf, dict(HURRICANE_SETTINGS, sfcP=101000.0, centre=c, envR=300000))
+    model.setVortex({"x": c[0], "y": c[1], "vmax": 35.0, "rmw": 40000.0, "r0": 250000.0, "ztop": 16000.0,
+                     "moistR": 100000.0, "moistRH": 0.9})
+    model.reset()
+    return 6.0 * model.dx / 1000.0
