@@ -1208,7 +1208,60 @@ __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V
 // ====================== hurricane mode: advection, PREP, FILT ======================
 // Hurricane runs trace with the wind at the trajectory's midpoint (cn.w, model.js:420-430, 494-499) and carry the
 // values of Vp (V less part of the last pressure force, WGSL_M_PREP) in the wind advection; ice settles (vtI) and
-// Lin graupel is maritime. These kernels read every field from global memory, one work-item per point.
+// Lin graupel is maritime. These kernels read every field from global memory through inline samplers, one
+// work-item per point.
+// A global-memory accessor whose samplers are inline (the same texels and arithmetic as the out-of-window
+// paths of obl_tile.h), one sample's eight texels loaded together; four-channel samples load float4 texels.
+struct GIn {
+  const float4* g;
+  int W, H, D;
+  __device__ __forceinline__ uint32_t gidx(int i, int j, int k) const {
+    return off3(W, H, clampi(i, 0, W - 1), clampi(j, 0, H - 1), clampi(k, 0, D - 1));
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return ldxc(g, gidx(i, j, k), ch); }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return ldx(g, gidx(i, j, k)); }
+};
+__device__ __forceinline__ float samp1(const GIn& a, float tx, float ty, float tz, int ch) {
+  const float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  const int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
+  const int x0 = clampi(i0, 0, a.W - 1), x1 = clampi(i0 + 1, 0, a.W - 1);
+  const int y0 = clampi(j0, 0, a.H - 1), y1 = clampi(j0 + 1, 0, a.H - 1);
+  const int z0 = clampi(k0, 0, a.D - 1), z1 = clampi(k0 + 1, 0, a.D - 1);
+  const float c000 = ldxc(a.g, off3(a.W, a.H, x0, y0, z0), ch), c100 = ldxc(a.g, off3(a.W, a.H, x1, y0, z0), ch);
+  const float c010 = ldxc(a.g, off3(a.W, a.H, x0, y1, z0), ch), c110 = ldxc(a.g, off3(a.W, a.H, x1, y1, z0), ch);
+  const float c001 = ldxc(a.g, off3(a.W, a.H, x0, y0, z1), ch), c101 = ldxc(a.g, off3(a.W, a.H, x1, y0, z1), ch);
+  const float c011 = ldxc(a.g, off3(a.W, a.H, x0, y1, z1), ch), c111 = ldxc(a.g, off3(a.W, a.H, x1, y1, z1), ch);
+  const float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
+  const float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+__device__ __forceinline__ float4 samp4(const GIn& a, float tx, float ty, float tz) {
+  const float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  const int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
+  const int x0 = clampi(i0, 0, a.W - 1), x1 = clampi(i0 + 1, 0, a.W - 1);
+  const int y0 = clampi(j0, 0, a.H - 1), y1 = clampi(j0 + 1, 0, a.H - 1);
+  const int z0 = clampi(k0, 0, a.D - 1), z1 = clampi(k0 + 1, 0, a.D - 1);
+  const float4 c000 = ldx(a.g, off3(a.W, a.H, x0, y0, z0)), c100 = ldx(a.g, off3(a.W, a.H, x1, y0, z0));
+  const float4 c010 = ldx(a.g, off3(a.W, a.H, x0, y1, z0)), c110 = ldx(a.g, off3(a.W, a.H, x1, y1, z0));
+  const float4 c001 = ldx(a.g, off3(a.W, a.H, x0, y0, z1)), c101 = ldx(a.g, off3(a.W, a.H, x1, y0, z1));
+  const float4 c011 = ldx(a.g, off3(a.W, a.H, x0, y1, z1)), c111 = ldx(a.g, off3(a.W, a.H, x1, y1, z1));
+#define OBL_TRI(ch)                                                                                 \
+  lerpf(lerpf(lerpf(c000.ch, c100.ch, ax), lerpf(c010.ch, c110.ch, ax), ay),                        \
+        lerpf(lerpf(c001.ch, c101.ch, ax), lerpf(c011.ch, c111.ch, ax), ay), az)
+  return make_float4(OBL_TRI(x), OBL_TRI(y), OBL_TRI(z), OBL_TRI(w));
+#undef OBL_TRI
+}
+__device__ __forceinline__ float2 minmax1(const GIn& a, int i0, int j0, int k0, int ch) {
+  float lo = 1e30f, hi = -1e30f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const float v = a.ldc(i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+  return make_float2(lo, hi);
+}
+
 template <class A>
 __device__ __forceinline__ float3 velAt(const A& V, float3 p) { return make_float3(sampU(V, p), sampV(V, p), sampW(V, p)); }
 __device__ __forceinline__ float3 clampX3(const Dims& d, float3 p) {
@@ -1234,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_advv1_h(MU u, const float4* __restrict_
                                                  float4* __restrict__ Vt) {
   HUR_FACE
   if (x > d.nx || y > d.ny || z > d.nz) return;
-  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1};
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv, x, y, z);
   const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
@@ -1250,8 +1303,8 @@ __global__ void __launch_bounds__(256) k_advv2_h(MU u, const float4* __restrict_
                                                  const float4* __restrict__ Vt, float4* __restrict__ Vout) {
   HUR_FACE
   if (x > d.nx || y > d.ny || z > d.nz) return;
-  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
-  const GAcc gt{Vt, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1};
+  const GIn gt{Vt, d.nx + 1, d.ny + 1, d.nz + 1};
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv, x, y, z);
   const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
@@ -1290,17 +1343,18 @@ __global__ void __launch_bounds__(256) k_advq1_h(MU u, const float4* __restrict_
                                                  float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
   HUR_FACE
   if (x >= d.nx || y >= d.ny || z >= d.nz) return;
-  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
   const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
   const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   const uint32_t i = cidx(d, x, y, z);
   const float4 q1 = ldx(Q1, i), q2 = ldx(Q2, i);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
-  float4 a = sampQ(g1, traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid));
+  const float3 t0 = traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid);
+  float4 a = sampQ(g1, t0);
   if (f.i > 0.0f) a.z = sampQ1(g1, traceQM(d, gv, k, p, v, f.i, 1.0f, mid), 2);
   const float r = sampQ1(g1, traceQM(d, gv, k, p, v, f.r, 1.0f, mid), 3);
-  const float4 b = sampQ(g2, traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid));
+  const float4 b = sampQ(g2, t0);
   const float sv = sampQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0);
   const float g = sampQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1);
   const float nr = thm ? sampQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2) : b.z;
@@ -1313,8 +1367,8 @@ __global__ void __launch_bounds__(256) k_advq2_h(MU u, const float4* __restrict_
                                                  float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
   HUR_FACE
   if (x >= d.nx || y >= d.ny || z >= d.nz) return;
-  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr}, g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
-  const GAcc g1t{Q1t, d.nx, d.ny, d.nz, nullptr}, g2t{Q2t, d.nx, d.ny, d.nz, nullptr};
+  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
+  const GIn g1t{Q1t, d.nx, d.ny, d.nz}, g2t{Q2t, d.nx, d.ny, d.nz};
   const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
   const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
@@ -1324,9 +1378,10 @@ __global__ void __launch_bounds__(256) k_advq2_h(MU u, const float4* __restrict_
   const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  const float4 b1 = sampQ(g1t, traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid));
+  const float3 f0 = traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid);
+  const float4 b1 = sampQ(g1t, f0);
   const float br = sampQ1(g1t, traceQM(d, gv, k, p, v, f.r, -1.0f, mid), 3);
-  const float4 b2 = sampQ(g2t, traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid));
+  const float4 b2 = sampQ(g2t, f0);
   const float bs = sampQ1(g2t, traceQM(d, gv, k, p, v, f.s, -1.0f, mid), 0);
   const float bg = sampQ1(g2t, traceQM(d, gv, k, p, v, f.g, -1.0f, mid), 1);
   const float bn = sampQ1(g2t, traceQM(d, gv, k, p, v, f.n, -1.0f, mid), 2);
@@ -1365,7 +1420,7 @@ __global__ void __launch_bounds__(256) k_prep(MU u, const float4* __restrict__ V
   stx(Vp, vi, o);
 }
 // FILT (model.js:948-994): a weak 4th-order horizontal hyperdiffusion, every 4th step at 4x strength
-__device__ __forceinline__ float4 bih4(const GAcc& g, int x, int y, int z) {
+__device__ __forceinline__ float4 bih4(const GIn& g, int x, int y, int z) {
   const float4 c0 = g.ld4(x, y, z);
   const float4 xp = g.ld4(x + 1, y, z), xm = g.ld4(x - 1, y, z), yp = g.ld4(x, y + 1, z), ym = g.ld4(x, y - 1, z);
   const float4 pp = g.ld4(x + 1, y + 1, z), pm = g.ld4(x + 1, y - 1, z), mp = g.ld4(x - 1, y + 1, z), mm = g.ld4(x - 1, y - 1, z);
@@ -1383,13 +1438,13 @@ __global__ void __launch_bounds__(256) k_filt(MU u, const float4* __restrict__ V
   HUR_FACE
   if (x > d.nx || y > d.ny || z > d.nz) return;
   const float a = u.v[U_HUR5].z;
-  const GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1};
   const float4 v = gv.ld4(x, y, z), bv = bih4(gv, x, y, z);
   float4 o = make_float4(v.x - a * bv.x, v.y - a * bv.y, v.z - a * bv.z, v.w - a * bv.w);
   if (z == 0 || z >= d.nz) o.z = 0.0f;  // the lid / ground faces of w stay zero
   stx(Vo, vidx(d, x, y, z), o);
   if (x < d.nx && y < d.ny && z < d.nz) {
-    const GAcc g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+    const GIn g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
     const uint32_t i = cidx(d, x, y, z);
     const float4 q = ldx(Q1, i), bq = bih4(g1, x, y, z), r = ldx(Q2, i), br = bih4(g2, x, y, z);
     stx(Q1o, i, make_float4(fmaxf(q.x - a * bq.x, 0.0f), fmaxf(q.y - a * bq.y, 0.0f), fmaxf(q.z - a * bq.z, 0.0f),
