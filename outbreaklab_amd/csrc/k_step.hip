@@ -861,9 +861,21 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     }
     float Tp = 0.0f, qvp = 0.0f, qcp = 0.0f, qip = 0.0f;  // after the process part (teacher targets)
     if (!handed) {
-      micro_processes(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
-      Tp = T; qvp = qv; qcp = qc; qip = qi;
-      micro_satadj(B.y, T, qv, qc, qi);
+      // Condensate-free air below saturation: every process rate of the block is zero (each is a product with a
+      // zero mixing ratio, or sits behind a qr / qs / qg threshold), and each adjustment iteration's step
+      // (qv - qs) / (...) <= 0 is clamped to -qcl = 0, so the block returns T, qv and the zero condensate
+      // unchanged and only sets Thompson's Nr to 0 (model.js:856-866). The 0.1 % margin keeps the test clear of
+      // the adjustment's own saturation value. Whole waves of clear air skip the block's arithmetic.
+      const bool clear = qc == 0.0f && qi == 0.0f && qr == 0.0f && qs == 0.0f && qg == 0.0f &&
+                         qv < 0.999f * qvsat(T, B.y, fIce(T));
+      if (clear) {
+        if (thm) nr = 0.0f;
+        Tp = T; qvp = qv; qcp = qc; qip = qi;
+      } else {
+        micro_processes(u, dt, B.y, A.z, thm, T, qv, qc, qi, qr, qs, qg, nr);
+        Tp = T; qvp = qv; qcp = qc; qip = qi;
+        micro_satadj(B.y, T, qv, qc, qi);
+      }
     }
     if (MODE == PHYS_DUMP) {
       // teacher pairs of the microphysics block: stratified sample, points where the block is

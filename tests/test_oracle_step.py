@@ -229,6 +229,41 @@ def test_saturation_adjustment_reaches_saturation():
     assert np.array_equal(o.micro_point(k, st), st)
 
 
+@pytest.mark.parametrize("scheme", ["lin", "thompson"])
+def test_microphysics_is_the_identity_in_clear_subsaturated_air(scheme):
+    """The block leaves a condensate-free point below saturation unchanged but for Thompson's Nr (set to 0) and
+    theta's round trip through T: the property the physics kernel's clear-air branch relies on. Points range from
+    warm to far below freezing and from dry to 0.999 of saturation (the kernel's margin)."""
+    m, U = small(scheme=scheme)
+    o = Oracle(U, m.B, "f32")
+    B = m.B.astype(np.float32).reshape(-1, 8)
+    rng = np.random.default_rng(11)
+    f32 = np.float32
+    n = 0
+    for k in range(B.shape[0] - 1):
+        pi0, p0 = B[k, 4], B[k, 5]
+        for T in rng.uniform(200.0, 310.0, 40):
+            f = min(max((273.15 - T) / 40.0, 0.0), 1.0)
+            esl = 611.2 * math.exp(17.67 * (T - 273.15) / (T - 29.65))
+            esi = 611.2 * math.exp(21.8745584 * (T - 273.16) / (T - 7.66))
+            e = esl + (esi - esl) * f
+            qs = 0.622 * e / max(p0 - e, 100.0)
+            qv = qs * rng.uniform(0.0, 0.997)
+            th = f32(f32(T) / pi0)
+            st = np.array([th, qv, 0, 0, 0, 0, 0, rng.uniform(0, 1e5)], np.float32)
+            out = o.micro_point(k, st)
+            assert out[0] == f32(f32(th * pi0) / pi0)
+            assert np.array_equal(out[1:7], st[1:7])
+            assert out[7] == (0.0 if scheme == "thompson" else st[7])
+            n += 1
+    assert n >= 400
+    # a point with a trace of any one species, or at saturation, is not left alone
+    k, T = 4, 285.0
+    pi0 = B[k, 4]
+    st = np.array([T / pi0, 0.03, 0, 0, 0, 0, 0, 0], np.float32)
+    assert not np.array_equal(o.micro_point(k, st)[1:3], st[1:3])
+
+
 def test_microphysics_warm_branches():
     m, U = small(dt=6.0)
     o = Oracle(U, m.B, "f64")
