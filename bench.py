@@ -200,7 +200,7 @@ def hurricane_step(key="h2", steps=20, warmup=8):
                 "cell_updates_per_s": round(cells / (ms * 1e-3), 1),
                 "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()},
                 "note": "filter launches every 4th step (its per-launch time is shown); the hurricane advections "
-                        "read from global memory, one work-item per point"}
+                        "walk z over LDS rings, one block per 32x8 column tile"}
     finally:
         m.destroy()
 

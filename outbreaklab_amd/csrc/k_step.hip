@@ -1220,10 +1220,8 @@ __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V
 // ====================== hurricane mode: advection, PREP, FILT ======================
 // Hurricane runs trace with the wind at the trajectory's midpoint (cn.w, model.js:420-430, 494-499) and carry the
 // values of Vp (V less part of the last pressure force, WGSL_M_PREP) in the wind advection; ice settles (vtI) and
-// Lin graupel is maritime. These kernels read every field from global memory through inline samplers, one
-// work-item per point.
-// A global-memory accessor whose samplers are inline (the same texels and arithmetic as the out-of-window
-// paths of obl_tile.h), one sample's eight texels loaded together; four-channel samples load float4 texels.
+// Lin graupel is maritime. The advections walk z over LDS rings like the outbreak ones; PREP and FILT are face
+// kernels over global memory (GIn, clamped texel reads).
 struct GIn {
   const float4* g;
   int W, H, D;
@@ -1233,46 +1231,6 @@ struct GIn {
   __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return ldxc(g, gidx(i, j, k), ch); }
   __device__ __forceinline__ float4 ld4(int i, int j, int k) const { return ldx(g, gidx(i, j, k)); }
 };
-__device__ __forceinline__ float samp1(const GIn& a, float tx, float ty, float tz, int ch) {
-  const float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
-  const int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
-  const int x0 = clampi(i0, 0, a.W - 1), x1 = clampi(i0 + 1, 0, a.W - 1);
-  const int y0 = clampi(j0, 0, a.H - 1), y1 = clampi(j0 + 1, 0, a.H - 1);
-  const int z0 = clampi(k0, 0, a.D - 1), z1 = clampi(k0 + 1, 0, a.D - 1);
-  const float c000 = ldxc(a.g, off3(a.W, a.H, x0, y0, z0), ch), c100 = ldxc(a.g, off3(a.W, a.H, x1, y0, z0), ch);
-  const float c010 = ldxc(a.g, off3(a.W, a.H, x0, y1, z0), ch), c110 = ldxc(a.g, off3(a.W, a.H, x1, y1, z0), ch);
-  const float c001 = ldxc(a.g, off3(a.W, a.H, x0, y0, z1), ch), c101 = ldxc(a.g, off3(a.W, a.H, x1, y0, z1), ch);
-  const float c011 = ldxc(a.g, off3(a.W, a.H, x0, y1, z1), ch), c111 = ldxc(a.g, off3(a.W, a.H, x1, y1, z1), ch);
-  const float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
-  const float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
-  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
-}
-__device__ __forceinline__ float4 samp4(const GIn& a, float tx, float ty, float tz) {
-  const float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
-  const int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
-  const int x0 = clampi(i0, 0, a.W - 1), x1 = clampi(i0 + 1, 0, a.W - 1);
-  const int y0 = clampi(j0, 0, a.H - 1), y1 = clampi(j0 + 1, 0, a.H - 1);
-  const int z0 = clampi(k0, 0, a.D - 1), z1 = clampi(k0 + 1, 0, a.D - 1);
-  const float4 c000 = ldx(a.g, off3(a.W, a.H, x0, y0, z0)), c100 = ldx(a.g, off3(a.W, a.H, x1, y0, z0));
-  const float4 c010 = ldx(a.g, off3(a.W, a.H, x0, y1, z0)), c110 = ldx(a.g, off3(a.W, a.H, x1, y1, z0));
-  const float4 c001 = ldx(a.g, off3(a.W, a.H, x0, y0, z1)), c101 = ldx(a.g, off3(a.W, a.H, x1, y0, z1));
-  const float4 c011 = ldx(a.g, off3(a.W, a.H, x0, y1, z1)), c111 = ldx(a.g, off3(a.W, a.H, x1, y1, z1));
-#define OBL_TRI(ch)                                                                                 \
-  lerpf(lerpf(lerpf(c000.ch, c100.ch, ax), lerpf(c010.ch, c110.ch, ax), ay),                        \
-        lerpf(lerpf(c001.ch, c101.ch, ax), lerpf(c011.ch, c111.ch, ax), ay), az)
-  return make_float4(OBL_TRI(x), OBL_TRI(y), OBL_TRI(z), OBL_TRI(w));
-#undef OBL_TRI
-}
-__device__ __forceinline__ float2 minmax1(const GIn& a, int i0, int j0, int k0, int ch) {
-  float lo = 1e30f, hi = -1e30f;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const float v = a.ldc(i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
-    lo = fminf(lo, v);
-    hi = fmaxf(hi, v);
-  }
-  return make_float2(lo, hi);
-}
 
 template <class A>
 __device__ __forceinline__ float3 velAt(const A& V, float3 p) { return make_float3(sampU(V, p), sampV(V, p), sampW(V, p)); }
@@ -1295,11 +1253,13 @@ __device__ __forceinline__ float3 traceQM(const Dims& d, const A& V, float3 k, f
 #define HUR_FACE                                                                                     \
   Dims d = dims(u);                                                                                  \
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
-__global__ void __launch_bounds__(256) k_advv1_h(MU u, const float4* __restrict__ V, const float4* __restrict__ Vval,
-                                                 float4* __restrict__ Vt) {
-  HUR_FACE
-  if (x > d.nx || y > d.ny || z > d.nz) return;
-  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1};
+// The hurricane-mode advections (midpoint trajectories, the prepared winds Vval, settling ice and
+// maritime graupel), written once over accessors: the kernels below walk z with every field they
+// sample held in an LDS ring (obl_tile.h). Samples whose 2 x 2 x 2 box leaves the window read the
+// same texels from global memory, so results do not depend on the window.
+template <class AV, class AQ>
+__device__ __forceinline__ float4 advv1_h_point(const MU& u, const Dims& d, const AV& gv, const AQ& gq, int x, int y,
+                                                int z) {
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv, x, y, z);
   const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
@@ -1309,14 +1269,11 @@ __global__ void __launch_bounds__(256) k_advv1_h(MU u, const float4* __restrict_
   o.y = sampV(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f, mid));
   o.z = sampW(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f, mid));
   o.w = sampC(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f, mid));
-  stx(Vt, vidx(d, x, y, z), o);
+  return o;
 }
-__global__ void __launch_bounds__(256) k_advv2_h(MU u, const float4* __restrict__ V, const float4* __restrict__ Vval,
-                                                 const float4* __restrict__ Vt, float4* __restrict__ Vout) {
-  HUR_FACE
-  if (x > d.nx || y > d.ny || z > d.nz) return;
-  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, gq{Vval, d.nx + 1, d.ny + 1, d.nz + 1};
-  const GIn gt{Vt, d.nx + 1, d.ny + 1, d.nz + 1};
+template <class AV, class AQ, class AT>
+__device__ __forceinline__ float4 advv2_h_point(const MU& u, const Dims& d, const AV& gv, const AQ& gq, const AT& gt,
+                                                int x, int y, int z) {
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv, x, y, z);
   const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
@@ -1339,7 +1296,7 @@ __global__ void __launch_bounds__(256) k_advv2_h(MU u, const float4* __restrict_
   m = mmV(gq, traceM(d, gv, k, pv, L.vv, 1.0f, mid), 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
   m = mmV(gq, traceM(d, gv, k, pw, L.vw, 1.0f, mid), 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
   m = mmV(gq, traceM(d, gv, k, pc, L.vc, 1.0f, mid), 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
-  stx(Vout, vidx(d, x, y, z), r);
+  return r;
 }
 struct FallH {
   float r, n, s, g, i;
@@ -1350,17 +1307,13 @@ __device__ __forceinline__ FallH fall_h(const MU& u, float4 q1, float4 q2, float
   const float2 fv = vtRain(thm, q1.w, q2.z, rho, rs);
   return FallH{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rs), vtGrauM(thm, maritime_of(u), q2.y, rho, rs), vtI(q1.z, rho)};
 }
-__global__ void __launch_bounds__(256) k_advq1_h(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                 const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                                 float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
-  HUR_FACE
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
-  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
+template <class AV, class AQ>
+__device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const float4* BS, const AV& gv, const AQ& g1,
+                                              const AQ& g2, int x, int y, int z, float4& o1, float4& o2) {
   const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
   const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  const uint32_t i = cidx(d, x, y, z);
-  const float4 q1 = ldx(Q1, i), q2 = ldx(Q2, i);
+  const float4 q1 = g1.ld4(x, y, z), q2 = g2.ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
   const float3 t0 = traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid);
   float4 a = sampQ(g1, t0);
@@ -1370,22 +1323,17 @@ __global__ void __launch_bounds__(256) k_advq1_h(MU u, const float4* __restrict_
   const float sv = sampQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0);
   const float g = sampQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1);
   const float nr = thm ? sampQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2) : b.z;
-  stx(Q1t, i, make_float4(a.x, a.y, a.z, r));
-  stx(Q2t, i, make_float4(sv, g, nr, b.w));
+  o1 = make_float4(a.x, a.y, a.z, r);
+  o2 = make_float4(sv, g, nr, b.w);
 }
-__global__ void __launch_bounds__(256) k_advq2_h(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                 const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                                 const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
-                                                 float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
-  HUR_FACE
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
-  const GIn gv{V, d.nx + 1, d.ny + 1, d.nz + 1}, g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
-  const GIn g1t{Q1t, d.nx, d.ny, d.nz}, g2t{Q2t, d.nx, d.ny, d.nz};
-  const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
+template <class AV, class AQ>
+__device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const float4* BS, const AV& gv, const AQ& g1,
+                                              const AQ& g2, const AQ& g1t, const AQ& g2t, int x, int y, int z,
+                                              float4& o1, float4& o2) {
+  const bool mid = u.v[U_CN].w > 0.5f;
   const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  const uint32_t i = cidx(d, x, y, z);
-  const float4 q1 = ldx(Q1, i), q2 = ldx(Q2, i), h1 = ldx(Q1t, i), h2 = ldx(Q2t, i);
+  const float4 q1 = g1.ld4(x, y, z), q2 = g2.ld4(x, y, z), h1 = g1t.ld4(x, y, z), h2 = g2t.ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
   const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
@@ -1412,8 +1360,138 @@ __global__ void __launch_bounds__(256) k_advq2_h(MU u, const float4* __restrict_
   m = mmQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1); r2.y = vclamp_(r2.y, m);
   m = mmQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2); r2.z = vclamp_(r2.z, m);
   m = mmQ1(g2, d0, 3); r2.w = vclamp_(r2.w, m);
-  stx(Q1o, i, make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f)));
-  stx(Q2o, i, make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f)));
+  o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
+  o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
+}
+
+// z-marching drivers (as k_advv1_t ... k_advq2_t). PREP: the carried winds Vval are a field of
+// their own (the prep field), with a ring of their own; otherwise they are V.
+template <bool PREP>
+__global__ void __launch_bounds__(TILE_THREADS) k_advv1_hr(MU u, const float4* __restrict__ V,
+                                                           const float4* __restrict__ Vval, float4* __restrict__ Vt) {
+  __shared__ float lds[(PREP ? 2 : 1) * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rq{lds + (PREP ? RING_FLOATS : 0), Vval, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) {
+    rv.load(k, tid);
+    if (PREP) rq.load(k, tid);
+  }
+  __syncthreads();
+  const bool act = x <= d.nx && y <= d.ny;
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv, pq;
+    const bool more = z < d.nz;
+    if (more) {
+      rv.fetch(z + 2, tid, pv);
+      if (PREP) rq.fetch(z + 2, tid, pq);
+    }
+    rv.center(z);
+    if (PREP) rq.center(z);
+    if (act) stx(Vt, vidx(d, x, y, z), advv1_h_point(u, d, rv, PREP ? rq : rv, x, y, z));
+    __syncthreads();
+    if (more) {
+      rv.store(z + 2, tid, pv);
+      if (PREP) rq.store(z + 2, tid, pq);
+    }
+    __syncthreads();
+  }
+}
+template <bool PREP>
+__global__ void __launch_bounds__(TILE_THREADS) k_advv2_hr(MU u, const float4* __restrict__ V,
+                                                           const float4* __restrict__ Vval, const float4* __restrict__ Vt,
+                                                           float4* __restrict__ Vout) {
+  __shared__ float lds[(PREP ? 3 : 2) * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rt{lds + RING_FLOATS, Vt, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring rq{lds + (PREP ? 2 * RING_FLOATS : 0), Vval, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) {
+    rv.load(k, tid);
+    rt.load(k, tid);
+    if (PREP) rq.load(k, tid);
+  }
+  __syncthreads();
+  const bool act = x <= d.nx && y <= d.ny;
+  for (int z = 0; z <= d.nz; z++) {
+    Ring::Pre pv, pt, pq;
+    const bool more = z < d.nz;
+    if (more) {
+      rv.fetch(z + 2, tid, pv);
+      rt.fetch(z + 2, tid, pt);
+      if (PREP) rq.fetch(z + 2, tid, pq);
+    }
+    rv.center(z);
+    rt.center(z);
+    if (PREP) rq.center(z);
+    if (act) stx(Vout, vidx(d, x, y, z), advv2_h_point(u, d, rv, PREP ? rq : rv, rt, x, y, z));
+    __syncthreads();
+    if (more) {
+      rv.store(z + 2, tid, pv);
+      rt.store(z + 2, tid, pt);
+      if (PREP) rq.store(z + 2, tid, pq);
+    }
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(TILE_THREADS) k_advq1_hr(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                           const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                           float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
+  __shared__ float lds[3 * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring r1{lds + RING_FLOATS, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r2{lds + 2 * RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) { rv.load(k, tid); r1.load(k, tid); r2.load(k, tid); }
+  __syncthreads();
+  const bool act = x < d.nx && y < d.ny;
+  for (int z = 0; z < d.nz; z++) {
+    Ring::Pre pv, p1, p2;
+    rv.fetch(z + 2, tid, pv); r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2);
+    rv.center(z); r1.center(z); r2.center(z);
+    if (act) {
+      float4 o1, o2;
+      advq1_h_point(u, d, BS, rv, r1, r2, x, y, z, o1, o2);
+      const uint32_t i = cidx(d, x, y, z);
+      stx(Q1t, i, o1);
+      stx(Q2t, i, o2);
+    }
+    __syncthreads();
+    rv.store(z + 2, tid, pv); r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2);
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(TILE_THREADS) k_advq2_hr(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                           const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                           const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
+                                                           float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  __shared__ float lds[5 * RING_FLOATS];
+  TILE_SETUP
+  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
+  Ring r1{lds + RING_FLOATS, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r2{lds + 2 * RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r3{lds + 3 * RING_FLOATS, Q1t, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring r4{lds + 4 * RING_FLOATS, Q2t, d.nx, d.ny, d.nz, x0, y0, 0};
+  for (int k = -1; k <= 1; k++) { rv.load(k, tid); r1.load(k, tid); r2.load(k, tid); r3.load(k, tid); r4.load(k, tid); }
+  __syncthreads();
+  const bool act = x < d.nx && y < d.ny;
+  for (int z = 0; z < d.nz; z++) {
+    Ring::Pre pv, p1, p2, p3, p4;
+    rv.fetch(z + 2, tid, pv); r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2);
+    r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4);
+    rv.center(z); r1.center(z); r2.center(z); r3.center(z); r4.center(z);
+    if (act) {
+      float4 o1, o2;
+      advq2_h_point(u, d, BS, rv, r1, r2, r3, r4, x, y, z, o1, o2);
+      const uint32_t i = cidx(d, x, y, z);
+      stx(Q1o, i, o1);
+      stx(Q2o, i, o2);
+    }
+    __syncthreads();
+    rv.store(z + 2, tid, pv); r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2);
+    r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4);
+    __syncthreads();
+  }
 }
 // PREP (model.js:996-1026): the winds the advection carries, V less cn.x of the last pressure force
 __global__ void __launch_bounds__(256) k_prep(MU u, const float4* __restrict__ V, const float* __restrict__ phi,
@@ -1551,19 +1629,21 @@ void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, con
 }
 
 void launch_advv1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, float4* Vt) {
-  hipLaunchKernelGGL(k_advv1_h, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vval, Vt);
+  if (Vval != V) hipLaunchKernelGGL(k_advv1_hr<true>, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vval, Vt);
+  else hipLaunchKernelGGL(k_advv1_hr<false>, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vval, Vt);
 }
 void launch_advv2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vval, const float4* Vt,
                       float4* Vout) {
-  hipLaunchKernelGGL(k_advv2_h, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Vval, Vt, Vout);
+  if (Vval != V) hipLaunchKernelGGL(k_advv2_hr<true>, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vval, Vt, Vout);
+  else hipLaunchKernelGGL(k_advv2_hr<false>, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vval, Vt, Vout);
 }
 void launch_advq1_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                       const float4* Q2, float4* Q1t, float4* Q2t) {
-  hipLaunchKernelGGL(k_advq1_h, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
+  hipLaunchKernelGGL(k_advq1_hr, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
 }
 void launch_advq2_hur(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                       const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o) {
-  hipLaunchKernelGGL(k_advq2_h, cells_grid(d), dim3(64, 4, 1), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
+  hipLaunchKernelGGL(k_advq2_hr, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
 }
 void launch_prep(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vp) {
   hipLaunchKernelGGL(k_prep, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, phi, Vp);
