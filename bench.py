@@ -304,9 +304,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--form", default="plain", choices=["plain", "c5"],
+    ap.add_argument("--form", default="plain", choices=["plain", "c5", "hurricane"],
                     help="plain: the plain FP32 step at 1024x1024x80 (and the combined form on the same grid); "
-                         "c5: the combined form at 1024x1024x100, 250 m")
+                         "c5: the combined form at 1024x1024x100, 250 m; hurricane: only the hurricane-mode step "
+                         "on the h2 preset (640x640x56, 2 km)")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined-form measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM-counter calibration) and exit")
@@ -321,6 +322,9 @@ def main():
         return
     if args.form == "c5":
         bench_c5(args)
+        return
+    if args.form == "hurricane":
+        print(json.dumps({"hurricane": hurricane_step(steps=args.steps, warmup=args.warmup)}))
         return
 
     from outbreaklab_amd import RegionalModel
