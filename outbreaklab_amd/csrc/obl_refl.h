@@ -1,6 +1,6 @@
 // Equivalent radar reflectivity of the model's hydrometeors (gfx950 device functions), shared by
 // the surrogate's judge and the 2-D maps kernel: zRain / zGrau (model.js:94-106) and the dBZ of a
-// point (model.js:1042-1050), outbreak mode (the hurricane-only maritime graupel is not built).
+// point (model.js:1042-1050); `mar` selects hurricane mode's maritime graupel (model.js:101).
 #pragma once
 #include "obl_device.h"
 

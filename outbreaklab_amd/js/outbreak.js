@@ -656,7 +656,7 @@ class RadarSim {
     U.set('mdl', m.nx, m.ny, m.nz, m.dx);
     U.set('mdl2', m.dz, g.groundMSL, -m.Lx / 2, -m.Ly / 2);
     U.set('phys', m.rhoSfc || 1.15, dflt(R.opts.minDbz, 0), m.params.scheme === 'thompson' ? 1 : 0, nTor || 0);
-    U.set('phys2', m.maritime ? 1 : 0, 0, 0, 0);
+    U.set('phys2', m.hur && m.hur.maritime ? 1 : 0, 0, 0, 0);  // radar.js:384
     return U.f;
   }
   encCut(enc, R, c, simT) {

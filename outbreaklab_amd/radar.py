@@ -170,7 +170,8 @@ def cut_block(m, geo, R, c, simT, n_tor=0):
     U.set("mdl", m.nx, m.ny, m.nz, m.dx)
     U.set("mdl2", m.dz, geo.groundMSL, -m.Lx / 2, -m.Ly / 2)
     U.set("phys", m.rhoSfc or 1.15, R["opts"].get("minDbz", 0), 1 if m.params["scheme"] == "thompson" else 0, n_tor)
-    U.set("phys2", 1 if getattr(m, "maritime", False) else 0, 0, 0, 0)
+    hur = getattr(m, "hur", None)
+    U.set("phys2", 1 if hur and hur.get("maritime") else 0, 0, 0, 0)  # radar.js:384
     return U.f.copy()
 
 

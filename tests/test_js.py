@@ -75,7 +75,13 @@ def test_facade_radar_host_equals_python_host():
     assert R["phase"] == rd["phase"] and R["az0"] == rd["az0"]
     assert (R["bytes"], R["ngRef"], R["ngDop"]) == (rd["bytes"], rd["ngRef"], rd["ngDop"])
     assert [c["t0"] for c in R["cuts"]] == rd["t0"]
-    assert np.array_equal(cut_block(m, geo, R, R["cuts"][5], 123.5, 2), np.array(rd["block"], np.float32))
+    # the Node block is taken with hurricane settings on the model: the maritime graupel flag (phys2.x, the
+    # eleventh vec4) follows them, as radar.js:384
+    assert cut_block(m, geo, R, R["cuts"][5], 123.5, 2)[40] == 0.0
+    m.hur = {"maritime": True}
+    blk = cut_block(m, geo, R, R["cuts"][5], 123.5, 2)
+    assert blk[40] == 1.0
+    assert np.array_equal(blk, np.array(rd["block"], np.float32))
 
 
 def test_rng_is_uniform():
