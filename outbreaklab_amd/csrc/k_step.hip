@@ -1254,48 +1254,89 @@ __device__ __forceinline__ float3 traceQM(const Dims& d, const A& V, float3 k, f
   Dims d = dims(u);                                                                                  \
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
 // The hurricane-mode advections (midpoint trajectories, the prepared winds Vval, settling ice and
-// maritime graupel), written once over accessors: the kernels below walk z with every field they
-// sample held in an LDS ring (obl_tile.h). Samples whose 2 x 2 x 2 box leaves the window read the
-// same texels from global memory, so results do not depend on the window.
-template <class AV, class AQ>
-__device__ __forceinline__ float4 advv1_h_point(const MU& u, const Dims& d, const AV& gv, const AQ& gq, int x, int y,
+// maritime graupel) over LDS rings. Each sample reads through the window-guaranteed view (RingW:
+// no window tests, plane offsets selected) when its whole wave's boxes lie within one point of
+// their cells, and through the checked ring (global fallback) otherwise; both read the same texels
+// with the same arithmetic. Two wave votes decide it: one per point on the Courant numbers, which
+// keeps every midpoint sample's box in the window (|m - p| < 1/2), and one per trace on its final
+// point (|t - p| < 1 on each axis).
+__device__ __forceinline__ bool near_box(float3 t, float3 p) {
+  return fabsf(t.x - p.x) < 0.999f && fabsf(t.y - p.y) < 0.999f && fabsf(t.z - p.z) < 0.999f;
+}
+__device__ __forceinline__ float sampU_r(const Ring& r, float3 t, float3 p) {
+  return wave_all(near_box(t, p)) ? sampU(r.w(), t) : sampU(r, t);
+}
+__device__ __forceinline__ float sampV_r(const Ring& r, float3 t, float3 p) {
+  return wave_all(near_box(t, p)) ? sampV(r.w(), t) : sampV(r, t);
+}
+__device__ __forceinline__ float sampW_r(const Ring& r, float3 t, float3 p) {
+  return wave_all(near_box(t, p)) ? sampW(r.w(), t) : sampW(r, t);
+}
+__device__ __forceinline__ float sampC_r(const Ring& r, float3 t, float3 p) {
+  return wave_all(near_box(t, p)) ? sampC(r.w(), t) : sampC(r, t);
+}
+__device__ __forceinline__ float2 mmV_r(const Ring& r, float3 t, float3 p, float ox, float oy, float oz, int ch) {
+  return wave_all(near_box(t, p)) ? mmV(r.w(), t, ox, oy, oz, ch) : mmV(r, t, ox, oy, oz, ch);
+}
+__device__ __forceinline__ float4 sampQ_r(const Ring& r, float3 t, float3 p) {
+  return wave_all(near_box(t, p)) ? sampQ(r.w(), t) : sampQ(r, t);
+}
+__device__ __forceinline__ float sampQ1_r(const Ring& r, float3 t, float3 p, int ch) {
+  return wave_all(near_box(t, p)) ? sampQ1(r.w(), t, ch) : sampQ1(r, t, ch);
+}
+__device__ __forceinline__ float2 mmQ1_r(const Ring& r, float3 t, float3 p, int ch) {
+  return wave_all(near_box(t, p)) ? mmQ1(r.w(), t, ch) : mmQ1(r, t, ch);
+}
+// the trace of one point: the midpoint's wind from the window view when the point's wave is near
+__device__ __forceinline__ float3 traceM_r(const Dims& d, const Ring& V, bool nearA, float3 k, float3 p, float3 v0,
+                                           float s, bool mid) {
+  return nearA ? traceM(d, V.w(), k, p, v0, s, mid) : traceM(d, V, k, p, v0, s, mid);
+}
+__device__ __forceinline__ float3 traceQM_r(const Dims& d, const Ring& V, bool nearA, float3 k, float3 p, float3 v0,
+                                            float fall, float s, bool mid) {
+  return nearA ? traceQM(d, V.w(), k, p, v0, fall, s, mid) : traceQM(d, V, k, p, v0, fall, s, mid);
+}
+__device__ __forceinline__ float4 advv1_h_point(const MU& u, const Dims& d, const Ring& gv, const Ring& gq, int x, int y,
                                                 int z) {
   const bool mid = u.v[U_CN].w > 0.5f;
-  const Loc L = localVel(gv, x, y, z);
-  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  const float fx = (float)x, fy = (float)y, fz = (float)z;
-  float4 o;
-  o.x = sampU(gq, traceM(d, gv, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f, mid));
-  o.y = sampV(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f, mid));
-  o.z = sampW(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f, mid));
-  o.w = sampC(gq, traceM(d, gv, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f, mid));
-  return o;
-}
-template <class AV, class AQ, class AT>
-__device__ __forceinline__ float4 advv2_h_point(const MU& u, const Dims& d, const AV& gv, const AQ& gq, const AT& gt,
-                                                int x, int y, int z) {
-  const bool mid = u.v[U_CN].w > 0.5f;
-  const Loc L = localVel(gv, x, y, z);
+  const Loc L = localVel(gv.w(), x, y, z);
+  const bool nA = wave_all(near_traces(d, L));
   const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float fx = (float)x, fy = (float)y, fz = (float)z;
   const float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
   const float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
-  const float4 hat = gt.ld4(x, y, z), old = gq.ld4(x, y, z);
+  float4 o;
+  o.x = sampU_r(gq, traceM_r(d, gv, nA, k, pu, L.vu, 1.0f, mid), pu);
+  o.y = sampV_r(gq, traceM_r(d, gv, nA, k, pv, L.vv, 1.0f, mid), pv);
+  o.z = sampW_r(gq, traceM_r(d, gv, nA, k, pw, L.vw, 1.0f, mid), pw);
+  o.w = sampC_r(gq, traceM_r(d, gv, nA, k, pc, L.vc, 1.0f, mid), pc);
+  return o;
+}
+__device__ __forceinline__ float4 advv2_h_point(const MU& u, const Dims& d, const Ring& gv, const Ring& gq, const Ring& gt,
+                                                int x, int y, int z) {
+  const bool mid = u.v[U_CN].w > 0.5f;
+  const Loc L = localVel(gv.w(), x, y, z);
+  const bool nA = wave_all(near_traces(d, L));
+  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  const float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
+  const float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
+  const float4 hat = gt.w().ld4(x, y, z), old = gq.w().ld4(x, y, z);
   float4 b;
-  b.x = sampU(gt, traceM(d, gv, k, pu, L.vu, -1.0f, mid));
-  b.y = sampV(gt, traceM(d, gv, k, pv, L.vv, -1.0f, mid));
-  b.z = sampW(gt, traceM(d, gv, k, pw, L.vw, -1.0f, mid));
-  b.w = sampC(gt, traceM(d, gv, k, pc, L.vc, -1.0f, mid));
+  b.x = sampU_r(gt, traceM_r(d, gv, nA, k, pu, L.vu, -1.0f, mid), pu);
+  b.y = sampV_r(gt, traceM_r(d, gv, nA, k, pv, L.vv, -1.0f, mid), pv);
+  b.z = sampW_r(gt, traceM_r(d, gv, nA, k, pw, L.vw, -1.0f, mid), pw);
+  b.w = sampC_r(gt, traceM_r(d, gv, nA, k, pc, L.vc, -1.0f, mid), pc);
   const float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
   const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
   float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
                          hat.w + sm * (old.w - b.w));
   float2 m;
-  m = mmV(gq, traceM(d, gv, k, pu, L.vu, 1.0f, mid), 0.0f, 0.5f, 0.5f, 0); r.x = vclamp_(r.x, m);
-  m = mmV(gq, traceM(d, gv, k, pv, L.vv, 1.0f, mid), 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
-  m = mmV(gq, traceM(d, gv, k, pw, L.vw, 1.0f, mid), 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
-  m = mmV(gq, traceM(d, gv, k, pc, L.vc, 1.0f, mid), 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
+  m = mmV_r(gq, traceM_r(d, gv, nA, k, pu, L.vu, 1.0f, mid), pu, 0.0f, 0.5f, 0.5f, 0); r.x = vclamp_(r.x, m);
+  m = mmV_r(gq, traceM_r(d, gv, nA, k, pv, L.vv, 1.0f, mid), pv, 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
+  m = mmV_r(gq, traceM_r(d, gv, nA, k, pw, L.vw, 1.0f, mid), pw, 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
+  m = mmV_r(gq, traceM_r(d, gv, nA, k, pc, L.vc, 1.0f, mid), pc, 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
   return r;
 }
 struct FallH {
@@ -1307,59 +1348,63 @@ __device__ __forceinline__ FallH fall_h(const MU& u, float4 q1, float4 q2, float
   const float2 fv = vtRain(thm, q1.w, q2.z, rho, rs);
   return FallH{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rs), vtGrauM(thm, maritime_of(u), q2.y, rho, rs), vtI(q1.z, rho)};
 }
-template <class AV, class AQ>
-__device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const float4* BS, const AV& gv, const AQ& g1,
-                                              const AQ& g2, int x, int y, int z, float4& o1, float4& o2) {
+// as near_traces_q, with the settling ice's fall speed
+__device__ __forceinline__ bool near_traces_qh(const Dims& d, float3 v, const FallH& f) {
+  return near_traces_q(d, v, Fall{f.r, f.n, f.s, f.g}) && near_traces_q(d, v, Fall{f.i, 0.0f, 0.0f, 0.0f});
+}
+__device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const float4* BS, const Ring& gv, const Ring& g1,
+                                              const Ring& g2, int x, int y, int z, float4& o1, float4& o2) {
   const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
-  const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  const float4 q1 = g1.ld4(x, y, z), q2 = g2.ld4(x, y, z);
+  const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
-  const float3 t0 = traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid);
-  float4 a = sampQ(g1, t0);
-  if (f.i > 0.0f) a.z = sampQ1(g1, traceQM(d, gv, k, p, v, f.i, 1.0f, mid), 2);
-  const float r = sampQ1(g1, traceQM(d, gv, k, p, v, f.r, 1.0f, mid), 3);
-  const float4 b = sampQ(g2, t0);
-  const float sv = sampQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0);
-  const float g = sampQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1);
-  const float nr = thm ? sampQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2) : b.z;
+  const bool nA = wave_all(near_traces_qh(d, v, f));
+  const float3 t0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, 1.0f, mid);
+  float4 a = sampQ_r(g1, t0, p);
+  if (f.i > 0.0f) a.z = sampQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.i, 1.0f, mid), p, 2);
+  const float r = sampQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.r, 1.0f, mid), p, 3);
+  const float4 b = sampQ_r(g2, t0, p);
+  const float sv = sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.s, 1.0f, mid), p, 0);
+  const float g = sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.g, 1.0f, mid), p, 1);
+  const float nr = thm ? sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.n, 1.0f, mid), p, 2) : b.z;
   o1 = make_float4(a.x, a.y, a.z, r);
   o2 = make_float4(sv, g, nr, b.w);
 }
-template <class AV, class AQ>
-__device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const float4* BS, const AV& gv, const AQ& g1,
-                                              const AQ& g2, const AQ& g1t, const AQ& g2t, int x, int y, int z,
+__device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const float4* BS, const Ring& gv, const Ring& g1,
+                                              const Ring& g2, const Ring& g1t, const Ring& g2t, int x, int y, int z,
                                               float4& o1, float4& o2) {
   const bool mid = u.v[U_CN].w > 0.5f;
-  const float3 v = cellVel(gv, x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  const float4 q1 = g1.ld4(x, y, z), q2 = g2.ld4(x, y, z), h1 = g1t.ld4(x, y, z), h2 = g2t.ld4(x, y, z);
+  const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z), h1 = g1t.w().ld4(x, y, z), h2 = g2t.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
+  const bool nA = wave_all(near_traces_qh(d, v, f));
   const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  const float3 f0 = traceQM(d, gv, k, p, v, 0.0f, -1.0f, mid);
-  const float4 b1 = sampQ(g1t, f0);
-  const float br = sampQ1(g1t, traceQM(d, gv, k, p, v, f.r, -1.0f, mid), 3);
-  const float4 b2 = sampQ(g2t, f0);
-  const float bs = sampQ1(g2t, traceQM(d, gv, k, p, v, f.s, -1.0f, mid), 0);
-  const float bg = sampQ1(g2t, traceQM(d, gv, k, p, v, f.g, -1.0f, mid), 1);
-  const float bn = sampQ1(g2t, traceQM(d, gv, k, p, v, f.n, -1.0f, mid), 2);
-  const float bi = f.i > 0.0f ? sampQ1(g1t, traceQM(d, gv, k, p, v, f.i, -1.0f, mid), 2) : b1.z;
+  const float3 f0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, -1.0f, mid);
+  const float4 b1 = sampQ_r(g1t, f0, p);
+  const float br = sampQ1_r(g1t, traceQM_r(d, gv, nA, k, p, v, f.r, -1.0f, mid), p, 3);
+  const float4 b2 = sampQ_r(g2t, f0, p);
+  const float bs = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.s, -1.0f, mid), p, 0);
+  const float bg = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.g, -1.0f, mid), p, 1);
+  const float bn = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.n, -1.0f, mid), p, 2);
+  const float bi = f.i > 0.0f ? sampQ1_r(g1t, traceQM_r(d, gv, nA, k, p, v, f.i, -1.0f, mid), p, 2) : b1.z;
   float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - bi),
                           h1.w + sm * (q1.w - br));
   float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
                           h2.w + sm * (q2.w - b2.w));
-  const float3 d0 = traceQM(d, gv, k, p, v, 0.0f, 1.0f, mid);
+  const float3 d0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, 1.0f, mid);
   float2 m;
-  m = mmQ1(g1, d0, 0); r1.x = vclamp_(r1.x, m);
-  m = mmQ1(g1, d0, 1); r1.y = vclamp_(r1.y, m);
-  m = mmQ1(g1, traceQM(d, gv, k, p, v, f.i, 1.0f, mid), 2); r1.z = vclamp_(r1.z, m);
-  m = mmQ1(g1, traceQM(d, gv, k, p, v, f.r, 1.0f, mid), 3); r1.w = vclamp_(r1.w, m);
-  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.s, 1.0f, mid), 0); r2.x = vclamp_(r2.x, m);
-  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.g, 1.0f, mid), 1); r2.y = vclamp_(r2.y, m);
-  m = mmQ1(g2, traceQM(d, gv, k, p, v, f.n, 1.0f, mid), 2); r2.z = vclamp_(r2.z, m);
-  m = mmQ1(g2, d0, 3); r2.w = vclamp_(r2.w, m);
+  m = mmQ1_r(g1, d0, p, 0); r1.x = vclamp_(r1.x, m);
+  m = mmQ1_r(g1, d0, p, 1); r1.y = vclamp_(r1.y, m);
+  m = mmQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.i, 1.0f, mid), p, 2); r1.z = vclamp_(r1.z, m);
+  m = mmQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.r, 1.0f, mid), p, 3); r1.w = vclamp_(r1.w, m);
+  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.s, 1.0f, mid), p, 0); r2.x = vclamp_(r2.x, m);
+  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.g, 1.0f, mid), p, 1); r2.y = vclamp_(r2.y, m);
+  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.n, 1.0f, mid), p, 2); r2.z = vclamp_(r2.z, m);
+  m = mmQ1_r(g2, d0, p, 3); r2.w = vclamp_(r2.w, m);
   o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
   o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
 }
