@@ -172,6 +172,33 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     }
 
 
+def plain_128_run(steps=600):
+    """The 128 x 128 x 40, 2 km plain run (Lin, one 2 K bubble, dt 6 s) timed over all its steps: ms per step,
+    sim-s per wall-s, and the strongest updraft at the end."""
+    import numpy as np
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_plain_128
+
+    m = RegionalModel()
+    try:
+        dt = setup_plain_128(m, seed=0, scheme="lin")
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.step(dt)
+        m.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        V, Q1, _ = m.state()
+        cells = m.nx * m.ny * m.nz
+        return {"workload": f"plain 128x128x40 at 2000 m, Lin, dt {dt:.0f} s, one 2 K bubble", "steps": steps,
+                "ms_per_step": round(ms, 4), "sim_s_per_wall_s": round(dt / (ms * 1e-3), 1),
+                "cell_updates_per_s": round(cells / (ms * 1e-3), 1), "w_max": round(float(V[..., 2].max()), 2),
+                "state_finite": bool(np.isfinite(V).all() and np.isfinite(Q1).all()),
+                "note": "whole run timed, including its first steps; 6.6e5 cells per step, so launch-bound"}
+    finally:
+        m.destroy()
+
+
 def hurricane_step(key="h2", steps=20, warmup=8):
     """The hurricane-mode step (filter every 4th step, PREP, midpoint trajectories, sea fluxes) on the reference's
     hurricane grid ``key``: ms per step over ``steps`` steps after ``warmup``, sim-s per wall-s, per-kernel times."""
@@ -387,6 +414,13 @@ def main():
         radar = radar_volume(m)
     except Exception as e:  # reported, not fatal to the plain measurement
         radar = {"error": f"{type(e).__name__}: {e}"}
+    c1 = None
+    if not args.no_combined:
+        print("# plain 128x128x40 run (600 steps) ...", file=sys.stderr, flush=True)
+        try:
+            c1 = plain_128_run()
+        except Exception as e:  # reported, not fatal to the plain measurement
+            c1 = {"error": f"{type(e).__name__}: {e}"}
     hur = None
     if not args.no_combined:
         print("# hurricane step (h2) ...", file=sys.stderr, flush=True)
@@ -436,6 +470,7 @@ def main():
         "solve_alone": solve,
         "maps": maps,
         "radar": radar,
+        "plain_128": c1,
         "hurricane": hur,
         "vs_baseline_basis": "reference's published RTX 5090 step at 1024x1024x80 (README.md:47: 32x real time "
                              "at dt 3 s = 8.95e8 cell-updates/s), i.e. 21.3 sim-s/wall-s at dt 2 s",
