@@ -335,6 +335,8 @@ def main():
                     help="plain: the plain FP32 step at 1024x1024x80 (and the combined form on the same grid); "
                          "c5: the combined form at 1024x1024x100, 250 m; hurricane: only the hurricane-mode step "
                          "on the h2 preset (640x640x56, 2 km)")
+    ap.add_argument("--preset", default="h2", choices=["h2", "h3", "h15"],
+                    help="the hurricane grid of --form hurricane (model.js:1151-1153)")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined-form measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM-counter calibration) and exit")
@@ -351,7 +353,7 @@ def main():
         bench_c5(args)
         return
     if args.form == "hurricane":
-        print(json.dumps({"hurricane": hurricane_step(steps=args.steps, warmup=args.warmup)}))
+        print(json.dumps({"hurricane": hurricane_step(args.preset, steps=args.steps, warmup=args.warmup)}))
         return
 
     from outbreaklab_amd import RegionalModel
