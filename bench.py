@@ -199,6 +199,39 @@ def plain_128_run(steps=600):
         m.destroy()
 
 
+# the reference's published speeds on its outbreak grids, no radars, RTX 50-series (README.md:23-28), x real time
+README_SPEEDS = {"quick": 2900, "standard": 870, "large": 520, "fine": 290, "m500": 190, "m250": 70, "m100": 22}
+
+
+def presets_sweep(steps=20, warmup=5):
+    """The plain step on each of the reference's outbreak grids at dt 6 dx_km (app.js:1666), next to the speed the
+    reference's README quotes for that grid."""
+    from outbreaklab_amd import RegionalModel
+    from outbreaklab_amd.configs import setup_outbreak_preset
+
+    out = {}
+    for key, ref in README_SPEEDS.items():
+        m = RegionalModel()
+        try:
+            dt = setup_outbreak_preset(m, key)
+            for _ in range(warmup):
+                m.step(dt)
+            m.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                m.step(dt)
+            m.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            out[key] = {"grid": f"{m.nx}x{m.ny}x{m.nz} at {m.dx:.0f} m", "dt": dt, "ms_per_step": round(ms, 3),
+                        "x_real_time": round(dt / (ms * 1e-3), 1), "readme_x_real_time_rtx50": ref,
+                        "ratio": round(dt / (ms * 1e-3) / ref, 2)}
+            print(f"# {key}: {out[key]['grid']}, {ms:.3f} ms/step, {out[key]['x_real_time']}x real time "
+                  f"(README ~{ref}x)", file=sys.stderr, flush=True)
+        finally:
+            m.destroy()
+    return out
+
+
 def hurricane_step(key="h2", steps=20, warmup=8):
     """The hurricane-mode step (filter every 4th step, PREP, midpoint trajectories, sea fluxes) on the reference's
     hurricane grid ``key``: ms per step over ``steps`` steps after ``warmup``, sim-s per wall-s, per-kernel times."""
@@ -331,10 +364,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--form", default="plain", choices=["plain", "c5", "hurricane"],
+    ap.add_argument("--form", default="plain", choices=["plain", "c5", "hurricane", "presets"],
                     help="plain: the plain FP32 step at 1024x1024x80 (and the combined form on the same grid); "
                          "c5: the combined form at 1024x1024x100, 250 m; hurricane: only the hurricane-mode step "
-                         "on the h2 preset (640x640x56, 2 km)")
+                         "(--preset); presets: the plain step on each of the reference's outbreak grids")
     ap.add_argument("--preset", default="h2", choices=["h2", "h3", "h15"],
                     help="the hurricane grid of --form hurricane (model.js:1151-1153)")
     ap.add_argument("--no-combined", action="store_true", help="skip the combined-form measurement")
@@ -351,6 +384,9 @@ def main():
         return
     if args.form == "c5":
         bench_c5(args)
+        return
+    if args.form == "presets":
+        print(json.dumps({"presets": presets_sweep(steps=args.steps, warmup=args.warmup)}))
         return
     if args.form == "hurricane":
         print(json.dumps({"hurricane": hurricane_step(args.preset, steps=args.steps, warmup=args.warmup)}))

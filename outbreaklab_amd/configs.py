@@ -87,6 +87,30 @@ def setup_plain_1024(model, seed: int = 1, scheme: str = "thompson", n=(1024, 10
     return 2.0
 
 
+def setup_outbreak_preset(model, key: str, seed: int = 1, scheme: str = "thompson", cape: float = 3000.0):
+    """The 1024 x 1024 x 80 workload's storm environment on one of the reference's outbreak grids (model.js:1143-1149:
+    quick, standard, large, fine, m500, m250, m100): WK at `cape` J/kg SBCAPE, 25 m/s 0-6 km shear, 2 K bubbles
+    (8 where the domain has room, else one at the centre), dt 6 dx_km s as app.js:1666 at time step x1."""
+    model.allocate(key)
+    model.params["scheme"] = scheme
+    model.params["noise"] = 0.1
+    model.params["lat"] = 35.9
+    model.noise_seed = seed
+    model.setSounding(wk_text_for_cape(cape, 25.0))
+    model.setBoundaries([])
+    model.setCI([])
+    model.setSurface(None, None)
+    model.reset()
+    margin = model.params["spongeW"] + 10000.0
+    if min(model.Lx, model.Ly) > 2 * margin + 70000:
+        pts = bubble_positions(8, model.Lx, model.Ly, margin, 35000.0, seed)
+    else:
+        pts = [(0.5 * model.Lx, 0.5 * model.Ly)]
+    for x, y in pts:
+        model.add_bubble(x, y, 1500.0, 10000.0, 1500.0, 2.0)
+    return 6.0 * model.dx / 1000.0
+
+
 def setup_combined_1024(model, seed: int = 1, n=(1024, 1024, 100), cape: float = 4000.0):
     """1024 x 1024 x 100 at 250 m (dz 200 m, pad 40): WK at 4000 J/kg SBCAPE with 25 m/s 0-6 km shear,
     Thompson, one seeded outflow-type boundary (dT -6 K, 1.5 km deep, along-front variation 0.8 with
