@@ -1065,11 +1065,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
   const bool act = x < d.nx && y < d.ny;
   // the cell velocity's texels are loaded one level ahead (see k_nut_t)
   CellVelTexels vtn = cellVelTexels(gv, x, y, 0);
+  // planes are fetched two levels ahead: level z + 2 is in registers when level z starts, level z + 3 is
+  // fetched during it (the VGPR budget of this kernel's LDS-limited occupancy has room for both)
+  Ring::Pre p1, p2, p3, p4;
+  r1.fetch(2, tid, p1); r2.fetch(2, tid, p2); r3.fetch(2, tid, p3); r4.fetch(2, tid, p4);
   for (int z = 0; z < d.nz; z++) {
-    Ring::Pre p1, p2, p3, p4;
+    Ring::Pre n1, n2, n3, n4;
     const CellVelTexels vt = vtn;
     vtn = cellVelTexels(gv, x, y, z + 1);
-    r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2); r3.fetch(z + 2, tid, p3); r4.fetch(z + 2, tid, p4);
+    r1.fetch(z + 3, tid, n1); r2.fetch(z + 3, tid, n2); r3.fetch(z + 3, tid, n3); r4.fetch(z + 3, tid, n4);
     r1.center(z); r2.center(z); r3.center(z); r4.center(z);
     if (act) {
       float4 o1, o2;
@@ -1080,6 +1084,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __
     }
     __syncthreads();
     r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4);
+    p1 = n1; p2 = n2; p3 = n3; p4 = n4;
     __syncthreads();
   }
 }
