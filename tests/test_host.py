@@ -122,3 +122,13 @@ def test_library_exports_every_declared_symbol():
     # loading it (no device calls) and the ctypes signature table cover every declaration
     ctypes.CDLL(so)
     assert set(_lib.symbols()) == set(declared)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N > 1 exits with status 2 and says so, before touching a device."""
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
