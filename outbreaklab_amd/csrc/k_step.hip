@@ -1353,9 +1353,66 @@ __device__ __forceinline__ FallH fall_h(const MU& u, float4 q1, float4 q2, float
   const float2 fv = vtRain(thm, q1.w, q2.z, rho, rs);
   return FallH{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rs), vtGrauM(thm, maritime_of(u), q2.y, rho, rs), vtI(q1.z, rho)};
 }
+// The midpoints of a cell's traces in one direction (s) differ only in z (each species' fall speed), so
+// the wind's x-y bilinears on the window's three planes are shared: computed once, each midpoint wind
+// is then one z lerp per channel between the planes around it. The same texels and arithmetic as
+// velAt over the window view (x, then y, then z lerps), valid when the point's vote keeps every
+// midpoint within half a cell.
+struct MidWind {
+  float u[3], v[3], w[3];  // bilinears of planes zc - 1, zc, zc + 1
+  int zc;
+  __device__ __forceinline__ static float bil(const RingW& R, float tx, float ty, int r, int ch) {
+    const float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty);
+    const int b = (r == 0 ? R.om : (r == 1 ? R.oc : R.op)) + ch * TPS + __mul24(floori_(ty) - R.y0 + 1, TPW) +
+                  (floori_(tx) - R.x0 + 1);
+    return lerpf(lerpf(R.s[b], R.s[b + 1], ax), lerpf(R.s[b + TPW], R.s[b + TPW + 1], ax), ay);
+  }
+  __device__ __forceinline__ MidWind(const RingW& R, float mx, float my) : zc(R.zc) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      u[r] = bil(R, mx, my - 0.5f, r, 0);
+      v[r] = bil(R, mx - 0.5f, my, r, 1);
+      w[r] = bil(R, mx - 0.5f, my - 0.5f, r, 2);
+    }
+  }
+  __device__ __forceinline__ static float zl(const float* c, float tz, int zc) {
+    const float az = __builtin_amdgcn_fractf(tz);
+    const bool lo = floori_(tz) < zc;  // the box's lower plane is zc - 1, else zc
+    return lerpf(lo ? c[0] : c[1], lo ? c[1] : c[2], az);
+  }
+  __device__ __forceinline__ float3 at(float mz) const {
+    return make_float3(zl(u, mz - 0.5f, zc), zl(v, mz - 0.5f, zc), zl(w, mz, zc));
+  }
+};
+// traceQM's midpoint trace with the midpoint wind from a MidWind of the same direction
+__device__ __forceinline__ float3 traceQ_mw(const Dims& d, const MidWind& M, float3 k, float3 p, float3 v0, float fall,
+                                            float s) {
+  const float mz = clampt_(p.z - 0.5f * s * k.z * (v0.z - fall), 0.0f, (float)d.nz);
+  return traceQ(d, k, p, M.at(mz), fall, s);
+}
+// the midpoint's x and y (shared by every species' trace in direction s)
+__device__ __forceinline__ float2 mid_xy(const Dims& d, float3 k, float3 p, float3 v0, float s) {
+  return make_float2(clampt_(p.x - 0.5f * s * k.x * v0.x, 0.0f, (float)d.nx),
+                     clampt_(p.y - 0.5f * s * k.y * v0.y, 0.0f, (float)d.ny));
+}
+
 // as near_traces_q, with the settling ice's fall speed
 __device__ __forceinline__ bool near_traces_qh(const Dims& d, float3 v, const FallH& f) {
   return near_traces_q(d, v, Fall{f.r, f.n, f.s, f.g}) && near_traces_q(d, v, Fall{f.i, 0.0f, 0.0f, 0.0f});
+}
+template <class TR>
+__device__ __forceinline__ void advq1_h_body(bool thm, const Ring& g1, const Ring& g2, const FallH& f, float3 p,
+                                             const TR& tr, float4& o1, float4& o2) {
+  const float3 t0 = tr(0.0f, 1.0f);
+  float4 a = sampQ_r(g1, t0, p);
+  if (f.i > 0.0f) a.z = sampQ1_r(g1, tr(f.i, 1.0f), p, 2);
+  const float r = sampQ1_r(g1, tr(f.r, 1.0f), p, 3);
+  const float4 b = sampQ_r(g2, t0, p);
+  const float sv = sampQ1_r(g2, tr(f.s, 1.0f), p, 0);
+  const float g = sampQ1_r(g2, tr(f.g, 1.0f), p, 1);
+  const float nr = thm ? sampQ1_r(g2, tr(f.n, 1.0f), p, 2) : b.z;
+  o1 = make_float4(a.x, a.y, a.z, r);
+  o2 = make_float4(sv, g, nr, b.w);
 }
 __device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const float4* BS, const Ring& gv, const Ring& g1,
                                               const Ring& g2, int x, int y, int z, float4& o1, float4& o2) {
@@ -1365,16 +1422,46 @@ __device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const 
   const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
   const bool nA = wave_all(near_traces_qh(d, v, f));
-  const float3 t0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, 1.0f, mid);
-  float4 a = sampQ_r(g1, t0, p);
-  if (f.i > 0.0f) a.z = sampQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.i, 1.0f, mid), p, 2);
-  const float r = sampQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.r, 1.0f, mid), p, 3);
-  const float4 b = sampQ_r(g2, t0, p);
-  const float sv = sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.s, 1.0f, mid), p, 0);
-  const float g = sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.g, 1.0f, mid), p, 1);
-  const float nr = thm ? sampQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.n, 1.0f, mid), p, 2) : b.z;
-  o1 = make_float4(a.x, a.y, a.z, r);
-  o2 = make_float4(sv, g, nr, b.w);
+  if (mid && nA) {
+    const float2 md = mid_xy(d, k, p, v, 1.0f);
+    const MidWind M(gv.w(), md.x, md.y);
+    advq1_h_body(thm, g1, g2, f, p, [&](float fall, float s) { return traceQ_mw(d, M, k, p, v, fall, s); }, o1, o2);
+  } else {
+    advq1_h_body(thm, g1, g2, f, p, [&](float fall, float s) { return traceQM_r(d, gv, nA, k, p, v, fall, s, mid); },
+                 o1, o2);
+  }
+}
+template <class TR>
+__device__ __forceinline__ void advq2_h_body(const MU& u, const Ring& g1, const Ring& g2, const Ring& g1t, const Ring& g2t,
+                                             const FallH& f, float3 p, float3 v, float3 k, float4 q1, float4 q2, float4 h1,
+                                             float4 h2, const TR& tr, float4& o1, float4& o2) {
+  const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
+  const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  const float3 f0 = tr(0.0f, -1.0f);
+  const float4 b1 = sampQ_r(g1t, f0, p);
+  const float br = sampQ1_r(g1t, tr(f.r, -1.0f), p, 3);
+  const float4 b2 = sampQ_r(g2t, f0, p);
+  const float bs = sampQ1_r(g2t, tr(f.s, -1.0f), p, 0);
+  const float bg = sampQ1_r(g2t, tr(f.g, -1.0f), p, 1);
+  const float bn = sampQ1_r(g2t, tr(f.n, -1.0f), p, 2);
+  const float bi = f.i > 0.0f ? sampQ1_r(g1t, tr(f.i, -1.0f), p, 2) : b1.z;
+  float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - bi),
+                          h1.w + sm * (q1.w - br));
+  float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
+                          h2.w + sm * (q2.w - b2.w));
+  const float3 d0 = tr(0.0f, 1.0f);
+  float2 m;
+  m = mmQ1_r(g1, d0, p, 0); r1.x = vclamp_(r1.x, m);
+  m = mmQ1_r(g1, d0, p, 1); r1.y = vclamp_(r1.y, m);
+  m = mmQ1_r(g1, tr(f.i, 1.0f), p, 2); r1.z = vclamp_(r1.z, m);
+  m = mmQ1_r(g1, tr(f.r, 1.0f), p, 3); r1.w = vclamp_(r1.w, m);
+  m = mmQ1_r(g2, tr(f.s, 1.0f), p, 0); r2.x = vclamp_(r2.x, m);
+  m = mmQ1_r(g2, tr(f.g, 1.0f), p, 1); r2.y = vclamp_(r2.y, m);
+  m = mmQ1_r(g2, tr(f.n, 1.0f), p, 2); r2.z = vclamp_(r2.z, m);
+  m = mmQ1_r(g2, d0, p, 3); r2.w = vclamp_(r2.w, m);
+  o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
+  o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
 }
 __device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const float4* BS, const Ring& gv, const Ring& g1,
                                               const Ring& g2, const Ring& g1t, const Ring& g2t, int x, int y, int z,
@@ -1385,33 +1472,15 @@ __device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const 
   const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z), h1 = g1t.w().ld4(x, y, z), h2 = g2t.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
   const bool nA = wave_all(near_traces_qh(d, v, f));
-  const float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
-  const float cour = sqrtf(cx * cx + cy * cy + cz * cz);
-  const float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  const float3 f0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, -1.0f, mid);
-  const float4 b1 = sampQ_r(g1t, f0, p);
-  const float br = sampQ1_r(g1t, traceQM_r(d, gv, nA, k, p, v, f.r, -1.0f, mid), p, 3);
-  const float4 b2 = sampQ_r(g2t, f0, p);
-  const float bs = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.s, -1.0f, mid), p, 0);
-  const float bg = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.g, -1.0f, mid), p, 1);
-  const float bn = sampQ1_r(g2t, traceQM_r(d, gv, nA, k, p, v, f.n, -1.0f, mid), p, 2);
-  const float bi = f.i > 0.0f ? sampQ1_r(g1t, traceQM_r(d, gv, nA, k, p, v, f.i, -1.0f, mid), p, 2) : b1.z;
-  float4 r1 = make_float4(h1.x + sm * (q1.x - b1.x), h1.y + sm * (q1.y - b1.y), h1.z + sm * (q1.z - bi),
-                          h1.w + sm * (q1.w - br));
-  float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
-                          h2.w + sm * (q2.w - b2.w));
-  const float3 d0 = traceQM_r(d, gv, nA, k, p, v, 0.0f, 1.0f, mid);
-  float2 m;
-  m = mmQ1_r(g1, d0, p, 0); r1.x = vclamp_(r1.x, m);
-  m = mmQ1_r(g1, d0, p, 1); r1.y = vclamp_(r1.y, m);
-  m = mmQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.i, 1.0f, mid), p, 2); r1.z = vclamp_(r1.z, m);
-  m = mmQ1_r(g1, traceQM_r(d, gv, nA, k, p, v, f.r, 1.0f, mid), p, 3); r1.w = vclamp_(r1.w, m);
-  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.s, 1.0f, mid), p, 0); r2.x = vclamp_(r2.x, m);
-  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.g, 1.0f, mid), p, 1); r2.y = vclamp_(r2.y, m);
-  m = mmQ1_r(g2, traceQM_r(d, gv, nA, k, p, v, f.n, 1.0f, mid), p, 2); r2.z = vclamp_(r2.z, m);
-  m = mmQ1_r(g2, d0, p, 3); r2.w = vclamp_(r2.w, m);
-  o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
-  o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
+  if (mid && nA) {
+    const float2 mf = mid_xy(d, k, p, v, -1.0f), md = mid_xy(d, k, p, v, 1.0f);
+    const MidWind Mf(gv.w(), mf.x, mf.y), Md(gv.w(), md.x, md.y);
+    advq2_h_body(u, g1, g2, g1t, g2t, f, p, v, k, q1, q2, h1, h2,
+                 [&](float fall, float s) { return traceQ_mw(d, s > 0.0f ? Md : Mf, k, p, v, fall, s); }, o1, o2);
+  } else {
+    advq2_h_body(u, g1, g2, g1t, g2t, f, p, v, k, q1, q2, h1, h2,
+                 [&](float fall, float s) { return traceQM_r(d, gv, nA, k, p, v, fall, s, mid); }, o1, o2);
+  }
 }
 
 // z-marching drivers (as k_advv1_t ... k_advq2_t). PREP: the carried winds Vval are a field of
