@@ -36,10 +36,11 @@
 extern "C" {
 #endif
 
-#define OBL_ABI_VERSION 1
+#define OBL_ABI_VERSION 2
 #define OBL_UNIFORM_FLOATS 76
 #define OBL_BOUND_FLOATS 148 /* one boundary record, model.js:1371-1381 */
 #define OBL_MAX_BOUNDS 8     /* MAX_BOUNDS, model.js:38 */
+#define OBL_BOUND_POINTS 32  /* points per boundary polyline, model.js:1373-1380 */
 #define OBL_CI_FLOATS 12     /* one CI tower, model.js:1383-1389 */
 #define OBL_MAX_CI 256       /* MAX_CI, model.js:38 */
 
@@ -105,20 +106,23 @@ int obl_create(const obl_grid* g, obl_model** out);
 void obl_destroy(obl_model* h);
 /* setSounding -> bs + solver.setDensity (model.js:1276-1288): B holds (nz+1)*8 floats */
 int obl_set_base_state(obl_model* h, const float* B, int nz);
-/* setBoundaries (model.js:1371-1381): count records of 148 floats */
+/* setBoundaries (model.js:1371-1381): count records of 148 floats; a record's first float is its point
+ * count, 0..32 (larger counts are rejected) */
 int obl_set_boundaries(obl_model* h, const float* f, int count);
 /* setCI (model.js:1383-1389): count towers of 12 floats */
 int obl_set_ci(obl_model* h, const float* f, int count);
 /* setSurface (model.js:1298-1303): surf = nx*ny float4 (land fraction, SST K, 0, 0) per column, the
- * sea that hurricane mode (hur.x = 1 in the uniform block) reads; NULL = outbreak mode (no sea). */
-int obl_set_surface(obl_model* h, const float* surf);
+ * sea that hurricane mode (hur.x = 1 in the uniform block) reads; NULL = outbreak mode (no sea).
+ * bytes is the size of the caller's buffer, which must hold nx*ny*16 bytes. */
+int obl_set_surface(obl_model* h, const float* surf, size_t bytes);
 /* setPrepField (model.js:1290-1296): allocate (on) or free the Vp field of hurricane runs; while it
  * exists every step first runs WGSL_M_PREP and the wind advection carries Vp's values. */
 int obl_set_prep_field(obl_model* h, int on);
 /* setVortex (model.js:1318-1365), its device half: the table the init kernel reads, nz x nr float4
  * (tangential wind m/s, theta' K, 0, 0) at radii i * dr, built on the host; used by obl_reset when
- * vtx2.z = 1, with vtx = (x, y, dr, nr). NULL / nr 0 drops it. */
-int obl_set_vortex(obl_model* h, const float* table, int nr);
+ * vtx2.z = 1, with vtx = (x, y, dr, nr). NULL / nr 0 drops it. bytes is the size of the caller's
+ * buffer, which must hold nz*nr*16 bytes. */
+int obl_set_vortex(obl_model* h, const float* table, int nr, size_t bytes);
 /* reset (model.js:1427-1440): lsf + init with the given uniform block (written
  * for dt = 1), clears phi/rhs, copies V0 -> V1. seed 0 = the reference's noise. */
 int obl_reset(obl_model* h, const float* uniforms, int seed);

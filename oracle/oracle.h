@@ -46,6 +46,7 @@ extern "C" {
                     const R* aux, const R* lsfA, const R* lsfB, const R* phT, const R* ciMap,       \
                     R* Vout, R* Q1o, R* Q2o);                                                       \
   void orc_micro_point_##S(const float* U, const float* BS, int k, R* st);                          \
+  void orc_fall_##S(const float* U, double q, double n, double rho, double* out);                \
   void orc_div_##S(const float* U, const float* BS, const R* V, R* rhs);                            \
   void orc_proj_##S(const float* U, const float* BS, const R* V, const R* phi, R* Vout);            \
   void orc_bubble_##S(const float* U, R* V, double x, double y, double z, double rh, double rz,      \
@@ -105,6 +106,11 @@ void orc_step_f32(const float* U, const float* BS, const float* BD, const float*
                   int parity, float* V2, float* Q12, float* Q22, float* Vt, float* Q1t, float* Q2t,
                   float* aux, float* phT, float* lsfA, float* lsfB, float* ciMap, float* rhs,
                   float* phi, int nthreads);
+/* The same step with the FP64 kernels and the FP64 solve (every array double). */
+void orc_step_f64(const float* U, const float* BS, const float* BD, const float* CI, int run_lsf,
+                  int parity, double* V2, double* Q12, double* Q22, double* Vt, double* Q1t, double* Q2t,
+                  double* aux, double* phT, double* lsfA, double* lsfB, double* ciMap, double* rhs,
+                  double* phi, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -12,15 +12,29 @@
  * and WGSL's transcendental ULPs are implementation-defined; here weights are
  * exact FP32 and exp/pow/sin are the C library's.
  *
- * Parity pins (tests/test_oracle_*.py):
- *   hash/rnd3      integer-exact against hand-computed PCG values
- *   lsf/ci         closed forms for no boundaries / no events; phT hydrostatic
+ * Parity pins (tests/test_oracle_*.py; each would fail on a plausible transcription mistake):
+ *   hash/rnd3      integer-exact against the published PCG-RXS-M-XS hash
+ *   lsf/ci         closed forms for no boundaries / no events, a straight front and one tower;
+ *                  phT the downward trapezoid integral of the target buoyancy
  *   init           target state + bounded noise, qv <= 0.95 qvs
  *   nut            zero for uniform flow; linear shear gives the closed-form nu
  *   adv*           identity at zero wind; constants preserved; integer-Courant
  *                  shifts exact; no new extrema; brute-force trilinear
- *   phys forces    linear fields have zero Laplacian, closed-form buoyancy
- *   microphysics   total-water conservation, positivity, branch closed forms
+ *   phys forces    the exact Laplacians of quadratics (Prandtl 1/3 for theta and vapour); lateral and
+ *                  top sponge rates at known positions and condensate removal; air-mass nudging to
+ *                  the smoothstep target profile; f-plane Coriolis on the departure wind; the target
+ *                  phi gradient; semi-slip log-law drag; buoyancy of theta', qv' and each condensate;
+ *                  the one-sided CI push; speed cap and NaN guard
+ *   fall speeds    Klemp-Wilhelmson rain; Thompson rain and graupel slopes and fall speeds as moments
+ *                  of the exponential size distribution (gamma functions); power-law exponents,
+ *                  coefficients and bounds of snow, Lin and maritime graupel; Heymsfield-Donner ice
+ *   microphysics   each process isolated on hand-built points: Kessler autoconversion and accretion,
+ *                  new-drop number, Thompson accretion by its size distribution, Lin ice
+ *                  autoconversion, collection and riming, rain freezing (Bigg) and collection by
+ *                  graupel, snow-to-graupel, melting (Lin and Thompson Lin-1983 form with drop
+ *                  numbers), Klemp-Wilhelmson and Thompson evaporation, Seifert-Beheng
+ *                  self-collection and breakup with the drop bounds, sublimation, mixed-phase
+ *                  saturation adjustment; total-water conservation and positivity throughout
  *   div/proj       div(proj(V)) ~ 0 with the solved phi
  *   maps           zero wind gives zero vorticity, helicity, convergence; solid-body rotation
  *                  gives zeta = 2 Omega; a uniform divergence gives its convergence; a cloud-free
@@ -270,6 +284,18 @@ static R cdSfc(const Ctx* c, R u1, R land) {
 static R sfcAt(const Ctx* c, const float* sfc, int i, int j, int ch) {
   if (!sfc) return 0;
   return (R)sfc[4 * ((size_t)clampi(i, 0, c->nx - 1) + (size_t)c->nx * clampi(j, 0, c->ny - 1)) + ch];
+}
+
+/* The fall speeds and size-distribution slopes of one point (model.js:59-106), for their pins:
+ * out = (vtR, vtS, vtG, vtRT mass-weighted, vtRT number-weighted, vtGT, vtGM, vtI, lamR, lamG, n0G);
+ * q is the mixing ratio of the species (kg/kg), n Thompson's rain number (1/kg), rho the air density. */
+void S_(orc_fall)(const float* U, double q, double n, double rho, double* out) {
+  Ctx C = mkctx(U, NULL), *c = &C;
+  R q_ = (R)q, n_ = (R)n, r_ = (R)rho;
+  R2 rt = vtRT(c, q_, n_, r_);
+  out[0] = vtR(c, q_, r_); out[1] = vtS(c, q_, r_); out[2] = vtG(c, q_, r_);
+  out[3] = rt.x; out[4] = rt.y; out[5] = vtGT(c, q_, r_); out[6] = vtGM(c, q_, r_); out[7] = vtI(c, q_, r_);
+  out[8] = lamR(q_, n_); out[9] = lamG(q_, r_); out[10] = n0G(q_, r_);
 }
 
 /* ---------------- large-scale target (model.js:125-134) ---------------- */

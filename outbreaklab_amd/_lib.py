@@ -40,9 +40,9 @@ _SIGS = {
     "obl_set_base_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_set_ci": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
-    "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "obl_set_prep_field": (C.c_int, [C.c_void_p, C.c_int]),
-    "obl_set_vortex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "obl_set_vortex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
     "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
     "obl_maps_request": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),

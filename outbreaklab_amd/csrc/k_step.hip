@@ -7,6 +7,7 @@
 // data movement is this build's own: one work-item per point, x fastest so
 // that every wave reads contiguous float4 texels, and per-channel gathers for
 // the trilinear samples.
+#include "../../include/outbreak_hip.h"
 #include "obl_device.h"
 #include "obl_kernels.h"
 #include "obl_micro.h"
@@ -24,7 +25,7 @@ __device__ static void bpt(const float* b, int i, float t, float& x, float& y) {
   y = q[1] + q[3] * t;
 }
 __device__ static void sdist(const float* b, float t, float px, float py, float& sd, float& over_, float& along_) {
-  int np = (int)b[0];
+  int np = min((int)b[0], OBL_BOUND_POINTS);  // the record holds 32 points; the host rejects more
   float best = 1e30f, sg = 1.0f, over = 0.0f, along = 0.0f, s0 = 0.0f;
   for (int i = 0; i < np - 1; i++) {
     float ax, ay, ex, ey;
@@ -83,9 +84,9 @@ __global__ void __launch_bounds__(256) k_lsf(MU u, const float4* __restrict__ BS
   float time = u.v[U_STEP].y;
   float px = ((float)i + 0.5f) * d.dx, py = ((float)j + 0.5f) * d.dx;
   float sT = 0, sH = 0, sQ = 0, sHq = 0, sU = 0, sV = 0, sHw = 0, sw = 0, nud = 0;
-  int nb = (int)u.v[U_COR].z;
+  int nb = min((int)u.v[U_COR].z, OBL_MAX_BOUNDS);
   for (int bi = 0; bi < nb; bi++) {
-    const float* b = BD + 148 * bi;
+    const float* b = BD + OBL_BOUND_FLOATS * bi;
     float dd, over, along;
     sdist(b, time, px, py, dd, over, along);
     float w = sstep(-b[3], b[3], dd) * (1.0f - sstep(0.0f, 60000.0f, over));
@@ -133,9 +134,9 @@ __global__ void __launch_bounds__(256) k_ci(MU u, const float* __restrict__ CI, 
   float t = u.v[U_STEP].y;
   float px = ((float)i + 0.5f) * d.dx, py = ((float)j + 0.5f) * d.dx;
   float4 best = make_float4(0.0f, 0.0f, 1500.0f, 1500.0f);
-  int ne = (int)u.v[U_CI].x;
+  int ne = min((int)u.v[U_CI].x, OBL_MAX_CI);
   for (int e = 0; e < ne; e++) {
-    const float* E = CI + 12 * e;
+    const float* E = CI + OBL_CI_FLOATS * e;
     if (t < E[6] || t > E[7]) continue;
     float dxv = px - (E[0] + E[8] * (t - E[6]));
     float dyv = py - (E[1] + E[9] * (t - E[6]));

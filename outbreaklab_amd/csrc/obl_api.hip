@@ -565,6 +565,11 @@ static int check_u(obl_model* h, const float* u) {
   if (u[44] > 0.5f && !h->sfc) return h->fail("hurricane mode (hur.x = 1) needs a sea surface: setSurface first");
   if (u[75] > 0.5f && !(u[44] > 0.5f)) return h->fail("midpoint trajectories (cn.w = 1) are a hurricane-mode setting");
   if (!(u[8] > 0.0f)) return h->fail("dt must be positive");
+  // the kernels loop over these counts in fixed-size device tables
+  if (!(u[30] >= 0.0f && u[30] <= (float)OBL_MAX_BOUNDS))
+    return h->fail("uniform cor.z (boundary count %g) must lie in 0..%d", u[30], OBL_MAX_BOUNDS);
+  if (!(u[32] >= 0.0f && u[32] <= (float)OBL_MAX_CI))
+    return h->fail("uniform ci.x (tower count %g) must lie in 0..%d", u[32], OBL_MAX_CI);
   return 0;
 }
 
@@ -590,6 +595,11 @@ extern "C" int obl_set_base_state(obl_model* h, const float* B, int nz) {
 extern "C" int obl_set_boundaries(obl_model* h, const float* f, int count) {
   if (!h) return set_global("null handle");
   if (count < 0 || count > OBL_MAX_BOUNDS || (count && !f)) return h->fail("setBoundaries: 0..%d records", OBL_MAX_BOUNDS);
+  for (int i = 0; i < count; i++) {
+    const float np = f[(size_t)i * OBL_BOUND_FLOATS];
+    if (!(np >= 0.0f && np <= (float)OBL_BOUND_POINTS))
+      return h->fail("setBoundaries: record %d has %g points (0..%d)", i, np, OBL_BOUND_POINTS);
+  }
   std::vector<float> buf((size_t)OBL_MAX_BOUNDS * OBL_BOUND_FLOATS, 0.0f);
   if (count) memcpy(buf.data(), f, (size_t)count * OBL_BOUND_FLOATS * 4);
   if (hip_ok(hipMemcpyAsync(h->BD, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, h->stream), &h->err, "setBoundaries"))
@@ -606,7 +616,7 @@ extern "C" int obl_set_ci(obl_model* h, const float* f, int count) {
   return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setCI");
 }
 
-extern "C" int obl_set_surface(obl_model* h, const float* surf) {
+extern "C" int obl_set_surface(obl_model* h, const float* surf, size_t bytes) {
   if (!h) return set_global("null handle");
   if (!surf) {
     if (h->sfc) {
@@ -617,6 +627,7 @@ extern "C" int obl_set_surface(obl_model* h, const float* surf) {
     return 0;
   }
   const size_t b = (size_t)h->nx * h->ny * 16;
+  if (bytes < b) return h->fail("setSurface: %zu bytes given, %zu (nx*ny float4) needed", bytes, b);
   if (!h->sfc && hip_ok(hipMalloc(&h->sfc, b), &h->err, "hipMalloc(surface)")) return -1;
   if (hip_ok(hipMemcpyAsync(h->sfc, surf, b, hipMemcpyHostToDevice, h->stream), &h->err, "setSurface")) return -1;
   return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setSurface");
@@ -637,7 +648,7 @@ extern "C" int obl_set_prep_field(obl_model* h, int on) {
   return hip_ok(hipMemsetAsync(h->Vp, 0, h->vcount() * 16, h->stream), &h->err, "prep field clear");
 }
 
-extern "C" int obl_set_vortex(obl_model* h, const float* table, int nr) {
+extern "C" int obl_set_vortex(obl_model* h, const float* table, int nr, size_t bytes) {
   if (!h) return set_global("null handle");
   if (h->vx) {
     hipStreamSynchronize(h->stream);
@@ -648,6 +659,7 @@ extern "C" int obl_set_vortex(obl_model* h, const float* table, int nr) {
   if (!table || nr == 0) return 0;
   if (nr < 2 || nr > 65536) return h->fail("setVortex: the table needs 2..65536 radii (got %d)", nr);
   const size_t b = (size_t)h->nz * nr * 16;
+  if (bytes < b) return h->fail("setVortex: %zu bytes given, %zu (nz*nr float4) needed", bytes, b);
   if (hip_ok(hipMalloc(&h->vx, b), &h->err, "hipMalloc(vortex)")) return -1;
   h->vx_nr = nr;
   if (hip_ok(hipMemcpyAsync(h->vx, table, b, hipMemcpyHostToDevice, h->stream), &h->err, "setVortex")) return -1;

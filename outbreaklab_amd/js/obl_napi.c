@@ -29,9 +29,9 @@ typedef struct {
   int (*set_base_state)(obl_model*, const float*, int);
   int (*set_boundaries)(obl_model*, const float*, int);
   int (*set_ci)(obl_model*, const float*, int);
-  int (*set_surface)(obl_model*, const float*);
+  int (*set_surface)(obl_model*, const float*, size_t);
   int (*set_prep_field)(obl_model*, int);
-  int (*set_vortex)(obl_model*, const float*, int);
+  int (*set_vortex)(obl_model*, const float*, int, size_t);
   int (*reset)(obl_model*, const float*, int);
   int (*step)(obl_model*, const float*, int, unsigned);
   int (*maps_request)(obl_model*, const float*, int, int*);
@@ -280,7 +280,7 @@ static napi_value js_set_surface(napi_env env, napi_callback_info info) {
   obl_model* h = ext(env, argv[0]);
   size_t b;
   float* f = tarr(env, argv[1], &b);
-  return ok_or_throw(env, A.set_surface(h, f), h);
+  return ok_or_throw(env, A.set_surface(h, f, f ? b : 0), h);
 }
 /* setPrepField(h, on) */
 static napi_value js_set_prep_field(napi_env env, napi_callback_info info) {
@@ -297,8 +297,7 @@ static napi_value js_set_vortex(napi_env env, napi_callback_info info) {
   size_t b;
   float* f = tarr(env, argv[1], &b);
   int nr = (int)num(env, argv[2]);
-  if (f && b < (size_t)nr * 16) return throw_msg(env, "setVortex: short table");
-  return ok_or_throw(env, A.set_vortex(h, f, f ? nr : 0), h);
+  return ok_or_throw(env, A.set_vortex(h, f, f ? nr : 0, f ? b : 0), h);
 }
 
 static napi_value js_reset(napi_env env, napi_callback_info info) {

@@ -267,7 +267,7 @@ class RegionalModel:
             raise ValueError(f"setSurface: {self.nx * self.ny * 4} floats expected, got {a.size}")
         if a is None and hur is not None:
             a = np.zeros(self.nx * self.ny * 4, np.float32)
-        _lib.check(_lib.lib().obl_set_surface(self.h, None if a is None else _ptr(a)), self.h)
+        _lib.check(_lib.lib().obl_set_surface(self.h, None if a is None else _ptr(a), 0 if a is None else a.nbytes), self.h)
         self.sfc = None if a is None else Field(self, "sfc", 0, (self.ny, self.nx, 4))
         self.hur = dict(hur) if hur else None
         self.setPrepField(bool(hur))
@@ -279,10 +279,10 @@ class RegionalModel:
         (table spacing, size, central pressure deficit hPa, least core stability K/km, warm-core peak K)."""
         if not v:
             self.vortex = None
-            _lib.check(_lib.lib().obl_set_vortex(self.h, None, 0), self.h)
+            _lib.check(_lib.lib().obl_set_vortex(self.h, None, 0, 0), self.h)
             return None
         tab, info = vortex_table(self.nz, self.dz, self.dx, self.params["lat"], self.base["B"], v)
-        _lib.check(_lib.lib().obl_set_vortex(self.h, _ptr(tab), info["nr"]), self.h)
+        _lib.check(_lib.lib().obl_set_vortex(self.h, _ptr(tab), info["nr"], tab.nbytes), self.h)
         self.vortex = info
         self._vx = tab
         return info

@@ -9,7 +9,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# OBL_ORACLE_SO names another build of the same sources (a planted-mistake build, to check that the pins fail)
+ORACLE_SO = os.environ.get("OBL_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 _L = None
 
@@ -263,6 +264,21 @@ def step_f32(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, ph
     arrs = [V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi]
     for a in arrs:
         assert a.dtype == np.float32 and a.flags.c_contiguous
+    ci = np.ascontiguousarray(CI if CI is not None and len(CI) else np.zeros(12), np.float32)
+    f(P(np.ascontiguousarray(U, np.float32)), P(np.ascontiguousarray(BS, np.float32)),
+      P(np.ascontiguousarray(BD, np.float32)), P(ci), int(run_lsf), int(parity), *[P(a) for a in arrs],
+      threads or min(8, os.cpu_count() or 1))
+
+
+def step_f64(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi,
+             threads=None):
+    """The whole step in FP64 (FP64 kernels and solve); every array is float64, modified in place."""
+    f = lib().orc_step_f64
+    f.restype = None
+    f.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int] + [C.c_void_p] * 13 + [C.c_int]
+    arrs = [V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi]
+    for a in arrs:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
     ci = np.ascontiguousarray(CI if CI is not None and len(CI) else np.zeros(12), np.float32)
     f(P(np.ascontiguousarray(U, np.float32)), P(np.ascontiguousarray(BS, np.float32)),
       P(np.ascontiguousarray(BD, np.float32)), P(ci), int(run_lsf), int(parity), *[P(a) for a in arrs],
