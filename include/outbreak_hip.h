@@ -42,6 +42,7 @@ extern "C" {
 #define OBL_MAX_BOUNDS 8     /* MAX_BOUNDS, model.js:38 */
 #define OBL_BOUND_POINTS 32  /* points per boundary polyline, model.js:1373-1380 */
 #define OBL_CI_FLOATS 12     /* one CI tower, model.js:1383-1389 */
+#define OBL_VORTEX_MAX_RADII 2048 /* radii of the vortex table, VX_MAXR model.js:38 */
 #define OBL_MAX_CI 256       /* MAX_CI, model.js:38 */
 
 typedef struct obl_model obl_model;
@@ -102,6 +103,8 @@ const char* obl_global_error(void);
 /* allocate (model.js:1186-1238): all fields, the padded solve and its matrices.
  * Fails with the reference's size message when a field would not fit. */
 int obl_create(const obl_grid* g, obl_model** out);
+/* the device bytes obl_create allocates for a grid (gridBytes, model.js:1125-1128, for this layout); host only */
+double obl_grid_bytes(const obl_grid* g);
 /* destroy (model.js:1239-1247) */
 void obl_destroy(obl_model* h);
 /* setSounding -> bs + solver.setDensity (model.js:1276-1288): B holds (nz+1)*8 floats */
@@ -123,6 +126,13 @@ int obl_set_prep_field(obl_model* h, int on);
  * vtx2.z = 1, with vtx = (x, y, dr, nr). NULL / nr 0 drops it. bytes is the size of the caller's
  * buffer, which must hold nz*nr*16 bytes. */
 int obl_set_vortex(obl_model* h, const float* table, int nr, size_t bytes);
+/* setVortex's table builder (model.js:1318-1365), host code only (no GPU, no handle): the balanced warm-core
+ * vortex v(r, z) = V(r) F(z) on the base state B ((nz+1)*8 floats), spec = (vmax m/s, rmw m, r0 m, ztop m),
+ * mid = 1 for a mid-level vortex. Writes tab (nz x nr float4: tangential wind, theta', 0, 0; tab_floats is its
+ * capacity, nz * 2048 * 4 always suffices) and info = (dr m, nr, central pressure deficit hPa, least core
+ * d theta / dz K/km, warm-core peak K). Returns 0, -1 for bad arguments, -2 if tab is too small. */
+int obl_vortex_table(int nz, double dz, double dx, double lat, const float* B, const double* spec, int mid,
+                     float* tab, size_t tab_floats, double* info);
 /* reset (model.js:1427-1440): lsf + init with the given uniform block (written
  * for dt = 1), clears phi/rhs, copies V0 -> V1. seed 0 = the reference's noise. */
 int obl_reset(obl_model* h, const float* uniforms, int seed);

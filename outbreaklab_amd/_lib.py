@@ -33,6 +33,7 @@ class Grid(C.Structure):
 # name -> (restype, argtypes)
 _SIGS = {
     "obl_abi_version": (C.c_int, []),
+    "obl_grid_bytes": (C.c_double, [C.c_void_p]),
     "obl_last_error": (C.c_char_p, [C.c_void_p]),
     "obl_global_error": (C.c_char_p, []),
     "obl_create": (C.c_int, [C.POINTER(Grid), C.POINTER(C.c_void_p)]),
@@ -43,6 +44,8 @@ _SIGS = {
     "obl_set_surface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "obl_set_prep_field": (C.c_int, [C.c_void_p, C.c_int]),
     "obl_set_vortex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
+    "obl_vortex_table": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p, C.c_size_t, C.c_void_p]),
     "obl_reset": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint]),
     "obl_maps_request": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),

@@ -485,12 +485,15 @@ extern "C" void obl_destroy(obl_model* h) {
   delete h;
 }
 
-// bytes of all fields (for the out-of-memory message, like gridBytes, model.js:1125-1128)
-static double grid_bytes(const obl_model* h) {
-  double v = (double)h->vcount() * 16 * 3, c = (double)h->ccount();
-  double pc = (double)h->pcount();
-  return v + c * (16 * 6 + 8 + 4) + pc * (4 * 5 + 16) + (double)h->nx * h->ny * 48;
+// device bytes a grid's fields take: V x 3, Q x 6, aux, phT; the padded solve's phi, rhs, tmp, hat and FP32 / FP64
+// Thomas scratch; the 2-D maps (for the out-of-memory message, the counterpart of gridBytes, model.js:1125-1128)
+static double fields_bytes(int nx, int ny, int nz, int pad) {
+  const double v = (double)(nx + 1) * (ny + 1) * (nz + 1), c = (double)nx * ny * nz;
+  const double pc = (double)(nx + 2 * pad) * (ny + 2 * pad) * nz;
+  return v * 16 * 3 + c * (16 * 6 + 8 + 4) + pc * (4 * 5 + 16) + (double)nx * ny * 48;
 }
+static double grid_bytes(const obl_model* h) { return fields_bytes(h->nx, h->ny, h->nz, h->pad); }
+extern "C" double obl_grid_bytes(const obl_grid* g) { return g ? fields_bytes(g->nx, g->ny, g->nz, g->pad) : 0.0; }
 
 extern "C" int obl_create(const obl_grid* g, obl_model** out) {
   if (!g || !out) return set_global("obl_create: null argument");

@@ -32,6 +32,8 @@ typedef struct {
   int (*set_surface)(obl_model*, const float*, size_t);
   int (*set_prep_field)(obl_model*, int);
   int (*set_vortex)(obl_model*, const float*, int, size_t);
+  int (*vortex_table)(int, double, double, double, const float*, const double*, int, float*, size_t, double*);
+  double (*grid_bytes)(const obl_grid*);
   int (*reset)(obl_model*, const float*, int);
   int (*step)(obl_model*, const float*, int, unsigned);
   int (*maps_request)(obl_model*, const float*, int, int*);
@@ -111,6 +113,8 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(set_surface, "obl_set_surface");
   SYM(set_prep_field, "obl_set_prep_field");
   SYM(set_vortex, "obl_set_vortex");
+  SYM(vortex_table, "obl_vortex_table");
+  SYM(grid_bytes, "obl_grid_bytes");
   SYM(reset, "obl_reset");
   SYM(step, "obl_step");
   SYM(maps_request, "obl_maps_request");
@@ -298,6 +302,57 @@ static napi_value js_set_vortex(napi_env env, napi_callback_info info) {
   float* f = tarr(env, argv[1], &b);
   int nr = (int)num(env, argv[2]);
   return ok_or_throw(env, A.set_vortex(h, f, f ? nr : 0, f ? b : 0), h);
+}
+
+/* vortexTable(nz, dz, dx, lat, Float32Array B, Float64Array(4) spec, mid) -> {tab: Float32Array(nz nr 4),
+ * info: [dr, nr, pc, minStab, warm]} (host code, no model handle) */
+static napi_value js_vortex_table(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  get_args(env, info, 7, argv);
+  if (!A.so) return throw_msg(env, "outbreak_hip library not loaded");
+  int nz = (int)num(env, argv[0]);
+  size_t bb, sb;
+  float* B = tarr(env, argv[4], &bb);
+  double* spec = tarr(env, argv[5], &sb);
+  if (nz < 2 || !B || bb < (size_t)(nz + 1) * 32 || !spec || sb < 32)
+    return throw_msg(env, "vortexTable: Float32Array((nz + 1) * 8) base state and Float64Array(4) spec expected");
+  size_t cap = (size_t)nz * OBL_VORTEX_MAX_RADII * 4;
+  float* tab = malloc(cap * 4);
+  double out[5];
+  if (!tab) return throw_msg(env, "vortexTable: out of host memory");
+  int rc = A.vortex_table(nz, num(env, argv[1]), num(env, argv[2]), num(env, argv[3]), B, spec, (int)num(env, argv[6]), tab,
+                          cap, out);
+  if (rc) {
+    free(tab);
+    return throw_msg(env, "setVortex: bad vortex");
+  }
+  size_t nf = (size_t)nz * (size_t)out[1] * 4;
+  napi_value ab, arr, inf, res, v;
+  void* data = NULL;
+  napi_status st = napi_create_arraybuffer(env, nf * 4, &data, &ab);
+  if (st == napi_ok) memcpy(data, tab, nf * 4);
+  free(tab);
+  if (st != napi_ok) return throw_msg(env, "vortexTable: N-API allocation failed");
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, nf, ab, 0, &arr));
+  CHECK_NAPI(napi_create_array_with_length(env, 5, &inf));
+  for (uint32_t i = 0; i < 5; i++) {
+    CHECK_NAPI(napi_create_double(env, out[i], &v));
+    CHECK_NAPI(napi_set_element(env, inf, i, v));
+  }
+  CHECK_NAPI(napi_create_object(env, &res));
+  CHECK_NAPI(napi_set_named_property(env, res, "tab", arr));
+  CHECK_NAPI(napi_set_named_property(env, res, "info", inf));
+  return res;
+}
+/* gridBytes(nx, ny, nz, dx, dz, pad): the device bytes obl_create allocates (host code) */
+static napi_value js_grid_bytes(napi_env env, napi_callback_info info) {
+  napi_value argv[6], out;
+  get_args(env, info, 6, argv);
+  if (!A.so) return throw_msg(env, "outbreak_hip library not loaded");
+  obl_grid g = {(int)num(env, argv[0]), (int)num(env, argv[1]), (int)num(env, argv[2]), num(env, argv[3]),
+                num(env, argv[4]), (int)num(env, argv[5])};
+  CHECK_NAPI(napi_create_double(env, A.grid_bytes(&g), &out));
+  return out;
 }
 
 static napi_value js_reset(napi_env env, napi_callback_info info) {
@@ -760,6 +815,8 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("setSurface", js_set_surface);
   EXPORT("setPrepField", js_set_prep_field);
   EXPORT("setVortex", js_set_vortex);
+  EXPORT("vortexTable", js_vortex_table);
+  EXPORT("gridBytes", js_grid_bytes);
   EXPORT("step", js_step);
   EXPORT("mapsRequest", js_maps_request);
   EXPORT("mapsCollect", js_maps_collect);

@@ -21,14 +21,12 @@ addon.load(process.env.OBL_LIB || path.join(__dirname, '..', 'liboutbreak_hip.so
 
 const DEG = Math.PI / 180;
 const clamp = (v, a, b) => Math.min(Math.max(v, a), b);
-const lerp = (a, b, t) => a + (b - a) * t;
 const dflt = (v, d) => (v === undefined || v === null ? d : v);
 const FIELD = { V: 0, Q1: 1, Q2: 2, Vt: 3, Q1t: 4, Q2t: 5, aux: 6, phT: 7, lsfA: 8, lsfB: 9, ciMap: 10, phi: 11, rhs: 12, bs: 13, sfc: 14, Vp: 15 };
 const GEMM = { fp32: 0, bf16x3: 1 };
 const THOMAS = { fp32: 0, fp64: 1, mixed: 2 };
-const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256, VX_MAXR = 2048;
+const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256;
 const CN_OLD = 0.4;  // hurricane mode: the share of the last step's pressure force applied before advection
-const smoothstep = (a, b, x) => { const t = clamp((x - a) / (b - a), 0, 1); return t * t * (3 - 2 * t); };
 
 // ── the uniform block: MODEL_FIELDS (model.js:23-37), UBlock (util.js:83-98) ──
 const MODEL_FIELDS = ['dims', 'dzv', 'step', 'sponge', 'turb', 'micro', 'misc', 'cor', 'ci', 'pad', 'micro2',
@@ -48,27 +46,31 @@ class UBlock {
   }
 }
 
-// ── grid helpers (model.js:1104-1154) ──
+// ── grid helpers (model.js:1104-1154): level counts, custom grids, the solve's padding, device bytes ──
+// (grid spacing m, levels over 20 km), model.js:1104
 const LEVELS_BY_DX = [[100, 160], [250, 100], [500, 80], [1000, 64], [1500, 60], [2000, 56], [3000, 52], [4000, 48]];
+const LOG_DX = LEVELS_BY_DX.map(r => Math.log(r[0]));
+const nearestMultiple = (x, m) => Math.round(x / m) * m;
+// levelsFor: linear in log(dx) between the table's rows, to the nearest multiple of 4; the end rows beyond it
 function levelsFor(dx) {
-  const T = LEVELS_BY_DX, l = Math.log(clamp(dx, T[0][0], T[T.length - 1][0]));
-  for (let i = 0; i < T.length - 1; i++) {
-    const a = Math.log(T[i][0]), b = Math.log(T[i + 1][0]);
-    if (l <= b) return Math.round(lerp(T[i][1], T[i + 1][1], (l - a) / (b - a)) / 4) * 4;
-  }
-  return T[T.length - 1][1];
+  const l = Math.log(clamp(dx, LEVELS_BY_DX[0][0], LEVELS_BY_DX[LEVELS_BY_DX.length - 1][0]));
+  const i = LOG_DX.findIndex((v, j) => j > 0 && l <= v);
+  if (i < 0) return LEVELS_BY_DX[LEVELS_BY_DX.length - 1][1];
+  const t = (l - LOG_DX[i - 1]) / (LOG_DX[i] - LOG_DX[i - 1]), n0 = LEVELS_BY_DX[i - 1][1], n1 = LEVELS_BY_DX[i][1];
+  return nearestMultiple(n0 + (n1 - n0) * t, 4);
 }
+// customGrid: spacing 100-4000 m (default 500) and a domain in km (default 256) -> cells per side, the nearest
+// multiple of 8 within 64..2040
 function customGrid(c) {
   const dx = clamp(+c.dx || 500, 100, 4000);
-  const side = km => clamp(Math.round((+km || 256) * 1000 / dx / 8) * 8, 64, 2040);
-  const nx = side(c.w), ny = side(c.h), nz = levelsFor(dx);
+  const [nx, ny] = [c.w, c.h].map(km => clamp(nearestMultiple((+km || 256) * 1000 / dx, 8), 64, 2040));
+  const nz = levelsFor(dx);
   return { key: `custom:${dx}:${nx}x${ny}x${nz}`, dx, n: [nx, ny, nz] };
 }
-function gridPad(dx) { return clamp(Math.round(10000 / dx / 8) * 8, 8, 64); }
-function gridBytes(g) {
-  const [nx, ny, nz] = g.n, p = gridPad(g.dx), nxp = nx + 2 * p, nyp = ny + 2 * p;
-  return 1.05 * ((nx + 1) * (ny + 1) * (nz + 1) * 48 + nx * ny * nz * 108 + nxp * nyp * nz * 20 + nx * ny * 192 + 8 * (nxp * nxp + nyp * nyp));
-}
+// gridPad: about 10 km of padding cells per side, the nearest multiple of 8 within 8..64
+function gridPad(dx) { return clamp(nearestMultiple(10000 / dx, 8), 8, 64); }
+// the device bytes the library allocates for a grid (its own layout; the counterpart of gridBytes)
+function gridBytes(g) { return addon.gridBytes(g.n[0], g.n[1], g.n[2], g.dx, 20000 / g.n[2], gridPad(g.dx)); }
 const GRID_PRESETS = {
   quick: { dx: 2000, n: [320, 320, 56] }, standard: { dx: 1500, n: [448, 448, 60] }, large: { dx: 1500, n: [640, 640, 60] },
   fine: { dx: 1000, n: [640, 640, 64] }, m500: { dx: 500, n: [512, 512, 80] }, m250: { dx: 250, n: [512, 512, 100] },
@@ -76,99 +78,15 @@ const GRID_PRESETS = {
   h15: { dx: 1500, n: [960, 960, 60] },
 };
 
-// ── soundings: SHARPpy parser, winds and the hydrostatic base state (sounding.js:10-197) ──
-const Sounding = (function () {
-  const KT = 0.514444, RD = 287.04, CP = 1005.7, EPS = 0.622, G = 9.80665;
-  function parseSharppy(text) {
-    const out = { title: '', station: '', time: null, levels: [] };
-    let mode = '';
-    for (const raw of text.replace(/\r/g, '').split('\n')) {
-      const L = raw.trim();
-      if (!L) continue;
-      if (/^%TITLE%/i.test(L)) { mode = 'title'; continue; }
-      if (/^%RAW%/i.test(L)) { mode = 'raw'; continue; }
-      if (/^%END%/i.test(L)) { mode = ''; continue; }
-      if (mode === 'title') { if (!out.title) { out.title = L; out.station = L.split(/\s+/)[0]; } continue; }
-      if (mode === 'raw' || (!mode && /^-?\d/.test(L))) {
-        const f = L.split(/[,\s]+/).filter(Boolean).map(Number);
-        if (f.length < 6 || f.some(v => !isFinite(v))) continue;
-        const [p, h, t, td, wd, ws] = f;
-        if (p <= 0 || h < -1000 || h > 40000 || p === -9999) continue;
-        out.levels.push({ p, h, t: t > -9990 ? t : NaN, td: td > -9990 ? td : NaN, wd: wd > -9990 ? wd : NaN, ws: ws > -9990 ? ws : NaN });
-      }
-    }
-    out.levels.sort((a, b) => b.p - a.p);
-    if (out.levels.length < 4) throw new Error('Could not read at least 4 levels. Expected SHARPpy format: %RAW% rows of  PRES, HGHT, TEMP, DWPT, WDIR, WSPD(kt)  ... %END%');
-    return out;
-  }
-  function es(t) { return 6.112 * Math.exp(17.67 * t / (t + 243.5)); }
-  function mixr(p, t) { const e = es(t); return EPS * e / Math.max(p - e, 1e-3); }
-  function winds(snd) {
-    const lv = snd.levels.filter(l => isFinite(l.h));
-    const sfcH = lv[0].h;
-    const wl = lv.filter(l => isFinite(l.wd) && isFinite(l.ws)).map(l => {
-      const d = l.wd * Math.PI / 180, s = l.ws * KT;
-      return { z: (l.h - sfcH) / 1000, u: -s * Math.sin(d), v: -s * Math.cos(d) };
-    });
-    return function windAt(z) {
-      if (z <= wl[0].z) return [wl[0].u, wl[0].v];
-      for (let i = 0; i < wl.length - 1; i++) {
-        if (z >= wl[i].z && z <= wl[i + 1].z) {
-          const f = (z - wl[i].z) / Math.max(1e-6, wl[i + 1].z - wl[i].z);
-          return [wl[i].u + (wl[i + 1].u - wl[i].u) * f, wl[i].v + (wl[i + 1].v - wl[i].v) * f];
-        }
-      }
-      const l = wl[wl.length - 1];
-      return [l.u, l.v];
-    };
-  }
-  function baseState(snd, nz, dz, motion) {
-    const lv = snd.levels.filter(l => isFinite(l.h) && isFinite(l.t)).slice().sort((a, b) => a.h - b.h);
-    const sfcH = lv[0].h, sfcP = lv[0].p, top = lv[lv.length - 1];
-    const byH = (key, zAgl) => {
-      const h = zAgl + sfcH;
-      if (h <= lv[0].h) return lv[0][key];
-      for (let i = 0; i < lv.length - 1; i++) {
-        if (h <= lv[i + 1].h) {
-          const a = lv[i], b = lv[i + 1], f = (h - a.h) / Math.max(b.h - a.h, 1e-3), va = a[key], vb = b[key];
-          if (!isFinite(va) || !isFinite(vb)) return isFinite(va) ? va : vb;
-          if (key === 'p') return Math.exp(Math.log(va) + (Math.log(vb) - Math.log(va)) * f);
-          return va + (vb - va) * f;
-        }
-      }
-      return NaN;
-    };
-    const windAt = winds(snd);
-    const B = new Float32Array((nz + 1) * 8), th = [], qv = [];
-    for (let k = 0; k < nz; k++) {
-      const z = (k + 0.5) * dz;
-      let T = byH('t', z), Td = byH('td', z), p = byH('p', z);
-      if (!isFinite(T) || !isFinite(p)) {
-        const dzt = z + sfcH - top.h;
-        T = top.t; p = top.p * Math.exp(-G * dzt / (RD * (top.t + 273.15))); Td = Math.min(top.td, -80);
-      }
-      if (!isFinite(Td)) Td = T - 30;
-      Td = Math.min(Td, T - 0.3);
-      th.push((T + 273.15) * Math.pow(1000 / p, RD / CP));
-      qv.push(Math.max(1e-7, mixr(p, Td)));
-    }
-    const piF = new Float64Array(nz + 1);
-    piF[0] = Math.pow(sfcP / 1000, RD / CP);
-    for (let k = 0; k < nz; k++) piF[k + 1] = piF[k] - G * dz / (CP * th[k] * (1 + 0.61 * qv[k]));
-    for (let k = 0; k < nz; k++) {
-      const pi = 0.5 * (piF[k] + piF[k + 1]), p = 100000 * Math.pow(pi, CP / RD), T = th[k] * pi;
-      const esx = 611.2 * Math.exp(17.67 * (T - 273.15) / (T - 29.65)), qvs = EPS * esx / (p - esx);
-      qv[k] = Math.min(qv[k], 0.97 * qvs);
-      const rho = p / (RD * T * (1 + 0.61 * qv[k])), w = windAt((k + 0.5) * dz / 1000);
-      B.set([th[k], qv[k], rho, 0, pi, p, w[0] - motion[0], w[1] - motion[1]], k * 8);
-    }
-    for (let k = 0; k <= nz; k++) {
-      B[k * 8 + 3] = k === 0 ? B[2] + (B[2] - B[10]) * 0.5 : k === nz ? B[(nz - 1) * 8 + 2] * 0.94 : 0.5 * (B[(k - 1) * 8 + 2] + B[k * 8 + 2]);
-    }
-    return { B, sfcP, sfcH };
-  }
-  return { parseSharppy, baseState, winds, mixr, KT };
-})();
+// ── soundings: the caller's Sounding module ──
+// The base state comes from the application's own sounding code (sounding.js's Sounding: parseSharppy, analyze,
+// baseState), as in the reference, where model.js calls it from setSounding; this facade does not carry a copy.
+// useSounding(S) hands it over once; setBaseState takes a prepared table instead.
+let SOUNDING = null;
+function useSounding(S) {
+  if (!S || typeof S.parseSharppy !== 'function' || typeof S.baseState !== 'function') throw new Error('useSounding: a Sounding module with parseSharppy and baseState expected');
+  SOUNDING = S;
+}
 
 // ── a minimal stand-in for the driver's GPU object (util.js:101-191): queue calls only ──
 function makeGPU() {
@@ -215,46 +133,12 @@ class Field {
   get ptr() { return addon.fieldPtr(this.m.h, FIELD[this.name], this.copy); }
 }
 
-// setVortex's table (model.js:1318-1365): v(r, z) = Vs(r) F(z), Vs a Rankine-type profile made inertially stable,
-// phi from the gradient-wind balance, theta' = theta0 (d phi / dz) / g; the same arithmetic as the Python host
+// setVortex's table (model.js:1318-1365), built by the library's host routine obl_vortex_table (the one the Python
+// host calls): tangential wind v(r, z) = V(r) F(z), phi in gradient-wind balance, theta' = theta0 (d phi / dz) / g
 function vortexTable(nz, dz, dx, lat, B, v) {
-  const f = 2 * 7.292e-5 * Math.sin(lat * DEG), dr = Math.max(500, dx / 2), nr = Math.min(VX_MAXR, Math.ceil(v.r0 * 2 / dr) + 3);
-  const sg = f < 0 ? -1 : 1, fa = Math.abs(f), al = 0.6, vs = new Float64Array(nr);
-  let Mp = 0;
-  for (let i = 0; i < nr; i++) {
-    const r = i * dr, x = r / v.rmw;
-    let s = v.vmax * (1 + al) * x / (al + Math.pow(x, 1 + al)) * (1 - smoothstep(0.5 * v.r0, v.r0, r));
-    if (r > 0) s = Math.max(s, (Mp - 0.5 * fa * r * r) / r, 0);
-    vs[i] = s; Mp = r * s + 0.5 * fa * r * r;
-  }
-  const Vs = r => sg * vs[Math.min(nr - 1, Math.round(r / dr))];
-  const Fz = v.shape === 'mid' ? z => Math.exp(-Math.pow((z - 5000) / 2500, 2))
-    : z => (z >= v.ztop ? 0 : 1 - (z / v.ztop - Math.sin(2 * Math.PI * z / v.ztop) / (2 * Math.PI)));
-  const vt = [], ph = [];
-  for (let k = 0; k < nz; k++) {
-    const F = Fz((k + 0.5) * dz), vk = new Float64Array(nr), pk = new Float64Array(nr);
-    for (let i = 0; i < nr; i++) vk[i] = Vs(i * dr) * F;
-    const g = i => (i === 0 ? 0 : vk[i] * vk[i] / (i * dr) + f * vk[i]);
-    for (let i = nr - 2; i >= 0; i--) pk[i] = pk[i + 1] - 0.5 * (g(i) + g(i + 1)) * dr;
-    vt.push(vk); ph.push(pk);
-  }
-  const tab = new Float32Array(nz * nr * 4);
-  for (let k = 0; k < nz; k++) for (let i = 0; i < nr; i++) {
-    const k0 = Math.max(k - 1, 0), k1 = Math.min(k + 1, nz - 1), b = (ph[k1][i] - ph[k0][i]) / ((k1 - k0) * dz);
-    tab[(k * nr + i) * 4] = vt[k][i]; tab[(k * nr + i) * 4 + 1] = B[k * 8] * b / 9.81;
-  }
-  let minStab = 1e9, warm = 0;
-  for (let k = 1; k < nz; k++) {
-    const z = k * dz;
-    if (z < 2000 || z > v.ztop) continue;
-    for (let i = 0; i < Math.min(nr, Math.ceil(2 * v.rmw / dr)); i++) {
-      const a = B[(k - 1) * 8] + tab[((k - 1) * nr + i) * 4 + 1], b = B[k * 8] + tab[(k * nr + i) * 4 + 1];
-      minStab = Math.min(minStab, (b - a) / dz * 1000);
-    }
-    warm = Math.max(warm, tab[(k * nr) * 4 + 1]);
-  }
-  const info = { x: v.x, y: v.y, dr, nr, moistR: v.moistR || 1, moistRH: v.moistR ? v.moistRH : 0, pc: ph[0][0] * B[2] / 100, minStab, warm };
-  return { tab, info };
+  const r = addon.vortexTable(nz, dz, dx, lat, Float32Array.from(B), Float64Array.from([v.vmax, v.rmw, v.r0, v.ztop]), v.shape === 'mid' ? 1 : 0);
+  const [dr, nr, pc, minStab, warm] = r.info;
+  return { tab: r.tab, info: { x: v.x, y: v.y, dr, nr, moistR: v.moistR || 1, moistRH: v.moistR ? v.moistRH : 0, pc, minStab, warm } };
 }
 
 class RegionalModel {
@@ -307,12 +191,22 @@ class RegionalModel {
     GPU.untrack(this);
     this.h = null; this.V = null; this.solver = null;
   }
+  // setSounding (model.js:1276-1288): the caller's Sounding module parses and analyses the text and builds the base
+  // state; returns its analysis
   setSounding(text) {
-    const snd = Sounding.parseSharppy(text);
-    const bsx = Sounding.baseState(snd, this.nz, this.dz, [0, 0]);
-    this.snd = snd; this.base = bsx; this.rhoSfc = bsx.B[2];
-    addon.setBaseState(this.h, bsx.B, this.nz);
-    return { snd, windAt: Sounding.winds(snd) };
+    if (!SOUNDING) throw new Error('setSounding: no Sounding module (useSounding(S) first, or setBaseState)');
+    const snd = SOUNDING.parseSharppy(text);
+    const A = typeof SOUNDING.analyze === 'function' ? SOUNDING.analyze(snd) : null;
+    this.analysis = A; this.snd = snd;
+    this.setBaseState(SOUNDING.baseState(snd, this.nz, this.dz, [0, 0]));
+    return A;
+  }
+  // the base state table (bsx.B: (nz + 1) * 8 floats, sounding.js:139-197) -> bs and the solver's densities
+  setBaseState(bsx) {
+    const B = Float32Array.from(bsx.B);
+    if (B.length !== (this.nz + 1) * 8) throw new Error('setBaseState: ' + (this.nz + 1) * 8 + ' floats expected');
+    this.base = Object.assign({}, bsx, { B }); this.rhoSfc = B[2];
+    addon.setBaseState(this.h, B, this.nz);
   }
   // the extra wind field the trajectory-averaged pressure force needs (hurricane runs only)
   setPrepField(on) {
@@ -494,7 +388,7 @@ class Surrogate {
   infer(rows, stride) { return addon.mlpInfer(this.h, rows instanceof Float32Array ? rows : Float32Array.from(rows), stride || 24); }
 }
 
-// ── virtual WSR-88D radars (radar.js:18-418), azimuthal-equidistant projection (geo.js:6-29) ──
+// ── virtual WSR-88D radars: volume coverage patterns, moments and Archive II scales (radar.js:18-39) ──
 const RE_M = 6371000, KE = 4 / 3;
 const GATE0 = 2125, GATEDR = 250, MAX_REF_GATES = 1832, MAX_DOP_GATES = 1192;
 const L2_SCALE = { REF: [2, 66], VEL: [2, 129], SW: [2, 129], ZDR: [16, 128], PHI: [253 / 360, 2], RHO: [300, -60.5] };
@@ -540,32 +434,41 @@ class Rng {
   }
 }
 
-class Geo {
-  constructor(lat0, lon0, Lx, Ly, groundMSL) {
-    Object.assign(this, { lat0, lon0, Lx: Lx || 0, Ly: Ly || 0, groundMSL: groundMSL || 0 });
-    this.p0 = lat0 * DEG; this.l0 = lon0 * DEG; this.sp0 = Math.sin(this.p0); this.cp0 = Math.cos(this.p0);
-  }
-  fwd(lat, lon) {
-    const p = lat * DEG, dl = lon * DEG - this.l0, sp = Math.sin(p), cp = Math.cos(p), cdl = Math.cos(dl);
-    const c = Math.acos(clamp(this.sp0 * sp + this.cp0 * cp * cdl, -1, 1));
-    const k = c < 1e-9 ? 1 : c / Math.sin(c);
-    return [RE_M * k * cp * Math.sin(dl), RE_M * k * (this.cp0 * sp - this.sp0 * cp * cdl)];
-  }
-  inv(x, y) {
-    const rho = Math.hypot(x, y);
-    if (rho < 1e-6) return [this.lat0, this.lon0];
-    const c = rho / RE_M, sc = Math.sin(c), cc = Math.cos(c);
-    const p = Math.asin(cc * this.sp0 + y * sc * this.cp0 / rho);
-    const l = this.l0 + Math.atan2(x * sc, rho * this.cp0 * cc - y * this.sp0 * sc);
-    return [p / DEG, ((l / DEG + 540) % 360) - 180];
-  }
-  toModel(lat, lon) { const xy = this.fwd(lat, lon); return [xy[0] + this.Lx / 2, xy[1] + this.Ly / 2]; }
-  fromModel(mx, my) { return this.inv(mx - this.Lx / 2, my - this.Ly / 2); }
+// Gate counts of a site: enough 250 m gates to reach the domain's farthest corner plus 10 km (at most 460 km),
+// a multiple of 4 (four gates pack one 32-bit word of codes), the Doppler moments capped shorter (radar.js:289-300)
+function gateCounts(sx, sy, Lx, Ly) {
+  const far = Math.max(...[[0, 0], [Lx, 0], [0, Ly], [Lx, Ly]].map(c => Math.hypot(c[0] - sx, c[1] - sy)));
+  const want = Math.ceil((Math.min(460000, far + 10000) - GATE0) / GATEDR);
+  const ngRef = Math.max(64, Math.ceil(Math.min(MAX_REF_GATES, want) / 4) * 4);
+  return { ngRef, ngDop: Math.min(MAX_DOP_GATES, ngRef) };
+}
+// A volume's byte layout and timing: cut by cut, each moment's radials x gates of 8-bit codes follow one another
+// (the layout the Level II writer reads); each cut gets a share of the VCP's duration, super-resolution cuts
+// twice the share of the others (radar.js:301-316)
+function layVolume(vcp, ngRef, ngDop) {
+  const share = c => (c.superRes ? 2 : 1);
+  const shares = vcp.cuts.reduce((n, c) => n + share(c), 0);
+  let bytes = 0, before = 0;
+  const cuts = vcp.cuts.map((c, index) => {
+    const nrad = c.superRes ? 720 : 360;
+    const mom = CUT_MOMENTS[c.kind].map(name => {
+      const ngates = name === 'REF' ? ngRef : ngDop, at = bytes;
+      bytes += nrad * ngates;
+      return { name, ngates, off: at };
+    });
+    const t0 = before / shares * vcp.dur;
+    before += share(c);
+    return Object.assign({}, c, { index, nrad, mom, t0, dur: share(c) / shares * vcp.dur });
+  });
+  return { cuts, bytes };
 }
 
+// Virtual WSR-88D radars over the model (the host half of radar.js's RadarSim, radar.js:271-418; the kernels are
+// the library's). geo is the application's map projection (geo.js's GEO: toModel(lat, lon), lat0, lon0,
+// groundMSL), as in the reference.
 class RadarSim {
   constructor(model, geo) { this.m = model; this.geo = geo; this.radars = []; this.nTor = 0; }
-  // tornadoes: [{x, y, R, v, debrisTop, debrisDbz, debris}] model metres
+  // sub-grid tornadoes: [{x, y, R, v, debrisTop, debrisDbz, debris}] model metres, at most 64
   setTornadoes(list) {
     const act = list.slice(0, 64), f = new Float32Array(Math.max(1, act.length) * 8);
     act.forEach((t, i) => f.set([t.x, t.y, t.R, t.v, t.debrisTop, t.debrisDbz, t.debris, 0], i * 8));
@@ -579,84 +482,73 @@ class RadarSim {
     this.radars.push(R);
     return R;
   }
-  // addRadar's layout without the device buffers (radar.js:289-316)
-  static plan(m, g, site, opts) {
-    opts = opts || {};
-    const vcp = VCPS[opts.vcp] || VCPS[212];
-    const sxy = g.toModel(site.lat, site.lon), sx = sxy[0], sy = sxy[1];
-    let far = 0;
-    [[0, 0], [m.Lx, 0], [0, m.Ly], [m.Lx, m.Ly]].forEach(c => { far = Math.max(far, Math.hypot(c[0] - sx, c[1] - sy)); });
-    const rng = Math.min(460000, far + 10000);
-    let ngRef = Math.min(MAX_REF_GATES, Math.ceil((rng - GATE0) / GATEDR));
-    ngRef = Math.max(64, Math.ceil(ngRef / 4) * 4);
-    const ngDop = Math.min(MAX_DOP_GATES, ngRef);
-    let off = 0;
-    const cuts = vcp.cuts.map((c, ci) => {
-      const nrad = c.superRes ? 720 : 360;
-      const mom = CUT_MOMENTS[c.kind].map(name => {
-        const ng = name === 'REF' ? ngRef : ngDop, o = { name, ngates: ng, off };
-        off += nrad * ng;
-        return o;
-      });
-      return Object.assign({}, c, { index: ci, nrad, mom });
-    });
-    const wsum = cuts.reduce((s, c) => s + (c.superRes ? 2 : 1), 0);
-    let acc = 0;
-    cuts.forEach(c => { const w = c.superRes ? 2 : 1; c.t0 = acc / wsum * vcp.dur; acc += w; c.dur = w / wsum * vcp.dur; });
-    const rand = opts.seed !== undefined ? (r => () => r.u())(new Rng(opts.seed)) : Math.random;
+  // a radar's layout, timing and schedule state, without its device buffers
+  static plan(m, geo, site, opts) {
+    const o = opts || {};
+    const vcp = VCPS[o.vcp] || VCPS[212];
+    const [sx, sy] = geo.toModel(site.lat, site.lon);
+    const { ngRef, ngDop } = gateCounts(sx, sy, m.Lx, m.Ly);
+    const { cuts, bytes } = layVolume(vcp, ngRef, ngDop);
+    const rng = o.seed !== undefined ? new Rng(o.seed) : null;
+    const rand = rng ? () => rng.u() : Math.random;
     return {
-      site, opts, vcp, vcpId: +opts.vcp || 212, cuts, bytes: off, ngRef, ngDop, sx, sy, rand, U: new UBlock(RADAR_FIELDS),
-      h: null, readBusy: [false, false], phase: rand() * vcp.dur, vol: -1, next: 0, volStart: 0, cutsDone: 0, az0: 0,
-      done: 0, failed: 0,
+      site, opts: o, vcp, vcpId: +o.vcp || 212, cuts, bytes, ngRef, ngDop, sx, sy, rand, U: new UBlock(RADAR_FIELDS), h: null,
+      readBusy: [false, false], phase: rand() * vcp.dur, vol: -1, next: 0, volStart: 0, az0: 0,
     };
   }
   destroy() {
     if (this.m.h) this.radars.forEach(R => addon.radarDestroy(R.h));  // else the model freed them
     this.radars = [];
   }
-  // encode every cut whose scan time has been reached; returns the volumes that completed
+  // Enqueue, in scan order, every cut whose start the model clock simT has reached; returns the volumes that
+  // completed (their codes are on their way to a readback slot).
   encDue(enc, simT) {
-    const done = [];
+    const finished = [];
     for (const R of this.radars) {
-      for (;;) {
-        if (R.vol < 0 || R.next >= R.cuts.length) {
-          const vstart = R.vol < 0 ? R.phase : R.volStart + R.vcp.dur;
-          if (simT < vstart || (R.readBusy[0] && R.readBusy[1])) break;
-          R.vol++; R.volStart = vstart; R.next = 0; R.az0 = R.rand();
-        }
-        const c = R.cuts[R.next];
-        if (simT < R.volStart + c.t0) break;
+      for (let c = RadarSim.dueCut(R, simT); c; c = RadarSim.dueCut(R, simT)) {
         this.encCut(enc, R, c, simT);
-        R.next++;
-        if (R.next >= R.cuts.length) {
-          const slot = addon.radarReadback(this.m.h, R.h);
-          if (slot < 0) throw new Error('radar ' + R.site.id + ': both readback slots are pending');
-          R.readBusy[slot] = true;
-          done.push({ R, slot, vol: R.vol, volStart: R.volStart, az0: R.az0 });
-        }
+        if (++R.next === R.cuts.length) finished.push(this.endVolume(R));
       }
     }
-    return done;
+    return finished;
   }
-  // the cut's RADAR_FIELDS block (radar.js:369-382)
+  // the radar's next cut if simT has reached it, opening the next volume (a new random start azimuth) once the
+  // last one is complete, its start time has come and a readback slot is free for it; else null
+  static dueCut(R, simT) {
+    if (R.vol < 0 || R.next >= R.cuts.length) {
+      const start = R.vol < 0 ? R.phase : R.volStart + R.vcp.dur;
+      if (simT < start || R.readBusy.every(Boolean)) return null;
+      R.vol += 1; R.volStart = start; R.next = 0; R.az0 = R.rand();
+    }
+    const c = R.cuts[R.next];
+    return simT >= R.volStart + c.t0 ? c : null;
+  }
+  // the complete volume goes to a readback slot
+  endVolume(R) {
+    const slot = addon.radarReadback(this.m.h, R.h);
+    if (slot < 0) throw new Error('radar ' + R.site.id + ': both readback slots are pending');
+    R.readBusy[slot] = true;
+    return { R, slot, vol: R.vol, volStart: R.volStart, az0: R.az0 };
+  }
   cutBlock(R, c, simT) { return RadarSim.block(this.m, this.geo, R, c, simT, this.nTor); }
-  static block(m, g, R, c, simT, nTor) {
-    const U = R.U, off = {};
-    c.mom.forEach(o => { off[o.name] = o.off / 4; });
-    let mask = 0;
-    MOMENT_ORDER.forEach((n, i) => { if (off[n] !== undefined) mask |= 1 << i; });
-    const o = n => dflt(off[n], 0);
+  // the cut's RADAR_FIELDS block (radar.js:369-384): site, projection, cut geometry, gates, Nyquist and clock, each
+  // moment's word offset in the volume with a mask of the moments present, the model grid and its physics switches
+  static block(m, geo, R, c, simT, nTor) {
+    const U = R.U, word = {};
+    c.mom.forEach(x => { word[x.name] = x.off / 4; });
+    const mask = MOMENT_ORDER.reduce((b, n, i) => (n in word ? b | (1 << i) : b), 0);
+    const w = n => (n in word ? word[n] : 0);
     U.set('site', R.sx, R.sy, R.site.h + 20, KE * RE_M);
-    U.set('geo', g.lat0 * DEG, g.lon0 * DEG, R.site.lat * DEG, R.site.lon * DEG);
+    U.set('geo', geo.lat0 * DEG, geo.lon0 * DEG, R.site.lat * DEG, R.site.lon * DEG);
     U.set('cut', c.angle * DEG, R.az0, 360 / c.nrad, c.nrad);
     U.set('gates', R.ngRef, R.ngDop, GATE0, GATEDR);
     U.set('nyq', R.opts.aliasing === false ? 63.5 : c.nyquist, simT, (R.vol * 97 + c.index * 13) % 100000, 25);
-    U.set('offA', o('REF'), o('VEL'), o('SW'), o('ZDR'));
-    U.set('offB', o('PHI'), o('RHO'), 0, mask);
+    U.set('offA', w('REF'), w('VEL'), w('SW'), w('ZDR'));
+    U.set('offB', w('PHI'), w('RHO'), 0, mask);
     U.set('mdl', m.nx, m.ny, m.nz, m.dx);
-    U.set('mdl2', m.dz, g.groundMSL, -m.Lx / 2, -m.Ly / 2);
+    U.set('mdl2', m.dz, geo.groundMSL || 0, -m.Lx / 2, -m.Ly / 2);
     U.set('phys', m.rhoSfc || 1.15, dflt(R.opts.minDbz, 0), m.params.scheme === 'thompson' ? 1 : 0, nTor || 0);
-    U.set('phys2', m.hur && m.hur.maritime ? 1 : 0, 0, 0, 0);  // radar.js:384
+    U.set('phys2', m.hur && m.hur.maritime ? 1 : 0, 0, 0, 0);
     return U.f;
   }
   encCut(enc, R, c, simT) {
@@ -664,7 +556,17 @@ class RadarSim {
     addon.radarCut(this.m.h, R.h, this.cutBlock(R, c, simT), this.m.parity);
     c.simT = simT;
   }
-  // a finished volume's bytes and the header the server reads (radar.js:394-416)
+  // the header the upload server reads for a finished volume (radar.js:394-416; server.py:249-284)
+  static header(R, d, run, epoch0) {
+    const t0 = dflt(epoch0, 0) + d.volStart * 1000, nyq = c => (R.opts.aliasing === false ? 63.5 : c.nyquist);
+    const sweep = c => ({
+      angle: c.angle, superRes: c.superRes, kind: c.kind, az0: d.az0, nrad: c.nrad, nyquist: nyq(c),
+      unamb_km: c.kind === 'surv' ? 466 : 300, start_ms: Math.round(t0 + c.t0 * 1000), dur_ms: Math.round(c.dur * 1000),
+      moments: c.mom.map(x => ({ name: x.name, ngates: x.ngates, off: x.off })),
+    });
+    return { run, icao: R.site.id, lat: R.site.lat, lon: R.site.lon, elev: R.site.h + 20, vcp: R.vcpId, epoch_ms: Math.round(t0), sweeps: R.cuts.map(sweep) };
+  }
+  // a finished volume's codes and header; frees its readback slot
   collect(d, run, epoch0) {
     const R = d.R;
     return new Promise((resolve, reject) => setImmediate(() => {
@@ -672,17 +574,7 @@ class RadarSim {
         const bytes = new Uint8Array(R.bytes);
         addon.radarCollect(this.m.h, R.h, d.slot, bytes);  // a failed collect leaves the slot pending
         R.readBusy[d.slot] = false;
-        const t0 = dflt(epoch0, 0) + d.volStart * 1000;
-        const header = {
-          run, icao: R.site.id, lat: R.site.lat, lon: R.site.lon, elev: R.site.h + 20, vcp: R.vcpId, epoch_ms: Math.round(t0),
-          sweeps: R.cuts.map(c => ({
-            angle: c.angle, superRes: c.superRes, kind: c.kind, az0: d.az0, nrad: c.nrad,
-            nyquist: R.opts.aliasing === false ? 63.5 : c.nyquist, unamb_km: c.kind === 'surv' ? 466 : 300,
-            start_ms: Math.round(t0 + c.t0 * 1000), dur_ms: Math.round(c.dur * 1000),
-            moments: c.mom.map(o => ({ name: o.name, ngates: o.ngates, off: o.off })),
-          })),
-        };
-        resolve({ R, header, bytes });
+        resolve({ R, header: RadarSim.header(R, d, run, epoch0), bytes });
       } catch (e) { reject(e); }
     }));
   }
@@ -697,4 +589,4 @@ function volumeBody(header, bytes) {
   return zlib.gzipSync(Buffer.concat([pre, hj, Buffer.from(bytes.buffer, bytes.byteOffset, bytes.byteLength)]));
 }
 
-module.exports = { vortexTable, volumeBody, RadarSim, Geo, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, Sounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };
+module.exports = { vortexTable, volumeBody, RadarSim, Rng, VCPS, L2_SCALE, RADAR_FIELDS, Surrogate, RegionalModel, PoissonDirect, useSounding, UBlock, MODEL_FIELDS, GPU, levelsFor, customGrid, gridPad, gridBytes, GRID_PRESETS, addon };

@@ -55,75 +55,25 @@ class Field:
 VX_MAXR = 2048  # radii of the vortex table (model.js:38)
 
 
-def _smooth(a, b, x):
-    t = min(max((x - a) / (b - a), 0.0), 1.0)
-    return t * t * (3 - 2 * t)
-
-
 def vortex_table(nz, dz, dx, lat, B, v):
-    """setVortex's table (model.js:1318-1365) in FP64, stored FP32: tangential wind v(r, z) = Vs(r) F(z) with Vs a
-    Rankine-type profile vmax (1+a) x / (a + x^(1+a)), x = r / rmw, a = 0.6, tapered to r0 and held inertially
-    stable (the absolute angular momentum never decreases outward); phi(r) integrated inward from the gradient-wind
-    balance d phi / dr = v^2 / r + f v; theta' = theta0 (d phi / dz) / g. Returns (table nz x nr x 4, info)."""
-    f = 2 * 7.292e-5 * math.sin(lat * DEG)
-    dr = max(500.0, dx / 2)
-    nr = min(VX_MAXR, math.ceil(v["r0"] * 2 / dr) + 3)
-    sg = -1 if f < 0 else 1
-    fa, al = abs(f), 0.6
-    vs = np.zeros(nr)
-    Mp = 0.0
-    for i in range(nr):
-        r = i * dr
-        x = r / v["rmw"]
-        s = v["vmax"] * (1 + al) * x / (al + x ** (1 + al)) * (1 - _smooth(0.5 * v["r0"], v["r0"], r))
-        if r > 0:
-            s = max(s, (Mp - 0.5 * fa * r * r) / r, 0.0)
-        vs[i] = s
-        Mp = r * s + 0.5 * fa * r * r
-
-    def Vs(r):
-        return sg * vs[min(nr - 1, _jsround(r / dr))]
-
-    if v.get("shape") == "mid":
-        def Fz(z):
-            return math.exp(-((z - 5000) / 2500) ** 2)
-    else:
-        def Fz(z):
-            return 0.0 if z >= v["ztop"] else 1 - (z / v["ztop"] - math.sin(2 * math.pi * z / v["ztop"]) / (2 * math.pi))
-    vt, ph = [], []
-    for k in range(nz):
-        F = Fz((k + 0.5) * dz)
-        vk = np.array([Vs(i * dr) * F for i in range(nr)])
-        g = np.array([0.0] + [vk[i] * vk[i] / (i * dr) + f * vk[i] for i in range(1, nr)])
-        pk = np.zeros(nr)
-        for i in range(nr - 2, -1, -1):
-            pk[i] = pk[i + 1] - 0.5 * (g[i] + g[i + 1]) * dr
-        vt.append(vk)
-        ph.append(pk)
-    tab = np.zeros((nz, nr, 4), np.float32)
-    for k in range(nz):
-        k0, k1 = max(k - 1, 0), min(k + 1, nz - 1)
-        b = (ph[k1] - ph[k0]) / ((k1 - k0) * dz)
-        tab[k, :, 0] = vt[k]
-        tab[k, :, 1] = float(B[k * 8]) * b / 9.81
-    min_stab, warm = 1e9, 0.0
-    for k in range(1, nz):
-        z = k * dz
-        if z < 2000 or z > v["ztop"]:
-            continue
-        for i in range(min(nr, math.ceil(2 * v["rmw"] / dr))):
-            a = float(B[(k - 1) * 8]) + float(tab[k - 1, i, 1])
-            bb = float(B[k * 8]) + float(tab[k, i, 1])
-            min_stab = min(min_stab, (bb - a) / dz * 1000)
-        warm = max(warm, float(tab[k, 0, 1]))
+    """setVortex's table (model.js:1318-1365), built by the library's host routine obl_vortex_table (the one the
+    Node host calls too): tangential wind v(r, z) = V(r) F(z) with V a Rankine-type profile held inertially stable,
+    phi in gradient-wind balance, theta' = theta0 (d phi / dz) / g. Returns (table nz x nr x 4 float32, info)."""
+    spec = np.array([v["vmax"], v["rmw"], v["r0"], v["ztop"]], np.float64)
+    Bf = np.ascontiguousarray(B, np.float32)
+    tab = np.zeros((nz, VX_MAXR, 4), np.float32)
+    out = np.zeros(5)
+    rc = _lib.lib().obl_vortex_table(int(nz), float(dz), float(dx), float(lat), _ptr(Bf), _ptr(spec),
+                                     1 if v.get("shape") == "mid" else 0, _ptr(tab), tab.size, _ptr(out))
+    if rc != 0:
+        raise ValueError(f"setVortex: bad vortex (code {rc})")
+    nr = int(out[1])
+    tab = np.ascontiguousarray(tab.reshape(-1)[:nz * nr * 4].reshape(nz, nr, 4))
     mr = v.get("moistR") or 0
-    info = {"x": v["x"], "y": v["y"], "dr": dr, "nr": nr, "moistR": mr or 1, "moistRH": v.get("moistRH", 0) if mr else 0,
-            "pc": ph[0][0] * float(B[2]) / 100, "minStab": min_stab, "warm": warm}
+    info = {"x": v["x"], "y": v["y"], "dr": float(out[0]), "nr": nr, "moistR": mr or 1,
+            "moistRH": v.get("moistRH", 0) if mr else 0, "pc": float(out[2]), "minStab": float(out[3]),
+            "warm": float(out[4])}
     return tab, info
-
-
-def _jsround(x):
-    return int(math.floor(x + 0.5))
 
 
 class RegionalModel:

@@ -1,9 +1,14 @@
 // GPU run of the Node facade (run by tests/test_gpu_js.py): the 128 x 128 x 40, 2 km Thompson
 // workload driven through RegionalModel / the N-API addon for 20 steps; prints the state.
+// argv[2]: the output folder; argv[3]: a JSON file from the Python side with the WK base state (40 levels of
+// 500 m) that the application's Sounding module would build, and the radar site's position in the application's
+// projection (the facade takes both from the application, as model.js and radar.js do).
 'use strict';
 const fs = require('fs');
 const path = require('path');
 const O = require(path.join(__dirname, '..', 'outbreaklab_amd', 'js', 'outbreak.js'));
+const IN = JSON.parse(fs.readFileSync(process.argv[3], 'utf8'));
+O.useSounding({ parseSharppy: text => ({ text }), analyze: () => ({ stub: true }), baseState: () => ({ B: Float32Array.from(IN.B) }) });
 (async () => {
   const m = new O.RegionalModel();
   await m.allocate('custom', { dx: 2000, n: [128, 128, 40] });
@@ -26,11 +31,10 @@ const O = require(path.join(__dirname, '..', 'outbreaklab_amd', 'js', 'outbreak.
   const mout = path.join(process.argv[2] || '.', 'js_maps.bin');
   fs.writeFileSync(mout, Buffer.concat([0, 1, 2, 3].map(q => Buffer.from(maps['m' + q].slice().buffer))));
   // one seeded VCP 35 volume of a radar 40 km west / 30 km south of the centre, one sub-grid tornado
-  const geo = new O.Geo(35.9, -97.0, m.Lx, m.Ly, 0);
+  const geo = { lat0: 35.9, lon0: -97.0, groundMSL: 0, toModel: () => [IN.site.sx, IN.site.sy] };
   const sim = new O.RadarSim(m, geo);
   sim.setTornadoes([{ x: 0.5 * m.Lx, y: 0.5 * m.Ly, R: 150, v: 60, debrisTop: 1200, debrisDbz: 55, debris: 1 }]);
-  const ll = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3);
-  const R = sim.addRadar({ id: 'KTST', lat: ll[0], lon: ll[1], h: 0 }, { vcp: 35, seed: 11 });
+  const R = sim.addRadar({ id: 'KTST', lat: IN.site.lat, lon: IN.site.lon, h: 0 }, { vcp: 35, seed: 11 });
   const done = sim.encDue(null, R.phase + R.vcp.dur - 1);
   const vol = await sim.collect(done[0], 'js', 0);
   const rout = path.join(process.argv[2] || '.', 'js_radar.bin');

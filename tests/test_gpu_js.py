@@ -18,12 +18,30 @@ pytestmark = [pytest.mark.gpu,
                                  reason="node or the built addon is missing")]
 
 
+GOLD_WK = os.path.join(ROOT, "tests", "golden", "wk_sounding.txt")
+
+
+def _node_inputs(tmp_path):
+    """The WK base state (40 levels of 500 m) and the radar site, 40 km west / 30 km south of the centre of the
+    128 x 128 x 40 grid at 2 km, in the Python host's projection: what the application hands the Node facade."""
+    from outbreaklab_amd.radar import Geo
+    from outbreaklab_amd.sounding import base_state, parse_sharppy
+
+    B = base_state(parse_sharppy(open(GOLD_WK).read()), 40, 500.0)["B"]
+    geo = Geo(35.9, -97.0, 256000.0, 256000.0, 0.0)
+    lat, lon = geo.fromModel(0.5 * 256000 - 40e3, 0.5 * 256000 - 30e3)
+    sx, sy = geo.toModel(lat, lon)
+    p = tmp_path / "node_in.json"
+    p.write_text(json.dumps({"B": [float(x) for x in B], "site": {"lat": lat, "lon": lon, "sx": sx, "sy": sy}}))
+    return str(p)
+
+
 def test_node_facade_steps_equal_python_host(tmp_path):
     from outbreaklab_amd import RegionalModel
     from outbreaklab_amd.configs import setup_plain_128
 
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path)], capture_output=True,
-                       text=True, timeout=180)
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path), _node_inputs(tmp_path)],
+                       capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     raw = np.fromfile(info["out"], np.float32)
@@ -66,8 +84,8 @@ def test_node_hurricane_run_equals_python_host(tmp_path):
     from outbreaklab_amd.configs import setup_hurricane_128
     from _util import rel_l2
 
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path)], capture_output=True,
-                       text=True, timeout=240)
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_gpu_run.js"), str(tmp_path), _node_inputs(tmp_path)],
+                       capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])["hinfo"]
     m = RegionalModel()

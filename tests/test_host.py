@@ -26,7 +26,7 @@ def test_grid_helpers_closed_forms():
     g = grid.custom_grid({"dx": 500, "w": 512, "h": 512})
     assert g["n"] == [1024, 1024, 80] and g["key"] == "custom:500:1024x1024x80"
     assert grid.custom_grid({"dx": 2000, "w": 5000, "h": 10})["n"][:2] == [2040, 64]
-    # gridBytes at 1024^2 x 80 (the reference's measured 15.8 GB class)
+    # the library's allocation at 1024^2 x 80: 3 wind copies, 6 moisture copies, aux, phT and the padded solve
     assert 14e9 < grid.grid_bytes({"dx": 500, "n": [1024, 1024, 80]}) < 18e9
 
 

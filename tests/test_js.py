@@ -1,5 +1,6 @@
-"""The Node N-API facade (outbreaklab_amd/js): loads under the image's Node, reproduces the
-reference base state and packs the same uniform block as the Python host (CPU only)."""
+"""The Node N-API facade (outbreaklab_amd/js): loads under the image's Node and, for the same model state, packs the
+same uniform block, surrogate targets, radar layout and RADAR_FIELDS block and builds the same vortex table and grid
+figures as the Python host (CPU only)."""
 import json
 import os
 import shutil
@@ -17,29 +18,63 @@ pytestmark = pytest.mark.skipif(shutil.which("node") is None or not os.path.exis
                                 reason="node or the built addon is missing")
 
 
+GOLD_WK = os.path.join(ROOT, "tests", "golden", "wk_sounding.txt")
+
+
+def _inputs():
+    """The golden WK base state (40 levels of 500 m) and a radar site 40 km west / 30 km south of the centre of a
+    128 x 128 x 40 grid at 2 km, in the Python host's projection."""
+    from outbreaklab_amd.radar import Geo
+    from outbreaklab_amd.sounding import base_state, parse_sharppy
+
+    B = base_state(parse_sharppy(open(GOLD_WK).read()), 40, 500.0)["B"]
+    geo = Geo(35.9, -97.0, 256000.0, 256000.0, 300.0)
+    lat, lon = geo.fromModel(0.5 * 256000 - 40e3, 0.5 * 256000 - 30e3)
+    sx, sy = geo.toModel(lat, lon)
+    return {"B": [float(x) for x in B], "site": {"lat": lat, "lon": lon, "sx": sx, "sy": sy}}
+
+
+_OUT = {}
+
+
 def _run():
-    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_facade_check.js")], capture_output=True, text=True,
-                       timeout=120)
-    assert r.returncode == 0, r.stderr
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    if "out" not in _OUT:
+        import tempfile
+
+        with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+            json.dump(_inputs(), f)
+        try:
+            r = subprocess.run(["node", os.path.join(ROOT, "tests", "js_facade_check.js"), f.name], capture_output=True,
+                               text=True, timeout=120)
+        finally:
+            os.unlink(f.name)
+        assert r.returncode == 0, r.stderr
+        _OUT["out"] = json.loads(r.stdout.strip().splitlines()[-1])
+    return _OUT["out"]
 
 
-def test_facade_loads_and_matches_golden():
+def test_facade_loads_and_matches_python_grid_helpers():
+    from outbreaklab_amd import grid
+
     out = _run()
     for name in ["create", "step", "reset", "download", "upload", "sync", "poissonSolve", "fieldPtr", "setBaseState",
                  "sampleMicro", "microSamples", "setSurrogate", "surrogateStats", "mlpCreate", "mlpTrain", "mlpInfer",
                  "mapsRequest", "mapsCollect", "radarCreate", "radarDestroy", "radarSetTornadoes", "radarCut",
-                 "radarReadback", "radarCollect", "setSurface", "setPrepField", "setVortex"]:
+                 "radarReadback", "radarCollect", "setSurface", "setPrepField", "setVortex", "vortexTable", "gridBytes"]:
         assert name in out["exports"]
-    assert out["mismatches"] == 0
-    assert out["levels"] == [56, 80, 100] and out["pads"] == [8, 24, 40]
+    assert out["levels"] == [grid.levels_for(d) for d in (2000, 500, 250, 750, 120, 5000)]
+    assert out["levels"][:3] == [56, 80, 100]
+    assert out["pads"] == [grid.grid_pad(d) for d in (2000, 500, 250, 100)] == [8, 24, 40, 64]
+    assert [c["key"] for c in out["custom"]] == [grid.custom_grid(c)["key"] for c in
+                                                 ({"dx": 500, "w": 512, "h": 512}, {"dx": 2000, "w": 5000, "h": 10})]
+    assert out["gridBytes"] == grid.grid_bytes({"dx": 500, "n": [1024, 1024, 80]})
     assert "multiples of 8" in out["poissonError"]
+    assert "useSounding" in out["soundingError"]
 
 
 def test_facade_uniforms_equal_python_host():
     out = _run()
-    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(os.path.join(ROOT, "tests", "golden",
-                                                                                  "wk_sounding.txt")).read())
+    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(GOLD_WK).read())
     m.time = 42.0
     u = uniforms(m, 6.0)
     assert np.array_equal(np.array(out["uniforms"], np.float32), u)
@@ -56,7 +91,7 @@ def test_facade_surrogate_targets_equal_python_host():
 
 
 def test_facade_radar_host_equals_python_host():
-    """Seeded draws (rng.js:3-27), the volume layout and a cut's RADAR_FIELDS block are the same in both
+    """Seeded draws (rng.js:3-27), the volume layout and timing and a cut's RADAR_FIELDS block are the same in both
     hosts, so a seeded radar scans the same azimuths at the same times."""
     from outbreaklab_amd.radar import Geo, Rng, cut_block, plan_radar
 
@@ -64,17 +99,16 @@ def test_facade_radar_host_equals_python_host():
     for sd, got in zip([7, "KTLX", 123456789], out["rng"]):
         r = Rng(sd)
         assert got == [r.u(), r.u(), r.u()]
-    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(os.path.join(ROOT, "tests", "golden",
-                                                                                  "wk_sounding.txt")).read())
+    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(GOLD_WK).read())
     geo = Geo(35.9, -97.0, m.Lx, m.Ly, 300.0)
+    site = _inputs()["site"]
     rd = out["radar"]
-    lat, lon = geo.fromModel(0.5 * m.Lx - 40e3, 0.5 * m.Ly - 30e3)
-    assert [lat, lon] == rd["ll"] and list(geo.fwd(36.4, -96.1)) == rd["xy"]
-    R = plan_radar(m, geo, {"id": "KTST", "lat": lat, "lon": lon, "h": 350.0}, {"vcp": 212, "seed": 7})
+    R = plan_radar(m, geo, {"id": "KTST", "lat": site["lat"], "lon": site["lon"], "h": 350.0}, {"vcp": 212, "seed": 7})
     R["vol"], R["az0"] = 3, R["rand"]()
     assert R["phase"] == rd["phase"] and R["az0"] == rd["az0"]
     assert (R["bytes"], R["ngRef"], R["ngDop"]) == (rd["bytes"], rd["ngRef"], rd["ngDop"])
-    assert [c["t0"] for c in R["cuts"]] == rd["t0"]
+    assert [c["t0"] for c in R["cuts"]] == rd["t0"] and [c["dur"] for c in R["cuts"]] == rd["dur"]
+    assert [[x["off"] for x in c["mom"]] for c in R["cuts"]] == rd["offs"]
     # the Node block is taken with hurricane settings on the model: the maritime graupel flag (phys2.x, the
     # eleventh vec4) follows them, as radar.js:384
     assert cut_block(m, geo, R, R["cuts"][5], 123.5, 2)[40] == 0.0
@@ -102,8 +136,7 @@ def test_facade_hurricane_host_equals_python_host():
     from outbreaklab_amd.model import vortex_table
 
     out = _run()
-    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(os.path.join(ROOT, "tests", "golden",
-                                                                                  "wk_sounding.txt")).read())
+    m = host_model(128, 128, 40, 2000, scheme="thompson", text=open(GOLD_WK).read())
     m.time = 42.0
     m.params["lat"] = 20.0
     m.hur = dict(HURRICANE_SETTINGS, sfcP=101000.0, centre=[128000.0, 128000.0], envR=100000)
@@ -115,8 +148,8 @@ def test_facade_hurricane_host_equals_python_host():
     tab, info = vortex_table(40, 500.0, 2000.0, 20.0, m.B, {"x": 128000.0, "y": 128000.0, "vmax": 30.0,
                                                             "rmw": 30000.0, "r0": 120000.0, "ztop": 16000.0,
                                                             "moistR": 60000.0, "moistRH": 0.9})
+    # one routine (the library's obl_vortex_table) builds the table for both hosts
     jt = np.array(out["vortex"]["tab"], np.float32).reshape(tab.shape)
-    assert np.abs(jt - tab).max() <= 1e-6 * np.abs(tab).max()
-    for k in ("dr", "nr", "moistR", "moistRH"):
+    assert np.array_equal(jt, tab)
+    for k in ("dr", "nr", "moistR", "moistRH", "pc", "minStab", "warm"):
         assert out["vortex"]["info"][k] == info[k]
-    assert abs(out["vortex"]["info"]["pc"] - info["pc"]) < 1e-9 * abs(info["pc"])
