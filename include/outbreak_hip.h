@@ -43,6 +43,8 @@ extern "C" {
 #define OBL_BOUND_POINTS 32  /* points per boundary polyline, model.js:1373-1380 */
 #define OBL_CI_FLOATS 12     /* one CI tower, model.js:1383-1389 */
 #define OBL_VORTEX_MAX_RADII 2048 /* radii of the vortex table, VX_MAXR model.js:38 */
+#define OBL_RADAR_SCRATCH_FLOATS (720 * 1192) /* obl_radar_cut_into's KDP scratch: radials x Doppler gates */
+#define OBL_MAX_DEVICES 64
 #define OBL_MAX_CI 256       /* MAX_CI, model.js:38 */
 
 typedef struct obl_model obl_model;
@@ -173,6 +175,14 @@ int obl_radar_set_tornadoes(obl_model* h, const float* f, int n);
  * on the current state (V[parity], Q1[0], Q2[0], aux); PHIDP's pass runs when the moment mask holds
  * it. The block must name this model's dimensions and fit the radar's buffers. */
 int obl_radar_cut(obl_radar* r, const float* ru, int parity);
+/* One radar cut, as obl_radar_cut, into a volume buffer of the caller's (out_dev: device memory of out_bytes,
+ * the cut's moments at the block's word offsets), allocating nothing: no hipMalloc, hipFree or allocating
+ * library call, so callers that cannot allocate (code under HIP graph capture, serving loops) can render cuts.
+ * No counterpart in the reference. The KDP scratch between its two kernels is a fixed device array of
+ * OBL_RADAR_SCRATCH_FLOATS floats per GPU (720 radials x 1192 Doppler gates, every VCP's largest cut), held until
+ * the process ends; a cut needing more is rejected. Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two streams, two threads) race. Enqueued on the model's stream; no timing (obl_profile) is taken. */
+int obl_radar_cut_into(obl_model* h, const float* ru, int parity, uint32_t* out_dev, size_t out_bytes);
 /* the volume's copy into a pinned slot (the reference's copyBufferToBuffer into read[slot]):
  * *slot = 0 / 1, or -1 while both slots await collection */
 int obl_radar_readback(obl_radar* r, int* slot);

@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -401,6 +402,7 @@ extern "C" int obl_poisson_profile_read(obl_poisson* p, double* ms5) {
 // ---------------------------------------------------------------- RegionalModel
 struct obl_model {
   int nx = 0, ny = 0, nz = 0, pad = 0, nxp = 0, nyp = 0;
+  int dev = 0;  // the device the model's fields live on (current at obl_create)
   double dx = 0, dz = 0;
   hipStream_t stream = nullptr;
   float4 *V[2] = {nullptr, nullptr}, *Vt = nullptr;
@@ -509,6 +511,7 @@ extern "C" int obl_create(const obl_grid* g, obl_model** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_global("allocate: no ROCm GPU is available");
   obl_model* h = new obl_model();
+  hipGetDevice(&h->dev);
   h->nx = g->nx; h->ny = g->ny; h->nz = g->nz; h->dx = g->dx; h->dz = g->dz; h->pad = g->pad;
   h->nxp = h->nx + 2 * h->pad; h->nyp = h->ny + 2 * h->pad;
   size_t freeb = 0, totalb = 0;
@@ -927,6 +930,28 @@ extern "C" int obl_radar_set_tornadoes(obl_model* h, const float* f, int n) {
   return hip_ok(hipStreamSynchronize(h->stream), &h->err, "setTornadoes");
 }
 
+// The cut's uniform block checked against the model and against a volume of `words` 32-bit words and a KDP
+// scratch of `kdp_floats` floats (the counts that the kernels index with).
+static int check_cut(obl_model* h, const RadarU& R, size_t words, size_t kdp_floats, size_t volume_bytes) {
+  if ((int)R.mdl.x != h->nx || (int)R.mdl.y != h->ny || (int)R.mdl.z != h->nz)
+    return h->fail("radar cut: mdl (%g, %g, %g) is not the model's %d x %d x %d", R.mdl.x, R.mdl.y, R.mdl.z, h->nx, h->ny, h->nz);
+  const long nrad = (long)R.cut.w, ngr = (long)R.gates.x, ngd = (long)R.gates.y;
+  const unsigned mask = (unsigned)R.offB.w;
+  if (nrad < 1 || ngr < 4 || ngr % 4 || ngd < 0 || ngd % 4 || ngd > ngr || R.phys.w < 0 || (int)R.phys.w > h->ntor)
+    return h->fail("radar cut: %ld radials, %ld / %ld gates, %g tornadoes (%d set)", nrad, ngr, ngd, R.phys.w, h->ntor);
+  if ((size_t)(nrad * ngd) > kdp_floats)
+    return h->fail("radar cut: %ld radials x %ld Doppler gates exceed the %zu-float KDP scratch", nrad, ngd, kdp_floats);
+  auto fits = [&](float off, long n) { return off >= 0 && (size_t)off + (size_t)n <= words; };
+  bool ok = fits(R.offA.x, nrad * ngr / 4);
+  if (mask & 2u) ok = ok && fits(R.offA.y, nrad * ngd / 4);
+  if (mask & 4u) ok = ok && fits(R.offA.z, nrad * ngd / 4);
+  if (mask & 8u) ok = ok && fits(R.offA.w, nrad * ngd / 4);
+  if (mask & 16u) ok = ok && fits(R.offB.x, nrad * ngd / 4);
+  if (mask & 32u) ok = ok && fits(R.offB.y, nrad * ngd / 4);
+  if (!ok) return h->fail("radar cut: the cut's moments do not fit the %zu-byte volume", volume_bytes);
+  return 0;
+}
+
 extern "C" int obl_radar_cut(obl_radar* r, const float* ru, int parity) {
   if (!r) return set_global("null radar");
   obl_model* h = r->m;
@@ -935,23 +960,55 @@ extern "C" int obl_radar_cut(obl_radar* r, const float* ru, int parity) {
   RadarU R;
   memcpy(&R, ru, sizeof(R));
   // the block must describe this model and stay inside this radar's buffers
-  if ((int)R.mdl.x != h->nx || (int)R.mdl.y != h->ny || (int)R.mdl.z != h->nz)
-    return h->fail("radar cut: mdl (%g, %g, %g) is not the model's %d x %d x %d", R.mdl.x, R.mdl.y, R.mdl.z, h->nx, h->ny, h->nz);
-  const long nrad = (long)R.cut.w, ngr = (long)R.gates.x, ngd = (long)R.gates.y;
-  const unsigned mask = (unsigned)R.offB.w;
-  if (nrad < 1 || ngr < 4 || ngr % 4 || ngd < 0 || ngd % 4 || ngd > ngr || R.phys.w < 0 || (int)R.phys.w > h->ntor)
-    return h->fail("radar cut: %ld radials, %ld / %ld gates, %g tornadoes (%d set)", nrad, ngr, ngd, R.phys.w, h->ntor);
-  const size_t words = r->bytes / 4;
-  auto fits = [&](float off, long n) { return off >= 0 && (size_t)off + (size_t)n <= words; };
-  bool ok = fits(R.offA.x, nrad * ngr / 4) && (size_t)(nrad * ngd) <= r->kdp_floats;
-  if (mask & 2u) ok = ok && fits(R.offA.y, nrad * ngd / 4);
-  if (mask & 4u) ok = ok && fits(R.offA.z, nrad * ngd / 4);
-  if (mask & 8u) ok = ok && fits(R.offA.w, nrad * ngd / 4);
-  if (mask & 16u) ok = ok && fits(R.offB.x, nrad * ngd / 4);
-  if (mask & 32u) ok = ok && fits(R.offB.y, nrad * ngd / 4);
-  if (!ok) return h->fail("radar cut: the cut's moments do not fit the %zu-byte volume", r->bytes);
+  if (check_cut(h, R, r->bytes / 4, r->kdp_floats, r->bytes)) return -1;
   RadarArgs a{h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->aux, h->tor, r->out, r->kdp};
   TIMED(h->timer, h->stream, "r.cut", launch_radar_cut(h->stream, R, a));
+  return hip_ok(hipGetLastError(), &h->err, "radar cut launch");
+}
+
+// The allocation-free radar cut: its KDP scratch (nrad x Doppler gates floats between the cut kernel and the PHIDP
+// kernel, fully written by the first before the second reads it) is this file-scope device array, of which each
+// GPU holds its own copy until the process ends. Host code never passes the array's name to a kernel: its device
+// address is looked up once per device (hipGetSymbolAddress, with that device current) and kept here.
+static __device__ __align__(256) float scratch[OBL_RADAR_SCRATCH_FLOATS];
+static float* g_scratch_dev[OBL_MAX_DEVICES] = {nullptr};
+static std::mutex g_scratch_mu;
+
+static float* radar_scratch(obl_model* h) {
+  if (h->dev < 0 || h->dev >= OBL_MAX_DEVICES) {
+    h->fail("radar cut: device %d is beyond the %d the library serves", h->dev, OBL_MAX_DEVICES);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[h->dev]) {
+    int cur = 0;
+    hipGetDevice(&cur);
+    if (cur != h->dev && hip_ok(hipSetDevice(h->dev), &h->err, "radar scratch: hipSetDevice")) return nullptr;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != h->dev) hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      h->fail("radar scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[h->dev] = (float*)p;
+  }
+  return g_scratch_dev[h->dev];
+}
+
+extern "C" int obl_radar_cut_into(obl_model* h, const float* ru, int parity, uint32_t* out_dev, size_t out_bytes) {
+  if (!h) return set_global("null handle");
+  if (!ru || !out_dev) return h->fail("radar cut: null uniform block or volume");
+  if (parity != 0 && parity != 1) return h->fail("parity must be 0 or 1");
+  if (out_bytes % 4) return h->fail("radar cut: the volume must be whole 32-bit words");
+  RadarU R;
+  memcpy(&R, ru, sizeof(R));
+  if (check_cut(h, R, out_bytes / 4, OBL_RADAR_SCRATCH_FLOATS, out_bytes)) return -1;
+  float* kdp = radar_scratch(h);
+  if (!kdp) return -1;
+  if ((int)R.phys.w > 0 && !h->tor) return h->fail("radar cut: tornadoes without a tornado table (setTornadoes)");
+  RadarArgs a{h->BS, h->V[parity], h->Q1[0], h->Q2[0], h->aux, h->tor, out_dev, kdp};
+  launch_radar_cut(h->stream, R, a);
   return hip_ok(hipGetLastError(), &h->err, "radar cut launch");
 }
 
