@@ -462,89 +462,6 @@ __device__ __forceinline__ bool near_traces_q(const Dims& d, float3 v, const Fal
          fabsf(v.z - fmax) * kz < 0.999f;
 }
 
-template <class AQ>
-__device__ __forceinline__ void advq1_samples(const Dims& d, bool thm, const AQ& Q1, const AQ& Q2, float3 v, const Fall& f,
-                                              int x, int y, int z, float4& o1, float4& o2) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
-  float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
-  float a0 = samp1(Q1, tx, ty, tz, 0), a1 = samp1(Q1, tx, ty, tz, 1), a2 = samp1(Q1, tx, ty, tz, 2);
-  float r = sampQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3);
-  float bw = samp1(Q2, tx, ty, tz, 3);
-  float s = sampQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0);
-  float g = sampQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1);
-  float nr = thm ? sampQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2) : samp1(Q2, tx, ty, tz, 2);
-  o1 = make_float4(a0, a1, a2, r);
-  o2 = make_float4(s, g, nr, bw);
-}
-__device__ __forceinline__ void advq1_point(const MU& u, const Dims& d, const float4* BS, float3 v, const Ring& Q1,
-                                            const Ring& Q2, int x, int y, int z, float4& o1, float4& o2) {
-  bool thm = u.v[U_MICRO2].z > 0.5f;
-  RingW w1 = Q1.w(), w2 = Q2.w();
-  float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
-  Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
-  if (wave_all(near_traces_q(d, v, f))) {
-    advq1_samples(d, thm, w1, w2, v, f, x, y, z, o1, o2);
-  } else {
-    // a far trace somewhere in the wave: every sample from global memory (same texels)
-    GAcc g1{Q1.g, Q1.W, Q1.H, Q1.D, nullptr}, g2{Q2.g, Q2.W, Q2.H, Q2.D, nullptr};
-    advq1_samples(d, thm, g1, g2, v, f, x, y, z, o1, o2);
-  }
-}
-
-template <class AQ>
-__device__ __forceinline__ void advq2_rest(const MU& u, const Dims& d, const AQ& Q1, const AQ& Q2, const AQ& Q1t,
-                                           const AQ& Q2t, float3 v, const Fall& f, float4 q1, float4 q2, float4 h1,
-                                           float4 h2, int x, int y, int z, float4& o1, float4& o2) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
-  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
-  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f);
-  float tx = f0.x - 0.5f, ty = f0.y - 0.5f, tz = f0.z - 0.5f;
-  float b1x = samp1(Q1t, tx, ty, tz, 0), b1y = samp1(Q1t, tx, ty, tz, 1), b1z = samp1(Q1t, tx, ty, tz, 2);
-  float br = sampQ1(Q1t, traceQ(d, k, p, v, f.r, -1.0f), 3);
-  float b2w = samp1(Q2t, tx, ty, tz, 3);
-  float bs = sampQ1(Q2t, traceQ(d, k, p, v, f.s, -1.0f), 0);
-  float bg = sampQ1(Q2t, traceQ(d, k, p, v, f.g, -1.0f), 1);
-  float bn = sampQ1(Q2t, traceQ(d, k, p, v, f.n, -1.0f), 2);
-  float4 r1 = make_float4(h1.x + sm * (q1.x - b1x), h1.y + sm * (q1.y - b1y), h1.z + sm * (q1.z - b1z),
-                          h1.w + sm * (q1.w - br));
-  float4 r2 = make_float4(h2.x + sm * (q2.x - bs), h2.y + sm * (q2.y - bg), h2.z + sm * (q2.z - bn),
-                          h2.w + sm * (q2.w - b2w));
-  float3 d0 = traceQ(d, k, p, v, 0.0f, 1.0f);
-  float2 m;
-  m = mmQ1(Q1, d0, 0); r1.x = vclamp_(r1.x, m);
-  m = mmQ1(Q1, d0, 1); r1.y = vclamp_(r1.y, m);
-  m = mmQ1(Q1, d0, 2); r1.z = vclamp_(r1.z, m);
-  m = mmQ1(Q1, traceQ(d, k, p, v, f.r, 1.0f), 3); r1.w = vclamp_(r1.w, m);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.s, 1.0f), 0); r2.x = vclamp_(r2.x, m);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.g, 1.0f), 1); r2.y = vclamp_(r2.y, m);
-  m = mmQ1(Q2, traceQ(d, k, p, v, f.n, 1.0f), 2); r2.z = vclamp_(r2.z, m);
-  m = mmQ1(Q2, d0, 3); r2.w = vclamp_(r2.w, m);
-  o1 = make_float4(vmax_(r1.x, 0.0f), vmax_(r1.y, 0.0f), vmax_(r1.z, 0.0f), vmax_(r1.w, 0.0f));
-  o2 = make_float4(vmax_(r2.x, 0.0f), vmax_(r2.y, 0.0f), vmax_(r2.z, 0.0f), vmax_(r2.w, 0.0f));
-}
-__device__ __forceinline__ void advq2_point(const MU& u, const Dims& d, const float4* BS, float3 v, const Ring& Q1,
-                                            const Ring& Q2, const Ring& Q1t, const Ring& Q2t, int x, int y, int z,
-                                            float4& o1, float4& o2) {
-  bool thm = u.v[U_MICRO2].z > 0.5f;
-  RingW w1 = Q1.w(), w2 = Q2.w(), w1t = Q1t.w(), w2t = Q2t.w();
-  float4 q1 = w1.ld4(x, y, z), q2 = w2.ld4(x, y, z);
-  float4 h1 = w1t.ld4(x, y, z), h2 = w2t.ld4(x, y, z);
-  Fall f = fall_speeds(thm, q1, q2, bsA(BS, z, d.nz).z, u.v[U_STEP].z);
-  if (wave_all(near_traces_q(d, v, f))) {
-    advq2_rest(u, d, w1, w2, w1t, w2t, v, f, q1, q2, h1, h2, x, y, z, o1, o2);
-  } else {
-    // a far trace somewhere in the wave: every sample from global memory (same texels)
-    GAcc g1{Q1.g, Q1.W, Q1.H, Q1.D, nullptr}, g2{Q2.g, Q2.W, Q2.H, Q2.D, nullptr};
-    GAcc g1t{Q1t.g, Q1t.W, Q1t.H, Q1t.D, nullptr}, g2t{Q2t.g, Q2t.W, Q2t.H, Q2t.D, nullptr};
-    advq2_rest(u, d, g1, g2, g1t, g2t, v, f, q1, q2, h1, h2, x, y, z, o1, o2);
-  }
-}
-
 // ====================== forces + microphysics (model.js:579-907) ======================
 template <class A>
 __device__ __forceinline__ float lapH(const A& V, int x, int y, int z, int ch, float me) {
@@ -1018,74 +935,297 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advv2_t(MU u, const float4* __
   }
 }
 
-__global__ void __launch_bounds__(TILE_THREADS) k_advq1_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                                          float4* __restrict__ Q1t, float4* __restrict__ Q2t) {
-  __shared__ float lds[2 * RING_FLOATS];
-  TILE_SETUP
-  Ring r1{lds, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
-  Ring r2{lds + RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
-  GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
-  for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); }
-  __syncthreads();
-  const bool act = x < d.nx && y < d.ny;
-  // the cell velocity's texels are loaded one level ahead (see k_nut_t)
-  CellVelTexels vtn = cellVelTexels(gv, x, y, 0);
-  for (int z = 0; z < d.nz; z++) {
-    Ring::Pre p1, p2;
-    const CellVelTexels vt = vtn;
-    vtn = cellVelTexels(gv, x, y, z + 1);
-    r1.fetch(z + 2, tid, p1); r2.fetch(z + 2, tid, p2);
-    r1.center(z); r2.center(z);
-    if (act) {
-      float4 o1, o2;
-      advq1_point(u, d, BS, cellVelOf(vt), r1, r2, x, y, z, o1, o2);
-      const uint32_t i = cidx(d, x, y, z);
-      stx(Q1t, i, o1);
-      stx(Q2t, i, o2);
-    }
-    __syncthreads();
-    r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2);
-    __syncthreads();
+// ---------------------------------------------------------------------------------------------------
+// Moisture advection in one pass: predictor (WGSL_M_ADVQ1, model.js:502-530) and corrector with its limiter
+// (WGSL_M_ADVQ2, model.js:532-576) in one z-marching kernel, so the predicted fields never go through HBM.
+//
+// A block owns a 32 x 8 column tile. At level z it first predicts plane z + 1 for the tile and a one-point halo
+// (the points whose predicted values the corrector's forward traces can reach) into an LDS ring of three
+// predicted planes, from LDS rings of Q1 and Q2 with a two-point halo; then it corrects level z from the ring.
+// The limiter's min / max over the 8 texels around each departure point are the predictor's own trilinear
+// texels (same departure point, same fall speed), so they are taken while predicting and carried in registers
+// to the next level, with the point's own texels, fall speeds and cell velocity.
+//
+// Waves whose traces all stay within one point (|Courant| < 1 on each axis for every species) read LDS only.
+// Any other wave takes the same arithmetic on global memory: the predictor samples Q1 / Q2 there, and the
+// corrector recomputes the predicted value of each texel it samples from Q1 / Q2 / V in global memory.
+// ---------------------------------------------------------------------------------------------------
+constexpr int QW = TX + 4, QH = TY + 4, QS = QW * QH;  // two-point halo
+constexpr int QRING_FLOATS = 3 * 4 * QS;
+constexpr int NHALO = TPS - TILE_THREADS;  // the one-point halo's slots (84)
+
+// three planes of a (TX+4) x (TY+4) tile, channel-major, clamped texels (as Ring, two-point halo)
+struct Ring2 {
+  float* s;
+  const float4* g;
+  int W, H, D, x0, y0;
+  __device__ __forceinline__ static int slot(int k) { return (k + 3 * 4096) % 3; }
+  __device__ __forceinline__ uint32_t tofs(int t) const {
+    return (uint32_t)clampi(x0 - 2 + t % QW, 0, W - 1) + __umul24((uint32_t)W, (uint32_t)clampi(y0 - 2 + t / QW, 0, H - 1));
   }
+  __device__ __forceinline__ uint32_t pofs(int k) const { return __umul24((uint32_t)clampi(k, 0, D - 1), (uint32_t)(W * H)); }
+  struct Pre {
+    float4 v[(QS + TILE_THREADS - 1) / TILE_THREADS];
+  };
+  __device__ __forceinline__ void fetch(int k, int tid, Pre& r) const {
+    const uint32_t po = pofs(k);
+#pragma unroll
+    for (int q = 0; q < (QS + TILE_THREADS - 1) / TILE_THREADS; q++) r.v[q] = ldx(g, tofs(min(tid + q * TILE_THREADS, QS - 1)) + po);
+  }
+  __device__ __forceinline__ void store(int k, int tid, const Pre& r) {
+    float* p0 = s + slot(k) * 4 * QS;
+#pragma unroll
+    for (int q = 0; q < (QS + TILE_THREADS - 1) / TILE_THREADS; q++) {
+      int t = tid + q * TILE_THREADS;
+      if (t < QS) {
+        p0[t] = r.v[q].x;
+        p0[QS + t] = r.v[q].y;
+        p0[2 * QS + t] = r.v[q].z;
+        p0[3 * QS + t] = r.v[q].w;
+      }
+    }
+  }
+  __device__ __forceinline__ void load(int k, int tid) {
+    Pre r;
+    fetch(k, tid, r);
+    store(k, tid, r);
+  }
+};
+// window view of a Ring2 centred at plane kc (reads within one point of the halo-1 region and of plane kc)
+struct Win2 {
+  const lds_float* s;
+  int x0, y0, kc, om, oc, op;
+  __device__ __forceinline__ int base(int i0, int j0, int k0, int ch, int& dz) const {
+    bool lo = k0 < kc;
+    dz = lo ? oc - om : op - oc;
+    return (lo ? om : oc) + ch * QS + __mul24(j0 - y0 + 2, QW) + (i0 - x0 + 2);
+  }
+  __device__ __forceinline__ float4 ld4(int i, int j) const {
+    int b = oc + __mul24(j - y0 + 2, QW) + (i - x0 + 2);
+    return make_float4(s[b], s[b + QS], s[b + 2 * QS], s[b + 3 * QS]);
+  }
+};
+__device__ __forceinline__ Win2 win2(const Ring2& r, int kc) {
+  return Win2{(const lds_float*)r.s, r.x0, r.y0, kc, Ring2::slot(kc - 1) * 4 * QS, Ring2::slot(kc) * 4 * QS,
+              Ring2::slot(kc + 1) * 4 * QS};
+}
+// trilinear sample of one channel at texel coordinates (tx, ty, tz) and the min / max of its 8 texels
+__device__ __forceinline__ float samp_mm(const Win2& a, float tx, float ty, float tz, int ch, float2& mm) {
+  float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  int dz;
+  int b = a.base(floori_(tx), floori_(ty), floori_(tz), ch, dz);
+  float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + QW], c110 = a.s[b + QW + 1];
+  float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + QW], c111 = a.s[b + dz + QW + 1];
+  mm = make_float2(vmin3_(vmin3_(c000, c100, c010), vmin3_(c110, c001, c101), vmin_(c011, c111)),
+                   vmax3_(vmax3_(c000, c100, c010), vmax3_(c110, c001, c101), vmax_(c011, c111)));
+  float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
+  float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+// the same from global memory (clamped texels)
+__device__ __forceinline__ float samp_mm(const GAcc& a, float tx, float ty, float tz, int ch, float2& mm) {
+  int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
+  mm = minmax1_global(a.g, a.W, a.H, a.D, i0, j0, k0, ch);
+  return samp1_global(a.g, a.W, a.H, a.D, i0, j0, k0, __builtin_amdgcn_fractf(tx), __builtin_amdgcn_fractf(ty),
+                      __builtin_amdgcn_fractf(tz), ch);
 }
 
-__global__ void __launch_bounds__(TILE_THREADS) k_advq2_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                                          const float4* __restrict__ Q1t, const float4* __restrict__ Q2t,
-                                                          float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
-  __shared__ float lds[4 * RING_FLOATS];
-  TILE_SETUP
-  Ring r1{lds, Q1, d.nx, d.ny, d.nz, x0, y0, 0};
-  Ring r2{lds + RING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0, 0};
-  Ring r3{lds + 2 * RING_FLOATS, Q1t, d.nx, d.ny, d.nz, x0, y0, 0};
-  Ring r4{lds + 3 * RING_FLOATS, Q2t, d.nx, d.ny, d.nz, x0, y0, 0};
+// The predictor of one cell (model.js:502-530): Q1t = (qv, qc, qi at the departure point, qr at the rain's),
+// Q2t = (qs at the snow's, qg at the graupel's, Nr at the number-weighted rain's (Thompson) or the departure
+// point, spare at the departure point); mm: the min / max of each channel's 8 texels (the corrector's limiter)
+struct Pred {
+  float4 h1, h2;
+  float2 mm[8];
+};
+template <class A>
+__device__ __forceinline__ void predict(const Dims& d, bool thm, const A& Q1, const A& Q2, float3 v, const Fall& f,
+                                        int x, int y, int z, Pred& o) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f), pr = traceQ(d, k, p, v, f.r, 1.0f);
+  float3 ps = traceQ(d, k, p, v, f.s, 1.0f), pg = traceQ(d, k, p, v, f.g, 1.0f), pn = traceQ(d, k, p, v, f.n, 1.0f);
+  float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
+  o.h1.x = samp_mm(Q1, tx, ty, tz, 0, o.mm[0]);
+  o.h1.y = samp_mm(Q1, tx, ty, tz, 1, o.mm[1]);
+  o.h1.z = samp_mm(Q1, tx, ty, tz, 2, o.mm[2]);
+  o.h1.w = samp_mm(Q1, pr.x - 0.5f, pr.y - 0.5f, pr.z - 0.5f, 3, o.mm[3]);
+  o.h2.x = samp_mm(Q2, ps.x - 0.5f, ps.y - 0.5f, ps.z - 0.5f, 0, o.mm[4]);
+  o.h2.y = samp_mm(Q2, pg.x - 0.5f, pg.y - 0.5f, pg.z - 0.5f, 1, o.mm[5]);
+  // f.n is 0 in the Lin scheme: Nr then comes from the departure point, as model.js:527
+  o.h2.z = samp_mm(Q2, pn.x - 0.5f, pn.y - 0.5f, pn.z - 0.5f, 2, o.mm[6]);
+  o.h2.w = samp_mm(Q2, tx, ty, tz, 3, o.mm[7]);
+  (void)thm;
+}
+
+// the predicted value of channel ch (0-3 of Q1t, 4-7 of Q2t) at texel (i, j, k) (clamped), from global memory
+__device__ __noinline__ static float qhat_at(const MU& u, const Dims& d, const float4* BS, const float4* V,
+                                             const float4* Q1, const float4* Q2, int i, int j, int k, int ch) {
+  const bool thm = u.v[U_MICRO2].z > 0.5f;
+  i = clampi(i, 0, d.nx - 1); j = clampi(j, 0, d.ny - 1); k = clampi(k, 0, d.nz - 1);
   GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
-  for (int k = -1; k <= 1; k++) { r1.load(k, tid); r2.load(k, tid); r3.load(k, tid); r4.load(k, tid); }
-  __syncthreads();
+  GAcc g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
+  float3 v = cellVel(gv, i, j, k);
+  float4 q1 = ldx(Q1, cidx(d, i, j, k)), q2 = ldx(Q2, cidx(d, i, j, k));
+  Fall f = fall_speeds(thm, q1, q2, bsA(BS, k, d.nz).z, u.v[U_STEP].z);
+  const float fall = ch == 3 ? f.r : ch == 4 ? f.s : ch == 5 ? f.g : ch == 6 ? f.n : 0.0f;
+  float3 kk = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 p = traceQ(d, kk, make_float3((float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f), v, fall, 1.0f);
+  return ch < 4 ? samp1(g1, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch) : samp1(g2, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch - 4);
+}
+// trilinear sample of the predicted channel ch at texel coordinates (tx, ty, tz), every texel recomputed
+__device__ __noinline__ static float qhat_sample_global(const MU& u, const Dims& d, const float4* BS, const float4* V,
+                                                        const float4* Q1, const float4* Q2, float tx, float ty, float tz,
+                                                        int ch) {
+  int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
+  float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  float c[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) c[q] = qhat_at(u, d, BS, V, Q1, Q2, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
+  float c00 = lerpf(c[0], c[1], ax), c10 = lerpf(c[2], c[3], ax), c01 = lerpf(c[4], c[5], ax), c11 = lerpf(c[6], c[7], ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+
+// what the corrector of a cell needs from its predictor, carried to the next level in registers
+struct Carry {
+  float4 q1, q2;  // the cell's own texels
+  float3 v;       // its cell velocity
+  Fall f;
+  bool near;
+  float2 mm[8];
+};
+
+// the corrector of one cell (model.js:532-576) from the predicted ring (halo 1) or, for far traces, recomputed
+template <bool NEAR>
+__device__ __forceinline__ void correct(const MU& u, const Dims& d, const RingW& w1t, const RingW& w2t, const Carry& c,
+                                        const float4* BS, const float4* V, const float4* Q1, const float4* Q2, int x,
+                                        int y, int z, float4& o1, float4& o2) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
+  const float3 v = c.v;
+  float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
+  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f), fr = traceQ(d, k, p, v, c.f.r, -1.0f);
+  float3 fs = traceQ(d, k, p, v, c.f.s, -1.0f), fg = traceQ(d, k, p, v, c.f.g, -1.0f), fn = traceQ(d, k, p, v, c.f.n, -1.0f);
+  float b[8];
+  const float3 at[8] = {f0, f0, f0, fr, fs, fg, fn, f0};
+#pragma unroll
+  for (int ch = 0; ch < 8; ch++) {
+    float tx = at[ch].x - 0.5f, ty = at[ch].y - 0.5f, tz = at[ch].z - 0.5f;
+    if (NEAR)
+      b[ch] = ch < 4 ? samp1(w1t, tx, ty, tz, ch) : samp1(w2t, tx, ty, tz, ch - 4);
+    else
+      b[ch] = qhat_sample_global(u, d, BS, V, Q1, Q2, tx, ty, tz, ch);
+  }
+  const float4 h1 = w1t.ld4(x, y, z), h2 = w2t.ld4(x, y, z);
+  const float4 q1 = c.q1, q2 = c.q2;
+  float r[8] = {h1.x + sm * (q1.x - b[0]), h1.y + sm * (q1.y - b[1]), h1.z + sm * (q1.z - b[2]), h1.w + sm * (q1.w - b[3]),
+                h2.x + sm * (q2.x - b[4]), h2.y + sm * (q2.y - b[5]), h2.z + sm * (q2.z - b[6]), h2.w + sm * (q2.w - b[7])};
+#pragma unroll
+  for (int ch = 0; ch < 8; ch++) r[ch] = vmax_(vclamp_(r[ch], c.mm[ch]), 0.0f);
+  o1 = make_float4(r[0], r[1], r[2], r[3]);
+  o2 = make_float4(r[4], r[5], r[6], r[7]);
+}
+
+// the slot of the one-point halo ring that thread tid predicts in round 1 (its own cell) or round 2 (the halo)
+__device__ __forceinline__ void halo_slot(int t, int& si, int& sj) {
+  // t < 34: the row below the tile, t < 68: the row above, then the left and right columns
+  if (t < TPW) { si = t; sj = 0; }
+  else if (t < 2 * TPW) { si = t - TPW; sj = TPH - 1; }
+  else if (t < 2 * TPW + TY) { si = 0; sj = 1 + t - 2 * TPW; }
+  else { si = TPW - 1; sj = 1 + t - 2 * TPW - TY; }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                         const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                         float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
+  __shared__ float lds[2 * QRING_FLOATS + 2 * RING_FLOATS];
+  TILE_SETUP
+  const bool thm = u.v[U_MICRO2].z > 0.5f;
+  const float rsfc = u.v[U_STEP].z;
+  Ring2 q1r{lds, Q1, d.nx, d.ny, d.nz, x0, y0};
+  Ring2 q2r{lds + QRING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0};
+  // the predicted planes (one-point halo): written by the predictor, read through RingW views
+  Ring h1r{lds + 2 * QRING_FLOATS, nullptr, d.nx, d.ny, d.nz, x0, y0, 0};
+  Ring h2r{lds + 2 * QRING_FLOATS + RING_FLOATS, nullptr, d.nx, d.ny, d.nz, x0, y0, 0};
+  GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  GAcc g1{Q1, d.nx, d.ny, d.nz, nullptr}, g2{Q2, d.nx, d.ny, d.nz, nullptr};
   const bool act = x < d.nx && y < d.ny;
-  // the cell velocity's texels are loaded one level ahead (see k_nut_t)
-  CellVelTexels vtn = cellVelTexels(gv, x, y, 0);
-  // planes are fetched two levels ahead: level z + 2 is in registers when level z starts, level z + 3 is
-  // fetched during it (the VGPR budget of this kernel's LDS-limited occupancy has room for both)
-  Ring::Pre p1, p2, p3, p4;
-  r1.fetch(2, tid, p1); r2.fetch(2, tid, p2); r3.fetch(2, tid, p3); r4.fetch(2, tid, p4);
+  // round 1: this thread's own cell (clamped into the field); round 2 (threads below NHALO): a halo slot
+  const int cx1 = clampi(x, 0, d.nx - 1), cy1 = clampi(y, 0, d.ny - 1);
+  int hsi = 0, hsj = 0;
+  halo_slot(min(tid, NHALO - 1), hsi, hsj);
+  const int cx2 = clampi(x0 - 1 + hsi, 0, d.nx - 1), cy2 = clampi(y0 - 1 + hsj, 0, d.ny - 1);
+  const int own = __mul24(tid / TX + 1, TPW) + tid % TX + 1, hal = __mul24(hsj, TPW) + hsi;
+
+  // predicts plane kp (0..nz-1) into the predicted ring, from the Q rings centred at kp; returns this thread's
+  // own cell's carry
+  auto predict_plane = [&](int kp, Carry& cr) {
+    const Win2 w1 = win2(q1r, kp), w2 = win2(q2r, kp);
+    const float rho = bsA(BS, kp, d.nz).z;
+    float* H1 = h1r.s + Ring::slot(kp) * 4 * TPS;
+    float* H2 = h2r.s + Ring::slot(kp) * 4 * TPS;
+    Pred pr;
+    {
+      cr.q1 = w1.ld4(cx1, cy1);
+      cr.q2 = w2.ld4(cx1, cy1);
+      cr.v = cellVel(gv, cx1, cy1, kp);
+      cr.f = fall_speeds(thm, cr.q1, cr.q2, rho, rsfc);
+      cr.near = near_traces_q(d, cr.v, cr.f);
+      if (wave_all(cr.near)) predict(d, thm, w1, w2, cr.v, cr.f, cx1, cy1, kp, pr);
+      else predict(d, thm, g1, g2, cr.v, cr.f, cx1, cy1, kp, pr);
+      H1[own] = pr.h1.x; H1[TPS + own] = pr.h1.y; H1[2 * TPS + own] = pr.h1.z; H1[3 * TPS + own] = pr.h1.w;
+      H2[own] = pr.h2.x; H2[TPS + own] = pr.h2.y; H2[2 * TPS + own] = pr.h2.z; H2[3 * TPS + own] = pr.h2.w;
+#pragma unroll
+      for (int q = 0; q < 8; q++) cr.mm[q] = pr.mm[q];
+    }
+    if (tid < NHALO) {
+      const float4 q1 = w1.ld4(cx2, cy2), q2 = w2.ld4(cx2, cy2);
+      const float3 v = cellVel(gv, cx2, cy2, kp);
+      const Fall f = fall_speeds(thm, q1, q2, rho, rsfc);
+      if (wave_all(near_traces_q(d, v, f))) predict(d, thm, w1, w2, v, f, cx2, cy2, kp, pr);
+      else predict(d, thm, g1, g2, v, f, cx2, cy2, kp, pr);
+      H1[hal] = pr.h1.x; H1[TPS + hal] = pr.h1.y; H1[2 * TPS + hal] = pr.h1.z; H1[3 * TPS + hal] = pr.h1.w;
+      H2[hal] = pr.h2.x; H2[TPS + hal] = pr.h2.y; H2[2 * TPS + hal] = pr.h2.z; H2[3 * TPS + hal] = pr.h2.w;
+    }
+  };
+
+  // prologue: Q planes -1, 0, 1 (clamped), then plane 0's prediction
+  q1r.load(-1, tid); q1r.load(0, tid); q1r.load(1, tid);
+  q2r.load(-1, tid); q2r.load(0, tid); q2r.load(1, tid);
+  __syncthreads();
+  Carry cur;
+  predict_plane(0, cur);
+  __syncthreads();
+  {
+    // plane 2 replaces plane -1 (slot(2) == slot(-1)): the prediction of plane 1 needs planes 0..2
+    Ring2::Pre a, b;
+    q1r.fetch(2, tid, a); q2r.fetch(2, tid, b);
+    q1r.store(2, tid, a); q2r.store(2, tid, b);
+  }
+  __syncthreads();
   for (int z = 0; z < d.nz; z++) {
-    Ring::Pre n1, n2, n3, n4;
-    const CellVelTexels vt = vtn;
-    vtn = cellVelTexels(gv, x, y, z + 1);
-    r1.fetch(z + 3, tid, n1); r2.fetch(z + 3, tid, n2); r3.fetch(z + 3, tid, n3); r4.fetch(z + 3, tid, n4);
-    r1.center(z); r2.center(z); r3.center(z); r4.center(z);
+    Ring2::Pre p1, p2;
+    const bool more = z + 1 < d.nz;
+    // plane z + 3 for the prediction of plane z + 2 at the next level
+    q1r.fetch(z + 3, tid, p1); q2r.fetch(z + 3, tid, p2);
+    Carry nxt;
+    if (more) predict_plane(z + 1, nxt);
+    __syncthreads();
+    q1r.store(z + 3, tid, p1); q2r.store(z + 3, tid, p2);
+    // the predicted ring centred at z: below the ground and above the lid the clamped planes 0 and nz - 1
+    h1r.center(z); h2r.center(z);
+    if (z == 0) { h1r.om = h1r.oc; h2r.om = h2r.oc; }
+    if (!more) { h1r.op = h1r.oc; h2r.op = h2r.oc; }
     if (act) {
       float4 o1, o2;
-      advq2_point(u, d, BS, cellVelOf(vt), r1, r2, r3, r4, x, y, z, o1, o2);
+      if (wave_all(cur.near)) correct<true>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, o1, o2);
+      else correct<false>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, o1, o2);
       const uint32_t i = cidx(d, x, y, z);
       stx(Q1o, i, o1);
       stx(Q2o, i, o2);
     }
-    __syncthreads();
-    r1.store(z + 2, tid, p1); r2.store(z + 2, tid, p2); r3.store(z + 2, tid, p3); r4.store(z + 2, tid, p4);
-    p1 = n1; p2 = n2; p3 = n3; p4 = n4;
+    if (more) cur = nxt;
     __syncthreads();
   }
 }
@@ -1700,13 +1840,9 @@ void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, fl
 void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout) {
   hipLaunchKernelGGL(k_advv2_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt, Vout);
 }
-void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                  const float4* Q2, float4* Q1t, float4* Q2t) {
-  hipLaunchKernelGGL(k_advq1_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t);
-}
-void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                  const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o) {
-  hipLaunchKernelGGL(k_advq2_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1t, Q2t, Q1o, Q2o);
+void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                 const float4* Q2, float4* Q1o, float4* Q2o) {
+  hipLaunchKernelGGL(k_advq_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1o, Q2o);
 }
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,

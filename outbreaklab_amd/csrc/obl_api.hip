@@ -1092,8 +1092,7 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], nullptr, h->aux));
   TIMED(T, s, "m.advV1", launch_advv1(s, u, d, Vc, h->Vt));
   TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
-  TIMED(T, s, "m.advQ1", launch_advq1(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t));
-  TIMED(T, s, "m.advQ2", launch_advq2(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1t, h->Q2t, h->Q1[1], h->Q2[1]));
+  TIMED(T, s, "m.advQ", launch_advq(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1[1], h->Q2[1]));
   if (h->net) {
     MlpView mv;
     mlp_view(h->net, &mv);

@@ -20,10 +20,9 @@ void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, con
                 const float4* sfc, float2* aux);
 void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt);
 void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout);
-void launch_advq1(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                  const float4* Q2, float4* Q1t, float4* Q2t);
-void launch_advq2(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                  const float4* Q2, const float4* Q1t, const float4* Q2t, float4* Q1o, float4* Q2o);
+// the predictor and corrector of the moisture advection in one pass (outbreak mode): Q1, Q2 -> Q1o, Q2o
+void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                 const float4* Q2, float4* Q1o, float4* Q2o);
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o);
