@@ -29,8 +29,10 @@ RTX5090_SIM_S_PER_WALL_S_DT2 = 2.0 / 0.09375
 
 # compulsory bytes per model cell of each stencil kernel: each field read once + written once
 # (float4 texel 16 B, aux 8 B, rhs 4 B); halo re-reads excluded
-# (a fused kernel counts only its own inputs and outputs: m.advQ reads V, Q1, Q2 and writes Q1, Q2)
-CELL_BYTES = {"m.nut": 40, "m.advV1": 32, "m.advV2": 48, "m.advQ1": 80, "m.advQ2": 112, "m.advQ": 80, "m.phys": 112,
+# (a fused kernel counts only its own inputs and outputs: m.advV reads V, Q1 and writes V, aux; m.advQ reads V, Q1,
+# Q2 and writes Q1, Q2)
+CELL_BYTES = {"m.nut": 40, "m.advV1": 32, "m.advV2": 48, "m.advV": 56, "m.advQ1": 80, "m.advQ2": 112, "m.advQ": 80,
+              "m.phys": 112,
               "m.div": 20, "m.proj": 36}
 
 

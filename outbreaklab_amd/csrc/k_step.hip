@@ -354,63 +354,9 @@ __device__ __forceinline__ bool near_traces(const Dims& d, const Loc& L) {
 }
 
 template <class A>
-__device__ __forceinline__ float4 advv1_samples(const Dims& d, const A& V, const Loc& L, int x, int y, int z) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float fx = (float)x, fy = (float)y, fz = (float)z;
-  float4 o;
-  o.x = sampU(V, traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
-  o.y = sampV(V, traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
-  o.z = sampW(V, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f));
-  o.w = sampC(V, traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
-  return o;
-}
-// Neighbour reads (one point away) always hit the ring's window; the samples take the
-// window-guaranteed path when the whole wave's traces are near (wave-uniform branch), the
-// checked path otherwise. Both read the same texels with the same arithmetic.
-__device__ __forceinline__ float4 advv1_point(const Dims& d, const Ring& rv, int x, int y, int z) {
-  RingW w = rv.w();
-  Loc L = localVel(w, x, y, z);
-  if (wave_all(near_traces(d, L))) return advv1_samples(d, w, L, x, y, z);
-  return advv1_samples(d, rv, L, x, y, z);
-}
-
-template <class A>
 __device__ __forceinline__ float2 mmV(const A& T, float3 p, float ox, float oy, float oz, int ch) {
   return minmax1(T, floori_(p.x - ox), floori_(p.y - oy), floori_(p.z - oz), ch);
 }
-template <class A, class B>
-__device__ __forceinline__ float4 advv2_rest(const MU& u, const Dims& d, const A& V, const B& Vt, const Loc& L, float4 hat,
-                                             float4 old, int x, int y, int z) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
-  float fx = (float)x, fy = (float)y, fz = (float)z;
-  float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
-  float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
-  float4 b;
-  b.x = sampU(Vt, traceV(d, k, pu, L.vu, -1.0f));
-  b.y = sampV(Vt, traceV(d, k, pv, L.vv, -1.0f));
-  b.z = sampW(Vt, traceV(d, k, pw, L.vw, -1.0f));
-  b.w = sampC(Vt, traceV(d, k, pc, L.vc, -1.0f));
-  float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
-  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
-  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  float4 r = make_float4(hat.x + sm * (old.x - b.x), hat.y + sm * (old.y - b.y), hat.z + sm * (old.z - b.z),
-                         hat.w + sm * (old.w - b.w));
-  float2 m;
-  m = mmV(V, traceV(d, k, pu, L.vu, 1.0f), 0.0f, 0.5f, 0.5f, 0); r.x = vclamp_(r.x, m);
-  m = mmV(V, traceV(d, k, pv, L.vv, 1.0f), 0.5f, 0.0f, 0.5f, 1); r.y = vclamp_(r.y, m);
-  m = mmV(V, traceV(d, k, pw, L.vw, 1.0f), 0.5f, 0.5f, 0.0f, 2); r.z = vclamp_(r.z, m);
-  m = mmV(V, traceV(d, k, pc, L.vc, 1.0f), 0.5f, 0.5f, 0.5f, 3); r.w = vclamp_(r.w, m);
-  return r;
-}
-__device__ __forceinline__ float4 advv2_point(const MU& u, const Dims& d, const Ring& rv, const Ring& rt, int x, int y,
-                                              int z) {
-  RingW wv = rv.w(), wt = rt.w();
-  Loc L = localVel(wv, x, y, z);
-  float4 hat = wt.ld4(x, y, z), old = wv.ld4(x, y, z);
-  if (wave_all(near_traces(d, L))) return advv2_rest(u, d, wv, wt, L, hat, old, x, y, z);
-  return advv2_rest(u, d, rv, rt, L, hat, old, x, y, z);
-}
-
 // ====================== advection of moisture (model.js:482-576) ======================
 template <class A>
 __device__ __forceinline__ float3 cellVel(const A& V, int x, int y, int z) {
@@ -895,46 +841,6 @@ __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __re
   }
 }
 
-__global__ void __launch_bounds__(TILE_THREADS) k_advv1_t(MU u, const float4* __restrict__ V, float4* __restrict__ Vt) {
-  __shared__ float lds[RING_FLOATS];
-  TILE_SETUP
-  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
-  rv.load(-1, tid); rv.load(0, tid); rv.load(1, tid);
-  __syncthreads();
-  const bool act = x <= d.nx && y <= d.ny;
-  for (int z = 0; z <= d.nz; z++) {
-    Ring::Pre pv;
-    const bool more = z < d.nz;
-    if (more) rv.fetch(z + 2, tid, pv);
-    rv.center(z);
-    if (act) stx(Vt, vidx(d, x, y, z), advv1_point(d, rv, x, y, z));
-    __syncthreads();
-    if (more) rv.store(z + 2, tid, pv);
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(TILE_THREADS) k_advv2_t(MU u, const float4* __restrict__ V,
-                                                          const float4* __restrict__ Vt, float4* __restrict__ Vout) {
-  __shared__ float lds[2 * RING_FLOATS];
-  TILE_SETUP
-  Ring rv{lds, V, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
-  Ring rt{lds + RING_FLOATS, Vt, d.nx + 1, d.ny + 1, d.nz + 1, x0, y0, 0};
-  for (int k = -1; k <= 1; k++) { rv.load(k, tid); rt.load(k, tid); }
-  __syncthreads();
-  const bool act = x <= d.nx && y <= d.ny;
-  for (int z = 0; z <= d.nz; z++) {
-    Ring::Pre pv, pt;
-    const bool more = z < d.nz;
-    if (more) { rv.fetch(z + 2, tid, pv); rt.fetch(z + 2, tid, pt); }
-    rv.center(z); rt.center(z);
-    if (act) stx(Vout, vidx(d, x, y, z), advv2_point(u, d, rv, rt, x, y, z));
-    __syncthreads();
-    if (more) { rv.store(z + 2, tid, pv); rt.store(z + 2, tid, pt); }
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------------------------------------------
 // Moisture advection in one pass: predictor (WGSL_M_ADVQ1, model.js:502-530) and corrector with its limiter
 // (WGSL_M_ADVQ2, model.js:532-576) in one z-marching kernel, so the predicted fields never go through HBM.
@@ -1224,6 +1130,198 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
       const uint32_t i = cidx(d, x, y, z);
       stx(Q1o, i, o1);
       stx(Q2o, i, o2);
+    }
+    if (more) cur = nxt;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Wind advection in one pass, with the turbulent viscosity (outbreak mode): predictor (WGSL_M_ADVV1,
+// model.js:432-449), corrector with its limiter (WGSL_M_ADVV2, model.js:451-481) and WGSL_M_NUT
+// (model.js:342-389) in one z-marching kernel over the faces grid, as the moisture advection below: plane z + 1
+// is predicted for the tile and a one-point halo into an LDS ring, from an LDS ring of V with a two-point halo
+// that also holds every texel nut's stencil reads; the limiter's min / max come from the predictor's texels.
+// ---------------------------------------------------------------------------------------------------
+// a 3-plane view of a Ring2 for the accessor-templated point code (nut_point, localVel): reads within two
+// points of the tile and one plane of kc
+struct Win2A {
+  const lds_float* s;
+  int x0, y0, kc, om, oc, op;
+  __device__ __forceinline__ int pl(int k) const { return k < kc ? om : (k == kc ? oc : op); }
+  __device__ __forceinline__ int sidx(int i, int j, int k, int ch) const {
+    return pl(k) + ch * QS + __mul24(j - y0 + 2, QW) + (i - x0 + 2);
+  }
+  __device__ __forceinline__ float ldc(int i, int j, int k, int ch) const { return s[sidx(i, j, k, ch)]; }
+  __device__ __forceinline__ float4 ld4(int i, int j, int k) const {
+    int b = sidx(i, j, k, 0);
+    return make_float4(s[b], s[b + QS], s[b + 2 * QS], s[b + 3 * QS]);
+  }
+};
+__device__ __forceinline__ Win2A win2a(const Ring2& r, int kc) {
+  return Win2A{(const lds_float*)r.s, r.x0, r.y0, kc, Ring2::slot(kc - 1) * 4 * QS, Ring2::slot(kc) * 4 * QS,
+               Ring2::slot(kc + 1) * 4 * QS};
+}
+
+// the predictor of one face (model.js:432-449): each channel sampled at its own departure point, with the
+// min / max of the 8 texels there (the corrector's limiter)
+struct PredV {
+  float4 h;
+  float2 mm[4];
+};
+template <class A>
+__device__ __forceinline__ void predict_v(const Dims& d, const A& V, const Loc& L, int x, int y, int z, PredV& o) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float fx = (float)x, fy = (float)y, fz = (float)z;
+  float3 pu = traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f);
+  float3 pv = traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f);
+  float3 pw = traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f);
+  float3 pc = traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f);
+  o.h.x = samp_mm(V, pu.x, pu.y - 0.5f, pu.z - 0.5f, 0, o.mm[0]);
+  o.h.y = samp_mm(V, pv.x - 0.5f, pv.y, pv.z - 0.5f, 1, o.mm[1]);
+  o.h.z = samp_mm(V, pw.x - 0.5f, pw.y - 0.5f, pw.z, 2, o.mm[2]);
+  o.h.w = samp_mm(V, pc.x - 0.5f, pc.y - 0.5f, pc.z - 0.5f, 3, o.mm[3]);
+}
+// samp_mm on a 3-plane view (the predictor's texel boxes lie within one plane of kc)
+__device__ __forceinline__ float samp_mm(const Win2A& a, float tx, float ty, float tz, int ch, float2& mm) {
+  const Win2 w{a.s, a.x0, a.y0, a.kc, a.om, a.oc, a.op};
+  return samp_mm(w, tx, ty, tz, ch, mm);
+}
+// the predicted channel ch of face texel (i, j, k) (clamped), from global memory
+__device__ __noinline__ static float vhat_at(const Dims& d, const float4* V, int i, int j, int k, int ch) {
+  i = clampi(i, 0, d.nx); j = clampi(j, 0, d.ny); k = clampi(k, 0, d.nz);
+  GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
+  const Loc L = localVel(gv, i, j, k);
+  float3 kk = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float fx = (float)i, fy = (float)j, fz = (float)k;
+  if (ch == 0) return sampU(gv, traceV(d, kk, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
+  if (ch == 1) return sampV(gv, traceV(d, kk, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
+  if (ch == 2) return sampW(gv, traceV(d, kk, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, 1.0f));
+  return sampC(gv, traceV(d, kk, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, 1.0f));
+}
+__device__ __noinline__ static float vhat_sample_global(const Dims& d, const float4* V, float tx, float ty, float tz,
+                                                        int ch) {
+  int i0 = floori_(tx), j0 = floori_(ty), k0 = floori_(tz);
+  float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
+  float c[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) c[q] = vhat_at(d, V, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2), ch);
+  float c00 = lerpf(c[0], c[1], ax), c10 = lerpf(c[2], c[3], ax), c01 = lerpf(c[4], c[5], ax), c11 = lerpf(c[6], c[7], ax);
+  return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
+}
+struct CarryV {
+  Loc L;
+  float4 old;
+  bool near;
+  float2 mm[4];
+};
+// the corrector of one face (model.js:451-481)
+template <bool NEAR>
+__device__ __forceinline__ float4 correct_v(const MU& u, const Dims& d, const RingW& wt, const CarryV& c,
+                                            const float4* V, int x, int y, int z) {
+  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float fx = (float)x, fy = (float)y, fz = (float)z;
+  const Loc& L = c.L;
+  float3 pu = traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, -1.0f);
+  float3 pv = traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, -1.0f);
+  float3 pw = traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz), L.vw, -1.0f);
+  float3 pc = traceV(d, k, make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f), L.vc, -1.0f);
+  float4 b;
+  if (NEAR) {
+    b = make_float4(samp1(wt, pu.x, pu.y - 0.5f, pu.z - 0.5f, 0), samp1(wt, pv.x - 0.5f, pv.y, pv.z - 0.5f, 1),
+                    samp1(wt, pw.x - 0.5f, pw.y - 0.5f, pw.z, 2), samp1(wt, pc.x - 0.5f, pc.y - 0.5f, pc.z - 0.5f, 3));
+  } else {
+    b = make_float4(vhat_sample_global(d, V, pu.x, pu.y - 0.5f, pu.z - 0.5f, 0),
+                    vhat_sample_global(d, V, pv.x - 0.5f, pv.y, pv.z - 0.5f, 1),
+                    vhat_sample_global(d, V, pw.x - 0.5f, pw.y - 0.5f, pw.z, 2),
+                    vhat_sample_global(d, V, pc.x - 0.5f, pc.y - 0.5f, pc.z - 0.5f, 3));
+  }
+  const float4 hat = wt.ld4(x, y, z), old = c.old;
+  float cx = L.vc.x * k.x, cy = L.vc.y * k.y, cz = L.vc.z * k.z;
+  float cour = sqrtf(cx * cx + cy * cy + cz * cz);
+  float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
+  return make_float4(vclamp_(hat.x + sm * (old.x - b.x), c.mm[0]), vclamp_(hat.y + sm * (old.y - b.y), c.mm[1]),
+                     vclamp_(hat.z + sm * (old.z - b.z), c.mm[2]), vclamp_(hat.w + sm * (old.w - b.w), c.mm[3]));
+}
+
+// (the wind advection's point code needs ~210 VGPRs: two blocks per CU; ADVV_MIN_BLOCKS = 3 makes the compiler fit
+// three, with spills)
+#ifndef ADVV_MIN_BLOCKS
+#define ADVV_MIN_BLOCKS 2
+#endif
+__global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                         const float4* __restrict__ Q1, float4* __restrict__ Vout,
+                                                         float2* __restrict__ aux) {
+  __shared__ float lds[QRING_FLOATS + RING_FLOATS];
+  TILE_SETUP
+  const int W = d.nx + 1, H = d.ny + 1, D = d.nz + 1;
+  Ring2 vr{lds, V, W, H, D, x0, y0};
+  Ring hr{lds + QRING_FLOATS, nullptr, W, H, D, x0, y0, 0};
+  GAcc gv{V, W, H, D, nullptr};
+  const bool act = x <= d.nx && y <= d.ny, cell = x < d.nx && y < d.ny;
+  const int cx1 = clampi(x, 0, d.nx), cy1 = clampi(y, 0, d.ny);
+  int hsi = 0, hsj = 0;
+  halo_slot(min(tid, NHALO - 1), hsi, hsj);
+  const int cx2 = clampi(x0 - 1 + hsi, 0, d.nx), cy2 = clampi(y0 - 1 + hsj, 0, d.ny);
+  const int own = __mul24(tid / TX + 1, TPW) + tid % TX + 1, hal = __mul24(hsj, TPW) + hsi;
+
+  // predicts face plane kp into the predicted ring, and the viscosity of the cells of level kp (kp < nz)
+  auto predict_plane = [&](int kp, float4 q1, CarryV& cr) {
+    const Win2A w = win2a(vr, kp);
+    float* Hp = hr.s + Ring::slot(kp) * 4 * TPS;
+    PredV pr;
+    cr.L = localVel(w, cx1, cy1, kp);
+    cr.old = w.ld4(cx1, cy1, kp);
+    cr.near = near_traces(d, cr.L);
+    if (wave_all(cr.near)) predict_v(d, w, cr.L, cx1, cy1, kp, pr);
+    else predict_v(d, gv, cr.L, cx1, cy1, kp, pr);
+    Hp[own] = pr.h.x; Hp[TPS + own] = pr.h.y; Hp[2 * TPS + own] = pr.h.z; Hp[3 * TPS + own] = pr.h.w;
+#pragma unroll
+    for (int q = 0; q < 4; q++) cr.mm[q] = pr.mm[q];
+    if (cell && kp < d.nz) stx(aux, cidx(d, x, y, kp), nut_point(u, d, BS, w, q1, x, y, kp));
+    if (tid < NHALO) {
+      const Loc L = localVel(w, cx2, cy2, kp);
+      if (wave_all(near_traces(d, L))) predict_v(d, w, L, cx2, cy2, kp, pr);
+      else predict_v(d, gv, L, cx2, cy2, kp, pr);
+      Hp[hal] = pr.h.x; Hp[TPS + hal] = pr.h.y; Hp[2 * TPS + hal] = pr.h.z; Hp[3 * TPS + hal] = pr.h.w;
+    }
+  };
+
+  vr.load(-1, tid); vr.load(0, tid); vr.load(1, tid);
+  // the cell's Q1 texel for nut, loaded one level ahead
+  float4 q1n = cell ? ldQ(Q1, d, x, y, 0) : make_float4(0.f, 0.f, 0.f, 0.f);
+  __syncthreads();
+  CarryV cur;
+  {
+    const float4 q1 = q1n;
+    q1n = cell ? ldQ(Q1, d, x, y, 1) : q1;
+    predict_plane(0, q1, cur);
+  }
+  __syncthreads();
+  {
+    Ring2::Pre a;
+    vr.fetch(2, tid, a);
+    vr.store(2, tid, a);
+  }
+  __syncthreads();
+  for (int z = 0; z <= d.nz; z++) {
+    Ring2::Pre pv;
+    const bool more = z < d.nz;
+    vr.fetch(z + 3, tid, pv);
+    const float4 q1 = q1n;
+    q1n = cell ? ldQ(Q1, d, x, y, z + 2) : q1;
+    CarryV nxt;
+    if (more) predict_plane(z + 1, q1, nxt);
+    __syncthreads();
+    vr.store(z + 3, tid, pv);
+    hr.center(z);
+    if (z == 0) hr.om = hr.oc;
+    if (!more) hr.op = hr.oc;
+    if (act) {
+      float4 o;
+      if (wave_all(cur.near)) o = correct_v<true>(u, d, hr.w(), cur, V, x, y, z);
+      else o = correct_v<false>(u, d, hr.w(), cur, V, x, y, z);
+      stx(Vout, vidx(d, x, y, z), o);
     }
     if (more) cur = nxt;
     __syncthreads();
@@ -1624,7 +1722,7 @@ __device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const 
   }
 }
 
-// z-marching drivers (as k_advv1_t ... k_advq2_t). PREP: the carried winds Vval are a field of
+// z-marching drivers (as the outbreak advections). PREP: the carried winds Vval are a field of
 // their own (the prep field), with a ring of their own; otherwise they are V.
 template <bool PREP>
 __global__ void __launch_bounds__(TILE_THREADS) k_advv1_hr(MU u, const float4* __restrict__ V,
@@ -1834,11 +1932,9 @@ void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, con
   else
     hipLaunchKernelGGL(k_nut_t<false>, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, sfc, aux);
 }
-void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt) {
-  hipLaunchKernelGGL(k_advv1_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt);
-}
-void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout) {
-  hipLaunchKernelGGL(k_advv2_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, V, Vt, Vout);
+void launch_advv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                 float4* Vout, float2* aux) {
+  hipLaunchKernelGGL(k_advv_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Vout, aux);
 }
 void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                  const float4* Q2, float4* Q1o, float4* Q2o) {

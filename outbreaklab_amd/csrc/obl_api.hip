@@ -1089,9 +1089,7 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   if (flags & OBL_STEP_FILT) return h->fail("the 2-dx filter (OBL_STEP_FILT) is a hurricane-mode step");
   if (flags & OBL_STEP_LSF) TIMED(T, s, "m.lsf", launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT));
   TIMED(T, s, "m.ci", launch_ci(s, u, d, h->CI, h->ciMap));
-  TIMED(T, s, "m.nut", launch_nut(s, u, d, h->BS, Vc, h->Q1[0], nullptr, h->aux));
-  TIMED(T, s, "m.advV1", launch_advv1(s, u, d, Vc, h->Vt));
-  TIMED(T, s, "m.advV2", launch_advv2(s, u, d, Vc, h->Vt, Vn));
+  TIMED(T, s, "m.advV", launch_advv(s, u, d, h->BS, Vc, h->Q1[0], Vn, h->aux));
   TIMED(T, s, "m.advQ", launch_advq(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1[1], h->Q2[1]));
   if (h->net) {
     MlpView mv;

@@ -18,8 +18,10 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
 // sfc: the sea surface map (nx * ny float4: land fraction, SST K, 0, 0); null outside hurricane mode
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 const float4* sfc, float2* aux);
-void launch_advv1(hipStream_t s, const MU& u, const Dims& d, const float4* V, float4* Vt);
-void launch_advv2(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float4* Vt, float4* Vout);
+// the predictor and corrector of the wind advection and the turbulent viscosity in one pass (outbreak mode):
+// V, Q1 -> Vout, aux
+void launch_advv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                 float4* Vout, float2* aux);
 // the predictor and corrector of the moisture advection in one pass (outbreak mode): Q1, Q2 -> Q1o, Q2o
 void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                  const float4* Q2, float4* Q1o, float4* Q2o);
