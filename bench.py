@@ -361,7 +361,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--gemm", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--gemm", default="fp32", choices=["fp32", "bf16x3", "bf16x6"])
     ap.add_argument("--thomas", default="mixed", choices=["fp32", "fp64", "mixed"])
     ap.add_argument("--size", default="1024,1024,80")
     ap.add_argument("--no-cpu", action="store_true")

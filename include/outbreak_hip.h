@@ -88,7 +88,8 @@ enum {
 /* Solver precision (obl_set_solver_mode / obl_poisson_solve) */
 enum {
   OBL_GEMM_FP32 = 0,   /* FP32 MFMA sine transforms */
-  OBL_GEMM_BF16X3 = 1  /* split hi/lo BF16 products, FP32 accumulate */
+  OBL_GEMM_BF16X3 = 1, /* split hi/lo BF16 products, FP32 accumulate (about 16 bits) */
+  OBL_GEMM_BF16X6 = 2  /* three BF16 pieces and the six products down to order 2^-16: FP32 accuracy */
 };
 enum {
   OBL_THOMAS_FP32 = 0,  /* the reference's FP32 Thomas (solver.js:86-101) */

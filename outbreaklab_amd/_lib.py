@@ -98,7 +98,7 @@ FIELD = {"V": 0, "Q1": 1, "Q2": 2, "Vt": 3, "Q1t": 4, "Q2t": 5, "aux": 6, "phT":
 STEP_LSF = 1
 STEP_JUDGE = 2
 STEP_FILT = 4
-GEMM = {"fp32": 0, "bf16x3": 1}
+GEMM = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 THOMAS = {"fp32": 0, "fp64": 1, "mixed": 2}
 
 

@@ -4,16 +4,20 @@
 // Here every pass uses the even/odd symmetry of the DST-II basis,
 // v_l(N-1-i) = (-1)^l v_l(i): the even modes see only S(i) = F(i) + F(N-1-i),
 // the odd modes only D(i) = F(i) - F(N-1-i), i < N/2. Each pass is therefore
-// two half-size GEMMs, computed together by one kernel on the MFMA units
-// (v_mfma_f32_16x16x4f32), and the flops halve. Spectral arrays keep the even
-// modes in the first half of each axis and the odd modes in the second half;
-// the Thomas kernel uses eigenvalues permuted the same way.
+// two half-size GEMMs, computed together by one kernel on the MFMA units, and
+// the flops halve. Spectral arrays keep the even modes in the first half of each
+// axis and the odd modes in the second half; the Thomas kernel uses eigenvalues
+// permuted the same way.
 //
-// Passes (rows of the padded grid F[z][y][x], x fastest):
-//   XF  G[r][s]      = sum_i S/D[r][i] W[s][i]            r = y + nyp z
-//   YF  G[z][s][x]   = sum_j W[s][j] S/D[z][j][x]
-//   XI  F[r][i], F[r][N-1-i] = E +- O, E = sum_s G[r][s] WT[i][s]
-//   YI  F[z][j][x], F[z][N-1-j][x] = E +- O
+// Two GEMM forms:
+//  - FP32 (k_dst, v_mfma_f32_16x16x4f32) on FP32 arrays. Passes (rows of the
+//    padded grid F[z][y][x], x fastest):
+//      XF  G[r][s]      = sum_i S/D[r][i] W[s][i]            r = y + nyp z
+//      YF  G[z][s][x]   = sum_j W[s][j] S/D[z][j][x]
+//      XI  F[r][i], F[r][N-1-i] = E +- O, E = sum_s G[r][s] WT[i][s]
+//      YI  F[z][j][x], F[z][N-1-j][x] = E +- O
+//  - multi-piece BF16 (k_dst_mx, v_mfma_f32_32x32x16_bf16) on operands stored
+//    already split, each pass's epilogue writing the next pass's operand.
 // Thomas: one (sx, sy) mode per work-item, z swept in registers; FP32 with the
 // reference's operation order, FP64, or FP64 only for the modes whose vertical
 // system is near-singular.
@@ -25,17 +29,8 @@
 namespace obl {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef short v8s __attribute__((ext_vector_type(8)));
 
 constexpr int BM = 64, BN = 64, BK = 16;
-
-// bf16 split of an fp32 value: hi = RNE(x), lo = RNE(x - hi)
-__device__ __forceinline__ short bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
-  return (short)(r >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
 // Workgroup b of a launch runs on XCD b % 8. Renumber the tiles so that each XCD receives a
 // contiguous range of them: the tiles that share a data block (consecutive in the tile order)
@@ -234,244 +229,328 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
       }
 }
 
-// ---------------- split hi/lo BF16 form ----------------
-// Each FP32 operand x is split into hi = RNE_bf16(x) and lo = RNE_bf16(x - hi); the
-// product is hi*hi + hi*lo + lo*hi accumulated in FP32 (the lo*lo term, ~2^-18 relative,
-// is dropped). v_mfma_f32_16x16x32_bf16, 128 x 128 block tiles, 64 x 64 per wave.
-// The sine matrices are split once on the host and staged as BF16 hi/lo tiles. The data tile is
-// staged in LDS as FP32 (even / odd folded), and each wave splits the fragments it reads with the
-// hardware's packed conversion (v_cvt_pk_bf16_f32), so the split overlaps its MFMAs instead of
-// sitting between the global loads and the barrier.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
-constexpr int BBM = 128, BBN = 128, BBK = 32;
-constexpr int BLD = BBK + 8;   // matrix LDS row: 40 bf16 = 80 B
-constexpr int DLD = BBK + 4;   // data LDS row: 36 floats = 144 B
 
-// hi / lo BF16 halves of 8 consecutive floats (RNE, as the host split of the matrices)
-__device__ __forceinline__ void split8_hw(float4 v0, float4 v1, bf16x8& hi, bf16x8& lo) {
-  const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-  uint32_t h[4], l[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    bf16x2 hb = __builtin_convertvector(f32x2{x[2 * i], x[2 * i + 1]}, bf16x2);
-    uint32_t hu = __builtin_bit_cast(uint32_t, hb);
-    float h0 = __uint_as_float(hu << 16), h1 = __uint_as_float(hu & 0xffff0000u);
-    bf16x2 lb = __builtin_convertvector(f32x2{x[2 * i] - h0, x[2 * i + 1] - h1}, bf16x2);
-    h[i] = hu;
-    l[i] = __builtin_bit_cast(uint32_t, lb);
-  }
-  hi = __builtin_bit_cast(bf16x8, make_uint4(h[0], h[1], h[2], h[3]));
-  lo = __builtin_bit_cast(bf16x8, make_uint4(l[0], l[1], l[2], l[3]));
+// ---------------- multi-piece BF16 form (k_dst_mx) ----------------
+// Every FP32 operand x is carried as P BF16 pieces, x ~ x1 + x2 (+ x3), each piece the round-to-nearest-even BF16
+// of what the pieces before it leave. P = 2 keeps the products x1 y1 + x1 y2 + x2 y1 (split hi/lo BF16, about 16
+// bits); P = 3 adds x1 y3 + x3 y1 + x2 y2, the terms of order 2^-16, and carries the transform at FP32 accuracy.
+// v_mfma_f32_32x32x16_bf16 accumulates in FP32, the smallest products first.
+//
+// The operands are stored already split, in a tiled layout: an [R][K] operand (K the reduction index) is cut
+// into 32-row x 16-k tiles, 1 KB per piece; tile (rb, kb) of piece q and parity p is the 1 KB block number
+// ((p P + q) RB + rb) KB + kb, and inside a tile the 16-byte chunk c holds row c / 2, k-half (c & 1) ^ s(row),
+// s(r) = bit 2 of r xor bit 3 of r. A tile is then one contiguous 1 KB that one wave instruction copies into
+// LDS (global_load_lds_dwordx4: lane l fills bytes 16 l .. 16 l + 15), and the fragment reads of the MFMA (lane
+// l: row l % 32, k-half l / 32) fall in distinct banks. Each operand's producer writes it in this layout: the
+// fold-and-split pass for the x forward pass, the epilogue of the pass before, or the Thomas sweep.
+//
+// Block: 4 waves, 128 rows x 64 NF columns of each parity; waves 0-1 compute the even half, waves 2-3 the odd
+// half, each 128 x 32 NF (4 x NF fragments of 32 x 32). K advances 16 per stage through two LDS buffers: the
+// copies of stage s + 1 are in flight while the MFMAs of stage s run.
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int mx_chunk(int ri, int kc) { return 2 * ri + (kc ^ (((ri >> 2) ^ (ri >> 3)) & 1)); }
+// bf16 offset of element (row, k) inside one piece plane of a tiled operand with KB k-blocks
+__device__ __forceinline__ size_t mx_off(int row, int k, int KB) {
+  return ((size_t)(row >> 5) * KB + (k >> 4)) * 512 + mx_chunk(row & 31, (k >> 3) & 1) * 8 + (k & 7);
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_dst_bf16(DstArgs a) {
-  // One LDS stage. The global loads of stage s + 1 are issued into registers before the MFMAs of
-  // stage s and written to LDS after them, so their latency overlaps this stage's MFMAs.
-  constexpr bool XMODE = (MODE == DST_XF || MODE == DST_XI);
-  // data: [parity][row (X: data row; Y: output column n)][k] FP32; matrix: [parity][hi/lo][row][k]
-  __shared__ __attribute__((aligned(16))) float Ds[2][XMODE ? BBM : BBN][DLD];
-  __shared__ __attribute__((aligned(16))) uint16_t Ms[2][2][XMODE ? BBN : BBM][BLD];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int lr = lane & 15, g = lane >> 4;
-  int mt, nt, plane;
-  dst_tile<XMODE>(mt, nt, plane);
-  const int m0 = mt * BBM, n0 = nt * BBN;
-  const int N = a.N, Nh = a.Nh;
-  const size_t planeSz = (size_t)N * (size_t)a.inner;
-
-  v4f acc[2][4][4];
+// P pieces of 4 floats, 4 bf16 (8 bytes) per piece; the hardware conversion rounds to nearest even
+template <int P>
+__device__ __forceinline__ void mx_split4(float r0, float r1, float r2, float r3, uint2* out) {
 #pragma unroll
-  for (int p = 0; p < 2; p++)
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) acc[p][i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-
-  // staging registers: matrix [array][half of the 32 k] and data (X: 4 float4 of row `row` at
-  // k, then 4 at the mirrored / odd position; Y: 4 k rows of 4 columns, then their partners)
-  uint4 w0a, w0b, w1a, w1b, w2a, w2b, w3a, w3b;
-  float4 pd[8];
-  const int row = tid >> 1, half = tid & 1;
-  const int ng = tid & 31, kq = tid >> 5;
-  const float* pl = a.in + (XMODE ? 0 : (size_t)plane * planeSz);
-  auto fetch = [&](int k0) {
-    const size_t wo = (size_t)((XMODE ? n0 : m0) + row) * a.KpB + k0 + 16 * half;
-    const uint4* q0 = reinterpret_cast<const uint4*>(a.W0h + wo);
-    const uint4* q1 = reinterpret_cast<const uint4*>(a.W0l + wo);
-    const uint4* q2 = reinterpret_cast<const uint4*>(a.W1h + wo);
-    const uint4* q3 = reinterpret_cast<const uint4*>(a.W1l + wo);
-    w0a = q0[0]; w0b = q0[1];
-    w1a = q1[0]; w1b = q1[1];
-    w2a = q2[0]; w2b = q2[1];
-    w3a = q3[0]; w3b = q3[1];
-#pragma unroll
-    for (int c = 0; c < 8; c++) pd[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (XMODE) {
-      const int r = m0 + row;
-      const float* sr = a.in + (size_t)r * N;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int kk = k0 + 16 * half + 4 * c;
-        if (r < a.rows && kk < Nh) {
-          pd[c] = *reinterpret_cast<const float4*>(sr + kk);
-          pd[4 + c] = *reinterpret_cast<const float4*>(sr + (MODE == DST_XF ? N - 4 - kk : Nh + kk));
-        }
-      }
-    } else {
-      const int gn = n0 + 4 * ng;
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) {
-        const int k = k0 + 4 * kq + kk;
-        if (k < Nh && gn < a.inner) {
-          pd[kk] = *reinterpret_cast<const float4*>(pl + (size_t)k * a.inner + gn);
-          pd[4 + kk] = *reinterpret_cast<const float4*>(pl + (size_t)(MODE == DST_YF ? N - 1 - k : Nh + k) * a.inner + gn);
-        }
-      }
+  for (int q = 0; q < P; q++) {
+    uint32_t u01 = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r0, r1}, bf16x2));
+    uint32_t u23 = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r2, r3}, bf16x2));
+    out[q] = make_uint2(u01, u23);
+    if (q + 1 < P) {
+      r0 -= __uint_as_float(u01 << 16);
+      r1 -= __uint_as_float(u01 & 0xffff0000u);
+      r2 -= __uint_as_float(u23 << 16);
+      r3 -= __uint_as_float(u23 & 0xffff0000u);
     }
-  };
-  auto stash = [&]() {
-    *reinterpret_cast<uint4*>(&Ms[0][0][row][16 * half]) = w0a;
-    *reinterpret_cast<uint4*>(&Ms[0][0][row][16 * half + 8]) = w0b;
-    *reinterpret_cast<uint4*>(&Ms[0][1][row][16 * half]) = w1a;
-    *reinterpret_cast<uint4*>(&Ms[0][1][row][16 * half + 8]) = w1b;
-    *reinterpret_cast<uint4*>(&Ms[1][0][row][16 * half]) = w2a;
-    *reinterpret_cast<uint4*>(&Ms[1][0][row][16 * half + 8]) = w2b;
-    *reinterpret_cast<uint4*>(&Ms[1][1][row][16 * half]) = w3a;
-    *reinterpret_cast<uint4*>(&Ms[1][1][row][16 * half + 8]) = w3b;
-    if (XMODE) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const float4 f = pd[c], h = pd[4 + c];
-        float4 s0 = f, s1 = h;
-        if (MODE == DST_XF) {
-          s0 = make_float4(f.x + h.w, f.y + h.z, f.z + h.y, f.w + h.x);
-          s1 = make_float4(f.x - h.w, f.y - h.z, f.z - h.y, f.w - h.x);
-        }
-        *reinterpret_cast<float4*>(&Ds[0][row][16 * half + 4 * c]) = s0;
-        *reinterpret_cast<float4*>(&Ds[1][row][16 * half + 4 * c]) = s1;
-      }
-    } else {
-      // 4 k x 4 n per thread, transposed to [n][k] (float4 along k)
-      float s0[4][4], s1[4][4];
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) {
-        const float4 f = pd[kk], h = pd[4 + kk];
-        float4 f0 = f, f1 = h;
-        if (MODE == DST_YF) {
-          f0 = make_float4(f.x + h.x, f.y + h.y, f.z + h.z, f.w + h.w);
-          f1 = make_float4(f.x - h.x, f.y - h.y, f.z - h.z, f.w - h.w);
-        }
-        s0[kk][0] = f0.x; s0[kk][1] = f0.y; s0[kk][2] = f0.z; s0[kk][3] = f0.w;
-        s1[kk][0] = f1.x; s1[kk][1] = f1.y; s1[kk][2] = f1.z; s1[kk][3] = f1.w;
-      }
-#pragma unroll
-      for (int nn = 0; nn < 4; nn++) {
-        *reinterpret_cast<float4*>(&Ds[0][4 * ng + nn][4 * kq]) = make_float4(s0[0][nn], s0[1][nn], s0[2][nn], s0[3][nn]);
-        *reinterpret_cast<float4*>(&Ds[1][4 * ng + nn][4 * kq]) = make_float4(s1[0][nn], s1[1][nn], s1[2][nn], s1[3][nn]);
-      }
-    }
-  };
-
-  fetch(0);
-  for (int k0 = 0; k0 < Nh; k0 += BBK) {
-    stash();
-    __syncthreads();
-    if (k0 + BBK < Nh) fetch(k0 + BBK);
-    // ---- MFMA: lane group g feeds k = 8 g .. 8 g + 7 of both operands ----
-    // A = rows m (X: data, Y: matrix), B = columns n (X: matrix, Y: data); the four A fragments
-    // are held for the parity, one B fragment at a time.
-    const int dbase = XMODE ? wm * 64 : wn * 64, mbase = XMODE ? wn * 64 : wm * 64;
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-      bf16x8 ah[4], al[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if (XMODE) {
-          const float* dp = &Ds[p][dbase + i * 16 + lr][8 * g];
-          split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), ah[i], al[i]);
-        } else {
-          ah[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + i * 16 + lr][8 * g]);
-          al[i] = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + i * 16 + lr][8 * g]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        bf16x8 bh, bl;
-        if (XMODE) {
-          bh = *reinterpret_cast<const bf16x8*>(&Ms[p][0][mbase + j * 16 + lr][8 * g]);
-          bl = *reinterpret_cast<const bf16x8*>(&Ms[p][1][mbase + j * 16 + lr][8 * g]);
-        } else {
-          const float* dp = &Ds[p][dbase + j * 16 + lr][8 * g];
-          split8_hw(*reinterpret_cast<const float4*>(dp), *reinterpret_cast<const float4*>(dp + 4), bh, bl);
-        }
-        // small terms first; four independent accumulators between two updates of one
-#pragma unroll
-        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[p][i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[p][i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) acc[p][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[p][i][j], 0, 0, 0);
-      }
-    }
-    __syncthreads();
   }
-  // ---- epilogue (as the FP32 form) ----
+}
+template <int P>
+__device__ __forceinline__ void mx_split1(float r, uint16_t* out) {
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r, 0.0f}, bf16x2));
+    out[q] = (uint16_t)u;
+    if (q + 1 < P) r -= __uint_as_float(u << 16);
+  }
+}
+
+// Fold-and-split of the right-hand side into the x forward pass's operand. Its rows come in 128-row tiles of 64
+// (z, y) pairs: row 64 h + q of tile t is y' (h = 0) or ny - 1 - y' (h = 1) of pair 64 t + q = z (ny / 2) + y',
+// so the pass's epilogue finds both rows of a pair in one lane and can fold y as well. k runs over x' < nx / 2:
+// parity 0 holds F(x') + F(nx - 1 - x'), parity 1 F(x') - F(nx - 1 - x'). One wave writes one tile of every piece
+// and parity (lane l: chunk l).
+template <int P>
+__global__ void __launch_bounds__(256) k_mx_fold_x(const float* __restrict__ rhs, uint16_t* __restrict__ out, int Nx,
+                                                   int Ny, int nz, int RB, int KB) {
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (wave >= RB * KB) return;
+  const int rb = wave / KB, kb = wave % KB;
+  const int ri = lane >> 1, kc = (lane & 1) ^ (((ri >> 2) ^ (ri >> 3)) & 1);
+  const int r = rb * 32 + ri, t = r >> 7, rho = r & 127;
+  const int Hx = Nx >> 1, Hy = Ny >> 1;
+  const int pair = 64 * t + (rho & 63);
+  const int k0 = kb * 16 + kc * 8;
+  float s[8], d[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) s[e] = d[e] = 0.0f;
+  if (pair < nz * Hy) {
+    const int z = pair / Hy, yp = pair % Hy, y = (rho >> 6) ? Ny - 1 - yp : yp;
+    const float* row = rhs + ((size_t)z * Ny + y) * Nx;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int k = k0 + 4 * c;
+      if (k < Hx) {  // Hx is a multiple of 4: a float4 is all in or all out
+        float4 f = *reinterpret_cast<const float4*>(row + k);
+        float4 g = *reinterpret_cast<const float4*>(row + (Nx - 4 - k));
+        s[4 * c + 0] = f.x + g.w; d[4 * c + 0] = f.x - g.w;
+        s[4 * c + 1] = f.y + g.z; d[4 * c + 1] = f.y - g.z;
+        s[4 * c + 2] = f.z + g.y; d[4 * c + 2] = f.z - g.y;
+        s[4 * c + 3] = f.w + g.x; d[4 * c + 3] = f.w - g.x;
+      }
+    }
+  }
+  uint2 ps[2][P], pd[2][P];
+  mx_split4<P>(s[0], s[1], s[2], s[3], ps[0]);
+  mx_split4<P>(s[4], s[5], s[6], s[7], ps[1]);
+  mx_split4<P>(d[0], d[1], d[2], d[3], pd[0]);
+  mx_split4<P>(d[4], d[5], d[6], d[7], pd[1]);
+  const size_t plane = (size_t)RB * KB * 512, tile = ((size_t)rb * KB + kb) * 512 + lane * 8;
+#pragma unroll
+  for (int q = 0; q < P; q++) {
+    *reinterpret_cast<uint4*>(out + (size_t)q * plane + tile) = make_uint4(ps[0][q].x, ps[0][q].y, ps[1][q].x, ps[1][q].y);
+    *reinterpret_cast<uint4*>(out + (size_t)(P + q) * plane + tile) =
+        make_uint4(pd[0][q].x, pd[0][q].y, pd[1][q].x, pd[1][q].y);
+  }
+}
+
+template <int MODE, int P, int NF>
+__global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
+  constexpr int TB = 4 + 2 * NF;           // tiles per (parity, piece) and stage: 4 of A, 2 NF of B
+  constexpr int STAGE = 2 * P * TB * 512;  // bf16 per stage
+  constexpr int NPROD = P == 2 ? 3 : 6;
+  // products (piece of A, piece of B), smallest first
+  constexpr int PA[6] = {P == 2 ? 1 : 2, P == 2 ? 0 : 1, 0, 1, 0, 0};
+  constexpr int PB[6] = {0, P == 2 ? 1 : 1, P == 2 ? 0 : 2, 0, 1, 0};
+  __shared__ __attribute__((aligned(1024))) uint16_t L0[STAGE];
+  __shared__ __attribute__((aligned(1024))) uint16_t L1[STAGE];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wp = wave >> 1, wh = wave & 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int nt = t % a.tilesN, mt = t / a.tilesN;
+
+  // this wave's tile copies, for every (parity, piece): A row-block `wave` and B row-blocks wave + 4 m
+  const size_t planeA = (size_t)a.aRB * a.KB * 512, planeB = (size_t)a.bRB * a.KB * 512;
+  const uint16_t* srcA = a.A + ((size_t)(mt * 4 + wave) * a.KB) * 512 + lane * 8;
+  const uint16_t* srcB = a.B + ((size_t)(nt * 2 * NF + wave) * a.KB) * 512 + lane * 8;
+  auto copy = [&](uint16_t* buf, int kb) {
+#pragma unroll
+    for (int pq = 0; pq < 2 * P; pq++) {
+      __builtin_amdgcn_global_load_lds((const void*)(srcA + pq * planeA + (size_t)kb * 512),
+                                       (__attribute__((address_space(3))) void*)(buf + (pq * TB + wave) * 512), 16, 0,
+                                       0);
+#pragma unroll
+      for (int m = 0; m < (2 * NF + 3) / 4; m++)
+        if (wave + 4 * m < 2 * NF)
+          __builtin_amdgcn_global_load_lds(
+              (const void*)(srcB + pq * planeB + (size_t)(4 * m * a.KB + kb) * 512),
+              (__attribute__((address_space(3))) void*)(buf + (pq * TB + 4 + wave + 4 * m) * 512), 16, 0, 0);
+    }
+  };
+
+  v16f acc[4][NF];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < NF; j++)
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        int m = m0 + wm * 64 + i * 16 + 4 * g + e;
-        int n = n0 + wn * 64 + j * 16 + lr;
-        float v0 = acc[0][i][j][e], v1 = acc[1][i][j][e];
-        if (MODE == DST_XF) {
-          if (m < a.rows && n < Nh) {
-            float* dst = a.out + (size_t)m * N;
-            dst[n] = v0;
-            dst[Nh + n] = v1;
+      for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+
+  const int loff = mx_chunk(lane & 31, lane >> 5) * 8;
+  auto compute = [&](const uint16_t* buf) {
+    const uint16_t* Lp = buf + wp * P * TB * 512 + loff;
+    bf16x8 af[4][P];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int q = 0; q < P; q++) af[i][q] = *reinterpret_cast<const bf16x8*>(Lp + (q * TB + i) * 512);
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      bf16x8 bfr[P];
+#pragma unroll
+      for (int q = 0; q < P; q++) bfr[q] = *reinterpret_cast<const bf16x8*>(Lp + (q * TB + 4 + wh * NF + j) * 512);
+#pragma unroll
+      for (int c = 0; c < NPROD; c++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][PA[c]], bfr[PB[c]], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  // KB is even (the operands carry a zero k-block when N / 2 needs an odd number)
+  copy(L0, 0);
+  for (int kb = 0; kb < a.KB; kb += 2) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    copy(L1, kb + 1);
+    compute(L0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (kb + 2 < a.KB) copy(L0, kb + 2);
+    compute(L1);
+  }
+  __syncthreads();  // LDS free for the epilogue
+
+  // ---- epilogue: lane l, fragment (i, j), element e sits at row 32 i + 8 (e / 4) + 4 (l / 32) + e % 4 of the
+  // block tile and column 32 j + l % 32 of the wave's columns ----
+  const int Hx = a.Nx >> 1, Hy = a.Ny >> 1;
+  const int ncol0 = nt * 64 * NF + wh * 32 * NF + (lane & 31);
+  const int rsub = 4 * (lane >> 5);
+  if (MODE == DST_XF) {
+    // fold y: rows 32 i + r and 64 + 32 i + r of the tile are y' and ny - 1 - y' of one pair; write the next
+    // pass's operand: row z nx + sx, k = y', parity 0 the sum, 1 the difference
+    const size_t plane = (size_t)a.oRB * a.oKB * 512;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        const int n = ncol0 + 32 * j;
+        if (n >= Hx) continue;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const int pair = mt * 64 + 32 * i + 8 * g + rsub;
+          if (pair >= a.Mvalid) continue;
+          const int z = pair / Hy, yp = pair % Hy;
+          const int R = z * a.Nx + wp * Hx + n;
+          float s[4], d[4];
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const float lo = acc[i][j][4 * g + e], hi = acc[i + 2][j][4 * g + e];
+            s[e] = lo + hi;
+            d[e] = lo - hi;
           }
-        } else if (MODE == DST_XI) {
-          if (m < a.rows && n < Nh) {
-            float* dst = a.out + (size_t)m * N;
-            dst[n] = v0 + v1;
-            dst[N - 1 - n] = v0 - v1;
-          }
-        } else if (MODE == DST_YF) {
-          if (m < Nh && n < a.inner) {
-            float* pl = a.out + (size_t)plane * planeSz;
-            pl[(size_t)m * a.inner + n] = v0;
-            pl[(size_t)(Nh + m) * a.inner + n] = v1;
-          }
-        } else {
-          if (m < Nh && n < a.inner) {
-            float* pl = a.out + (size_t)plane * planeSz;
-            pl[(size_t)m * a.inner + n] = v0 + v1;
-            pl[(size_t)(N - 1 - m) * a.inner + n] = v0 - v1;
+          uint2 ps[P], pd[P];
+          mx_split4<P>(s[0], s[1], s[2], s[3], ps);
+          mx_split4<P>(d[0], d[1], d[2], d[3], pd);
+          const size_t o = mx_off(R, yp, a.oKB);
+#pragma unroll
+          for (int q = 0; q < P; q++) {
+            *reinterpret_cast<uint2*>(a.outT + (size_t)q * plane + o) = ps[q];
+            *reinterpret_cast<uint2*>(a.outT + (size_t)(P + q) * plane + o) = pd[q];
           }
         }
       }
+  } else if (MODE == DST_YF) {
+    // spectral array [z][sx][sy], sy = parity * ny / 2 + n
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        const int n = ncol0 + 32 * j;
+        if (n >= Hy) continue;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          const int r = mt * 128 + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
+          if (r < a.Mvalid) a.outF[(size_t)r * a.Ny + wp * Hy + n] = acc[i][j][e];
+        }
+      }
+  } else {
+    // inverse passes: the two waves of a column half hold E (even modes) and O (odd modes) of the same outputs;
+    // they swap them through LDS, one row fragment at a time, and the even wave writes F(j) = E + O, the odd
+    // wave F(N - 1 - j) = E - O
+    float* mine = reinterpret_cast<float*>(wp ? L0 : L1) + wh * 16 * NF * 64;  // [16 NF][64 lanes] per half
+    const float* other = reinterpret_cast<const float*>(wp ? L1 : L0) + wh * 16 * NF * 64;
+    const float sg = wp ? -1.0f : 1.0f;  // F = E + O = mine + other (even wave); E - O = other - mine (odd)
+    auto exch = [&](const int i) {
+#pragma unroll
+      for (int j = 0; j < NF; j++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) mine[(j * 16 + e) * 64 + lane] = acc[i][j][e];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        const int n = ncol0 + 32 * j;
+        if (MODE == DST_YI) {
+          if (n < Hy) {
+            const size_t plane = (size_t)a.oRB * a.oKB * 512;
+            const int y = wp ? a.Ny - 1 - n : n;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+              const int r = mt * 128 + 32 * i + 8 * g + rsub;  // (z, sx) row of this pass
+              if (r >= a.Mvalid) continue;
+              const int z = r / a.Nx, sx = r % a.Nx, px = sx >= Hx, k = sx - px * Hx;
+              float f[4];
+#pragma unroll
+              for (int e = 0; e < 4; e++) f[e] = other[(j * 16 + 4 * g + e) * 64 + lane] + sg * acc[i][j][4 * g + e];
+              uint2 pc[P];
+              mx_split4<P>(f[0], f[1], f[2], f[3], pc);
+              const size_t o = mx_off(z * a.Ny + y, k, a.oKB);
+#pragma unroll
+              for (int q = 0; q < P; q++) *reinterpret_cast<uint2*>(a.outT + (size_t)(px * P + q) * plane + o) = pc[q];
+            }
+          }
+        } else {  // DST_XI: phi rows (z, y), x = n (even wave) or nx - 1 - n (odd wave)
+          if (n < Hx) {
+            const int x = wp ? a.Nx - 1 - n : n;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+              const int r = mt * 128 + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
+              if (r < a.Mvalid) a.outF[(size_t)r * a.Nx + x] = other[(j * 16 + e) * 64 + lane] + sg * acc[i][j][e];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    };
+    exch(0);
+    exch(1);
+    exch(2);
+    exch(3);
+  }
 }
 
-void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
-  dim3 blk(256);
-  if (prec == GEMM_BF16X3) {
-    if (mode == DST_XF || mode == DST_XI) {
-      dim3 grid((a.Nh + BBN - 1) / BBN, (a.rows + BBM - 1) / BBM, 1);
-      if (mode == DST_XF) hipLaunchKernelGGL((k_dst_bf16<DST_XF>), grid, blk, 0, s, a);
-      else hipLaunchKernelGGL((k_dst_bf16<DST_XI>), grid, blk, 0, s, a);
-    } else {
-      dim3 grid((a.inner + BBN - 1) / BBN, (a.Nh + BBM - 1) / BBM, a.rows);
-      if (mode == DST_YF) hipLaunchKernelGGL((k_dst_bf16<DST_YF>), grid, blk, 0, s, a);
-      else hipLaunchKernelGGL((k_dst_bf16<DST_YI>), grid, blk, 0, s, a);
-    }
-    return;
+template <int P, int NF>
+static void launch_mx_nf(hipStream_t s, int mode, const MxArgs& a, int blocks) {
+  dim3 blk(256), grid(blocks);
+  switch (mode) {
+    case DST_XF: hipLaunchKernelGGL((k_dst_mx<DST_XF, P, NF>), grid, blk, 0, s, a); break;
+    case DST_YF: hipLaunchKernelGGL((k_dst_mx<DST_YF, P, NF>), grid, blk, 0, s, a); break;
+    case DST_YI: hipLaunchKernelGGL((k_dst_mx<DST_YI, P, NF>), grid, blk, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_dst_mx<DST_XI, P, NF>), grid, blk, 0, s, a); break;
   }
+}
+
+void launch_dst_mx(hipStream_t s, int mode, int P, int NF, const MxArgs& a) {
+  const int blocks = a.tilesN * (a.aRB / 4);
+  if (P == 2) {
+    if (NF == 2) launch_mx_nf<2, 2>(s, mode, a, blocks);
+    else if (NF == 3) launch_mx_nf<2, 3>(s, mode, a, blocks);
+    else launch_mx_nf<2, 4>(s, mode, a, blocks);
+  } else {
+    if (NF == 2) launch_mx_nf<3, 2>(s, mode, a, blocks);
+    else if (NF == 3) launch_mx_nf<3, 3>(s, mode, a, blocks);
+    else launch_mx_nf<3, 4>(s, mode, a, blocks);
+  }
+}
+
+void launch_mx_fold_x(hipStream_t s, int P, const float* rhs, uint16_t* out, int Nx, int Ny, int nz, int RB, int KB) {
+  dim3 blk(256), grid((RB * KB + 3) / 4);
+  if (P == 2) hipLaunchKernelGGL(k_mx_fold_x<2>, grid, blk, 0, s, rhs, out, Nx, Ny, nz, RB, KB);
+  else hipLaunchKernelGGL(k_mx_fold_x<3>, grid, blk, 0, s, rhs, out, Nx, Ny, nz, RB, KB);
+}
+
+void launch_dst(hipStream_t s, int mode, const DstArgs& a) {
+  dim3 blk(256);
   if (mode == DST_XF || mode == DST_XI) {
     dim3 grid((a.Nh + BN - 1) / BN, (a.rows + BM - 1) / BM, 1);
     if (mode == DST_XF) hipLaunchKernelGGL((k_dst<DST_XF, 0>), grid, blk, 0, s, a);
@@ -484,6 +563,40 @@ void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a) {
 }
 
 // ---------------- Thomas ----------------
+// Mode (sx, sy) of the spectral array: its column starts at `base` (layout 0: sx + nx sy, x fastest; layout 1:
+// sy + ny sx, y fastest) and steps nx ny per level. The solution goes back into the array, or into the y
+// inverse pass's tiled split operand (row z nx + sx, parity sy >= ny / 2, k = sy mod ny / 2).
+__device__ __forceinline__ void thomas_put(const ThomasArgs& a, int sx, int sy, size_t idx, int k, float v) {
+  if (!a.outT) {
+    a.hat[idx] = v;
+    return;
+  }
+  const int Hy = a.ny >> 1, par = sy >= Hy;
+  const size_t plane = (size_t)((a.nz * a.nx + 127) / 128 * 4) * a.oKB * 512;
+  uint16_t* o = a.outT + mx_off(k * a.nx + sx, sy - par * Hy, a.oKB);
+  uint16_t pc[3];
+  if (a.P == 3) {
+    mx_split1<3>(v, pc);
+    for (int q = 0; q < 3; q++) o[(size_t)(3 * par + q) * plane] = pc[q];
+  } else {
+    mx_split1<2>(v, pc);
+    for (int q = 0; q < 2; q++) o[(size_t)(2 * par + q) * plane] = pc[q];
+  }
+}
+
+__device__ __forceinline__ void thomas_mode(const ThomasArgs& a, int& sx, int& sy, size_t& base) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y * blockDim.y + threadIdx.y;
+  if (a.layout) {
+    sy = f;
+    sx = g;
+    base = (size_t)sy + (size_t)a.ny * sx;
+  } else {
+    sx = f;
+    sy = g;
+    base = (size_t)sx + (size_t)a.nx * sy;
+  }
+}
+
 // FP64 sweep of one mode (the near-singular ones, or every mode in the FP64 form): c' and d' go
 // through global scratch, as there are few such modes.
 __device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, size_t base, size_t sxy) {
@@ -499,11 +612,11 @@ __device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, si
     a.dp64[idx] = dprev;
   }
   double above = a.dp64[base + sxy * (a.nz - 1)];
-  a.hat[base + sxy * (a.nz - 1)] = (float)above;
+  thomas_put(a, sx, sy, base + sxy * (a.nz - 1), a.nz - 1, (float)above);
   for (int k = a.nz - 2; k >= 0; k--) {
     size_t idx = base + sxy * k;
     above = a.dp64[idx] - a.cp64[idx] * above;
-    a.hat[idx] = (float)above;
+    thomas_put(a, sx, sy, idx, k, (float)above);
   }
 }
 
@@ -511,9 +624,11 @@ __device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, si
 // (all loads issued before the recurrence needs them) and written once.
 template <int NZMAX>
 __global__ void __launch_bounds__(256) k_thomas(ThomasArgs a) {
-  int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y * blockDim.y + threadIdx.y;
+  int sx, sy;
+  size_t base;
+  thomas_mode(a, sx, sy, base);
   if (sx >= a.nx || sy >= a.ny) return;
-  const size_t sxy = (size_t)a.nx * a.ny, base = (size_t)sx + (size_t)a.nx * sy;
+  const size_t sxy = (size_t)a.nx * a.ny;
   bool use64 = a.mode == 1;
   if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
   if (use64) {
@@ -543,16 +658,18 @@ __global__ void __launch_bounds__(256) k_thomas(ThomasArgs a) {
   for (int k = NZMAX - 1; k >= 0; k--) {
     if (k < nz) {
       above = (k == nz - 1) ? d[k] : d[k] - c[k] * above;
-      a.hat[base + sxy * k] = above;
+      thomas_put(a, sx, sy, base + sxy * k, k, above);
     }
   }
 }
 
 // Any nz: c' goes through global scratch.
 __global__ void __launch_bounds__(256) k_thomas_any(ThomasArgs a) {
-  int sx = blockIdx.x * blockDim.x + threadIdx.x, sy = blockIdx.y * blockDim.y + threadIdx.y;
+  int sx, sy;
+  size_t base;
+  thomas_mode(a, sx, sy, base);
   if (sx >= a.nx || sy >= a.ny) return;
-  const size_t sxy = (size_t)a.nx * a.ny, base = (size_t)sx + (size_t)a.nx * sy;
+  const size_t sxy = (size_t)a.nx * a.ny;
   bool use64 = a.mode == 1;
   if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
   if (use64) {
@@ -571,15 +688,17 @@ __global__ void __launch_bounds__(256) k_thomas_any(ThomasArgs a) {
     a.hat[idx] = dprev;
   }
   float above = a.hat[base + sxy * (a.nz - 1)];
+  thomas_put(a, sx, sy, base + sxy * (a.nz - 1), a.nz - 1, above);
   for (int k = a.nz - 2; k >= 0; k--) {
     size_t idx = base + sxy * k;
     above = a.hat[idx] - a.cp32[idx] * above;
-    a.hat[idx] = above;
+    thomas_put(a, sx, sy, idx, k, above);
   }
 }
 
 void launch_thomas(hipStream_t s, const ThomasArgs& a) {
-  dim3 blk(64, 4, 1), grid((a.nx + 63) / 64, (a.ny + 3) / 4, 1);
+  const int nf = a.layout ? a.ny : a.nx, ns = a.layout ? a.nx : a.ny;
+  dim3 blk(64, 4, 1), grid((nf + 63) / 64, (ns + 3) / 4, 1);
   if (a.nz <= 64) hipLaunchKernelGGL((k_thomas<64>), grid, blk, 0, s, a);
   else if (a.nz <= 80) hipLaunchKernelGGL((k_thomas<80>), grid, blk, 0, s, a);
   else if (a.nz <= 104) hipLaunchKernelGGL((k_thomas<104>), grid, blk, 0, s, a);

@@ -109,10 +109,23 @@ struct Timer {
 
 // ---------------------------------------------------------------- PoissonDirect
 struct AxisMats {
-  int N = 0, Nh = 0, Npad = 0, Kp = 0, NpadB = 0, KpB = 0;
+  int N = 0, Nh = 0, Npad = 0, Kp = 0;
   float* W[4] = {nullptr, nullptr, nullptr, nullptr};  // We, Wo, WeT, WoT
-  uint16_t* Wb[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // hi, lo
+  // multi-piece BF16 form: tiled split matrices [P - 2][forward, inverse], each [2 parities][P][bRB][KB] tiles;
+  // NF = column fragments per wave (64 NF columns per block and parity), tilesN column tiles
+  int NF = 0, tilesN = 0, bRB = 0, KB = 0;
+  uint16_t* Wm[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
+
+// Multi-piece BF16 operand geometry (see k_solve.hip): 32-row blocks of a pass's data operand
+static int mx_rb(int rows) { return (rows + 127) / 128 * 4; }
+static size_t mx_bytes(int P, int RB, int KB) { return (size_t)2 * P * RB * KB * 1024; }
+// bf16 offset of (row, k) in one piece plane of a tiled operand (host copy of mx_off)
+static size_t mx_off_host(int row, int k, int KB) {
+  const int ri = row & 31, kc = (k >> 3) & 1;
+  const int chunk = 2 * ri + (kc ^ (((ri >> 2) ^ (ri >> 3)) & 1));
+  return ((size_t)(row >> 5) * KB + (k >> 4)) * 512 + chunk * 8 + (k & 7);
+}
 
 // round-to-nearest-even FP32 -> BF16 bits
 static uint16_t bf16_bits(float x) {
@@ -137,6 +150,10 @@ struct obl_poisson {
   double *cp64 = nullptr, *dp64 = nullptr, *lam64 = nullptr, *zop64 = nullptr;
   float4* zop32 = nullptr;
   float *hrhs = nullptr, *hphi = nullptr;  // device staging for solve_host
+  // multi-piece BF16 chain: the tiled split operands of the XF, YF, YI and XI passes (sized for 3 pieces,
+  // zeroed once: nothing writes their padding) and their 32-row block counts
+  uint16_t* mx[4] = {nullptr, nullptr, nullptr, nullptr};
+  int mxRB[4] = {0, 0, 0, 0};
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -187,43 +204,56 @@ static int build_axis(AxisMats& A, int N, std::string* err) {
     if (hip_ok(hipMemcpy(A.W[t], h[t].data(), n * sizeof(float), hipMemcpyHostToDevice), err, "hipMemcpy(sine matrix)"))
       return -1;
   }
-  // split-BF16 copies: hi = bf16(v), lo = bf16(v - hi) from the FP64 values, padded to the
-  // 128 x 32 tiles of that kernel
-  A.NpadB = (A.Nh + 127) / 128 * 128;
-  A.KpB = (A.Nh + 31) / 32 * 32;
-  size_t nb = (size_t)A.NpadB * A.KpB;
-  std::vector<uint16_t> hb[4][2];
-  for (int t = 0; t < 4; t++)
-    for (int q = 0; q < 2; q++) hb[t][q].assign(nb, 0);
-  auto put = [&](int t, size_t idx, double val) {
-    uint16_t hi = bf16_bits((float)val);
-    uint16_t lo = bf16_bits((float)(val - (double)bf16_value(hi)));
-    hb[t][0][idx] = hi;
-    hb[t][1][idx] = lo;
-  };
-  for (int s = 0; s < A.Nh; s++)
-    for (int i = 0; i < A.Nh; i++) {
-      double ve = v(2 * s, i), vo = v(2 * s + 1, i);
-      put(0, (size_t)s * A.KpB + i, ve);
-      put(1, (size_t)s * A.KpB + i, vo);
-      put(2, (size_t)i * A.KpB + s, ve);
-      put(3, (size_t)i * A.KpB + s, vo);
+  // multi-piece copies, in the tiled layout: the block width 64 NF (NF = 2, 3, 4) that pads the N / 2 columns
+  // least (the wider one on a tie); forward rows n = s (mode), k = i; inverse rows n = i, k = s. Pieces split
+  // from the FP64 values.
+  int best = 1 << 30;
+  for (int nf = 4; nf >= 2; nf--) {
+    const int w = 64 * nf, pad = (A.Nh + w - 1) / w * w;
+    if (pad < best) {
+      best = pad;
+      A.NF = nf;
     }
-  for (int t = 0; t < 4; t++)
-    for (int q = 0; q < 2; q++) {
-      if (hip_ok(hipMalloc(&A.Wb[t][q], nb * 2), err, "hipMalloc(bf16 sine matrix)")) return -1;
-      if (hip_ok(hipMemcpy(A.Wb[t][q], hb[t][q].data(), nb * 2, hipMemcpyHostToDevice), err, "hipMemcpy(bf16 matrix)"))
+  }
+  A.tilesN = (A.Nh + 64 * A.NF - 1) / (64 * A.NF);
+  A.bRB = A.tilesN * 2 * A.NF;
+  A.KB = (A.Nh + 31) / 32 * 2;  // even: the pass kernels take k-blocks two at a time
+  const size_t plane = (size_t)A.bRB * A.KB * 512;
+  for (int P = 2; P <= 3; P++) {
+    std::vector<uint16_t> hm[2];
+    for (int inv = 0; inv < 2; inv++) hm[inv].assign(2 * P * plane, 0);
+    for (int par = 0; par < 2; par++)
+      for (int s = 0; s < A.Nh; s++)
+        for (int i = 0; i < A.Nh; i++) {
+          const double val = v(2 * s + par, i);
+          uint16_t pc[3];
+          double r = val;
+          for (int q = 0; q < P; q++) {
+            pc[q] = bf16_bits((float)r);
+            r -= (double)bf16_value(pc[q]);
+          }
+          for (int q = 0; q < P; q++) {
+            hm[0][(size_t)(par * P + q) * plane + mx_off_host(s, i, A.KB)] = pc[q];
+            hm[1][(size_t)(par * P + q) * plane + mx_off_host(i, s, A.KB)] = pc[q];
+          }
+        }
+    for (int inv = 0; inv < 2; inv++) {
+      const size_t bytes = hm[inv].size() * 2;
+      if (hip_ok(hipMalloc(&A.Wm[P - 2][inv], bytes), err, "hipMalloc(tiled sine matrix)")) return -1;
+      if (hip_ok(hipMemcpy(A.Wm[P - 2][inv], hm[inv].data(), bytes, hipMemcpyHostToDevice), err,
+                 "hipMemcpy(tiled sine matrix)"))
         return -1;
     }
+  }
   return 0;
 }
 
 static void free_axis(AxisMats& A) {
-  for (int t = 0; t < 4; t++) {
+  for (int t = 0; t < 4; t++)
     if (A.W[t]) hipFree(A.W[t]);
-    for (int q = 0; q < 2; q++)
-      if (A.Wb[t][q]) hipFree(A.Wb[t][q]);
-  }
+  for (int P = 0; P < 2; P++)
+    for (int inv = 0; inv < 2; inv++)
+      if (A.Wm[P][inv]) hipFree(A.Wm[P][inv]);
 }
 
 static DstArgs dst_args(const float* in, float* out, const AxisMats& M, int inv, int rows, int inner) {
@@ -237,11 +267,6 @@ static DstArgs dst_args(const float* in, float* out, const AxisMats& M, int inv,
   a.rows = rows;
   a.inner = inner;
   a.Kp = M.Kp;
-  a.W0h = M.Wb[inv ? 2 : 0][0];
-  a.W0l = M.Wb[inv ? 2 : 0][1];
-  a.W1h = M.Wb[inv ? 3 : 1][0];
-  a.W1l = M.Wb[inv ? 3 : 1][1];
-  a.KpB = M.KpB;
   return a;
 }
 
@@ -280,7 +305,8 @@ extern "C" void obl_poisson_destroy(obl_poisson* p) {
   if (p->stream) hipStreamSynchronize(p->stream);
   free_axis(p->ax);
   free_axis(p->ay);
-  void* bufs[] = {p->tmp, p->hat, p->cp32, p->lam32, p->cp64, p->dp64, p->lam64, p->zop64, p->zop32, p->hrhs, p->hphi};
+  void* bufs[] = {p->tmp,   p->hat,   p->cp32,  p->lam32, p->cp64,  p->dp64,  p->lam64, p->zop64,
+                  p->zop32, p->hrhs,  p->hphi,  p->mx[0], p->mx[1], p->mx[2], p->mx[3]};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (p->stream) hipStreamDestroy(p->stream);
@@ -313,6 +339,19 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
     obl_poisson_destroy(p);
     return set_global("PoissonDirect(%d, %d, %d): %s", nx, ny, nz, e.c_str());
   }
+  // multi-piece operands: XF rows are 128-row tiles of 64 (z, y) pairs; YF and YI rows (z, sx); XI rows (z, y)
+  p->mxRB[0] = 4 * ((nz * (ny / 2) + 63) / 64);
+  p->mxRB[1] = p->mxRB[2] = mx_rb(nz * nx);
+  p->mxRB[3] = mx_rb(nz * ny);
+  const int mxKB[4] = {p->ax.KB, p->ay.KB, p->ay.KB, p->ax.KB};
+  for (int b = 0; b < 4; b++) {
+    const size_t bytes = mx_bytes(3, p->mxRB[b], mxKB[b]);
+    if (hip_ok(hipMalloc(&p->mx[b], bytes), &e, "hipMalloc(split operand)") ||
+        hip_ok(hipMemset(p->mx[b], 0, bytes), &e, "hipMemset(split operand)")) {
+      obl_poisson_destroy(p);
+      return set_global("PoissonDirect(%d, %d, %d): %s", nx, ny, nz, e.c_str());
+    }
+  }
   std::vector<double> l64(nx + ny);
   perm_eig(nx, dx, l64.data());
   perm_eig(ny, dx, l64.data() + nx);
@@ -332,29 +371,62 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
 
 static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, int thomas, double cond,
                        hipStream_t s) {
-  if (gemm != OBL_GEMM_FP32 && gemm != OBL_GEMM_BF16X3) return p->fail("unknown gemm mode %d", gemm);
+  if (gemm != OBL_GEMM_FP32 && gemm != OBL_GEMM_BF16X3 && gemm != OBL_GEMM_BF16X6)
+    return p->fail("unknown gemm mode %d", gemm);
   if (thomas < 0 || thomas > 2) return p->fail("unknown Thomas mode %d", thomas);
   int nx = p->nx, ny = p->ny, nz = p->nz;
+  if (gemm == OBL_GEMM_BF16X3 || gemm == OBL_GEMM_BF16X6) {
+    const int P = gemm == OBL_GEMM_BF16X6 ? 3 : 2;
+    const AxisMats &X = p->ax, &Y = p->ay;
+    TIMED(p->timer, s, "ps.fold", launch_mx_fold_x(s, P, rhs, p->mx[0], nx, ny, nz, p->mxRB[0], X.KB));
+    MxArgs m;
+    m.Nx = nx; m.Ny = ny; m.nz = nz;
+    // forward x: rows (z, y) pairs; the epilogue folds y into the YF operand
+    m.A = p->mx[0]; m.B = X.Wm[P - 2][0]; m.aRB = p->mxRB[0]; m.bRB = X.bRB; m.KB = X.KB;
+    m.Mvalid = nz * (ny / 2); m.tilesN = X.tilesN; m.outT = p->mx[1]; m.oRB = p->mxRB[1]; m.oKB = Y.KB; m.outF = nullptr;
+    TIMED(p->timer, s, "ps.xf", launch_dst_mx(s, DST_XF, P, X.NF, m));
+    // forward y: rows (z, sx) -> spectral array [z][sx][sy]
+    m.A = p->mx[1]; m.B = Y.Wm[P - 2][0]; m.aRB = p->mxRB[1]; m.bRB = Y.bRB; m.KB = Y.KB;
+    m.Mvalid = nz * nx; m.tilesN = Y.tilesN; m.outT = nullptr; m.outF = p->hat;
+    TIMED(p->timer, s, "ps.yf", launch_dst_mx(s, DST_YF, P, Y.NF, m));
+    ThomasArgs t;
+    t.hat = p->hat; t.cp32 = p->cp32; t.cp64 = p->cp64; t.dp64 = p->dp64;
+    t.lam32 = p->lam32; t.lam64 = p->lam64; t.zop32 = p->zop32; t.zop64 = p->zop64;
+    t.nx = nx; t.ny = ny; t.nz = nz; t.mode = thomas;
+    t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
+    t.layout = 1; t.outT = p->mx[2]; t.P = P; t.oKB = Y.KB;
+    TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
+    // inverse y: rows (z, sx); the epilogue unfolds y into the XI operand, rows (z, y)
+    m.A = p->mx[2]; m.B = Y.Wm[P - 2][1]; m.aRB = p->mxRB[2]; m.bRB = Y.bRB; m.KB = Y.KB;
+    m.Mvalid = nz * nx; m.tilesN = Y.tilesN; m.outT = p->mx[3]; m.oRB = p->mxRB[3]; m.oKB = X.KB; m.outF = nullptr;
+    TIMED(p->timer, s, "ps.yi", launch_dst_mx(s, DST_YI, P, Y.NF, m));
+    // inverse x -> phi
+    m.A = p->mx[3]; m.B = X.Wm[P - 2][1]; m.aRB = p->mxRB[3]; m.bRB = X.bRB; m.KB = X.KB;
+    m.Mvalid = nz * ny; m.tilesN = X.tilesN; m.outT = nullptr; m.outF = phi;
+    TIMED(p->timer, s, "ps.xi", launch_dst_mx(s, DST_XI, P, X.NF, m));
+    return hip_ok(hipGetLastError(), &p->err, "PoissonDirect.enc launch");
+  }
   DstArgs a;
   // forward x: rhs -> tmp
   a = dst_args(rhs, p->tmp, p->ax, 0, ny * nz, nx);
-  TIMED(p->timer, s, "ps.xf", launch_dst(s, DST_XF, gemm, a));
+  TIMED(p->timer, s, "ps.xf", launch_dst(s, DST_XF, a));
   // forward y: tmp -> hat
   a = dst_args(p->tmp, p->hat, p->ay, 0, nz, nx);
-  TIMED(p->timer, s, "ps.yf", launch_dst(s, DST_YF, gemm, a));
+  TIMED(p->timer, s, "ps.yf", launch_dst(s, DST_YF, a));
   ThomasArgs t;
   t.hat = p->hat; t.cp32 = p->cp32; t.cp64 = p->cp64; t.dp64 = p->dp64;
   t.lam32 = p->lam32; t.lam64 = p->lam64; t.zop32 = p->zop32; t.zop64 = p->zop64;
   t.nx = nx; t.ny = ny; t.nz = nz; t.mode = thomas;
   // FP64 where the estimated condition 4 / (|shift| dz^2) exceeds `cond`
   t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
+  t.layout = 0; t.outT = nullptr; t.P = 0; t.oKB = 0;
   TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
   // inverse y: hat -> tmp
   a = dst_args(p->hat, p->tmp, p->ay, 1, nz, nx);
-  TIMED(p->timer, s, "ps.yi", launch_dst(s, DST_YI, gemm, a));
+  TIMED(p->timer, s, "ps.yi", launch_dst(s, DST_YI, a));
   // inverse x: tmp -> phi
   a = dst_args(p->tmp, phi, p->ax, 1, ny * nz, nx);
-  TIMED(p->timer, s, "ps.xi", launch_dst(s, DST_XI, gemm, a));
+  TIMED(p->timer, s, "ps.xi", launch_dst(s, DST_XI, a));
   return hip_ok(hipGetLastError(), &p->err, "PoissonDirect.enc launch");
 }
 
@@ -1211,7 +1283,8 @@ extern "C" int obl_sync(obl_model* h) {
 
 extern "C" int obl_set_solver_mode(obl_model* h, int gemm_mode, int thomas_mode, double fp64_cond) {
   if (!h) return set_global("null handle");
-  if (gemm_mode != OBL_GEMM_FP32 && gemm_mode != OBL_GEMM_BF16X3) return h->fail("unknown gemm mode %d", gemm_mode);
+  if (gemm_mode != OBL_GEMM_FP32 && gemm_mode != OBL_GEMM_BF16X3 && gemm_mode != OBL_GEMM_BF16X6)
+    return h->fail("unknown gemm mode %d", gemm_mode);
   if (thomas_mode < 0 || thomas_mode > 2) return h->fail("unknown Thomas mode %d", thomas_mode);
   h->gemm_mode = gemm_mode;
   h->thomas_mode = thomas_mode;

@@ -122,16 +122,26 @@ struct DstArgs {
   int rows;         // X passes: rows (nyp * nz); Y passes: planes (nz)
   int inner;        // Y passes: row length (nxp); X passes: N
   int Kp;           // row stride of W0/W1
-  // split-BF16 form: hi/lo halves of the two matrices, [NpadB][KpB] bf16 (zero-padded)
-  const uint16_t* W0h;
-  const uint16_t* W0l;
-  const uint16_t* W1h;
-  const uint16_t* W1l;
-  int KpB;
 };
 enum { DST_XF = 0, DST_XI = 1, DST_YF = 2, DST_YI = 3 };
-enum { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };
-void launch_dst(hipStream_t s, int mode, int prec, const DstArgs& a);
+void launch_dst(hipStream_t s, int mode, const DstArgs& a);
+
+// Multi-piece BF16 passes (operands pre-split in the tiled layout of k_solve.hip). Pass chain:
+//   fold_x: rhs -> A_xf;  XF: A_xf -> A_yf;  YF: A_yf -> spectral [z][sx][sy];  Thomas -> A_yi;
+//   YI: A_yi -> A_xi;  XI: A_xi -> phi
+struct MxArgs {
+  const uint16_t* A;  // data operand, [2 parities][P pieces][aRB][KB] tiles
+  const uint16_t* B;  // sine matrices, [2][P][bRB][KB] tiles
+  int aRB, bRB, KB;   // 32-row blocks of A and of B per piece plane; 16-wide k-blocks
+  int Mvalid;         // rows of A that carry data (XF: (z, y) pairs)
+  int tilesN;         // column tiles (64 NF wide) per parity
+  int Nx, Ny, nz;     // padded grid
+  uint16_t* outT;     // tiled output (XF, YI)
+  int oRB, oKB;
+  float* outF;        // FP32 output (YF: spectral array; XI: phi)
+};
+void launch_dst_mx(hipStream_t s, int mode, int P, int NF, const MxArgs& a);
+void launch_mx_fold_x(hipStream_t s, int P, const float* rhs, uint16_t* out, int Nx, int Ny, int nz, int RB, int KB);
 
 struct ThomasArgs {
   float* hat;
@@ -145,6 +155,9 @@ struct ThomasArgs {
   int nx, ny, nz;
   int mode;             // 0 FP32 (reference order), 1 FP64 all, 2 FP64 where near-singular
   double shift_lo;      // mode 2: FP64 when |shift| < shift_lo
+  int layout;           // 0: hat[z][sy][sx]; 1: hat[z][sx][sy] (the multi-piece BF16 chain)
+  uint16_t* outT;       // non-null: the solution goes here as the y inverse pass's tiled split operand
+  int P, oKB;           //   its pieces and k-blocks (rows z nx + sx, parity sy >= ny / 2, k = sy mod ny / 2)
 };
 enum { THOMAS_FP32 = 0, THOMAS_FP64 = 1, THOMAS_MIXED = 2 };
 void launch_thomas(hipStream_t s, const ThomasArgs& a);

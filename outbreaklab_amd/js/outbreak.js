@@ -23,7 +23,7 @@ const DEG = Math.PI / 180;
 const clamp = (v, a, b) => Math.min(Math.max(v, a), b);
 const dflt = (v, d) => (v === undefined || v === null ? d : v);
 const FIELD = { V: 0, Q1: 1, Q2: 2, Vt: 3, Q1t: 4, Q2t: 5, aux: 6, phT: 7, lsfA: 8, lsfB: 9, ciMap: 10, phi: 11, rhs: 12, bs: 13, sfc: 14, Vp: 15 };
-const GEMM = { fp32: 0, bf16x3: 1 };
+const GEMM = { fp32: 0, bf16x3: 1, bf16x6: 2 };
 const THOMAS = { fp32: 0, fp64: 1, mixed: 2 };
 const MAX_BOUNDS = 8, MAX_BPTS = 32, MAX_CI = 256;
 const CN_OLD = 0.4;  // hurricane mode: the share of the last step's pressure force applied before advection

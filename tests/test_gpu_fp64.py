@@ -67,11 +67,18 @@ def fp64_errors(m, dt):
     return out
 
 
-def test_step_error_against_fp64(developed):
+@pytest.mark.parametrize("gemm", ["fp32", "bf16x6"])
+def test_step_error_against_fp64(developed, gemm):
+    """The plain step with the FP32-MFMA transforms, and with the three-piece BF16 transforms (FP32 accuracy),
+    held to the same bound."""
     m, dt, scheme = developed
-    e = fp64_errors(m, dt)
+    m.set_solver_mode(gemm, "mixed")
+    try:
+        e = fp64_errors(m, dt)
+    finally:
+        m.set_solver_mode("fp32", "mixed")
     for k, v in e.items():
-        print(f"{scheme:9s} {k:12s} " + " ".join(f"{x:.3e}" for x in v), flush=True)
+        print(f"{scheme:9s} {gemm:6s} {k:12s} " + " ".join(f"{x:.3e}" for x in v), flush=True)
     bad = {}
     for k, v in e.items():
         if k == "divergence":
