@@ -241,21 +241,23 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 //
 // The operands are stored already split, in a tiled layout: an [R][K] operand (K the reduction index) is cut
 // into 32-row x 16-k tiles, 1 KB per piece; tile (rb, kb) of piece q and parity p is the 1 KB block number
-// ((p P + q) RB + rb) KB + kb, and inside a tile the 16-byte chunk c holds row c / 2, k-half (c & 1) ^ s(row),
-// s(r) = bit 2 of r xor bit 3 of r. A tile is then one contiguous 1 KB that one wave instruction copies into
-// LDS (global_load_lds_dwordx4: lane l fills bytes 16 l .. 16 l + 15), and the fragment reads of the MFMA (lane
-// l: row l % 32, k-half l / 32) fall in distinct banks. Each operand's producer writes it in this layout: the
-// fold-and-split pass for the x forward pass, the epilogue of the pass before, or the Thomas sweep.
+// ((p P + q) RB + rb) (KB + 1) + kb, and inside a tile the 16-byte chunk c holds row c / 2, k-half (c & 1) ^ s(row),
+// s(r) = bit 2 of r xor bit 3 of r (a swizzle that keeps LDS reads of the layout free of bank conflicts). Each
+// operand's producer writes it in this layout: the fold-and-split pass for the x forward pass, the epilogue of
+// the pass before, or the Thomas sweep.
 //
-// Block: 4 waves, 128 rows x 64 NF columns of each parity; waves 0-1 compute the even half, waves 2-3 the odd
-// half, each 128 x 32 NF (4 x NF fragments of 32 x 32). K advances 16 per stage through two LDS buffers: the
-// copies of stage s + 1 are in flight while the MFMAs of stage s run.
+// Block: 4 waves, 32 MF rows (MF = MX_MF) x 64 NF columns of each parity; waves 0-1 compute the even half, waves
+// 2-3 the odd half, each 32 MF x 32 NF (MF x NF fragments of 32 x 32), K advancing 16 per step. With MF = 4 a
+// wave holds 192 accumulators and a CU runs one block (measured: 64-row blocks two per CU are 1.5 times slower,
+// the sine-matrix traffic doubling).
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int mx_chunk(int ri, int kc) { return 2 * ri + (kc ^ (((ri >> 2) ^ (ri >> 3)) & 1)); }
-// bf16 offset of element (row, k) inside one piece plane of a tiled operand with KB k-blocks
+// bf16 offset of element (row, k) inside one piece plane of a tiled operand with KB k-blocks; a row-block spans
+// KB + 1 tiles (an odd pitch: with power-of-two strides, as at N = 1024, the tiles of a k-block alias in the
+// memory channels)
 __device__ __forceinline__ size_t mx_off(int row, int k, int KB) {
-  return ((size_t)(row >> 5) * KB + (k >> 4)) * 512 + mx_chunk(row & 31, (k >> 3) & 1) * 8 + (k & 7);
+  return ((size_t)(row >> 5) * (KB + 1) + (k >> 4)) * 512 + mx_chunk(row & 31, (k >> 3) & 1) * 8 + (k & 7);
 }
 
 // P pieces of 4 floats, 4 bf16 (8 bytes) per piece; the hardware conversion rounds to nearest even
@@ -284,9 +286,9 @@ __device__ __forceinline__ void mx_split1(float r, uint16_t* out) {
   }
 }
 
-// Fold-and-split of the right-hand side into the x forward pass's operand. Its rows come in 128-row tiles of 64
-// (z, y) pairs: row 64 h + q of tile t is y' (h = 0) or ny - 1 - y' (h = 1) of pair 64 t + q = z (ny / 2) + y',
-// so the pass's epilogue finds both rows of a pair in one lane and can fold y as well. k runs over x' < nx / 2:
+// Fold-and-split of the right-hand side into the x forward pass's operand. Its rows come in tiles of 32 MF rows
+// holding 16 MF (z, y) pairs (MF = MX_MF): row 16 MF h + q of tile t is y' (h = 0) or ny - 1 - y' (h = 1) of pair
+// 16 MF t + q = z (ny / 2) + y', so the pass's epilogue finds both rows of a pair in one lane and can fold y too. k runs over x' < nx / 2:
 // parity 0 holds F(x') + F(nx - 1 - x'), parity 1 F(x') - F(nx - 1 - x'). One wave writes one tile of every piece
 // and parity (lane l: chunk l).
 template <int P>
@@ -296,15 +298,15 @@ __global__ void __launch_bounds__(256) k_mx_fold_x(const float* __restrict__ rhs
   if (wave >= RB * KB) return;
   const int rb = wave / KB, kb = wave % KB;
   const int ri = lane >> 1, kc = (lane & 1) ^ (((ri >> 2) ^ (ri >> 3)) & 1);
-  const int r = rb * 32 + ri, t = r >> 7, rho = r & 127;
+  const int r = rb * 32 + ri, t = r / (32 * MX_MF), rho = r % (32 * MX_MF);
   const int Hx = Nx >> 1, Hy = Ny >> 1;
-  const int pair = 64 * t + (rho & 63);
+  const int pair = 16 * MX_MF * t + rho % (16 * MX_MF);
   const int k0 = kb * 16 + kc * 8;
   float s[8], d[8];
 #pragma unroll
   for (int e = 0; e < 8; e++) s[e] = d[e] = 0.0f;
   if (pair < nz * Hy) {
-    const int z = pair / Hy, yp = pair % Hy, y = (rho >> 6) ? Ny - 1 - yp : yp;
+    const int z = pair / Hy, yp = pair % Hy, y = rho >= 16 * MX_MF ? Ny - 1 - yp : yp;
     const float* row = rhs + ((size_t)z * Ny + y) * Nx;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
@@ -324,7 +326,7 @@ __global__ void __launch_bounds__(256) k_mx_fold_x(const float* __restrict__ rhs
   mx_split4<P>(s[4], s[5], s[6], s[7], ps[1]);
   mx_split4<P>(d[0], d[1], d[2], d[3], pd[0]);
   mx_split4<P>(d[4], d[5], d[6], d[7], pd[1]);
-  const size_t plane = (size_t)RB * KB * 512, tile = ((size_t)rb * KB + kb) * 512 + lane * 8;
+  const size_t plane = (size_t)RB * (KB + 1) * 512, tile = ((size_t)rb * (KB + 1) + kb) * 512 + lane * 8;
 #pragma unroll
   for (int q = 0; q < P; q++) {
     *reinterpret_cast<uint4*>(out + (size_t)q * plane + tile) = make_uint4(ps[0][q].x, ps[0][q].y, ps[1][q].x, ps[1][q].y);
@@ -333,83 +335,85 @@ __global__ void __launch_bounds__(256) k_mx_fold_x(const float* __restrict__ rhs
   }
 }
 
+template <int V>
+struct IntC {
+  static constexpr int value = V;
+};
+
 template <int MODE, int P, int NF>
-__global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
-  constexpr int TB = 4 + 2 * NF;           // tiles per (parity, piece) and stage: 4 of A, 2 NF of B
-  constexpr int STAGE = 2 * P * TB * 512;  // bf16 per stage
+__global__ void __launch_bounds__(256, MX_MF == 2 ? 2 : 1) k_dst_mx(MxArgs a) {
+  constexpr int MF = MX_MF;
   constexpr int NPROD = P == 2 ? 3 : 6;
   // products (piece of A, piece of B), smallest first
   constexpr int PA[6] = {P == 2 ? 1 : 2, P == 2 ? 0 : 1, 0, 1, 0, 0};
   constexpr int PB[6] = {0, P == 2 ? 1 : 1, P == 2 ? 0 : 2, 0, 1, 0};
-  __shared__ __attribute__((aligned(1024))) uint16_t L0[STAGE];
-  __shared__ __attribute__((aligned(1024))) uint16_t L1[STAGE];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wp = wave >> 1, wh = wave & 1;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int nt = t % a.tilesN, mt = t / a.tilesN;
 
-  // this wave's tile copies, for every (parity, piece): A row-block `wave` and B row-blocks wave + 4 m
-  const size_t planeA = (size_t)a.aRB * a.KB * 512, planeB = (size_t)a.bRB * a.KB * 512;
-  const uint16_t* srcA = a.A + ((size_t)(mt * 4 + wave) * a.KB) * 512 + lane * 8;
-  const uint16_t* srcB = a.B + ((size_t)(nt * 2 * NF + wave) * a.KB) * 512 + lane * 8;
-  auto copy = [&](uint16_t* buf, int kb) {
+  // Each lane's MFMA fragment of a tile is one 16-byte chunk (row l % 32, k-half l / 32), and the 64 lanes of a
+  // wave read the 64 chunks of the tile: every fragment load is one coalesced 1 KB wave load. So each wave loads
+  // its own fragments from global memory into registers (the two waves of a parity read the same A fragments,
+  // the second from L1), double-buffered one k-block ahead; no LDS and no barrier in the k loop.
+  const int loff = mx_chunk(lane & 31, lane >> 5) * 8;
+  const int KBP = a.KB + 1;  // row-block pitch in tiles
+  const size_t planeA = (size_t)a.aRB * KBP * 512, planeB = (size_t)a.bRB * KBP * 512;
+  const uint16_t* pA = a.A + (size_t)(wp * P) * planeA + (size_t)(mt * MF) * KBP * 512 + loff;
+  const uint16_t* pB = a.B + (size_t)(wp * P) * planeB + (size_t)(nt * 2 * NF + wh * NF) * KBP * 512 + loff;
+  bf16x8 fa[2][MF][P], fb[2][NF][P];
+  auto load = [&](auto S, int kb) __attribute__((always_inline)) {
+    constexpr int s = decltype(S)::value;
 #pragma unroll
-    for (int pq = 0; pq < 2 * P; pq++) {
-      __builtin_amdgcn_global_load_lds((const void*)(srcA + pq * planeA + (size_t)kb * 512),
-                                       (__attribute__((address_space(3))) void*)(buf + (pq * TB + wave) * 512), 16, 0,
-                                       0);
+    for (int q = 0; q < P; q++)
 #pragma unroll
-      for (int m = 0; m < (2 * NF + 3) / 4; m++)
-        if (wave + 4 * m < 2 * NF)
-          __builtin_amdgcn_global_load_lds(
-              (const void*)(srcB + pq * planeB + (size_t)(4 * m * a.KB + kb) * 512),
-              (__attribute__((address_space(3))) void*)(buf + (pq * TB + 4 + wave + 4 * m) * 512), 16, 0, 0);
-    }
+      for (int i = 0; i < MF; i++)
+        fa[s][i][q] = *reinterpret_cast<const bf16x8*>(pA + q * planeA + (size_t)(i * KBP + kb) * 512);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+#pragma unroll
+      for (int j = 0; j < NF; j++)
+        fb[s][j][q] = *reinterpret_cast<const bf16x8*>(pB + q * planeB + (size_t)(j * KBP + kb) * 512);
   };
 
-  v16f acc[4][NF];
+  v16f acc[MF][NF];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < MF; i++)
 #pragma unroll
     for (int j = 0; j < NF; j++)
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 
-  const int loff = mx_chunk(lane & 31, lane >> 5) * 8;
-  auto compute = [&](const uint16_t* buf) {
-    const uint16_t* Lp = buf + wp * P * TB * 512 + loff;
-    bf16x8 af[4][P];
+  // product-major order: MF NF independent accumulators between two updates of one
+  auto compute = [&](auto S) __attribute__((always_inline)) {
+    constexpr int s = decltype(S)::value;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int c = 0; c < NPROD; c++)
 #pragma unroll
-      for (int q = 0; q < P; q++) af[i][q] = *reinterpret_cast<const bf16x8*>(Lp + (q * TB + i) * 512);
+      for (int j = 0; j < NF; j++)
 #pragma unroll
-    for (int j = 0; j < NF; j++) {
-      bf16x8 bfr[P];
-#pragma unroll
-      for (int q = 0; q < P; q++) bfr[q] = *reinterpret_cast<const bf16x8*>(Lp + (q * TB + 4 + wh * NF + j) * 512);
-#pragma unroll
-      for (int c = 0; c < NPROD; c++)
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][PA[c]], bfr[PB[c]], acc[i][j], 0, 0, 0);
-    }
+        for (int i = 0; i < MF; i++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[s][i][PA[c]], fb[s][j][PB[c]], acc[i][j], 0, 0, 0);
   };
 
-  // KB is even (the operands carry a zero k-block when N / 2 needs an odd number)
-  copy(L0, 0);
+  // KB is even (the operands carry a zero k-block when N / 2 needs an odd number). The loads run unconditionally
+  // (the last stage reloads the last k-block, unused): with a branch around them the wait before each compute
+  // would have to cover the path without them. The scheduling barriers keep the loads ahead of the MFMAs.
+  constexpr IntC<0> S0;
+  constexpr IntC<1> S1;
+  load(S0, 0);
   for (int kb = 0; kb < a.KB; kb += 2) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    copy(L1, kb + 1);
-    compute(L0);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (kb + 2 < a.KB) copy(L0, kb + 2);
-    compute(L1);
+    __builtin_amdgcn_sched_barrier(0);
+    load(S1, kb + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(S0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(S0, min(kb + 2, a.KB - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    compute(S1);
   }
-  __syncthreads();  // LDS free for the epilogue
+  __shared__ __attribute__((aligned(16))) float X[4 * 16 * NF * 64];  // the inverse passes' E / O exchange
 
   // ---- epilogue: lane l, fragment (i, j), element e sits at row 32 i + 8 (e / 4) + 4 (l / 32) + e % 4 of the
   // block tile and column 32 j + l % 32 of the wave's columns ----
@@ -417,25 +421,25 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
   const int ncol0 = nt * 64 * NF + wh * 32 * NF + (lane & 31);
   const int rsub = 4 * (lane >> 5);
   if (MODE == DST_XF) {
-    // fold y: rows 32 i + r and 64 + 32 i + r of the tile are y' and ny - 1 - y' of one pair; write the next
+    // fold y: rows 32 i + r and 16 MF + 32 i + r of the tile are y' and ny - 1 - y' of one pair; write the next
     // pass's operand: row z nx + sx, k = y', parity 0 the sum, 1 the difference
-    const size_t plane = (size_t)a.oRB * a.oKB * 512;
+    const size_t plane = (size_t)a.oRB * (a.oKB + 1) * 512;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MF / 2; i++)
 #pragma unroll
       for (int j = 0; j < NF; j++) {
         const int n = ncol0 + 32 * j;
         if (n >= Hx) continue;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-          const int pair = mt * 64 + 32 * i + 8 * g + rsub;
+          const int pair = mt * 16 * MF + 32 * i + 8 * g + rsub;
           if (pair >= a.Mvalid) continue;
           const int z = pair / Hy, yp = pair % Hy;
           const int R = z * a.Nx + wp * Hx + n;
           float s[4], d[4];
 #pragma unroll
           for (int e = 0; e < 4; e++) {
-            const float lo = acc[i][j][4 * g + e], hi = acc[i + 2][j][4 * g + e];
+            const float lo = acc[i][j][4 * g + e], hi = acc[i + MF / 2][j][4 * g + e];
             s[e] = lo + hi;
             d[e] = lo - hi;
           }
@@ -453,14 +457,14 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
   } else if (MODE == DST_YF) {
     // spectral array [z][sx][sy], sy = parity * ny / 2 + n
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MF; i++)
 #pragma unroll
       for (int j = 0; j < NF; j++) {
         const int n = ncol0 + 32 * j;
         if (n >= Hy) continue;
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-          const int r = mt * 128 + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
+          const int r = mt * 32 * MF + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
           if (r < a.Mvalid) a.outF[(size_t)r * a.Ny + wp * Hy + n] = acc[i][j][e];
         }
       }
@@ -468,8 +472,8 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
     // inverse passes: the two waves of a column half hold E (even modes) and O (odd modes) of the same outputs;
     // they swap them through LDS, one row fragment at a time, and the even wave writes F(j) = E + O, the odd
     // wave F(N - 1 - j) = E - O
-    float* mine = reinterpret_cast<float*>(wp ? L0 : L1) + wh * 16 * NF * 64;  // [16 NF][64 lanes] per half
-    const float* other = reinterpret_cast<const float*>(wp ? L1 : L0) + wh * 16 * NF * 64;
+    float* mine = X + wave * 16 * NF * 64;  // [16 NF][64 lanes] per wave
+    const float* other = X + (wave ^ 2) * 16 * NF * 64;
     const float sg = wp ? -1.0f : 1.0f;  // F = E + O = mine + other (even wave); E - O = other - mine (odd)
     auto exch = [&](const int i) {
 #pragma unroll
@@ -482,11 +486,11 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
         const int n = ncol0 + 32 * j;
         if (MODE == DST_YI) {
           if (n < Hy) {
-            const size_t plane = (size_t)a.oRB * a.oKB * 512;
+            const size_t plane = (size_t)a.oRB * (a.oKB + 1) * 512;
             const int y = wp ? a.Ny - 1 - n : n;
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-              const int r = mt * 128 + 32 * i + 8 * g + rsub;  // (z, sx) row of this pass
+              const int r = mt * 32 * MF + 32 * i + 8 * g + rsub;  // (z, sx) row of this pass
               if (r >= a.Mvalid) continue;
               const int z = r / a.Nx, sx = r % a.Nx, px = sx >= Hx, k = sx - px * Hx;
               float f[4];
@@ -504,7 +508,7 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
             const int x = wp ? a.Nx - 1 - n : n;
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-              const int r = mt * 128 + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
+              const int r = mt * 32 * MF + 32 * i + 8 * (e >> 2) + rsub + (e & 3);
               if (r < a.Mvalid) a.outF[(size_t)r * a.Nx + x] = other[(j * 16 + e) * 64 + lane] + sg * acc[i][j][e];
             }
           }
@@ -514,8 +518,10 @@ __global__ void __launch_bounds__(256, 1) k_dst_mx(MxArgs a) {
     };
     exch(0);
     exch(1);
-    exch(2);
-    exch(3);
+    if (MF > 2) {
+      exch(2);
+      exch(3);
+    }
   }
 }
 
@@ -531,7 +537,7 @@ static void launch_mx_nf(hipStream_t s, int mode, const MxArgs& a, int blocks) {
 }
 
 void launch_dst_mx(hipStream_t s, int mode, int P, int NF, const MxArgs& a) {
-  const int blocks = a.tilesN * (a.aRB / 4);
+  const int blocks = a.tilesN * (a.aRB / MX_MF);
   if (P == 2) {
     if (NF == 2) launch_mx_nf<2, 2>(s, mode, a, blocks);
     else if (NF == 3) launch_mx_nf<2, 3>(s, mode, a, blocks);
@@ -572,7 +578,7 @@ __device__ __forceinline__ void thomas_put(const ThomasArgs& a, int sx, int sy, 
     return;
   }
   const int Hy = a.ny >> 1, par = sy >= Hy;
-  const size_t plane = (size_t)((a.nz * a.nx + 127) / 128 * 4) * a.oKB * 512;
+  const size_t plane = (size_t)((a.nz * a.nx + 32 * MX_MF - 1) / (32 * MX_MF) * MX_MF) * (a.oKB + 1) * 512;
   uint16_t* o = a.outT + mx_off(k * a.nx + sx, sy - par * Hy, a.oKB);
   uint16_t pc[3];
   if (a.P == 3) {

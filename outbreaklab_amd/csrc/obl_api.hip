@@ -118,13 +118,13 @@ struct AxisMats {
 };
 
 // Multi-piece BF16 operand geometry (see k_solve.hip): 32-row blocks of a pass's data operand
-static int mx_rb(int rows) { return (rows + 127) / 128 * 4; }
-static size_t mx_bytes(int P, int RB, int KB) { return (size_t)2 * P * RB * KB * 1024; }
+static int mx_rb(int rows) { return (rows + 32 * MX_MF - 1) / (32 * MX_MF) * MX_MF; }
+static size_t mx_bytes(int P, int RB, int KB) { return (size_t)2 * P * RB * (KB + 1) * 1024; }
 // bf16 offset of (row, k) in one piece plane of a tiled operand (host copy of mx_off)
 static size_t mx_off_host(int row, int k, int KB) {
   const int ri = row & 31, kc = (k >> 3) & 1;
   const int chunk = 2 * ri + (kc ^ (((ri >> 2) ^ (ri >> 3)) & 1));
-  return ((size_t)(row >> 5) * KB + (k >> 4)) * 512 + chunk * 8 + (k & 7);
+  return ((size_t)(row >> 5) * (KB + 1) + (k >> 4)) * 512 + chunk * 8 + (k & 7);
 }
 
 // round-to-nearest-even FP32 -> BF16 bits
@@ -204,9 +204,9 @@ static int build_axis(AxisMats& A, int N, std::string* err) {
     if (hip_ok(hipMemcpy(A.W[t], h[t].data(), n * sizeof(float), hipMemcpyHostToDevice), err, "hipMemcpy(sine matrix)"))
       return -1;
   }
-  // multi-piece copies, in the tiled layout: the block width 64 NF (NF = 2, 3, 4) that pads the N / 2 columns
-  // least (the wider one on a tie); forward rows n = s (mode), k = i; inverse rows n = i, k = s. Pieces split
-  // from the FP64 values.
+  // multi-piece copies, in the tiled layout: the block width 64 NF (NF = 2, 3 or 4) that pads the N / 2 columns
+  // least, the wider one on a tie (fewer fragment loads per MFMA: with NF = 2 the loads saturate the CU's load
+  // path); forward rows n = s (mode), k = i; inverse rows n = i, k = s. Pieces split from the FP64 values.
   int best = 1 << 30;
   for (int nf = 4; nf >= 2; nf--) {
     const int w = 64 * nf, pad = (A.Nh + w - 1) / w * w;
@@ -218,7 +218,7 @@ static int build_axis(AxisMats& A, int N, std::string* err) {
   A.tilesN = (A.Nh + 64 * A.NF - 1) / (64 * A.NF);
   A.bRB = A.tilesN * 2 * A.NF;
   A.KB = (A.Nh + 31) / 32 * 2;  // even: the pass kernels take k-blocks two at a time
-  const size_t plane = (size_t)A.bRB * A.KB * 512;
+  const size_t plane = (size_t)A.bRB * (A.KB + 1) * 512;
   for (int P = 2; P <= 3; P++) {
     std::vector<uint16_t> hm[2];
     for (int inv = 0; inv < 2; inv++) hm[inv].assign(2 * P * plane, 0);
@@ -339,8 +339,9 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
     obl_poisson_destroy(p);
     return set_global("PoissonDirect(%d, %d, %d): %s", nx, ny, nz, e.c_str());
   }
-  // multi-piece operands: XF rows are 128-row tiles of 64 (z, y) pairs; YF and YI rows (z, sx); XI rows (z, y)
-  p->mxRB[0] = 4 * ((nz * (ny / 2) + 63) / 64);
+  // multi-piece operands: XF rows are tiles of 32 MX_MF rows holding 16 MX_MF (z, y) pairs; YF and YI rows
+  // (z, sx); XI rows (z, y)
+  p->mxRB[0] = MX_MF * ((nz * (ny / 2) + 16 * MX_MF - 1) / (16 * MX_MF));
   p->mxRB[1] = p->mxRB[2] = mx_rb(nz * nx);
   p->mxRB[3] = mx_rb(nz * ny);
   const int mxKB[4] = {p->ax.KB, p->ay.KB, p->ay.KB, p->ax.KB};

@@ -129,10 +129,11 @@ void launch_dst(hipStream_t s, int mode, const DstArgs& a);
 // Multi-piece BF16 passes (operands pre-split in the tiled layout of k_solve.hip). Pass chain:
 //   fold_x: rhs -> A_xf;  XF: A_xf -> A_yf;  YF: A_yf -> spectral [z][sx][sy];  Thomas -> A_yi;
 //   YI: A_yi -> A_xi;  XI: A_xi -> phi
+constexpr int MX_MF = 4;  // 32-row fragments per wave: a block covers 32 MX_MF rows of A
 struct MxArgs {
-  const uint16_t* A;  // data operand, [2 parities][P pieces][aRB][KB] tiles
-  const uint16_t* B;  // sine matrices, [2][P][bRB][KB] tiles
-  int aRB, bRB, KB;   // 32-row blocks of A and of B per piece plane; 16-wide k-blocks
+  const uint16_t* A;  // data operand, [2 parities][P pieces][aRB][KB + 1] tiles (one pitch tile unused)
+  const uint16_t* B;  // sine matrices, [2][P][bRB][KB + 1] tiles
+  int aRB, bRB, KB;   // 32-row blocks of A and of B per piece plane; 16-wide k-blocks (even)
   int Mvalid;         // rows of A that carry data (XF: (z, y) pairs)
   int tilesN;         // column tiles (64 NF wide) per parity
   int Nx, Ny, nz;     // padded grid
