@@ -6,8 +6,9 @@ velocity and moisture advection, forces + Thompson microphysics, divergence, the
 1072 x 1072 x 80 pressure solve and the projection. Inputs are resident in HBM.
 
 Prints one JSON line: sim-s per wall-s (``value``), cell-updates/s, the per-kernel HIP-event
-times, the roofline of the dominant kernel and a CPU baseline (the oracle's FP32 step on a
-bounded sample, rank 0 only).
+times, the roofline of the dominant kernel, and the CPU leg: a CPU baseline (the oracle's FP32
+step on a bounded sample, rank 0 only) and the error of one step against the FP64 oracle of the
+same step.
 """
 from __future__ import annotations
 
@@ -46,9 +47,15 @@ def kernel_roofline(name, ms, nx, ny, nz, nxp, nyp, gemm):
     if name in ("ps.xf", "ps.xi", "ps.yf", "ps.yi"):
         n = nxp if name in ("ps.xf", "ps.xi") else nyp
         fl = cells_p * n  # folded DST: two (M x n/2 x n/2) GEMMs per pass = cells_p * n flops
-        peak = FP32_MFMA_PEAK_TFS if gemm == "fp32" else BF16_MFMA_PEAK_TFS / 3.0
+        # FP32 MFMA, or BF16 MFMA shared by the 3 (bf16x3) or 6 (bf16x6) piece products of each flop
+        peak = {"fp32": FP32_MFMA_PEAK_TFS, "bf16x3": BF16_MFMA_PEAK_TFS / 3.0, "bf16x6": BF16_MFMA_PEAK_TFS / 6.0}[gemm]
         return {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
                 "algorithmic_flops": fl}
+    if name == "ps.fold" and gemm in ("bf16x3", "bf16x6"):
+        pieces = 2 if gemm == "bf16x3" else 3
+        b = (4 + 2 * pieces) * cells_p  # rhs read once, its folded pieces written once
+        return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "algorithmic_bytes": b}
     if name == "ps.tri":
         b = 8 * cells_p
         return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -69,7 +76,8 @@ def measured_traffic(kernel: str, size: str):
 
 
 def cpu_baseline(threads: int, dt: float, full_cells: int):
-    """The oracle's FP32 step (C, OpenMP) on a 256 x 256 x 80 slice of the workload at 500 m."""
+    """The oracle's FP32 step (C, OpenMP) on a 256 x 256 x 80 slice of the workload at 500 m, plus the 128 x 128 x
+    40 run of config 1 and the 1024 x 1024 x 80 solve alone, each on a bounded sample."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -105,11 +113,89 @@ def cpu_baseline(threads: int, dt: float, full_cells: int):
         par = 1 - par
     t = sorted(times[1:])[len(times[1:]) // 2]
     rate = nx * ny * nz / t
-    return {"value": rate * dt / full_cells, "unit": "sim-s/wall-s", "cores": threads, "kind": "port",
-            "cell_updates_per_s": rate,
-            "sample": f"oracle FP32 step (C/OpenMP, dense-GEMM solve as the reference) on 256x256x80 at 500 m "
-                      f"(pad 24), median of 2 steps after 1 warm-up ({t:.2f} s/step), scaled by cells to "
-                      f"1024x1024x80; the reference's own WGSL on a CPU WebGPU adapter is unavailable here"}
+    return {"value": rate * dt / full_cells, "unit": "sim-s/wall-s", "cores": threads,
+            "host_cpus": os.cpu_count(), "kind": "port", "cell_updates_per_s": rate,
+            "sample": f"oracle FP32 step (C/OpenMP, {threads} threads, dense-GEMM solve as the reference) on "
+                      f"256x256x80 at 500 m (pad 24), median of 2 steps after 1 warm-up ({t:.2f} s/step), scaled "
+                      f"by cells to 1024x1024x80; the reference's own WGSL on a CPU WebGPU adapter is unavailable "
+                      f"here",
+            "config1": cpu_config1(threads), "solve_alone": cpu_solve_alone(threads)}
+
+
+def cpu_config1(threads: int, steps: int = 40):
+    """The oracle's FP32 step on config 1 (128 x 128 x 40 at 2 km, Lin, one 2 K bubble, dt 6 s): ms per step over
+    `steps` steps from the bubble's start, and the 600-step run it implies."""
+    import numpy as np
+
+    from _cases import host_model, uniforms  # noqa: E402
+    from _oracle import Oracle, step_f32  # noqa: E402
+
+    nx = ny = 128
+    nz, dt = 40, 6.0
+    m = host_model(nx, ny, nz, 2000.0, scheme="lin")
+    o = Oracle(uniforms(m, 1.0), m.B, threads=threads)
+    A, B, ph = o.lsf(np.zeros(8 * 148, np.float32))
+    V, Q1, Q2 = o.init(A, B, 0)
+    V = o.bubble(V, 0.5 * m.Lx, 0.5 * m.Ly, 1500.0, 10000.0, 1500.0, 2.0)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    V2, Q12, Q22 = f32(np.stack([V, V])), f32(np.stack([Q1, Q1])), f32(np.stack([Q2, Q2]))
+    Vt, Q1t, Q2t = np.zeros_like(V2[0]), np.zeros_like(Q12[0]), np.zeros_like(Q12[0])
+    aux = np.zeros((nz, ny, nx, 2), np.float32)
+    ciMap = np.zeros((ny, nx, 4), np.float32)
+    rhs = np.zeros((nz, m.nyp, m.nxp), np.float32)
+    phi = np.zeros_like(rhs)
+    A, B, ph = f32(A), f32(B), f32(ph)
+    bd = np.zeros(8 * 148, np.float32)
+    par = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        m.time = s * dt
+        step_f32(uniforms(m, dt), m.B, bd, None, s % 10 == 0, par, V2, Q12, Q22, Vt, Q1t, Q2t, aux, ph, A, B, ciMap,
+                 rhs, phi, threads=threads)
+        par = 1 - par
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"ms_per_step": round(ms, 2), "sim_s_per_wall_s": round(dt / (ms * 1e-3), 1),
+            "run_600_steps_s": round(600 * ms * 1e-3, 1), "cores": threads,
+            "sample": f"{steps} oracle FP32 steps of 128x128x40 at 2 km (pad {m.pad}) from the bubble's start"}
+
+
+def cpu_solve_alone(threads: int):
+    """The oracle's FP32 solve with the reference's arithmetic (dense sine-matrix GEMMs, FP32 Thomas) on the
+    1024 x 1024 x 80 solve-alone workload: seconds per solve (one solve)."""
+    import numpy as np
+
+    from _oracle import solve as oracle_solve  # noqa: E402
+    from outbreaklab_amd.configs import solve_rhs_1024
+    from outbreaklab_amd.sounding import base_state, parse_sharppy, wk_sounding_text
+
+    nx = ny = 1024
+    nz, dx = 80, 500.0
+    dz = 20000.0 / nz
+    Bs = base_state(parse_sharppy(wk_sounding_text()), nz, dz)["B"]
+    rc, rw = Bs[2:8 * nz:8].astype(np.float64)[:nz], Bs[3::8].astype(np.float64)[: nz + 1]
+    rhs = solve_rhs_1024(nx, ny, nz, seed=1).astype(np.float32)
+    t0 = time.perf_counter()
+    oracle_solve(nx, ny, nz, dx, dz, rc, rw, rhs, "f32ref", threads=threads)
+    s = time.perf_counter() - t0
+    return {"s_per_solve": round(s, 2), "cores": threads,
+            "sample": "one oracle solve in the reference's FP32 arithmetic (solver.js:12-103: dense sine-matrix "
+                      "GEMMs, FP32 Thomas) of the 1024x1024x80 k^-5/3 RHS"}
+
+
+def error_vs_fp64(m, dt, forms, threads: int):
+    """Error of one GPU step against the FP64 oracle of the same step, from the model's current state, for each
+    solver form: every prognostic field, theta' (K RMS), phi and the post-projection divergence (relative to the
+    divergence the projection removed)."""
+    from _fp64 import fp64_report  # noqa: E402
+
+    t0 = time.perf_counter()
+    r = fp64_report(m, dt, forms, threads=threads)
+    out = {form: {k: float(f"{v:.4g}") for k, v in e.items()} for form, e in r.items()}
+    out["state"] = f"after {m.steps} steps ({m.time:.0f} s simulated)"
+    out["method"] = ("one GPU step per form from the same state; FP64 step = the oracle's FP64 kernels around an "
+                     "FP64 scipy DST solve of the same rhs (tests/_fp64.py)")
+    out["check_s"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def timed(m, dt, steps, warm=2):
@@ -132,11 +218,12 @@ def kernel_times(m, dt, steps=3):
     return {k: round(v[0], 4) for k, v in prof.items()}
 
 
-def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
+def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps, plain_gemm="bf16x6"):
     """The combined form on the same grid: the run continued until storms hold condensate, teacher
-    pairs from `teach_steps` plain steps, GPU training, then the plain FP32 step and the combined
-    form (surrogate + split-BF16 transforms + mixed FP32/FP64 Thomas) timed from the same developed
-    state, and one judged step (the surrogate against the plain block from the same inputs)."""
+    pairs from `teach_steps` plain steps, GPU training, then the plain step (transforms `plain_gemm`, and the
+    FP32-MFMA transforms) and the combined form (surrogate + split-BF16 transforms + mixed FP32/FP64 Thomas)
+    timed from the same developed state, and one judged step (the surrogate against the plain block from the
+    same inputs)."""
     from outbreaklab_amd.surrogate import Surrogate, fit, teach
 
     t0 = time.perf_counter()
@@ -147,6 +234,8 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     losses = fit(net, rows, steps=train_steps, batch=8192, lr=2e-3)
     prep = time.perf_counter() - t0
     m.set_solver_mode("fp32", "mixed")
+    ms_fp32 = timed(m, dt, steps)
+    m.set_solver_mode(plain_gemm, "mixed")
     ms_plain = timed(m, dt, steps)
     k_plain = kernel_times(m, dt)
     m.set_surrogate(net)
@@ -157,11 +246,13 @@ def combined_form(m, dt, steps, spin, teach_steps, p_active, train_steps):
     m.step(dt, judge=True)
     st = m.surrogate_stats(reset=True)
     m.set_surrogate(None)
-    m.set_solver_mode("fp32", "mixed")
+    m.set_solver_mode(plain_gemm, "mixed")
     net.destroy()
     return {
-        "ms_per_step_plain_fp32": ms_plain, "ms_per_step_combined": ms_comb,
+        "ms_per_step_plain": ms_plain, "plain_gemm": plain_gemm, "ms_per_step_plain_fp32_mfma": ms_fp32,
+        "ms_per_step_combined": ms_comb,
         "sim_s_per_wall_s_combined": dt / (ms_comb * 1e-3), "speedup_vs_plain": ms_plain / ms_comb,
+        "speedup_vs_plain_fp32_mfma": ms_fp32 / ms_comb,
         "state": f"after {m.steps} steps ({m.time:.0f} s simulated)",
         "surrogate_points_per_step": st["points"] / max(st["steps"], 1), "fallbacks": st["fallbacks"],
         "teacher_rows": int(rows.shape[0]), "training": f"{train_steps} Adam steps x 8192 rows, loss "
@@ -304,8 +395,8 @@ def radar_volume(m, volumes=2):
 
 def solve_alone(nx=1024, ny=1024, nz=80, dx=500.0):
     """The pressure solve alone on 1024 x 1024 x 80 (dz 250 m, no padding): a seeded k^-5/3 RHS,
-    FP32 and split-BF16 transforms with mixed Thomas, device time per solve and the relative error
-    against an FP64 separable solve (scipy DST-II, independent of this library)."""
+    FP32-MFMA, split hi/lo BF16 and three-piece BF16 transforms with mixed Thomas, device time per solve and the
+    relative error against an FP64 separable solve (scipy DST-II, independent of this library)."""
     import numpy as np
     import scipy.fft
 
@@ -326,7 +417,7 @@ def solve_alone(nx=1024, ny=1024, nz=80, dx=500.0):
     ps.setDensity(rc, rw)
     out = {"grid": f"{nx}x{ny}x{nz}, dx {dx:.0f} m, dz {dz:.0f} m", "thomas": "mixed (FP64 below condition 16)"}
     r32 = rhs.astype(np.float32)
-    for gemm in ("fp32", "bf16x3"):
+    for gemm in ("fp32", "bf16x3", "bf16x6"):
         phi = ps.solve(r32, gemm=gemm, thomas="mixed")
         ps.profile(True)
         for _ in range(3):
@@ -361,14 +452,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--gemm", default="fp32", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--gemm", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"],
+                    help="sine-transform GEMM form of the plain step: bf16x6 (three BF16 pieces, FP32 accuracy; the "
+                         "default), fp32 (FP32 MFMA) or bf16x3 (split hi/lo, about 16 bits)")
     ap.add_argument("--thomas", default="mixed", choices=["fp32", "fp64", "mixed"])
     ap.add_argument("--size", default="1024,1024,80")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--form", default="plain", choices=["plain", "c5", "hurricane", "presets"],
-                    help="plain: the plain FP32 step at 1024x1024x80 (and the combined form on the same grid); "
+                    help="plain: the plain step at 1024x1024x80 (and the combined form on the same grid); "
                          "c5: the combined form at 1024x1024x100, 250 m; hurricane: only the hurricane-mode step "
                          "(--preset); presets: the plain step on each of the reference's outbreak grids")
     ap.add_argument("--preset", default="h2", choices=["h2", "h3", "h15"],
@@ -415,6 +508,13 @@ def main():
     ms_step = wall / args.steps * 1e3
     cells = nx * ny * nz
     print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    # the same step with the FP32-MFMA transforms, for comparison (not the headline form unless --gemm fp32)
+    ms_fp32 = None
+    if args.gemm != "fp32":
+        m.set_solver_mode("fp32", args.thomas, 16.0)
+        ms_fp32 = timed(m, dt, args.steps, warm=1)
+        m.set_solver_mode(args.gemm, args.thomas, 16.0)
+        print(f"# FP32-MFMA transforms: {ms_fp32:.3f} ms/step", file=sys.stderr, flush=True)
 
     # per-kernel HIP-event times on the model's stream (separate, shorter run)
     m.profile(True)
@@ -473,7 +573,8 @@ def main():
     if not args.no_combined:
         print("# combined form (surrogate + split-BF16) on the same grid ...", file=sys.stderr, flush=True)
         try:
-            comb = combined_form(m, dt, args.steps, spin=750, teach_steps=40, p_active=0.02, train_steps=8000)
+            comb = combined_form(m, dt, args.steps, spin=750, teach_steps=40, p_active=0.02, train_steps=8000,
+                                 plain_gemm=args.gemm)
         except Exception as e:  # reported, not fatal to the plain measurement
             comb = {"error": f"{type(e).__name__}: {e}"}
     solve = None
@@ -483,12 +584,18 @@ def main():
             solve = solve_alone()
         except Exception as e:  # reported, not fatal to the plain measurement
             solve = {"error": f"{type(e).__name__}: {e}"}
-    cpu = None
+    cpu = fp64 = None
     if not args.no_cpu:
+        # the CPU leg: the oracle as the baseline and as the checker of one step against FP64
         print("# CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_threads, dt, cells)
+        print("# error of one step against the FP64 oracle of the same step ...", file=sys.stderr, flush=True)
+        try:
+            fp64 = error_vs_fp64(m, dt, (args.gemm, "fp32") if args.gemm != "fp32" else ("fp32",), args.cpu_threads)
+        except Exception as e:  # reported, not fatal to the plain measurement
+            fp64 = {"error": f"{type(e).__name__}: {e}"}
     out = {
-        "metric": "sim-s per wall-s at 1024x1024x80 (plain FP32 step)",
+        "metric": "sim-s per wall-s at 1024x1024x80 (plain step)",
         "value": dt / (ms_step * 1e-3),
         "unit": "sim-s/wall-s",
         "n_gpus": 1,
@@ -503,10 +610,12 @@ def main():
         "data": "synthetic (WK sounding tuned to 3000 J/kg SBCAPE, 25 m/s shear, 8 seeded 2 K bubbles)",
         "config": {"workload": f"plain step {nx}x{ny}x{nz} at 500 m, Thompson, dt {dt} s, solve "
                                f"{m.nxp}x{m.nyp}x{nz}", "gemm": args.gemm, "thomas": args.thomas},
+        "ms_per_step_fp32_mfma_transforms": ms_fp32,
         "state_finite": finite,
         "roofline": roof,
         "kernels": kern,
         "cpu_baseline": cpu,
+        "error_vs_fp64": fp64,
         "combined_form": comb,
         "solve_alone": solve,
         "maps": maps,

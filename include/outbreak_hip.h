@@ -199,7 +199,7 @@ int obl_download(obl_model* h, int field, int copy, void* dst, size_t bytes, siz
 int obl_field_ptr(obl_model* h, int field, int copy, void** ptr, size_t* bytes);
 /* onSubmittedWorkDone (app.js:1704) */
 int obl_sync(obl_model* h);
-/* Precision of the model's pressure solve (defaults: FP32 GEMM, mixed Thomas) */
+/* Precision of the model's pressure solve (defaults: three-piece BF16 GEMM at FP32 accuracy, mixed Thomas) */
 int obl_set_solver_mode(obl_model* h, int gemm_mode, int thomas_mode, double fp64_cond);
 /* Per-kernel HIP-event timing of subsequent obl_step calls (0 = off). */
 int obl_profile(obl_model* h, int on);

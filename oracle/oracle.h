@@ -111,6 +111,12 @@ void orc_step_f64(const float* U, const float* BS, const float* BD, const float*
                   int parity, double* V2, double* Q12, double* Q22, double* Vt, double* Q1t, double* Q2t,
                   double* aux, double* phT, double* lsfA, double* lsfB, double* ciMap, double* rhs,
                   double* phi, int nthreads);
+/* the FP64 step in two parts around the solve: part 0 runs every kernel up to div (rhs), part 1 the projection
+ * from phi (orc_driver.c) */
+void orc_step_f64_split(const float* U, const float* BS, const float* BD, const float* CI, int run_lsf, int parity,
+                        double* V2, double* Q12, double* Q22, double* Vt, double* Q1t, double* Q2t, double* aux,
+                        double* phT, double* lsfA, double* lsfB, double* ciMap, double* rhs, double* phi, int nthreads,
+                        int part);
 
 #ifdef __cplusplus
 }

@@ -484,7 +484,7 @@ struct obl_model {
   float2* aux = nullptr;
   float *phT = nullptr, *phi = nullptr, *rhs = nullptr, *BD = nullptr, *CI = nullptr;
   obl_poisson* solver = nullptr;
-  int gemm_mode = OBL_GEMM_FP32, thomas_mode = OBL_THOMAS_MIXED;
+  int gemm_mode = OBL_GEMM_BF16X6, thomas_mode = OBL_THOMAS_MIXED;
   double fp64_cond = 16.0;
   // microphysics teacher sampling (surrogate training data)
   float* dump_buf = nullptr;

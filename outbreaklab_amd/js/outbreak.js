@@ -114,12 +114,12 @@ class PoissonDirect {
     this.setDensity(new Array(nz).fill(1), new Array(nz + 1).fill(1));
   }
   setDensity(rhoC, rhoW) { addon.poissonSetDensity(this.h, Float64Array.from(rhoC), Float64Array.from(rhoW)); }
-  // host arrays (x fastest); precision: {gemm: 'fp32'|'bf16x3', thomas: 'fp32'|'fp64'|'mixed', cond}
+  // host arrays (x fastest); precision: {gemm: 'bf16x6'|'fp32'|'bf16x3', thomas: 'fp32'|'fp64'|'mixed', cond}
   solve(rhs, opt) {
     const o = opt || {};
     const phi = new Float32Array(rhs.length);
     addon.poissonSolve(this.h, rhs instanceof Float32Array ? rhs : Float32Array.from(rhs), phi,
-      GEMM[o.gemm || 'fp32'], THOMAS[o.thomas || 'mixed'], dflt(o.cond, 16));
+      GEMM[o.gemm || 'bf16x6'], THOMAS[o.thomas || 'mixed'], dflt(o.cond, 16));
     return phi;
   }
   destroy() { this.h = null; }
@@ -304,7 +304,7 @@ class RegionalModel {
     this.time += dt; this.steps++;
     this.phiScale = this.cnNow ? 1 / (1 - dflt(this.cnOld, CN_OLD)) : 1;
   }
-  setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'fp32'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
+  setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'bf16x6'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
   sync() { return addon.sync(this.h); }
   // requestMaps (model.js:1478-1494): the 2-D maps of the current state from the uniform block last
   // written; returns () => Promise<{m0, m1, m2, m3: Float32Array(nx*ny*4), t}>, or null while both

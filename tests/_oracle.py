@@ -285,6 +285,22 @@ def step_f64(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, ph
       threads or min(8, os.cpu_count() or 1))
 
 
+def step_f64_split(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi,
+                   part, threads=None):
+    """The FP64 step in two parts around the solve: part 0 every kernel up to div (writes rhs), part 1 the
+    projection from phi; every array is float64, modified in place."""
+    f = lib().orc_step_f64_split
+    f.restype = None
+    f.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int] + [C.c_void_p] * 13 + [C.c_int, C.c_int]
+    arrs = [V2, Q12, Q22, Vt, Q1t, Q2t, aux, phT, lsfA, lsfB, ciMap, rhs, phi]
+    for a in arrs:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    ci = np.ascontiguousarray(CI if CI is not None and len(CI) else np.zeros(12), np.float32)
+    f(P(np.ascontiguousarray(U, np.float32)), P(np.ascontiguousarray(BS, np.float32)),
+      P(np.ascontiguousarray(BD, np.float32)), P(ci), int(run_lsf), int(parity), *[P(a) for a in arrs],
+      threads or min(8, os.cpu_count() or 1), int(part))
+
+
 def radar_cut(ru, BS, V, Q1, Q2, aux, tor, words, kdp_len, out=None):
     """One radar cut (radar.js:41-268) into a uint32 volume of ``words`` words (updated in place when
     ``out`` is given); returns (out, kdp)."""

@@ -6,16 +6,11 @@ Both FP32 steps are fed the same inputs as the FP64 step (the GPU's state). The 
 error against FP64 is at most twice the FP32 restatement's plus 1e-6 (theta' RMS likewise in kelvin); the GPU's
 divergence after the projection is at most twice the restatement's.
 """
-import numpy as np
 import pytest
 
-from _oracle import Oracle, step_f32, step_f64
-from _util import rel_l2, theta_pert
+from _fp64 import fp64_errors
 
 pytestmark = pytest.mark.gpu
-
-FIELDS = {"u": ("V", 0), "v": ("V", 1), "w": ("V", 2), "qv": ("Q1", 0), "qc": ("Q1", 1), "qi": ("Q1", 2),
-          "qr": ("Q1", 3), "qs": ("Q2", 0), "qg": ("Q2", 1), "Nr": ("Q2", 2)}
 
 
 @pytest.fixture(scope="module", params=["lin", "thompson"])
@@ -32,41 +27,6 @@ def developed(request):
     m.destroy()
 
 
-def fp64_errors(m, dt):
-    """Run one GPU step and the FP32 and FP64 oracle steps from the same state; return per-field errors."""
-    p = m.parity
-    dl = lambda f: np.stack([f[0].download(), f[1].download()])  # noqa: E731
-    ins = dict(V2=dl(m.V), Q12=dl(m.Q1), Q22=dl(m.Q2), Vt=m.Vt.download(), Q1t=m.Q1t.download(), Q2t=m.Q2t.download(),
-               aux=m.aux.download(), phT=m.phT.download(), lsfA=m.lsfA.download(), lsfB=m.lsfB.download(),
-               ciMap=m.ciMap.download(), rhs=m.rhs.download(), phi=m.phi.download())
-    run_lsf = m.lsf_due()
-    m.step(dt)
-    m.sync()
-    U, ci = m._u_last, (m._ci if m.ciActive else None)
-    s32 = {k: np.ascontiguousarray(v, np.float32) for k, v in ins.items()}
-    s64 = {k: np.ascontiguousarray(v, np.float64) for k, v in ins.items()}
-    step_f32(U, m._B, m._bd, ci, run_lsf, p, **s32)
-    step_f64(U, m._B, m._bd, ci, run_lsf, p, **s64)
-    gpu = {"V": m.V[1 - p].download(), "Q1": m.Q1[0].download(), "Q2": m.Q2[0].download(), "phi": m.phi.download()}
-    f32 = {"V": s32["V2"][1 - p], "Q1": s32["Q12"][0], "Q2": s32["Q22"][0], "phi": s32["phi"]}
-    f64 = {"V": s64["V2"][1 - p], "Q1": s64["Q12"][0], "Q2": s64["Q22"][0], "phi": s64["phi"]}
-    out = {}
-    for name, (fld, ch) in FIELDS.items():
-        out[name] = (rel_l2(gpu[fld][..., ch], f64[fld][..., ch]), rel_l2(f32[fld][..., ch], f64[fld][..., ch]))
-    th = lambda V: theta_pert(V, m._B, m.nz)  # noqa: E731
-    rms = lambda d: float(np.sqrt(np.mean(d * d)))  # noqa: E731
-    out["theta_rms_K"] = (rms(th(gpu["V"]) - th(f64["V"])), rms(th(f32["V"]) - th(f64["V"])))
-    out["phi"] = (rel_l2(gpu["phi"], f64["phi"]), rel_l2(f32["phi"], f64["phi"]))
-    # divergence left by the projection, relative to the divergence it removed, over the model's cells
-    o = Oracle(U, m._B, "f64")
-    pd = m.pad
-    cut = np.s_[:, pd:pd + m.ny, pd:pd + m.nx]
-    d0 = np.linalg.norm(s64["rhs"][cut])
-    out["divergence"] = tuple(float(np.linalg.norm(o.div(np.asarray(V, np.float64))[cut]) / d0)
-                              for V in (gpu["V"], f32["V"], f64["V"]))
-    return out
-
-
 @pytest.mark.parametrize("gemm", ["fp32", "bf16x6"])
 def test_step_error_against_fp64(developed, gemm):
     """The plain step with the FP32-MFMA transforms, and with the three-piece BF16 transforms (FP32 accuracy),
@@ -76,7 +36,7 @@ def test_step_error_against_fp64(developed, gemm):
     try:
         e = fp64_errors(m, dt)
     finally:
-        m.set_solver_mode("fp32", "mixed")
+        m.set_solver_mode("bf16x6", "mixed")
     for k, v in e.items():
         print(f"{scheme:9s} {gemm:6s} {k:12s} " + " ".join(f"{x:.3e}" for x in v), flush=True)
     bad = {}

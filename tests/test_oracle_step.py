@@ -403,3 +403,45 @@ def test_fp32_and_fp64_oracles_agree():
     a = o32.advv2(V32, o32.advv1(V32))
     b = o64.advv2(V32.astype(np.float64), o64.advv1(V32.astype(np.float64)))
     assert rel_l2(a[..., :3], b[..., :3]) < 1e-6
+
+
+def test_split_fp64_step_equals_whole_step():
+    """The FP64 step in two parts around the solve (the bench's full-size FP64 report) is the whole FP64 step when
+    the caller supplies the oracle's own FP64 solve; with scipy's FP64 DST solve in its place the state agrees to
+    FP64 rounding."""
+    import numpy as np
+    from _cases import host_model, uniforms
+    from _oracle import Oracle, solve, step_f64, step_f64_split
+    from _ref_solve import solve_scipy
+
+    m = host_model(24, 16, 12, 2000.0, scheme="thompson")
+    o = Oracle(uniforms(m, 1.0), m.B, "f64")
+    A, B, ph = o.lsf(np.zeros(8 * 148, np.float32))
+    V, Q1, Q2 = o.init(A, B, 3)
+    V = o.bubble(V, 0.5 * m.Lx, 0.5 * m.Ly, 1500.0, 10000.0, 1500.0, 2.0)
+    c = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+    def state():
+        return dict(V2=c(np.stack([V, V])), Q12=c(np.stack([Q1, Q1])), Q22=c(np.stack([Q2, Q2])),
+                    Vt=np.zeros_like(c(V)), Q1t=np.zeros_like(c(Q1)), Q2t=np.zeros_like(c(Q1)),
+                    aux=np.zeros(Q1.shape[:3] + (2,)), phT=c(ph), lsfA=c(A), lsfB=c(B),
+                    ciMap=np.zeros(Q1.shape[1:3] + (4,)), rhs=np.zeros((m.nz, m.nyp, m.nxp)),
+                    phi=np.zeros((m.nz, m.nyp, m.nxp)))
+    U = uniforms(m, 6.0)
+    bd = np.zeros(8 * 148, np.float32)
+    whole = state()
+    step_f64(U, m.B, bd, None, True, 0, **whole)
+    rc, rw = o.densities()
+    for solver in ("oracle", "scipy"):
+        s = state()
+        step_f64_split(U, m.B, bd, None, True, 0, **s, part=0)
+        if solver == "oracle":
+            s["phi"][...] = solve(m.nxp, m.nyp, m.nz, float(U[3]), float(U[4]), rc, rw, s["rhs"], "f64")
+        else:
+            s["phi"][...] = solve_scipy(s["rhs"], float(U[3]), float(U[4]), rc, rw)
+        step_f64_split(U, m.B, bd, None, True, 0, **s, part=1)
+        for k in whole:
+            if solver == "oracle":
+                assert np.array_equal(s[k], whole[k]), k
+            else:
+                scale = np.abs(whole[k]).max() or 1.0
+                assert np.abs(s[k] - whole[k]).max() <= 1e-11 * scale, k
