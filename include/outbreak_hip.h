@@ -239,6 +239,9 @@ int obl_mlp_set_scales(obl_mlp* m, const float* out_scale, const float* in_lo, c
 /* a species absent before the block (below frac * its output scale) is changed at inference only
  * when its predicted tendency exceeds frac * its output scale (default 0.02) */
 int obl_mlp_set_dead_zone(obl_mlp* m, float frac);
+/* only points holding at least qmin kg/kg of condensate (qc + qi + qr + qs + qg) are handed to the network; the
+ * others run the plain block (default 1e-9, every point with condensate) */
+int obl_mlp_set_hand_min(obl_mlp* m, float qmin);
 /* copies nrows host rows (stride floats each) into device memory the network owns (replacing any
  * earlier copy); obl_mlp_train with rows_dev == NULL trains on them */
 int obl_mlp_load_rows(obl_mlp* m, const float* rows_host, size_t nrows, int stride);

@@ -80,6 +80,7 @@ _SIGS = {
     "obl_mlp_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_void_p,
                                C.c_void_p]),
     "obl_mlp_set_dead_zone": (C.c_int, [C.c_void_p, C.c_float]),
+    "obl_mlp_set_hand_min": (C.c_int, [C.c_void_p, C.c_float]),
     "obl_mlp_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "obl_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint]),
     "obl_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(SurrStats), C.c_int]),

@@ -706,10 +706,10 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     const float T0 = T, qv0 = qv, qc0 = qc, qi0 = qi, qr0 = qr, qs0 = qs, qg0 = qg, nr0 = nr;
     bool handed = false;
     if (MODE == PHYS_SURR) {
-      // points with condensate inside the training envelope go to the surrogate (k_surr.hip)
+      // points with at least qmin of condensate, inside the training envelope, go to the surrogate (k_surr.hip)
       float wc = 0.5f * (V.z + zcol.p.z);
       float in[14] = {T, B.y, A.z, qv, qc, qr, qi, qs, qg, nr, 0.0f, wc, dz, dt};
-      bool cand = micro_active(qc, qi, qr, qs, qg);
+      bool cand = micro_active(qc, qi, qr, qs, qg) && qc + qi + qr + qs + qg >= surr.qmin;
 #pragma unroll
       for (int t = 0; t < 14; t++) cand = cand && in[t] >= surr.lo[t] && in[t] <= surr.hi[t];
       unsigned slot;

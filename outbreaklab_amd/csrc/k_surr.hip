@@ -18,9 +18,8 @@
 //     the block, which the block itself conserves; condensate is scaled down if it exceeds it;
 //   - temperature from the latent heat of the phase changes, Lv (vapour -> liquid) and Lf
 //     (liquid -> ice), so that the step's moist enthalpy is consistent with the water changes;
-//   - Thompson Nr: the network predicts log((Nr' + 1) / (Nr + 1)); rain that is new in this step
-//     gets the block's own drop mass of new rain; clamped to the block's bounds for the new qr,
-//     0 below 1e-9;
+//   - Thompson Nr from the block's own drop-number rules applied to the network's mass changes
+//     (nr_from_mass);
 //   - a non-finite prediction falls back to the plain block for that point;
 // followed by the physics kernel's own tail (theta = T / pi, the sponge's condensate removal,
 // the final clamps).
@@ -42,6 +41,34 @@ constexpr int APPLY_THREADS = 512;
 
 __device__ __forceinline__ bool finite4(float a, float b, float c, float d) {
   return isfinite(a) && isfinite(b) && isfinite(c) && isfinite(d);
+}
+
+// Thompson Nr after the network's mass changes, by the block's own drop-number rules (model.js:783-866): the
+// autoconversion of the point's cloud water brings new drops of 2.68e-10 kg and accretion adds mass without drops
+// (both exact, from the state before the block); above 0 C the snow and graupel the network melts bring their
+// drops; any other change of the rain mass scales the drops in proportion (a chain of the block's proportional
+// losses is one ratio); then self-collection / breakup and the block's bounds. Given the block's own mass
+// changes this is the block's Nr; the network's own Nr output is not used (its per-step noise, large against the
+// few-per-cent changes of self-collection, random-walks Nr off over a storm's life).
+__device__ __forceinline__ float nr_from_mass(const MU& u, float dt, float rho, float T0, float qc0, float qr0,
+                                              float qs0, float qg0, float nr0, float qr, float qs, float qg) {
+  const float da = fminf(u.v[U_MICRO].x * fmaxf(qc0 - u.v[U_MICRO].y, 0.0f) * dt, qc0);
+  float accr = 2.2f * qc0 * pw(qr0, 0.875f);
+  if (qr0 > 1e-9f) accr = 3100.0f * rho * nr0 * pw(lamR(qr0, nr0), -2.8f) * qc0 * sqrtf(u.v[U_STEP].z / rho);
+  const float dc = fminf(accr * dt, qc0 - da);
+  float q1 = qr0 + da + dc, n = nr0 + da / 2.68e-10f;
+  if (T0 - OBL_T0K >= 0.0f) {
+    const float m1 = fmaxf(qs0 - qs, 0.0f), m2 = fmaxf(qg0 - qg, 0.0f);
+    if (qg0 > 1e-9f) n += m2 / qg0 * n0G(qg0, rho) / (rho * lamG(qg0, rho));
+    if (qs0 > 1e-9f) n += m1 / qs0 * 2e6f / (rho * pw(3.14159f * 100.0f * 2e6f / (rho * qs0), 0.25f));
+    q1 += m1 + m2;
+  }
+  if (q1 > 1e-12f && qr < q1) n *= qr / q1;
+  if (!(qr > 1e-9f)) return 0.0f;
+  const float dm = 1.817f / lamR(qr, n);
+  const float fb = clampf_(1.0f - ex(2300.0f * (dm - 1.1e-3f)), -2.0f, 1.0f);
+  n *= ex(-7.12f * rho * qr * fb * dt);
+  return clampf_(n, qr * 3.183e5f, qr * 3.183e11f);
 }
 
 __global__ void __launch_bounds__(APPLY_THREADS) k_surr_apply(MU u, SurrApply a) {
@@ -109,14 +136,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) k_surr_apply(MU u, SurrApply a)
         qv = fmaxf(W - cond, 0.0f);
         T = T0 + OBL_LVC / OBL_CPD * (qv0 - qv) + OBL_LFC / OBL_CPD * ((qi + qs + qg) - (qi0 + qs0 + qg0));
         nr = nr0;
-        if (thm) {
-          // rain that did not exist before the block (at most 1e-9 kg/kg, below which the block
-          // resets Nr) is new autoconversion rain: the block gives it
-          // one drop per 2.68e-10 kg (model.js:788); otherwise the predicted log ratio
-          // y7 = log((Nr' + 1) / (Nr + 1))
-          float nn = qr0 <= 1e-9f ? qr / 2.68e-10f : fmaxf((nr0 + 1.0f) * ex(y[7]) - 1.0f, 0.0f);
-          nr = qr > 1e-9f ? clampf_(nn, qr * 3.183e5f, qr * 3.183e11f) : 0.0f;
-        }
+        if (thm) nr = nr_from_mass(u, d.dt, rho, T0, qc0, qr0, qs0, qg0, nr0, qr, qs, qg);
         // the block's closing saturation adjustment, exact
         micro_satadj(p0, T, qv, qc, qi);
       } else {

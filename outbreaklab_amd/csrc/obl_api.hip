@@ -1181,6 +1181,7 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
       sp.lo[i] = mv.lo[i];
       sp.hi[i] = mv.hi[i];
     }
+    sp.qmin = mv.hand_min;
     SurrApply sa;
     sa.rec = h->srec;
     sa.count = h->scount;

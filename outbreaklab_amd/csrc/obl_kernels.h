@@ -52,6 +52,7 @@ struct SurrPhys {
   unsigned* count;
   unsigned cap;
   float lo[14], hi[14];  // training envelope of the 14 inputs
+  float qmin;            // least total condensate of a handed point
 };
 void launch_phys_surr(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                       const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,

@@ -70,6 +70,7 @@ struct MlpView {
   float out_scale[MLP_NOUT];
   float lo[MLP_NIN], hi[MLP_NIN];
   float dead_frac;  // dead zone of absent species, as a fraction of each output scale
+  float hand_min;   // least total condensate (kg/kg) of a point handed to the network
 };
 
 int mlp_grad_blocks(int batch);

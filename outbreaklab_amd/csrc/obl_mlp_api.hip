@@ -28,6 +28,7 @@ struct obl_mlp {
   float in_lo[MLP_NIN], in_hi[MLP_NIN];
   int adam_t = 0;
   float dead_frac = 0.02f;
+  float hand_min = 1e-9f;
   std::string err;
   int fail(const char* fmt, ...) {
     char buf[512];
@@ -64,6 +65,7 @@ int mlp_view(obl_mlp* m, MlpView* v) {
     v->hi[i] = m->in_hi[i];
   }
   v->dead_frac = m->dead_frac;
+  v->hand_min = m->hand_min;
   return 0;
 }
 }  // namespace obl
@@ -145,6 +147,13 @@ extern "C" int obl_mlp_set_dead_zone(obl_mlp* m, float frac) {
   if (!m) return -1;
   if (!(frac >= 0.0f && frac < 1.0f)) return m->fail("dead zone fraction must be in [0, 1)");
   m->dead_frac = frac;
+  return 0;
+}
+
+extern "C" int obl_mlp_set_hand_min(obl_mlp* m, float qmin) {
+  if (!m) return -1;
+  if (!(qmin >= 0.0f && qmin < 1.0f)) return m->fail("handing threshold must be in [0, 1) kg/kg");
+  m->hand_min = qmin;
   return 0;
 }
 

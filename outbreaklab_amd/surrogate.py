@@ -77,6 +77,11 @@ class Surrogate:
         """Absent species are created at inference only above frac * their output scale."""
         _lib.check(_lib.lib().obl_mlp_set_dead_zone(self.h, float(frac)), self.h, mlp=True)
 
+    def set_hand_min(self, qmin):
+        """Only points holding at least qmin kg/kg of condensate are handed to the network at inference; thinner
+        cloud runs the plain block."""
+        _lib.check(_lib.lib().obl_mlp_set_hand_min(self.h, float(qmin)), self.h, mlp=True)
+
     def load_rows(self, rows):
         """Copy host rows (n x stride float32) to device memory owned by the network."""
         rows = np.ascontiguousarray(rows, np.float32)

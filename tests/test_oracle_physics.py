@@ -694,3 +694,54 @@ def test_mixed_phase_saturation_adjustment():
     dsum = 1.05 * q0 - r["qv"]
     assert fo == pytest.approx((T0 - (T1 - LF / CP * (r["qi"] - fo * dsum))) / 40, rel=1e-9)
     assert T1 > T
+
+
+def _nr_from_mass(p, T0, qc0, qr0, qs0, qg0, nr0, qr, qs, qg):
+    """The surrogate's Thompson Nr (nr_from_mass in k_surr.hip) in float64: the block's drop-number rules applied
+    to given mass changes."""
+    U, dt, rho = p.U, p.dt, p.rho
+    pw = lambda x, y: x ** y  # noqa: E731
+    lamR = lambda q, n: min(max(pw(3.14159 * 1000.0 * max(n, 1e-3) / max(q, 1e-12), 0.33333), 1000.0), 1e5)  # noqa: E731
+    n0G = lambda q: min(max(200.0 / max(rho * q, 1e-9), 1e4), 3e6)  # noqa: E731
+    lamG = lambda q: pw(3.14159 * 500.0 * n0G(q) / max(rho * q, 1e-12), 0.25)  # noqa: E731
+    da = min(float(U[20]) * max(qc0 - float(U[21]), 0.0) * dt, qc0)
+    accr = 2.2 * qc0 * pw(qr0, 0.875)
+    if qr0 > 1e-9:
+        accr = 3100.0 * rho * nr0 * pw(lamR(qr0, nr0), -2.8) * qc0 * p.rs
+    dc = min(accr * dt, qc0 - da)
+    q1, n = qr0 + da + dc, nr0 + da / 2.68e-10
+    if T0 - 273.15 >= 0.0:
+        m1, m2 = max(qs0 - qs, 0.0), max(qg0 - qg, 0.0)
+        if qg0 > 1e-9:
+            n += m2 / qg0 * n0G(qg0) / (rho * lamG(qg0))
+        if qs0 > 1e-9:
+            n += m1 / qs0 * 2e6 / (rho * pw(3.14159 * 100.0 * 2e6 / (rho * qs0), 0.25))
+        q1 += m1 + m2
+    if q1 > 1e-12 and qr < q1:
+        n *= qr / q1
+    if not qr > 1e-9:
+        return 0.0
+    dm = 1.817 / lamR(qr, n)
+    fb = min(max(1.0 - math.exp(2300.0 * (dm - 1.1e-3)), -2.0), 1.0)
+    n *= math.exp(-7.12 * rho * qr * fb * dt)
+    return min(max(n, qr * 3.183e5), qr * 3.183e11)
+
+
+def test_surrogate_nr_rule_reproduces_the_block_nr():
+    """The surrogate sets Thompson Nr from the network's mass changes by the block's own drop-number rules. Given
+    the block's own mass changes (the oracle block's output), the rule must return the block's own Nr: cold and
+    warm points, with and without cloud water (autoconversion, accretion), rain (new, thin, heavy), melting snow and
+    graupel, in saturated and dry air (evaporation)."""
+    p = Point("thompson")
+    n = 0
+    for T in (265.0, 280.0, 284.0, 292.0):
+        for qc in (0.0, 5e-4, 3e-3):
+            for qr, nr in ((0.0, 0.0), (5e-10, 10.0), (1e-5, 3e3), (1e-3, 2e4), (2e-3, 5e5)):
+                for qs, qg in ((0.0, 0.0), (3e-4, 0.0), (0.0, 2e-3), (5e-4, 1e-3)):
+                    for qv in (p.wet(T), 0.6 * p.qvsl(T)):
+                        r = p(T, qv, qc=qc, qr=qr, qs=qs, qg=qg, nr=nr)
+                        want = r["nr"]
+                        got = _nr_from_mass(p, T, qc, qr, qs, qg, nr, r["qr"], r["qs"], r["qg"])
+                        assert got == pytest.approx(want, rel=1e-9, abs=1e-30), (T, qc, qr, nr, qs, qg, qv)
+                        n += want > 0
+    assert n > 150  # most cases carry rain drops
