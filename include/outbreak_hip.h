@@ -184,6 +184,12 @@ int obl_radar_cut(obl_radar* r, const float* ru, int parity);
  * the process ends; a cut needing more is rejected. Every call on one GPU shares that array: calls that overlap
  * on one GPU (two streams, two threads) race. Enqueued on the model's stream; no timing (obl_profile) is taken. */
 int obl_radar_cut_into(obl_model* h, const float* ru, int parity, uint32_t* out_dev, size_t out_bytes);
+/* Diagnostic, no counterpart in the reference: the geometry of listed gates of a cut (ru as obl_radar_cut) as the
+ * cut kernel computes it, for comparing the GPU's FP32 geometry with a CPU evaluation of the same formulas.
+ * rg = n (radial, gate) pairs (host), out = n x 7 floats (host): model x, y (m), the radial unit vector (ex, ey),
+ * the beam-centre height and the heights 0.4 beam widths below and above it (m above the model's ground).
+ * Allocates and frees two small device buffers; synchronous on the model's stream. */
+int obl_radar_geometry(obl_model* h, const float* ru, int n, const int* rg, float* out);
 /* the volume's copy into a pinned slot (the reference's copyBufferToBuffer into read[slot]):
  * *slot = 0 / 1, or -1 while both slots await collection */
 int obl_radar_readback(obl_radar* r, int* slot);

@@ -78,6 +78,10 @@ ORC_DECL(double, f64)
 /* Virtual radar cut (radar.js:41-268), FP32: both passes of one cut, and the beam geometry of a gate. */
 void orc_radar_cut(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2, const float* aux,
                    const float* tor, uint32_t* out, float* kdp);
+/* listed gates' moment values before rounding and decision margins, and the values at a given geometry (the
+   GPU's): rg = n (radial, gate) pairs, gg = null or n x 7 floats, out[n][24] */
+void orc_radar_gates(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2,
+                     const float* aux, const float* tor, int n, const int* rg, const float* gg, float* out);
 void orc_radar_geom(const float* ru, int ri, int gi, float* hs_xy);
 
 /* Noise hash (noise.js:5-9), integer-exact. */

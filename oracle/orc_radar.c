@@ -83,7 +83,28 @@ static F2 beam(const Ctx* c, float r, float el) {
   return o;
 }
 
-typedef struct { float zr, zs, zg, vt; F3 u; float T, qr, rho, nu, inside, lr; } Smp;
+/* a gate's geometry (radar.js:146-156): position, radial direction, the three beam heights */
+typedef struct { F2 p, e; float zc, hl, hu; } Geom;
+static Geom geom(const Ctx* c, int ri, int gi) {
+  const RU* R = c->R;
+  Geom g;
+  float r = R->gates.z + R->gates.w * (float)gi;
+  float el = R->cut.x, bw = 0.0165f;
+  float az = (R->cut.y + R->cut.z * (float)ri) * 0.017453293f;
+  F2 hc = beam(c, r, el);
+  g.p = gatePos(c, az, hc.y);
+  F2 p2 = gatePos(c, az, hc.y + 200.0f);
+  g.e.x = p2.x - g.p.x; g.e.y = p2.y - g.p.y;
+  float en = sqrtf(g.e.x * g.e.x + g.e.y * g.e.y);
+  g.e.x /= en; g.e.y /= en;
+  float zAnt = R->site.z - R->mdl2.y;
+  g.zc = zAnt + hc.x;
+  g.hl = beam(c, r, el - 0.4f * bw).x + zAnt;
+  g.hu = beam(c, r, el + 0.4f * bw).x + zAnt;
+  return g;
+}
+
+typedef struct { float zr, zs, zg, vt; F3 u; float T, qr, qg, rho, nu, inside, lr; } Smp;
 /* radar.js:95-139 */
 static Smp sample(const Ctx* c, F2 xy, float z) {
   const RU* R = c->R;
@@ -141,32 +162,55 @@ static Smp sample(const Ctx* c, F2 xy, float z) {
   float zt = o.zr + o.zs + o.zg;
   o.vt = zt > 1e-6f ? (o.zr * vr + o.zs * vs + o.zg * vg) / zt : 0.0f;
   o.u.x = u; o.u.y = v; o.u.z = w;
-  o.T = T; o.qr = qr; o.rho = rho;
+  o.T = T; o.qr = qr; o.qg = qg; o.rho = rho;
   o.nu = tsamp_uv(&c->aux, uc, 0);
   return o;
 }
 
-typedef struct { uint32_t zref, vel, sw, zdr, rho; float kdp; } Gate;
+/* a gate's codes and KDP; for orc_radar_vals also each moment's value before rounding and the margin of every
+ * discrete decision the gate went through (how far its input lay from the switch) */
+typedef struct {
+  uint32_t zref, vel, sw, zdr, rho;
+  float kdp;
+  float val[5];  /* REF, VEL, SW, ZDR, RHO: v * scale + offset before rounding (NAN where not formed) */
+  float mgeo;    /* m: distance of the beam centre from the model top and of s0 from the domain's edges */
+  float mecho;   /* dB: dbz - threshold (the echo decision), or log10(ztot / 1e-6) * 10 below it */
+  float mwrap;   /* m/s: distance of the radial velocity from a Nyquist wrap */
+  float mT;      /* K: least |T - 273.15| over the three beam samples (the melting / graupel switches) */
+  float mzg;     /* relative distance of zg from the 3.2e5 switch */
+  float md0;     /* mm: distance of the median drop size from the 2.5 mm switch */
+  float mq;      /* relative distance of qr and qg from the 1e-9 switches of the Thompson sizes */
+} Gate;
+static float relm(float v, float t) { return fabsf(v - t) / t; }
 /* radar.js:144-214 */
-static Gate gate(const Ctx* c, const float* tor, int ri, int gi) {
+/* gg: null, or the gate's geometry to use in place of its own (Geom's p, e, zc, hl, hu as 7 floats;
+   orc_radar_gates) */
+static Gate gate(const Ctx* c, const float* tor, int ri, int gi, const float* gg) {
   const RU* R = c->R;
-  Gate G = {0, 0, 0, 0, 0, -1.0f};
+  Gate G = {0, 0, 0, 0, 0, -1.0f, {NAN, NAN, NAN, NAN, NAN}, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY,
+            INFINITY, INFINITY};
   float r = R->gates.z + R->gates.w * (float)gi;
   float el = R->cut.x;
-  float az = (R->cut.y + R->cut.z * (float)ri) * 0.017453293f;
   float bw = 0.0165f;
-  F2 hc = beam(c, r, el);
-  F2 p = gatePos(c, az, hc.y), p2 = gatePos(c, az, hc.y + 200.0f);
-  F2 e = {p2.x - p.x, p2.y - p.y};
-  float en = sqrtf(e.x * e.x + e.y * e.y);
-  e.x /= en; e.y /= en;
-  float zAnt = R->site.z - R->mdl2.y;
-  float zc = zAnt + hc.x;
+  Geom gm = geom(c, ri, gi);
+  if (gg) {
+    gm.p.x = gg[0]; gm.p.y = gg[1]; gm.e.x = gg[2]; gm.e.y = gg[3]; gm.zc = gg[4]; gm.hl = gg[5]; gm.hu = gg[6];
+  }
+  F2 p = gm.p, e = gm.e;
+  float zc = gm.zc;
+  G.mgeo = fabsf(R->mdl.z * R->mdl2.x - 200.0f - zc);
+  {
+    float ex = fminf(fminf(p.x, R->mdl.x * R->mdl.w - p.x), fminf(p.y, R->mdl.y * R->mdl.w - p.y));
+    G.mgeo = fminf(G.mgeo, fabsf(ex));
+  }
   if (zc > R->mdl.z * R->mdl2.x - 200.0f) return G;
-  float hl = beam(c, r, el - 0.4f * bw).x + zAnt, hu = beam(c, r, el + 0.4f * bw).x + zAnt;
+  float hl = gm.hl, hu = gm.hu;
   F2 pp = p;
   Smp s0 = sample(c, pp, fmaxf(zc, 5.0f)), s1 = sample(c, pp, fmaxf(hl, 5.0f)), s2 = sample(c, pp, fmaxf(hu, 5.0f));
   if (s0.inside < 0.5f) return G;
+  G.mT = fminf(fabsf(s0.T - 273.15f), fminf(fabsf(s1.T - 273.15f), fabsf(s2.T - 273.15f)));
+  G.mq = fminf(fminf(relm(s0.qr, 1e-9f), relm(s1.qr, 1e-9f)), relm(s2.qr, 1e-9f));
+  G.mq = fminf(G.mq, fminf(fminf(relm(s0.qg, 1e-9f), relm(s1.qg, 1e-9f)), relm(s2.qg, 1e-9f)));
   float zr = 0.5f * s0.zr + 0.25f * (s1.zr + s2.zr), zs = 0.5f * s0.zs + 0.25f * (s1.zs + s2.zs);
   float zg = 0.5f * s0.zg + 0.25f * (s1.zg + s2.zg);
   float zmet = zr + zs + zg;
@@ -192,21 +236,30 @@ static Gate gate(const Ctx* c, const float* tor, int ri, int gi) {
     }
   }
   float ztot = zmet + zdeb;
+  G.mecho = ztot > 0.0f ? fabsf(10.0f * log10f(ztot / 1e-6f)) : INFINITY;
   if (ztot <= 1e-6f) return G;
   float dbz = 10.0f * logf(ztot) / 2.302585f;
+  G.mecho = fminf(G.mecho, fabsf(dbz - R->phys.y));
   if (dbz < R->phys.y) return G;
   G.zref = code(dbz, 2.0f, 66.0f);
+  G.val[0] = dbz * 2.0f + 66.0f;
   if (gi >= (int)R->gates.y) return G;
   float ce = cosf(el), se = sinf(el);
   float vtf = s0.vt * zmet / ztot;
   float vr = ((s0.u.x + dvx) * e.x + (s0.u.y + dvy) * e.y) * ce + (s0.u.z - vtf) * se;
   float nyq = R->nyq.x;
+  {
+    float w = (vr + nyq) / (2.0f * nyq);
+    G.mwrap = fabsf(w - rintf(w)) * 2.0f * nyq;
+  }
   vr = vr - 2.0f * nyq * floorf((vr + nyq) / (2.0f * nyq));
   G.vel = code(vr, 2.0f, 129.0f);
+  G.val[1] = vr * 2.0f + 129.0f;
   float vru = (s2.u.x * e.x + s2.u.y * e.y) * ce, vrl = (s1.u.x * e.x + s1.u.y * e.y) * ce;
   float swv = sqrtf(1.0f + sq_(clampf_(2.5f * sqrtf(fmaxf(s0.nu, 0.0f) / 1000.0f), 0.0f, 8.0f)) + sq_((vru - vrl) / 2.5f) +
                     sq_(0.15f * s0.vt * se) + sq_(swt));
   G.sw = code(fminf(swv, 19.5f), 2.0f, 129.0f);
+  G.val[2] = fminf(swv, 19.5f) * 2.0f + 129.0f;
   float d0 = 3.67f / s0.lr * 1000.0f;
   int melt = s0.T > 273.15f;
   float zdrR = clampf_(0.2f + 1.15f * (d0 - 0.5f), 0.1f, 4.8f);
@@ -215,10 +268,14 @@ static Gate gate(const Ctx* c, const float* tor, int ri, int gi) {
   float zv = zr / db2lin(zdrR) + zs / db2lin(zdrS) + zg / db2lin(zdrG) + zdeb / db2lin(0.2f);
   float zdr = 10.0f * logf(fmaxf(ztot, 1e-9f) / fmaxf(zv, 1e-9f)) / 2.302585f;
   G.zdr = code(clampf_(zdr, -7.9f, 7.9f), 16.0f, 128.0f);
+  G.val[3] = clampf_(zdr, -7.9f, 7.9f) * 16.0f + 128.0f;
+  G.mzg = relm(zg, 3.2e5f);
+  G.md0 = fabsf(d0 - 2.5f);
   float fr = zr / ztot, fs = zs / ztot, fg = zg / ztot, fd = zdeb / ztot;
   float cc = fr * (d0 > 2.5f ? 0.986f : 0.995f) + fs * (melt ? 0.93f : 0.985f) + fg * (melt ? (zg > 3.2e5f ? 0.90f : 0.94f) : 0.985f) + fd * 0.55f;
   cc -= 0.24f * fr * fg;
   G.rho = code(clampf_(cc, 0.2f, 1.05f), 300.0f, -60.5f);
+  G.val[4] = clampf_(cc, 0.2f, 1.05f) * 300.0f - 60.5f;
   G.kdp = powf(0.017f * powf(fmaxf(zr, 0.0f), 0.714f) / 44.0f, 1.2165f);
   return G;
 }
@@ -239,7 +296,7 @@ void orc_radar_cut(const float* ru, const float* BS, const float* V, const float
       uint32_t pr = 0, pv = 0, ps = 0, pz = 0, ph = 0;
       for (int k = 0; k < 4; k++) {
         int gi = q * 4 + k;
-        Gate G = gate(&C, tor, ri, gi);
+        Gate G = gate(&C, tor, ri, gi, NULL);
         pr |= G.zref << (8u * k); pv |= G.vel << (8u * k); ps |= G.sw << (8u * k);
         pz |= G.zdr << (8u * k); ph |= G.rho << (8u * k);
         if (gi < ng) kdp[ri * ng + gi] = G.kdp;
@@ -285,4 +342,32 @@ void orc_radar_geom(const float* ru, int ri, int gi, float* hs_xy) {
   F2 hc = beam(&C, r, R.cut.x);
   F2 p = gatePos(&C, az, hc.y);
   hs_xy[0] = hc.x; hs_xy[1] = hc.y; hs_xy[2] = p.x; hs_xy[3] = p.y;
+}
+
+#define ORC_NGATEV 24
+/* listed gates' values and decision margins, for the test that names the gates whose codes may differ between the
+   GPU and this oracle. rg = n (radial, gate) pairs; gg = null, or the GPU's geometry of each listed gate (7 floats:
+   x, y, ex, ey, zc, hl, hu). out[n][24] = val[0..4] (Gate.val), mgeo, mecho, mwrap, mT, mzg, md0, mq, then the
+   five values again at the GPU's geometry (NaN without gg), then this oracle's own geometry (7 floats). */
+void orc_radar_gates(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2,
+                     const float* aux, const float* tor, int n, const int* rg, const float* gg, float* out) {
+  RU R;
+  memcpy(&R, ru, sizeof(R));
+  int nx = (int)R.mdl.x, ny = (int)R.mdl.y, nz = (int)R.mdl.z;
+  Ctx C = {&R, BS, {V, nx + 1, ny + 1, nz + 1, 4}, {Q1, nx, ny, nz, 4}, {Q2, nx, ny, nz, 4}, {aux, nx, ny, nz, 2}};
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int q = 0; q < n; q++) {
+    int ri = rg[2 * q], gi = rg[2 * q + 1];
+    float* o = out + (size_t)q * ORC_NGATEV;
+    Gate G = gate(&C, tor, ri, gi, NULL);
+    for (int k = 0; k < 5; k++) o[k] = G.val[k];
+    o[5] = G.mgeo; o[6] = G.mecho; o[7] = G.mwrap; o[8] = G.mT; o[9] = G.mzg; o[10] = G.md0; o[11] = G.mq;
+    for (int k = 0; k < 5; k++) o[12 + k] = NAN;
+    if (gg) {
+      Gate H = gate(&C, tor, ri, gi, gg + (size_t)q * 7);
+      for (int k = 0; k < 5; k++) o[12 + k] = H.val[k];
+    }
+    Geom g = geom(&C, ri, gi);
+    o[17] = g.p.x; o[18] = g.p.y; o[19] = g.e.x; o[20] = g.e.y; o[21] = g.zc; o[22] = g.hl; o[23] = g.hu;
+  }
 }

@@ -55,6 +55,7 @@ _SIGS = {
     "obl_radar_set_tornadoes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_radar_cut": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "obl_radar_cut_into": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "obl_radar_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "obl_radar_readback": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "obl_radar_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "obl_add_bubble": (C.c_int, [C.c_void_p] + [C.c_double] * 6),

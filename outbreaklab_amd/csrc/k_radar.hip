@@ -151,24 +151,45 @@ struct Gate {
   float kdp;
 };
 
+// a gate's geometry: position, radial direction (unit, along the beam), beam-centre height and the heights
+// 0.4 beam widths below and above it, all above the model's ground
+struct GateGeom {
+  float2 p, e;
+  float zc, hl, hu;
+};
+constexpr float BEAM_W = 0.0165f;  // ~0.95 degree half-power beam width
+
+__device__ __forceinline__ GateGeom gate_geom(const RadarCtx& C, int ri, int gi) {
+  const RadarU& R = C.R;
+  GateGeom g;
+  const float r = R.gates.z + R.gates.w * (float)gi;
+  const float el = R.cut.x;
+  const float az = (R.cut.y + R.cut.z * (float)ri) * 0.017453293f;
+  const float2 hc = C.beam(r, el);
+  g.p = C.gatePos(az, hc.y);
+  const float2 p2 = C.gatePos(az, hc.y + 200.0f);
+  float2 e = make_float2(p2.x - g.p.x, p2.y - g.p.y);
+  const float el_ = sqrtf(e.x * e.x + e.y * e.y);
+  g.e = make_float2(e.x / el_, e.y / el_);
+  const float zAnt = R.site.z - R.mdl2.y;  // antenna above the model's ground
+  g.zc = zAnt + hc.x;
+  // beam filling: three heights across the beam, Gaussian weights
+  g.hl = C.beam(r, el - 0.4f * BEAM_W).x + zAnt;
+  g.hu = C.beam(r, el + 0.4f * BEAM_W).x + zAnt;
+  return g;
+}
+
 __device__ Gate gate(const RadarCtx& C, const float4* tor, int ri, int gi) {
   const RadarU& R = C.R;
   Gate G{0u, 0u, 0u, 0u, 0u, -1.0f};
   const float r = R.gates.z + R.gates.w * (float)gi;
   const float el = R.cut.x;
-  const float az = (R.cut.y + R.cut.z * (float)ri) * 0.017453293f;
-  const float bw = 0.0165f;  // ~0.95 degree half-power beam width
-  const float2 hc = C.beam(r, el);
-  const float2 p = C.gatePos(az, hc.y);
-  const float2 p2 = C.gatePos(az, hc.y + 200.0f);
-  float2 e = make_float2(p2.x - p.x, p2.y - p.y);
-  const float el_ = sqrtf(e.x * e.x + e.y * e.y);
-  e = make_float2(e.x / el_, e.y / el_);
-  const float zAnt = R.site.z - R.mdl2.y;  // antenna above the model's ground
-  const float zc = zAnt + hc.x;
+  const float bw = BEAM_W;
+  const GateGeom gg = gate_geom(C, ri, gi);
+  const float2 p = gg.p, e = gg.e;
+  const float zc = gg.zc;
   if (zc > R.mdl.z * R.mdl2.x - 200.0f) return G;
-  // beam filling: three heights across the beam, Gaussian weights
-  const float hl = C.beam(r, el - 0.4f * bw).x + zAnt, hu = C.beam(r, el + 0.4f * bw).x + zAnt;
+  const float hl = gg.hl, hu = gg.hu;
   const Smp s0 = C.sample(p, fmaxf(zc, 5.0f)), s1 = C.sample(p, fmaxf(hl, 5.0f)), s2 = C.sample(p, fmaxf(hu, 5.0f));
   if (s0.inside < 0.5f) return G;
   const float zr = 0.5f * s0.zr + 0.25f * (s1.zr + s2.zr), zs = 0.5f * s0.zs + 0.25f * (s1.zs + s2.zs);
@@ -303,6 +324,22 @@ __global__ void __launch_bounds__(64) k_radar_phi(RadarU R, RadarArgs a) {
     const uint32_t w = b | (__shfl_down(b, 1) << 8) | (__shfl_down(b, 2) << 16) | (__shfl_down(b, 3) << 24);
     if ((lane & 3) == 0 && g < ng) dst[g >> 2] = w;  // ng is a multiple of 4: the whole word is in range
   }
+}
+
+// the geometry of listed gates (rg: n (radial, gate) pairs) as the cut kernel computes it: 7 floats per gate
+// (x, y, ex, ey, zc, hl, hu)
+__global__ void __launch_bounds__(64) k_radar_geom(RadarU R, int n, const int* rg, float* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const RadarCtx C{R, nullptr, Tex{nullptr, 0, 0, 0, 0}, Tex{nullptr, 0, 0, 0, 0}, Tex{nullptr, 0, 0, 0, 0},
+                   Tex{nullptr, 0, 0, 0, 0}};
+  const GateGeom g = gate_geom(C, rg[2 * q], rg[2 * q + 1]);
+  float* o = out + (size_t)q * 7;
+  o[0] = g.p.x; o[1] = g.p.y; o[2] = g.e.x; o[3] = g.e.y; o[4] = g.zc; o[5] = g.hl; o[6] = g.hu;
+}
+
+void launch_radar_geom(hipStream_t s, const RadarU& R, int n, const int* rg, float* out) {
+  if (n > 0) hipLaunchKernelGGL(k_radar_geom, dim3((n + 63) / 64), dim3(64), 0, s, R, n, rg, out);
 }
 
 void launch_radar_cut(hipStream_t s, const RadarU& R, const RadarArgs& a) {

@@ -1085,6 +1085,39 @@ extern "C" int obl_radar_cut_into(obl_model* h, const float* ru, int parity, uin
   return hip_ok(hipGetLastError(), &h->err, "radar cut launch");
 }
 
+extern "C" int obl_radar_geometry(obl_model* h, const float* ru, int n, const int* rg, float* out) {
+  if (!h) return set_global("null handle");
+  if (!ru || n < 0 || (n > 0 && (!rg || !out))) return h->fail("radar geometry: null uniform block, gates or output");
+  RadarU R;
+  memcpy(&R, ru, sizeof(R));
+  const long nrad = (long)R.cut.w, ngr = (long)R.gates.x;
+  for (int q = 0; q < n; q++)
+    if (rg[2 * q] < 0 || rg[2 * q] >= nrad || rg[2 * q + 1] < 0 || rg[2 * q + 1] >= ngr)
+      return h->fail("radar geometry: gate %d (%d, %d) is outside the cut's %ld x %ld", q, rg[2 * q], rg[2 * q + 1],
+                     nrad, ngr);
+  if (n == 0) return 0;
+  int* drg = nullptr;
+  float* dout = nullptr;
+  int rc = 0;
+  if (hip_ok(hipMalloc(&drg, (size_t)n * 2 * sizeof(int)), &h->err, "radar geometry: hipMalloc") ||
+      hip_ok(hipMalloc(&dout, (size_t)n * 7 * sizeof(float)), &h->err, "radar geometry: hipMalloc") ||
+      hip_ok(hipMemcpyAsync(drg, rg, (size_t)n * 2 * sizeof(int), hipMemcpyHostToDevice, h->stream), &h->err,
+             "radar geometry: upload")) {
+    rc = -1;
+  } else {
+    launch_radar_geom(h->stream, R, n, drg, dout);
+    rc = hip_ok(hipGetLastError(), &h->err, "radar geometry launch") ||
+                 hip_ok(hipMemcpyAsync(out, dout, (size_t)n * 7 * sizeof(float), hipMemcpyDeviceToHost, h->stream),
+                        &h->err, "radar geometry: download") ||
+                 hip_ok(hipStreamSynchronize(h->stream), &h->err, "radar geometry: sync")
+             ? -1
+             : 0;
+  }
+  if (drg) hipFree(drg);
+  if (dout) hipFree(dout);
+  return rc;
+}
+
 extern "C" int obl_radar_readback(obl_radar* r, int* slot) {
   if (!r) return set_global("null radar");
   obl_model* h = r->m;

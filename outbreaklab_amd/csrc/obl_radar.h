@@ -28,5 +28,8 @@ struct RadarArgs {
   float* kdp;         // scratch, nrad x Doppler gates
 };
 void launch_radar_cut(hipStream_t s, const RadarU& R, const RadarArgs& a);
+// the geometry of listed gates, as the cut computes it: rg = n (radial, gate) pairs, out = n x 7 floats
+// (x, y, ex, ey, zc, hl, hu)
+void launch_radar_geom(hipStream_t s, const RadarU& R, int n, const int* rg, float* out);
 
 }  // namespace obl

@@ -175,6 +175,17 @@ def cut_block(m, geo, R, c, simT, n_tor=0):
     return U.f.copy()
 
 
+def gate_geometry(m, ru, rg):
+    """The GPU's geometry of listed gates of a cut (obl_radar_geometry, a diagnostic): ru = the cut's block,
+    rg = (n, 2) (radial, gate); returns (n, 7) float32 = x, y, ex, ey, beam-centre height, heights below / above."""
+    rg = np.ascontiguousarray(rg, np.int32).reshape(-1, 2)
+    ru = np.ascontiguousarray(ru, np.float32)
+    out = np.zeros((len(rg), 7), np.float32)
+    _lib.check(_lib.lib().obl_radar_geometry(m.h, ru.ctypes.data_as(C.c_void_p), len(rg), rg.ctypes.data_as(C.c_void_p),
+                                             out.ctypes.data_as(C.c_void_p)), m.h)
+    return out
+
+
 class RadarSim:
     """RadarSim (radar.js:271-418) over `obl_radar_*`."""
 

@@ -301,6 +301,26 @@ def step_f64_split(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, a
       threads or min(8, os.cpu_count() or 1), int(part))
 
 
+def radar_gates(ru, BS, V, Q1, Q2, aux, tor, rg, gg=None):
+    """Listed gates' moment values before rounding and decision margins (orc_radar_gates): rg = (n, 2) int
+    (radial, gate), gg = None or the GPU's geometry of each (n, 7: x, y, ex, ey, zc, hl, hu); returns [n][24] =
+    REF, VEL, SW, ZDR, RHO values (v * scale + offset, NaN where not formed), the margins mgeo (m), mecho (dB),
+    mwrap (m/s), mT (K), mzg (relative), md0 (mm), mq (relative), the five values at the GPU's geometry (NaN
+    without gg), then the oracle's own geometry (7)."""
+    rg = np.ascontiguousarray(rg, np.int32).reshape(-1, 2)
+    out = np.zeros((len(rg), 24), np.float32)
+    if not len(rg):
+        return out
+    f = lib().orc_radar_gates
+    f.restype = None
+    a32 = lambda x: np.ascontiguousarray(x, np.float32)  # noqa: E731
+    t = a32(tor if tor is not None and len(tor) else np.zeros(8, np.float32))
+    g = None if gg is None else a32(gg).reshape(len(rg), 7)
+    f(P(a32(ru)), P(a32(BS)), P(a32(V)), P(a32(Q1)), P(a32(Q2)), P(a32(aux)), P(t), C.c_int(len(rg)), P(rg), P(g),
+      P(out))
+    return out
+
+
 def radar_cut(ru, BS, V, Q1, Q2, aux, tor, words, kdp_len, out=None):
     """One radar cut (radar.js:41-268) into a uint32 volume of ``words`` words (updated in place when
     ``out`` is given); returns (out, kdp)."""

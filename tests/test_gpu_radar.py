@@ -5,9 +5,10 @@ the domain centre, VCP 212 (17 cuts, all six moments), one sub-grid tornado with
 strongest low-level rotation. The oracle is fed the same downloaded state (V of the current parity,
 Q1[0], Q2[0], aux) and the same RADAR_FIELDS blocks. The 8-bit Archive II codes must match: the GPU's
 sin/cos/asin/atan2/pow differ from the C library's by an ulp or two, which moves a code by one where
-its value sits on a rounding boundary, flips an edge gate in or out of the grid, or wraps a velocity at
-the Nyquist. Bar, per moment of each cut, over its echo gates: <= 1 % off by one (VEL: <= 4 %) and
-<= 0.1 % (at least 2 gates) off by more."""
+its value sits on a rounding boundary, and moves the FP32 gate geometry by up to a metre (a code then
+differs where reflectivity or the radial wind changes fast). Bar, per moment of each cut, over its echo
+gates: <= 1 % off by one (VEL: <= 4 %) and <= 0.1 % (at least 2 gates) off by more; and every differing
+code explained, by its value's rounding or by the oracle evaluated at the GPU's own geometry of the gate."""
 import numpy as np
 import pytest
 
@@ -153,3 +154,124 @@ def test_upload_body_of_a_gpu_volume(volume):
     assert H == json.loads(json.dumps(vol["header"]))
     payload = np.frombuffer(body, np.uint8, offset=4 + hl)
     assert np.array_equal(payload, vol["bytes"])
+
+
+# Which gates may differ, and by how much. The GPU and the oracle compute every gate in FP32 with different
+# transcendentals (hardware exp2 / log2 for powers, the device library's sin / cos / asin / atan2 against the C
+# library's) and different FMA contraction, so a gate's values differ by a few ulp, and by more in the radial
+# direction e (the difference of two gate positions 200 m apart, radar.js:152: about 5e-5 relative). A code can then
+# differ only where the gate sits near one of its discrete boundaries; each bound below is that error budget with a
+# margin, in the boundary's own units. The gate geometry differs by more: the great-circle formulas
+# (radar.js:57-69) cancel in FP32 (the longitude difference, the y-difference of products), so a transcendental an
+# ulp apart moves a gate by up to a metre and turns the radial direction e (the difference of two positions 200 m
+# apart) by up to a few 1e-3 rad; near an echo edge a metre moves reflectivity by tenths of a dB, and the turn
+# moves VEL and SW by up to a tenth of a code at storm winds. Those gates are checked by evaluating the oracle at
+# the GPU's own geometry of the gate (obl_radar_geometry, the cut kernel's geometry code).
+ROUND_CODES = {"REF": 1e-3, "VEL": 0.02, "SW": 0.02, "ZDR": 2e-3, "RHO": 1e-3}  # from a rounding boundary k + 1/2
+MOM = {"REF": 0, "VEL": 1, "SW": 2, "ZDR": 3, "RHO": 4}
+NEAR = {"geometry": (5, 0.05), "echo threshold": (6, 1e-3), "Nyquist wrap": (7, 0.01), "melting level": (8, 2e-3),
+        "graupel switch": (9, 1e-5), "drop-size switch": (10, 1e-5), "Thompson size switch": (11, 1e-5)}
+
+
+def phidp_values(kdp, ng, dr, offset):
+    """The oracle's PHI values before rounding (radar.js:243-267, the same FP32 running sum), per gate, with
+    each gate's distance from a 360-degree wrap and the running sum itself."""
+    acc = np.float32(offset)
+    vals, wrap, accs = np.full(ng, np.nan, np.float32), np.full(ng, np.inf), np.zeros(ng)
+    for g in range(ng):
+        kd = kdp[g]
+        if kd >= 0:
+            acc = np.float32(acc + np.float32(np.float32(2.0) * kd * np.float32(dr)) * np.float32(0.001))
+            phi = np.float32(acc - np.float32(360.0) * np.floor(acc / np.float32(360.0)))
+            vals[g] = phi * np.float32(253.0) / np.float32(360.0) + np.float32(2.0)
+            wrap[g] = abs(acc / 360.0 - round(acc / 360.0)) * 360.0
+        accs[g] = acc
+    return vals, wrap, accs
+
+
+def code_of(v):
+    return int(np.clip(np.rint(v), 2, 255)) if np.isfinite(v) else 0
+
+
+def classify(name, gpu, g):
+    """The boundary a gate's differing code of moment ``name`` sits near (None: none within its bound); gpu = the
+    GPU's code, g = the gate's row of radar_gates (with the GPU's geometry)."""
+    k = MOM.get(name, 0)  # a PHI code exists exactly where REF does
+    if name in MOM and np.isfinite(g[k]):
+        x = float(g[k]) - 0.5
+        if abs(x - round(x)) <= ROUND_CODES[name]:
+            return "rounding"
+    vg = float(g[12 + k])
+    if name in MOM:
+        if gpu == code_of(vg):
+            return "geometry"  # the oracle at the GPU's own geometry gives the GPU's code
+        if np.isfinite(vg) and abs(vg - 0.5 - round(vg - 0.5)) <= ROUND_CODES[name]:
+            return "geometry"
+    elif (gpu == 0) == (not np.isfinite(vg)) and np.isfinite(g[0]) != np.isfinite(vg):
+        return "geometry"
+    for c, (i, lim) in NEAR.items():
+        if g[i] <= lim:
+            return c
+    return None
+
+
+def test_volume_codes_differ_only_near_a_boundary(volume):
+    """Every code the GPU's volume and the oracle's differ in is explained: the gate's moment value lies within the
+    rounding bound of a code boundary; or the oracle, evaluated at the GPU's own FP32 geometry of that gate
+    (obl_radar_geometry), gives the GPU's code (or a value within the rounding bound of its boundary); or the gate
+    lies within the stated bound of the echo threshold, a Nyquist wrap, the beam leaving the model, the melting
+    level or the graupel / drop-size / Thompson size switches (or, for PHI, a rounding or 360-degree wrap of the
+    running sum within its accumulated bound). Any other difference fails. On a sample of gates the GPU's geometry
+    lies within a few metres and a few 1e-3 rad of the oracle's (what the FP32 formulas allow)."""
+    from collections import Counter
+
+    from _oracle import radar_gates
+    from outbreaklab_amd.radar import cut_block, gate_geometry
+
+    m, geo, sim, R, tor, vol = volume
+    got = vol["bytes"]
+    V, Q1, Q2 = m.state()
+    aux = m.aux.download()
+    t = np.array([tor["x"], tor["y"], tor["R"], tor["v"], tor["debrisTop"], tor["debrisDbz"], tor["debris"], 0],
+                 np.float32)
+    rng = np.random.default_rng(7)
+    classes, ndiff, ncodes = Counter(), 0, 0
+    dgeo, same, nsamp = np.zeros(4), 0, 0
+    out = None
+    for c in R["cuts"]:
+        ru = cut_block(m, geo, R, c, 269.0, 1)
+        out, kdp = radar_cut(ru, m._B, V, Q1, Q2, aux, t, R["bytes"] // 4, 720 * R["ngDop"], out)
+        ref = out.view(np.uint8)
+        diffs = []
+        for o in c["mom"]:
+            ng, name = o["ngates"], o["name"]
+            a = got[o["off"]:o["off"] + c["nrad"] * ng].reshape(c["nrad"], ng)
+            b = ref[o["off"]:o["off"] + c["nrad"] * ng].reshape(c["nrad"], ng)
+            ncodes += a.size
+            diffs += [(name, int(ri), int(gi), int(a[ri, gi])) for ri, gi in zip(*np.nonzero(a != b))]
+        sample = np.stack([rng.integers(0, c["nrad"], 2000), rng.integers(0, R["ngRef"], 2000)], 1)
+        rg = np.concatenate([np.array([d[1:3] for d in diffs], np.int32).reshape(-1, 2), sample]).astype(np.int32)
+        gg = gate_geometry(m, ru, rg)
+        rows = radar_gates(ru, m._B, V, Q1, Q2, aux, t, rg, gg)
+        d = np.abs(gg - rows[:, 17:24])
+        turn = np.abs(gg[:, 2] * rows[:, 20] - gg[:, 3] * rows[:, 19])
+        dgeo = np.maximum(dgeo, [d[:, 0].max(), d[:, 1].max(), d[:, 4:7].max(), turn.max()])
+        same += int(np.count_nonzero(d[len(diffs):].max(axis=1) == 0))
+        nsamp += len(sample)
+        for (name, ri, gi, gpu), g in zip(diffs, rows):
+            ndiff += 1
+            why = classify(name, gpu, g)
+            if why is None and name == "PHI":
+                pv, pw, acc = phidp_values(kdp[ri * R["ngDop"]:(ri + 1) * R["ngDop"]], R["ngDop"],
+                                           float(ru[4 * 3 + 3]), float(ru[4 * 4 + 3]))
+                bound = 1e-5 * abs(acc[gi]) + 1e-4  # degrees: the running sum of ulp-different KDP
+                x = float(pv[gi]) - 0.5
+                if np.isfinite(pv[gi]) and abs(x - round(x)) <= bound * 253.0 / 360.0:
+                    why = "rounding"
+                elif pw[gi] <= bound:
+                    why = "Nyquist wrap"  # the 360-degree wrap of PHI
+            classes[why or "unexplained"] += 1
+    print(f"{ndiff} of {ncodes} codes differ:", dict(classes), "| GPU vs oracle geometry, largest |dx| |dy| |dz| (m),"
+          f" turn (rad): {dgeo}; sampled gates with identical geometry: {same} of {nsamp}")
+    assert classes["unexplained"] == 0, dict(classes)
+    assert dgeo[0] < 5.0 and dgeo[1] < 5.0 and dgeo[2] < 0.1 and dgeo[3] < 0.02, dgeo
