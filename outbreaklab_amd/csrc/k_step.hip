@@ -920,12 +920,11 @@ __device__ __forceinline__ float samp_mm(const Win2& a, float tx, float ty, floa
   float ax = __builtin_amdgcn_fractf(tx), ay = __builtin_amdgcn_fractf(ty), az = __builtin_amdgcn_fractf(tz);
   int dz;
   int b = a.base(floori_(tx), floori_(ty), floori_(tz), ch, dz);
-  float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + QW], c110 = a.s[b + QW + 1];
-  float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + QW], c111 = a.s[b + dz + QW + 1];
-  mm = make_float2(vmin3_(vmin3_(c000, c100, c010), vmin3_(c110, c001, c101), vmin_(c011, c111)),
-                   vmax3_(vmax3_(c000, c100, c010), vmax3_(c110, c001, c101), vmax_(c011, c111)));
-  float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
-  float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  const Box8 c = box8(a.s, b, dz, QW);
+  mm = make_float2(vmin3_(vmin3_(c.c000, c.c100, c.c010), vmin3_(c.c110, c.c001, c.c101), vmin_(c.c011, c.c111)),
+                   vmax3_(vmax3_(c.c000, c.c100, c.c010), vmax3_(c.c110, c.c001, c.c101), vmax_(c.c011, c.c111)));
+  float c00 = lerpf(c.c000, c.c100, ax), c10 = lerpf(c.c010, c.c110, ax);
+  float c01 = lerpf(c.c001, c.c101, ax), c11 = lerpf(c.c011, c.c111, ax);
   return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
 }
 // the same from global memory (clamped texels)
@@ -1110,12 +1109,12 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
     q1r.store(2, tid, a); q2r.store(2, tid, b);
   }
   __syncthreads();
-  for (int z = 0; z < d.nz; z++) {
+  // one level: corrects plane z from cur and predicts plane z + 1 into nxt
+  auto level = [&](int z, const Carry& cur, Carry& nxt) {
     Ring2::Pre p1, p2;
     const bool more = z + 1 < d.nz;
     // plane z + 3 for the prediction of plane z + 2 at the next level
     q1r.fetch(z + 3, tid, p1); q2r.fetch(z + 3, tid, p2);
-    Carry nxt;
     if (more) predict_plane(z + 1, nxt);
     __syncthreads();
     q1r.store(z + 3, tid, p1); q2r.store(z + 3, tid, p2);
@@ -1131,8 +1130,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
       stx(Q1o, i, o1);
       stx(Q2o, i, o2);
     }
-    if (more) cur = nxt;
     __syncthreads();
+  };
+  // two levels per trip, the carries swapping roles (no copy of a carry between levels)
+  Carry alt;
+  for (int z = 0; z < d.nz; z += 2) {
+    level(z, cur, alt);
+    if (z + 1 < d.nz) level(z + 1, alt, cur);
   }
 }
 

@@ -214,6 +214,23 @@ __device__ __noinline__ static float2 minmax1_global(const float4* g4, int W, in
   return make_float2(lo, hi);
 }
 
+// The LDS address of a texel box's first texel, made opaque to the compiler (an empty asm, no instruction): the
+// box's other reads then carry their one-texel and one-row offsets in the read instructions themselves (two
+// ds_read2_b32 per plane from one address), instead of one address add per pair of reads.
+__device__ __forceinline__ const lds_float* lds_box(const lds_float* p) {
+  asm("" : "+v"(p));
+  return p;
+}
+// the eight texels of a box: rows of `row` floats, the upper plane dz floats above the lower
+struct Box8 {
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+};
+__device__ __forceinline__ Box8 box8(const lds_float* s, int b, int dz, int row) {
+  const lds_float* p = lds_box(s + b);
+  const lds_float* q = p + dz;
+  return Box8{p[0], p[1], p[row], p[row + 1], q[0], q[1], q[row], q[row + 1]};
+}
+
 // Window-guaranteed forms: the 2 x 2 x 2 texel box of the sample lies in the window, so
 // k0 is zc - 1 or zc.
 __device__ __forceinline__ float samp1(const RingW& a, float tx, float ty, float tz, int ch) {
@@ -222,20 +239,18 @@ __device__ __forceinline__ float samp1(const RingW& a, float tx, float ty, float
   bool lo = k0 < a.zc;
   int b = (lo ? a.om : a.oc) + ch * TPS + __mul24(j0 - a.y0 + 1, TPW) + (i0 - a.x0 + 1);
   int dz = lo ? a.oc - a.om : a.op - a.oc;
-  float c000 = a.s[b], c100 = a.s[b + 1], c010 = a.s[b + TPW], c110 = a.s[b + TPW + 1];
-  float c001 = a.s[b + dz], c101 = a.s[b + dz + 1], c011 = a.s[b + dz + TPW], c111 = a.s[b + dz + TPW + 1];
-  float c00 = lerpf(c000, c100, ax), c10 = lerpf(c010, c110, ax);
-  float c01 = lerpf(c001, c101, ax), c11 = lerpf(c011, c111, ax);
+  const Box8 c = box8(a.s, b, dz, TPW);
+  float c00 = lerpf(c.c000, c.c100, ax), c10 = lerpf(c.c010, c.c110, ax);
+  float c01 = lerpf(c.c001, c.c101, ax), c11 = lerpf(c.c011, c.c111, ax);
   return lerpf(lerpf(c00, c10, ay), lerpf(c01, c11, ay), az);
 }
 __device__ __forceinline__ float2 minmax1(const RingW& a, int i0, int j0, int k0, int ch) {
   bool lo = k0 < a.zc;
   int b = (lo ? a.om : a.oc) + ch * TPS + __mul24(j0 - a.y0 + 1, TPW) + (i0 - a.x0 + 1);
   int dz = lo ? a.oc - a.om : a.op - a.oc;
-  float v0 = a.s[b], v1 = a.s[b + 1], v2 = a.s[b + TPW], v3 = a.s[b + TPW + 1];
-  float v4 = a.s[b + dz], v5 = a.s[b + dz + 1], v6 = a.s[b + dz + TPW], v7 = a.s[b + dz + TPW + 1];
-  return make_float2(vmin3_(vmin3_(v0, v1, v2), vmin3_(v3, v4, v5), vmin_(v6, v7)),
-                     vmax3_(vmax3_(v0, v1, v2), vmax3_(v3, v4, v5), vmax_(v6, v7)));
+  const Box8 c = box8(a.s, b, dz, TPW);
+  return make_float2(vmin3_(vmin3_(c.c000, c.c100, c.c010), vmin3_(c.c110, c.c001, c.c101), vmin_(c.c011, c.c111)),
+                     vmax3_(vmax3_(c.c000, c.c100, c.c010), vmax3_(c.c110, c.c001, c.c101), vmax_(c.c011, c.c111)));
 }
 
 // Trilinear, clamp-to-edge sample of one channel at texel coordinates (t - 1/2 convention
