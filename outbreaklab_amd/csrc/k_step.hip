@@ -398,6 +398,16 @@ __device__ __forceinline__ Fall fall_speeds(bool thm, float4 q1, float4 q2, floa
   float2 fv = vtRain(thm, q1.w, q2.z, rho, rsfc);
   return Fall{fv.x, thm ? fv.y : 0.0f, vtS(q2.x, rho, rsfc), vtGrau(thm, q2.y, rho, rsfc)};
 }
+// A cell whose hydrometeors all lie below the clamps of their fall-speed laws falls as a clear cell: the same
+// function of the level's density alone. vtS, vtG and vtGT clamp 1000 rho q or rho q from below, lamR clamps q and
+// Nr, vtR is 0 for q <= 0, and vtG's 1 + (q / 0.009)^2 rounds to exactly 1 there. The tests below are the laws'
+// own clamped expressions, so such a cell's speeds equal the clear cell's bit for bit.
+__device__ __forceinline__ bool fall_clear(bool thm, float4 q1, float4 q2, float rho) {
+  const bool s = 1000.0f * rho * q2.x <= 1e-6f;
+  if (thm) return s && q1.w <= 1e-12f && q2.z <= 1e-3f && rho * q2.y <= 1e-12f;
+  return s && q1.w <= 0.0f && 1000.0f * rho * q2.y <= 1e-6f;
+}
+constexpr int FALL_TAB = 128;  // levels of a block's table of clear-cell fall speeds
 // as near_traces, for cell-centred traces with per-species fall speeds: |k_z (w - f)| is largest
 // at the extreme f, so the bound over f in [min(0, f...), max(0, f...)] covers every species
 __device__ __forceinline__ bool near_traces_q(const Dims& d, float3 v, const Fall& f) {
@@ -1045,9 +1055,29 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
                                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
                                                          float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
   __shared__ float lds[2 * QRING_FLOATS + 2 * RING_FLOATS];
+  __shared__ float4 ftab[FALL_TAB];  // per level: a clear cell's fall speeds (r, n, s, g)
   TILE_SETUP
   const bool thm = u.v[U_MICRO2].z > 0.5f;
   const float rsfc = u.v[U_STEP].z;
+  {
+    // computed at run time from zeros the compiler cannot see, so the table holds what the hardware's exp2 / log2
+    // give a clear cell (read after the prologue's barrier)
+    float zq = 0.0f;
+    asm("" : "+v"(zq));
+    const float4 z4 = make_float4(zq, zq, zq, zq);
+    for (int k = tid; k < min(d.nz, FALL_TAB); k += TILE_THREADS) {
+      const Fall f = fall_speeds(thm, z4, z4, bsA(BS, k, d.nz).z, rsfc);
+      ftab[k] = make_float4(f.r, f.n, f.s, f.g);
+    }
+  }
+  // a cell's fall speeds: the level's table entry when the whole wave is clear, else the laws
+  auto fall_at = [&](float4 q1, float4 q2, float rho, int k) {
+    if (k < FALL_TAB && wave_all(fall_clear(thm, q1, q2, rho))) {
+      const float4 t = ftab[k];
+      return Fall{t.x, t.y, t.z, t.w};
+    }
+    return fall_speeds(thm, q1, q2, rho, rsfc);
+  };
   Ring2 q1r{lds, Q1, d.nx, d.ny, d.nz, x0, y0};
   Ring2 q2r{lds + QRING_FLOATS, Q2, d.nx, d.ny, d.nz, x0, y0};
   // the predicted planes (one-point halo): written by the predictor, read through RingW views
@@ -1075,7 +1105,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
       cr.q1 = w1.ld4(cx1, cy1);
       cr.q2 = w2.ld4(cx1, cy1);
       cr.v = cellVel(gv, cx1, cy1, kp);
-      cr.f = fall_speeds(thm, cr.q1, cr.q2, rho, rsfc);
+      cr.f = fall_at(cr.q1, cr.q2, rho, kp);
       cr.near = near_traces_q(d, cr.v, cr.f);
       if (wave_all(cr.near)) predict(d, thm, w1, w2, cr.v, cr.f, cx1, cy1, kp, pr);
       else predict(d, thm, g1, g2, cr.v, cr.f, cx1, cy1, kp, pr);
@@ -1087,7 +1117,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
     if (tid < NHALO) {
       const float4 q1 = w1.ld4(cx2, cy2), q2 = w2.ld4(cx2, cy2);
       const float3 v = cellVel(gv, cx2, cy2, kp);
-      const Fall f = fall_speeds(thm, q1, q2, rho, rsfc);
+      const Fall f = fall_at(q1, q2, rho, kp);
       if (wave_all(near_traces_q(d, v, f))) predict(d, thm, w1, w2, v, f, cx2, cy2, kp, pr);
       else predict(d, thm, g1, g2, v, f, cx2, cy2, kp, pr);
       H1[hal] = pr.h1.x; H1[TPS + hal] = pr.h1.y; H1[2 * TPS + hal] = pr.h1.z; H1[3 * TPS + hal] = pr.h1.w;
