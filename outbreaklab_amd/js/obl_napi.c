@@ -70,6 +70,7 @@ typedef struct {
   int (*m_get_params)(obl_mlp*, float*, int);
   int (*m_set_scales)(obl_mlp*, const float*, const float*, const float*);
   int (*m_set_dead_zone)(obl_mlp*, float);
+  int (*m_set_hand_min)(obl_mlp*, float);
   int (*m_load_rows)(obl_mlp*, const float*, size_t, int);
   int (*m_train)(obl_mlp*, const float*, size_t, int, int, int, float, unsigned, int, float*);
   int (*m_infer)(obl_mlp*, const float*, int, int, float*);
@@ -150,6 +151,7 @@ static int load_api(const char* path, char* err, size_t n) {
   SYM(m_get_params, "obl_mlp_get_params");
   SYM(m_set_scales, "obl_mlp_set_scales");
   SYM(m_set_dead_zone, "obl_mlp_set_dead_zone");
+  SYM(m_set_hand_min, "obl_mlp_set_hand_min");
   SYM(m_load_rows, "obl_mlp_load_rows");
   SYM(m_train, "obl_mlp_train");
   SYM(m_infer, "obl_mlp_infer");
@@ -713,6 +715,12 @@ static napi_value js_mlp_set_dead_zone(napi_env env, napi_callback_info info) {
   obl_mlp* m = ext(env, argv[0]);
   return mlp_ok(env, A.m_set_dead_zone(m, (float)num(env, argv[1])), m);
 }
+static napi_value js_mlp_set_hand_min(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  obl_mlp* m = ext(env, argv[0]);
+  return mlp_ok(env, A.m_set_hand_min(m, (float)num(env, argv[1])), m);
+}
 static napi_value js_mlp_load_rows(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   get_args(env, info, 3, argv);
@@ -846,6 +854,7 @@ static napi_value init(napi_env env, napi_value exports) {
   EXPORT("mlpParams", js_mlp_params);
   EXPORT("mlpSetScales", js_mlp_set_scales);
   EXPORT("mlpSetDeadZone", js_mlp_set_dead_zone);
+  EXPORT("mlpSetHandMin", js_mlp_set_hand_min);
   EXPORT("mlpLoadRows", js_mlp_load_rows);
   EXPORT("mlpTrain", js_mlp_train);
   EXPORT("mlpInfer", js_mlp_infer);

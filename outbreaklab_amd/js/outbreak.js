@@ -352,6 +352,8 @@ class Surrogate {
   params() { const p = new Float32Array(this.n); addon.mlpParams(this.h, p, false); return p; }
   setParams(p) { addon.mlpParams(this.h, Float32Array.from(p), true); }
   setDeadZone(frac) { addon.mlpSetDeadZone(this.h, frac); }
+  // the least total condensate (kg/kg) of a point handed to the network; thinner cloud runs the plain block
+  setHandMin(q) { addon.mlpSetHandMin(this.h, q); }
   // the network's targets: the Nr output as log((Nr + dNr + 1) / (Nr + 1)); scales (RMS of each target)
   // and the input envelope (min / max) from the rows, as the Python host's `fit`
   static targets(rows) {
