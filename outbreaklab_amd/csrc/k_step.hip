@@ -1322,13 +1322,14 @@ __global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, 
   };
 
   vr.load(-1, tid); vr.load(0, tid); vr.load(1, tid);
-  // the cell's Q1 texel for nut, loaded one level ahead
-  float4 q1n = cell ? ldQ(Q1, d, x, y, 0) : make_float4(0.f, 0.f, 0.f, 0.f);
+  // the cell's Q1 texel for nut, loaded one level ahead (ldQ clamps into the field; lanes past the grid's cells
+  // load a clamped texel they do not use: unconditional loads, no select of addresses)
+  float4 q1n = ldQ(Q1, d, x, y, 0);
   __syncthreads();
   CarryV cur;
   {
     const float4 q1 = q1n;
-    q1n = cell ? ldQ(Q1, d, x, y, 1) : q1;
+    q1n = ldQ(Q1, d, x, y, 1);
     predict_plane(0, q1, cur);
   }
   __syncthreads();
@@ -1338,13 +1339,13 @@ __global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, 
     vr.store(2, tid, a);
   }
   __syncthreads();
-  for (int z = 0; z <= d.nz; z++) {
+  // one face level: corrects plane z from cur and predicts plane z + 1 into nxt, with the cells' Q1 of level
+  // z + 1 (q1); returns level z + 2's, loaded meanwhile
+  auto level = [&](int z, const CarryV& cur, CarryV& nxt, float4 q1) {
     Ring2::Pre pv;
     const bool more = z < d.nz;
     vr.fetch(z + 3, tid, pv);
-    const float4 q1 = q1n;
-    q1n = cell ? ldQ(Q1, d, x, y, z + 2) : q1;
-    CarryV nxt;
+    const float4 q1next = ldQ(Q1, d, x, y, z + 2);
     if (more) predict_plane(z + 1, q1, nxt);
     __syncthreads();
     vr.store(z + 3, tid, pv);
@@ -1357,8 +1358,14 @@ __global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, 
       else o = correct_v<false>(u, d, hr.w(), cur, V, x, y, z);
       stx(Vout, vidx(d, x, y, z), o);
     }
-    if (more) cur = nxt;
     __syncthreads();
+    return q1next;
+  };
+  // two levels per trip, the carries swapping roles (no carry copies between levels)
+  CarryV alt;
+  for (int z = 0; z <= d.nz; z += 2) {
+    const float4 q1b = level(z, cur, alt, q1n);
+    if (z + 1 <= d.nz) q1n = level(z + 1, alt, cur, q1b);
   }
 }
 
