@@ -952,15 +952,26 @@ struct Pred {
   float4 h1, h2;
   float2 mm[8];
 };
+// Channels 0-3 are Q1's, 4-7 Q2's. `clear` (wave-uniform) says that channels 1-7 are +0 at every texel the wave's
+// samples can reach: a trilinear sample of +0 texels is +0 and their min / max (+0, +0), so those channels are set to
+// exactly that without their traces or their reads, and only vapour is sampled.
 template <class A>
 __device__ __forceinline__ void predict(const Dims& d, bool thm, const A& Q1, const A& Q2, float3 v, const Fall& f,
-                                        int x, int y, int z, Pred& o) {
+                                        int x, int y, int z, bool clear, Pred& o) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
-  float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f), pr = traceQ(d, k, p, v, f.r, 1.0f);
-  float3 ps = traceQ(d, k, p, v, f.s, 1.0f), pg = traceQ(d, k, p, v, f.g, 1.0f), pn = traceQ(d, k, p, v, f.n, 1.0f);
+  float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
   float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
   o.h1.x = samp_mm(Q1, tx, ty, tz, 0, o.mm[0]);
+  if (clear) {
+    o.h1 = make_float4(o.h1.x, 0.0f, 0.0f, 0.0f);
+    o.h2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int ch = 1; ch < 8; ch++) o.mm[ch] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  float3 pr = traceQ(d, k, p, v, f.r, 1.0f);
+  float3 ps = traceQ(d, k, p, v, f.s, 1.0f), pg = traceQ(d, k, p, v, f.g, 1.0f), pn = traceQ(d, k, p, v, f.n, 1.0f);
   o.h1.y = samp_mm(Q1, tx, ty, tz, 1, o.mm[1]);
   o.h1.z = samp_mm(Q1, tx, ty, tz, 2, o.mm[2]);
   o.h1.w = samp_mm(Q1, pr.x - 0.5f, pr.y - 0.5f, pr.z - 0.5f, 3, o.mm[3]);
@@ -970,6 +981,21 @@ __device__ __forceinline__ void predict(const Dims& d, bool thm, const A& Q1, co
   o.h2.z = samp_mm(Q2, pn.x - 0.5f, pn.y - 0.5f, pn.z - 0.5f, 2, o.mm[6]);
   o.h2.w = samp_mm(Q2, tx, ty, tz, 3, o.mm[7]);
   (void)thm;
+}
+// channel bits (as `live`) of the texels of a float4 that are not +0
+__device__ __forceinline__ uint32_t nz_bits(float4 v) {
+  return (__float_as_uint(v.x) != 0u ? 1u : 0u) | (__float_as_uint(v.y) != 0u ? 2u : 0u) |
+         (__float_as_uint(v.z) != 0u ? 4u : 0u) | (__float_as_uint(v.w) != 0u ? 8u : 0u);
+}
+// OR of an 8-bit mask over the wave (every lane active), wave-uniform
+__device__ __forceinline__ uint32_t wave_or8(uint32_t m) {
+  uint32_t r = 0;
+  if (__ballot(m != 0u)) {
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (__ballot((m >> c) & 1u)) r |= 1u << c;
+  }
+  return r;
 }
 
 // the predicted value of channel ch (0-3 of Q1t, 4-7 of Q2t) at texel (i, j, k) (clamped), from global memory
@@ -1006,34 +1032,45 @@ struct Carry {
   float3 v;       // its cell velocity
   Fall f;
   bool near;
+  uint32_t liveq;  // the channels not +0 somewhere in the Q window of the cell's plane (wave-uniform)
   float2 mm[8];
+  // the face texels of the cell velocities of a later plane (own cell and halo slot), loaded a level ahead
+  CellVelTexels pv1, pv2;
 };
 
-// the corrector of one cell (model.js:532-576) from the predicted ring (halo 1) or, for far traces, recomputed
+// the corrector of one cell (model.js:532-576) from the predicted ring (halo 1) or, for far traces, recomputed.
+// On the near path `clear` (wave-uniform) says that channels 1-7 are +0 in the Q window of the plane and in the
+// three predicted planes around it: the cell's texel, its predicted value, the forward sample and the limiter's
+// bounds are then all +0, and so is the corrected value, which is set without computing it.
 template <bool NEAR>
 __device__ __forceinline__ void correct(const MU& u, const Dims& d, const RingW& w1t, const RingW& w2t, const Carry& c,
                                         const float4* BS, const float4* V, const float4* Q1, const float4* Q2, int x,
-                                        int y, int z, float4& o1, float4& o2) {
+                                        int y, int z, bool clear, float4& o1, float4& o2) {
   float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   const float3 v = c.v;
   float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
   float cour = sqrtf(cx * cx + cy * cy + cz * cz);
   float sm = 0.5f * u.v[U_MISC].y * clampf_(cour / 0.35f, u.v[U_DZV].w, 1.0f);
-  float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f), fr = traceQ(d, k, p, v, c.f.r, -1.0f);
+  float3 f0 = traceQ(d, k, p, v, 0.0f, -1.0f);
+  auto samp = [&](float3 at, int ch) {
+    float tx = at.x - 0.5f, ty = at.y - 0.5f, tz = at.z - 0.5f;
+    if (NEAR) return ch < 4 ? samp1(w1t, tx, ty, tz, ch) : samp1(w2t, tx, ty, tz, ch - 4);
+    return qhat_sample_global(u, d, BS, V, Q1, Q2, tx, ty, tz, ch);
+  };
+  const float4 h1 = w1t.ld4(x, y, z), q1 = c.q1;
+  if (NEAR && clear) {
+    o1 = make_float4(vmax_(vclamp_(h1.x + sm * (q1.x - samp(f0, 0)), c.mm[0]), 0.0f), 0.0f, 0.0f, 0.0f);
+    o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  float3 fr = traceQ(d, k, p, v, c.f.r, -1.0f);
   float3 fs = traceQ(d, k, p, v, c.f.s, -1.0f), fg = traceQ(d, k, p, v, c.f.g, -1.0f), fn = traceQ(d, k, p, v, c.f.n, -1.0f);
   float b[8];
   const float3 at[8] = {f0, f0, f0, fr, fs, fg, fn, f0};
 #pragma unroll
-  for (int ch = 0; ch < 8; ch++) {
-    float tx = at[ch].x - 0.5f, ty = at[ch].y - 0.5f, tz = at[ch].z - 0.5f;
-    if (NEAR)
-      b[ch] = ch < 4 ? samp1(w1t, tx, ty, tz, ch) : samp1(w2t, tx, ty, tz, ch - 4);
-    else
-      b[ch] = qhat_sample_global(u, d, BS, V, Q1, Q2, tx, ty, tz, ch);
-  }
-  const float4 h1 = w1t.ld4(x, y, z), h2 = w2t.ld4(x, y, z);
-  const float4 q1 = c.q1, q2 = c.q2;
+  for (int ch = 0; ch < 8; ch++) b[ch] = samp(at[ch], ch);
+  const float4 h2 = w2t.ld4(x, y, z), q2 = c.q2;
   float r[8] = {h1.x + sm * (q1.x - b[0]), h1.y + sm * (q1.y - b[1]), h1.z + sm * (q1.z - b[2]), h1.w + sm * (q1.w - b[3]),
                 h2.x + sm * (q2.x - b[4]), h2.y + sm * (q2.y - b[5]), h2.z + sm * (q2.z - b[6]), h2.w + sm * (q2.w - b[7])};
 #pragma unroll
@@ -1056,9 +1093,14 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
                                                          float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
   __shared__ float lds[2 * QRING_FLOATS + 2 * RING_FLOATS];
   __shared__ float4 ftab[FALL_TAB];  // per level: a clear cell's fall speeds (r, n, s, g)
+  // per ring slot and wave: the channels (Q1 bits 0-3, Q2 bits 4-7) not +0 somewhere in the texels the wave stored
+  // into the Q rings' plane (tile and two-point halo) / the predicted rings' plane (tile and one-point halo)
+  __shared__ __attribute__((aligned(16))) uint32_t nzq[3][4], nzh[3][4];
   TILE_SETUP
   const bool thm = u.v[U_MICRO2].z > 0.5f;
   const float rsfc = u.v[U_STEP].z;
+  const int wave = tid >> 6;
+  const bool lane0 = (tid & 63) == 0;
   {
     // computed at run time from zeros the compiler cannot see, so the table holds what the hardware's exp2 / log2
     // give a clear cell (read after the prologue's barrier)
@@ -1093,69 +1135,113 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
   const int cx2 = clampi(x0 - 1 + hsi, 0, d.nx - 1), cy2 = clampi(y0 - 1 + hsj, 0, d.ny - 1);
   const int own = __mul24(tid / TX + 1, TPW) + tid % TX + 1, hal = __mul24(hsj, TPW) + hsi;
 
-  // predicts plane kp (0..nz-1) into the predicted ring, from the Q rings centred at kp; returns this thread's
-  // own cell's carry
+  // the Q rings' planes: stored with their channel masks
+  auto store_q = [&](int k, const Ring2::Pre& a, const Ring2::Pre& b) {
+    q1r.store(k, tid, a);
+    q2r.store(k, tid, b);
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < (QS + TILE_THREADS - 1) / TILE_THREADS; q++) m |= nz_bits(a.v[q]) | (nz_bits(b.v[q]) << 4);
+    m = wave_or8(m);
+    if (lane0) nzq[Ring2::slot(k)][wave] = m;
+  };
+  // the channels not +0 somewhere in plane k of a ring (its slot's four wave words)
+  auto nzw = [&](const uint32_t (*nz)[4], int k) {
+    const uint4 w = *reinterpret_cast<const uint4*>(nz[Ring::slot(k)]);
+    return w.x | w.y | w.z | w.w;
+  };
+
+  // predicts plane kp (0..nz-1) into the predicted ring, from the Q rings centred at kp and the cell-velocity
+  // texels already in cr; returns this thread's own cell's carry
   auto predict_plane = [&](int kp, Carry& cr) {
     const Win2 w1 = win2(q1r, kp), w2 = win2(q2r, kp);
     const float rho = bsA(BS, kp, d.nz).z;
     float* H1 = h1r.s + Ring::slot(kp) * 4 * TPS;
     float* H2 = h2r.s + Ring::slot(kp) * 4 * TPS;
+    // the channels not +0 somewhere in the Q window of plane kp (planes kp - 1 .. kp + 1, two-point halo)
+    const uint32_t liveq = __builtin_amdgcn_readfirstlane(nzw(nzq, kp - 1) | nzw(nzq, kp) | nzw(nzq, kp + 1));
+    // clear air: vapour the only channel (qv) not +0 in the window
+    const bool clear = (liveq & ~0x01u) == 0u;
     Pred pr;
+    uint32_t mh;
     {
       cr.q1 = w1.ld4(cx1, cy1);
       cr.q2 = w2.ld4(cx1, cy1);
-      cr.v = cellVel(gv, cx1, cy1, kp);
+      cr.v = cellVelOf(cr.pv1);
       cr.f = fall_at(cr.q1, cr.q2, rho, kp);
       cr.near = near_traces_q(d, cr.v, cr.f);
-      if (wave_all(cr.near)) predict(d, thm, w1, w2, cr.v, cr.f, cx1, cy1, kp, pr);
-      else predict(d, thm, g1, g2, cr.v, cr.f, cx1, cy1, kp, pr);
+      cr.liveq = liveq;
+      if (wave_all(cr.near)) predict(d, thm, w1, w2, cr.v, cr.f, cx1, cy1, kp, clear, pr);
+      else predict(d, thm, g1, g2, cr.v, cr.f, cx1, cy1, kp, false, pr);
       H1[own] = pr.h1.x; H1[TPS + own] = pr.h1.y; H1[2 * TPS + own] = pr.h1.z; H1[3 * TPS + own] = pr.h1.w;
       H2[own] = pr.h2.x; H2[TPS + own] = pr.h2.y; H2[2 * TPS + own] = pr.h2.z; H2[3 * TPS + own] = pr.h2.w;
+      mh = nz_bits(pr.h1) | (nz_bits(pr.h2) << 4);
 #pragma unroll
       for (int q = 0; q < 8; q++) cr.mm[q] = pr.mm[q];
     }
     if (tid < NHALO) {
       const float4 q1 = w1.ld4(cx2, cy2), q2 = w2.ld4(cx2, cy2);
-      const float3 v = cellVel(gv, cx2, cy2, kp);
+      const float3 v = cellVelOf(cr.pv2);
       const Fall f = fall_at(q1, q2, rho, kp);
-      if (wave_all(near_traces_q(d, v, f))) predict(d, thm, w1, w2, v, f, cx2, cy2, kp, pr);
-      else predict(d, thm, g1, g2, v, f, cx2, cy2, kp, pr);
+      if (wave_all(near_traces_q(d, v, f))) predict(d, thm, w1, w2, v, f, cx2, cy2, kp, clear, pr);
+      else predict(d, thm, g1, g2, v, f, cx2, cy2, kp, false, pr);
       H1[hal] = pr.h1.x; H1[TPS + hal] = pr.h1.y; H1[2 * TPS + hal] = pr.h1.z; H1[3 * TPS + hal] = pr.h1.w;
       H2[hal] = pr.h2.x; H2[TPS + hal] = pr.h2.y; H2[2 * TPS + hal] = pr.h2.z; H2[3 * TPS + hal] = pr.h2.w;
+      mh |= nz_bits(pr.h1) | (nz_bits(pr.h2) << 4);
     }
+    mh = wave_or8(mh);
+    if (lane0) nzh[Ring::slot(kp)][wave] = mh;
   };
 
   // prologue: Q planes -1, 0, 1 (clamped), then plane 0's prediction
-  q1r.load(-1, tid); q1r.load(0, tid); q1r.load(1, tid);
-  q2r.load(-1, tid); q2r.load(0, tid); q2r.load(1, tid);
+  {
+    Ring2::Pre a, b;
+#pragma unroll
+    for (int k = -1; k <= 1; k++) {
+      q1r.fetch(k, tid, a); q2r.fetch(k, tid, b);
+      store_q(k, a, b);
+    }
+  }
+  Carry cur, alt;
+  auto load_vel = [&](int k, Carry& cr) {
+    cr.pv1 = cellVelTexels(gv, cx1, cy1, k);
+    if (tid < NHALO) cr.pv2 = cellVelTexels(gv, cx2, cy2, k);
+  };
+  load_vel(0, cur);
+  load_vel(1, alt);
   __syncthreads();
-  Carry cur;
   predict_plane(0, cur);
   __syncthreads();
   {
     // plane 2 replaces plane -1 (slot(2) == slot(-1)): the prediction of plane 1 needs planes 0..2
     Ring2::Pre a, b;
     q1r.fetch(2, tid, a); q2r.fetch(2, tid, b);
-    q1r.store(2, tid, a); q2r.store(2, tid, b);
+    store_q(2, a, b);
   }
   __syncthreads();
   // one level: corrects plane z from cur and predicts plane z + 1 into nxt
-  auto level = [&](int z, const Carry& cur, Carry& nxt) {
+  auto level = [&](int z, Carry& cur, Carry& nxt) {
     Ring2::Pre p1, p2;
     const bool more = z + 1 < d.nz;
-    // plane z + 3 for the prediction of plane z + 2 at the next level
+    // plane z + 3 for the prediction of plane z + 2 at the next level, and that prediction's cell velocities
+    // (into cur, which the next level refills)
     q1r.fetch(z + 3, tid, p1); q2r.fetch(z + 3, tid, p2);
+    load_vel(z + 2, cur);
     if (more) predict_plane(z + 1, nxt);
     __syncthreads();
-    q1r.store(z + 3, tid, p1); q2r.store(z + 3, tid, p2);
+    store_q(z + 3, p1, p2);
     // the predicted ring centred at z: below the ground and above the lid the clamped planes 0 and nz - 1
     h1r.center(z); h2r.center(z);
     if (z == 0) { h1r.om = h1r.oc; h2r.om = h2r.oc; }
     if (!more) { h1r.op = h1r.oc; h2r.op = h2r.oc; }
+    // the channels not +0 somewhere in the cell's Q window or in the predicted planes its corrector reads
+    const uint32_t live = __builtin_amdgcn_readfirstlane(
+        cur.liveq | nzw(nzh, max(z - 1, 0)) | nzw(nzh, z) | nzw(nzh, min(z + 1, d.nz - 1)));
     if (act) {
       float4 o1, o2;
-      if (wave_all(cur.near)) correct<true>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, o1, o2);
-      else correct<false>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, o1, o2);
+      if (wave_all(cur.near))
+        correct<true>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, (live & ~0x01u) == 0u, o1, o2);
+      else correct<false>(u, d, h1r.w(), h2r.w(), cur, BS, V, Q1, Q2, x, y, z, false, o1, o2);
       const uint32_t i = cidx(d, x, y, z);
       stx(Q1o, i, o1);
       stx(Q2o, i, o2);
@@ -1163,7 +1249,6 @@ __global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __r
     __syncthreads();
   };
   // two levels per trip, the carries swapping roles (no copy of a carry between levels)
-  Carry alt;
   for (int z = 0; z < d.nz; z += 2) {
     level(z, cur, alt);
     if (z + 1 < d.nz) level(z + 1, alt, cur);
