@@ -94,7 +94,8 @@ enum {
 enum {
   OBL_THOMAS_FP32 = 0,  /* the reference's FP32 Thomas (solver.js:86-101) */
   OBL_THOMAS_FP64 = 1,  /* FP64 arithmetic for every (l, m) */
-  OBL_THOMAS_MIXED = 2  /* FP64 only where the vertical system is near-singular */
+  OBL_THOMAS_MIXED = 2  /* FP64 where the vertical system's condition estimate 4 / (|lx + ly| dz^2) exceeds
+                           fp64_cond (model default 64); the other modes in FP32 through one reciprocal per level */
 };
 
 /* ---------------- errors and version ---------------- */

@@ -344,11 +344,12 @@ void orc_radar_geom(const float* ru, int ri, int gi, float* hs_xy) {
   hs_xy[0] = hc.x; hs_xy[1] = hc.y; hs_xy[2] = p.x; hs_xy[3] = p.y;
 }
 
-#define ORC_NGATEV 24
+#define ORC_NGATEV 26
 /* listed gates' values and decision margins, for the test that names the gates whose codes may differ between the
    GPU and this oracle. rg = n (radial, gate) pairs; gg = null, or the GPU's geometry of each listed gate (7 floats:
-   x, y, ex, ey, zc, hl, hu). out[n][24] = val[0..4] (Gate.val), mgeo, mecho, mwrap, mT, mzg, md0, mq, then the
-   five values again at the GPU's geometry (NaN without gg), then this oracle's own geometry (7 floats). */
+   x, y, ex, ey, zc, hl, hu). out[n][26] = val[0..4] (Gate.val), mgeo, mecho, mwrap, mT, mzg, md0, mq, then the
+   five values again at the GPU's geometry (NaN without gg), then this oracle's own geometry (7 floats), then the
+   gate's KDP (the PHI pass's input, -1 where no echo) at its own geometry and at the GPU's (NaN without gg). */
 void orc_radar_gates(const float* ru, const float* BS, const float* V, const float* Q1, const float* Q2,
                      const float* aux, const float* tor, int n, const int* rg, const float* gg, float* out) {
   RU R;
@@ -363,9 +364,12 @@ void orc_radar_gates(const float* ru, const float* BS, const float* V, const flo
     for (int k = 0; k < 5; k++) o[k] = G.val[k];
     o[5] = G.mgeo; o[6] = G.mecho; o[7] = G.mwrap; o[8] = G.mT; o[9] = G.mzg; o[10] = G.md0; o[11] = G.mq;
     for (int k = 0; k < 5; k++) o[12 + k] = NAN;
+    o[24] = G.kdp;
+    o[25] = NAN;
     if (gg) {
       Gate H = gate(&C, tor, ri, gi, gg + (size_t)q * 7);
       for (int k = 0; k < 5; k++) o[12 + k] = H.val[k];
+      o[25] = H.kdp;
     }
     Geom g = geom(&C, ri, gi);
     o[17] = g.p.x; o[18] = g.p.y; o[19] = g.e.x; o[20] = g.e.y; o[21] = g.zc; o[22] = g.hl; o[23] = g.hu;

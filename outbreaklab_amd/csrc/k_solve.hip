@@ -626,35 +626,115 @@ __device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, si
   }
 }
 
-// FP32 sweeps with c' and d' held in registers (nz <= NZMAX): the column of the mode is read once
-// (all loads issued before the recurrence needs them) and written once.
-template <int NZMAX>
-__global__ void __launch_bounds__(256) k_thomas(ThomasArgs a) {
-  int sx, sy;
-  size_t base;
-  thomas_mode(a, sx, sy, base);
-  if (sx >= a.nx || sy >= a.ny) return;
+// Thomas sweeps with the column in registers (nz <= NZMAX), one launch for both kinds of mode: the first nb64
+// blocks take the listed FP64 modes (one per work-item), the others every remaining mode in FP32 (64 x 4 modes
+// per block), so the FP64 modes' short serial chains run beside the FP32 blocks instead of after them. FAST (the
+// mixed form): the FP32 modes, all of condition below the mixed
+// bound, divide through one reciprocal per level (v_rcp_f32, within WGSL's 2.5 ULP for x / y); otherwise
+// correctly rounded division in the reference's order (solver.js:86-101).
+template <int NZMAX, bool FAST>
+__global__ void __launch_bounds__(256) k_thomas(ThomasArgs a, int nb64, int gx) {
+  // every level's coefficients in LDS, read once per block: a read per level from global memory would put one
+  // memory latency on each step of the recurrence
+  __shared__ float4 zs[NZMAX];
+  __shared__ double zd[4 * NZMAX];
+  const int t = threadIdx.x;
+  const int nz = a.nz;
+  if (t < nz) zs[t] = a.zop32[t];
+  if (nb64 > 0)
+    for (int q = t; q < 4 * nz; q += 256) zd[q] = a.zop64[q];
+  __syncthreads();
   const size_t sxy = (size_t)a.nx * a.ny;
-  bool use64 = a.mode == 1;
-  if (a.mode == 2) use64 = fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo;
-  if (use64) {
-    thomas64(a, sx, sy, base, sxy);
+  if ((int)blockIdx.x < nb64) {
+    // FP64 sweep of listed mode i: c' and d' to global scratch ([level][list index], coalesced), read back 8
+    // levels at a time, each chunk's loads issued while the chunk above is used (registers stay within the FP32
+    // modes' budget). One reciprocal per level, refined by a Newton step (relative error ~2^-52), serves both
+    // quotients.
+    const int i = blockIdx.x * 256 + t;
+    if (i >= a.n64) return;
+    const int2 m = a.modes64[i];
+    const int sx = m.x, sy = m.y;
+    const size_t base = a.layout ? (size_t)sy + (size_t)a.ny * sx : (size_t)sx + (size_t)a.nx * sy;
+    float h[NZMAX];
+    // unconditional loads (levels past nz repeat the top one): a load under a per-level branch is waited for at
+    // the branch's end, one memory latency per level
+#pragma unroll
+    for (int k = 0; k < NZMAX; k++) h[k] = a.hat[base + sxy * min(k, nz - 1)];
+    const double shift = a.lam64[sx] + a.lam64[a.nx + sy];
+    double cprev = 0.0, dprev = 0.0;
+#pragma unroll
+    for (int k = 0; k < NZMAX; k++) {
+      if (k < nz) {
+        const double* o = zd + 4 * k;
+        const double den = o[2] + shift * o[3] - o[0] * cprev;
+        double r = __builtin_amdgcn_rcp(den);
+        r = fma(r, fma(-den, r, 1.0), r);
+        cprev = o[1] * r;
+        dprev = ((double)h[k] - o[0] * dprev) * r;
+        a.cp64[(size_t)k * a.n64 + i] = cprev;
+        a.dp64[(size_t)k * a.n64 + i] = dprev;
+      }
+    }
+    constexpr int CH = 8, NCH = (NZMAX + CH - 1) / CH;
+    auto fetch = [&](int c, double* cc, double* dc) {
+#pragma unroll
+      for (int j = 0; j < CH; j++) {
+        const size_t q = (size_t)min(c * CH + j, nz - 1) * a.n64 + i;
+        cc[j] = a.cp64[q];
+        dc[j] = a.dp64[q];
+      }
+    };
+    double cbuf[2][CH], dbuf[2][CH];
+    fetch(NCH - 1, cbuf[(NCH - 1) & 1], dbuf[(NCH - 1) & 1]);
+    double above = 0.0;
+#pragma unroll
+    for (int c = NCH - 1; c >= 0; c--) {
+      if (c > 0) fetch(c - 1, cbuf[(c - 1) & 1], dbuf[(c - 1) & 1]);
+      const double *cc = cbuf[c & 1], *dc = dbuf[c & 1];
+#pragma unroll
+      for (int j = CH - 1; j >= 0; j--) {
+        const int k = c * CH + j;
+        if (k < nz) {
+          above = (k == nz - 1) ? dc[j] : dc[j] - cc[j] * above;
+          thomas_put(a, sx, sy, base + sxy * k, k, (float)above);
+        }
+      }
+    }
     return;
   }
-  const int nz = a.nz;
+  // FP32 modes
+  const int b = (int)blockIdx.x - nb64;
+  const int f = (b % gx) * 64 + (t & 63), g = (b / gx) * 4 + (t >> 6);
+  int sx, sy;
+  size_t base;
+  if (a.layout) {
+    sy = f; sx = g;
+    base = (size_t)sy + (size_t)a.ny * sx;
+  } else {
+    sx = f; sy = g;
+    base = (size_t)sx + (size_t)a.nx * sy;
+  }
+  if (sx >= a.nx || sy >= a.ny) return;
+  // the listed FP64 modes (mixed form) are the first blocks'
+  if (a.mode == THOMAS_MIXED && fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo) return;
   float d[NZMAX], c[NZMAX];
 #pragma unroll
-  for (int k = 0; k < NZMAX; k++)
-    if (k < nz) d[k] = a.hat[base + sxy * k];
+  for (int k = 0; k < NZMAX; k++) d[k] = a.hat[base + sxy * min(k, nz - 1)];
   const float shift = a.lam32[sx] + a.lam32[a.nx + sy];
   float cprev = 0.0f, dprev = 0.0f;
 #pragma unroll
   for (int k = 0; k < NZMAX; k++) {
     if (k < nz) {
-      float4 o = a.zop32[k];
+      float4 o = zs[k];
       float den = o.z + shift * o.w - o.x * cprev;
-      cprev = o.y / den;
-      dprev = (d[k] - o.x * dprev) / den;
+      if (FAST) {
+        const float r = __builtin_amdgcn_rcpf(den);
+        cprev = o.y * r;
+        dprev = (d[k] - o.x * dprev) * r;
+      } else {
+        cprev = o.y / den;
+        dprev = (d[k] - o.x * dprev) / den;
+      }
       c[k] = cprev;
       d[k] = dprev;
     }
@@ -702,13 +782,28 @@ __global__ void __launch_bounds__(256) k_thomas_any(ThomasArgs a) {
   }
 }
 
+template <int NZ>
+static void launch_thomas_nz(hipStream_t s, const ThomasArgs& a, dim3 grid, int nb64, int gx, bool fast) {
+  if (fast) hipLaunchKernelGGL((k_thomas<NZ, true>), grid, dim3(256), 0, s, a, nb64, gx);
+  else hipLaunchKernelGGL((k_thomas<NZ, false>), grid, dim3(256), 0, s, a, nb64, gx);
+}
+
 void launch_thomas(hipStream_t s, const ThomasArgs& a) {
   const int nf = a.layout ? a.ny : a.nx, ns = a.layout ? a.nx : a.ny;
-  dim3 blk(64, 4, 1), grid((nf + 63) / 64, (ns + 3) / 4, 1);
-  if (a.nz <= 64) hipLaunchKernelGGL((k_thomas<64>), grid, blk, 0, s, a);
-  else if (a.nz <= 80) hipLaunchKernelGGL((k_thomas<80>), grid, blk, 0, s, a);
-  else if (a.nz <= 104) hipLaunchKernelGGL((k_thomas<104>), grid, blk, 0, s, a);
-  else hipLaunchKernelGGL(k_thomas_any, grid, blk, 0, s, a);
+  if (a.nz > 104) {
+    dim3 blk(64, 4, 1), grid((nf + 63) / 64, (ns + 3) / 4, 1);
+    hipLaunchKernelGGL(k_thomas_any, grid, blk, 0, s, a);
+    return;
+  }
+  // FP64 modes (modes 1 and 2) from the list; mode 1 has no FP32 blocks
+  const int nb64 = a.mode != THOMAS_FP32 ? (a.n64 + 255) / 256 : 0;
+  const int gx = (nf + 63) / 64, nb32 = a.mode != THOMAS_FP64 ? gx * ((ns + 3) / 4) : 0;
+  const dim3 grid(nb64 + nb32), blk(256);
+  if (grid.x == 0) return;
+  const bool fast = a.mode == THOMAS_MIXED;
+  if (a.nz <= 64) launch_thomas_nz<64>(s, a, grid, nb64, gx, fast);
+  else if (a.nz <= 80) launch_thomas_nz<80>(s, a, grid, nb64, gx, fast);
+  else launch_thomas_nz<104>(s, a, grid, nb64, gx, fast);
 }
 
 }  // namespace obl

@@ -1497,13 +1497,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
   col.Q2m = ldQ(Q2in, d, x, y, 0);
   col.am = ldA(aux, d, x, y, 0);
   col.pm = ldP(phT, d, x, y, 0);
+  // the cell's Q2 texel, loaded one level ahead
+  float4 q2n = ldQ(Q2in, d, x, y, 0);
   __syncthreads();
   const T2d tb{(const lds_float*)sB, x0, y0};
   for (int z = 0; z <= d.nz; z++) {
     Ring::Pre pv, pq;
     float2 na[NQ];
     float np[NQ];
-    col.Q2c = ldQ(Q2in, d, x, y, z);
+    col.Q2c = q2n;
+    q2n = ldQ(Q2in, d, x, y, z + 1);
     {
       rv.fetch(z + 2, tid, pv);
       rq.fetch(z + 2, tid, pq);

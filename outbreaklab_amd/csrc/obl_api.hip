@@ -154,6 +154,12 @@ struct obl_poisson {
   // zeroed once: nothing writes their padding) and their 32-row block counts
   uint16_t* mx[4] = {nullptr, nullptr, nullptr, nullptr};
   int mxRB[4] = {0, 0, 0, 0};
+  // the modes the Thomas sweep solves in FP64 (capacity nx ny), built for one (Thomas mode, shift bound) at a time
+  int2* modes64 = nullptr;
+  int n64 = 0, list_mode = -1;
+  double list_lo = -1.0;
+  std::vector<double> lam64h;  // the permuted eigenvalues, x then y
+  std::vector<int2> hmodes;
   std::string err;
   Timer timer;
   int fail(const char* fmt, ...) {
@@ -306,7 +312,7 @@ extern "C" void obl_poisson_destroy(obl_poisson* p) {
   free_axis(p->ax);
   free_axis(p->ay);
   void* bufs[] = {p->tmp,   p->hat,   p->cp32,  p->lam32, p->cp64,  p->dp64,  p->lam64, p->zop64,
-                  p->zop32, p->hrhs,  p->hphi,  p->mx[0], p->mx[1], p->mx[2], p->mx[3]};
+                  p->zop32, p->hrhs,  p->hphi,  p->mx[0], p->mx[1], p->mx[2], p->mx[3], p->modes64};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (p->stream) hipStreamDestroy(p->stream);
@@ -335,7 +341,8 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
       hip_ok(hipMalloc(&p->lam32, (nx + ny) * 4), &e, "hipMalloc(lam)") ||
       hip_ok(hipMalloc(&p->lam64, (nx + ny) * 8), &e, "hipMalloc(lam64)") ||
       hip_ok(hipMalloc(&p->zop32, nz * 16), &e, "hipMalloc(zop)") ||
-      hip_ok(hipMalloc(&p->zop64, nz * 32), &e, "hipMalloc(zop64)")) {
+      hip_ok(hipMalloc(&p->zop64, nz * 32), &e, "hipMalloc(zop64)") ||
+      hip_ok(hipMalloc(&p->modes64, (size_t)nx * ny * sizeof(int2)), &e, "hipMalloc(modes64)")) {
     obl_poisson_destroy(p);
     return set_global("PoissonDirect(%d, %d, %d): %s", nx, ny, nz, e.c_str());
   }
@@ -359,6 +366,7 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
   std::vector<float> l32(nx + ny);
   for (int i = 0; i < nx + ny; i++) l32[i] = (float)l64[i];
   hipMemcpy(p->lam64, l64.data(), l64.size() * 8, hipMemcpyHostToDevice);
+  p->lam64h = l64;
   hipMemcpy(p->lam32, l32.data(), l32.size() * 4, hipMemcpyHostToDevice);
   std::vector<double> ones(nz + 1, 1.0);
   if (obl_poisson_set_density(p, ones.data(), ones.data())) {
@@ -367,6 +375,33 @@ extern "C" int obl_poisson_create(int nx, int ny, int nz, double dx, double dz, 
     return set_global("%s", m.c_str());
   }
   *out = p;
+  return 0;
+}
+
+// The FP64 modes of a Thomas mode and shift bound (mode 2: |lx + ly| < shift_lo, the sum in double as the FP32
+// kernel tests it; mode 1: every mode), uploaded when they change (a set-up step, not part of a steady run of
+// solves: it waits for the copy).
+static int thomas_modes(obl_poisson* p, int thomas, double shift_lo, hipStream_t s) {
+  if (thomas == OBL_THOMAS_FP32 || p->nz > 104) {
+    p->n64 = 0;
+    return 0;
+  }
+  if (thomas == p->list_mode && (thomas == OBL_THOMAS_FP64 || shift_lo == p->list_lo)) return 0;
+  const int nx = p->nx, ny = p->ny;
+  p->hmodes.clear();
+  for (int sx = 0; sx < nx; sx++)
+    for (int sy = 0; sy < ny; sy++)
+      if (thomas == OBL_THOMAS_FP64 || fabs(p->lam64h[sx] + p->lam64h[nx + sy]) < shift_lo)
+        p->hmodes.push_back(make_int2(sx, sy));
+  p->n64 = (int)p->hmodes.size();
+  if (p->n64 > 0) {
+    if (hip_ok(hipMemcpyAsync(p->modes64, p->hmodes.data(), p->hmodes.size() * sizeof(int2), hipMemcpyHostToDevice, s),
+               &p->err, "Thomas FP64 modes") ||
+        hip_ok(hipStreamSynchronize(s), &p->err, "Thomas FP64 modes"))
+      return -1;
+  }
+  p->list_mode = thomas;
+  p->list_lo = shift_lo;
   return 0;
 }
 
@@ -396,6 +431,8 @@ static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, i
     t.nx = nx; t.ny = ny; t.nz = nz; t.mode = thomas;
     t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
     t.layout = 1; t.outT = p->mx[2]; t.P = P; t.oKB = Y.KB;
+    if (thomas_modes(p, thomas, t.shift_lo, s)) return -1;
+    t.modes64 = p->modes64; t.n64 = p->n64;
     TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
     // inverse y: rows (z, sx); the epilogue unfolds y into the XI operand, rows (z, y)
     m.A = p->mx[2]; m.B = Y.Wm[P - 2][1]; m.aRB = p->mxRB[2]; m.bRB = Y.bRB; m.KB = Y.KB;
@@ -421,6 +458,8 @@ static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, i
   // FP64 where the estimated condition 4 / (|shift| dz^2) exceeds `cond`
   t.shift_lo = cond > 0 ? 4.0 / (cond * p->dz * p->dz) : 0.0;
   t.layout = 0; t.outT = nullptr; t.P = 0; t.oKB = 0;
+  if (thomas_modes(p, thomas, t.shift_lo, s)) return -1;
+  t.modes64 = p->modes64; t.n64 = p->n64;
   TIMED(p->timer, s, "ps.tri", launch_thomas(s, t));
   // inverse y: hat -> tmp
   a = dst_args(p->hat, p->tmp, p->ay, 1, nz, nx);
@@ -485,7 +524,7 @@ struct obl_model {
   float *phT = nullptr, *phi = nullptr, *rhs = nullptr, *BD = nullptr, *CI = nullptr;
   obl_poisson* solver = nullptr;
   int gemm_mode = OBL_GEMM_BF16X6, thomas_mode = OBL_THOMAS_MIXED;
-  double fp64_cond = 16.0;
+  double fp64_cond = 64.0;
   // microphysics teacher sampling (surrogate training data)
   float* dump_buf = nullptr;
   unsigned* dump_count = nullptr;

@@ -160,6 +160,8 @@ struct ThomasArgs {
   int layout;           // 0: hat[z][sy][sx]; 1: hat[z][sx][sy] (the multi-piece BF16 chain)
   uint16_t* outT;       // non-null: the solution goes here as the y inverse pass's tiled split operand
   int P, oKB;           //   its pieces and k-blocks (rows z nx + sx, parity sy >= ny / 2, k = sy mod ny / 2)
+  const int2* modes64;  // modes 1 and 2 with nz <= 104: the (sx, sy) modes solved in FP64 (mode 1: every mode),
+  int n64;              //   in a kernel of their own; the FP32 kernel leaves them alone
 };
 enum { THOMAS_FP32 = 0, THOMAS_FP64 = 1, THOMAS_MIXED = 2 };
 void launch_thomas(hipStream_t s, const ThomasArgs& a);

@@ -119,7 +119,7 @@ class PoissonDirect {
     const o = opt || {};
     const phi = new Float32Array(rhs.length);
     addon.poissonSolve(this.h, rhs instanceof Float32Array ? rhs : Float32Array.from(rhs), phi,
-      GEMM[o.gemm || 'bf16x6'], THOMAS[o.thomas || 'mixed'], dflt(o.cond, 16));
+      GEMM[o.gemm || 'bf16x6'], THOMAS[o.thomas || 'mixed'], dflt(o.cond, 64));
     return phi;
   }
   destroy() { this.h = null; }
@@ -304,7 +304,7 @@ class RegionalModel {
     this.time += dt; this.steps++;
     this.phiScale = this.cnNow ? 1 / (1 - dflt(this.cnOld, CN_OLD)) : 1;
   }
-  setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'bf16x6'], THOMAS[thomas || 'mixed'], dflt(cond, 16)); }
+  setSolverMode(gemm, thomas, cond) { addon.setSolverMode(this.h, GEMM[gemm || 'bf16x6'], THOMAS[thomas || 'mixed'], dflt(cond, 64)); }
   sync() { return addon.sync(this.h); }
   // requestMaps (model.js:1478-1494): the 2-D maps of the current state from the uniform block last
   // written; returns () => Promise<{m0, m1, m2, m3: Float32Array(nx*ny*4), t}>, or null while both

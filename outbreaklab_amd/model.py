@@ -285,7 +285,7 @@ class RegionalModel:
     def sync(self):
         _lib.check(_lib.lib().obl_sync(self.h), self.h)
 
-    def set_solver_mode(self, gemm="bf16x6", thomas="mixed", fp64_cond=16.0):
+    def set_solver_mode(self, gemm="bf16x6", thomas="mixed", fp64_cond=64.0):
         _lib.check(_lib.lib().obl_set_solver_mode(self.h, _lib.GEMM[gemm], _lib.THOMAS[thomas], float(fp64_cond)),
                    self.h)
 
@@ -388,7 +388,7 @@ class PoissonDirect:
             raise ValueError("setDensity: rhoC needs nz values and rhoW nz + 1")
         _lib.check(_lib.lib().obl_poisson_set_density(self.h, _ptr(rc), _ptr(rw)), self.h, poisson=True)
 
-    def solve(self, rhs, gemm="bf16x6", thomas="mixed", fp64_cond=16.0) -> np.ndarray:
+    def solve(self, rhs, gemm="bf16x6", thomas="mixed", fp64_cond=64.0) -> np.ndarray:
         """Host arrays in/out (x fastest, shape (nz, ny, nx))."""
         r = np.ascontiguousarray(rhs, dtype=np.float32).reshape(self.nz, self.ny, self.nx)
         phi = np.empty_like(r)
@@ -396,7 +396,7 @@ class PoissonDirect:
                                                      float(fp64_cond)), self.h, poisson=True)
         return phi
 
-    def enc_device(self, rhs_ptr: int, phi_ptr: int, gemm="bf16x6", thomas="mixed", fp64_cond=16.0, stream=None):
+    def enc_device(self, rhs_ptr: int, phi_ptr: int, gemm="bf16x6", thomas="mixed", fp64_cond=64.0, stream=None):
         _lib.check(_lib.lib().obl_poisson_solve(self.h, C.c_void_p(rhs_ptr), C.c_void_p(phi_ptr), _lib.GEMM[gemm],
                                                 _lib.THOMAS[thomas], float(fp64_cond), stream), self.h, poisson=True)
 

@@ -303,12 +303,13 @@ def step_f64_split(U, BS, BD, CI, run_lsf, parity, V2, Q12, Q22, Vt, Q1t, Q2t, a
 
 def radar_gates(ru, BS, V, Q1, Q2, aux, tor, rg, gg=None):
     """Listed gates' moment values before rounding and decision margins (orc_radar_gates): rg = (n, 2) int
-    (radial, gate), gg = None or the GPU's geometry of each (n, 7: x, y, ex, ey, zc, hl, hu); returns [n][24] =
+    (radial, gate), gg = None or the GPU's geometry of each (n, 7: x, y, ex, ey, zc, hl, hu); returns [n][26] =
     REF, VEL, SW, ZDR, RHO values (v * scale + offset, NaN where not formed), the margins mgeo (m), mecho (dB),
     mwrap (m/s), mT (K), mzg (relative), md0 (mm), mq (relative), the five values at the GPU's geometry (NaN
-    without gg), then the oracle's own geometry (7)."""
+    without gg), the oracle's own geometry (7), then the gate's KDP (-1 where no echo) at the oracle's geometry
+    and at the GPU's (NaN without gg)."""
     rg = np.ascontiguousarray(rg, np.int32).reshape(-1, 2)
-    out = np.zeros((len(rg), 24), np.float32)
+    out = np.zeros((len(rg), 26), np.float32)
     if not len(rg):
         return out
     f = lib().orc_radar_gates

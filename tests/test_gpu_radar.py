@@ -189,6 +189,23 @@ def phidp_values(kdp, ng, dr, offset):
     return vals, wrap, accs
 
 
+def phi_why(kdp, gi, gpu, ru):
+    """Why gate gi's PHI code may be the GPU's ``gpu`` given a radial's KDP (None: no reason): the running sum of
+    ulp-different KDP puts the value within its accumulated bound of a rounding boundary or of a 360-degree wrap,
+    or the sum gives the GPU's code."""
+    ng = len(kdp)
+    pv, pw, acc = phidp_values(kdp, ng, float(ru[4 * 3 + 3]), float(ru[4 * 4 + 3]))
+    bound = 1e-5 * abs(acc[gi]) + 1e-4  # degrees: the running sum of ulp-different KDP
+    x = float(pv[gi]) - 0.5
+    if np.isfinite(pv[gi]) and code_of(pv[gi]) == gpu:
+        return "geometry"
+    if np.isfinite(pv[gi]) and abs(x - round(x)) <= bound * 253.0 / 360.0:
+        return "rounding"
+    if pw[gi] <= bound:
+        return "Nyquist wrap"  # the 360-degree wrap of PHI
+    return None
+
+
 def code_of(v):
     return int(np.clip(np.rint(v), 2, 255)) if np.isfinite(v) else 0
 
@@ -221,7 +238,9 @@ def test_volume_codes_differ_only_near_a_boundary(volume):
     (obl_radar_geometry), gives the GPU's code (or a value within the rounding bound of its boundary); or the gate
     lies within the stated bound of the echo threshold, a Nyquist wrap, the beam leaving the model, the melting
     level or the graupel / drop-size / Thompson size switches (or, for PHI, a rounding or 360-degree wrap of the
-    running sum within its accumulated bound). Any other difference fails. On a sample of gates the GPU's geometry
+    running sum within its accumulated bound, of the oracle's KDP or of the oracle's KDP at the GPU's own geometry
+    of every gate of the radial, the latter also where it gives the GPU's code). Any other difference fails. On a
+    sample of gates the GPU's geometry
     lies within a few metres and a few 1e-3 rad of the oracle's (what the FP32 formulas allow)."""
     from collections import Counter
 
@@ -262,14 +281,13 @@ def test_volume_codes_differ_only_near_a_boundary(volume):
             ndiff += 1
             why = classify(name, gpu, g)
             if why is None and name == "PHI":
-                pv, pw, acc = phidp_values(kdp[ri * R["ngDop"]:(ri + 1) * R["ngDop"]], R["ngDop"],
-                                           float(ru[4 * 3 + 3]), float(ru[4 * 4 + 3]))
-                bound = 1e-5 * abs(acc[gi]) + 1e-4  # degrees: the running sum of ulp-different KDP
-                x = float(pv[gi]) - 0.5
-                if np.isfinite(pv[gi]) and abs(x - round(x)) <= bound * 253.0 / 360.0:
-                    why = "rounding"
-                elif pw[gi] <= bound:
-                    why = "Nyquist wrap"  # the 360-degree wrap of PHI
+                why = phi_why(kdp[ri * R["ngDop"]:(ri + 1) * R["ngDop"]], gi, gpu, ru)
+                if why is None:
+                    # the running sum of the oracle's KDP at the GPU's own geometry of every gate of the radial
+                    rr = np.stack([np.full(R["ngDop"], ri), np.arange(R["ngDop"])], 1).astype(np.int32)
+                    kg = radar_gates(ru, m._B, V, Q1, Q2, aux, t, rr, gate_geometry(m, ru, rr))[:, 25]
+                    if phi_why(kg, gi, gpu, ru) is not None:
+                        why = "geometry" 
             classes[why or "unexplained"] += 1
     print(f"{ndiff} of {ncodes} codes differ:", dict(classes), "| GPU vs oracle geometry, largest |dx| |dy| |dz| (m),"
           f" turn (rad): {dgeo}; sampled gates with identical geometry: {same} of {nsamp}")

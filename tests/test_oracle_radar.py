@@ -180,7 +180,7 @@ def test_gate_values_and_geometry_override():
     geo, R = mk_radar(m)
     U, Vw = 25.0, -9.0
     st = mk_state(m, qr=2e-3, u=U, v=Vw)
-    ru, out, _ = run_cut(m, geo, R, 1, st)
+    ru, out, kdp = run_cut(m, geo, R, 1, st)
     c = R["cuts"][1]
     rng = np.random.default_rng(3)
     rg = np.stack([rng.integers(0, c["nrad"], 3000), rng.integers(0, R["ngDop"], 3000)], 1).astype(np.int32)
@@ -200,6 +200,10 @@ def test_gate_values_and_geometry_override():
     gg = rows[:, 17:24].copy()
     same = radar_gates(ru, *st, None, rg, gg)
     assert np.array_equal(same[:, 12:17], rows[:, 0:5], equal_nan=True)
+    # the KDP columns: the cut's own KDP buffer at the oracle's geometry, and the same again at that geometry
+    # handed back (rain everywhere: every gate inside the domain has one)
+    assert np.array_equal(rows[:, 24], kdp[rg[:, 0] * R["ngDop"] + rg[:, 1]]) and np.count_nonzero(rows[:, 24] > 0) > 300
+    assert np.array_equal(same[:, 25], rows[:, 24]) and np.all(np.isnan(rows[:, 25]))
     out_ = gg.copy()
     out_[:, 0] = -1000.0
     assert np.all(np.isnan(radar_gates(ru, *st, None, rg, out_)[:, 12:17]))
