@@ -34,9 +34,9 @@ FP64_COND = 64.0
 
 # compulsory bytes per model cell of each stencil kernel: each field read once + written once
 # (float4 texel 16 B, aux 8 B, rhs 4 B); halo re-reads excluded
-# (a fused kernel counts only its own inputs and outputs: m.advV reads V, Q1 and writes V, aux; m.advQ reads V, Q1,
-# Q2 and writes Q1, Q2)
-CELL_BYTES = {"m.nut": 40, "m.advV1": 32, "m.advV2": 48, "m.advV": 56, "m.advQ1": 80, "m.advQ2": 112, "m.advQ": 80,
+# (a fused kernel counts only its own inputs and outputs: m.adv, nut and both advections in one launch, reads V,
+# Q1, Q2 and writes V, aux, Q1, Q2; the hurricane step's separate advection kernels likewise)
+CELL_BYTES = {"m.nut": 40, "m.advV1": 32, "m.advV2": 48, "m.adv": 104, "m.advQ1": 80, "m.advQ2": 112,
               "m.phys": 112,
               "m.div": 20, "m.proj": 36}
 

@@ -799,11 +799,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
 // One block per TX x TY column of points, walking z upward; every field a kernel gathers
 // from is held as a three-plane LDS ring (obl_tile.h), the next plane prefetched into
 // registers while the current one is computed.
-#define TILE_SETUP                                          \
+#define TILE_SETUP_AT(BX, BY)                               \
   Dims d = dims(u);                                         \
   const int tid = threadIdx.x;                              \
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;     \
+  const int x0 = (BX) * TX, y0 = (BY) * TY;                 \
   const int x = x0 + tid % TX, y = y0 + tid / TX;
+#define TILE_SETUP TILE_SETUP_AT(blockIdx.x, blockIdx.y)
 
 template <bool HUR>
 __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
@@ -1088,15 +1089,25 @@ __device__ __forceinline__ void halo_slot(int t, int& si, int& sj) {
   else { si = TPW - 1; sj = 1 + t - 2 * TPW - TY; }
 }
 
-__global__ void __launch_bounds__(TILE_THREADS) k_advq_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                         const float4* __restrict__ Q1, const float4* __restrict__ Q2,
-                                                         float4* __restrict__ Q1o, float4* __restrict__ Q2o) {
-  __shared__ float lds[2 * QRING_FLOATS + 2 * RING_FLOATS];
-  __shared__ float4 ftab[FALL_TAB];  // per level: a clear cell's fall speeds (r, n, s, g)
+// the moisture advection's LDS
+struct AdvqSmem {
+  float lds[2 * QRING_FLOATS + 2 * RING_FLOATS];
+  float4 ftab[FALL_TAB];  // per level: a clear cell's fall speeds (r, n, s, g)
   // per ring slot and wave: the channels (Q1 bits 0-3, Q2 bits 4-7) not +0 somewhere in the texels the wave stored
   // into the Q rings' plane (tile and two-point halo) / the predicted rings' plane (tile and one-point halo)
-  __shared__ __attribute__((aligned(16))) uint32_t nzq[3][4], nzh[3][4];
-  TILE_SETUP
+  uint32_t nzq[3][4] __attribute__((aligned(16)));
+  uint32_t nzh[3][4] __attribute__((aligned(16)));
+};
+// the moisture advection of column tile (bx, by)
+__device__ __forceinline__ void advq_tile(const MU& u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                          const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                          float4* __restrict__ Q1o, float4* __restrict__ Q2o, int bx, int by,
+                                          AdvqSmem& sm) {
+  float* const lds = sm.lds;
+  float4* const ftab = sm.ftab;
+  uint32_t(*const nzq)[4] = sm.nzq;
+  uint32_t(*const nzh)[4] = sm.nzh;
+  TILE_SETUP_AT(bx, by)
   const bool thm = u.v[U_MICRO2].z > 0.5f;
   const float rsfc = u.v[U_STEP].z;
   const int wave = tid >> 6;
@@ -1368,11 +1379,16 @@ __device__ __forceinline__ float4 correct_v(const MU& u, const Dims& d, const Ri
 #ifndef ADVV_MIN_BLOCKS
 #define ADVV_MIN_BLOCKS 2
 #endif
-__global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
-                                                         const float4* __restrict__ Q1, float4* __restrict__ Vout,
-                                                         float2* __restrict__ aux) {
-  __shared__ float lds[QRING_FLOATS + RING_FLOATS];
-  TILE_SETUP
+// the wind advection's LDS
+struct AdvvSmem {
+  float lds[QRING_FLOATS + RING_FLOATS];
+};
+// nut and the wind advection of face tile (bx, by)
+__device__ __forceinline__ void advv_tile(const MU& u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                          const float4* __restrict__ Q1, float4* __restrict__ Vout,
+                                          float2* __restrict__ aux, int bx, int by, AdvvSmem& sm) {
+  float* const lds = sm.lds;
+  TILE_SETUP_AT(bx, by)
   const int W = d.nx + 1, H = d.ny + 1, D = d.nz + 1;
   Ring2 vr{lds, V, W, H, D, x0, y0};
   Ring hr{lds + QRING_FLOATS, nullptr, W, H, D, x0, y0, 0};
@@ -1452,6 +1468,41 @@ __global__ void __launch_bounds__(TILE_THREADS, ADVV_MIN_BLOCKS) k_advv_t(MU u, 
     const float4 q1b = level(z, cur, alt, q1n);
     if (z + 1 <= d.nz) q1n = level(z + 1, alt, cur, q1b);
   }
+}
+
+// Both advections in one launch (they read the same state and write different fields), so that a CU runs a wind
+// tile and a moisture tile side by side and the moisture advection's waits at its barriers and on memory overlap
+// the wind advection's instruction issue (the two kernels back to back leave each bottleneck alone). Block b runs
+// on XCD b mod 8 and an XCD's 32 CUs take its blocks in turn, so roles alternate in runs of 256 blocks (one block
+// per CU of every XCD); measured against runs of 1 (roles split between XCDs) and of 8. One LDS union: a block
+// holds one role's rings.
+union AdvSmem {
+  AdvqSmem q;
+  AdvvSmem v;
+};
+__global__ void __launch_bounds__(TILE_THREADS, 2) k_adv_t(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                           const float4* __restrict__ Q1, const float4* __restrict__ Q2,
+                                                           float4* __restrict__ Vout, float2* __restrict__ aux,
+                                                           float4* __restrict__ Q1o, float4* __restrict__ Q2o, int gvx,
+                                                           int nv, int gqx, int nq) {
+  __shared__ AdvSmem sm;
+  const int b = blockIdx.x, m = min(nv, nq);
+  bool wind;
+  int i;
+  constexpr int S = 8;
+  const int M = (2 * m) & ~((2 << S) - 1);
+  if (b < M) {
+    wind = ((b >> S) & 1) == 0;
+    i = ((b >> (S + 1)) << S) | (b & ((1 << S) - 1));
+  } else if (b < 2 * m) {
+    wind = (b & 1) == 0;
+    i = M / 2 + ((b - M) >> 1);
+  } else {
+    wind = nv > nq;
+    i = m + (b - 2 * m);
+  }
+  if (wind) advv_tile(u, BS, V, Q1, Vout, aux, i % gvx, i / gvx, sm.v);
+  else advq_tile(u, BS, V, Q1, Q2, Q1o, Q2o, i % gqx, i / gqx, sm.q);
 }
 
 // One block per 32 x 8 column tile of the faces grid, walking z upward like the advection
@@ -2061,13 +2112,12 @@ void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, con
   else
     hipLaunchKernelGGL(k_nut_t<false>, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, sfc, aux);
 }
-void launch_advv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 float4* Vout, float2* aux) {
-  hipLaunchKernelGGL(k_advv_t, tface_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Vout, aux);
-}
-void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 const float4* Q2, float4* Q1o, float4* Q2o) {
-  hipLaunchKernelGGL(k_advq_t, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Q1o, Q2o);
+void launch_adv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                const float4* Q2, float4* Vout, float2* aux, float4* Q1o, float4* Q2o) {
+  const dim3 gv = tface_grid(d), gq = tcell_grid(d);
+  const int nv = (int)(gv.x * gv.y), nq = (int)(gq.x * gq.y);
+  hipLaunchKernelGGL(k_adv_t, dim3(nv + nq), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Vout, aux, Q1o, Q2o, (int)gv.x, nv,
+                     (int)gq.x, nq);
 }
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,

@@ -1234,8 +1234,8 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
   if (flags & OBL_STEP_FILT) return h->fail("the 2-dx filter (OBL_STEP_FILT) is a hurricane-mode step");
   if (flags & OBL_STEP_LSF) TIMED(T, s, "m.lsf", launch_lsf(s, u, d, h->BS, h->BD, h->lsfA, h->lsfB, h->phT));
   TIMED(T, s, "m.ci", launch_ci(s, u, d, h->CI, h->ciMap));
-  TIMED(T, s, "m.advV", launch_advv(s, u, d, h->BS, Vc, h->Q1[0], Vn, h->aux));
-  TIMED(T, s, "m.advQ", launch_advq(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], h->Q1[1], h->Q2[1]));
+  // nut and the wind advection (into Vn, aux), and the moisture advection (into Q1[1], Q2[1]), in one launch
+  TIMED(T, s, "m.adv", launch_adv(s, u, d, h->BS, Vc, h->Q1[0], h->Q2[0], Vn, h->aux, h->Q1[1], h->Q2[1]));
   if (h->net) {
     MlpView mv;
     mlp_view(h->net, &mv);

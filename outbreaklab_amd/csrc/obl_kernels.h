@@ -18,13 +18,10 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
 // sfc: the sea surface map (nx * ny float4: land fraction, SST K, 0, 0); null outside hurricane mode
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 const float4* sfc, float2* aux);
-// the predictor and corrector of the wind advection and the turbulent viscosity in one pass (outbreak mode):
-// V, Q1 -> Vout, aux
-void launch_advv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 float4* Vout, float2* aux);
-// the predictor and corrector of the moisture advection in one pass (outbreak mode): Q1, Q2 -> Q1o, Q2o
-void launch_advq(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
-                 const float4* Q2, float4* Q1o, float4* Q2o);
+// outbreak mode, in one launch (k_adv_t): the turbulent viscosity and the wind advection's predictor and corrector
+// (V, Q1 -> Vout, aux), and the moisture advection's (V, Q1, Q2 -> Q1o, Q2o)
+void launch_adv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
+                const float4* Q2, float4* Vout, float2* aux, float4* Q1o, float4* Q2o);
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o);
