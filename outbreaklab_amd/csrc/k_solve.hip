@@ -626,6 +626,23 @@ __device__ __forceinline__ void thomas64(const ThomasArgs& a, int sx, int sy, si
   }
 }
 
+// The modes of a staged chunk whose bits are set in om, value by value (a chunk that also holds modes this
+// kernel does not write)
+__device__ __noinline__ static void thomas_put_sparse(uint16_t* o, size_t plane, int P, uint32_t om, float4 v0,
+                                                      float4 v1) {
+  const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    if ((om >> e) & 1u) {
+      uint16_t pc[3];
+      mx_split1<3>(v[e], pc);
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        if (q < P) o[(size_t)q * plane + e] = pc[q];
+    }
+  }
+}
+
 // Thomas sweeps with the column in registers (nz <= NZMAX), one launch for both kinds of mode: the first nb64
 // blocks take the listed FP64 modes (one per work-item), the others every remaining mode in FP32 (64 x 4 modes
 // per block), so the FP64 modes' short serial chains run beside the FP32 blocks instead of after them. FAST (the
@@ -714,9 +731,29 @@ __global__ void __launch_bounds__(256) k_thomas(ThomasArgs a, int nb64, int gx) 
     sx = f; sy = g;
     base = (size_t)sx + (size_t)a.nx * sy;
   }
-  if (sx >= a.nx || sy >= a.ny) return;
+  // Into the tiled operand (layout 1, ny a multiple of 16) the block's solutions are staged in LDS eight levels at
+  // a time ([level][sx][sy]: 4 x 64 modes) and stored as whole 16-byte chunks of each piece (8 consecutive sy of
+  // one row): at each flush a thread splits one chunk's 8 values and stores its P chunks, instead of P 2-byte
+  // stores per mode and level, and a wave's stores fill whole 128-byte lines (4 rows x 2 k-halves). A chunk that
+  // holds a listed FP64 mode, or a mode past the array, is stored value by value, leaving those modes' bytes alone.
+  const bool packed = a.outT != nullptr && a.layout == 1 && (a.ny & 15) == 0;
+  if (!packed && (sx >= a.nx || sy >= a.ny)) return;
   // the listed FP64 modes (mixed form) are the first blocks'
-  if (a.mode == THOMAS_MIXED && fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo) return;
+  bool own = sx < a.nx && sy < a.ny;
+  if (own && a.mode == THOMAS_MIXED && fabs(a.lam64[sx] + a.lam64[a.nx + sy]) < a.shift_lo) {
+    if (!packed) return;
+    own = false;
+  }
+  __shared__ __align__(16) float sol[2][8][4][64];
+  __shared__ uint64_t ownw[4];
+  if (packed) {
+    // every thread reaches the staging barriers: a lane past the array sweeps a clamped mode and stores nothing
+    const uint64_t m = __ballot(own);
+    if ((t & 63) == 0) ownw[t >> 6] = m;
+    sx = min(sx, a.nx - 1);
+    sy = min(sy, a.ny - 1);
+    base = (size_t)sy + (size_t)a.ny * sx;
+  }
   float d[NZMAX], c[NZMAX];
 #pragma unroll
   for (int k = 0; k < NZMAX; k++) d[k] = a.hat[base + sxy * min(k, nz - 1)];
@@ -740,11 +777,59 @@ __global__ void __launch_bounds__(256) k_thomas(ThomasArgs a, int nb64, int gx) 
     }
   }
   float above = 0.0f;
+  if (!packed) {
 #pragma unroll
-  for (int k = NZMAX - 1; k >= 0; k--) {
-    if (k < nz) {
-      above = (k == nz - 1) ? d[k] : d[k] - c[k] * above;
-      thomas_put(a, sx, sy, base + sxy * k, k, above);
+    for (int k = NZMAX - 1; k >= 0; k--) {
+      if (k < nz) {
+        above = (k == nz - 1) ? d[k] : d[k] - c[k] * above;
+        thomas_put(a, sx, sy, base + sxy * k, k, above);
+      }
+    }
+    return;
+  }
+  // this thread's chunk at a flush: level k0 + fl, row sx0 + fw, the 8 modes fsy .. fsy + 7 (k-half kc of the
+  // wave's 16-mode group kb); lanes 8 apart in a wave write adjacent 128-byte lines
+  const int w = t >> 6, lane = t & 63;
+  const int fw = t & 3, kc = (t >> 2) & 1, kb = (t >> 3) & 3, fl = t >> 5, fg = 2 * kb + kc;
+  const int fsx = (b / gx) * 4 + fw, fsy = (b % gx) * 64 + 8 * fg;
+  const int Hy = a.ny >> 1, par = fsy >= Hy, P = a.P;
+  const bool fin = fsx < a.nx && fsy < a.ny;
+  const size_t plane = (size_t)((a.nz * a.nx + 32 * MX_MF - 1) / (32 * MX_MF) * MX_MF) * (a.oKB + 1) * 512;
+  uint16_t* const ob = a.outT + (size_t)(P * par) * plane;
+  auto flush = [&](int k0, int buf) {
+    __syncthreads();
+    const int k = k0 + fl;
+    if (k < nz && fin) {
+      const float4 v0 = *reinterpret_cast<const float4*>(&sol[buf][fl][fw][8 * fg]);
+      const float4 v1 = *reinterpret_cast<const float4*>(&sol[buf][fl][fw][8 * fg + 4]);
+      const uint32_t om = (uint32_t)(ownw[fw] >> (8 * fg)) & 0xffu;
+      uint16_t* const o = ob + mx_off(k * a.nx + fsx, fsy - par * Hy, a.oKB);
+      // (the first two of three pieces are the two-piece split)
+      if (om == 0xffu) {
+        uint2 lo[3], hi[3];
+        mx_split4<3>(v0.x, v0.y, v0.z, v0.w, lo);
+        mx_split4<3>(v1.x, v1.y, v1.z, v1.w, hi);
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+          if (q < P) *reinterpret_cast<uint4*>(o + (size_t)q * plane) = make_uint4(lo[q].x, lo[q].y, hi[q].x, hi[q].y);
+      } else if (om) {
+        thomas_put_sparse(o, plane, P, om, v0, v1);
+      }
+    }
+  };
+  constexpr int NB = (NZMAX + 7) / 8;
+#pragma unroll
+  for (int kb8 = NB - 1; kb8 >= 0; kb8--) {
+    if (kb8 * 8 < nz) {
+#pragma unroll
+      for (int j = 7; j >= 0; j--) {
+        const int k = kb8 * 8 + j;
+        if (k < NZMAX && k < nz) {
+          above = (k == nz - 1) ? d[k] : d[k] - c[k] * above;
+          sol[kb8 & 1][j][w][lane] = above;
+        }
+      }
+      flush(kb8 * 8, kb8 & 1);
     }
   }
 }
