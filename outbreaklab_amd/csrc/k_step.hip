@@ -284,19 +284,21 @@ __device__ __forceinline__ float2 nut_point(const MU& u, const Dims& d, const fl
   float dx = d.dx, dz = d.dz;
   int nz = d.nz;
   float4 v0 = V.ld4(x, y, z);
-  float dudx = (V.ldc(x + 1, y, z, 0) - v0.x) / dx;
-  float dvdy = (V.ldc(x, y + 1, z, 1) - v0.y) / dx;
-  float dwdz = (V.ldc(x, y, z + 1, 2) - v0.z) / dz;
-  float dudy = (nut_uc(V, BS, nz, x, y + 1, z) - nut_uc(V, BS, nz, x, y - 1, z)) / (2.0f * dx);
-  float dudz = (nut_uc(V, BS, nz, x, y, z + 1) - nut_uc(V, BS, nz, x, y, z - 1)) / (2.0f * dz);
-  float dvdx = (nut_vc(V, BS, nz, x + 1, y, z) - nut_vc(V, BS, nz, x - 1, y, z)) / (2.0f * dx);
-  float dvdz = (nut_vc(V, BS, nz, x, y, z + 1) - nut_vc(V, BS, nz, x, y, z - 1)) / (2.0f * dz);
-  float dwdx = (nut_wc(V, x + 1, y, z) - nut_wc(V, x - 1, y, z)) / (2.0f * dx);
-  float dwdy = (nut_wc(V, x, y + 1, z) - nut_wc(V, x, y - 1, z)) / (2.0f * dx);
+  // (the divisions by dx, dz, 2 dx and 2 dz as products with their reciprocals, bit for bit)
+  float dudx = rmul(V.ldc(x + 1, y, z, 0) - v0.x, d.rdx);
+  float dvdy = rmul(V.ldc(x, y + 1, z, 1) - v0.y, d.rdx);
+  float dwdz = rmul(V.ldc(x, y, z + 1, 2) - v0.z, d.rdz);
+  float dudy = rmul(nut_uc(V, BS, nz, x, y + 1, z) - nut_uc(V, BS, nz, x, y - 1, z), d.r2dx);
+  float dudz = rmul(nut_uc(V, BS, nz, x, y, z + 1) - nut_uc(V, BS, nz, x, y, z - 1), d.r2dz);
+  float dvdx = rmul(nut_vc(V, BS, nz, x + 1, y, z) - nut_vc(V, BS, nz, x - 1, y, z), d.r2dx);
+  float dvdz = rmul(nut_vc(V, BS, nz, x, y, z + 1) - nut_vc(V, BS, nz, x, y, z - 1), d.r2dz);
+  float dwdx = rmul(nut_wc(V, x + 1, y, z) - nut_wc(V, x - 1, y, z), d.r2dx);
+  float dwdy = rmul(nut_wc(V, x, y + 1, z) - nut_wc(V, x, y - 1, z), d.r2dx);
   float s12 = 0.5f * (dudy + dvdx), s13 = 0.5f * (dudz + dwdx), s23 = 0.5f * (dvdz + dwdy);
   float S2 = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + 4.0f * (s12 * s12 + s13 * s13 + s23 * s23);
   int kz0 = max(z - 1, 0), kz1 = min(z + 1, nz - 1);
-  float N2 = 9.81f / bsA(BS, z, nz).x * (V.ldc(x, y, kz1, 3) - V.ldc(x, y, kz0, 3)) / ((float)(kz1 - kz0) * dz);
+  float N2 = rmul(9.81f / bsA(BS, z, nz).x * (V.ldc(x, y, kz1, 3) - V.ldc(x, y, kz0, 3)),
+                       kz1 - kz0 == 2 ? d.r2dz : (kz1 > kz0 ? d.rdz : __builtin_inff()));
   if (u.v[U_COR].w > 0.5f && q1.y + q1.z > 1e-5f) N2 = fminf(N2, 0.0f);
   float Ri = N2 / fmaxf(S2, 1e-10f);
   float fst = sqrtf(fmaxf(0.0f, 1.0f - Ri / 0.33f));
@@ -346,7 +348,7 @@ __device__ __forceinline__ float sampC(const A& T, float3 p) { return samp1(T, p
 // it while each Courant number of the local velocities is below one (traceV moves a point by
 // k * v; clamping moves it inward). Then every sample falls in an LDS ring's window.
 __device__ __forceinline__ bool near_traces(const Dims& d, const Loc& L) {
-  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  const float kx = d.kx, kz = d.kz;
   float mx = fmaxf(fmaxf(fabsf(L.vu.x), fabsf(L.vv.x)), fmaxf(fabsf(L.vw.x), fabsf(L.vc.x)));
   float my = fmaxf(fmaxf(fabsf(L.vu.y), fabsf(L.vv.y)), fmaxf(fabsf(L.vw.y), fabsf(L.vc.y)));
   float mz = fmaxf(fmaxf(fabsf(L.vu.z), fabsf(L.vv.z)), fmaxf(fabsf(L.vw.z), fabsf(L.vc.z)));
@@ -411,7 +413,7 @@ constexpr int FALL_TAB = 128;  // levels of a block's table of clear-cell fall s
 // as near_traces, for cell-centred traces with per-species fall speeds: |k_z (w - f)| is largest
 // at the extreme f, so the bound over f in [min(0, f...), max(0, f...)] covers every species
 __device__ __forceinline__ bool near_traces_q(const Dims& d, float3 v, const Fall& f) {
-  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  const float kx = d.kx, kz = d.kz;
   float fmin = fminf(fminf(0.0f, fminf(f.r, f.n)), fminf(f.s, f.g));
   float fmax = fmaxf(fmaxf(0.0f, fmaxf(f.r, f.n)), fmaxf(f.s, f.g));
   return fabsf(v.x) * kx < 0.999f && fabsf(v.y) * kx < 0.999f && fabsf(v.z - fmin) * kz < 0.999f &&
@@ -577,12 +579,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float nw = 0.5f * (lsfB(x - 1, y).w + lsfB(x, y).w);
     du += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (ut - V.x);
     du += fc * (vbar - vtb);
-    if (x > 0 && x < nx) du -= (phT(x, y, z) - phT(x - 1, y, z)) / dx;
+    if (x > 0 && x < nx) du -= rmul(phT(x, y, z) - phT(x - 1, y, z), d.rdx);
     if (z == 0 && drag) {
       float lg = sqrtf(V.x * V.x + vbar * vbar), lg0 = sqrtf(ut * ut + vtb * vtb);
       float cd = u.v[U_MICRO].w;
       if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, max(x - 1, 0), y).x + sfcAt(hu, d, x, y).x));
-      du += cd * (-lg * V.x + lg0 * ut) / dz;
+      du += rmul(cd * (-lg * V.x + lg0 * ut), d.rdz);
     }
     o.x = V.x + dt * du;
   } else {
@@ -600,12 +602,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float nw = 0.5f * (lsfB(x, y - 1).w + lsfB(x, y).w);
     dv += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (vt - V.y);
     dv -= fc * (ubar - utb);
-    if (y > 0 && y < ny) dv -= (phT(x, y, z) - phT(x, y - 1, z)) / dx;
+    if (y > 0 && y < ny) dv -= rmul(phT(x, y, z) - phT(x, y - 1, z), d.rdx);
     if (z == 0 && drag) {
       float lg = sqrtf(ubar * ubar + V.y * V.y), lg0 = sqrtf(utb * utb + vt * vt);
       float cd = u.v[U_MICRO].w;
       if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, x, max(y - 1, 0)).x + sfcAt(hu, d, x, y).x));
-      dv += cd * (-lg * V.y + lg0 * vt) / dz;
+      dv += rmul(cd * (-lg * V.y + lg0 * vt), d.rdz);
     }
     o.y = V.y + dt * dv;
   } else {
@@ -620,7 +622,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float dw = nux * lapH(Vin, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
     dw += 0.5f * (buoyv(bsA(BS, z - 1, nz), col.Vm.w, col.Q1m, col.Q2m) +
                   buoyv(bsA(BS, z, nz), V.w, Q1in.ld4(x, y, z), col.Q2c)) -
-          (phT(x, y, z) - col.pm) / dz;
+          rmul(phT(x, y, z) - col.pm, d.rdz);
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
     float4 ci = col.ci;
     if (ci.x > 0.0f && !HUR) {  // hurricane mode starts its storms as warm, moist bubbles instead (theta below)
@@ -959,7 +961,7 @@ struct Pred {
 template <class A>
 __device__ __forceinline__ void predict(const Dims& d, bool thm, const A& Q1, const A& Q2, float3 v, const Fall& f,
                                         int x, int y, int z, bool clear, Pred& o) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 k = make_float3(d.kx, d.kx, d.kz);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   float3 p0 = traceQ(d, k, p, v, 0.0f, 1.0f);
   float tx = p0.x - 0.5f, ty = p0.y - 0.5f, tz = p0.z - 0.5f;
@@ -1010,7 +1012,7 @@ __device__ __noinline__ static float qhat_at(const MU& u, const Dims& d, const f
   float4 q1 = ldx(Q1, cidx(d, i, j, k)), q2 = ldx(Q2, cidx(d, i, j, k));
   Fall f = fall_speeds(thm, q1, q2, bsA(BS, k, d.nz).z, u.v[U_STEP].z);
   const float fall = ch == 3 ? f.r : ch == 4 ? f.s : ch == 5 ? f.g : ch == 6 ? f.n : 0.0f;
-  float3 kk = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 kk = make_float3(d.kx, d.kx, d.kz);
   float3 p = traceQ(d, kk, make_float3((float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f), v, fall, 1.0f);
   return ch < 4 ? samp1(g1, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch) : samp1(g2, p.x - 0.5f, p.y - 0.5f, p.z - 0.5f, ch - 4);
 }
@@ -1047,7 +1049,7 @@ template <bool NEAR>
 __device__ __forceinline__ void correct(const MU& u, const Dims& d, const RingW& w1t, const RingW& w2t, const Carry& c,
                                         const float4* BS, const float4* V, const float4* Q1, const float4* Q2, int x,
                                         int y, int z, bool clear, float4& o1, float4& o2) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 k = make_float3(d.kx, d.kx, d.kz);
   float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   const float3 v = c.v;
   float cx = v.x * k.x, cy = v.y * k.y, cz = v.z * k.z;
@@ -1301,7 +1303,7 @@ struct PredV {
 };
 template <class A>
 __device__ __forceinline__ void predict_v(const Dims& d, const A& V, const Loc& L, int x, int y, int z, PredV& o) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 k = make_float3(d.kx, d.kx, d.kz);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   float3 pu = traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f);
   float3 pv = traceV(d, k, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f);
@@ -1322,7 +1324,7 @@ __device__ __noinline__ static float vhat_at(const Dims& d, const float4* V, int
   i = clampi(i, 0, d.nx); j = clampi(j, 0, d.ny); k = clampi(k, 0, d.nz);
   GAcc gv{V, d.nx + 1, d.ny + 1, d.nz + 1, nullptr};
   const Loc L = localVel(gv, i, j, k);
-  float3 kk = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 kk = make_float3(d.kx, d.kx, d.kz);
   float fx = (float)i, fy = (float)j, fz = (float)k;
   if (ch == 0) return sampU(gv, traceV(d, kk, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, 1.0f));
   if (ch == 1) return sampV(gv, traceV(d, kk, make_float3(fx + 0.5f, fy, fz + 0.5f), L.vv, 1.0f));
@@ -1349,7 +1351,7 @@ struct CarryV {
 template <bool NEAR>
 __device__ __forceinline__ float4 correct_v(const MU& u, const Dims& d, const RingW& wt, const CarryV& c,
                                             const float4* V, int x, int y, int z) {
-  float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  float3 k = make_float3(d.kx, d.kx, d.kz);
   float fx = (float)x, fy = (float)y, fz = (float)z;
   const Loc& L = c.L;
   float3 pu = traceV(d, k, make_float3(fx, fy + 0.5f, fz + 0.5f), L.vu, -1.0f);
@@ -1630,9 +1632,9 @@ __global__ void __launch_bounds__(256) k_div(MU u, const float4* __restrict__ BS
       float rc = bsA(BS, z, d.nz).z;
       float wt = z == d.nz - 1 ? 0.0f : pv[j + 1].z;
       float wb = z == 0 ? 0.0f : p.z;
-      float dd = rc * (ux[j] - p.x + vy[j] - p.y) / d.dx +
-                 (rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb) / d.dz;
-      stx(rhs, off3(nxp, nyp, x + pd, y + pd, z), dd / d.dt);
+      float dd = rmul(rc * (ux[j] - p.x + vy[j] - p.y), d.rdx) +
+                 rmul(rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb, d.rdz);
+      stx(rhs, off3(nxp, nyp, x + pd, y + pd, z), rmul(dd, d.rdt));
     }
   }
 }
@@ -1650,7 +1652,7 @@ __global__ void __launch_bounds__(256) k_proj(MU u, const float4* __restrict__ V
   Dims d = dims(u);
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y, z = blockIdx.z;
   if (x > d.nx || y > d.ny || z > d.nz) return;
-  float kx = d.dt / d.dx, kz = d.dt / d.dz;
+  const float kx = d.kx, kz = d.kz;
   const uint32_t i = vidx(d, x, y, z);
   float4 Vv = ldx(V, i), o = Vv;
   float pc = Pphi(phi, u, d, x, y, z);
@@ -1745,7 +1747,7 @@ __device__ __forceinline__ float4 advv1_h_point(const MU& u, const Dims& d, cons
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv.w(), x, y, z);
   const bool nA = wave_all(near_traces(d, L));
-  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 k = make_float3(d.kx, d.kx, d.kz);
   const float fx = (float)x, fy = (float)y, fz = (float)z;
   const float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
   const float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
@@ -1761,7 +1763,7 @@ __device__ __forceinline__ float4 advv2_h_point(const MU& u, const Dims& d, cons
   const bool mid = u.v[U_CN].w > 0.5f;
   const Loc L = localVel(gv.w(), x, y, z);
   const bool nA = wave_all(near_traces(d, L));
-  const float3 k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 k = make_float3(d.kx, d.kx, d.kz);
   const float fx = (float)x, fy = (float)y, fz = (float)z;
   const float3 pu = make_float3(fx, fy + 0.5f, fz + 0.5f), pv = make_float3(fx + 0.5f, fy, fz + 0.5f);
   const float3 pw = make_float3(fx + 0.5f, fy + 0.5f, fz), pc = make_float3(fx + 0.5f, fy + 0.5f, fz + 0.5f);
@@ -1856,7 +1858,7 @@ __device__ __forceinline__ void advq1_h_body(bool thm, const Ring& g1, const Rin
 __device__ __forceinline__ void advq1_h_point(const MU& u, const Dims& d, const float4* BS, const Ring& gv, const Ring& g1,
                                               const Ring& g2, int x, int y, int z, float4& o1, float4& o2) {
   const bool mid = u.v[U_CN].w > 0.5f, thm = u.v[U_MICRO2].z > 0.5f;
-  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.kx, d.kx, d.kz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);
@@ -1906,7 +1908,7 @@ __device__ __forceinline__ void advq2_h_point(const MU& u, const Dims& d, const 
                                               const Ring& g2, const Ring& g1t, const Ring& g2t, int x, int y, int z,
                                               float4& o1, float4& o2) {
   const bool mid = u.v[U_CN].w > 0.5f;
-  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.dt / d.dx, d.dt / d.dx, d.dt / d.dz);
+  const float3 v = cellVel(gv.w(), x, y, z), k = make_float3(d.kx, d.kx, d.kz);
   const float3 p = make_float3((float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f);
   const float4 q1 = g1.w().ld4(x, y, z), q2 = g2.w().ld4(x, y, z), h1 = g1t.w().ld4(x, y, z), h2 = g2t.w().ld4(x, y, z);
   const FallH f = fall_h(u, q1, q2, bsA(BS, z, d.nz).z);

@@ -788,6 +788,10 @@ static Dims dims_of(const obl_model* h, const MU& u) {
   Dims d;
   d.nx = h->nx; d.ny = h->ny; d.nz = h->nz;
   d.dx = u.v[U_DIMS].w; d.dz = u.v[U_DZV].x; d.dt = u.v[U_STEP].x;
+  // (the kernels make their own in dims(u); the host uses the sizes)
+  d.kx = d.dt / d.dx; d.kz = d.dt / d.dz;
+  d.rdx = 1.0f / d.dx; d.rdz = 1.0f / d.dz; d.r2dx = 0.5f / d.dx; d.r2dz = 0.5f / d.dz;
+  d.rdt = 1.0f / d.dt;
   return d;
 }
 

@@ -29,12 +29,28 @@ struct MU {
 struct Dims {
   int nx, ny, nz;
   float dx, dz, dt;
+  // computed once per thread by dims(): the Courant factors dt / dx, dt / dz, and the reciprocals that the loops
+  // multiply by (rmul). x / b with the WGSL-accuracy division of these kernels is x * rcp(b) rounded (for
+  // |b| <= 2^96, any grid), so rmul(x, rcp(b)) gives the same bits without recomputing rcp(b) and its range check
+  // at every use.
+  float kx, kz, rdx, rdz, r2dx, r2dz, rdt;
 };
+
+// x * r rounded on its own, as a quotient is: the empty asm keeps the product from being fused into a following
+// add (which would round once for both)
+__device__ __forceinline__ float rmul(float x, float r) {
+  float p = x * r;
+  asm("" : "+v"(p));
+  return p;
+}
 
 __device__ __forceinline__ Dims dims(const MU& u) {
   Dims d;
   d.nx = (int)u.v[U_DIMS].x; d.ny = (int)u.v[U_DIMS].y; d.nz = (int)u.v[U_DIMS].z;
   d.dx = u.v[U_DIMS].w; d.dz = u.v[U_DZV].x; d.dt = u.v[U_STEP].x;
+  d.kx = d.dt / d.dx; d.kz = d.dt / d.dz;
+  d.rdx = 1.0f / d.dx; d.rdz = 1.0f / d.dz; d.r2dx = 1.0f / (2.0f * d.dx); d.r2dz = 1.0f / (2.0f * d.dz);
+  d.rdt = 1.0f / d.dt;
   return d;
 }
 
