@@ -60,6 +60,12 @@ def kernel_roofline(name, ms, nx, ny, nz, nxp, nyp, gemm):
         b = (4 + 2 * pieces) * cells_p  # rhs read once, its folded pieces written once
         return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "algorithmic_bytes": b}
+    if name == "m.divf" and gemm in ("bf16x3", "bf16x6"):
+        pieces = 2 if gemm == "bf16x3" else 3
+        # V read once and rhs written once per interior cell, the folded pieces written once per padded cell
+        b = 20 * cells + 2 * pieces * cells_p
+        return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "algorithmic_bytes": b}
     if name == "ps.tri":
         b = 8 * cells_p
         return {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include "obl_kernels.h"
+#include "obl_mx.h"
 
 namespace obl {
 
@@ -230,8 +231,6 @@ __global__ void __launch_bounds__(256) k_dst(DstArgs a) {
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // ---------------- multi-piece BF16 form (k_dst_mx) ----------------
 // Every FP32 operand x is carried as P BF16 pieces, x ~ x1 + x2 (+ x3), each piece the round-to-nearest-even BF16
@@ -251,40 +250,6 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 // wave holds 192 accumulators and a CU runs one block (measured: 64-row blocks two per CU are 1.5 times slower,
 // the sine-matrix traffic doubling).
 typedef float v16f __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ int mx_chunk(int ri, int kc) { return 2 * ri + (kc ^ (((ri >> 2) ^ (ri >> 3)) & 1)); }
-// bf16 offset of element (row, k) inside one piece plane of a tiled operand with KB k-blocks; a row-block spans
-// KB + 1 tiles (an odd pitch: with power-of-two strides, as at N = 1024, the tiles of a k-block alias in the
-// memory channels)
-__device__ __forceinline__ size_t mx_off(int row, int k, int KB) {
-  return ((size_t)(row >> 5) * (KB + 1) + (k >> 4)) * 512 + mx_chunk(row & 31, (k >> 3) & 1) * 8 + (k & 7);
-}
-
-// P pieces of 4 floats, 4 bf16 (8 bytes) per piece; the hardware conversion rounds to nearest even
-template <int P>
-__device__ __forceinline__ void mx_split4(float r0, float r1, float r2, float r3, uint2* out) {
-#pragma unroll
-  for (int q = 0; q < P; q++) {
-    uint32_t u01 = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r0, r1}, bf16x2));
-    uint32_t u23 = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r2, r3}, bf16x2));
-    out[q] = make_uint2(u01, u23);
-    if (q + 1 < P) {
-      r0 -= __uint_as_float(u01 << 16);
-      r1 -= __uint_as_float(u01 & 0xffff0000u);
-      r2 -= __uint_as_float(u23 << 16);
-      r3 -= __uint_as_float(u23 & 0xffff0000u);
-    }
-  }
-}
-template <int P>
-__device__ __forceinline__ void mx_split1(float r, uint16_t* out) {
-#pragma unroll
-  for (int q = 0; q < P; q++) {
-    uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{r, 0.0f}, bf16x2));
-    out[q] = (uint16_t)u;
-    if (q + 1 < P) r -= __uint_as_float(u << 16);
-  }
-}
 
 // Fold-and-split of the right-hand side into the x forward pass's operand. Its rows come in tiles of 32 MF rows
 // holding 16 MF (z, y) pairs (MF = MX_MF): row 16 MF h + q of tile t is y' (h = 0) or ny - 1 - y' (h = 1) of pair

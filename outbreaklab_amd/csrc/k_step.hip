@@ -11,6 +11,7 @@
 #include "obl_device.h"
 #include "obl_kernels.h"
 #include "obl_micro.h"
+#include "obl_mx.h"
 #include "obl_tile.h"
 
 namespace obl {
@@ -1608,6 +1609,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_phys(MU u, const float4* __res
 // level's own texels, read once for both (one block per level read each plane twice; 4 or 8 levels per block lose the
 // row reuse that neighbouring blocks of one plane share in L2: 0.43 -> 0.38 ms with 2, 0.39 with 4, 0.43 with 8).
 constexpr int ZL = 2;
+// The divergence right-hand side of cell (x, y, z) from its face texel p, the texel above (its top w), u of the
+// face at x + 1 and v of the face at y + 1 (model.js:910-925)
+__device__ __forceinline__ float div_cell(const Dims& d, const float4* __restrict__ BS, float4 p, float4 pup, float ux,
+                                          float vy, int z) {
+  const float rc = bsA(BS, z, d.nz).z;
+  const float wt = z == d.nz - 1 ? 0.0f : pup.z;
+  const float wb = z == 0 ? 0.0f : p.z;
+  const float dd = rmul(rc * (ux - p.x + vy - p.y), d.rdx) + rmul(rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb, d.rdz);
+  return rmul(dd, d.rdt);
+}
 __global__ void __launch_bounds__(256) k_div(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
                                              float* __restrict__ rhs) {
   Dims d = dims(u);
@@ -1627,16 +1638,77 @@ __global__ void __launch_bounds__(256) k_div(MU u, const float4* __restrict__ BS
 #pragma unroll
   for (int j = 0; j < ZL; j++) {
     const int z = z0 + j;
-    if (z < d.nz) {
-      float4 p = pv[j];
-      float rc = bsA(BS, z, d.nz).z;
-      float wt = z == d.nz - 1 ? 0.0f : pv[j + 1].z;
-      float wb = z == 0 ? 0.0f : p.z;
-      float dd = rmul(rc * (ux[j] - p.x + vy[j] - p.y), d.rdx) +
-                 rmul(rhoW(BS, z + 1, d.nz) * wt - rhoW(BS, z, d.nz) * wb, d.rdz);
-      stx(rhs, off3(nxp, nyp, x + pd, y + pd, z), rmul(dd, d.rdt));
+    if (z < d.nz) stx(rhs, off3(nxp, nyp, x + pd, y + pd, z), div_cell(d, BS, pv[j], pv[j + 1], ux[j], vy[j], z));
+  }
+}
+
+// DIV and the fold-and-split of the multi-piece BF16 solve's x forward pass in one kernel. A block takes two levels
+// of four rows y = 4 g .. 4 g + 3 of the solve's padded grid (Nx x Ny, the interior at pd) over a 64-wide span of
+// the fold index x' (64 i .. 64 i + 63, x' < Nx / 2): threads 0-255 the cells x = x', threads 256-511 their mirrors
+// x = Nx - 1 - x' (32- and 128-wide spans measured slower). Each computes its cell's divergence as k_div does
+// (stored to rhs; zero outside the interior) into LDS, and two waves then write the x forward pass's tiled operand
+// for the span as the fold kernel does from rhs:
+// parity 0 holds S = F(x') + F(Nx - 1 - x'), parity 1 D = F(x') - F(Nx - 1 - x'), in P BF16 pieces. Row y of level
+// z is operand row t 32 MF + 16 MF h + (q mod 16 MF) of pair q = z Ny / 2 + y' (y' = y in the lower half, h = 0;
+// Ny - 1 - y in the upper, h = 1; t = q / 16 MF), so with Ny a multiple of 8 the four rows are four consecutive
+// operand rows, and 8 lanes (4 rows x 2 k-halves) fill one 128-byte line. The divergence skips its round trip
+// through rhs for the fold; the loads are k_div's (two levels per block, the level above read once for both).
+template <int P>
+__global__ void __launch_bounds__(512) k_div_fold(MU u, const float4* __restrict__ BS, const float4* __restrict__ V,
+                                                  float* __restrict__ rhs, uint16_t* __restrict__ out, int RB, int KB) {
+  __shared__ __align__(16) float fs[2][2][4][64];  // [cell / mirror][level][row][x' - 64 i]
+  const Dims d = dims(u);
+  const int pd = (int)u.v[U_PAD].x, Nx = (int)u.v[U_PAD].y, Ny = (int)u.v[U_PAD].z;
+  const int Hx = Nx >> 1, Hy = Ny >> 1;
+  const int tid = threadIdx.x, half = tid >> 8, r = (tid >> 6) & 3, xi = tid & 63;
+  const int z0 = 2 * blockIdx.z, xq = 64 * blockIdx.x + xi;
+  const int xp = half ? Nx - 1 - xq : xq, x = xp - pd, yp = 4 * blockIdx.y + r, y = yp - pd;
+  float f0 = 0.0f, f1 = 0.0f;
+  // (x' >= Nx / 2: the cell is another span's mirror)
+  if (xq < Hx && x >= 0 && x < d.nx && y >= 0 && y < d.ny) {
+    float4 pv[3];
+    float ux[2], vy[2];
+#pragma unroll
+    for (int j = 0; j <= 2; j++) pv[j] = ldx(V, vidx(d, x, y, min(z0 + j, d.nz)));
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      ux[j] = ldVc(V, d, x + 1, y, z0 + j, 0);
+      vy[j] = ldVc(V, d, x, y + 1, z0 + j, 1);
+    }
+    f0 = div_cell(d, BS, pv[0], pv[1], ux[0], vy[0], z0);
+    stx(rhs, off3(Nx, Ny, xp, yp, z0), f0);
+    if (z0 + 1 < d.nz) {
+      f1 = div_cell(d, BS, pv[1], pv[2], ux[1], vy[1], z0 + 1);
+      stx(rhs, off3(Nx, Ny, xp, yp, z0 + 1), f1);
     }
   }
+  fs[half][0][r][xi] = f0;
+  fs[half][1][r][xi] = f1;
+  __syncthreads();
+  if (tid >= 128) return;
+  // lane: row (bits 0-1), k-half (bit 2), k-block of the span (bits 3-4), level (bit 5), parity (bit 6)
+  const int lr = tid & 3, kc = (tid >> 2) & 1, kbl = (tid >> 3) & 3, j = (tid >> 5) & 1, par = tid >> 6;
+  const int z = z0 + j, kb = 4 * blockIdx.x + kbl;
+  if (z >= d.nz || kb >= KB) return;
+  const int xo = 16 * kbl + 8 * kc;
+  const float4 a0 = *reinterpret_cast<const float4*>(&fs[0][j][lr][xo]);
+  const float4 a1 = *reinterpret_cast<const float4*>(&fs[0][j][lr][xo + 4]);
+  const float4 m0 = *reinterpret_cast<const float4*>(&fs[1][j][lr][xo]);
+  const float4 m1 = *reinterpret_cast<const float4*>(&fs[1][j][lr][xo + 4]);
+  uint2 lo[P], hi[P];
+  if (par == 0) {
+    mx_split4<P>(a0.x + m0.x, a0.y + m0.y, a0.z + m0.z, a0.w + m0.w, lo);
+    mx_split4<P>(a1.x + m1.x, a1.y + m1.y, a1.z + m1.z, a1.w + m1.w, hi);
+  } else {
+    mx_split4<P>(a0.x - m0.x, a0.y - m0.y, a0.z - m0.z, a0.w - m0.w, lo);
+    mx_split4<P>(a1.x - m1.x, a1.y - m1.y, a1.z - m1.z, a1.w - m1.w, hi);
+  }
+  const int yq = 4 * blockIdx.y + lr, h = yq >= Hy, q = z * Hy + (h ? Ny - 1 - yq : yq);
+  const int ro = q / (16 * MX_MF) * (32 * MX_MF) + 16 * MX_MF * h + q % (16 * MX_MF);
+  uint16_t* const o = out + (size_t)(par * P) * ((size_t)RB * (KB + 1) * 512) + mx_off(ro, 16 * kb + 8 * kc, KB);
+#pragma unroll
+  for (int w = 0; w < P; w++)
+    *reinterpret_cast<uint4*>(o + (size_t)w * ((size_t)RB * (KB + 1) * 512)) = make_uint4(lo[w].x, lo[w].y, hi[w].x, hi[w].y);
 }
 
 // ====================== PROJ (model.js:928-946) ======================
@@ -2176,6 +2248,14 @@ void launch_phys_dump(hipStream_t s, const MU& u, const Dims& d, const float4* B
 }
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs) {
   hipLaunchKernelGGL(k_div, dim3((d.nx + 63) / 64, (d.ny + 3) / 4, (d.nz + ZL - 1) / ZL), dim3(64, 4, 1), 0, s, u, BS, V, rhs);
+}
+bool div_fold_fits(int Nx, int Ny) { return Nx % 8 == 0 && Ny % 8 == 0; }
+void launch_div_fold(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs, int P,
+                     uint16_t* out, int RB, int KB) {
+  const int Nx = (int)u.v[U_PAD].y, Ny = (int)u.v[U_PAD].z;
+  const dim3 grid((Nx / 2 + 63) / 64, Ny / 4, (d.nz + 1) / 2);
+  if (P == 2) hipLaunchKernelGGL(k_div_fold<2>, grid, dim3(512), 0, s, u, BS, V, rhs, out, RB, KB);
+  else hipLaunchKernelGGL(k_div_fold<3>, grid, dim3(512), 0, s, u, BS, V, rhs, out, RB, KB);
 }
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout) {
   hipLaunchKernelGGL(k_proj, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, phi, Vout);

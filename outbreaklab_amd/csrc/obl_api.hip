@@ -405,8 +405,9 @@ static int thomas_modes(obl_poisson* p, int thomas, double shift_lo, hipStream_t
   return 0;
 }
 
+// folded: the x forward pass's operand is already written (launch_div_fold, the model step's divergence)
 static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, int thomas, double cond,
-                       hipStream_t s) {
+                       hipStream_t s, bool folded = false) {
   if (gemm != OBL_GEMM_FP32 && gemm != OBL_GEMM_BF16X3 && gemm != OBL_GEMM_BF16X6)
     return p->fail("unknown gemm mode %d", gemm);
   if (thomas < 0 || thomas > 2) return p->fail("unknown Thomas mode %d", thomas);
@@ -414,7 +415,7 @@ static int poisson_enc(obl_poisson* p, const float* rhs, float* phi, int gemm, i
   if (gemm == OBL_GEMM_BF16X3 || gemm == OBL_GEMM_BF16X6) {
     const int P = gemm == OBL_GEMM_BF16X6 ? 3 : 2;
     const AxisMats &X = p->ax, &Y = p->ay;
-    TIMED(p->timer, s, "ps.fold", launch_mx_fold_x(s, P, rhs, p->mx[0], nx, ny, nz, p->mxRB[0], X.KB));
+    if (!folded) TIMED(p->timer, s, "ps.fold", launch_mx_fold_x(s, P, rhs, p->mx[0], nx, ny, nz, p->mxRB[0], X.KB));
     MxArgs m;
     m.Nx = nx; m.Ny = ny; m.nz = nz;
     // forward x: rows (z, y) pairs; the epilogue folds y into the YF operand
@@ -1189,6 +1190,19 @@ extern "C" int obl_radar_collect(obl_radar* r, int slot, void* dst, size_t dst_b
 // The hurricane-mode step (model.js:1442-1476 with this.hur): the 2-dx filter every 4th step (state ->
 // the spare copies -> back), PREP while the prep field exists, then the step with the hurricane kernels.
 // The surrogate, its judge and teacher sampling are outbreak-mode tools and are refused here.
+// The divergence before the solve: with the multi-piece BF16 transforms (and a padded grid the fused kernel
+// takes), one kernel also writes the x forward pass's operand, and the solve skips its fold; returns whether it did.
+static bool step_div(obl_model* h, const MU& u, const Dims& d, const float4* Vc, Timer& T, hipStream_t s) {
+  obl_poisson* p = h->solver;
+  if (h->gemm_mode != OBL_GEMM_FP32 && div_fold_fits(p->nx, p->ny)) {
+    const int P = h->gemm_mode == OBL_GEMM_BF16X6 ? 3 : 2;
+    TIMED(T, s, "m.divf", launch_div_fold(s, u, d, h->BS, Vc, h->rhs, P, p->mx[0], p->mxRB[0], p->ax.KB));
+    return true;
+  }
+  TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
+  return false;
+}
+
 static int step_hurricane(obl_model* h, const MU& u, const Dims& d, int parity, unsigned flags) {
   hipStream_t s = h->stream;
   Timer& T = h->timer;
@@ -1217,9 +1231,9 @@ static int step_hurricane(obl_model* h, const MU& u, const Dims& d, int parity, 
   HurArgs hu = {h->sfc, h->phi, Vn};
   TIMED(T, s, "m.phys", launch_phys_hur(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT,
                                         h->ciMap, Vc, h->Q1[0], h->Q2[0], hu));
-  TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
+  const bool folded = step_div(h, u, d, Vc, T, s);
   h->solver->timer.on = T.on;
-  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s))
+  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s, folded))
     return h->fail("%s", h->solver->err.c_str());
   TIMED(T, s, "m.proj", launch_proj(s, u, d, Vc, h->phi, Vn));
   return hip_ok(hipGetLastError(), &h->err, "encStep");
@@ -1289,9 +1303,9 @@ extern "C" int obl_step(obl_model* h, const float* uniforms, int parity, unsigne
     TIMED(T, s, "m.phys", launch_phys(s, u, d, h->BS, Vn, h->Q1[1], h->Q2[1], h->aux, h->lsfA, h->lsfB, h->phT, h->ciMap,
                                       Vc, h->Q1[0], h->Q2[0]));
   }
-  TIMED(T, s, "m.div", launch_div(s, u, d, h->BS, Vc, h->rhs));
+  const bool folded = step_div(h, u, d, Vc, T, s);
   h->solver->timer.on = T.on;
-  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s))
+  if (poisson_enc(h->solver, h->rhs, h->phi, h->gemm_mode, h->thomas_mode, h->fp64_cond, s, folded))
     return h->fail("%s", h->solver->err.c_str());
   TIMED(T, s, "m.proj", launch_proj(s, u, d, Vc, h->phi, Vn));
   return hip_ok(hipGetLastError(), &h->err, "encStep");

@@ -104,6 +104,11 @@ void launch_filt(hipStream_t s, const MU& u, const Dims& d, const float4* V, con
                  float4* Vo, float4* Q1o, float4* Q2o);
 
 void launch_div(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs);
+// the divergence and the fold-and-split of the multi-piece solve's x forward operand in one kernel (padded Nx and Ny
+// multiples of 8: div_fold_fits)
+bool div_fold_fits(int Nx, int Ny);
+void launch_div_fold(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, float* rhs, int P,
+                     uint16_t* out, int RB, int KB);
 void launch_proj(hipStream_t s, const MU& u, const Dims& d, const float4* V, const float* phi, float4* Vout);
 // 2-D maps of the current state (model.js:1028-1100): out = 4 planar maps of nx*ny float4
 void launch_maps(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
