@@ -580,12 +580,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float nw = 0.5f * (lsfB(x - 1, y).w + lsfB(x, y).w);
     du += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (ut - V.x);
     du += fc * (vbar - vtb);
-    if (x > 0 && x < nx) du -= rmul(phT(x, y, z) - phT(x - 1, y, z), d.rdx);
+    if (x > 0 && x < nx) du -= (phT(x, y, z) - phT(x - 1, y, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(V.x * V.x + vbar * vbar), lg0 = sqrtf(ut * ut + vtb * vtb);
       float cd = u.v[U_MICRO].w;
       if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, max(x - 1, 0), y).x + sfcAt(hu, d, x, y).x));
-      du += rmul(cd * (-lg * V.x + lg0 * ut), d.rdz);
+      du += cd * (-lg * V.x + lg0 * ut) / dz;
     }
     o.x = V.x + dt * du;
   } else {
@@ -603,12 +603,12 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float nw = 0.5f * (lsfB(x, y - 1).w + lsfB(x, y).w);
     dv += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz) + nw * nudR) * (vt - V.y);
     dv -= fc * (ubar - utb);
-    if (y > 0 && y < ny) dv -= rmul(phT(x, y, z) - phT(x, y - 1, z), d.rdx);
+    if (y > 0 && y < ny) dv -= (phT(x, y, z) - phT(x, y - 1, z)) / dx;
     if (z == 0 && drag) {
       float lg = sqrtf(ubar * ubar + V.y * V.y), lg0 = sqrtf(utb * utb + vt * vt);
       float cd = u.v[U_MICRO].w;
       if (HUR) cd = cdSfc(u, lg, 0.5f * (sfcAt(hu, d, x, max(y - 1, 0)).x + sfcAt(hu, d, x, y).x));
-      dv += rmul(cd * (-lg * V.y + lg0 * vt), d.rdz);
+      dv += cd * (-lg * V.y + lg0 * vt) / dz;
     }
     o.y = V.y + dt * dv;
   } else {
@@ -623,7 +623,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float dw = nux * lapH(Vin, x, y, z, 2, V.z) * ix2 + nuy * lz * iz2;
     dw += 0.5f * (buoyv(bsA(BS, z - 1, nz), col.Vm.w, col.Q1m, col.Q2m) +
                   buoyv(bsA(BS, z, nz), V.w, Q1in.ld4(x, y, z), col.Q2c)) -
-          rmul(phT(x, y, z) - col.pm, d.rdz);
+          (phT(x, y, z) - col.pm) / dz;
     dw += (spongeLat(u, d, xmx, xmy) + spongeTop(u, d, zz)) * (0.0f - V.z);
     float4 ci = col.ci;
     if (ci.x > 0.0f && !HUR) {  // hurricane mode starts its storms as warm, moist bubbles instead (theta below)

@@ -44,13 +44,20 @@ __device__ __forceinline__ float rmul(float x, float r) {
   return p;
 }
 
+// a wave-uniform float moved to a scalar register
+__device__ __forceinline__ float sreg(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 __device__ __forceinline__ Dims dims(const MU& u) {
   Dims d;
   d.nx = (int)u.v[U_DIMS].x; d.ny = (int)u.v[U_DIMS].y; d.nz = (int)u.v[U_DIMS].z;
   d.dx = u.v[U_DIMS].w; d.dz = u.v[U_DZV].x; d.dt = u.v[U_STEP].x;
-  d.kx = d.dt / d.dx; d.kz = d.dt / d.dz;
-  d.rdx = 1.0f / d.dx; d.rdz = 1.0f / d.dz; d.r2dx = 1.0f / (2.0f * d.dx); d.r2dz = 1.0f / (2.0f * d.dz);
-  d.rdt = 1.0f / d.dt;
+  // (kept in scalar registers: uniform values made by vector instructions would otherwise hold a vector register
+  // each for the whole kernel)
+  d.kx = sreg(d.dt / d.dx); d.kz = sreg(d.dt / d.dz);
+  d.rdx = sreg(1.0f / d.dx); d.rdz = sreg(1.0f / d.dz); d.r2dx = sreg(1.0f / (2.0f * d.dx));
+  d.r2dz = sreg(1.0f / (2.0f * d.dz)); d.rdt = sreg(1.0f / d.dt);
   return d;
 }
 
