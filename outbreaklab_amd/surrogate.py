@@ -135,7 +135,20 @@ def teach(model, dt, steps, capacity=2_000_000, p_active=0.5, p_clear=0.0):
     model.sync()
     rows = model.micro_samples(copy=True)
     model.sample_micro(0)
-    return rows
+    return canonical_order(rows)
+
+
+def canonical_order(rows):
+    """The teacher rows sorted by a 64-bit key of their bit patterns. The GPU appends sampled rows with one atomic per
+    wave, so their order changes from run to run while the set of rows does not; training draws its batches by row
+    index, so without a fixed order two runs of the same seeds train different networks."""
+    u = np.ascontiguousarray(rows, dtype=np.float32).view(np.uint32)
+    key = np.zeros(u.shape[0], np.uint64)
+    mult = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        for c in range(u.shape[1]):
+            key = key * mult + u[:, c].astype(np.uint64)
+    return np.ascontiguousarray(rows[np.argsort(key, kind="stable")])
 
 
 NR_IN, NR_OUT = 9, NIN + 7  # Nr input and dNr output columns of a teacher row

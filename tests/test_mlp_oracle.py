@@ -54,3 +54,18 @@ def test_features():
     assert abs(f[5] - (math.log(1e-8) + 12) / 4) < 1e-12  # qr = 0
     assert abs(f[9] - (math.log(5e4 + 1) - 5) / 5) < 1e-12
     assert f[11] == 0.3 and f[12] == -0.5 and abs(f[13] + 0.6) < 1e-12 and f[14] == 0 and f[15] == 0
+
+
+def test_teacher_rows_get_one_order_whatever_order_the_gpu_appended_them():
+    """Training draws batches by row index: the same set of teacher rows in any order must reach it in one order."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
+    from outbreaklab_amd.surrogate import canonical_order
+
+    rng = np.random.default_rng(5)
+    rows = rng.standard_normal((5000, 24)).astype(np.float32)
+    rows[100] = rows[7]  # identical rows are interchangeable
+    a = canonical_order(rows)
+    for seed in (1, 2):
+        b = canonical_order(rows[np.random.default_rng(seed).permutation(len(rows))])
+        assert np.array_equal(a, b)
+    assert np.array_equal(np.sort(a, axis=0), np.sort(rows, axis=0))  # a permutation of the rows
