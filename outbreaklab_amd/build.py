@@ -38,6 +38,9 @@ HIP_FLAGS = [
 # multiply. The solver (k_solve.hip) keeps correctly rounded FP32 division for the Thomas sweeps.
 FILE_FLAGS = {
     "k_step.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
+    # the joint advection kernel alone (k_step.hip with OBL_ADV_ONLY), scheduled for instruction-level parallelism
+    "k_adv.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
+                  "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
     "k_surr.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "k_maps.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     "k_radar.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],

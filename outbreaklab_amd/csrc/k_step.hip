@@ -16,6 +16,9 @@
 
 namespace obl {
 
+// k_adv.hip compiles this file a second time with OBL_ADV_ONLY defined, keeping only the joint advection kernel and
+// its launcher (built with an instruction-scheduling strategy of its own); this compilation leaves those two out.
+
 // ====================== LSF (model.js:142-244) ======================
 struct BndView {
   const float* b;  // 148 floats: p0..p4 then 32 points
@@ -76,6 +79,7 @@ __device__ static float hotSpots(const float* b, float time, float s) {
   return clampf_(acc, 0.0f, 1.0f);
 }
 
+#ifndef OBL_ADV_ONLY
 __global__ void __launch_bounds__(256) k_lsf(MU u, const float4* __restrict__ BS, const float* __restrict__ BD,
                                              float4* __restrict__ lsfA, float4* __restrict__ lsfB,
                                              float* __restrict__ phT) {
@@ -126,8 +130,10 @@ __global__ void __launch_bounds__(256) k_lsf(MU u, const float4* __restrict__ BS
     phT[cidx(d, i, j, k)] = ph;
   }
 }
+#endif
 
 // ====================== CI map (model.js:249-281) ======================
+#ifndef OBL_ADV_ONLY
 __global__ void __launch_bounds__(256) k_ci(MU u, const float* __restrict__ CI, float4* __restrict__ ciMap) {
   Dims d = dims(u);
   int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -155,6 +161,7 @@ __global__ void __launch_bounds__(256) k_ci(MU u, const float* __restrict__ CI, 
   }
   ciMap[(size_t)i + (size_t)d.nx * j] = best;
 }
+#endif
 
 // ---------------- target state helpers (model.js:125-134) ----------------
 __device__ __forceinline__ float tgtTh(const float4* BS, const Dims& d, float4 A, int k) {
@@ -202,6 +209,7 @@ __device__ __forceinline__ float2 vxAt(const MU& u, const float4* VX, int nz, fl
   float4 a = VX[o], b = VX[o + 1];
   return make_float2(a.x * (1.0f - t) + b.x * t, a.y * (1.0f - t) + b.y * t);
 }
+#ifndef OBL_ADV_ONLY
 __global__ void __launch_bounds__(256) k_init(MU u, const float4* __restrict__ BS, const float4* __restrict__ lsfA,
                                               const float4* __restrict__ lsfB, const float4* __restrict__ VX,
                                               float4* __restrict__ V, float4* __restrict__ Q1, float4* __restrict__ Q2,
@@ -246,9 +254,11 @@ __global__ void __launch_bounds__(256) k_init(MU u, const float4* __restrict__ B
     Q2[cidx(d, x, y, z)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
+#endif
 
 // Build-defined warm bubble (no reference counterpart): theta += dth cos^2(pi r / 2)
 // at each texel's cell centre, applied to both state copies.
+#ifndef OBL_ADV_ONLY
 __global__ void __launch_bounds__(256) k_bubble(MU u, float4* __restrict__ V0, float4* __restrict__ V1, double x0,
                                                 double y0, double z0, double rh, double rz, double dth) {
   Dims d = dims(u);
@@ -265,6 +275,7 @@ __global__ void __launch_bounds__(256) k_bubble(MU u, float4* __restrict__ V0, f
     V1[i].w += add;
   }
 }
+#endif
 
 // ====================== NUT (model.js:342-389), outbreak mode ======================
 template <class A>
@@ -1473,6 +1484,7 @@ __device__ __forceinline__ void advv_tile(const MU& u, const float4* __restrict_
   }
 }
 
+#ifdef OBL_ADV_ONLY
 // Both advections in one launch (they read the same state and write different fields), so that a CU runs a wind
 // tile and a moisture tile side by side and the moisture advection's waits at its barriers and on memory overlap
 // the wind advection's instruction issue (the two kernels back to back leave each bottleneck alone). Block b runs
@@ -1507,7 +1519,13 @@ __global__ void __launch_bounds__(TILE_THREADS, 2) k_adv_t(MU u, const float4* _
   if (wind) advv_tile(u, BS, V, Q1, Vout, aux, i % gvx, i / gvx, sm.v);
   else advq_tile(u, BS, V, Q1, Q2, Q1o, Q2o, i % gqx, i / gqx, sm.q);
 }
+#endif  // OBL_ADV_ONLY
 
+// launch grids of the tile kernels
+static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
+static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
+
+#ifndef OBL_ADV_ONLY
 // One block per 32 x 8 column tile of the faces grid, walking z upward like the advection
 // kernels: V and Q1 are three-plane LDS rings (the next plane prefetched into registers during a
 // level's compute), aux and phT the current plane, lsfB the 2-D tile; the vertical stencils read
@@ -2197,8 +2215,6 @@ void launch_bubble(hipStream_t s, const MU& u, const Dims& d, float4* V0, float4
                    double rh, double rz, double dth) {
   hipLaunchKernelGGL(k_bubble, faces_grid(d), dim3(64, 4, 1), 0, s, u, V0, V1, x, y, z, rh, rz, dth);
 }
-static dim3 tface_grid(const Dims& d) { return dim3((d.nx + 1 + TX - 1) / TX, (d.ny + 1 + TY - 1) / TY, 1); }
-static dim3 tcell_grid(const Dims& d) { return dim3((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY, 1); }
 void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 const float4* sfc, float2* aux) {
   if (sfc && hur_on(u))
@@ -2206,6 +2222,8 @@ void launch_nut(hipStream_t s, const MU& u, const Dims& d, const float4* BS, con
   else
     hipLaunchKernelGGL(k_nut_t<false>, tcell_grid(d), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, sfc, aux);
 }
+#endif  // OBL_ADV_ONLY
+#ifdef OBL_ADV_ONLY
 void launch_adv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* V, const float4* Q1,
                 const float4* Q2, float4* Vout, float2* aux, float4* Q1o, float4* Q2o) {
   const dim3 gv = tface_grid(d), gq = tcell_grid(d);
@@ -2213,6 +2231,7 @@ void launch_adv(hipStream_t s, const MU& u, const Dims& d, const float4* BS, con
   hipLaunchKernelGGL(k_adv_t, dim3(nv + nq), dim3(TILE_THREADS), 0, s, u, BS, V, Q1, Q2, Vout, aux, Q1o, Q2o, (int)gv.x, nv,
                      (int)gq.x, nq);
 }
+#else
 void launch_phys(hipStream_t s, const MU& u, const Dims& d, const float4* BS, const float4* Vin, const float4* Q1in,
                  const float4* Q2in, const float2* aux, const float4* lsfA, const float4* lsfB, const float* phT,
                  const float4* ciMap, float4* Vout, float4* Q1o, float4* Q2o) {
@@ -2285,5 +2304,7 @@ void launch_filt(hipStream_t s, const MU& u, const Dims& d, const float4* V, con
                  float4* Vo, float4* Q1o, float4* Q2o) {
   hipLaunchKernelGGL(k_filt, faces_grid(d), dim3(64, 4, 1), 0, s, u, V, Q1, Q2, Vo, Q1o, Q2o);
 }
+
+#endif  // OBL_ADV_ONLY
 
 }  // namespace obl

@@ -18,8 +18,10 @@ CSRC = os.path.join(os.path.dirname(B.__file__), "csrc")
 
 # kernel (demangled, without arguments) -> minimum waves per SIMD
 FLOORS = {
-    "k_step.hip": {
+    "k_adv.hip": {
         "obl::k_adv_t": 2,
+    },
+    "k_step.hip": {
         "void obl::k_phys<0, false>": 3,
         "void obl::k_phys<2, false>": 3,
         "void obl::k_phys<0, true>": 2,
