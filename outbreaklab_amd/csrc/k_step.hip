@@ -790,8 +790,8 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
       th = T / B.x;
       float keep = 1.0f / (1.0f + 2.0f * dt * sl + dt * envDecay);
       qc *= keep; qi *= keep; qr *= keep; qs *= keep; qg *= keep; nr *= keep;
-      stx(Q1o, ci_, make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f)));
-      stx(Q2o, ci_, make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w));
+      stnt(Q1o, ci_, make_float4(fmaxf(qv, 0.0f), fmaxf(qc, 0.0f), fmaxf(qi, 0.0f), fmaxf(qr, 0.0f)));
+      stnt(Q2o, ci_, make_float4(fmaxf(qs, 0.0f), fmaxf(qg, 0.0f), thm ? fmaxf(nr, 0.0f) : q2.z, q2.w));
     }
     o.w = th;  // a handed point's theta is written by the surrogate
   }
@@ -805,7 +805,7 @@ __device__ __forceinline__ void phys_point(const MU& u, const Dims& d, const flo
     float4 Bz = bsB(BS, z, nz);
     o = make_float4(Bz.z, Bz.w, 0.0f, bsA(BS, z, nz).x);
   }
-  stx(Vout, vi, o);
+  stnt(Vout, vi, o);
 }
 
 
@@ -1694,10 +1694,10 @@ __global__ void __launch_bounds__(512) k_div_fold(MU u, const float4* __restrict
       vy[j] = ldVc(V, d, x, y + 1, z0 + j, 1);
     }
     f0 = div_cell(d, BS, pv[0], pv[1], ux[0], vy[0], z0);
-    stx(rhs, off3(Nx, Ny, xp, yp, z0), f0);
+    stnt(rhs, off3(Nx, Ny, xp, yp, z0), f0);
     if (z0 + 1 < d.nz) {
       f1 = div_cell(d, BS, pv[1], pv[2], ux[1], vy[1], z0 + 1);
-      stx(rhs, off3(Nx, Ny, xp, yp, z0 + 1), f1);
+      stnt(rhs, off3(Nx, Ny, xp, yp, z0 + 1), f1);
     }
   }
   fs[half][0][r][xi] = f0;
