@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_nut_t(MU u, const float4* __re
           nu.y = fmaxf(nu.y, fminf(0.4f * ust * zk * q * q, 0.15f * d.dz * d.dz / fmaxf(d.dt, 0.1f)));
         }
       }
-      stx(aux, cidx(d, x, y, z), nu);
+      stnt(aux, cidx(d, x, y, z), nu);
     }
     __syncthreads();
     rv.store(z + 2, tid, pv);
@@ -2182,14 +2182,14 @@ __global__ void __launch_bounds__(256) k_filt(MU u, const float4* __restrict__ V
   const float4 v = gv.ld4(x, y, z), bv = bih4(gv, x, y, z);
   float4 o = make_float4(v.x - a * bv.x, v.y - a * bv.y, v.z - a * bv.z, v.w - a * bv.w);
   if (z == 0 || z >= d.nz) o.z = 0.0f;  // the lid / ground faces of w stay zero
-  stx(Vo, vidx(d, x, y, z), o);
+  stnt(Vo, vidx(d, x, y, z), o);
   if (x < d.nx && y < d.ny && z < d.nz) {
     const GIn g1{Q1, d.nx, d.ny, d.nz}, g2{Q2, d.nx, d.ny, d.nz};
     const uint32_t i = cidx(d, x, y, z);
     const float4 q = ldx(Q1, i), bq = bih4(g1, x, y, z), r = ldx(Q2, i), br = bih4(g2, x, y, z);
-    stx(Q1o, i, make_float4(fmaxf(q.x - a * bq.x, 0.0f), fmaxf(q.y - a * bq.y, 0.0f), fmaxf(q.z - a * bq.z, 0.0f),
+    stnt(Q1o, i, make_float4(fmaxf(q.x - a * bq.x, 0.0f), fmaxf(q.y - a * bq.y, 0.0f), fmaxf(q.z - a * bq.z, 0.0f),
                             fmaxf(q.w - a * bq.w, 0.0f)));
-    stx(Q2o, i, make_float4(fmaxf(r.x - a * br.x, 0.0f), fmaxf(r.y - a * br.y, 0.0f), fmaxf(r.z - a * br.z, 0.0f),
+    stnt(Q2o, i, make_float4(fmaxf(r.x - a * br.x, 0.0f), fmaxf(r.y - a * br.y, 0.0f), fmaxf(r.z - a * br.z, 0.0f),
                             fmaxf(r.w - a * br.w, 0.0f)));
   }
 }
